@@ -1,0 +1,128 @@
+// bb_scene.h — device-resident scene layout and the host-side scene object.
+//
+// HBM layout (all arrays 16-byte aligned, indexed as noted):
+//   nodes   [n_nodes]  BbBvhNode, 32 B: the reference's LinearBVHNode
+//                      (include/bulbit/bvh.h:46-58), pre-order, child1 = i + 1
+//   tris    [n_slots]  3 x float4 = 48 B per BVH slot (leaf order):
+//                      {p0.xyz, prim}, {e1.xyz, sphere index}, {e2.xyz, kind}
+//                      with e1 = p1 - p0 and e2 = p2 - p0 computed on the host
+//                      in float32, bit-identical to Triangle::Intersect's
+//                      (src/geometry/triangle.cpp:61-62)
+//   slot_shade[n_slots] int4 {v0, v1, v2, material}  (vertex indices, material)
+//   slot_light[n_slots] int  area light index or -1
+//   normals, tangents [n_vertices] float4 (xyz used)
+//   spheres [n_spheres] BbSphere
+//   materials [n_materials] GpuMaterial, lights [n_lights] GpuLight,
+//   light_cdf [n_lights + 1], light_func [n_lights]
+#pragma once
+
+#include "../../include/bbgpu.h"
+#include "bb_math.h"
+
+#include <string>
+#include <vector>
+
+namespace bb
+{
+
+enum : int32_t
+{
+    KIND_TRIANGLE = 0,
+    KIND_SPHERE = 1
+};
+
+// BxDF flags, include/bulbit/bxdf.h:7-22
+enum : int32_t
+{
+    BX_REFLECTION = 1,
+    BX_TRANSMISSION = 2,
+    BX_DIFFUSE = 4,
+    BX_GLOSSY = 8,
+    BX_SPECULAR = 16
+};
+
+enum : int32_t
+{
+    MK_LAMBERT = 0,
+    MK_DIELECTRIC = 1,
+    MK_CONDUCTOR = 2,
+    MK_METALLIC_ROUGHNESS = 3
+};
+
+// Material with its BxDF parameters precomputed on the host
+// (the Material::GetBSDF arithmetic that does not depend on the hit).
+struct GpuMaterial
+{
+    int32_t kind;
+    float eta;   // dielectric eta (front face); the back face uses 1 / eta
+    float ax, ay;
+    spec r;      // Lambert reflectance | dielectric Sqrt(reflectance) | conductor lobe reflectance
+    spec ceta;   // conductor eta
+    spec ck;     // conductor k
+    spec color;  // metallic-roughness base colour
+    float metallic;
+    int32_t flags_front; // BxDF flags (front and back face give the same flags)
+    float pad[2];
+};
+
+struct GpuLight
+{
+    int32_t type;
+    int32_t two_sided;
+    float cos_min, cos_max; // spot
+    f3 p;                   // spot position | triangle p0
+    float world_radius;     // infinite
+    f3 e1, e2;              // triangle edges
+    f3 z;                   // spot frame z
+    f3 n;                   // triangle unit normal (Triangle::Sample)
+    spec L;                 // spot intensity | infinite scale * l | area emission
+    float area;             // triangle area (Triangle::Sample, Triangle::Area)
+    float pmf;              // PowerLightSampler::EvaluatePMF
+    float pad;
+};
+
+struct SceneView
+{
+    const float4* nodes; // 2 float4 per node
+    const float4* tris;  // 3 float4 per slot
+    const int4* slot_shade;
+    const int32_t* slot_light;
+    const float4* normals;
+    const float4* tangents;
+    const BbSphere* spheres;
+    const GpuMaterial* materials;
+    const GpuLight* lights;
+    const float* light_cdf;
+    const float* light_func;
+    const int32_t* inf_lights;
+    int32_t n_lights;
+    int32_t n_inf;
+    float func_integral;
+    int32_t stack_depth;
+    int32_t has_null_material;
+};
+
+// Host-side build products
+struct HostBvh
+{
+    std::vector<BbBvhNode> nodes;
+    std::vector<int32_t> ordered; // slot -> primitive
+    int32_t max_depth = 0;
+};
+
+// Builds the reference's SAH BVH (src/accel/bvh.cpp:56-297) on the host.
+bool build_bvh(const BbSceneDesc& d, HostBvh& out, std::string& err);
+
+} // namespace bb
+
+struct BbScene
+{
+    int device = 0;
+    int32_t n_prims = 0, n_slots = 0, n_nodes = 0, n_lights = 0, n_inf = 0;
+    int32_t max_depth = 0;
+    float world_min[3], world_max[3];
+    double build_seconds = 0.0;
+    size_t device_bytes = 0;
+    bb::SceneView view{};
+    std::vector<void*> allocations;
+};

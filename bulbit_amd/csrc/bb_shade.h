@@ -1,0 +1,576 @@
+// bb_shade.h — device-side BSDFs, microfacet sampling, Fresnel terms, light
+// sampling and MIS for the wavefront path tracer (gfx950).
+//
+// Restates, in the reference's evaluation order:
+//   include/bulbit/sampling.h:42-48,64-74,114-169,203-232   sampling, MIS
+//   include/bulbit/scattering.h:9-132                        Refract, Fresnel
+//   include/bulbit/microfacet.h:13-93                        Trowbridge-Reitz (GGX)
+//   src/bxdf/{lambertian,dielectric,conductor,metallic_roughness}_bxdf.cpp
+//   include/bulbit/bsdf.h:75-125                             BSDF wrapper
+//   src/light/{spot_light,uniform_infinite_light,diffuse_area_light}.cpp
+//
+// Transcendentals: sin, cos and pow are evaluated in double precision and
+// rounded once to float, which reproduces a correctly rounded float result;
+// the reference's glibc sinf/cosf agree with it except in ~1% of arguments
+// (by one ulp).  Division and sqrt are IEEE (correctly rounded) on gfx950
+// with the library's compile flags.
+#pragma once
+
+#include "bb_scene.h"
+
+namespace bb
+{
+
+BB_D float dsin(float x) { return (float)::sin((double)x); }
+BB_D float dcos(float x) { return (float)::cos((double)x); }
+BB_D float dpow(float x, float y) { return (float)::pow((double)x, (double)y); }
+
+// ------------------------------------------------------------------ sampling
+
+// SampleUniformUnitDiskConcentric, sampling.h:114-134
+BB_D f2 sample_concentric(f2 u)
+{
+    f2 uo = mk2(u.x * 2.0f - 1.0f, u.y * 2.0f - 1.0f);
+    if (uo.x == 0.0f && uo.y == 0.0f) return mk2(0.0f, 0.0f);
+    float theta, r;
+    if (fabsf(uo.x) > fabsf(uo.y)) {
+        r = uo.x;
+        theta = (kPi / 4.0f) * (uo.y / uo.x);
+    } else {
+        r = uo.y;
+        theta = (kPi / 2.0f) - (kPi / 4.0f) * (uo.x / uo.y);
+    }
+    return mk2(dcos(theta) * r, dsin(theta) * r);
+}
+
+// SampleUniformSphere, sampling.h:64-74
+BB_D f3 sample_uniform_sphere(f2 u)
+{
+    float z = 1.0f - 2.0f * u.x;
+    float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+    float phi = kTwoPi * u.y;
+    return mk3(r * dcos(phi), r * dsin(phi), z);
+}
+
+// SampleCosineHemisphere (Malley), sampling.h:151-169
+BB_D f3 sample_cosine_hemisphere(f2 u)
+{
+    f2 d = sample_concentric(u);
+    float z = safe_sqrt(1.0f - sqr(d.x) - sqr(d.y));
+    return mk3(d.x, d.y, z);
+}
+
+// PowerHeuristic(1, f, 1, g), sampling.h:42-48
+BB_D float power_heuristic(float pdf_f, float pdf_g)
+{
+    float f = 1.0f * pdf_f;
+    float g = 1.0f * pdf_g;
+    return (f * f) / (f * f + g * g);
+}
+
+// ------------------------------------------------------------------ Fresnel
+
+// FresnelDielectric, scattering.h:51-77
+BB_D float fresnel_dielectric(float cos_theta_i, float eta)
+{
+    cos_theta_i = clampf_(cos_theta_i, -1.0f, 1.0f);
+    if (cos_theta_i < 0.0f) {
+        eta = 1.0f / eta;
+        cos_theta_i = -cos_theta_i;
+    }
+    float sin2_theta_i = 1.0f - sqr(cos_theta_i);
+    float sin2_theta_t = sin2_theta_i / sqr(eta);
+    if (sin2_theta_t >= 1.0f) return 1.0f;
+    float cos_theta_t = sqrtf(smax(0.0f, 1.0f - sin2_theta_t));
+    float r_parl = (eta * cos_theta_i - cos_theta_t) / (eta * cos_theta_i + cos_theta_t);
+    float r_perp = (cos_theta_i - eta * cos_theta_t) / (cos_theta_i + eta * cos_theta_t);
+    return (sqr(r_parl) + sqr(r_perp)) / 2.0f;
+}
+
+// std::complex<float> as the reference's compiler evaluates it: float
+// multiplication, division by the naive formula in double, |z| = hypot.
+struct cpx
+{
+    float re, im;
+};
+BB_D cpx cmul(cpx a, cpx b) { return cpx{ a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re }; }
+BB_D cpx cdiv(cpx a, cpx b)
+{
+    double ar = a.re, ai = a.im, br = b.re, bi = b.im;
+    double den = br * br + bi * bi;
+    return cpx{ (float)((ar * br + ai * bi) / den), (float)((ai * br - ar * bi) / den) };
+}
+BB_D float cabs_(cpx z)
+{
+    double r = z.re, i = z.im;
+    return (float)::sqrt(r * r + i * i);
+}
+BB_D float cnorm(cpx z) { return z.re * z.re + z.im * z.im; }
+
+// SqrtComplex, scattering.h:81-100
+BB_D cpx csqrt_(cpx z)
+{
+    float n = cabs_(z);
+    float t1 = sqrtf(0.5f * (n + fabsf(z.re)));
+    float t2 = 0.5f * z.im / t1;
+    if (n == 0.0f) return cpx{ 0.0f, 0.0f };
+    if (z.re >= 0.0f) return cpx{ t1, t2 };
+    return cpx{ fabsf(t2), copysignf(t1, z.im) };
+}
+
+// FresnelComplex, scattering.h:102-121
+BB_D float fresnel_complex(float cos_theta_i, cpx eta)
+{
+    cos_theta_i = clampf_(cos_theta_i, 0.0f, 1.0f);
+    float sin2_theta_i = 1.0f - sqr(cos_theta_i);
+    cpx sin2_theta_t = cdiv(cpx{ sin2_theta_i, 0.0f }, cmul(eta, eta));
+    cpx cos_theta_t = csqrt_(cpx{ 1.0f - sin2_theta_t.re, 0.0f - sin2_theta_t.im });
+    cpx ec = cpx{ eta.re * cos_theta_i, eta.im * cos_theta_i };
+    cpx r_parl = cdiv(cpx{ ec.re - cos_theta_t.re, ec.im - cos_theta_t.im }, cpx{ ec.re + cos_theta_t.re, ec.im + cos_theta_t.im });
+    cpx et = cmul(eta, cos_theta_t);
+    cpx r_perp = cdiv(cpx{ cos_theta_i - et.re, -et.im }, cpx{ cos_theta_i + et.re, et.im });
+    return (cnorm(r_parl) + cnorm(r_perp)) / 2.0f;
+}
+
+BB_D spec fresnel_complex_s(float c, spec eta, spec k)
+{
+    return mks(fresnel_complex(c, cpx{ eta.r, k.r }), fresnel_complex(c, cpx{ eta.g, k.g }),
+               fresnel_complex(c, cpx{ eta.b, k.b }));
+}
+
+// Refract, scattering.h:9-39
+BB_D bool refract(f3& wt, f3 wi, f3 n, float eta, float& eta_p)
+{
+    float cos_theta_i = dot(n, wi);
+    if (cos_theta_i < 0.0f) {
+        eta = 1.0f / eta;
+        cos_theta_i = -cos_theta_i;
+        n = -n;
+    }
+    float sin2_theta_i = smax(0.0f, 1.0f - sqr(cos_theta_i));
+    float sin2_theta_t = sin2_theta_i / sqr(eta);
+    if (sin2_theta_t >= 1.0f) return false;
+    float cos_theta_t = sqrtf(1.0f - sin2_theta_t);
+    float inv_eta = 1.0f / eta;
+    wt = (-wi) * inv_eta + n * (cos_theta_i * inv_eta - cos_theta_t);
+    eta_p = eta;
+    return true;
+}
+
+// ---------------------------------------------------- Trowbridge-Reitz (GGX)
+struct ggx
+{
+    float ax, ay;
+};
+BB_D bool ggx_smooth(ggx m) { return smax(m.ax, m.ay) < 1e-3f; }
+
+BB_D float ggx_D(ggx m, f3 wm)
+{
+    float tan2 = tan2_theta(wm);
+    if (isinf(tan2)) return 0.0f;
+    float cos4 = sqr(cos2_theta(wm));
+    if (cos4 < 1e-16f) return 0.0f;
+    float e = tan2 * (sqr(cos_phi(wm) / m.ax) + sqr(sin_phi(wm) / m.ay));
+    return 1.0f / (kPi * m.ax * m.ay * cos4 * sqr(1.0f + e));
+}
+BB_D float ggx_lambda(ggx m, f3 w)
+{
+    float tan2 = tan2_theta(w);
+    if (isinf(tan2)) return 0.0f;
+    float alpha2 = sqr(cos_phi(w) * m.ax) + sqr(sin_phi(w) * m.ay);
+    return (sqrtf(1.0f + alpha2 * tan2) - 1.0f) / 2.0f;
+}
+BB_D float ggx_G1(ggx m, f3 w) { return 1.0f / (1.0f + ggx_lambda(m, w)); }
+BB_D float ggx_G(ggx m, f3 wo, f3 wi) { return 1.0f / (1.0f + ggx_lambda(m, wo) + ggx_lambda(m, wi)); }
+BB_D float ggx_pdf(ggx m, f3 w, f3 wm) { return ggx_G1(m, w) / abs_cos_theta(w) * ggx_D(m, wm) * absdot(w, wm); }
+
+// Sample_GGX_VNDF_Dupuy_Benyoub, sampling.h:203-232 (explicit fma at :210)
+BB_D f3 ggx_sample_wm(ggx m, f3 wo, f2 u)
+{
+    f3 wo_std = normalize(mk3(wo.x * m.ax, wo.y * m.ay, wo.z));
+    float phi = kTwoPi * u.x;
+    float z = fmaf((1.0f - u.y), (1.0f + wo_std.z), -wo_std.z);
+    float sin_t = sqrtf(clampf_(1.0f - z * z, 0.0f, 1.0f));
+    float x = sin_t * dcos(phi);
+    float y = sin_t * dsin(phi);
+    f3 h = mk3(x, y, z) + wo_std;
+    return normalize(mk3(h.x * m.ax, h.y * m.ay, h.z));
+}
+
+// -------------------------------------------------------------------- BxDFs
+
+struct bxdf
+{
+    int32_t kind;
+    int32_t flags;
+    float eta;
+    ggx mf;
+    spec r, ceta, ck, color;
+    float metallic;
+};
+
+struct bsample
+{
+    spec f;
+    f3 wi;
+    float pdf;
+    int32_t flags;
+    float eta;
+};
+
+BB_D bxdf make_bxdf(const GpuMaterial& m, bool front_face)
+{
+    bxdf b;
+    b.kind = m.kind;
+    b.flags = m.flags_front;
+    b.eta = (m.kind == MK_DIELECTRIC) ? (front_face ? m.eta : 1.0f / m.eta) : 1.0f;
+    b.mf = ggx{ m.ax, m.ay };
+    b.r = m.r;
+    b.ceta = m.ceta;
+    b.ck = m.ck;
+    b.color = m.color;
+    b.metallic = m.metallic;
+    return b;
+}
+
+// Lambertian, lambertian_bxdf.cpp:8-53
+BB_D spec lambert_f(const bxdf& b, f3 wo, f3 wi)
+{
+    if (!same_hemisphere(wo, wi)) return mks(0.0f);
+    return b.r * kInvPi;
+}
+BB_D float lambert_pdf(f3 wo, f3 wi)
+{
+    if (!same_hemisphere(wo, wi)) return 0.0f;
+    return abs_cos_theta(wi) * kInvPi;
+}
+
+// Conductor, conductor_bxdf.cpp:7-112
+BB_D spec conductor_f(const bxdf& b, f3 wo, f3 wi)
+{
+    if (!same_hemisphere(wo, wi) || ggx_smooth(b.mf)) return mks(0.0f);
+    float co = abs_cos_theta(wo), ci = abs_cos_theta(wi);
+    if (ci == 0.0f || co == 0.0f) return mks(0.0f);
+    f3 wm = wo + wi;
+    if (length2(wm) == 0.0f) return mks(0.0f);
+    normalize_m(wm);
+    spec F = fresnel_complex_s(absdot(wo, wm), b.ceta, b.ck);
+    return (((b.r * F) * ggx_D(b.mf, wm)) * ggx_G(b.mf, wo, wi)) / (4.0f * ci * co);
+}
+BB_D float conductor_pdf(const bxdf& b, f3 wo, f3 wi)
+{
+    if (!same_hemisphere(wo, wi) || ggx_smooth(b.mf)) return 0.0f;
+    f3 wm = wo + wi;
+    if (length2(wm) == 0.0f) return 0.0f;
+    normalize_m(wm);
+    if (dot_z(wm) < 0.0f) wm = -wm;
+    return ggx_pdf(b.mf, wo, wm) / (4.0f * absdot(wo, wm));
+}
+
+// Dielectric, dielectric_bxdf.cpp:9-129
+BB_D spec dielectric_f(const bxdf& b, f3 wo, f3 wi)
+{
+    if (b.eta == 1.0f || ggx_smooth(b.mf)) return mks(0.0f);
+    float co = wo.z, ci = wi.z;
+    float eta_p = 1.0f;
+    bool refl = ci * co > 0.0f;
+    if (!refl) eta_p = co > 0.0f ? b.eta : (1.0f / b.eta);
+    f3 wm = wi * eta_p + wo;
+    if (ci == 0.0f || co == 0.0f || length2(wm) == 0.0f) return mks(0.0f);
+    normalize_m(wm);
+    if (dot_z(wm) < 0.0f) wm = -wm;
+    if (dot(wm, wi) * ci < 0.0f || dot(wm, wo) * co < 0.0f) return mks(0.0f);
+    float F = fresnel_dielectric(dot(wo, wm), b.eta);
+    if (refl) return mks(F * ggx_D(b.mf, wm) * ggx_G(b.mf, wo, wi) / fabsf(4.0f * ci * co));
+    float denom = sqr(dot(wi, wm) + dot(wo, wm) / eta_p) * ci * co;
+    float ft = (1.0f - F) * ggx_D(b.mf, wm) * ggx_G(b.mf, wo, wi) * fabsf(dot(wi, wm) * dot(wo, wm) / denom);
+    ft /= sqr(eta_p);
+    return b.r * ft;
+}
+BB_D float dielectric_pdf(const bxdf& b, f3 wo, f3 wi)
+{
+    if (b.eta == 1.0f || ggx_smooth(b.mf)) return 0.0f;
+    float co = wo.z, ci = wi.z;
+    float eta_p = 1.0f;
+    bool refl = ci * co > 0.0f;
+    if (!refl) eta_p = co > 0.0f ? b.eta : (1.0f / b.eta);
+    f3 wm = wi * eta_p + wo;
+    if (ci == 0.0f || co == 0.0f || length2(wm) == 0.0f) return 0.0f;
+    normalize_m(wm);
+    if (dot_z(wm) < 0.0f) wm = -wm;
+    if (dot(wm, wi) * ci < 0.0f || dot(wm, wo) * co < 0.0f) return 0.0f;
+    float R = fresnel_dielectric(dot(wo, wm), b.eta);
+    float T = 1.0f - R;
+    float pr = R, pt = T;
+    if (pr == 0.0f && pt == 0.0f) return 0.0f;
+    if (refl) return ggx_pdf(b.mf, wo, wm) / (4.0f * absdot(wo, wm)) * pr / (pr + pt);
+    float dwm_dwi = absdot(wi, wm) / sqr(dot(wi, wm) + dot(wo, wm) / eta_p);
+    return ggx_pdf(b.mf, wo, wm) * dwm_dwi * pt / (pr + pt);
+}
+
+// Metallic-roughness, metallic_roughness_bxdf.cpp:8-80, bxdfs.h:466-489
+BB_D spec mr_F0(const bxdf& b) { return mks(0.04f) * (1.0f - b.metallic) + b.color * b.metallic; }
+BB_D spec mr_schlick(spec f0, float c) { return f0 + (mks(1.0f) - f0) * dpow(1.0f - c, 5.0f); }
+BB_D spec mr_f(const bxdf& b, f3 wo, f3 wi)
+{
+    if (!same_hemisphere(wo, wi)) return mks(0.0f);
+    float co = abs_cos_theta(wo), ci = abs_cos_theta(wi);
+    if (ci == 0.0f || co == 0.0f) return mks(0.0f);
+    f3 wm = wo + wi;
+    if (length2(wm) == 0.0f) return mks(0.0f);
+    normalize_m(wm);
+    spec F = mr_schlick(mr_F0(b), dot(wi, wm));
+    spec f_s = ((F * ggx_D(b.mf, wm)) * ggx_G(b.mf, wo, wi)) / (4.0f * ci * co);
+    spec f_d = ((mks(1.0f) - F) * (1.0f - b.metallic)) * (b.color * kInvPi);
+    return f_d + f_s;
+}
+BB_D float mr_pdf(const bxdf& b, f3 wo, f3 wi)
+{
+    if (!same_hemisphere(wo, wi) || ggx_smooth(b.mf)) return 0.0f;
+    f3 wm = wo + wi;
+    if (length2(wm) == 0.0f) return 0.0f;
+    normalize_m(wm);
+    if (dot_z(wm) < 0.0f) wm = -wm;
+    float R = fresnel_dielectric(dot(wo, wm), 1.5f);
+    float T = 1.0f - R;
+    float pr = R, pt = T * (1.0f - b.metallic);
+    if (pr == 0.0f && pt == 0.0f) return 0.0f;
+    float p_sum = pr + pt;
+    pr /= p_sum;
+    pt /= p_sum;
+    float pdf_r = ggx_pdf(b.mf, wo, wm) / (4.0f * absdot(wo, wm));
+    float pdf_d = abs_cos_theta(wi) * kInvPi;
+    return pr * pdf_r + pt * pdf_d;
+}
+
+// BSDF::f / BSDF::PDF on local directions (bsdf.h:75-99)
+BB_D spec bxdf_f(const bxdf& b, f3 wo, f3 wi)
+{
+    switch (b.kind) {
+    case MK_LAMBERT: return lambert_f(b, wo, wi);
+    case MK_DIELECTRIC: return dielectric_f(b, wo, wi);
+    case MK_CONDUCTOR: return conductor_f(b, wo, wi);
+    default: return mr_f(b, wo, wi);
+    }
+}
+BB_D float bxdf_pdf(const bxdf& b, f3 wo, f3 wi)
+{
+    switch (b.kind) {
+    case MK_LAMBERT: return lambert_pdf(wo, wi);
+    case MK_DIELECTRIC: return dielectric_pdf(b, wo, wi);
+    case MK_CONDUCTOR: return conductor_pdf(b, wo, wi);
+    default: return mr_pdf(b, wo, wi);
+    }
+}
+
+// BxDF::Sample_f on local wo (returns false for a failed sample)
+BB_D bool bxdf_sample(const bxdf& b, f3 wo, float u0, f2 u12, bsample& s)
+{
+    s.eta = 1.0f;
+    switch (b.kind) {
+    case MK_LAMBERT: {
+        f3 wi = sample_cosine_hemisphere(u12);
+        float pdf = wi.z * kInvPi;
+        if (wo.z < 0.0f) wi.z = -wi.z;
+        s.f = b.r * kInvPi;
+        s.wi = wi;
+        s.pdf = pdf;
+        s.flags = BX_DIFFUSE | BX_REFLECTION;
+        return true;
+    }
+    case MK_CONDUCTOR: {
+        if (ggx_smooth(b.mf)) {
+            f3 wi = mk3(-wo.x, -wo.y, wo.z);
+            spec f = fresnel_complex_s(abs_cos_theta(wi), b.ceta, b.ck) / abs_cos_theta(wi);
+            s.f = b.r * f;
+            s.wi = wi;
+            s.pdf = 1.0f;
+            s.flags = BX_SPECULAR | BX_REFLECTION;
+            return true;
+        }
+        if (wo.z == 0.0f) return false;
+        f3 wm = ggx_sample_wm(b.mf, wo, u12);
+        f3 wi = reflect(wo, wm);
+        if (!same_hemisphere(wo, wi)) return false;
+        float co = abs_cos_theta(wo), ci = abs_cos_theta(wi);
+        if (ci == 0.0f || co == 0.0f) return false;
+        float pdf = ggx_pdf(b.mf, wo, wm) / (4.0f * absdot(wo, wm));
+        spec F = fresnel_complex_s(absdot(wo, wm), b.ceta, b.ck);
+        spec f = ((F * ggx_D(b.mf, wm)) * ggx_G(b.mf, wo, wi)) / (4.0f * ci * co);
+        s.f = b.r * f;
+        s.wi = wi;
+        s.pdf = pdf;
+        s.flags = BX_GLOSSY | BX_REFLECTION;
+        return true;
+    }
+    case MK_DIELECTRIC: {
+        if (b.eta == 1.0f || ggx_smooth(b.mf)) {
+            float R = fresnel_dielectric(wo.z, b.eta);
+            float T = 1.0f - R;
+            float pr = R, pt = T;
+            if (pr == 0.0f && pt == 0.0f) return false;
+            if (u0 < pr / (pr + pt)) {
+                f3 wi = mk3(-wo.x, -wo.y, wo.z);
+                s.f = mks(R / abs_cos_theta(wi));
+                s.wi = wi;
+                s.pdf = pr / (pr + pt);
+                s.flags = BX_SPECULAR | BX_REFLECTION;
+            } else {
+                f3 wi;
+                float eta_p;
+                if (!refract(wi, wo, mk3(0.0f, 0.0f, 1.0f), b.eta, eta_p)) return false;
+                float ft = T / abs_cos_theta(wi);
+                ft /= sqr(eta_p);
+                s.f = b.r * ft;
+                s.wi = wi;
+                s.pdf = pt / (pr + pt);
+                s.flags = BX_SPECULAR | BX_TRANSMISSION;
+                s.eta = eta_p;
+            }
+            return true;
+        }
+        f3 wm = ggx_sample_wm(b.mf, wo, u12);
+        float R = fresnel_dielectric(dot(wo, wm), b.eta);
+        float T = 1.0f - R;
+        float pr = R, pt = T;
+        if (pr == 0.0f && pt == 0.0f) return false;
+        if (u0 < pr / (pr + pt)) {
+            f3 wi = reflect(wo, wm);
+            if (!same_hemisphere(wo, wi)) return false;
+            float pdf = ggx_pdf(b.mf, wo, wm) / (4.0f * absdot(wo, wm)) * pr / (pr + pt);
+            s.f = mks(R * ggx_D(b.mf, wm) * ggx_G(b.mf, wo, wi) / (4.0f * wi.z * wo.z));
+            s.wi = wi;
+            s.pdf = pdf;
+            s.flags = BX_GLOSSY | BX_REFLECTION;
+        } else {
+            float eta_p = 0.0f;
+            f3 wi = mk3(0.0f);
+            bool tir = !refract(wi, wo, wm, b.eta, eta_p);
+            if (same_hemisphere(wo, wi) || wi.z == 0.0f || tir) return false;
+            float denom = sqr(dot(wi, wm) + dot(wo, wm) / eta_p);
+            float dwm_dwi = absdot(wi, wm) / denom;
+            float pdf = ggx_pdf(b.mf, wo, wm) * dwm_dwi * pt / (pr + pt);
+            float ft = T * ggx_D(b.mf, wm) * ggx_G(b.mf, wo, wi) * fabsf(dot(wi, wm) * dot(wo, wm) / (wi.z * wo.z * denom));
+            ft /= sqr(eta_p);
+            s.f = b.r * ft;
+            s.wi = wi;
+            s.pdf = pdf;
+            s.flags = BX_GLOSSY | BX_TRANSMISSION;
+            s.eta = eta_p;
+        }
+        return true;
+    }
+    default: {
+        if (wo.z == 0.0f) return false;
+        f3 wm = ggx_sample_wm(b.mf, wo, u12);
+        f3 wi = reflect(wo, wm);
+        float R = fresnel_dielectric(dot(wo, wm), 1.5f);
+        float T = 1.0f - R;
+        float pr = R, pt = T * (1.0f - b.metallic);
+        if (pr == 0.0f && pt == 0.0f) return false;
+        float p_sum = pr + pt;
+        pr /= p_sum;
+        pt /= p_sum;
+        int32_t flag;
+        if (u0 < pr) {
+            if (!same_hemisphere(wo, wi)) return false;
+            flag = BX_GLOSSY | BX_REFLECTION;
+        } else {
+            wi = sample_cosine_hemisphere(u12);
+            wm = normalize(wi + wo);
+            flag = BX_DIFFUSE | BX_REFLECTION;
+        }
+        float co = abs_cos_theta(wo), ci = abs_cos_theta(wi);
+        if (ci == 0.0f || co == 0.0f) return false;
+        spec F = mr_schlick(mr_F0(b), dot(wi, wm));
+        spec f_r = ((F * ggx_D(b.mf, wm)) * ggx_G(b.mf, wo, wi)) / (4.0f * ci * co);
+        spec f_d = ((mks(1.0f) - F) * (1.0f - b.metallic)) * (b.color * kInvPi);
+        float pdf_r = ggx_pdf(b.mf, wo, wm) / (4.0f * absdot(wo, wm));
+        float pdf_d = ci * kInvPi;
+        s.f = f_r + f_d;
+        s.wi = wi;
+        s.pdf = pr * pdf_r + pt * pdf_d;
+        s.flags = flag;
+        return true;
+    }
+    }
+}
+
+// ------------------------------------------------------------------- lights
+
+// Distribution1D::SampleDiscrete + FindInterval (sampling.h:383-397, math.h:117-135)
+BB_D int32_t light_select(const SceneView& S, float u, float& pmf)
+{
+    int32_t size = S.n_lights + 1;
+    int32_t first = 0, len = size;
+    while (len > 0) {
+        int32_t half = len >> 1, middle = first + half;
+        if (S.light_cdf[middle] <= u) {
+            first = middle + 1;
+            len -= half + 1;
+        } else {
+            len = half;
+        }
+    }
+    int32_t offset = first - 1;
+    if (offset < 0) offset = 0;
+    if (offset > size - 2) offset = size - 2;
+    pmf = S.light_func[offset] / (S.func_integral * float(S.n_lights));
+    return offset;
+}
+
+struct light_sample
+{
+    f3 wi;
+    float visibility;
+    spec Li;
+    float pdf;
+};
+
+// Light::Sample_Li for spot, uniform infinite and diffuse area (triangle) lights
+BB_D bool light_sample_li(const GpuLight& L, f3 ref, f2 u, light_sample& ls)
+{
+    if (L.type == BB_LIGHT_SPOT) {
+        // spot_light.cpp:46-68
+        f3 wi = L.p - ref;
+        float distance = normalize_m(wi);
+        float cos_t = -dot(wi, L.z);
+        spec l = L.L * smooth_step(L.cos_max, L.cos_min, cos_t);
+        if (is_black(l)) return false;
+        ls.wi = wi;
+        ls.visibility = distance;
+        ls.pdf = 1.0f;
+        ls.Li = l / (distance * distance);
+        return true;
+    }
+    if (L.type == BB_LIGHT_UNIFORM_INFINITE) {
+        // uniform_infinite_light.cpp:30-43
+        ls.wi = sample_uniform_sphere(u);
+        ls.pdf = kInvFourPi;
+        ls.visibility = 2.0f * L.world_radius;
+        ls.Li = L.L;
+        return true;
+    }
+    // diffuse_area_light.cpp:42-72 with Triangle::Sample(ref, u), triangle.cpp:159-229
+    float uu = u.x, vv = u.y;
+    if (uu + vv > 1.0f) {
+        uu = 1.0f - uu;
+        vv = 1.0f - vv;
+    }
+    f3 point = (L.p + L.e1 * uu) + L.e2 * vv;
+    float pdf = 1.0f / L.area;
+    f3 d = point - ref;
+    float dist2 = dot(d, d);
+    float cosine = dot(d, L.n) / sqrtf(dist2);
+    if (cosine < 0.0f) cosine = -cosine;
+    pdf *= dist2 / cosine;
+    f3 wi = point - ref;
+    ls.visibility = normalize_m(wi) - kRayEps;
+    ls.wi = wi;
+    ls.pdf = pdf;
+    bool front = dot(L.n, wi) < 0.0f;
+    ls.Li = (front || L.two_sided) ? L.L : mks(0.0f);
+    return true;
+}
+
+} // namespace bb

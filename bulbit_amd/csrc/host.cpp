@@ -1,0 +1,726 @@
+// host.cpp — host side of libbbgpu.so: the SAH BVH build, scene packing and
+// upload, error reporting and host-only helpers of the C ABI.
+#include "bb_scene.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <future>
+#include <memory>
+
+namespace bb
+{
+
+static thread_local std::string g_last_error;
+
+void set_error(const char* fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+}
+
+// ------------------------------------------------------------------- BVH
+
+namespace
+{
+
+struct BPrim
+{
+    int32_t index;
+    f3 bmin, bmax;
+};
+
+struct BNode
+{
+    f3 bmin, bmax;
+    int32_t axis, offset, count;
+    int32_t c1, c2;
+};
+
+// bounding_box.h:376-381
+inline float box_area(f3 mn, f3 mx)
+{
+    f3 w = mx - mn;
+    return 2.0f * ((w.x * w.y) + (w.y * w.z) + (w.z * w.x));
+}
+
+inline f3 empty_min() { return mk3(FLT_MAX); }
+inline f3 empty_max() { return mk3(-FLT_MAX); }
+
+inline float center(const BPrim& p, int axis) { return (comp(p.bmin, axis) + comp(p.bmax, axis)) * 0.5f; }
+
+// bvh.cpp:162-168 and 215-221
+inline int bucket_of(const BPrim& p, int axis, float cmin, float extent)
+{
+    int b = int(12.0f * (center(p, axis) - cmin) / extent);
+    if (b == 12) b = 11;
+    return b;
+}
+
+struct Builder
+{
+    std::vector<BPrim> prims;
+    std::vector<BNode> pool;
+    std::atomic<int32_t> next{ 0 };
+    std::atomic<int32_t> too_big{ 0 };
+
+    int32_t alloc() { return next.fetch_add(1, std::memory_order_relaxed); }
+
+    int32_t leaf(int32_t start, int32_t n, f3 mn, f3 mx)
+    {
+        int32_t id = alloc();
+        BNode& nd = pool[id];
+        nd.bmin = mn;
+        nd.bmax = mx;
+        nd.axis = 0;
+        // Leaves cover consecutive spans in depth-first order, so the
+        // reference's running offset (bvh.cpp:78-85) equals the span start.
+        nd.offset = start;
+        nd.count = n;
+        nd.c1 = nd.c2 = -1;
+        if (n > 65535) too_big = 1;
+        return id;
+    }
+
+    // BVH::BuildRecursive, src/accel/bvh.cpp:56-271
+    int32_t build(int32_t start, int32_t end, int depth)
+    {
+        int32_t n = end - start;
+        BPrim* P = prims.data() + start;
+        f3 mn = empty_min(), mx = empty_max();
+        for (int32_t i = 0; i < n; ++i) {
+            mn = vmin(mn, P[i].bmin);
+            mx = vmax(mx, P[i].bmax);
+        }
+        if (box_area(mn, mx) == 0.0f || n == 1) return leaf(start, n, mn, mx);
+
+        f3 cmn = empty_min(), cmx = empty_max();
+        for (int32_t i = 0; i < n; ++i) {
+            f3 c = (P[i].bmin + P[i].bmax) * 0.5f;
+            cmn = vmin(cmn, c);
+            cmx = vmax(cmx, c);
+        }
+        f3 ext = cmx - cmn;
+        int axis = 0;
+        if (ext.y > ext.x) axis = 1;
+        if (ext.z > comp(ext, axis)) axis = 2;
+        const float extent = comp(ext, axis);
+        if (extent == 0.0f) return leaf(start, n, mn, mx);
+
+        int32_t mid;
+        if (n <= 2) {
+            mid = n / 2;
+            std::nth_element(P, P + mid, P + n,
+                             [axis](const BPrim& a, const BPrim& b) { return center(a, axis) < center(b, axis); });
+        } else {
+            int count[12] = { 0 };
+            f3 bmn[12], bmx[12];
+            for (int b = 0; b < 12; ++b) {
+                bmn[b] = empty_min();
+                bmx[b] = empty_max();
+            }
+            const float cmin = comp(cmn, axis);
+            for (int32_t i = 0; i < n; ++i) {
+                int b = bucket_of(P[i], axis, cmin, extent);
+                count[b]++;
+                bmn[b] = vmin(bmn[b], P[i].bmin);
+                bmx[b] = vmax(bmx[b], P[i].bmax);
+            }
+            f3 lmn = empty_min(), lmx = empty_max(), rmn = empty_min(), rmx = empty_max();
+            int left_count[11], right_count[11];
+            float left_area[11], right_area[11];
+            int left_sum = 0, right_sum = 0;
+            for (int i = 0; i < 11; ++i) {
+                left_sum += count[i];
+                left_count[i] = left_sum;
+                lmn = vmin(lmn, bmn[i]);
+                lmx = vmax(lmx, bmx[i]);
+                left_area[i] = box_area(lmn, lmx);
+                right_sum += count[11 - i];
+                right_count[10 - i] = right_sum;
+                rmn = vmin(rmn, bmn[11 - i]);
+                rmx = vmax(rmx, bmx[11 - i]);
+                right_area[10 - i] = box_area(rmn, rmx);
+            }
+            int best = -1;
+            float min_cost = INFINITY;
+            for (int i = 0; i < 11; ++i) {
+                float cost = float(left_count[i]) * left_area[i] + float(right_count[i]) * right_area[i];
+                if (cost < min_cost) {
+                    min_cost = cost;
+                    best = i;
+                }
+            }
+            min_cost = 0.5f + min_cost / box_area(mn, mx);
+            if (!(min_cost < float(n))) return leaf(start, n, mn, mx);
+            // std::partition over a random-access range: libstdc++'s
+            // bidirectional form, the same call the reference makes
+            BPrim* it = std::partition(P, P + n, [=](const BPrim& p) { return bucket_of(p, axis, cmin, extent) <= best; });
+            mid = int32_t(it - P);
+        }
+
+        int32_t c1, c2;
+        if (n > 64 * 1024 && depth < 16) {
+            auto f = std::async(std::launch::async, [this, start, mid, depth] { return build(start, start + mid, depth + 1); });
+            c2 = build(start + mid, end, depth + 1);
+            c1 = f.get();
+        } else {
+            c1 = build(start, start + mid, depth + 1);
+            c2 = build(start + mid, end, depth + 1);
+        }
+        int32_t id = alloc();
+        BNode& nd = pool[id];
+        // BuildNode::InitInternal (include/bulbit/bvh.h:90-97)
+        nd.bmin = vmin(pool[c1].bmin, pool[c2].bmin);
+        nd.bmax = vmax(pool[c1].bmax, pool[c2].bmax);
+        nd.axis = axis;
+        nd.offset = 0;
+        nd.count = 0;
+        nd.c1 = c1;
+        nd.c2 = c2;
+        return id;
+    }
+};
+
+// BVH::FlattenBVH, src/accel/bvh.cpp:273-297: depth-first pre-order
+void flatten(const Builder& B, int32_t root, HostBvh& out)
+{
+    struct Item
+    {
+        int32_t node, parent, depth;
+    };
+    std::vector<Item> stack;
+    stack.push_back({ root, -1, 0 });
+    out.nodes.clear();
+    out.nodes.reserve(B.next.load());
+    out.max_depth = 0;
+    while (!stack.empty()) {
+        Item it = stack.back();
+        stack.pop_back();
+        const BNode& bn = B.pool[it.node];
+        int32_t idx = int32_t(out.nodes.size());
+        if (it.parent >= 0) out.nodes[it.parent].offset = idx; // this item is child2 of parent
+        BbBvhNode ln{};
+        ln.bmin[0] = bn.bmin.x;
+        ln.bmin[1] = bn.bmin.y;
+        ln.bmin[2] = bn.bmin.z;
+        ln.bmax[0] = bn.bmax.x;
+        ln.bmax[1] = bn.bmax.y;
+        ln.bmax[2] = bn.bmax.z;
+        out.max_depth = std::max(out.max_depth, it.depth);
+        if (bn.count > 0) {
+            ln.offset = bn.offset;
+            ln.count = uint16_t(bn.count);
+            ln.axis = 0;
+            out.nodes.push_back(ln);
+        } else {
+            ln.count = 0;
+            ln.axis = uint8_t(bn.axis);
+            out.nodes.push_back(ln);
+            // child1 is visited next (index + 1); child2 after child1's subtree
+            stack.push_back({ bn.c2, idx, it.depth + 1 });
+            stack.push_back({ bn.c1, -1, it.depth + 1 });
+        }
+    }
+}
+
+inline f3 load3(const float* a, int64_t i) { return mk3(a[3 * i], a[3 * i + 1], a[3 * i + 2]); }
+
+} // namespace
+
+bool validate_desc(const BbSceneDesc& d, std::string& err)
+{
+    char buf[256];
+    if (d.n_primitives <= 0 || !d.primitives) {
+        err = "scene has no primitives";
+        return false;
+    }
+    if (d.n_triangles > 0 && (!d.positions || !d.normals || !d.tangents || !d.indices)) {
+        err = "triangles need positions, normals, tangents and indices";
+        return false;
+    }
+    if (d.n_spheres > 0 && !d.spheres) {
+        err = "spheres pointer is null";
+        return false;
+    }
+    if (d.n_lights > 0 && !d.lights) {
+        err = "lights pointer is null";
+        return false;
+    }
+    if (d.n_materials > 0 && !d.materials) {
+        err = "materials pointer is null";
+        return false;
+    }
+    for (int64_t i = 0; i < 3 * int64_t(d.n_triangles); ++i)
+        if (d.indices[i] < 0 || d.indices[i] >= d.n_vertices) {
+            err = "vertex index out of range";
+            return false;
+        }
+    for (int32_t i = 0; i < d.n_primitives; ++i) {
+        const BbPrimitive& p = d.primitives[i];
+        bool ok = (p.shape_kind == BB_SHAPE_TRIANGLE && p.shape_index >= 0 && p.shape_index < d.n_triangles) ||
+                  (p.shape_kind == BB_SHAPE_SPHERE && p.shape_index >= 0 && p.shape_index < d.n_spheres);
+        if (!ok || p.material >= d.n_materials) {
+            snprintf(buf, sizeof(buf), "primitive %d: bad shape or material index", i);
+            err = buf;
+            return false;
+        }
+    }
+    return true;
+}
+
+bool build_bvh(const BbSceneDesc& d, HostBvh& out, std::string& err)
+{
+    if (!validate_desc(d, err)) return false;
+    const int32_t n = d.n_primitives;
+    Builder B;
+    B.prims.resize(n);
+    B.pool.resize(size_t(2) * size_t(n));
+    for (int32_t i = 0; i < n; ++i) {
+        const BbPrimitive& p = d.primitives[i];
+        BPrim& bp = B.prims[i];
+        bp.index = i;
+        if (p.shape_kind == BB_SHAPE_TRIANGLE) {
+            // Triangle::GetAABB, src/geometry/triangle.cpp:40-52
+            const int32_t* v = d.indices + 3 * int64_t(p.shape_index);
+            f3 p0 = load3(d.positions, v[0]), p1 = load3(d.positions, v[1]), p2 = load3(d.positions, v[2]);
+            f3 off = mk3(kEps * 10.0f);
+            bp.bmin = vmin(vmin(p0, p1), p2) - off;
+            bp.bmax = vmax(vmax(p0, p1), p2) + off;
+        } else {
+            // Sphere::GetAABB, src/geometry/sphere.cpp:30-33
+            const BbSphere& s = d.spheres[p.shape_index];
+            f3 c = mk3(s.p[0], s.p[1], s.p[2]);
+            bp.bmin = c - mk3(s.radius);
+            bp.bmax = c + mk3(s.radius);
+        }
+    }
+    int32_t root = B.build(0, n, 0);
+    if (B.too_big) {
+        err = "a BVH leaf holds more than 65535 primitives";
+        return false;
+    }
+    flatten(B, root, out);
+    out.ordered.resize(n);
+    for (int32_t i = 0; i < n; ++i) out.ordered[i] = B.prims[i].index;
+    return true;
+}
+
+// ------------------------------------------------------------ scene set-up
+
+namespace
+{
+
+bool material_to_gpu(const BbMaterial& m, GpuMaterial& g, std::string& err)
+{
+    memset(&g, 0, sizeof(g));
+    switch (m.type) {
+    case BB_MAT_DIFFUSE:
+        // DiffuseMaterial::GetBSDF: Lambertian when roughness <= 0 (diffuse_material.cpp:21-36)
+        if (m.u_roughness > 0.0f) {
+            err = "diffuse roughness > 0 (EON BxDF) is not supported";
+            return false;
+        }
+        g.kind = MK_LAMBERT;
+        g.r = mks(m.color[0], m.color[1], m.color[2]);
+        g.flags_front = !is_black(g.r) ? (BX_DIFFUSE | BX_REFLECTION) : 0;
+        return true;
+    case BB_MAT_DIELECTRIC: {
+        // DielectricMaterial::GetBSDF (dielectric_material.cpp:28-52), DielectricBxDF (bxdfs.h:80-90)
+        g.kind = MK_DIELECTRIC;
+        g.eta = m.eta[0];
+        g.ax = sqr(m.u_roughness);
+        g.ay = sqr(m.v_roughness);
+        g.r = sqrt_s(mks(m.color[0], m.color[1], m.color[2]));
+        bool smooth = smax(g.ax, g.ay) < 1e-3f;
+        if (m.energy_compensation && !smooth) {
+            err = "energy-compensated (multi-scattering) rough dielectric is not supported";
+            return false;
+        }
+        int f = (g.eta == 1.0f) ? BX_TRANSMISSION : (BX_REFLECTION | BX_TRANSMISSION);
+        g.flags_front = f | (smooth ? BX_SPECULAR : BX_GLOSSY);
+        return true;
+    }
+    case BB_MAT_CONDUCTOR:
+    case BB_MAT_CONDUCTOR_R: {
+        // ConductorMaterial::GetBSDF (conductor_material.cpp:52-99), ConductorBxDF (bxdfs.h:253-277)
+        if (m.energy_compensation) {
+            err = "energy-compensated (multi-scattering) conductor is not supported";
+            return false;
+        }
+        g.kind = MK_CONDUCTOR;
+        g.ax = sqr(m.u_roughness);
+        g.ay = sqr(m.v_roughness);
+        g.r = mks(m.color[0], m.color[1], m.color[2]);
+        if (m.type == BB_MAT_CONDUCTOR) {
+            g.ceta = mks(m.eta[0], m.eta[1], m.eta[2]);
+            g.ck = mks(m.k[0], m.k[1], m.k[2]);
+        } else {
+            const float hi = 1.0f - 1e-4f;
+            spec r = mks(clampf_(m.k[0], 0.0f, hi), clampf_(m.k[1], 0.0f, hi), clampf_(m.k[2], 0.0f, hi));
+            spec om = mks(1.0f) - r;
+            spec mx = mks(smax(0.0f, om.r), smax(0.0f, om.g), smax(0.0f, om.b));
+            g.ceta = mks(1.0f);
+            g.ck = (sqrt_s(r) * 2.0f) / sqrt_s(mx);
+        }
+        bool smooth = smax(g.ax, g.ay) < 1e-3f;
+        g.flags_front = smooth ? (BX_SPECULAR | BX_REFLECTION) : (BX_GLOSSY | BX_REFLECTION);
+        return true;
+    }
+    case BB_MAT_METALLIC_ROUGHNESS:
+        // MetallicRoughnessMaterial::GetBSDF (metallic_roughness_material.cpp:28-39)
+        g.kind = MK_METALLIC_ROUGHNESS;
+        g.color = mks(m.color[0], m.color[1], m.color[2]);
+        g.metallic = m.metallic;
+        g.ax = fmaxf(sqr(m.u_roughness), 1e-3f);
+        g.ay = fmaxf(sqr(m.v_roughness), 1e-3f);
+        g.flags_front = BX_DIFFUSE | BX_GLOSSY | BX_REFLECTION;
+        return true;
+    }
+    err = "unknown material type";
+    return false;
+}
+
+template <typename T>
+bool upload(BbScene* s, const T* host, size_t count, const T** dev_out, std::string& err)
+{
+    size_t bytes = std::max<size_t>(sizeof(T) * count, 16);
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        err = "hipMalloc failed";
+        return false;
+    }
+    s->allocations.push_back(p);
+    s->device_bytes += bytes;
+    if (count && hipMemcpy(p, host, sizeof(T) * count, hipMemcpyHostToDevice) != hipSuccess) {
+        err = "hipMemcpy failed";
+        return false;
+    }
+    *dev_out = reinterpret_cast<const T*>(p);
+    return true;
+}
+
+} // namespace
+
+bool create_scene(const BbSceneDesc& d, BbScene* s, std::string& err)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    HostBvh bvh;
+    if (!build_bvh(d, bvh, err)) return false;
+    s->build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    s->n_prims = d.n_primitives;
+    s->n_slots = d.n_primitives;
+    s->n_nodes = int32_t(bvh.nodes.size());
+    s->max_depth = bvh.max_depth;
+    for (int k = 0; k < 3; ++k) {
+        s->world_min[k] = bvh.nodes[0].bmin[k];
+        s->world_max[k] = bvh.nodes[0].bmax[k];
+    }
+
+    std::vector<GpuMaterial> mats(std::max(1, d.n_materials));
+    for (int32_t i = 0; i < d.n_materials; ++i)
+        if (!material_to_gpu(d.materials[i], mats[i], err)) return false;
+
+    // lights: Integrator::Integrator (integrator.cpp:107-147)
+    std::vector<int32_t> prim_light(d.n_primitives, -1);
+    std::vector<GpuLight> lights(std::max(1, d.n_lights));
+    std::vector<int32_t> inf;
+    f3 wmin = mk3(s->world_min[0], s->world_min[1], s->world_min[2]);
+    f3 wmax = mk3(s->world_max[0], s->world_max[1], s->world_max[2]);
+    std::vector<float> func(std::max(1, d.n_lights)), cdf(d.n_lights + 1);
+    for (int32_t i = 0; i < d.n_lights; ++i) {
+        const BbLight& L = d.lights[i];
+        GpuLight& g = lights[i];
+        memset(&g, 0, sizeof(g));
+        g.type = L.type;
+        g.two_sided = L.two_sided;
+        spec c = mks(L.spectrum[0], L.spectrum[1], L.spectrum[2]);
+        spec phi;
+        if (L.type == BB_LIGHT_SPOT) {
+            // SpotLight ctor (spot_light.cpp:7-24), Phi (:111-114)
+            g.p = mk3(L.position[0], L.position[1], L.position[2]);
+            g.z = -normalize(mk3(L.direction[0], L.direction[1], L.direction[2]));
+            g.cos_min = std::cos(L.angle_falloff_start * kPi / 180.0f);
+            g.cos_max = std::cos(L.angle_max * kPi / 180.0f);
+            g.L = c;
+            phi = (c * kTwoPi) * ((1.0f - g.cos_min) + (g.cos_min - g.cos_max) / 2.0f);
+        } else if (L.type == BB_LIGHT_UNIFORM_INFINITE) {
+            // UniformInfiniteLight::Preprocess (uniform_infinite_light.cpp:14-17) with
+            // AABB::ComputeBoundingSphere (bounding_box.h:524-528); Phi (:90-93)
+            f3 ctr = (wmin + wmax) * 0.5f;
+            bool inside = !(wmin.x > ctr.x || wmax.x < ctr.x) && !(wmin.y > ctr.y || wmax.y < ctr.y) &&
+                          !(wmin.z > ctr.z || wmax.z < ctr.z);
+            g.world_radius = inside ? length(wmax - ctr) : 0.0f;
+            g.L = c * L.scale;
+            phi = (((c * L.scale) * kFourPi) * kPi) * sqr(g.world_radius);
+            inf.push_back(i);
+        } else if (L.type == BB_LIGHT_DIFFUSE_AREA) {
+            if (L.primitive < 0 || L.primitive >= d.n_primitives ||
+                d.primitives[L.primitive].shape_kind != BB_SHAPE_TRIANGLE) {
+                err = "area lights must reference a triangle primitive";
+                return false;
+            }
+            const int32_t* v = d.indices + 3 * int64_t(d.primitives[L.primitive].shape_index);
+            f3 p0 = load3(d.positions, v[0]), p1 = load3(d.positions, v[1]), p2 = load3(d.positions, v[2]);
+            g.p = p0;
+            g.e1 = p1 - p0;
+            g.e2 = p2 - p0;
+            // Triangle::Sample (triangle.cpp:159-185): normal.Normalize() * 0.5f
+            f3 n = cross(g.e1, g.e2);
+            g.area = normalize_m(n) * 0.5f;
+            g.n = n;
+            g.L = c;
+            prim_light[L.primitive] = i;
+            // DiffuseAreaLight::Phi (diffuse_area_light.cpp:144-148), Triangle::Area (triangle.cpp:255-264)
+            float area = 0.5f * length(cross(g.e1, g.e2));
+            phi = ((c * area) * kPi) * float(L.two_sided ? 2 : 1);
+        } else {
+            err = "unknown light type";
+            return false;
+        }
+        func[i] = luminance(phi);
+    }
+    // Distribution1D (sampling.h:345-375)
+    const int32_t nl = d.n_lights;
+    float func_integral = 0.0f;
+    if (nl > 0) {
+        cdf[0] = 0.0f;
+        for (int32_t i = 1; i < nl + 1; ++i) cdf[i] = cdf[i - 1] + func[i - 1] / float(nl);
+        func_integral = cdf[nl];
+        if (func_integral == 0.0f) {
+            for (int32_t i = 1; i < nl + 1; ++i) cdf[i] = float(i) / float(nl);
+        } else {
+            for (int32_t i = 1; i < nl + 1; ++i) cdf[i] /= func_integral;
+        }
+        // Distribution1D::DiscretePDF (sampling.h:404-407)
+        for (int32_t i = 0; i < nl; ++i) lights[i].pmf = func[i] / (func_integral * float(nl));
+    }
+    s->n_lights = nl;
+    s->n_inf = int32_t(inf.size());
+
+    // BVH slot data
+    const int32_t ns = s->n_slots;
+    std::vector<float4> tris(size_t(3) * ns);
+    std::vector<int4> slot_shade(ns);
+    std::vector<int32_t> slot_light(ns);
+    std::vector<BbBvhNode> nodes = bvh.nodes;
+    bool has_null = false;
+    for (int32_t j = 0; j < ns; ++j) {
+        int32_t prim = bvh.ordered[j];
+        const BbPrimitive& p = d.primitives[prim];
+        int32_t prim_bits = prim;
+        float prim_f;
+        memcpy(&prim_f, &prim_bits, 4);
+        slot_light[j] = prim_light[prim];
+        if (p.material < 0) has_null = true;
+        if (p.shape_kind == BB_SHAPE_TRIANGLE) {
+            const int32_t* v = d.indices + 3 * int64_t(p.shape_index);
+            f3 p0 = load3(d.positions, v[0]), p1 = load3(d.positions, v[1]), p2 = load3(d.positions, v[2]);
+            f3 e1 = p1 - p0, e2 = p2 - p0;
+            tris[3 * size_t(j) + 0] = make_float4(p0.x, p0.y, p0.z, prim_f);
+            tris[3 * size_t(j) + 1] = make_float4(e1.x, e1.y, e1.z, 0.0f);
+            tris[3 * size_t(j) + 2] = make_float4(e2.x, e2.y, e2.z, 0.0f);
+            slot_shade[j] = make_int4(v[0], v[1], v[2], p.material);
+        } else {
+            int32_t kind = KIND_SPHERE, sidx = p.shape_index;
+            float kind_f, sidx_f;
+            memcpy(&kind_f, &kind, 4);
+            memcpy(&sidx_f, &sidx, 4);
+            tris[3 * size_t(j) + 0] = make_float4(0.0f, 0.0f, 0.0f, prim_f);
+            tris[3 * size_t(j) + 1] = make_float4(0.0f, 0.0f, 0.0f, sidx_f);
+            tris[3 * size_t(j) + 2] = make_float4(0.0f, 0.0f, 0.0f, kind_f);
+            slot_shade[j] = make_int4(0, 0, 0, p.material);
+        }
+    }
+    // flag leaves holding a sphere (axis is unused for leaves)
+    for (BbBvhNode& nd : nodes)
+        if (nd.count > 0)
+            for (int32_t k = 0; k < nd.count; ++k)
+                if (d.primitives[bvh.ordered[nd.offset + k]].shape_kind == BB_SHAPE_SPHERE) nd.axis = 1;
+
+    std::vector<float4> normals(std::max(1, d.n_vertices)), tangents(std::max(1, d.n_vertices));
+    for (int32_t i = 0; i < d.n_vertices; ++i) {
+        normals[i] = make_float4(d.normals[3 * i], d.normals[3 * i + 1], d.normals[3 * i + 2], 0.0f);
+        tangents[i] = make_float4(d.tangents[3 * i], d.tangents[3 * i + 1], d.tangents[3 * i + 2], 0.0f);
+    }
+
+    SceneView& V = s->view;
+    hipGetDevice(&s->device);
+    const float4* nodes_dev = nullptr;
+    const BbBvhNode* nodes_raw = nullptr;
+    if (!upload(s, nodes.data(), nodes.size(), &nodes_raw, err)) return false;
+    nodes_dev = reinterpret_cast<const float4*>(nodes_raw);
+    V.nodes = nodes_dev;
+    if (!upload(s, tris.data(), tris.size(), &V.tris, err)) return false;
+    if (!upload(s, slot_shade.data(), slot_shade.size(), &V.slot_shade, err)) return false;
+    if (!upload(s, slot_light.data(), slot_light.size(), &V.slot_light, err)) return false;
+    if (!upload(s, normals.data(), normals.size(), &V.normals, err)) return false;
+    if (!upload(s, tangents.data(), tangents.size(), &V.tangents, err)) return false;
+    std::vector<BbSphere> spheres(d.spheres, d.spheres + d.n_spheres);
+    spheres.resize(std::max<size_t>(1, spheres.size()));
+    if (!upload(s, spheres.data(), spheres.size(), &V.spheres, err)) return false;
+    if (!upload(s, mats.data(), mats.size(), &V.materials, err)) return false;
+    if (!upload(s, lights.data(), lights.size(), &V.lights, err)) return false;
+    cdf.resize(std::max<size_t>(2, cdf.size()));
+    if (!upload(s, cdf.data(), cdf.size(), &V.light_cdf, err)) return false;
+    if (!upload(s, func.data(), func.size(), &V.light_func, err)) return false;
+    inf.resize(std::max<size_t>(1, inf.size()));
+    if (!upload(s, inf.data(), inf.size(), &V.inf_lights, err)) return false;
+    V.n_lights = nl;
+    V.n_inf = s->n_inf;
+    V.func_integral = func_integral;
+    V.stack_depth = s->max_depth + 2;
+    V.has_null_material = has_null ? 1 : 0;
+    return true;
+}
+
+void release_scene_workspace(BbScene* s);
+
+void destroy_scene(BbScene* s)
+{
+    release_scene_workspace(s);
+    for (void* p : s->allocations) hipFree(p);
+    s->allocations.clear();
+}
+
+} // namespace bb
+
+// =================================================================== C ABI
+
+extern "C" {
+
+int bbgpu_abi_version(void) { return BBGPU_ABI_VERSION; }
+
+int bbgpu_last_error(char* buf, size_t buf_size)
+{
+    if (!buf || buf_size == 0) return BBGPU_ERR_INVALID;
+    snprintf(buf, buf_size, "%s", bb::g_last_error.c_str());
+    return BBGPU_OK;
+}
+
+int bbgpu_device_count(int32_t* count)
+{
+    if (!count) return BBGPU_ERR_INVALID;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+    return BBGPU_OK;
+}
+
+int bbgpu_build_bvh(const BbSceneDesc* desc, BbBvhNode* nodes_out, int32_t* ordered_out, int32_t* n_nodes_out)
+{
+    if (!desc || !nodes_out || !ordered_out || !n_nodes_out) {
+        bb::set_error("bbgpu_build_bvh: null argument");
+        return BBGPU_ERR_INVALID;
+    }
+    bb::HostBvh bvh;
+    std::string err;
+    if (!bb::build_bvh(*desc, bvh, err)) {
+        bb::set_error("bbgpu_build_bvh: %s", err.c_str());
+        return BBGPU_ERR_INVALID;
+    }
+    memcpy(nodes_out, bvh.nodes.data(), sizeof(BbBvhNode) * bvh.nodes.size());
+    memcpy(ordered_out, bvh.ordered.data(), sizeof(int32_t) * bvh.ordered.size());
+    *n_nodes_out = int32_t(bvh.nodes.size());
+    return BBGPU_OK;
+}
+
+int bbgpu_scene_create(const BbSceneDesc* desc, BbScene** scene_out)
+{
+    if (!desc || !scene_out) {
+        bb::set_error("bbgpu_scene_create: null argument");
+        return BBGPU_ERR_INVALID;
+    }
+    *scene_out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        bb::set_error("bbgpu_scene_create: no HIP device");
+        return BBGPU_ERR_DEVICE;
+    }
+    auto* s = new BbScene();
+    std::string err;
+    if (!bb::create_scene(*desc, s, err)) {
+        bb::destroy_scene(s);
+        delete s;
+        bb::set_error("bbgpu_scene_create: %s", err.c_str());
+        return err.find("not supported") != std::string::npos ? BBGPU_ERR_UNSUPPORTED : BBGPU_ERR_INVALID;
+    }
+    *scene_out = s;
+    return BBGPU_OK;
+}
+
+int bbgpu_scene_destroy(BbScene* scene)
+{
+    if (!scene) return BBGPU_ERR_INVALID;
+    bb::destroy_scene(scene);
+    delete scene;
+    return BBGPU_OK;
+}
+
+int bbgpu_scene_info(const BbScene* s, BbSceneInfo* info)
+{
+    if (!s || !info) {
+        bb::set_error("bbgpu_scene_info: null argument");
+        return BBGPU_ERR_INVALID;
+    }
+    info->n_nodes = s->n_nodes;
+    info->n_primitives = s->n_prims;
+    info->n_lights = s->n_lights;
+    info->n_infinite_lights = s->n_inf;
+    for (int k = 0; k < 3; ++k) {
+        info->world_min[k] = s->world_min[k];
+        info->world_max[k] = s->world_max[k];
+    }
+    info->build_seconds = s->build_seconds;
+    info->device_bytes = s->device_bytes;
+    return BBGPU_OK;
+}
+
+// Transform::LookAt (transform.h:112-122) with Quat(front, up) (quaternion.h:64-75)
+// and Quat(const Mat3&) (quaternion.h:34-62)
+int bbgpu_look_at(const float position[3], const float target[3], const float up[3], float q_out[4])
+{
+    using namespace bb;
+    if (!position || !target || !up || !q_out) return BBGPU_ERR_INVALID;
+    f3 w = mk3(target[0], target[1], target[2]) - mk3(position[0], position[1], position[2]);
+    normalize_m(w);
+    f3 ez = w;
+    f3 ex = cross(mk3(up[0], up[1], up[2]), ez);
+    normalize_m(ex);
+    f3 ey = cross(ez, ex);
+    float x, y, z, ww, t, sc;
+    if (ez.z < 0.0f) {
+        if (ex.x > ey.y) {
+            t = 1.0f + ex.x - ey.y - ez.z;
+            x = t, y = ex.y + ey.x, z = ez.x + ex.z, ww = ey.z - ez.y;
+        } else {
+            t = 1.0f - ex.x + ey.y - ez.z;
+            x = ex.y + ey.x, y = t, z = ey.z + ez.y, ww = ez.x - ex.z;
+        }
+    } else {
+        if (ex.x < -ey.y) {
+            t = 1.0f - ex.x - ey.y + ez.z;
+            x = ez.x + ex.z, y = ey.z + ez.y, z = t, ww = ex.y - ey.x;
+        } else {
+            t = 1.0f + ex.x + ey.y + ez.z;
+            x = ey.z - ez.y, y = ez.x - ex.z, z = ex.y - ey.x, ww = t;
+        }
+    }
+    sc = 0.5f / std::sqrt(t);
+    q_out[0] = x * sc;
+    q_out[1] = y * sc;
+    q_out[2] = z * sc;
+    q_out[3] = ww * sc;
+    return BBGPU_OK;
+}
+
+} // extern "C"

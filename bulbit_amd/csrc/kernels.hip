@@ -1,0 +1,1009 @@
+// kernels.hip — gfx950 kernels of libbbgpu.so and the C-ABI entry points that
+// launch them: batch traversal (bbgpu_trace) and the wavefront path tracer
+// (bbgpu_render).
+//
+// Traversal follows BVH::RayCast (include/bulbit/bvh.h:101-155) exactly: LIFO
+// stack seeded with the root, box test when a node is taken with the current
+// t_max, leaf primitives in stored order, far child pushed and near child
+// taken next (near = child2 when the ray direction along the split axis is
+// negative).  Taking the near child directly instead of pushing and popping it
+// tests the same nodes in the same order with the same t_max.  The stack lives
+// in LDS, laid out [depth][thread] so that the 64 lanes of a wave touch 64
+// consecutive words.
+//
+// The path tracer is a wavefront pipeline per pass of samples:
+//   raygen -> for depth in 0..max_bounces:
+//                trace_closest -> shade (emission, NEE, BSDF sample, RR,
+//                compaction of continuing paths and shadow rays) -> trace_shadow
+//          -> film (per pixel, samples in order)
+// Each path keeps its own PCG32 stream seeded as IndependentSampler does, and
+// draws in the reference's order, so every sample equals the reference's
+// sample up to transcendental rounding.
+#include "bb_shade.h"
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace bb
+{
+void set_error(const char* fmt, ...);
+}
+
+namespace bb
+{
+
+constexpr int kTraceBlock = 128;
+constexpr int kShadeBlock = 256;
+constexpr int kMaxGridBlocks = 256 * 32;
+
+// ----------------------------------------------------------------- helpers
+
+BB_D f3 xyz(float4 v) { return mk3(v.x, v.y, v.z); }
+
+// AABB3::TestRay with precomputed inverse direction (bounding_box.h:443-484).
+// a = (min.x, min.y, min.z, max.x), b = (max.y, max.z, offset, meta)
+BB_D bool box_test(float4 a, float4 b, f3 o, float t_min, float t_max, f3 inv, bool n0, bool n1, bool n2)
+{
+    float t_min_x = ((n0 ? a.w : a.x) - o.x) * inv.x;
+    float t_max_x = ((n0 ? a.x : a.w) - o.x) * inv.x;
+    if (t_min > t_max_x || t_max < t_min_x) return false;
+    if (t_min_x > t_min) t_min = t_min_x;
+    if (t_max_x < t_max) t_max = t_max_x;
+    float t_min_y = ((n1 ? b.x : a.y) - o.y) * inv.y;
+    float t_max_y = ((n1 ? a.y : b.x) - o.y) * inv.y;
+    if (t_min > t_max_y || t_max < t_min_y) return false;
+    if (t_min_y > t_min) t_min = t_min_y;
+    if (t_max_y < t_max) t_max = t_max_y;
+    float t_min_z = ((n2 ? b.y : a.z) - o.z) * inv.z;
+    float t_max_z = ((n2 ? a.z : b.y) - o.z) * inv.z;
+    if (t_min > t_max_z || t_max < t_min_z) return false;
+    return true;
+}
+
+// Triangle::Intersect's test (src/geometry/triangle.cpp:55-97) with
+// e1 = p1 - p0, e2 = p2 - p0 precomputed (same float32 subtraction) and the
+// direction normalised once per ray (same Vector3::Normalize result).
+BB_D bool mt_test(f3 p0, f3 e1, f3 e2, f3 o, f3 dn, float l, float t_min, float t_max, float& t, float& u, float& v)
+{
+    f3 pvec = cross(dn, e2);
+    float det = dot(e1, pvec);
+    if (fabsf(det) < kEps) return false;
+    float inv_det = 1.0f / det;
+    f3 tvec = o - p0;
+    u = dot(tvec, pvec) * inv_det;
+    if (u < 0.0f || u > 1.0f) return false;
+    f3 qvec = cross(tvec, e1);
+    v = dot(dn, qvec) * inv_det;
+    if (v < 0.0f || u + v > 1.0f) return false;
+    t = dot(e2, qvec) * inv_det / l;
+    if (t < t_min || t > t_max) return false;
+    return true;
+}
+
+// Sphere::Intersect's root search (src/geometry/sphere.cpp:35-61)
+BB_D void sphere_local_ray(const BbSphere& sp, f3 o, f3 d, f3& ro, f3& rd)
+{
+    quat q{ sp.q[0], sp.q[1], sp.q[2], sp.q[3] };
+    f3 p = mk3(sp.p[0], sp.p[1], sp.p[2]);
+    f3 s = mk3(sp.s[0], sp.s[1], sp.s[2]);
+    ro = quat_rotate_inv(q, o - p) / s;
+    rd = quat_rotate_inv(q, d / s);
+}
+
+BB_D bool sphere_test(const BbSphere& sp, f3 o, f3 d, float t_min, float t_max, float& t)
+{
+    f3 ro, rd;
+    sphere_local_ray(sp, o, d, ro, rd);
+    float a = length2(rd);
+    float half_b = dot(ro, rd);
+    float c = length2(ro) - sp.radius * sp.radius;
+    float disc = half_b * half_b - a * c;
+    if (disc < 0.0f) return false;
+    float sqrt_d = sqrtf(disc);
+    float root = (-half_b - sqrt_d) / a;
+    if (root < t_min || t_max < root) {
+        root = (-half_b + sqrt_d) / a;
+        if (root < t_min || t_max < root) return false;
+    }
+    t = root;
+    return true;
+}
+
+struct TraceCount
+{
+    uint32_t nodes, prims, accepts;
+};
+
+// BVH::RayCast with the Intersect / IntersectAny callbacks
+// (src/accel/bvh.cpp:299-361).  Returns false on stack overflow (err set).
+template <bool ANY, bool COUNT>
+BB_D bool traverse(const SceneView& S, f3 o, f3 d, float t_min, float t_max, int32_t* stack, int stride, int depth_cap,
+                   int32_t& hit_slot, float& hit_t, float& hit_u, float& hit_v, TraceCount& c, int* err)
+{
+    const f3 inv = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    const bool n0 = inv.x < 0.0f, n1 = inv.y < 0.0f, n2 = inv.z < 0.0f;
+    f3 dn = d;
+    const float l = normalize_m(dn);
+    int sp = 0;
+    int32_t node = 0;
+    for (;;) {
+        const float4 a = S.nodes[2 * node];
+        const float4 b = S.nodes[2 * node + 1];
+        if (COUNT) c.nodes++;
+        if (box_test(a, b, o, t_min, t_max, inv, n0, n1, n2)) {
+            const int32_t offset = __float_as_int(b.z);
+            const uint32_t meta = __float_as_uint(b.w);
+            const uint32_t count = meta & 0xffffu;
+            const uint32_t axis = (meta >> 16) & 0xffu;
+            if (count > 0) {
+                for (uint32_t i = 0; i < count; ++i) {
+                    const int32_t slot = offset + int32_t(i);
+                    const float4 t0 = S.tris[3 * slot];
+                    const float4 t1 = S.tris[3 * slot + 1];
+                    const float4 t2 = S.tris[3 * slot + 2];
+                    if (COUNT) c.prims++;
+                    float t = 0.0f, u = 0.0f, v = 0.0f;
+                    bool h;
+                    if ((axis & 1u) && __float_as_int(t2.w) == KIND_SPHERE)
+                        h = sphere_test(S.spheres[__float_as_int(t1.w)], o, d, t_min, t_max, t);
+                    else
+                        h = mt_test(xyz(t0), xyz(t1), xyz(t2), o, dn, l, t_min, t_max, t, u, v);
+                    if (h) {
+                        if (COUNT) c.accepts++;
+                        if (ANY) {
+                            hit_slot = slot;
+                            return true;
+                        }
+                        hit_slot = slot;
+                        hit_t = t;
+                        hit_u = u;
+                        hit_v = v;
+                        if (t <= t_min) return true;
+                        t_max = t;
+                    }
+                }
+            } else {
+                const int32_t child1 = node + 1;
+                const int32_t child2 = offset;
+                const bool neg = axis == 0 ? n0 : (axis == 1 ? n1 : n2);
+                if (sp >= depth_cap) {
+                    atomicOr(err, 1);
+                    return hit_slot >= 0;
+                }
+                stack[sp * stride] = neg ? child1 : child2;
+                ++sp;
+                node = neg ? child2 : child1;
+                continue;
+            }
+        }
+        if (sp == 0) break;
+        --sp;
+        node = stack[sp * stride];
+    }
+    return hit_slot >= 0;
+}
+
+// wave-level compaction: returns this lane's slot (or -1) in the queue whose
+// length is *counter; every lane of the wave must call it
+BB_D int wave_alloc(bool pred, int* counter)
+{
+    const unsigned long long mask = __ballot(pred ? 1 : 0);
+    if (mask == 0ull) return -1;
+    const int lane = __lane_id();
+    const int leader = __builtin_ctzll(mask);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counter, __popcll(mask));
+    base = __shfl(base, leader);
+    const unsigned long long below = (lane == 0) ? 0ull : (mask & (~0ull >> (64 - lane)));
+    return pred ? base + __popcll(below) : -1;
+}
+
+BB_D void wave_count_add(TraceCount c, unsigned long long* out)
+{
+    unsigned long long n = c.nodes, p = c.prims, a = c.accepts;
+    for (int off = 32; off > 0; off >>= 1) {
+        n += __shfl_xor(n, off);
+        p += __shfl_xor(p, off);
+        a += __shfl_xor(a, off);
+    }
+    if (__lane_id() == 0) {
+        atomicAdd(&out[0], n);
+        atomicAdd(&out[1], p);
+        atomicAdd(&out[2], a);
+    }
+}
+
+// ------------------------------------------------------- batch traversal
+
+template <bool COUNT>
+__global__ __launch_bounds__(kTraceBlock) void k_trace_batch(SceneView S, const float4* __restrict__ rays, int n,
+                                                               int any_hit, int4* __restrict__ hits,
+                                                               unsigned long long* counters, int* err)
+{
+    extern __shared__ int32_t lds_stack[];
+    int32_t* stack = lds_stack + threadIdx.x;
+    TraceCount c{ 0, 0, 0 };
+    for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        const int i = base + threadIdx.x;
+        if (i < n) {
+            const float4 r0 = rays[2 * i], r1 = rays[2 * i + 1];
+            int32_t slot = -1;
+            float t = 0.0f, u = 0.0f, v = 0.0f;
+            bool h;
+            if (any_hit)
+                h = traverse<true, COUNT>(S, xyz(r0), xyz(r1), r0.w, r1.w, stack, kTraceBlock, S.stack_depth, slot, t, u,
+                                          v, c, err);
+            else
+                h = traverse<false, COUNT>(S, xyz(r0), xyz(r1), r0.w, r1.w, stack, kTraceBlock, S.stack_depth, slot, t,
+                                           u, v, c, err);
+            int4 out;
+            if (any_hit) {
+                out = make_int4(h ? 1 : -1, 0, 0, 0);
+            } else if (h) {
+                out = make_int4(__float_as_int(S.tris[3 * slot].w), __float_as_int(t), __float_as_int(u),
+                                __float_as_int(v));
+            } else {
+                out = make_int4(-1, 0, 0, 0);
+            }
+            hits[i] = out;
+        }
+    }
+    if (COUNT) wave_count_add(c, counters);
+}
+
+// -------------------------------------------------------- path tracer state
+
+struct GpuCamera
+{
+    f3 origin, lower_left, horizontal, vertical, u, v;
+    float lens_radius, filter_extent;
+    int32_t width, height;
+};
+
+struct PathBufs
+{
+    float4* L;     // L.rgb, eta_scale
+    float4* beta;  // beta.rgb, prev_bsdf_pdf
+    uint64_t* rng; // PCG32 state
+    int32_t* flags;  // bounce | specular << 16
+    int32_t* queue[2];
+    float4* ray[2];  // 2 float4 per queue slot: (o, -), (d, -)
+    int4* hit;       // (slot, t, u, v) per queue slot
+    float4* sray;    // 2 float4 per shadow slot: (o, t_max), (d, path id)
+    float4* scontrib;
+    int* counts;     // [depth][2]: active paths, shadow rays
+    int* err;
+    unsigned long long* counters; // [6]: closest nodes/prims/accepts, shadow nodes/prims/accepts
+};
+
+struct RenderConsts
+{
+    int32_t n_paths, npix, width, seed, sample0, max_bounces, rr_min_bounces;
+};
+
+// UniDirectionalRayIntegrator::Render per (pixel, sample) start
+// (integrator.cpp:187-190): seed, then SampleRay(Next2D(), Next2D()) with the
+// lens sample u1 drawn first (argument evaluation of the reference's compiler),
+// PerspectiveCamera::SampleRay (perspective_camera.cpp:40-52) with
+// BoxFilter::Sample (box_filter.cpp:17-21).
+__global__ __launch_bounds__(kShadeBlock) void k_raygen(GpuCamera cam, RenderConsts rc, PathBufs P)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        P.counts[0] = rc.n_paths;
+    }
+    if (i >= rc.n_paths) return;
+    const int pix = i % rc.npix;
+    const int sample = rc.sample0 + i / rc.npix;
+    const int px = pix % rc.width, py = pix / rc.width;
+    uint64_t st = pcg_seed(pixel_sample_hash(px, py, rc.seed, sample));
+    f2 u1;
+    u1.x = pcg_float(st);
+    u1.y = pcg_float(st);
+    f2 u0;
+    u0.x = pcg_float(st);
+    u0.y = pcg_float(st);
+    f2 off = mk2((u0.x * 2.0f - 1.0f) * cam.filter_extent / 2.0f, (u0.y * 2.0f - 1.0f) * cam.filter_extent / 2.0f);
+    off = mk2(off.x + 0.5f, off.y + 0.5f);
+    f3 pixel_center = (cam.lower_left + (cam.horizontal * (float(px) + off.x)) / float(cam.width)) +
+                      (cam.vertical * (float(py) + off.y)) / float(cam.height);
+    f2 ap = sample_concentric(u1);
+    ap = mk2(ap.x * cam.lens_radius, ap.y * cam.lens_radius);
+    f3 offset = cam.u * ap.x + cam.v * ap.y;
+    f3 o = cam.origin + offset;
+    f3 d = normalize(pixel_center - o);
+    P.ray[0][2 * i] = make_float4(o.x, o.y, o.z, 0.0f);
+    P.ray[0][2 * i + 1] = make_float4(d.x, d.y, d.z, 0.0f);
+    P.queue[0][i] = i;
+    P.L[i] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    P.beta[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    P.rng[i] = st;
+    P.flags[i] = 0;
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(kTraceBlock) void k_trace_closest(SceneView S, const int* __restrict__ n_ptr,
+                                                                 const float4* __restrict__ rays, int4* __restrict__ hits,
+                                                                 unsigned long long* counters, int* err)
+{
+    extern __shared__ int32_t lds_stack[];
+    int32_t* stack = lds_stack + threadIdx.x;
+    const int n = *n_ptr;
+    TraceCount c{ 0, 0, 0 };
+    for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        const int i = base + threadIdx.x;
+        if (i < n) {
+            const float4 r0 = rays[2 * i], r1 = rays[2 * i + 1];
+            int32_t slot = -1;
+            float t = 0.0f, u = 0.0f, v = 0.0f;
+            traverse<false, COUNT>(S, xyz(r0), xyz(r1), kRayEps, INFINITY, stack, kTraceBlock, S.stack_depth, slot, t, u,
+                                   v, c, err);
+            hits[i] = make_int4(slot, __float_as_int(t), __float_as_int(u), __float_as_int(v));
+        }
+    }
+    if (COUNT) wave_count_add(c, counters);
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(kTraceBlock) void k_trace_shadow(SceneView S, const int* __restrict__ n_ptr,
+                                                                const float4* __restrict__ srays,
+                                                                const float4* __restrict__ contrib, float4* __restrict__ L,
+                                                                unsigned long long* counters, int* err)
+{
+    extern __shared__ int32_t lds_stack[];
+    int32_t* stack = lds_stack + threadIdx.x;
+    const int n = *n_ptr;
+    TraceCount c{ 0, 0, 0 };
+    for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        const int i = base + threadIdx.x;
+        if (i < n) {
+            const float4 r0 = srays[2 * i], r1 = srays[2 * i + 1];
+            int32_t slot = -1;
+            float t = 0.0f, u = 0.0f, v = 0.0f;
+            bool occluded = traverse<true, COUNT>(S, xyz(r0), xyz(r1), kRayEps, r0.w, stack, kTraceBlock, S.stack_depth,
+                                                  slot, t, u, v, c, err);
+            if (!occluded) {
+                // L += SampleDirectLight(...)  (path_integrator.cpp:109-112)
+                const int pid = __float_as_int(r1.w);
+                const float4 cv = contrib[i];
+                float4 l = L[pid];
+                l.x = l.x + cv.x;
+                l.y = l.y + cv.y;
+                l.z = l.z + cv.z;
+                L[pid] = l;
+            }
+        }
+    }
+    if (COUNT) wave_count_add(c, counters + 3);
+}
+
+// Surface data of the closest hit: the shading part of Triangle::Intersect
+// (triangle.cpp:99-107) / Sphere::Intersect (sphere.cpp:63-82) and
+// Shape::SetFaceNormal (shape.h:32-43).
+struct SurfaceHit
+{
+    f3 point, normal, ns, ts;
+    bool front;
+};
+
+BB_D SurfaceHit surface_at(const SceneView& S, int32_t slot, float t, float u, float v, f3 o, f3 d)
+{
+    SurfaceHit h;
+    const float4 t0 = S.tris[3 * slot], t1 = S.tris[3 * slot + 1], t2 = S.tris[3 * slot + 2];
+    f3 normal, ns, ts;
+    if (__float_as_int(t2.w) == KIND_SPHERE) {
+        const BbSphere& sp = S.spheres[__float_as_int(t1.w)];
+        quat q{ sp.q[0], sp.q[1], sp.q[2], sp.q[3] };
+        f3 ro, rd;
+        sphere_local_ray(sp, o, d, ro, rd);
+        f3 pl = ro + rd * t;
+        f3 nl = normalize(pl / sp.radius);
+        h.point = quat_rotate(q, mk3(sp.s[0], sp.s[1], sp.s[2]) * pl) + mk3(sp.p[0], sp.p[1], sp.p[2]);
+        f3 n = quat_rotate(q, nl);
+        f3 tt = (n.y < 1.0f - 1e-4f) ? normalize(cross(mk3(0.0f, 1.0f, 0.0f), n)) : normalize(cross(n, mk3(1.0f, 0.0f, 0.0f)));
+        normal = n;
+        ns = n;
+        ts = tt;
+    } else {
+        const int4 vi = S.slot_shade[slot];
+        const f3 e1 = xyz(t1), e2 = xyz(t2);
+        const float w = 1.0f - u - v;
+        h.point = o + d * t;
+        normal = normalize(cross(e1, e2));
+        const f3 n0 = xyz(S.normals[vi.x]), n1 = xyz(S.normals[vi.y]), n2 = xyz(S.normals[vi.z]);
+        const f3 g0 = xyz(S.tangents[vi.x]), g1 = xyz(S.tangents[vi.y]), g2 = xyz(S.tangents[vi.z]);
+        ns = normalize(n0 * w + n1 * u + n2 * v);
+        ts = normalize(g0 * w + g1 * u + g2 * v);
+    }
+    h.front = dot(d, normal) < 0.0f;
+    const float sign = h.front ? 1.0f : -1.0f;
+    h.normal = normal * sign;
+    h.ns = ns * sign;
+    h.ts = ts * sign;
+    return h;
+}
+
+// PathIntegrator::Li, one bounce (path_integrator.cpp:38-147) and the
+// unoccluded part of SampleDirectLight (:152-194).
+__global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView S, RenderConsts rc, PathBufs P, int depth)
+{
+    const int in = depth & 1, out = in ^ 1;
+    const int n = P.counts[2 * depth];
+    int* next_count = &P.counts[2 * (depth + 1)];
+    int* shadow_count = &P.counts[2 * depth + 1];
+    for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        const int k = base + threadIdx.x;
+        bool cont = false, shadow = false;
+        int pid = 0;
+        f3 new_o = mk3(0.0f), new_d = mk3(0.0f), sh_o = mk3(0.0f), sh_d = mk3(0.0f);
+        float sh_tmax = 0.0f;
+        spec sh_c = mks(0.0f);
+        if (k < n) {
+            pid = P.queue[in][k];
+            const float4 r0 = P.ray[in][2 * k], r1 = P.ray[in][2 * k + 1];
+            const f3 o = xyz(r0), d = xyz(r1);
+            const int4 hh = P.hit[k];
+            float4 Lv = P.L[pid];
+            float4 Bv = P.beta[pid];
+            uint64_t st = P.rng[pid];
+            int32_t fl = P.flags[pid];
+            int32_t bounce = fl & 0xffff;
+            bool specular = (fl >> 16) & 1;
+            spec L = mks(Lv.x, Lv.y, Lv.z);
+            float eta_scale = Lv.w;
+            spec beta = mks(Bv.x, Bv.y, Bv.z);
+            float prev_pdf = Bv.w;
+            const int32_t slot = hh.x;
+            if (slot < 0) {
+                // escaped: infinite lights (path_integrator.cpp:41-62)
+                for (int32_t k2 = 0; k2 < S.n_inf; ++k2) {
+                    const GpuLight& Lt = S.lights[S.inf_lights[k2]];
+                    if (bounce == 0 || specular) {
+                        L = L + beta * Lt.L;
+                    } else {
+                        float light_pdf = kInvFourPi * Lt.pmf;
+                        float mis = power_heuristic(prev_pdf, light_pdf);
+                        L = L + (beta * mis) * Lt.L;
+                    }
+                }
+            } else {
+                const float t = __int_as_float(hh.y), u = __int_as_float(hh.z), v = __int_as_float(hh.w);
+                const SurfaceHit sh = surface_at(S, slot, t, u, v, o, d);
+                const f3 wo = normalize(-d);
+                const int32_t al = S.slot_light[slot];
+                if (al >= 0) {
+                    // area light emission (path_integrator.cpp:67-84)
+                    const GpuLight& Lt = S.lights[al];
+                    spec Le = (sh.front || Lt.two_sided) ? Lt.L : mks(0.0f);
+                    if (!is_black(Le)) {
+                        if (bounce == 0 || specular) {
+                            L = L + beta * Le;
+                        } else {
+                            // Triangle::PDF(isect, ray) (triangle.cpp:247-253)
+                            float dist2 = t * t * length2(d);
+                            float cosine = fabsf(dot(d, sh.normal) / length(d));
+                            float area = 0.5f * length(cross(xyz(S.tris[3 * slot + 1]), xyz(S.tris[3 * slot + 2])));
+                            float light_pdf = (dist2 / (cosine * area)) * Lt.pmf;
+                            float mis = power_heuristic(prev_pdf, light_pdf);
+                            L = L + (beta * mis) * Le;
+                        }
+                    }
+                }
+                if (bounce++ < rc.max_bounces) {
+                    const int32_t mi = S.slot_shade[slot].w;
+                    if (mi < 0) {
+                        // null material: continue straight (path_integrator.cpp:95-100)
+                        --bounce;
+                        new_o = sh.point;
+                        new_d = -wo;
+                        cont = true;
+                    } else {
+                        const bxdf b = make_bxdf(S.materials[mi], sh.front);
+                        const frame3 fr = frame_from_xz(sh.ts, sh.ns);
+                        const f3 wo_l = to_local(fr, wo);
+                        if (b.flags & (BX_DIFFUSE | BX_GLOSSY)) {
+                            // SampleDirectLight: Next1D, Next2D drawn first
+                            const float lu0 = pcg_float(st);
+                            f2 lu12;
+                            lu12.x = pcg_float(st);
+                            lu12.y = pcg_float(st);
+                            if (S.n_lights > 0) {
+                                float pmf;
+                                const int32_t li = light_select(S, lu0, pmf);
+                                const GpuLight& Lt = S.lights[li];
+                                light_sample ls;
+                                if (light_sample_li(Lt, sh.point, lu12, ls)) {
+                                    const f3 wi_l = to_local(fr, ls.wi);
+                                    const float bsdf_pdf = (wo_l.z == 0.0f) ? 0.0f : bxdf_pdf(b, wo_l, wi_l);
+                                    if (!(is_black(ls.Li) || bsdf_pdf == 0.0f)) {
+                                        const spec f = (wo_l.z == 0.0f) ? mks(0.0f) : bxdf_f(b, wo_l, wi_l);
+                                        const float light_pdf = pmf * ls.pdf;
+                                        const spec f_cos = f * absdot(sh.ns, ls.wi);
+                                        if (Lt.type == BB_LIGHT_SPOT) {
+                                            sh_c = ((beta * ls.Li) * f_cos) / light_pdf;
+                                        } else {
+                                            const float mis = power_heuristic(light_pdf, bsdf_pdf);
+                                            sh_c = (((beta * mis) * ls.Li) * f_cos) / light_pdf;
+                                        }
+                                        shadow = true;
+                                        sh_o = sh.point;
+                                        sh_d = ls.wi;
+                                        sh_tmax = ls.visibility;
+                                    }
+                                }
+                            }
+                        }
+                        // Sample_f(wo, Next1D(), Next2D()): u12 drawn before u0
+                        f2 u12;
+                        u12.x = pcg_float(st);
+                        u12.y = pcg_float(st);
+                        const float u0 = pcg_float(st);
+                        bsample s;
+                        bool ok = wo_l.z != 0.0f && (b.flags & (BX_REFLECTION | BX_TRANSMISSION)) &&
+                                  bxdf_sample(b, wo_l, u0, u12, s);
+                        if (ok) ok = !(is_black(s.f) || s.pdf == 0.0f || s.wi.z == 0.0f);
+                        if (ok) {
+                            const f3 wi = from_local(fr, s.wi);
+                            specular = (s.flags & BX_SPECULAR) != 0;
+                            if (s.flags & BX_TRANSMISSION) eta_scale *= sqr(s.eta);
+                            prev_pdf = s.pdf;
+                            beta = beta * ((s.f * absdot(sh.ns, wi)) / s.pdf);
+                            new_o = sh.point;
+                            new_d = wi;
+                            cont = true;
+                            // Russian roulette (path_integrator.cpp:133-146)
+                            if (bounce > rc.rr_min_bounces) {
+                                const float p = max_component(beta) * eta_scale;
+                                if (p < 1.0f) {
+                                    if (pcg_float(st) > p)
+                                        cont = false;
+                                    else
+                                        beta = beta / p;
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+            P.L[pid] = make_float4(L.r, L.g, L.b, eta_scale);
+            P.beta[pid] = make_float4(beta.r, beta.g, beta.b, prev_pdf);
+            P.rng[pid] = st;
+            P.flags[pid] = (bounce & 0xffff) | (specular ? (1 << 16) : 0);
+        }
+        const int ss = wave_alloc(shadow, shadow_count);
+        if (shadow) {
+            P.sray[2 * ss] = make_float4(sh_o.x, sh_o.y, sh_o.z, sh_tmax);
+            P.sray[2 * ss + 1] = make_float4(sh_d.x, sh_d.y, sh_d.z, __int_as_float(pid));
+            P.scontrib[ss] = make_float4(sh_c.r, sh_c.g, sh_c.b, 0.0f);
+        }
+        const int ns = wave_alloc(cont, next_count);
+        if (cont) {
+            P.queue[out][ns] = pid;
+            P.ray[out][2 * ns] = make_float4(new_o.x, new_o.y, new_o.z, 0.0f);
+            P.ray[out][2 * ns + 1] = make_float4(new_d.x, new_d.y, new_d.z, 0.0f);
+        }
+    }
+}
+
+// Film::AddSample in sample order (film.cpp:26-37); NaN/Inf samples dropped
+// (integrator.cpp:193-196).  Path i of a pass is pixel i % npix, sample i / npix.
+__global__ __launch_bounds__(kShadeBlock) void k_film(RenderConsts rc, const float4* __restrict__ L, float* film_sum,
+                                                       int32_t* film_count)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= rc.npix) return;
+    float sr = film_sum[3 * p], sg = film_sum[3 * p + 1], sb = film_sum[3 * p + 2];
+    int32_t cnt = film_count[p];
+    for (int pid = p; pid < rc.n_paths; pid += rc.npix) {
+        const float4 l = L[pid];
+        if (!is_nullish(mks(l.x, l.y, l.z))) {
+            sr = sr + l.x;
+            sg = sg + l.y;
+            sb = sb + l.z;
+            cnt += 1;
+        }
+    }
+    film_sum[3 * p] = sr;
+    film_sum[3 * p + 1] = sg;
+    film_sum[3 * p + 2] = sb;
+    film_count[p] = cnt;
+}
+
+__global__ void k_zero_ints(int* p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+} // namespace bb
+
+// ============================================================ host control
+
+namespace
+{
+
+using namespace bb;
+
+struct Workspace
+{
+    size_t capacity = 0; // paths
+    int max_depth_slots = 0;
+    std::vector<void*> allocs;
+    PathBufs P{};
+};
+
+struct SceneExtra
+{
+    Workspace ws;
+};
+
+std::vector<std::pair<BbScene*, SceneExtra*>> g_extras;
+std::mutex g_extras_lock;
+
+SceneExtra* extra_for(BbScene* s)
+{
+    std::lock_guard<std::mutex> g(g_extras_lock);
+    for (auto& e : g_extras)
+        if (e.first == s) return e.second;
+    auto* x = new SceneExtra();
+    g_extras.push_back({ s, x });
+    return x;
+}
+
+template <typename T>
+bool dalloc(Workspace& w, T** p, size_t count)
+{
+    void* q = nullptr;
+    if (hipMalloc(&q, std::max<size_t>(16, sizeof(T) * count)) != hipSuccess) return false;
+    w.allocs.push_back(q);
+    *p = reinterpret_cast<T*>(q);
+    return true;
+}
+
+void free_workspace(Workspace& w)
+{
+    for (void* p : w.allocs) hipFree(p);
+    w.allocs.clear();
+    w.capacity = 0;
+    w.P = PathBufs{};
+}
+
+bool ensure_workspace(Workspace& w, size_t paths, int depth_slots)
+{
+    if (w.capacity >= paths && w.max_depth_slots >= depth_slots) return true;
+    paths = std::max(paths, w.capacity);
+    depth_slots = std::max(depth_slots, w.max_depth_slots);
+    free_workspace(w);
+    PathBufs& P = w.P;
+    bool ok = dalloc(w, &P.L, paths) && dalloc(w, &P.beta, paths) && dalloc(w, &P.rng, paths) &&
+              dalloc(w, &P.flags, paths) && dalloc(w, &P.queue[0], paths) && dalloc(w, &P.queue[1], paths) &&
+              dalloc(w, &P.ray[0], 2 * paths) && dalloc(w, &P.ray[1], 2 * paths) && dalloc(w, &P.hit, paths) &&
+              dalloc(w, &P.sray, 2 * paths) && dalloc(w, &P.scontrib, paths) &&
+              dalloc(w, &P.counts, size_t(2) * (depth_slots + 2)) && dalloc(w, &P.err, 4) && dalloc(w, &P.counters, 8);
+    if (!ok) {
+        free_workspace(w);
+        return false;
+    }
+    w.capacity = paths;
+    w.max_depth_slots = depth_slots;
+    return true;
+}
+
+int grid_for(long long n, int block)
+{
+    long long g = (n + block - 1) / block;
+    return int(std::max<long long>(1, std::min<long long>(g, kMaxGridBlocks)));
+}
+
+// PerspectiveCamera constructor (src/camera/perspective_camera.cpp:8-38)
+GpuCamera make_camera(const BbCameraDesc& c)
+{
+    GpuCamera g;
+    quat q{ c.orientation[0], c.orientation[1], c.orientation[2], c.orientation[3] };
+    float theta = c.vfov_deg * kPi / 180.0f;
+    float h = std::tan(theta / 2.0f);
+    float aspect_ratio = float(c.width) / float(c.height);
+    float viewport_height = 2.0f * h;
+    float viewport_width = aspect_ratio * viewport_height;
+    // Quat::GetBasisX/Y/Z (quaternion.h:181-205)
+    float x2 = q.x * 2.0f, y2 = q.y * 2.0f, z2 = q.z * 2.0f, w2 = q.w * 2.0f;
+    f3 bx = mk3((q.w * w2) - 1.0f + q.x * x2, (q.z * w2) + q.y * x2, (-q.y * w2) + q.z * x2);
+    f3 by = mk3((-q.z * w2) + q.x * y2, (q.w * w2) - 1.0f + q.y * y2, (q.x * w2) + q.z * y2);
+    f3 bz = mk3((q.y * w2) + q.x * z2, (-q.x * w2) + q.y * z2, (q.w * w2) - 1.0f + q.z * z2);
+    g.u = -bx;
+    g.v = by;
+    f3 w = bz;
+    g.origin = mk3(c.position[0], c.position[1], c.position[2]);
+    g.horizontal = g.u * (c.focus_distance * viewport_width);
+    g.vertical = g.v * (c.focus_distance * viewport_height);
+    g.lower_left = ((g.origin - g.horizontal / 2.0f) - g.vertical / 2.0f) + w * c.focus_distance;
+    g.lens_radius = c.aperture_radius;
+    g.filter_extent = c.filter_extent;
+    g.width = c.width;
+    g.height = c.height;
+    return g;
+}
+
+#define HIP_OK(call)                                                                                                   \
+    do {                                                                                                               \
+        hipError_t e_ = (call);                                                                                        \
+        if (e_ != hipSuccess) {                                                                                        \
+            bb::set_error("%s: %s", #call, hipGetErrorString(e_));                                                     \
+            return BBGPU_ERR_DEVICE;                                                                                   \
+        }                                                                                                              \
+    } while (0)
+
+struct EventPool
+{
+    std::vector<hipEvent_t> ev;
+    size_t used = 0;
+    hipEvent_t get()
+    {
+        if (used == ev.size()) {
+            hipEvent_t e;
+            hipEventCreate(&e);
+            ev.push_back(e);
+        }
+        return ev[used++];
+    }
+    ~EventPool()
+    {
+        for (auto e : ev) hipEventDestroy(e);
+    }
+};
+
+size_t trace_lds_bytes(const SceneView& V) { return size_t(V.stack_depth) * kTraceBlock * sizeof(int32_t); }
+
+} // namespace
+
+extern "C" {
+
+int bbgpu_trace(BbScene* scene, const BbRay* rays_dev, int32_t n_rays, int32_t any_hit, BbHit* hits_dev,
+                BbTraceCounters* counters, void* stream)
+{
+    if (!scene || n_rays < 0 || (n_rays > 0 && (!rays_dev || !hits_dev))) {
+        bb::set_error("bbgpu_trace: invalid argument");
+        return BBGPU_ERR_INVALID;
+    }
+    if (trace_lds_bytes(scene->view) > 65536) {
+        bb::set_error("bbgpu_trace: BVH depth %d exceeds the LDS traversal stack", scene->max_depth);
+        return BBGPU_ERR_UNSUPPORTED;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    SceneExtra* x = extra_for(scene);
+    if (!ensure_workspace(x->ws, 1, 16)) {
+        bb::set_error("bbgpu_trace: workspace allocation failed");
+        return BBGPU_ERR_NOMEM;
+    }
+    PathBufs& P = x->ws.P;
+    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, st, P.err, 1);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, P.counters, 8);
+    if (n_rays > 0) {
+        const int grid = grid_for(n_rays, kTraceBlock);
+        const size_t lds = trace_lds_bytes(scene->view);
+        if (counters)
+            hipLaunchKernelGGL(k_trace_batch<true>, dim3(grid), dim3(kTraceBlock), lds, st, scene->view,
+                               reinterpret_cast<const float4*>(rays_dev), n_rays, any_hit,
+                               reinterpret_cast<int4*>(hits_dev), P.counters, P.err);
+        else
+            hipLaunchKernelGGL(k_trace_batch<false>, dim3(grid), dim3(kTraceBlock), lds, st, scene->view,
+                               reinterpret_cast<const float4*>(rays_dev), n_rays, any_hit,
+                               reinterpret_cast<int4*>(hits_dev), P.counters, P.err);
+        HIP_OK(hipGetLastError());
+    }
+    int err = 0;
+    unsigned long long c[8] = { 0 };
+    HIP_OK(hipMemcpyAsync(&err, P.err, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(c, P.counters, sizeof(c), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    if (err) {
+        bb::set_error("bbgpu_trace: traversal stack overflow");
+        return BBGPU_ERR_DEVICE;
+    }
+    if (counters) {
+        counters->rays = uint64_t(n_rays);
+        counters->nodes = c[0];
+        counters->prims = c[1];
+        counters->accepts = c[2];
+    }
+    return BBGPU_OK;
+}
+
+int bbgpu_render(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams* params, float* film_sum_dev,
+                 int32_t* film_count_dev, void* stream, BbRenderStats* stats, int32_t count_traversal)
+{
+    if (!scene || !camera || !params || !film_sum_dev || !film_count_dev) {
+        bb::set_error("bbgpu_render: null argument");
+        return BBGPU_ERR_INVALID;
+    }
+    const BbRenderParams& p = *params;
+    if (camera->width <= 0 || camera->height <= 0 || p.spp < 0 || p.max_bounces < 0 || p.sample_begin < 0) {
+        bb::set_error("bbgpu_render: invalid camera or parameters");
+        return BBGPU_ERR_INVALID;
+    }
+    if (p.regularize_bsdf) {
+        bb::set_error("bbgpu_render: regularize_bsdf is not supported");
+        return BBGPU_ERR_UNSUPPORTED;
+    }
+    if (scene->view.has_null_material) {
+        bb::set_error("bbgpu_render: primitives without material are not supported");
+        return BBGPU_ERR_UNSUPPORTED;
+    }
+    if (trace_lds_bytes(scene->view) > 65536) {
+        bb::set_error("bbgpu_render: BVH depth %d exceeds the LDS traversal stack", scene->max_depth);
+        return BBGPU_ERR_UNSUPPORTED;
+    }
+    if (p.max_bounces > 1000) {
+        bb::set_error("bbgpu_render: max_bounces above 1000");
+        return BBGPU_ERR_INVALID;
+    }
+    const long long npix = (long long)camera->width * camera->height;
+    if (npix > (1ll << 30)) {
+        bb::set_error("bbgpu_render: image too large");
+        return BBGPU_ERR_INVALID;
+    }
+    if (p.spp == 0) return BBGPU_OK;
+    long long want = p.wavefront_paths > 0 ? p.wavefront_paths : (1ll << 22);
+    long long per_pass = std::max<long long>(1, want / npix); // samples per pass
+    per_pass = std::min<long long>(per_pass, p.spp);
+    const long long cap = per_pass * npix;
+    if (cap > (1ll << 30)) {
+        bb::set_error("bbgpu_render: wavefront too large");
+        return BBGPU_ERR_INVALID;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    SceneExtra* x = extra_for(scene);
+    if (!ensure_workspace(x->ws, size_t(cap), p.max_bounces + 1)) {
+        bb::set_error("bbgpu_render: workspace allocation of %lld paths failed", cap);
+        return BBGPU_ERR_NOMEM;
+    }
+    PathBufs& P = x->ws.P;
+    const GpuCamera cam = make_camera(*camera);
+    const SceneView& V = scene->view;
+    const size_t lds = trace_lds_bytes(V);
+    const bool timed = stats != nullptr;
+    const bool count = timed && count_traversal;
+    if (timed) memset(stats, 0, sizeof(*stats));
+
+    EventPool pool;
+    struct Span
+    {
+        hipEvent_t a, b;
+        int kind; // 0 closest, 1 shadow, 2 shade
+    };
+    std::vector<Span> spans;
+    auto mark = [&]() { hipEvent_t e = pool.get(); hipEventRecord(e, st); return e; };
+
+    hipEvent_t t_begin = nullptr;
+    if (timed) t_begin = mark();
+    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, st, P.err, 1);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, P.counters, 8);
+    const int nslots = 2 * (p.max_bounces + 2);
+    int64_t paths_total = 0;
+    for (long long s0 = 0; s0 < p.spp; s0 += per_pass) {
+        const long long k = std::min<long long>(per_pass, p.spp - s0);
+        const long long n_paths = k * npix;
+        RenderConsts rc;
+        rc.n_paths = int32_t(n_paths);
+        rc.npix = int32_t(npix);
+        rc.width = camera->width;
+        rc.seed = p.seed;
+        rc.sample0 = int32_t(p.sample_begin + s0);
+        rc.max_bounces = p.max_bounces;
+        rc.rr_min_bounces = p.rr_min_bounces;
+        paths_total += n_paths;
+        hipEvent_t a = timed ? mark() : nullptr;
+        hipLaunchKernelGGL(k_zero_ints, dim3((nslots + 255) / 256), dim3(256), 0, st, P.counts, nslots);
+        hipLaunchKernelGGL(k_raygen, dim3(unsigned((n_paths + kShadeBlock - 1) / kShadeBlock)), dim3(kShadeBlock), 0, st,
+                           cam, rc, P);
+        if (timed) spans.push_back({ a, mark(), 2 });
+        const int g_trace = grid_for(n_paths, kTraceBlock);
+        const int g_shade = grid_for(n_paths, kShadeBlock);
+        for (int depth = 0; depth <= p.max_bounces; ++depth) {
+            const int in = depth & 1;
+            a = timed ? mark() : nullptr;
+            if (count)
+                hipLaunchKernelGGL(k_trace_closest<true>, dim3(g_trace), dim3(kTraceBlock), lds, st, V,
+                                   P.counts + 2 * depth, P.ray[in], P.hit, P.counters, P.err);
+            else
+                hipLaunchKernelGGL(k_trace_closest<false>, dim3(g_trace), dim3(kTraceBlock), lds, st, V,
+                                   P.counts + 2 * depth, P.ray[in], P.hit, P.counters, P.err);
+            hipEvent_t b = timed ? mark() : nullptr;
+            if (timed) spans.push_back({ a, b, 0 });
+            hipLaunchKernelGGL(k_shade, dim3(g_shade), dim3(kShadeBlock), 0, st, V, rc, P, depth);
+            hipEvent_t c2 = timed ? mark() : nullptr;
+            if (timed) spans.push_back({ b, c2, 2 });
+            if (count)
+                hipLaunchKernelGGL(k_trace_shadow<true>, dim3(g_trace), dim3(kTraceBlock), lds, st, V,
+                                   P.counts + 2 * depth + 1, P.sray, P.scontrib, P.L, P.counters, P.err);
+            else
+                hipLaunchKernelGGL(k_trace_shadow<false>, dim3(g_trace), dim3(kTraceBlock), lds, st, V,
+                                   P.counts + 2 * depth + 1, P.sray, P.scontrib, P.L, P.counters, P.err);
+            if (timed) spans.push_back({ c2, mark(), 1 });
+        }
+        a = timed ? mark() : nullptr;
+        hipLaunchKernelGGL(k_film, dim3((npix + kShadeBlock - 1) / kShadeBlock), dim3(kShadeBlock), 0, st, rc, P.L,
+                           film_sum_dev, film_count_dev);
+        if (timed) spans.push_back({ a, mark(), 2 });
+        HIP_OK(hipGetLastError());
+        if (timed) {
+            // ray counts of this pass (counts are reused by the next pass)
+            std::vector<int> h(nslots);
+            HIP_OK(hipMemcpyAsync(h.data(), P.counts, sizeof(int) * nslots, hipMemcpyDeviceToHost, st));
+            HIP_OK(hipStreamSynchronize(st));
+            for (int dd = 0; dd <= p.max_bounces; ++dd) {
+                stats->rays_closest += h[2 * dd];
+                stats->rays_shadow += h[2 * dd + 1];
+            }
+        }
+    }
+    hipEvent_t t_end = timed ? mark() : nullptr;
+    int err = 0;
+    unsigned long long c[8] = { 0 };
+    HIP_OK(hipMemcpyAsync(&err, P.err, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(c, P.counters, sizeof(c), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    if (err) {
+        bb::set_error("bbgpu_render: traversal stack overflow");
+        return BBGPU_ERR_DEVICE;
+    }
+    if (timed) {
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, t_begin, t_end);
+        stats->ms_total = ms;
+        stats->ms_trace_closest = stats->ms_trace_shadow = stats->ms_shade = 0.0;
+        stats->launches_closest = stats->launches_shadow = 0;
+        for (const Span& sp : spans) {
+            float d = 0.0f;
+            hipEventElapsedTime(&d, sp.a, sp.b);
+            if (sp.kind == 0) {
+                stats->ms_trace_closest += d;
+                stats->launches_closest++;
+            } else if (sp.kind == 1) {
+                stats->ms_trace_shadow += d;
+                stats->launches_shadow++;
+            } else {
+                stats->ms_shade += d;
+            }
+        }
+        stats->paths = paths_total;
+        if (count) {
+            stats->closest.rays = uint64_t(stats->rays_closest);
+            stats->closest.nodes = c[0];
+            stats->closest.prims = c[1];
+            stats->closest.accepts = c[2];
+            stats->shadow.rays = uint64_t(stats->rays_shadow);
+            stats->shadow.nodes = c[3];
+            stats->shadow.prims = c[4];
+            stats->shadow.accepts = c[5];
+        }
+    }
+    return BBGPU_OK;
+}
+
+} // extern "C"
+
+// releases the per-scene render workspace; called from bbgpu_scene_destroy
+namespace bb
+{
+void release_scene_workspace(BbScene* s)
+{
+    std::lock_guard<std::mutex> g(g_extras_lock);
+    for (size_t i = 0; i < g_extras.size(); ++i)
+        if (g_extras[i].first == s) {
+            free_workspace(g_extras[i].second->ws);
+            delete g_extras[i].second;
+            g_extras.erase(g_extras.begin() + long(i));
+            return;
+        }
+}
+} // namespace bb

@@ -1,0 +1,289 @@
+/*
+ * bbgpu.h — C ABI of the MI355X path-tracer library (libbbgpu.so).
+ *
+ * The library replaces the hot path of Bulbit's unidirectional path tracer:
+ *   UniDirectionalRayIntegrator::Render  src/integrator/integrator.cpp:162-210
+ *   PathIntegrator::Li / SampleDirectLight src/integrator/path_integrator.cpp:26-194
+ *   BVH::BVH / Intersect / IntersectAny   src/accel/bvh.cpp:8-361, include/bulbit/bvh.h:101-155
+ * behind plain C entry points: opaque handles, plain pointers and sizes, int
+ * status codes (0 = ok), no C++ or torch types.  The C++ drop-in classes in
+ * include/bulbit/ are the intended callers; tests and benchmarks may call this
+ * ABI directly.
+ *
+ * All floating point data is IEEE float32.  Device pointers are HIP device
+ * allocations; a stream argument of NULL means the default stream.
+ */
+#ifndef BBGPU_H
+#define BBGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BBGPU_ABI_VERSION 1
+
+/* status codes */
+enum {
+    BBGPU_OK = 0,
+    BBGPU_ERR_INVALID = 1,     /* bad argument: null pointer, bad size, bad index */
+    BBGPU_ERR_UNSUPPORTED = 2, /* a feature outside the implemented subset */
+    BBGPU_ERR_DEVICE = 3,      /* a HIP runtime call failed */
+    BBGPU_ERR_NOMEM = 4,
+};
+
+/* ---------------------------------------------------------------- scene -- */
+
+enum { BB_SHAPE_TRIANGLE = 0, BB_SHAPE_SPHERE = 1 };
+
+/* One entry of Scene::GetPrimitives() (include/bulbit/scene.h:40,116-121,
+ * include/bulbit/primitive.h:14-31): a shape plus an optional material.
+ * Hit identities reported by the library are indices into this array. */
+typedef struct BbPrimitive {
+    int32_t shape_kind;  /* BB_SHAPE_* */
+    int32_t shape_index; /* triangle index into BbSceneDesc.indices, or sphere index */
+    int32_t material;    /* index into BbSceneDesc.materials, -1 = none (pass-through) */
+    int32_t reserved;
+} BbPrimitive;
+
+/* Sphere(const Transform&, Float radius)  include/bulbit/shapes.h:12-31,
+ * src/geometry/sphere.cpp:18-28.  Transform = position, quaternion (x,y,z,w),
+ * per-axis scale (include/bulbit/transform.h:9-13). */
+typedef struct BbSphere {
+    float p[3];
+    float q[4];
+    float s[3];
+    float radius;
+} BbSphere;
+
+enum {
+    BB_MAT_DIFFUSE = 0,            /* DiffuseMaterial     src/material/diffuse_material.cpp:21-36 */
+    BB_MAT_DIELECTRIC = 1,         /* DielectricMaterial  src/material/dielectric_material.cpp:28-52 */
+    BB_MAT_CONDUCTOR = 2,          /* ConductorMaterial(eta, k, ...)  conductor_material.cpp:52-99 */
+    BB_MAT_CONDUCTOR_R = 3,        /* ConductorMaterial(R, ...)       conductor_material.cpp:33-50 */
+    BB_MAT_METALLIC_ROUGHNESS = 4, /* MetallicRoughnessMaterial  metallic_roughness_material.cpp:28-39 */
+};
+
+/* Materials with constant textures.  Field meaning per type:
+ *   DIFFUSE:      color = reflectance, u_roughness = roughness (only 0 is supported: Lambertian)
+ *   DIELECTRIC:   eta[0] = eta, color = reflectance, u/v_roughness
+ *   CONDUCTOR:    eta, k, color = lobe reflectance (1,1,1 when absent), u/v_roughness
+ *   CONDUCTOR_R:  k = physical reflectance R, color = lobe reflectance, u/v_roughness
+ *   METALLIC_ROUGHNESS: color = basecolor, metallic, u/v_roughness
+ * energy_compensation selects the *MultiScattering BxDF variants, which are
+ * supported only where they coincide with the single-scattering BxDF
+ * (smooth dielectric). */
+typedef struct BbMaterial {
+    int32_t type;
+    int32_t energy_compensation;
+    float color[3];
+    float eta[3];
+    float k[3];
+    float u_roughness;
+    float v_roughness;
+    float metallic;
+} BbMaterial;
+
+enum {
+    BB_LIGHT_SPOT = 0,              /* SpotLight            src/light/spot_light.cpp:7-24 */
+    BB_LIGHT_UNIFORM_INFINITE = 1,  /* UniformInfiniteLight src/light/uniform_infinite_light.cpp:7-12 */
+    BB_LIGHT_DIFFUSE_AREA = 2,      /* DiffuseAreaLight     src/light/diffuse_area_light.cpp:9-15 */
+};
+
+/* Constructor arguments of the three light types, in Scene::GetLights() order
+ * (the order fixes the power-sampler CDF, power_light_sampler.cpp:8-23). */
+typedef struct BbLight {
+    int32_t type;
+    int32_t primitive;          /* DIFFUSE_AREA: index into primitives (a triangle) */
+    int32_t two_sided;          /* DIFFUSE_AREA */
+    int32_t reserved;
+    float position[3];          /* SPOT */
+    float direction[3];         /* SPOT: the constructor's direction argument */
+    float spectrum[3];          /* SPOT intensity / INFINITE l / AREA emission */
+    float scale;                /* INFINITE scale */
+    float angle_max;            /* SPOT, degrees */
+    float angle_falloff_start;  /* SPOT, degrees */
+} BbLight;
+
+/* Host-side scene description.  Vertex data is in world space (the Mesh
+ * constructor's transform already applied, src/geometry/mesh.cpp:8-39).
+ * normals and tangents are required when triangles are present; uvs may be
+ * NULL (they only feed texture lookups, and all textures are constant). */
+typedef struct BbSceneDesc {
+    int32_t n_vertices;
+    const float* positions;  /* 3 * n_vertices */
+    const float* normals;    /* 3 * n_vertices */
+    const float* tangents;   /* 3 * n_vertices */
+    const float* uvs;        /* 2 * n_vertices, may be NULL */
+    int32_t n_triangles;
+    const int32_t* indices;  /* 3 * n_triangles */
+    int32_t n_spheres;
+    const BbSphere* spheres;
+    int32_t n_primitives;
+    const BbPrimitive* primitives;
+    int32_t n_materials;
+    const BbMaterial* materials;
+    int32_t n_lights;
+    const BbLight* lights;
+} BbSceneDesc;
+
+/* LinearBVHNode, include/bulbit/bvh.h:46-58: 32 bytes, same field meaning. */
+typedef struct BbBvhNode {
+    float bmin[3];
+    float bmax[3];
+    int32_t offset;   /* leaf: first slot in the ordered primitive list; internal: child2 index */
+    uint16_t count;   /* leaf: primitive count; internal: 0 */
+    uint8_t axis;     /* internal: split axis; leaf: bit 0 set when the leaf holds a sphere */
+    uint8_t pad;
+} BbBvhNode;
+
+typedef struct BbScene BbScene; /* opaque */
+
+typedef struct BbSceneInfo {
+    int32_t n_nodes;
+    int32_t n_primitives;
+    int32_t n_lights;
+    int32_t n_infinite_lights;
+    float world_min[3];       /* root box, BVH::GetAABB  src/accel/bvh.cpp:363-366 */
+    float world_max[3];
+    double build_seconds;     /* host BVH build time */
+    size_t device_bytes;      /* device memory held by the scene */
+} BbSceneInfo;
+
+/* ----------------------------------------------------------- traversal -- */
+
+/* One ray: origin, t_min, direction, t_max (8 floats, 32 bytes). */
+typedef struct BbRay {
+    float o[3];
+    float t_min;
+    float d[3];
+    float t_max;
+} BbRay;
+
+/* Closest hit: prim = index into BbSceneDesc.primitives (-1 = miss), t, and
+ * the barycentrics (u, v) of the triangle test (0 for spheres).  Any-hit
+ * queries fill prim with 1 (occluded) or -1 only. */
+typedef struct BbHit {
+    int32_t prim;
+    float t;
+    float u;
+    float v;
+} BbHit;
+
+/* Traversal counters, summed over a batch: node boxes tested, primitive tests,
+ * accepted closer candidates.  They equal the work of the reference's
+ * RayCast (include/bulbit/bvh.h:101-155) on the same rays. */
+typedef struct BbTraceCounters {
+    uint64_t rays;
+    uint64_t nodes;
+    uint64_t prims;
+    uint64_t accepts;
+} BbTraceCounters;
+
+/* ------------------------------------------------------------- render -- */
+
+/* PerspectiveCamera(const Transform&, vfov, aperture_radius, focus_distance,
+ * resolution, medium, filter) with a BoxFilter(extent):
+ * src/camera/perspective_camera.cpp:8-38, src/reconstruction/box_filter.cpp:17-21.
+ * The transform's scale does not enter the camera. */
+typedef struct BbCameraDesc {
+    float position[3];
+    float orientation[4]; /* quaternion x, y, z, w */
+    float vfov_deg;
+    float aperture_radius;
+    float focus_distance;
+    int32_t width;
+    int32_t height;
+    float filter_extent;  /* BoxFilter extent (default 1) */
+} BbCameraDesc;
+
+/* PathIntegrator(accel, lights, sampler, max_bounces, rr_min_bounces,
+ * regularize_bsdf) + IndependentSampler(spp, seed):
+ * include/bulbit/integrators.h:92-114, src/sampler/independent_sampler.cpp:8-12.
+ * The remaining fields tune the GPU pipeline; 0 selects the default. */
+typedef struct BbRenderParams {
+    int32_t spp;
+    int32_t seed;
+    int32_t max_bounces;
+    int32_t rr_min_bounces;
+    int32_t regularize_bsdf;  /* only 0 is supported */
+    int32_t wavefront_paths;  /* paths in flight per pass (0 = auto) */
+    int32_t block_size;       /* threads per block for the wavefront kernels (0 = auto) */
+    int32_t sample_begin;     /* render samples [sample_begin, sample_begin+spp) */
+} BbRenderParams;
+
+/* Per-render statistics, filled when a pointer is passed to bbgpu_render.
+ * Kernel times come from HIP events recorded around each launch on the
+ * render stream. */
+typedef struct BbRenderStats {
+    double ms_total;          /* render wall time on the stream (events) */
+    double ms_trace_closest;  /* summed over all closest-hit launches */
+    double ms_trace_shadow;   /* summed over all any-hit launches */
+    double ms_shade;          /* raygen + shade + film kernels */
+    int64_t launches_closest;
+    int64_t launches_shadow;
+    int64_t rays_closest;     /* closest-hit rays traced */
+    int64_t rays_shadow;      /* shadow rays traced */
+    int64_t paths;            /* camera samples */
+    BbTraceCounters closest;  /* filled only with count_traversal = 1 */
+    BbTraceCounters shadow;
+} BbRenderStats;
+
+/* ------------------------------------------------------------ entry points */
+
+/* ABI version, BBGPU_ABI_VERSION. */
+int bbgpu_abi_version(void);
+
+/* Copies the last error message of the calling thread into buf. */
+int bbgpu_last_error(char* buf, size_t buf_size);
+
+/* Number of visible HIP devices (0 when none); selects nothing. */
+int bbgpu_device_count(int32_t* count);
+
+/* Builds the SAH BVH on the host exactly as BVH::BVH
+ * (src/accel/bvh.cpp:8-49, 56-271, 273-297) and returns the flattened nodes and
+ * the primitive order.  Host-only: makes no HIP call.  nodes_out must hold
+ * 2 * n_primitives - 1 entries, ordered_prims_out n_primitives entries. */
+int bbgpu_build_bvh(const BbSceneDesc* desc, BbBvhNode* nodes_out, int32_t* ordered_prims_out,
+                    int32_t* n_nodes_out);
+
+/* Replaces BVH construction + Integrator::Integrator's light set-up
+ * (src/accel/bvh.cpp:8-49, src/integrator/integrator.cpp:107-152): builds the
+ * BVH, preprocesses lights, builds the power-sampler CDF
+ * (src/light_sampler/power_light_sampler.cpp:8-23) and uploads everything to
+ * the current HIP device.  The library owns the device memory. */
+int bbgpu_scene_create(const BbSceneDesc* desc, BbScene** scene_out);
+int bbgpu_scene_destroy(BbScene* scene);
+int bbgpu_scene_info(const BbScene* scene, BbSceneInfo* info_out);
+
+/* Replaces BVH::Intersect / BVH::IntersectAny over a ray batch
+ * (src/accel/bvh.cpp:299-361): rays and hits are device pointers.
+ * counters (host pointer, may be NULL) requests traversal counts; the call
+ * then synchronizes the stream. */
+int bbgpu_trace(BbScene* scene, const BbRay* rays_dev, int32_t n_rays, int32_t any_hit, BbHit* hits_dev,
+                BbTraceCounters* counters, void* stream);
+
+/* Replaces UniDirectionalRayIntegrator::Render with PathIntegrator::Li
+ * (src/integrator/integrator.cpp:162-210, path_integrator.cpp:26-194):
+ * accumulates samples [sample_begin, sample_begin + spp) of every pixel into
+ * film_sum_dev (3 floats per pixel, row-major, added to the existing contents
+ * in sample order) and film_count_dev (1 int32 per pixel), as Film::AddSample
+ * does (src/reconstruction/film.cpp:26-37).  NaN/Inf samples are dropped.
+ * stats (host pointer, may be NULL) requests timing; the call then
+ * synchronizes the stream.  count_traversal = 1 also fills the traversal
+ * counters (slower kernels; for measurement only). */
+int bbgpu_render(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams* params, float* film_sum_dev,
+                 int32_t* film_count_dev, void* stream, BbRenderStats* stats, int32_t count_traversal);
+
+/* Transform::LookAt (include/bulbit/transform.h:112-122): position and
+ * orientation quaternion (x, y, z, w) of a camera at `position` looking at
+ * `target`.  Host-only helper for callers building a BbCameraDesc. */
+int bbgpu_look_at(const float position[3], const float target[3], const float up[3], float orientation_out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BBGPU_H */

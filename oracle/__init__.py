@@ -1,0 +1,7 @@
+"""The oracle: a CPU restatement of Bulbit's path tracer (test infrastructure).
+
+Only tests/, the smoke check and the benchmark's CPU-baseline leg import this
+package, and only as the checker or the CPU baseline; the product never calls
+it.
+"""
+from .binding import OracleScene, build_oracle, load_oracle  # noqa: F401
