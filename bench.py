@@ -1,0 +1,189 @@
+"""Benchmark: Msamples/s of the wavefront path tracer at 1080p, 8 bounces.
+
+Workload (one step = one full render): the displaced-icosphere scene at
+1920x1080, 64 samples per pixel, max_bounces 8, 1,310,786 primitives
+(icosphere level 8 + ground + 64 emissive triangles), PowerLightSampler,
+float32.  The scene and its BVH are built and resident in HBM before timing.
+
+Printed on stdout: one JSON line with the metric, a roofline entry for the
+dominant kernel (closest-hit traversal) from HIP events recorded around every
+launch inside the timed region, and the CPU baseline: the oracle (a CPU
+restatement of the reference path tracer) on a bounded, strided sample of the
+same frame on the host cores.  Progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Msamples/s at 1080p, 8 bounces; relMSE at equal time, cache vs plain PT"
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per closest-hit ray: 32 B per node box tested, 36 B per
+# triangle tested (p0, p1, p2), 4 B per accepted candidate, 28 B ray in
+# (o, d, t_max), 16 B hit out
+BYTES_NODE, BYTES_PRIM, BYTES_ACCEPT, BYTES_RAY = 32, 36, 4, 28 + 16
+PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "trace_closest_c2_traffic.json")
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--spp", type=int, default=64)
+    ap.add_argument("--max-bounces", type=int, default=8)
+    ap.add_argument("--level", type=int, default=8)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU work of the baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--wavefront-paths", type=int, default=0)
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        sys.exit("bench.py runs on exactly one GPU: the path tracer is a single-GPU build "
+                 "(multi-GPU sharding is a design note only); rerun with --gpus 1")
+
+    import numpy as np
+    import torch
+
+    from bulbit_amd import scenes
+    from bulbit_amd.gpu import GpuScene
+
+    if not torch.cuda.is_available():
+        sys.exit("no HIP device visible")
+
+    log(f"[bench] generating scene level {args.level} at {args.width}x{args.height}")
+    t0 = time.time()
+    sc, cam = scenes.icosphere_scene(width=args.width, height=args.height, level=args.level)
+    log(f"[bench] scene generated in {time.time() - t0:.2f}s: {len(sc.prims)} primitives")
+    g = GpuScene(sc)
+    log(f"[bench] BVH {g.info.n_nodes} nodes built in {g.info.build_seconds:.2f}s, {g.info.device_bytes / 1e6:.1f} MB on device")
+    params = scenes.render_params(args.spp, max_bounces=args.max_bounces, wavefront_paths=args.wavefront_paths)
+    s_dev, c_dev = g.new_film(cam)
+
+    for i in range(args.warmup):
+        s_dev.zero_()
+        c_dev.zero_()
+        g.render_device(cam, params, s_dev.data_ptr(), c_dev.data_ptr())
+        log(f"[bench] warmup {i + 1}/{args.warmup} done")
+
+    # counted run (untimed): traversal work of exactly the rays the timed steps trace
+    s_dev.zero_()
+    c_dev.zero_()
+    cst = g.render_device(cam, params, s_dev.data_ptr(), c_dev.data_ptr(), stats=True, count_traversal=True)
+    cl = cst.closest
+    log(f"[bench] counted run: {cl.rays} closest rays, {cl.nodes / max(cl.rays, 1):.1f} nodes/ray, "
+        f"{cl.prims / max(cl.rays, 1):.2f} prims/ray")
+
+    torch.cuda.synchronize()
+    stats = []
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        s_dev.zero_()
+        c_dev.zero_()
+        st = g.render_device(cam, params, s_dev.data_ptr(), c_dev.data_ptr(), stats=True)
+        stats.append(st)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    for i, st in enumerate(stats):
+        log(f"[bench] step {i + 1}: {st.ms_total:.1f} ms (closest {st.ms_trace_closest:.1f}, shadow "
+            f"{st.ms_trace_shadow:.1f}, shade {st.ms_shade:.1f})")
+
+    samples_per_step = args.width * args.height * args.spp
+    value = samples_per_step * args.steps / elapsed / 1e6
+    ms_closest = float(np.mean([s.ms_trace_closest for s in stats]))
+    launches = int(stats[0].launches_closest)
+    bytes_step = (BYTES_NODE * cl.nodes + BYTES_PRIM * cl.prims + BYTES_ACCEPT * cl.accepts + BYTES_RAY * cl.rays)
+    achieved = bytes_step / (ms_closest / 1e3) / 1e9
+    traffic = None
+    if os.path.exists(PROFILE_TRAFFIC):
+        with open(PROFILE_TRAFFIC) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    rays_total = stats[0].rays_closest + stats[0].rays_shadow
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "displaced icosphere (level %d, %d primitives, 64 area lights), %dx%d, %d spp, max_bounces %d, "
+                        "plain wavefront path tracer" % (args.level, len(sc.prims), args.width, args.height, args.spp,
+                                                         args.max_bounces),
+            "width": args.width,
+            "height": args.height,
+            "spp": args.spp,
+            "max_bounces": args.max_bounces,
+            "primitives": len(sc.prims),
+        },
+        "roofline": {
+            "kernel": "k_trace_closest (BVH2 closest-hit traversal)",
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "launches_per_step": launches,
+            "avg_launch_ms": round(ms_closest / max(launches, 1), 4),
+            "algorithmic_bytes_per_launch": int(bytes_step / max(launches, 1)),
+        },
+        "rays_per_s": round(rays_total * args.steps / elapsed, 1),
+        "relmse_equal_time_cache_vs_plain": None,
+    }
+
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(sc, cam, params, args, samples_per_step)
+        out["speedup_vs_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(sc, cam, params, args, samples_full):
+    """The oracle on the host cores: full frames of the same scene, fewer spp."""
+    from oracle import OracleScene
+    from bulbit_amd import scenes
+
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    t0 = time.time()
+    o = OracleScene(sc)
+    log(f"[bench] oracle scene built in {time.time() - t0:.2f}s; calibrating the CPU sample on {threads} threads")
+    _, _, secs = o.render(cam, scenes.render_params(1, max_bounces=args.max_bounces), threads=threads)
+    npix = cam.width * cam.height
+    rate = npix / max(secs, 1e-9)
+    spp = int(max(1, min(args.spp, round(args.cpu_seconds * rate / npix))))
+    p = scenes.render_params(spp, max_bounces=args.max_bounces, sample_begin=1)
+    _, _, secs = o.render(cam, p, threads=threads)
+    n = npix * spp
+    value = n / secs / 1e6
+    log(f"[bench] CPU baseline: {n} samples in {secs:.2f}s = {value:.3f} Msamples/s")
+    return {
+        "value": round(value, 4),
+        "unit": "Msamples/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"oracle (CPU restatement of PathIntegrator) rendering the same {cam.width}x{cam.height} frame at "
+                  f"{spp} spp (samples 1..{spp}), {n} samples in {secs:.1f}s on {threads} threads; BVH build excluded",
+    }
+
+
+if __name__ == "__main__":
+    main()

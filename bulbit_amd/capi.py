@@ -102,6 +102,14 @@ BVH_NODE_DTYPE = np.dtype(
 assert BVH_NODE_DTYPE.itemsize == 32
 
 
+class BbRay(ctypes.Structure):
+    _fields_ = [("o", c_float * 3), ("t_min", c_float), ("d", c_float * 3), ("t_max", c_float)]
+
+
+class BbHit(ctypes.Structure):
+    _fields_ = [("prim", c_int32), ("t", c_float), ("u", c_float), ("v", c_float)]
+
+
 class BbSceneInfo(ctypes.Structure):
     _fields_ = [
         ("n_nodes", c_int32),
@@ -197,6 +205,14 @@ def load_library(path: str | None = None):
     p = path or LIB_PATH
     if not os.path.exists(p):
         raise BbError(f"{p} is missing: build it with bulbit_amd.build.build() (no CPU fallback exists)")
+    # torch ships its own libamdhip64.so / libhsa-runtime64.so under the same
+    # sonames as /opt/rocm's.  Loading torch first makes libbbgpu.so bind to
+    # the runtime torch already holds, so one process never carries two HIP
+    # runtimes (a second one sees no device).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(p)
     lib.bbgpu_abi_version.restype = c_int32
     lib.bbgpu_last_error.argtypes = [c_char_p, c_size_t]

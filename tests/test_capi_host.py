@@ -1,0 +1,103 @@
+"""The C ABI library on the CPU: exports, host-only entry points (BVH build,
+LookAt) against the oracle, and error behaviour.  No kernel is launched."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from bulbit_amd import capi, scenes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_entry_points():
+    text = open(os.path.join(ROOT, "include", "bbgpu.h")).read()
+    return sorted(set(re.findall(r"^int (bbgpu_\w+)\(", text, flags=re.M)))
+
+
+def test_library_exports_every_declared_entry_point(bblib):
+    names = declared_entry_points()
+    assert names and sorted(capi.EXPORTS) == names
+    for n in names:
+        assert hasattr(bblib, n), n
+    assert bblib.bbgpu_abi_version() == 1
+
+
+def _build(bblib, sc):
+    d = sc.desc()
+    n = len(sc.prims)
+    nodes = np.zeros(max(1, 2 * n - 1), capi.BVH_NODE_DTYPE)
+    order = np.zeros(n, np.int32)
+    nn = ctypes.c_int32()
+    st = bblib.bbgpu_build_bvh(ctypes.byref(d), nodes.ctypes.data, order.ctypes.data, ctypes.byref(nn))
+    return st, nodes[: nn.value], order
+
+
+def _mixed_scene():
+    sc = scenes.SceneData()
+    m = sc.add_material(scenes.diffuse((0.5, 0.5, 0.5)))
+    rng = np.random.default_rng(4)
+    for i in range(40):
+        sc.add_sphere(tuple(rng.uniform(-5, 5, 3)), float(rng.uniform(0.1, 1.0)), m)
+    p = rng.uniform(-6, 6, (300, 3))
+    sc.add_mesh(p, np.tile([0, 0, 1.0], (300, 1)), np.tile([1.0, 0, 0], (300, 1)), np.arange(300).reshape(-1, 3), m)
+    return sc
+
+
+@pytest.mark.parametrize("make", [
+    lambda: scenes.readme_scene(16, 9)[0],
+    lambda: scenes.icosphere_scene(32, 18, level=4)[0],
+    lambda: scenes.icosphere_scene(32, 18, level=7)[0],
+    _mixed_scene,
+], ids=["readme", "ico4", "ico7", "mixed"])
+def test_host_bvh_identical_to_oracle(bblib, make):
+    from oracle import OracleScene
+
+    sc = make()
+    st, nodes, order = _build(bblib, sc)
+    assert st == 0, capi.last_error(bblib)
+    on, oo = OracleScene(sc).bvh()
+    assert nodes.tobytes() == on.tobytes()
+    assert np.array_equal(order, oo)
+
+
+def test_look_at_identical_to_oracle(bblib, orc):
+    rng = np.random.default_rng(9)
+    F3 = ctypes.c_float * 3
+    for _ in range(300):
+        p, t = rng.uniform(-10, 10, 3), rng.uniform(-10, 10, 3)
+        up = (0.0, 1.0, 0.0) if rng.random() < 0.7 else tuple(rng.uniform(-1, 1, 3))
+        qa, qb = (ctypes.c_float * 4)(), (ctypes.c_float * 4)()
+        assert bblib.bbgpu_look_at(F3(*p), F3(*t), F3(*up), qa) == 0
+        orc.orc_look_at(F3(*p), F3(*t), F3(*up), qb)
+        assert np.array_equal(np.array(qa, np.float32), np.array(qb, np.float32))
+
+
+def test_invalid_descriptions_are_rejected(bblib):
+    sc = scenes.icosphere_scene(16, 9, level=1)[0]
+    sc.indices = sc.indices.copy()
+    sc.indices[3, 1] = len(sc.positions) + 5
+    st, _, _ = _build(bblib, sc)
+    assert st == 1
+    assert "index" in capi.last_error(bblib)
+    sc2 = scenes.icosphere_scene(16, 9, level=1)[0]
+    sc2.prims = sc2.prims.copy()
+    sc2.prims[0, 2] = 999
+    assert _build(bblib, sc2)[0] == 1
+    assert bblib.bbgpu_build_bvh(None, None, None, None) == 1
+
+
+def test_scene_create_reports_missing_device_or_unsupported(bblib):
+    n = ctypes.c_int32(-1)
+    assert bblib.bbgpu_device_count(ctypes.byref(n)) == 0
+    sc, _ = scenes.readme_scene(16, 9)
+    sc.materials[0] = scenes.conductor(scenes.GOLD_ETA, scenes.GOLD_K, 0.3, energy_compensation=1)
+    h = ctypes.c_void_p()
+    st = bblib.bbgpu_scene_create(ctypes.byref(sc.desc()), ctypes.byref(h))
+    if n.value == 0:
+        assert st == 3 and "no HIP device" in capi.last_error(bblib)
+    else:
+        assert st == 2 and "not supported" in capi.last_error(bblib)
+    assert not h.value

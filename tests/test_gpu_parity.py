@@ -1,0 +1,203 @@
+"""Parity of the HIP path with the oracle, through the C ABI, on an MI355X.
+
+* closest-hit primitive ids, hit distances and barycentrics: bit-exact on
+  seeded random rays, camera rays and secondary rays; traversal counters equal;
+* renders with identical RNG streams: >= 99.9% of pixels within 1e-3 relative
+  (the residue comes from glibc sinf/cosf vs correctly rounded sin/cos);
+  sample counts identical;
+* at the benchmark's full size: determinism, sample-range splitting and
+  wavefront-size invariance (properties that need no oracle run).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from bulbit_amd import scenes
+
+from conftest import pixel_agreement
+
+pytestmark = pytest.mark.gpu
+
+THREADS = max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def readme():
+    from bulbit_amd.gpu import GpuScene
+    from oracle import OracleScene
+
+    sc, cam = scenes.readme_scene()
+    return sc, cam, GpuScene(sc), OracleScene(sc)
+
+
+@pytest.fixture(scope="module")
+def ico6():
+    from bulbit_amd.gpu import GpuScene
+    from oracle import OracleScene
+
+    sc, cam = scenes.icosphere_scene(width=320, height=180, level=6)
+    return sc, cam, GpuScene(sc), OracleScene(sc)
+
+
+@pytest.fixture(scope="module")
+def c2():
+    from bulbit_amd.gpu import GpuScene
+    from oracle import OracleScene
+
+    sc, cam = scenes.icosphere_scene()
+    return sc, cam, GpuScene(sc), OracleScene(sc)
+
+
+def _check_trace(g, o, rays, any_hit=False):
+    gh, gc = g.trace(rays, any_hit=any_hit, count=True)
+    oh, oc = o.trace(rays, any_hit=any_hit, threads=THREADS)
+    assert np.array_equal(gh["prim"], oh["prim"])
+    if not any_hit:
+        for f in ("t", "u", "v"):
+            assert np.array_equal(gh[f].view(np.int32), oh[f].view(np.int32)), f
+    assert gc.as_dict() == oc.as_dict()
+    return gh
+
+
+def test_trace_readme_sphere(readme):
+    sc, cam, g, o = readme
+    bmin, bmax = o.world_bounds()
+    rays = scenes.random_rays(bmin, bmax, 200_000, seed=1)
+    h = _check_trace(g, o, rays)
+    assert 0.2 < np.mean(h["prim"] >= 0) < 0.8
+    _check_trace(g, o, rays, any_hit=True)
+
+
+def test_trace_icosphere_random_and_bounded(ico6):
+    sc, cam, g, o = ico6
+    bmin, bmax = o.world_bounds()
+    rays = scenes.random_rays(bmin, bmax, 500_000, seed=2)
+    _check_trace(g, o, rays)
+    rng = np.random.default_rng(3)
+    bounded = scenes.random_rays(bmin, bmax, 500_000, seed=4, t_max=rng.uniform(0, 10, 500_000))
+    _check_trace(g, o, bounded)
+    _check_trace(g, o, bounded, any_hit=True)
+
+
+def test_trace_full_scene_rays(c2):
+    """1.31M-primitive scene: 2M random rays plus camera and secondary rays."""
+    sc, cam, g, o = c2
+    bmin, bmax = o.world_bounds()
+    rays = scenes.random_rays(bmin, bmax, 2_000_000, seed=5)
+    h = _check_trace(g, o, rays)
+    log(f"random rays hit fraction {np.mean(h['prim'] >= 0):.3f}")
+    # camera rays through a grid of pixels and secondary rays from their hits
+    from oracle import load_oracle
+    import ctypes
+    from bulbit_amd import capi
+
+    lib = load_oracle()
+    ys, xs = np.mgrid[0:cam.height:8, 0:cam.width:8]
+    cr = np.zeros(xs.size, capi.RAY_DTYPE)
+    ray = capi.BbRay()
+    u = (ctypes.c_float * 2)(0.5, 0.5)
+    for i, (x, y) in enumerate(zip(xs.ravel(), ys.ravel())):
+        lib.orc_camera_ray(ctypes.byref(cam), int(x), int(y), u, u, ctypes.byref(ray))
+        cr[i] = (tuple(ray.o), ray.t_min, tuple(ray.d), ray.t_max)
+    h = _check_trace(g, o, cr)
+    hit = h["prim"] >= 0
+    sec = np.zeros(int(hit.sum()), capi.RAY_DTYPE)
+    sec["o"] = cr["o"][hit] + cr["d"][hit] * h["t"][hit, None]
+    d = np.random.default_rng(6).normal(size=(len(sec), 3)).astype(np.float32)
+    sec["d"] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    sec["t_min"] = np.float32(1e-4)
+    sec["t_max"] = np.float32(np.inf)
+    _check_trace(g, o, sec)
+    _check_trace(g, o, sec, any_hit=True)
+
+
+def _render_pair(g, o, cam, params):
+    gs, gc, _ = g.render(cam, params)
+    os_, oc, _ = o.render(cam, params, threads=THREADS)
+    return gs / gc[..., None], os_ / oc[..., None], gc, oc
+
+
+def test_render_readme_scene_parity(readme):
+    sc, cam, g, o = readme
+    gi, oi, gc, oc = _render_pair(g, o, cam, scenes.render_params(16))
+    assert np.array_equal(gc, oc)
+    frac = pixel_agreement(gi, oi)
+    log(f"readme 640x360x16: {frac:.6f} within 1e-3, {np.mean(gi == oi):.4f} bit-equal")
+    assert frac >= 0.999
+
+
+def test_render_icosphere_parity(ico6):
+    sc, cam, g, o = ico6
+    gi, oi, gc, oc = _render_pair(g, o, cam, scenes.render_params(8))
+    assert np.array_equal(gc, oc)
+    frac = pixel_agreement(gi, oi)
+    log(f"ico6 320x180x8: {frac:.6f} within 1e-3, {np.mean(gi == oi):.4f} bit-equal")
+    assert frac >= 0.999
+
+
+def test_render_all_materials_parity():
+    """Rough dielectric, smooth and rough conductor, metallic-roughness and
+    Lambertian groups plus spot and environment lights."""
+    from bulbit_amd.gpu import GpuScene
+    from oracle import OracleScene
+
+    sc, cam = scenes.icosphere_scene(width=256, height=144, level=5, n_lights=16)
+    rng = np.random.default_rng(8)
+    mats = [scenes.dielectric(1.5, roughness=0.3), scenes.conductor(scenes.GOLD_ETA, scenes.GOLD_K, 0.0),
+            scenes.conductor(scenes.GOLD_ETA, scenes.GOLD_K, 0.4), scenes.metallic_roughness((0.8, 0.3, 0.2), 0.0, 0.4),
+            scenes.metallic_roughness((0.9, 0.8, 0.5), 1.0, 0.2), scenes.diffuse((0.7, 0.7, 0.7))]
+    base = len(sc.materials)
+    sc.materials += mats
+    n_ico = 20 * 4 ** 5
+    sc.prims[:n_ico, 2] = base + rng.integers(0, len(mats), n_ico // 64).repeat(64)
+    sc.add_spot_light((2.0, 4.0, 2.0), scenes.normalize32(np.array([2.0, 3.0, 2.0], np.float32)), (40.0, 40.0, 40.0), 30.0,
+                      20.0)
+    sc.add_infinite_light((0.05, 0.06, 0.08))
+    g, o = GpuScene(sc), OracleScene(sc)
+    gi, oi, gc, oc = _render_pair(g, o, cam, scenes.render_params(8))
+    assert np.array_equal(gc, oc)
+    frac = pixel_agreement(gi, oi)
+    log(f"all-materials 256x144x8: {frac:.6f} within 1e-3, {np.mean(gi == oi):.4f} bit-equal")
+    assert frac >= 0.999
+
+
+def test_render_full_scene_crop_parity(c2):
+    """The benchmark frame itself: GPU image rows against the oracle on a
+    strided set of rows at 2 spp."""
+    sc, cam, g, o = c2
+    params = scenes.render_params(2)
+    gs, gc, _ = g.render(cam, params)
+    rows = list(range(3, cam.height, 27))
+    os_ = np.zeros_like(gs)
+    oc = np.zeros_like(gc)
+    for y in rows:
+        o.render(cam, params, rect=(0, y, cam.width, y + 1), threads=THREADS, film=(os_, oc))
+    gi = gs[rows] / gc[rows][..., None]
+    oi = os_[rows] / oc[rows][..., None]
+    assert np.array_equal(gc[rows], oc[rows])
+    frac = pixel_agreement(gi, oi)
+    log(f"C2 rows x 2 spp: {frac:.6f} within 1e-3, {np.mean(gi == oi):.4f} bit-equal")
+    assert frac >= 0.999
+
+
+def test_full_scene_properties(c2):
+    """At 1920x1080: a 4-spp render is deterministic, equals two 2-spp renders
+    accumulated with sample_begin 0 and 2, and does not depend on the
+    wavefront size; every sample is finite here, so counts equal spp."""
+    sc, cam, g, o = c2
+    a, ca, _ = g.render(cam, scenes.render_params(4))
+    b, cb, _ = g.render(cam, scenes.render_params(4))
+    assert np.array_equal(a, b) and np.array_equal(ca, cb)
+    assert np.all(ca == 4)
+    film = g.new_film(cam)
+    g.render(cam, scenes.render_params(2), film=film)
+    s2, c2_, _ = g.render(cam, scenes.render_params(2, sample_begin=2), film=film)
+    assert np.array_equal(s2, a) and np.array_equal(c2_, ca)
+    c, cc, _ = g.render(cam, scenes.render_params(4, wavefront_paths=1 << 20))
+    assert np.array_equal(c, a) and np.array_equal(cc, ca)
+    assert np.isfinite(a).all() and a.min() >= 0

@@ -1,0 +1,454 @@
+"""Pins of the oracle (the CPU restatement of Bulbit's path tracer) against
+published vectors, closed forms, invariants and brute force.  CPU only.
+
+The reference has no tests or golden data and cannot be compiled here (it
+needs <format>, absent from this image's libstdc++, and NanoVDB headers), so
+each function of the oracle is pinned by an independent fact instead."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from bulbit_amd import capi, scenes
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_DIR = os.path.join(os.path.dirname(HERE), "oracle")
+F = ctypes.c_float
+
+
+def farr(*v):
+    return (F * len(v))(*v)
+
+
+# ------------------------------------------------------------ RNG / hashing
+
+def test_pcg32_published_demo_vector(orc):
+    # pcg32-demo of the PCG reference implementation: pcg32_srandom_r(42, 54)
+    out = (ctypes.c_uint32 * 6)()
+    orc.orc_pcg32_outputs(42, 54, 6, out)
+    assert list(out) == [0xA15C02B7, 0x7B47F409, 0xBA1D3330, 0x83D2F293, 0xBFA4784B, 0xCBED606E]
+
+
+def test_murmurhash64a_smhasher_verification(orc):
+    # SMHasher's verification value of MurmurHash64A ("Murmur2B"): hash keys
+    # {0..i-1} with seed 256 - i, hash the concatenated results with seed 0.
+    key = (ctypes.c_uint8 * 256)(*range(256))
+    hashes = np.zeros(256, np.uint64)
+    for i in range(256):
+        hashes[i] = orc.orc_murmur64a(key, i, 256 - i)
+    buf = hashes.tobytes()
+    final = orc.orc_murmur64a(buf, len(buf), 0)
+    assert final & 0xFFFFFFFF == 0x1F0D3804
+
+
+def test_pixel_hash_packs_sixteen_bytes(orc):
+    # Hash(pixel, seed, sample) = MurmurHash64A over the 16 packed bytes
+    b = np.array([3, 7, 11, 5], np.int32).tobytes()
+    assert orc.orc_pixel_sample_hash(3, 7, 11, 5) == orc.orc_murmur64a(b, 16, 0)
+
+
+def test_sampler_floats_are_pcg_outputs(orc):
+    # NextFloat = min(1 - eps, float(u32) * 2^-32), stream seeded (hash, initseq 1)
+    h = orc.orc_pixel_sample_hash(12, 34, 0, 9)
+    raw = (ctypes.c_uint32 * 8)()
+    orc.orc_pcg32_outputs(h, 1, 8, raw)
+    fl = (F * 8)()
+    orc.orc_sampler_floats(12, 34, 0, 9, 8, fl)
+    expect = np.minimum(np.float32(1) - np.finfo(np.float32).eps,
+                        np.array(raw, np.uint32).astype(np.float32) * np.float32(2.0 ** -32))
+    assert np.array_equal(np.array(fl, np.float32), expect.astype(np.float32))
+
+
+# ------------------------------------------------------ compiler semantics
+
+@pytest.fixture(scope="module")
+def probes():
+    subprocess.run(["make", "-C", ORACLE_DIR, "probes"], check=True, capture_output=True)
+    return os.path.join(ORACLE_DIR, "_build")
+
+
+def test_gxx_evaluates_sampler_arguments_right_to_left(probes):
+    out = subprocess.run([os.path.join(probes, "probe_argorder")], capture_output=True, text=True, check=True).stdout
+    # the first draw (1) lands in the last argument: u1 (lens) and u12
+    assert "camera u0=2 u1=1" in out
+    assert "bsdf u0=2 u12=1" in out
+
+
+def test_gxx_complex_float_semantics(probes):
+    out = subprocess.run([os.path.join(probes, "probe_complex")], capture_output=True, text=True, check=True).stdout
+    vals = dict(line.split() for line in out.strip().splitlines())
+    n = int(vals["n"])
+    # the oracle's FresnelComplex uses exactly these formulas
+    for k in ("div_double", "mul_float", "mul_real", "sub_real", "abs_hypot", "norm"):
+        assert int(vals[k]) == n, k
+    assert int(vals["div_smith"]) < n  # Smith's algorithm is not what g++ emits here
+
+
+# ------------------------------------------------------------------ camera
+
+def _pinhole(width, height, vfov=90.0, aperture=0.0, focus=1.0):
+    cam = capi.BbCameraDesc()
+    cam.position[:] = [0.0, 0.0, 0.0]
+    q = farr(0, 0, 0, 0)
+    orc_lib = __import__("oracle").load_oracle()
+    orc_lib.orc_look_at(farr(0, 0, 0), farr(0, 0, -1), farr(0, 1, 0), q)
+    cam.orientation[:] = list(q)
+    cam.vfov_deg = vfov
+    cam.aperture_radius = aperture
+    cam.focus_distance = focus
+    cam.width = width
+    cam.height = height
+    cam.filter_extent = 1.0
+    return cam
+
+
+def test_look_at_down_negative_z_is_half_turn_about_y(orc):
+    q = farr(0, 0, 0, 0)
+    orc.orc_look_at(farr(0, 0, 0), farr(0, 0, -1), farr(0, 1, 0), q)
+    assert list(q) == [0.0, 1.0, 0.0, 0.0]
+
+
+def test_camera_centre_ray_closed_form(orc):
+    cam = _pinhole(3, 3)
+    ray = capi.BbRay()
+    orc.orc_camera_ray(ctypes.byref(cam), 1, 1, farr(0.5, 0.5), farr(0.5, 0.5), ctypes.byref(ray))
+    assert list(ray.o) == [0.0, 0.0, 0.0]
+    assert list(ray.d) == [0.0, 0.0, -1.0]
+    orc.orc_camera_ray(ctypes.byref(cam), 0, 0, farr(0.5, 0.5), farr(0.5, 0.5), ctypes.byref(ray))
+    expect = np.array([-2.0, -2.0, -3.0]) / np.sqrt(17.0)
+    assert np.allclose(list(ray.d), expect, rtol=0, atol=1e-6)
+
+
+def test_camera_filter_and_lens_offsets(orc):
+    cam = _pinhole(3, 3, aperture=0.25)
+    ray = capi.BbRay()
+    # u1 = (1, 0.5): concentric disk -> (1, 0): origin moves by the lens radius along u
+    orc.orc_camera_ray(ctypes.byref(cam), 1, 1, farr(0.5, 0.5), farr(1.0, 0.5), ctypes.byref(ray))
+    assert np.allclose(list(ray.o), [0.25, 0.0, 0.0], atol=1e-7)
+    # u0 = (1, 0.5): box filter offset +0.5 px in x -> pixel point x = -1 + 2 * 2 / 3
+    orc.orc_camera_ray(ctypes.byref(cam), 1, 1, farr(1.0, 0.5), farr(0.5, 0.5), ctypes.byref(ray))
+    p = np.array([-1 + 2 * 2.0 / 3.0, 0.0, -1.0])
+    assert np.allclose(list(ray.d), p / np.linalg.norm(p), atol=1e-6)
+
+
+def test_concentric_disk_and_sphere_samples(orc):
+    out = farr(0, 0)
+    orc.orc_sample_concentric(farr(0.5, 0.5), out)
+    assert list(out) == [0.0, 0.0]
+    orc.orc_sample_concentric(farr(0.5, 1.0), out)
+    assert np.allclose(list(out), [0.0, 1.0], atol=1e-7)
+    rng = np.random.default_rng(0)
+    s = farr(0, 0, 0)
+    for u in rng.random((200, 2)):
+        orc.orc_sample_uniform_sphere(farr(*u), s)
+        assert abs(np.linalg.norm(list(s)) - 1.0) < 1e-6
+        assert np.isclose(s[2], 1.0 - 2.0 * np.float32(u[0]), atol=1e-6)
+
+
+# ------------------------------------------------------------- triangles
+
+def _tri_hit(orc, o, d, t_min=1e-4, t_max=np.inf):
+    ray = capi.BbRay()
+    ray.o[:] = list(o)
+    ray.d[:] = list(d)
+    ray.t_min = t_min
+    ray.t_max = t_max
+    t, u, v = F(), F(), F()
+    hit = orc.orc_triangle_intersect(farr(0, 0, 0), farr(1, 0, 0), farr(0, 1, 0), ctypes.byref(ray), ctypes.byref(t),
+                                     ctypes.byref(u), ctypes.byref(v))
+    return hit, t.value, u.value, v.value
+
+
+def test_moller_trumbore_exact_cases(orc):
+    assert _tri_hit(orc, (0.25, 0.25, 1.0), (0, 0, -1)) == (1, 1.0, 0.25, 0.25)
+    # t is measured in units of the (unnormalised) direction
+    assert _tri_hit(orc, (0.25, 0.25, 1.0), (0, 0, -2)) == (1, 0.5, 0.25, 0.25)
+    # back faces are hit too (no culling)
+    assert _tri_hit(orc, (0.25, 0.5, -1.0), (0, 0, 1)) == (1, 1.0, 0.25, 0.5)
+    # u + v == 1 on the hypotenuse is inside, beyond it is not
+    assert _tri_hit(orc, (0.5, 0.5, 1.0), (0, 0, -1))[0] == 1
+    assert _tri_hit(orc, (0.5, 0.625, 1.0), (0, 0, -1))[0] == 0
+    assert _tri_hit(orc, (-0.125, 0.25, 1.0), (0, 0, -1))[0] == 0
+    # parallel ray
+    assert _tri_hit(orc, (0.25, 0.25, 0.0), (1, 0, 0))[0] == 0
+    # t range, inclusive bounds
+    assert _tri_hit(orc, (0.25, 0.25, 1.0), (0, 0, -1), t_max=0.5)[0] == 0
+    assert _tri_hit(orc, (0.25, 0.25, 1.0), (0, 0, -1), t_max=1.0)[0] == 1
+    assert _tri_hit(orc, (0.25, 0.25, 1.0), (0, 0, -1), t_min=1.0)[0] == 1
+
+
+# ------------------------------------------------------------------ BVH
+
+def _soup(n, seed, spread=4.0, size=0.3):
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-spread, spread, (n, 3))
+    p = c[:, None, :] + rng.uniform(-size, size, (n, 3, 3))
+    sc = scenes.SceneData()
+    m = sc.add_material(scenes.diffuse((0.5, 0.5, 0.5)))
+    pos = p.reshape(-1, 3)
+    nrm = np.tile([0.0, 0.0, 1.0], (len(pos), 1))
+    tan = np.tile([1.0, 0.0, 0.0], (len(pos), 1))
+    sc.add_mesh(pos, nrm, tan, np.arange(3 * n).reshape(-1, 3), m)
+    return sc
+
+
+def _check_bvh_invariants(sc, nodes, order):
+    n = len(sc.prims)
+    assert sorted(order.tolist()) == list(range(n))
+    covered = np.zeros(n, np.int32)
+    pos = sc.positions
+    tri = sc.indices
+    eps = np.float32(np.finfo(np.float32).eps * 10)
+    for i, nd in enumerate(nodes):
+        if nd["count"] > 0:
+            sl = order[nd["offset"]: nd["offset"] + nd["count"]]
+            covered[nd["offset"]: nd["offset"] + nd["count"]] += 1
+            prims = sc.prims[sl]
+            if np.all(prims[:, 0] == capi.BB_SHAPE_TRIANGLE):
+                v = pos[tri[prims[:, 1]]]
+                assert np.array_equal(nd["bmin"], (v.min(axis=(0, 1)) - eps).astype(np.float32))
+                assert np.array_equal(nd["bmax"], (v.max(axis=(0, 1)) + eps).astype(np.float32))
+        else:
+            c1, c2 = nodes[i + 1], nodes[nd["offset"]]
+            assert nd["offset"] > i + 1
+            assert np.array_equal(nd["bmin"], np.minimum(c1["bmin"], c2["bmin"]))
+            assert np.array_equal(nd["bmax"], np.maximum(c1["bmax"], c2["bmax"]))
+    assert np.all(covered == 1)
+    assert len(nodes) <= 2 * n - 1
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_bvh_invariants_and_brute_force(seed):
+    from oracle import OracleScene
+
+    sc = _soup(3000, seed)
+    o = OracleScene(sc)
+    nodes, order = o.bvh()
+    _check_bvh_invariants(sc, nodes, order)
+    bmin, bmax = o.world_bounds()
+    rays = scenes.random_rays(bmin, bmax, 4000, seed + 10)
+    h, cnt = o.trace(rays, threads=4)
+    b = o.trace_brute(rays)
+    assert np.array_equal(h["prim"], b["prim"])
+    assert np.array_equal(h["t"].view(np.int32), b["t"].view(np.int32))
+    assert cnt.rays == 4000 and cnt.nodes >= cnt.rays
+    # any-hit agrees with closest-hit on whether something is hit
+    a, _ = o.trace(rays, any_hit=True, threads=4)
+    assert np.array_equal(a["prim"] >= 0, h["prim"] >= 0)
+
+
+def test_bvh_degenerate_centroids_make_one_leaf():
+    from oracle import OracleScene
+
+    # identical triangles: centroid extent 0 -> a single leaf in input order
+    sc = scenes.SceneData()
+    m = sc.add_material(scenes.diffuse((0.5, 0.5, 0.5)))
+    p = np.tile([[0, 0, 0], [1, 0, 0], [0, 1, 0]], (7, 1)).astype(np.float64)
+    sc.add_mesh(p, np.tile([0, 0, 1.0], (21, 1)), np.tile([1.0, 0, 0], (21, 1)), np.arange(21).reshape(-1, 3), m)
+    nodes, order = OracleScene(sc).bvh()
+    assert len(nodes) == 1 and nodes[0]["count"] == 7
+    assert order.tolist() == list(range(7))
+    # ray through all seven: the last tested one wins the tie
+    rays = scenes.random_rays([0, 0, 0], [1, 1, 1], 1, 0)
+    rays["o"] = [0.25, 0.25, 1.0]
+    rays["d"] = [0.0, 0.0, -1.0]
+    h, _ = OracleScene(sc).trace(rays, threads=1)
+    assert h["prim"][0] == 6 and h["t"][0] == 1.0
+
+
+def test_bvh_two_primitives_sorted_by_centroid():
+    from oracle import OracleScene
+
+    sc = scenes.SceneData()
+    m = sc.add_material(scenes.diffuse((0.5, 0.5, 0.5)))
+    p = np.array([[5, 0, 0], [6, 0, 0], [5, 1, 0], [0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float64)
+    sc.add_mesh(p, np.tile([0, 0, 1.0], (6, 1)), np.tile([1.0, 0, 0], (6, 1)), np.arange(6).reshape(-1, 3), m)
+    nodes, order = OracleScene(sc).bvh()
+    # nth_element on two: the smaller centroid first
+    assert len(nodes) == 3 and order.tolist() == [1, 0] and nodes[0]["axis"] == 0
+
+
+# --------------------------------------------------------------- Fresnel
+
+def test_fresnel_closed_forms(orc):
+    for eta in (1.33, 1.5, 2.4):
+        r0 = ((eta - 1) / (eta + 1)) ** 2
+        assert np.isclose(orc.orc_fresnel_dielectric(1.0, eta), r0, rtol=1e-6)
+        # total internal reflection from the dense side
+        assert orc.orc_fresnel_dielectric(-0.1, eta) == 1.0
+        # grazing incidence reflects everything
+        assert np.isclose(orc.orc_fresnel_dielectric(0.0, eta), 1.0, atol=1e-6)
+        # k = 0 conductor equals the dielectric for eta > 1
+        for c in (0.2, 0.5, 0.9):
+            assert np.isclose(orc.orc_fresnel_complex(c, eta, 0.0), orc.orc_fresnel_dielectric(c, eta), rtol=2e-6)
+    for eta, k in [(0.161, 4.08769), (0.492, 2.32625), (1.426, 1.846)]:
+        expect = ((eta - 1) ** 2 + k * k) / ((eta + 1) ** 2 + k * k)
+        assert np.isclose(orc.orc_fresnel_complex(1.0, eta, k), expect, rtol=2e-6)
+
+
+# ----------------------------------------------------------------- BSDFs
+
+N = farr(0, 0, 1)
+T = farr(1, 0, 0)
+
+
+def _eval(orc, m, wo, wi, front=1):
+    f = farr(0, 0, 0)
+    pdf = F()
+    flags = orc.orc_bsdf_eval(ctypes.byref(m), N, T, front, farr(*wo), farr(*wi), f, ctypes.byref(pdf))
+    return np.array(list(f)), pdf.value, flags
+
+
+def _sample(orc, m, wo, u0, u12, front=1):
+    f, wi, pdf, fl, eta = farr(0, 0, 0), farr(0, 0, 0), F(), ctypes.c_int32(), F()
+    ok = orc.orc_bsdf_sample(ctypes.byref(m), N, T, front, farr(*wo), u0, farr(*u12), f, wi, ctypes.byref(pdf),
+                             ctypes.byref(fl), ctypes.byref(eta))
+    return ok, np.array(list(f)), np.array(list(wi)), pdf.value, fl.value, eta.value
+
+
+def _dir(theta, phi):
+    return (np.sin(theta) * np.cos(phi), np.sin(theta) * np.sin(phi), np.cos(theta))
+
+
+def test_lambertian_closed_form(orc):
+    m = scenes.diffuse((0.8, 0.4, 0.2))
+    wo, wi = _dir(0.3, 0.1), _dir(0.9, 2.0)
+    f, pdf, flags = _eval(orc, m, wo, wi)
+    assert np.allclose(f, np.array([0.8, 0.4, 0.2]) / np.pi, rtol=1e-6)
+    assert np.isclose(pdf, np.cos(0.9) / np.pi, rtol=1e-5)
+    assert flags == 4 | 1
+    f, pdf, _ = _eval(orc, m, wo, (wi[0], wi[1], -wi[2]))
+    assert np.all(f == 0) and pdf == 0
+
+
+@pytest.mark.parametrize("mat,pdf_consistent", [
+    (scenes.conductor(scenes.GOLD_ETA, scenes.GOLD_K, 0.3), True),
+    (scenes.conductor(scenes.GOLD_ETA, scenes.GOLD_K, 0.6), True),
+    (scenes.dielectric(1.5, roughness=0.4), True),
+    # MetallicRoughnessBxDF::Sample_f weighs its lobes with the VNDF-sampled
+    # half vector even when it then samples the diffuse lobe
+    # (metallic_roughness_bxdf.cpp:106-151), while PDF() uses the half vector
+    # of (wo, wi): the two pdfs differ by design, so only f is compared.
+    (scenes.metallic_roughness((0.7, 0.5, 0.3), 0.0, 0.5), False),
+    (scenes.metallic_roughness((0.7, 0.5, 0.3), 1.0, 0.3), False),
+    (scenes.diffuse((0.5, 0.6, 0.7)), True),
+], ids=["gold0.3", "gold0.6", "roughglass", "mr_dielectric", "mr_metal", "lambert"])
+def test_sampled_pdf_and_value_match_evaluation(orc, mat, pdf_consistent):
+    rng = np.random.default_rng(3)
+    n_ok = 0
+    for _ in range(300):
+        wo = _dir(rng.uniform(0.05, 1.4), rng.uniform(0, 2 * np.pi))
+        ok, f, wi, pdf, fl, eta = _sample(orc, mat, wo, rng.random(), rng.random(2))
+        if not ok:
+            continue
+        n_ok += 1
+        f2, pdf2, _ = _eval(orc, mat, wo, wi)
+        assert pdf > 0
+        if pdf_consistent:
+            assert np.isclose(pdf, pdf2, rtol=2e-4, atol=1e-7)
+        assert np.allclose(f, f2, rtol=2e-4, atol=1e-7)
+    assert n_ok > 100
+
+
+@pytest.mark.parametrize("mat", [
+    scenes.conductor(scenes.GOLD_ETA, scenes.GOLD_K, 0.4),
+    scenes.dielectric(1.5, roughness=0.4),
+    scenes.diffuse((0.3, 0.6, 0.9)),
+], ids=["gold", "roughglass", "lambert"])
+def test_bsdf_reciprocity(orc, mat):
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        wo = _dir(rng.uniform(0.05, 1.4), rng.uniform(0, 2 * np.pi))
+        wi = _dir(rng.uniform(0.05, 1.4), rng.uniform(0, 2 * np.pi))
+        f1, _, _ = _eval(orc, mat, wo, wi)
+        f2, _, _ = _eval(orc, mat, wi, wo)
+        assert np.allclose(f1, f2, rtol=1e-4, atol=1e-7)
+
+
+def test_ggx_albedo_near_one_for_perfect_reflector(orc):
+    # R -> 1 conductor (F ~ 1): the directional albedo E[f cos / pdf] of a
+    # smooth-ish GGX lobe approaches 1 and never exceeds it
+    m = capi.BbMaterial()
+    m.type = capi.BB_MAT_CONDUCTOR_R
+    m.k[:] = [1.0, 1.0, 1.0]
+    m.color[:] = [1.0, 1.0, 1.0]
+    rng = np.random.default_rng(11)
+    for rough, lo in ((0.1, 0.99), (0.5, 0.85)):
+        m.u_roughness = m.v_roughness = rough
+        wo = _dir(0.5, 0.2)
+        acc = []
+        for u in rng.random((3000, 3)):
+            ok, f, wi, pdf, _, _ = _sample(orc, m, wo, u[0], u[1:])
+            acc.append(f[0] * abs(wi[2]) / pdf if ok else 0.0)
+        e = float(np.mean(acc))
+        assert lo < e <= 1.0 + 3 * np.std(acc) / np.sqrt(len(acc)), (rough, e)
+
+
+def test_smooth_dielectric_throughput(orc):
+    m = scenes.dielectric(1.5)
+    wo = _dir(0.6, 0.0)
+    R = orc.orc_fresnel_dielectric(np.cos(0.6), 1.5)
+    ok, f, wi, pdf, fl, eta = _sample(orc, m, wo, 0.0, (0.5, 0.5))  # u0 < R: reflection
+    assert ok and fl == (16 | 1) and np.isclose(pdf, R, rtol=1e-6)
+    assert np.isclose(f[0] * abs(wi[2]) / pdf, 1.0, rtol=1e-5)
+    ok, f, wi, pdf, fl, eta = _sample(orc, m, wo, 0.99, (0.5, 0.5))  # transmission
+    assert ok and fl == (16 | 2) and eta == np.float32(1.5)
+    assert np.isclose(f[0] * abs(wi[2]) / pdf, 1.0 / 1.5 ** 2, rtol=1e-5)
+    # Snell's law
+    assert np.isclose(np.sqrt(wi[0] ** 2 + wi[1] ** 2), np.sin(0.6) / 1.5, rtol=1e-5)
+
+
+# ---------------------------------------------------------------- lights
+
+def test_power_light_pmf_closed_form():
+    from oracle import OracleScene
+
+    sc, _ = scenes.icosphere_scene(width=32, height=18, level=2, n_lights=16)
+    o = OracleScene(sc)
+    pmf = o.light_pmf()
+    areas = []
+    for L in sc.lights:
+        v = sc.positions[sc.indices[sc.prims[L.primitive][1]]].astype(np.float64)
+        areas.append(0.5 * np.linalg.norm(np.cross(v[1] - v[0], v[2] - v[0])))
+    power = np.array([L.spectrum[0] for L in sc.lights]) * np.array(areas) * np.pi
+    assert np.isclose(pmf.sum(), 1.0, atol=1e-6)
+    assert np.allclose(pmf, power / power.sum(), rtol=1e-5)
+
+
+# ------------------------------------------------------------ integrator
+
+def test_furnace_convex_lambertian_sphere_in_uniform_light():
+    """A convex Lambertian sphere lit only by a uniform environment L_env has
+    radiance albedo * L_env everywhere; the background is L_env.  Pins NEE,
+    BSDF sampling and their MIS weights together."""
+    from oracle import OracleScene
+
+    sc, cam = scenes.readme_scene(width=96, height=54, with_spot=False)
+    o = OracleScene(sc)
+    s, c, _ = o.render(cam, scenes.render_params(256), threads=os.cpu_count())
+    img = s / c[..., None]
+    assert np.all(c == 256)
+    bg = img[:4].reshape(-1, 3)
+    assert np.allclose(bg, 0.1, rtol=1e-5)  # float32 sum of 256 samples
+    # pixels well inside the sphere's silhouette
+    cy, cx = 54 // 2, 96 // 2
+    inner = img[cy - 6:cy + 6, cx - 6:cx + 6].reshape(-1, 3)
+    mean = inner.mean(axis=0)
+    assert np.allclose(mean, [0.085, 0.02, 0.02], rtol=0.01), mean
+
+
+def test_readme_render_is_deterministic_and_sample_ordered():
+    from oracle import OracleScene
+
+    sc, cam = scenes.readme_scene(width=48, height=27)
+    o = OracleScene(sc)
+    a, ca, _ = o.render(cam, scenes.render_params(4), threads=4)
+    b, cb, _ = o.render(cam, scenes.render_params(4), threads=2)
+    assert np.array_equal(a, b) and np.array_equal(ca, cb)
+    # samples [0, 2) then [2, 4) accumulate to the same sums as [0, 4)
+    film = (np.zeros_like(a), np.zeros_like(ca))
+    o.render(cam, scenes.render_params(2), threads=4, film=film)
+    o.render(cam, scenes.render_params(2, sample_begin=2), threads=4, film=film)
+    assert np.array_equal(film[0], a) and np.array_equal(film[1], ca)
