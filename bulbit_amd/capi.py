@@ -151,8 +151,9 @@ class BbRenderParams(ctypes.Structure):
         ("rr_min_bounces", c_int32),
         ("regularize_bsdf", c_int32),
         ("wavefront_paths", c_int32),
-        ("block_size", c_int32),
+        ("trace_lds_stack", c_int32),
         ("sample_begin", c_int32),
+        ("trace_refill", c_int32),
     ]
 
 

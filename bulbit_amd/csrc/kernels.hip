@@ -45,22 +45,26 @@ BB_D f3 xyz(float4 v) { return mk3(v.x, v.y, v.z); }
 
 // AABB3::TestRay with precomputed inverse direction (bounding_box.h:443-484).
 // a = (min.x, min.y, min.z, max.x), b = (max.y, max.z, offset, meta)
+// The reference returns at the first failing axis; here every axis is
+// evaluated and the three outcomes are combined, which yields the same result
+// (an axis after a failing one cannot turn the result back to true) while
+// letting both 16-byte node loads issue before the first comparison.
 BB_D bool box_test(float4 a, float4 b, f3 o, float t_min, float t_max, f3 inv, bool n0, bool n1, bool n2)
 {
-    float t_min_x = ((n0 ? a.w : a.x) - o.x) * inv.x;
-    float t_max_x = ((n0 ? a.x : a.w) - o.x) * inv.x;
-    if (t_min > t_max_x || t_max < t_min_x) return false;
-    if (t_min_x > t_min) t_min = t_min_x;
-    if (t_max_x < t_max) t_max = t_max_x;
-    float t_min_y = ((n1 ? b.x : a.y) - o.y) * inv.y;
-    float t_max_y = ((n1 ? a.y : b.x) - o.y) * inv.y;
-    if (t_min > t_max_y || t_max < t_min_y) return false;
-    if (t_min_y > t_min) t_min = t_min_y;
-    if (t_max_y < t_max) t_max = t_max_y;
-    float t_min_z = ((n2 ? b.y : a.z) - o.z) * inv.z;
-    float t_max_z = ((n2 ? a.z : b.y) - o.z) * inv.z;
-    if (t_min > t_max_z || t_max < t_min_z) return false;
-    return true;
+    const float t_min_x = ((n0 ? a.w : a.x) - o.x) * inv.x;
+    const float t_max_x = ((n0 ? a.x : a.w) - o.x) * inv.x;
+    const bool px = !(t_min > t_max_x || t_max < t_min_x);
+    const float t_min1 = (t_min_x > t_min) ? t_min_x : t_min;
+    const float t_max1 = (t_max_x < t_max) ? t_max_x : t_max;
+    const float t_min_y = ((n1 ? b.x : a.y) - o.y) * inv.y;
+    const float t_max_y = ((n1 ? a.y : b.x) - o.y) * inv.y;
+    const bool py = !(t_min1 > t_max_y || t_max1 < t_min_y);
+    const float t_min2 = (t_min_y > t_min1) ? t_min_y : t_min1;
+    const float t_max2 = (t_max_y < t_max1) ? t_max_y : t_max1;
+    const float t_min_z = ((n2 ? b.y : a.z) - o.z) * inv.z;
+    const float t_max_z = ((n2 ? a.z : b.y) - o.z) * inv.z;
+    const bool pz = !(t_min2 > t_max_z || t_max2 < t_min_z);
+    return px & py & pz;
 }
 
 // Triangle::Intersect's test (src/geometry/triangle.cpp:55-97) with
@@ -117,73 +121,164 @@ struct TraceCount
     uint32_t nodes, prims, accepts;
 };
 
-// BVH::RayCast with the Intersect / IntersectAny callbacks
-// (src/accel/bvh.cpp:299-361).  Returns false on stack overflow (err set).
-template <bool ANY, bool COUNT>
-BB_D bool traverse(const SceneView& S, f3 o, f3 d, float t_min, float t_max, int32_t* stack, int stride, int depth_cap,
-                   int32_t& hit_slot, float& hit_t, float& hit_u, float& hit_v, TraceCount& c, int* err)
+// Per-lane traversal state of BVH::RayCast (include/bulbit/bvh.h:101-155).
+struct Lane
 {
-    const f3 inv = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-    const bool n0 = inv.x < 0.0f, n1 = inv.y < 0.0f, n2 = inv.z < 0.0f;
-    f3 dn = d;
-    const float l = normalize_m(dn);
-    int sp = 0;
-    int32_t node = 0;
-    for (;;) {
-        const float4 a = S.nodes[2 * node];
-        const float4 b = S.nodes[2 * node + 1];
-        if (COUNT) c.nodes++;
-        if (box_test(a, b, o, t_min, t_max, inv, n0, n1, n2)) {
-            const int32_t offset = __float_as_int(b.z);
-            const uint32_t meta = __float_as_uint(b.w);
-            const uint32_t count = meta & 0xffffu;
-            const uint32_t axis = (meta >> 16) & 0xffu;
-            if (count > 0) {
-                for (uint32_t i = 0; i < count; ++i) {
-                    const int32_t slot = offset + int32_t(i);
-                    const float4 t0 = S.tris[3 * slot];
-                    const float4 t1 = S.tris[3 * slot + 1];
-                    const float4 t2 = S.tris[3 * slot + 2];
-                    if (COUNT) c.prims++;
-                    float t = 0.0f, u = 0.0f, v = 0.0f;
-                    bool h;
-                    if ((axis & 1u) && __float_as_int(t2.w) == KIND_SPHERE)
-                        h = sphere_test(S.spheres[__float_as_int(t1.w)], o, d, t_min, t_max, t);
-                    else
-                        h = mt_test(xyz(t0), xyz(t1), xyz(t2), o, dn, l, t_min, t_max, t, u, v);
-                    if (h) {
-                        if (COUNT) c.accepts++;
-                        if (ANY) {
-                            hit_slot = slot;
-                            return true;
-                        }
-                        hit_slot = slot;
-                        hit_t = t;
-                        hit_u = u;
-                        hit_v = v;
-                        if (t <= t_min) return true;
-                        t_max = t;
-                    }
+    f3 o, d, inv, dn;
+    float l, t_min, t_max;
+    bool n0, n1, n2;
+    int32_t sp, node;
+    int32_t hit_slot;
+    float hit_t, hit_u, hit_v;
+};
+
+BB_D void lane_start(Lane& L, f3 o, f3 d, float t_min, float t_max)
+{
+    L.o = o;
+    L.d = d;
+    L.t_min = t_min;
+    L.t_max = t_max;
+    L.inv = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    L.n0 = L.inv.x < 0.0f;
+    L.n1 = L.inv.y < 0.0f;
+    L.n2 = L.inv.z < 0.0f;
+    L.dn = d;
+    L.l = normalize_m(L.dn);
+    L.sp = 0;
+    L.node = 0;
+    L.hit_slot = -1;
+    L.hit_t = L.hit_u = L.hit_v = 0.0f;
+}
+
+// Traversal stack: entries [0, lds_entries) in LDS ([entry][thread] layout,
+// conflict-free for a wave), deeper entries in a global overflow area
+// ([entry][global thread]); cap = BVH depth + 2 bounds every push.
+struct TraceStack
+{
+    int32_t* lds;
+    int32_t* gbl;
+    int32_t lds_entries;
+    int32_t gstride;
+    int32_t cap;
+};
+
+BB_D void stack_put(const TraceStack& s, int32_t sp, int32_t v)
+{
+    if (sp < s.lds_entries)
+        s.lds[sp * kTraceBlock] = v;
+    else
+        s.gbl[(sp - s.lds_entries) * s.gstride] = v;
+}
+
+BB_D int32_t stack_get(const TraceStack& s, int32_t sp)
+{
+    return sp < s.lds_entries ? s.lds[sp * kTraceBlock] : s.gbl[(sp - s.lds_entries) * s.gstride];
+}
+
+// One step of RayCast for one lane: the node taken is box-tested with the
+// current t_max; a leaf tests its primitives in stored order (the callbacks of
+// BVH::Intersect / IntersectAny, src/accel/bvh.cpp:299-361); an internal node
+// pushes the far child and takes the near child next (pushing and popping it
+// would test it at the same point with the same t_max).  Returns true when the
+// lane's query is finished.
+template <bool ANY, bool COUNT>
+BB_D bool trace_step(const SceneView& S, Lane& L, const TraceStack& stk, TraceCount& c, int* err)
+{
+    const float4 a = S.nodes[2 * L.node];
+    const float4 b = S.nodes[2 * L.node + 1];
+    // both halves of the 32-byte node in one round trip
+    asm volatile("" ::"v"(a.x), "v"(a.y), "v"(a.z), "v"(a.w), "v"(b.x), "v"(b.y), "v"(b.z), "v"(b.w));
+    if (COUNT) c.nodes++;
+    if (box_test(a, b, L.o, L.t_min, L.t_max, L.inv, L.n0, L.n1, L.n2)) {
+        const int32_t offset = __float_as_int(b.z);
+        const uint32_t meta = __float_as_uint(b.w);
+        const uint32_t count = meta & 0xffffu;
+        const uint32_t axis = (meta >> 16) & 0xffu;
+        if (count > 0) {
+            for (uint32_t i = 0; i < count; ++i) {
+                const int32_t slot = offset + int32_t(i);
+                const float4 t0 = S.tris[3 * slot];
+                const float4 t1 = S.tris[3 * slot + 1];
+                const float4 t2 = S.tris[3 * slot + 2];
+                // keep the three 16-byte loads together ahead of the first use
+                asm volatile("" ::"v"(t0.x), "v"(t0.y), "v"(t0.z), "v"(t1.x), "v"(t1.y), "v"(t1.z), "v"(t2.x),
+                             "v"(t2.y), "v"(t2.z), "v"(t2.w));
+                if (COUNT) c.prims++;
+                float t = 0.0f, u = 0.0f, v = 0.0f;
+                // the triangle test runs on every slot so that the three
+                // loads above issue together; a sphere slot (zero edges)
+                // fails it and takes the sphere test instead
+                bool h = mt_test(xyz(t0), xyz(t1), xyz(t2), L.o, L.dn, L.l, L.t_min, L.t_max, t, u, v);
+                if ((axis & 1u) && __float_as_int(t2.w) == KIND_SPHERE) {
+                    u = v = 0.0f;
+                    h = sphere_test(S.spheres[__float_as_int(t1.w)], L.o, L.d, L.t_min, L.t_max, t);
                 }
-            } else {
-                const int32_t child1 = node + 1;
-                const int32_t child2 = offset;
-                const bool neg = axis == 0 ? n0 : (axis == 1 ? n1 : n2);
-                if (sp >= depth_cap) {
-                    atomicOr(err, 1);
-                    return hit_slot >= 0;
+                if (h) {
+                    if (COUNT) c.accepts++;
+                    L.hit_slot = slot;
+                    if (ANY) return true;
+                    L.hit_t = t;
+                    L.hit_u = u;
+                    L.hit_v = v;
+                    if (t <= L.t_min) return true;
+                    L.t_max = t;
                 }
-                stack[sp * stride] = neg ? child1 : child2;
-                ++sp;
-                node = neg ? child2 : child1;
-                continue;
             }
+        } else {
+            const int32_t child1 = L.node + 1;
+            const int32_t child2 = offset;
+            const bool neg = axis == 0 ? L.n0 : (axis == 1 ? L.n1 : L.n2);
+            if (L.sp >= stk.cap) {
+                atomicOr(err, 1);
+                return true;
+            }
+            stack_put(stk, L.sp, neg ? child1 : child2);
+            ++L.sp;
+            L.node = neg ? child2 : child1;
+            return false;
         }
-        if (sp == 0) break;
-        --sp;
-        node = stack[sp * stride];
     }
-    return hit_slot >= 0;
+    if (L.sp == 0) return true;
+    --L.sp;
+    L.node = stack_get(stk, L.sp);
+    return false;
+}
+
+// Persistent traversal: a wave takes consecutive queries from the shared
+// counter `fetch` and hands a new query to its idle lanes once at least
+// `refill_min` of its 64 lanes are idle (64: whole-wave batches, one atomic
+// per 64 queries).  Src provides load(index, Lane&) and finish(index, const Lane&).
+template <bool ANY, bool COUNT, typename Src>
+BB_D void persistent_trace(const SceneView& S, const Src& src, int n, int* fetch, const TraceStack& stk, TraceCount& c,
+                           int* err, int refill_min)
+{
+    const int lane = __lane_id();
+    const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    Lane L;
+    bool active = false, more = true;
+    int id = 0;
+    for (;;) {
+        unsigned long long idle = __ballot(active ? 0 : 1);
+        if (more && (__popcll(idle) >= refill_min || idle == ~0ull)) {
+            const int leader = __builtin_ctzll(idle);
+            int base = 0;
+            if (lane == leader) base = atomicAdd(fetch, __popcll(idle));
+            base = __shfl(base, leader);
+            if (base >= n) more = false;
+            const int mine = base + __popcll(idle & below);
+            if (!active && mine < n) {
+                id = mine;
+                src.load(id, L);
+                active = true;
+            }
+            idle = __ballot(active ? 0 : 1);
+        }
+        if (idle == ~0ull) break;
+        if (active && trace_step<ANY, COUNT>(S, L, stk, c, err)) {
+            src.finish(id, L);
+            active = false;
+        }
+    }
 }
 
 // wave-level compaction: returns this lane's slot (or -1) in the queue whose
@@ -218,39 +313,51 @@ BB_D void wave_count_add(TraceCount c, unsigned long long* out)
 
 // ------------------------------------------------------- batch traversal
 
-template <bool COUNT>
-__global__ __launch_bounds__(kTraceBlock) void k_trace_batch(SceneView S, const float4* __restrict__ rays, int n,
-                                                               int any_hit, int4* __restrict__ hits,
+struct BatchSrc
+{
+    const float4* rays;
+    int4* hits;
+    const float4* tris;
+    int any_hit;
+    BB_D void load(int i, Lane& L) const
+    {
+        const float4 r0 = rays[2 * i], r1 = rays[2 * i + 1];
+        lane_start(L, xyz(r0), xyz(r1), r0.w, r1.w);
+    }
+    BB_D void finish(int i, const Lane& L) const
+    {
+        int4 out;
+        if (any_hit)
+            out = make_int4(L.hit_slot >= 0 ? 1 : -1, 0, 0, 0);
+        else if (L.hit_slot >= 0)
+            out = make_int4(__float_as_int(tris[3 * L.hit_slot].w), __float_as_int(L.hit_t), __float_as_int(L.hit_u),
+                            __float_as_int(L.hit_v));
+        else
+            out = make_int4(-1, 0, 0, 0);
+        hits[i] = out;
+    }
+};
+
+BB_D TraceStack make_stack(int32_t* lds_stack, int32_t* gstack, int lds_entries, int cap)
+{
+    TraceStack t;
+    t.lds = lds_stack + threadIdx.x;
+    t.gstride = gridDim.x * blockDim.x;
+    t.gbl = gstack + blockIdx.x * blockDim.x + threadIdx.x;
+    t.lds_entries = lds_entries;
+    t.cap = cap;
+    return t;
+}
+
+template <bool ANY, bool COUNT>
+__global__ __launch_bounds__(kTraceBlock) void k_trace_batch(SceneView S, BatchSrc src, int n, int* fetch,
+                                                               int32_t* gstack, int lds_entries, int refill_min,
                                                                unsigned long long* counters, int* err)
 {
     extern __shared__ int32_t lds_stack[];
-    int32_t* stack = lds_stack + threadIdx.x;
+    const TraceStack stk = make_stack(lds_stack, gstack, lds_entries, S.stack_depth);
     TraceCount c{ 0, 0, 0 };
-    for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
-        const int i = base + threadIdx.x;
-        if (i < n) {
-            const float4 r0 = rays[2 * i], r1 = rays[2 * i + 1];
-            int32_t slot = -1;
-            float t = 0.0f, u = 0.0f, v = 0.0f;
-            bool h;
-            if (any_hit)
-                h = traverse<true, COUNT>(S, xyz(r0), xyz(r1), r0.w, r1.w, stack, kTraceBlock, S.stack_depth, slot, t, u,
-                                          v, c, err);
-            else
-                h = traverse<false, COUNT>(S, xyz(r0), xyz(r1), r0.w, r1.w, stack, kTraceBlock, S.stack_depth, slot, t,
-                                           u, v, c, err);
-            int4 out;
-            if (any_hit) {
-                out = make_int4(h ? 1 : -1, 0, 0, 0);
-            } else if (h) {
-                out = make_int4(__float_as_int(S.tris[3 * slot].w), __float_as_int(t), __float_as_int(u),
-                                __float_as_int(v));
-            } else {
-                out = make_int4(-1, 0, 0, 0);
-            }
-            hits[i] = out;
-        }
-    }
+    persistent_trace<ANY, COUNT>(S, src, n, fetch, stk, c, err, refill_min);
     if (COUNT) wave_count_add(c, counters);
 }
 
@@ -274,7 +381,8 @@ struct PathBufs
     int4* hit;       // (slot, t, u, v) per queue slot
     float4* sray;    // 2 float4 per shadow slot: (o, t_max), (d, path id)
     float4* scontrib;
-    int* counts;     // [depth][2]: active paths, shadow rays
+    int* counts;     // [depth][4]: active paths, shadow rays, closest fetch, shadow fetch
+    int32_t* gstack; // traversal stack overflow, [entry][persistent thread]
     int* err;
     unsigned long long* counters; // [6]: closest nodes/prims/accepts, shadow nodes/prims/accepts
 };
@@ -324,59 +432,68 @@ __global__ __launch_bounds__(kShadeBlock) void k_raygen(GpuCamera cam, RenderCon
     P.flags[i] = 0;
 }
 
+struct ClosestSrc
+{
+    const float4* rays;
+    int4* hits;
+    BB_D void load(int i, Lane& L) const
+    {
+        const float4 r0 = rays[2 * i], r1 = rays[2 * i + 1];
+        lane_start(L, xyz(r0), xyz(r1), kRayEps, INFINITY);
+    }
+    BB_D void finish(int i, const Lane& L) const
+    {
+        hits[i] = make_int4(L.hit_slot, __float_as_int(L.hit_t), __float_as_int(L.hit_u), __float_as_int(L.hit_v));
+    }
+};
+
+// Shadow rays of SampleDirectLight (path_integrator.cpp:178-193): an
+// unoccluded ray adds its precomputed contribution to the path's L
+// (L += SampleDirectLight(...), path_integrator.cpp:109-112).
+struct ShadowSrc
+{
+    const float4* srays;
+    const float4* contrib;
+    float4* L;
+    BB_D void load(int i, Lane& Ln) const
+    {
+        const float4 r0 = srays[2 * i], r1 = srays[2 * i + 1];
+        lane_start(Ln, xyz(r0), xyz(r1), kRayEps, r0.w);
+    }
+    BB_D void finish(int i, const Lane& Ln) const
+    {
+        if (Ln.hit_slot >= 0) return;
+        const int pid = __float_as_int(srays[2 * i + 1].w);
+        const float4 cv = contrib[i];
+        float4 l = L[pid];
+        l.x = l.x + cv.x;
+        l.y = l.y + cv.y;
+        l.z = l.z + cv.z;
+        L[pid] = l;
+    }
+};
+
 template <bool COUNT>
-__global__ __launch_bounds__(kTraceBlock) void k_trace_closest(SceneView S, const int* __restrict__ n_ptr,
-                                                                 const float4* __restrict__ rays, int4* __restrict__ hits,
+__global__ __launch_bounds__(kTraceBlock) void k_trace_closest(SceneView S, const int* __restrict__ n_ptr, int* fetch,
+                                                                 ClosestSrc src, int32_t* gstack, int lds_entries, int refill_min,
                                                                  unsigned long long* counters, int* err)
 {
     extern __shared__ int32_t lds_stack[];
-    int32_t* stack = lds_stack + threadIdx.x;
-    const int n = *n_ptr;
+    const TraceStack stk = make_stack(lds_stack, gstack, lds_entries, S.stack_depth);
     TraceCount c{ 0, 0, 0 };
-    for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
-        const int i = base + threadIdx.x;
-        if (i < n) {
-            const float4 r0 = rays[2 * i], r1 = rays[2 * i + 1];
-            int32_t slot = -1;
-            float t = 0.0f, u = 0.0f, v = 0.0f;
-            traverse<false, COUNT>(S, xyz(r0), xyz(r1), kRayEps, INFINITY, stack, kTraceBlock, S.stack_depth, slot, t, u,
-                                   v, c, err);
-            hits[i] = make_int4(slot, __float_as_int(t), __float_as_int(u), __float_as_int(v));
-        }
-    }
+    persistent_trace<false, COUNT>(S, src, *n_ptr, fetch, stk, c, err, refill_min);
     if (COUNT) wave_count_add(c, counters);
 }
 
 template <bool COUNT>
-__global__ __launch_bounds__(kTraceBlock) void k_trace_shadow(SceneView S, const int* __restrict__ n_ptr,
-                                                                const float4* __restrict__ srays,
-                                                                const float4* __restrict__ contrib, float4* __restrict__ L,
+__global__ __launch_bounds__(kTraceBlock) void k_trace_shadow(SceneView S, const int* __restrict__ n_ptr, int* fetch,
+                                                                ShadowSrc src, int32_t* gstack, int lds_entries, int refill_min,
                                                                 unsigned long long* counters, int* err)
 {
     extern __shared__ int32_t lds_stack[];
-    int32_t* stack = lds_stack + threadIdx.x;
-    const int n = *n_ptr;
+    const TraceStack stk = make_stack(lds_stack, gstack, lds_entries, S.stack_depth);
     TraceCount c{ 0, 0, 0 };
-    for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
-        const int i = base + threadIdx.x;
-        if (i < n) {
-            const float4 r0 = srays[2 * i], r1 = srays[2 * i + 1];
-            int32_t slot = -1;
-            float t = 0.0f, u = 0.0f, v = 0.0f;
-            bool occluded = traverse<true, COUNT>(S, xyz(r0), xyz(r1), kRayEps, r0.w, stack, kTraceBlock, S.stack_depth,
-                                                  slot, t, u, v, c, err);
-            if (!occluded) {
-                // L += SampleDirectLight(...)  (path_integrator.cpp:109-112)
-                const int pid = __float_as_int(r1.w);
-                const float4 cv = contrib[i];
-                float4 l = L[pid];
-                l.x = l.x + cv.x;
-                l.y = l.y + cv.y;
-                l.z = l.z + cv.z;
-                L[pid] = l;
-            }
-        }
-    }
+    persistent_trace<true, COUNT>(S, src, *n_ptr, fetch, stk, c, err, refill_min);
     if (COUNT) wave_count_add(c, counters + 3);
 }
 
@@ -431,9 +548,9 @@ BB_D SurfaceHit surface_at(const SceneView& S, int32_t slot, float t, float u, f
 __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView S, RenderConsts rc, PathBufs P, int depth)
 {
     const int in = depth & 1, out = in ^ 1;
-    const int n = P.counts[2 * depth];
-    int* next_count = &P.counts[2 * (depth + 1)];
-    int* shadow_count = &P.counts[2 * depth + 1];
+    const int n = P.counts[4 * depth];
+    int* next_count = &P.counts[4 * (depth + 1)];
+    int* shadow_count = &P.counts[4 * depth + 1];
     for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
         const int k = base + threadIdx.x;
         bool cont = false, shadow = false;
@@ -639,6 +756,8 @@ struct Workspace
     int max_depth_slots = 0;
     std::vector<void*> allocs;
     PathBufs P{};
+    size_t gstack_entries = 0;
+    void* gstack = nullptr;
 };
 
 struct SceneExtra
@@ -674,7 +793,29 @@ void free_workspace(Workspace& w)
     for (void* p : w.allocs) hipFree(p);
     w.allocs.clear();
     w.capacity = 0;
+    w.max_depth_slots = 0;
     w.P = PathBufs{};
+    if (w.gstack) hipFree(w.gstack);
+    w.gstack = nullptr;
+    w.gstack_entries = 0;
+}
+
+// global overflow area of the traversal stack: (cap - lds_entries) entries
+// for every thread of the persistent grid
+bool ensure_gstack(Workspace& w, size_t entries)
+{
+    entries = std::max<size_t>(entries, 64);
+    if (w.gstack_entries >= entries) {
+        w.P.gstack = reinterpret_cast<int32_t*>(w.gstack);
+        return true;
+    }
+    if (w.gstack) hipFree(w.gstack);
+    w.gstack = nullptr;
+    w.gstack_entries = 0;
+    if (hipMalloc(&w.gstack, entries * sizeof(int32_t)) != hipSuccess) return false;
+    w.gstack_entries = entries;
+    w.P.gstack = reinterpret_cast<int32_t*>(w.gstack);
+    return true;
 }
 
 bool ensure_workspace(Workspace& w, size_t paths, int depth_slots)
@@ -682,19 +823,24 @@ bool ensure_workspace(Workspace& w, size_t paths, int depth_slots)
     if (w.capacity >= paths && w.max_depth_slots >= depth_slots) return true;
     paths = std::max(paths, w.capacity);
     depth_slots = std::max(depth_slots, w.max_depth_slots);
-    free_workspace(w);
+    for (void* p : w.allocs) hipFree(p);
+    w.allocs.clear();
+    w.capacity = 0;
+    w.max_depth_slots = 0;
     PathBufs& P = w.P;
     bool ok = dalloc(w, &P.L, paths) && dalloc(w, &P.beta, paths) && dalloc(w, &P.rng, paths) &&
               dalloc(w, &P.flags, paths) && dalloc(w, &P.queue[0], paths) && dalloc(w, &P.queue[1], paths) &&
               dalloc(w, &P.ray[0], 2 * paths) && dalloc(w, &P.ray[1], 2 * paths) && dalloc(w, &P.hit, paths) &&
               dalloc(w, &P.sray, 2 * paths) && dalloc(w, &P.scontrib, paths) &&
-              dalloc(w, &P.counts, size_t(2) * (depth_slots + 2)) && dalloc(w, &P.err, 4) && dalloc(w, &P.counters, 8);
+              dalloc(w, &P.counts, size_t(4) * (depth_slots + 2)) && dalloc(w, &P.err, 4) && dalloc(w, &P.counters, 8);
     if (!ok) {
-        free_workspace(w);
+        for (void* p : w.allocs) hipFree(p);
+        w.allocs.clear();
         return false;
     }
     w.capacity = paths;
     w.max_depth_slots = depth_slots;
+    w.P.gstack = reinterpret_cast<int32_t*>(w.gstack);
     return true;
 }
 
@@ -761,7 +907,31 @@ struct EventPool
     }
 };
 
-size_t trace_lds_bytes(const SceneView& V) { return size_t(V.stack_depth) * kTraceBlock * sizeof(int32_t); }
+// LDS entries of the traversal stack per lane: `requested` or, by default,
+// the whole stack up to 32 entries (deeper entries overflow to global memory)
+int lds_entries_for(const SceneView& V, int requested)
+{
+    int e = requested > 0 ? requested : 32;
+    return std::max(1, std::min(e, V.stack_depth));
+}
+
+size_t trace_lds_bytes(int lds_entries) { return size_t(lds_entries) * kTraceBlock * sizeof(int32_t); }
+
+// persistent grid of a traversal kernel: resident blocks per CU x CUs
+int trace_grid(const void* kernel, size_t lds_bytes)
+{
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        hipGetDevice(&dev);
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (cus <= 0) cus = 256;
+    }
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kTraceBlock, lds_bytes) != hipSuccess || per_cu <= 0)
+        per_cu = 4;
+    return per_cu * cus;
+}
 
 } // namespace
 
@@ -774,30 +944,40 @@ int bbgpu_trace(BbScene* scene, const BbRay* rays_dev, int32_t n_rays, int32_t a
         bb::set_error("bbgpu_trace: invalid argument");
         return BBGPU_ERR_INVALID;
     }
-    if (trace_lds_bytes(scene->view) > 65536) {
-        bb::set_error("bbgpu_trace: BVH depth %d exceeds the LDS traversal stack", scene->max_depth);
-        return BBGPU_ERR_UNSUPPORTED;
-    }
     hipStream_t st = (hipStream_t)stream;
     SceneExtra* x = extra_for(scene);
-    if (!ensure_workspace(x->ws, 1, 16)) {
+    const SceneView& V = scene->view;
+    const int lds_entries = lds_entries_for(V, 0);
+    const size_t lds = trace_lds_bytes(lds_entries);
+    const void* kfn = counters ? reinterpret_cast<const void*>(&k_trace_batch<false, true>)
+                               : reinterpret_cast<const void*>(&k_trace_batch<false, false>);
+    const int grid = std::min(trace_grid(kfn, lds), grid_for(std::max(n_rays, 1), kTraceBlock));
+    if (!ensure_workspace(x->ws, 1, 16) ||
+        !ensure_gstack(x->ws, size_t(std::max(0, V.stack_depth - lds_entries)) * size_t(grid) * kTraceBlock)) {
         bb::set_error("bbgpu_trace: workspace allocation failed");
         return BBGPU_ERR_NOMEM;
     }
     PathBufs& P = x->ws.P;
     hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, st, P.err, 1);
+    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, st, P.counts, 4);
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, P.counters, 8);
     if (n_rays > 0) {
-        const int grid = grid_for(n_rays, kTraceBlock);
-        const size_t lds = trace_lds_bytes(scene->view);
-        if (counters)
-            hipLaunchKernelGGL(k_trace_batch<true>, dim3(grid), dim3(kTraceBlock), lds, st, scene->view,
-                               reinterpret_cast<const float4*>(rays_dev), n_rays, any_hit,
-                               reinterpret_cast<int4*>(hits_dev), P.counters, P.err);
-        else
-            hipLaunchKernelGGL(k_trace_batch<false>, dim3(grid), dim3(kTraceBlock), lds, st, scene->view,
-                               reinterpret_cast<const float4*>(rays_dev), n_rays, any_hit,
-                               reinterpret_cast<int4*>(hits_dev), P.counters, P.err);
+        BatchSrc src{ reinterpret_cast<const float4*>(rays_dev), reinterpret_cast<int4*>(hits_dev), V.tris, any_hit };
+        if (any_hit) {
+            if (counters)
+                hipLaunchKernelGGL((k_trace_batch<true, true>), dim3(grid), dim3(kTraceBlock), lds, st, V, src, n_rays,
+                                   P.counts, P.gstack, lds_entries, 48, P.counters, P.err);
+            else
+                hipLaunchKernelGGL((k_trace_batch<true, false>), dim3(grid), dim3(kTraceBlock), lds, st, V, src, n_rays,
+                                   P.counts, P.gstack, lds_entries, 48, P.counters, P.err);
+        } else {
+            if (counters)
+                hipLaunchKernelGGL((k_trace_batch<false, true>), dim3(grid), dim3(kTraceBlock), lds, st, V, src, n_rays,
+                                   P.counts, P.gstack, lds_entries, 48, P.counters, P.err);
+            else
+                hipLaunchKernelGGL((k_trace_batch<false, false>), dim3(grid), dim3(kTraceBlock), lds, st, V, src, n_rays,
+                                   P.counts, P.gstack, lds_entries, 48, P.counters, P.err);
+        }
         HIP_OK(hipGetLastError());
     }
     int err = 0;
@@ -838,10 +1018,6 @@ int bbgpu_render(BbScene* scene, const BbCameraDesc* camera, const BbRenderParam
         bb::set_error("bbgpu_render: primitives without material are not supported");
         return BBGPU_ERR_UNSUPPORTED;
     }
-    if (trace_lds_bytes(scene->view) > 65536) {
-        bb::set_error("bbgpu_render: BVH depth %d exceeds the LDS traversal stack", scene->max_depth);
-        return BBGPU_ERR_UNSUPPORTED;
-    }
     if (p.max_bounces > 1000) {
         bb::set_error("bbgpu_render: max_bounces above 1000");
         return BBGPU_ERR_INVALID;
@@ -866,12 +1042,24 @@ int bbgpu_render(BbScene* scene, const BbCameraDesc* camera, const BbRenderParam
         bb::set_error("bbgpu_render: workspace allocation of %lld paths failed", cap);
         return BBGPU_ERR_NOMEM;
     }
-    PathBufs& P = x->ws.P;
-    const GpuCamera cam = make_camera(*camera);
     const SceneView& V = scene->view;
-    const size_t lds = trace_lds_bytes(V);
     const bool timed = stats != nullptr;
     const bool count = timed && count_traversal;
+    const int lds_entries = lds_entries_for(V, p.trace_lds_stack);
+    const size_t lds = trace_lds_bytes(lds_entries);
+    const int refill_min = p.trace_refill > 0 ? std::min(p.trace_refill, 64) : 48;
+    const void* k_closest = count ? reinterpret_cast<const void*>(&k_trace_closest<true>)
+                                  : reinterpret_cast<const void*>(&k_trace_closest<false>);
+    const void* k_shadow = count ? reinterpret_cast<const void*>(&k_trace_shadow<true>)
+                                 : reinterpret_cast<const void*>(&k_trace_shadow<false>);
+    const int g_trace = std::min(std::max(trace_grid(k_closest, lds), trace_grid(k_shadow, lds)),
+                                 grid_for(cap, kTraceBlock));
+    if (!ensure_gstack(x->ws, size_t(std::max(0, V.stack_depth - lds_entries)) * size_t(g_trace) * kTraceBlock)) {
+        bb::set_error("bbgpu_render: traversal stack allocation failed");
+        return BBGPU_ERR_NOMEM;
+    }
+    PathBufs& P = x->ws.P;
+    const GpuCamera cam = make_camera(*camera);
     if (timed) memset(stats, 0, sizeof(*stats));
 
     EventPool pool;
@@ -887,7 +1075,7 @@ int bbgpu_render(BbScene* scene, const BbCameraDesc* camera, const BbRenderParam
     if (timed) t_begin = mark();
     hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, st, P.err, 1);
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, P.counters, 8);
-    const int nslots = 2 * (p.max_bounces + 2);
+    const int nslots = 4 * (p.max_bounces + 2);
     int64_t paths_total = 0;
     for (long long s0 = 0; s0 < p.spp; s0 += per_pass) {
         const long long k = std::min<long long>(per_pass, p.spp - s0);
@@ -906,28 +1094,33 @@ int bbgpu_render(BbScene* scene, const BbCameraDesc* camera, const BbRenderParam
         hipLaunchKernelGGL(k_raygen, dim3(unsigned((n_paths + kShadeBlock - 1) / kShadeBlock)), dim3(kShadeBlock), 0, st,
                            cam, rc, P);
         if (timed) spans.push_back({ a, mark(), 2 });
-        const int g_trace = grid_for(n_paths, kTraceBlock);
         const int g_shade = grid_for(n_paths, kShadeBlock);
         for (int depth = 0; depth <= p.max_bounces; ++depth) {
             const int in = depth & 1;
             a = timed ? mark() : nullptr;
+            const ClosestSrc csrc{ P.ray[in], P.hit };
             if (count)
                 hipLaunchKernelGGL(k_trace_closest<true>, dim3(g_trace), dim3(kTraceBlock), lds, st, V,
-                                   P.counts + 2 * depth, P.ray[in], P.hit, P.counters, P.err);
+                                   P.counts + 4 * depth, P.counts + 4 * depth + 2, csrc, P.gstack, lds_entries,
+                                   refill_min, P.counters, P.err);
             else
                 hipLaunchKernelGGL(k_trace_closest<false>, dim3(g_trace), dim3(kTraceBlock), lds, st, V,
-                                   P.counts + 2 * depth, P.ray[in], P.hit, P.counters, P.err);
+                                   P.counts + 4 * depth, P.counts + 4 * depth + 2, csrc, P.gstack, lds_entries,
+                                   refill_min, P.counters, P.err);
             hipEvent_t b = timed ? mark() : nullptr;
             if (timed) spans.push_back({ a, b, 0 });
             hipLaunchKernelGGL(k_shade, dim3(g_shade), dim3(kShadeBlock), 0, st, V, rc, P, depth);
             hipEvent_t c2 = timed ? mark() : nullptr;
             if (timed) spans.push_back({ b, c2, 2 });
+            const ShadowSrc ssrc{ P.sray, P.scontrib, P.L };
             if (count)
                 hipLaunchKernelGGL(k_trace_shadow<true>, dim3(g_trace), dim3(kTraceBlock), lds, st, V,
-                                   P.counts + 2 * depth + 1, P.sray, P.scontrib, P.L, P.counters, P.err);
+                                   P.counts + 4 * depth + 1, P.counts + 4 * depth + 3, ssrc, P.gstack, lds_entries,
+                                   refill_min, P.counters, P.err);
             else
                 hipLaunchKernelGGL(k_trace_shadow<false>, dim3(g_trace), dim3(kTraceBlock), lds, st, V,
-                                   P.counts + 2 * depth + 1, P.sray, P.scontrib, P.L, P.counters, P.err);
+                                   P.counts + 4 * depth + 1, P.counts + 4 * depth + 3, ssrc, P.gstack, lds_entries,
+                                   refill_min, P.counters, P.err);
             if (timed) spans.push_back({ c2, mark(), 1 });
         }
         a = timed ? mark() : nullptr;
@@ -941,8 +1134,8 @@ int bbgpu_render(BbScene* scene, const BbCameraDesc* camera, const BbRenderParam
             HIP_OK(hipMemcpyAsync(h.data(), P.counts, sizeof(int) * nslots, hipMemcpyDeviceToHost, st));
             HIP_OK(hipStreamSynchronize(st));
             for (int dd = 0; dd <= p.max_bounces; ++dd) {
-                stats->rays_closest += h[2 * dd];
-                stats->rays_shadow += h[2 * dd + 1];
+                stats->rays_closest += h[4 * dd];
+                stats->rays_shadow += h[4 * dd + 1];
             }
         }
     }

@@ -210,8 +210,9 @@ typedef struct BbRenderParams {
     int32_t rr_min_bounces;
     int32_t regularize_bsdf;  /* only 0 is supported */
     int32_t wavefront_paths;  /* paths in flight per pass (0 = auto) */
-    int32_t block_size;       /* threads per block for the wavefront kernels (0 = auto) */
+    int32_t trace_lds_stack;  /* traversal stack entries per lane kept in LDS (0 = auto) */
     int32_t sample_begin;     /* render samples [sample_begin, sample_begin+spp) */
+    int32_t trace_refill;     /* a traversal wave takes new rays once this many of its lanes are idle (0 = 48) */
 } BbRenderParams;
 
 /* Per-render statistics, filled when a pointer is passed to bbgpu_render.
