@@ -45,26 +45,30 @@ BB_D f3 xyz(float4 v) { return mk3(v.x, v.y, v.z); }
 
 // AABB3::TestRay with precomputed inverse direction (bounding_box.h:443-484).
 // a = (min.x, min.y, min.z, max.x), b = (max.y, max.z, offset, meta)
-// The reference returns at the first failing axis; here every axis is
-// evaluated and the three outcomes are combined, which yields the same result
-// (an axis after a failing one cannot turn the result back to true) while
-// letting both 16-byte node loads issue before the first comparison.
+//
+// The reference tests x, then y against the x-tightened interval, then z
+// against the x,y-tightened interval, and ignores any NaN slab distance
+// (every comparison with NaN is false).  Unrolled, it fails exactly when one
+// of these pairs is out of order:
+//   t_min  > t_max_i  (i = x, y, z)      t_min_i > t_max          (i = x, y, z)
+//   t_min_i > t_max_j (i != j)
+// (t_min is never compared with t_max, and t_min_i <= t_max_i always holds
+// for one axis because both are computed with the same sign of inv.)  With
+// NaN-ignoring max/min over the three axes that is
+//   fail = t_min > min_i t_max_i  or  max_i t_min_i > t_max
+//          or  max_i t_min_i > min_i t_max_i,
+// the same predicate with three comparisons and no sequential tightening.
 BB_D bool box_test(float4 a, float4 b, f3 o, float t_min, float t_max, f3 inv, bool n0, bool n1, bool n2)
 {
-    const float t_min_x = ((n0 ? a.w : a.x) - o.x) * inv.x;
-    const float t_max_x = ((n0 ? a.x : a.w) - o.x) * inv.x;
-    const bool px = !(t_min > t_max_x || t_max < t_min_x);
-    const float t_min1 = (t_min_x > t_min) ? t_min_x : t_min;
-    const float t_max1 = (t_max_x < t_max) ? t_max_x : t_max;
-    const float t_min_y = ((n1 ? b.x : a.y) - o.y) * inv.y;
-    const float t_max_y = ((n1 ? a.y : b.x) - o.y) * inv.y;
-    const bool py = !(t_min1 > t_max_y || t_max1 < t_min_y);
-    const float t_min2 = (t_min_y > t_min1) ? t_min_y : t_min1;
-    const float t_max2 = (t_max_y < t_max1) ? t_max_y : t_max1;
-    const float t_min_z = ((n2 ? b.y : a.z) - o.z) * inv.z;
-    const float t_max_z = ((n2 ? a.z : b.y) - o.z) * inv.z;
-    const bool pz = !(t_min2 > t_max_z || t_max2 < t_min_z);
-    return px & py & pz;
+    const float lx = ((n0 ? a.w : a.x) - o.x) * inv.x;
+    const float hx = ((n0 ? a.x : a.w) - o.x) * inv.x;
+    const float ly = ((n1 ? b.x : a.y) - o.y) * inv.y;
+    const float hy = ((n1 ? a.y : b.x) - o.y) * inv.y;
+    const float lz = ((n2 ? b.y : a.z) - o.z) * inv.z;
+    const float hz = ((n2 ? a.z : b.y) - o.z) * inv.z;
+    const float lo = fmaxf(fmaxf(lx, ly), lz);
+    const float hi = fminf(fminf(hx, hy), hz);
+    return !((t_min > hi) | (lo > t_max) | (lo > hi));
 }
 
 // Triangle::Intersect's test (src/geometry/triangle.cpp:55-97) with
@@ -162,17 +166,20 @@ struct TraceStack
     int32_t cap;
 };
 
+// FULL: the whole stack fits in LDS (cap <= lds_entries), no overflow branch
+template <bool FULL>
 BB_D void stack_put(const TraceStack& s, int32_t sp, int32_t v)
 {
-    if (sp < s.lds_entries)
+    if (FULL || sp < s.lds_entries)
         s.lds[sp * kTraceBlock] = v;
     else
         s.gbl[(sp - s.lds_entries) * s.gstride] = v;
 }
 
+template <bool FULL>
 BB_D int32_t stack_get(const TraceStack& s, int32_t sp)
 {
-    return sp < s.lds_entries ? s.lds[sp * kTraceBlock] : s.gbl[(sp - s.lds_entries) * s.gstride];
+    return (FULL || sp < s.lds_entries) ? s.lds[sp * kTraceBlock] : s.gbl[(sp - s.lds_entries) * s.gstride];
 }
 
 // One step of RayCast for one lane: the node taken is box-tested with the
@@ -181,7 +188,7 @@ BB_D int32_t stack_get(const TraceStack& s, int32_t sp)
 // pushes the far child and takes the near child next (pushing and popping it
 // would test it at the same point with the same t_max).  Returns true when the
 // lane's query is finished.
-template <bool ANY, bool COUNT>
+template <bool ANY, bool COUNT, bool FULL>
 BB_D bool trace_step(const SceneView& S, Lane& L, const TraceStack& stk, TraceCount& c, int* err)
 {
     const float4 a = S.nodes[2 * L.node];
@@ -189,66 +196,67 @@ BB_D bool trace_step(const SceneView& S, Lane& L, const TraceStack& stk, TraceCo
     // both halves of the 32-byte node in one round trip
     asm volatile("" ::"v"(a.x), "v"(a.y), "v"(a.z), "v"(a.w), "v"(b.x), "v"(b.y), "v"(b.z), "v"(b.w));
     if (COUNT) c.nodes++;
-    if (box_test(a, b, L.o, L.t_min, L.t_max, L.inv, L.n0, L.n1, L.n2)) {
-        const int32_t offset = __float_as_int(b.z);
-        const uint32_t meta = __float_as_uint(b.w);
-        const uint32_t count = meta & 0xffffu;
-        const uint32_t axis = (meta >> 16) & 0xffu;
-        if (count > 0) {
-            for (uint32_t i = 0; i < count; ++i) {
-                const int32_t slot = offset + int32_t(i);
-                const float4 t0 = S.tris[3 * slot];
-                const float4 t1 = S.tris[3 * slot + 1];
-                const float4 t2 = S.tris[3 * slot + 2];
-                // keep the three 16-byte loads together ahead of the first use
-                asm volatile("" ::"v"(t0.x), "v"(t0.y), "v"(t0.z), "v"(t1.x), "v"(t1.y), "v"(t1.z), "v"(t2.x),
-                             "v"(t2.y), "v"(t2.z), "v"(t2.w));
-                if (COUNT) c.prims++;
-                float t = 0.0f, u = 0.0f, v = 0.0f;
-                // the triangle test runs on every slot so that the three
-                // loads above issue together; a sphere slot (zero edges)
-                // fails it and takes the sphere test instead
-                bool h = mt_test(xyz(t0), xyz(t1), xyz(t2), L.o, L.dn, L.l, L.t_min, L.t_max, t, u, v);
-                if ((axis & 1u) && __float_as_int(t2.w) == KIND_SPHERE) {
-                    u = v = 0.0f;
-                    h = sphere_test(S.spheres[__float_as_int(t1.w)], L.o, L.d, L.t_min, L.t_max, t);
-                }
-                if (h) {
-                    if (COUNT) c.accepts++;
-                    L.hit_slot = slot;
-                    if (ANY) return true;
-                    L.hit_t = t;
-                    L.hit_u = u;
-                    L.hit_v = v;
-                    if (t <= L.t_min) return true;
-                    L.t_max = t;
-                }
+    const bool pass = box_test(a, b, L.o, L.t_min, L.t_max, L.inv, L.n0, L.n1, L.n2);
+    const int32_t offset = __float_as_int(b.z);
+    const uint32_t meta = __float_as_uint(b.w);
+    const uint32_t count = meta & 0xffffu;
+    const uint32_t axis = (meta >> 16) & 0xffu;
+    if (pass && count == 0) {
+        const int32_t child1 = L.node + 1;
+        const int32_t child2 = offset;
+        const bool neg = axis == 0 ? L.n0 : (axis == 1 ? L.n1 : L.n2);
+        if (L.sp >= stk.cap) {
+            atomicOr(err, 1);
+            return true;
+        }
+        stack_put<FULL>(stk, L.sp, neg ? child1 : child2);
+        ++L.sp;
+        L.node = neg ? child2 : child1;
+        return false;
+    }
+    if (pass) {
+        for (uint32_t i = 0; i < count; ++i) {
+            const int32_t slot = offset + int32_t(i);
+            const float4 t0 = S.tris[3 * slot];
+            const float4 t1 = S.tris[3 * slot + 1];
+            const float4 t2 = S.tris[3 * slot + 2];
+            // keep the three 16-byte loads together ahead of the first use
+            asm volatile("" ::"v"(t0.x), "v"(t0.y), "v"(t0.z), "v"(t1.x), "v"(t1.y), "v"(t1.z), "v"(t2.x), "v"(t2.y),
+                         "v"(t2.z), "v"(t2.w));
+            if (COUNT) c.prims++;
+            float t = 0.0f, u = 0.0f, v = 0.0f;
+            // the triangle test runs on every slot so that the three loads
+            // above issue together; a sphere slot (zero edges) fails it and
+            // takes the sphere test instead
+            bool h = mt_test(xyz(t0), xyz(t1), xyz(t2), L.o, L.dn, L.l, L.t_min, L.t_max, t, u, v);
+            if ((axis & 1u) && __float_as_int(t2.w) == KIND_SPHERE) {
+                u = v = 0.0f;
+                h = sphere_test(S.spheres[__float_as_int(t1.w)], L.o, L.d, L.t_min, L.t_max, t);
             }
-        } else {
-            const int32_t child1 = L.node + 1;
-            const int32_t child2 = offset;
-            const bool neg = axis == 0 ? L.n0 : (axis == 1 ? L.n1 : L.n2);
-            if (L.sp >= stk.cap) {
-                atomicOr(err, 1);
-                return true;
+            if (h) {
+                if (COUNT) c.accepts++;
+                L.hit_slot = slot;
+                if (ANY) return true;
+                L.hit_t = t;
+                L.hit_u = u;
+                L.hit_v = v;
+                if (t <= L.t_min) return true;
+                L.t_max = t;
             }
-            stack_put(stk, L.sp, neg ? child1 : child2);
-            ++L.sp;
-            L.node = neg ? child2 : child1;
-            return false;
         }
     }
     if (L.sp == 0) return true;
     --L.sp;
-    L.node = stack_get(stk, L.sp);
+    L.node = stack_get<FULL>(stk, L.sp);
     return false;
 }
 
 // Persistent traversal: a wave takes consecutive queries from the shared
 // counter `fetch` and hands a new query to its idle lanes once at least
 // `refill_min` of its 64 lanes are idle (64: whole-wave batches, one atomic
-// per 64 queries).  Src provides load(index, Lane&) and finish(index, const Lane&).
-template <bool ANY, bool COUNT, typename Src>
+// per 64 queries).  Between refill checks every lane takes STEPS traversal
+// steps.  Src provides load(index, Lane&) and finish(index, const Lane&).
+template <bool ANY, bool COUNT, bool FULL, int STEPS, typename Src>
 BB_D void persistent_trace(const SceneView& S, const Src& src, int n, int* fetch, const TraceStack& stk, TraceCount& c,
                            int* err, int refill_min)
 {
@@ -274,9 +282,12 @@ BB_D void persistent_trace(const SceneView& S, const Src& src, int n, int* fetch
             idle = __ballot(active ? 0 : 1);
         }
         if (idle == ~0ull) break;
-        if (active && trace_step<ANY, COUNT>(S, L, stk, c, err)) {
-            src.finish(id, L);
-            active = false;
+#pragma unroll
+        for (int k = 0; k < STEPS; ++k) {
+            if (active && trace_step<ANY, COUNT, FULL>(S, L, stk, c, err)) {
+                src.finish(id, L);
+                active = false;
+            }
         }
     }
 }
@@ -349,7 +360,12 @@ BB_D TraceStack make_stack(int32_t* lds_stack, int32_t* gstack, int lds_entries,
     return t;
 }
 
-template <bool ANY, bool COUNT>
+#ifndef BB_TRACE_STEPS
+#define BB_TRACE_STEPS 2
+#endif
+constexpr int kTraceSteps = BB_TRACE_STEPS;
+
+template <bool ANY, bool COUNT, bool FULL>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_batch(SceneView S, BatchSrc src, int n, int* fetch,
                                                                int32_t* gstack, int lds_entries, int refill_min,
                                                                unsigned long long* counters, int* err)
@@ -357,7 +373,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_batch(SceneView S, BatchS
     extern __shared__ int32_t lds_stack[];
     const TraceStack stk = make_stack(lds_stack, gstack, lds_entries, S.stack_depth);
     TraceCount c{ 0, 0, 0 };
-    persistent_trace<ANY, COUNT>(S, src, n, fetch, stk, c, err, refill_min);
+    persistent_trace<ANY, COUNT, FULL, kTraceSteps>(S, src, n, fetch, stk, c, err, refill_min);
     if (COUNT) wave_count_add(c, counters);
 }
 
@@ -473,7 +489,7 @@ struct ShadowSrc
     }
 };
 
-template <bool COUNT>
+template <bool COUNT, bool FULL>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(SceneView S, const int* __restrict__ n_ptr, int* fetch,
                                                                  ClosestSrc src, int32_t* gstack, int lds_entries, int refill_min,
                                                                  unsigned long long* counters, int* err)
@@ -481,11 +497,11 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(SceneView S, cons
     extern __shared__ int32_t lds_stack[];
     const TraceStack stk = make_stack(lds_stack, gstack, lds_entries, S.stack_depth);
     TraceCount c{ 0, 0, 0 };
-    persistent_trace<false, COUNT>(S, src, *n_ptr, fetch, stk, c, err, refill_min);
+    persistent_trace<false, COUNT, FULL, kTraceSteps>(S, src, *n_ptr, fetch, stk, c, err, refill_min);
     if (COUNT) wave_count_add(c, counters);
 }
 
-template <bool COUNT>
+template <bool COUNT, bool FULL>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_shadow(SceneView S, const int* __restrict__ n_ptr, int* fetch,
                                                                 ShadowSrc src, int32_t* gstack, int lds_entries, int refill_min,
                                                                 unsigned long long* counters, int* err)
@@ -493,7 +509,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_shadow(SceneView S, const
     extern __shared__ int32_t lds_stack[];
     const TraceStack stk = make_stack(lds_stack, gstack, lds_entries, S.stack_depth);
     TraceCount c{ 0, 0, 0 };
-    persistent_trace<true, COUNT>(S, src, *n_ptr, fetch, stk, c, err, refill_min);
+    persistent_trace<true, COUNT, FULL, kTraceSteps>(S, src, *n_ptr, fetch, stk, c, err, refill_min);
     if (COUNT) wave_count_add(c, counters + 3);
 }
 
@@ -933,6 +949,34 @@ int trace_grid(const void* kernel, size_t lds_bytes)
     return per_cu * cus;
 }
 
+using BatchKernel = void (*)(SceneView, BatchSrc, int, int*, int32_t*, int, int, unsigned long long*, int*);
+using ClosestKernel = void (*)(SceneView, const int*, int*, ClosestSrc, int32_t*, int, int, unsigned long long*, int*);
+using ShadowKernel = void (*)(SceneView, const int*, int*, ShadowSrc, int32_t*, int, int, unsigned long long*, int*);
+
+template <bool ANY>
+BatchKernel batch_kernel_any(bool count, bool full)
+{
+    if (count) return full ? k_trace_batch<ANY, true, true> : k_trace_batch<ANY, true, false>;
+    return full ? k_trace_batch<ANY, false, true> : k_trace_batch<ANY, false, false>;
+}
+
+BatchKernel batch_kernel(bool any, bool count, bool full)
+{
+    return any ? batch_kernel_any<true>(count, full) : batch_kernel_any<false>(count, full);
+}
+
+ClosestKernel closest_kernel(bool count, bool full)
+{
+    if (count) return full ? k_trace_closest<true, true> : k_trace_closest<true, false>;
+    return full ? k_trace_closest<false, true> : k_trace_closest<false, false>;
+}
+
+ShadowKernel shadow_kernel(bool count, bool full)
+{
+    if (count) return full ? k_trace_shadow<true, true> : k_trace_shadow<true, false>;
+    return full ? k_trace_shadow<false, true> : k_trace_shadow<false, false>;
+}
+
 } // namespace
 
 extern "C" {
@@ -949,9 +993,9 @@ int bbgpu_trace(BbScene* scene, const BbRay* rays_dev, int32_t n_rays, int32_t a
     const SceneView& V = scene->view;
     const int lds_entries = lds_entries_for(V, 0);
     const size_t lds = trace_lds_bytes(lds_entries);
-    const void* kfn = counters ? reinterpret_cast<const void*>(&k_trace_batch<false, true>)
-                               : reinterpret_cast<const void*>(&k_trace_batch<false, false>);
-    const int grid = std::min(trace_grid(kfn, lds), grid_for(std::max(n_rays, 1), kTraceBlock));
+    const bool full = V.stack_depth <= lds_entries;
+    const BatchKernel kfn = batch_kernel(any_hit != 0, counters != nullptr, full);
+    const int grid = std::min(trace_grid(reinterpret_cast<const void*>(kfn), lds), grid_for(std::max(n_rays, 1), kTraceBlock));
     if (!ensure_workspace(x->ws, 1, 16) ||
         !ensure_gstack(x->ws, size_t(std::max(0, V.stack_depth - lds_entries)) * size_t(grid) * kTraceBlock)) {
         bb::set_error("bbgpu_trace: workspace allocation failed");
@@ -963,21 +1007,8 @@ int bbgpu_trace(BbScene* scene, const BbRay* rays_dev, int32_t n_rays, int32_t a
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, P.counters, 8);
     if (n_rays > 0) {
         BatchSrc src{ reinterpret_cast<const float4*>(rays_dev), reinterpret_cast<int4*>(hits_dev), V.tris, any_hit };
-        if (any_hit) {
-            if (counters)
-                hipLaunchKernelGGL((k_trace_batch<true, true>), dim3(grid), dim3(kTraceBlock), lds, st, V, src, n_rays,
-                                   P.counts, P.gstack, lds_entries, 48, P.counters, P.err);
-            else
-                hipLaunchKernelGGL((k_trace_batch<true, false>), dim3(grid), dim3(kTraceBlock), lds, st, V, src, n_rays,
-                                   P.counts, P.gstack, lds_entries, 48, P.counters, P.err);
-        } else {
-            if (counters)
-                hipLaunchKernelGGL((k_trace_batch<false, true>), dim3(grid), dim3(kTraceBlock), lds, st, V, src, n_rays,
-                                   P.counts, P.gstack, lds_entries, 48, P.counters, P.err);
-            else
-                hipLaunchKernelGGL((k_trace_batch<false, false>), dim3(grid), dim3(kTraceBlock), lds, st, V, src, n_rays,
-                                   P.counts, P.gstack, lds_entries, 48, P.counters, P.err);
-        }
+        hipLaunchKernelGGL(kfn, dim3(grid), dim3(kTraceBlock), lds, st, V, src, n_rays, P.counts, P.gstack, lds_entries,
+                           48, P.counters, P.err);
         HIP_OK(hipGetLastError());
     }
     int err = 0;
@@ -1048,11 +1079,11 @@ int bbgpu_render(BbScene* scene, const BbCameraDesc* camera, const BbRenderParam
     const int lds_entries = lds_entries_for(V, p.trace_lds_stack);
     const size_t lds = trace_lds_bytes(lds_entries);
     const int refill_min = p.trace_refill > 0 ? std::min(p.trace_refill, 64) : 48;
-    const void* k_closest = count ? reinterpret_cast<const void*>(&k_trace_closest<true>)
-                                  : reinterpret_cast<const void*>(&k_trace_closest<false>);
-    const void* k_shadow = count ? reinterpret_cast<const void*>(&k_trace_shadow<true>)
-                                 : reinterpret_cast<const void*>(&k_trace_shadow<false>);
-    const int g_trace = std::min(std::max(trace_grid(k_closest, lds), trace_grid(k_shadow, lds)),
+    const bool full = V.stack_depth <= lds_entries;
+    const ClosestKernel k_closest = closest_kernel(count, full);
+    const ShadowKernel k_shadow = shadow_kernel(count, full);
+    const int g_trace = std::min(std::max(trace_grid(reinterpret_cast<const void*>(k_closest), lds),
+                                          trace_grid(reinterpret_cast<const void*>(k_shadow), lds)),
                                  grid_for(cap, kTraceBlock));
     if (!ensure_gstack(x->ws, size_t(std::max(0, V.stack_depth - lds_entries)) * size_t(g_trace) * kTraceBlock)) {
         bb::set_error("bbgpu_render: traversal stack allocation failed");
@@ -1099,28 +1130,16 @@ int bbgpu_render(BbScene* scene, const BbCameraDesc* camera, const BbRenderParam
             const int in = depth & 1;
             a = timed ? mark() : nullptr;
             const ClosestSrc csrc{ P.ray[in], P.hit };
-            if (count)
-                hipLaunchKernelGGL(k_trace_closest<true>, dim3(g_trace), dim3(kTraceBlock), lds, st, V,
-                                   P.counts + 4 * depth, P.counts + 4 * depth + 2, csrc, P.gstack, lds_entries,
-                                   refill_min, P.counters, P.err);
-            else
-                hipLaunchKernelGGL(k_trace_closest<false>, dim3(g_trace), dim3(kTraceBlock), lds, st, V,
-                                   P.counts + 4 * depth, P.counts + 4 * depth + 2, csrc, P.gstack, lds_entries,
-                                   refill_min, P.counters, P.err);
+            hipLaunchKernelGGL(k_closest, dim3(g_trace), dim3(kTraceBlock), lds, st, V, P.counts + 4 * depth,
+                               P.counts + 4 * depth + 2, csrc, P.gstack, lds_entries, refill_min, P.counters, P.err);
             hipEvent_t b = timed ? mark() : nullptr;
             if (timed) spans.push_back({ a, b, 0 });
             hipLaunchKernelGGL(k_shade, dim3(g_shade), dim3(kShadeBlock), 0, st, V, rc, P, depth);
             hipEvent_t c2 = timed ? mark() : nullptr;
             if (timed) spans.push_back({ b, c2, 2 });
             const ShadowSrc ssrc{ P.sray, P.scontrib, P.L };
-            if (count)
-                hipLaunchKernelGGL(k_trace_shadow<true>, dim3(g_trace), dim3(kTraceBlock), lds, st, V,
-                                   P.counts + 4 * depth + 1, P.counts + 4 * depth + 3, ssrc, P.gstack, lds_entries,
-                                   refill_min, P.counters, P.err);
-            else
-                hipLaunchKernelGGL(k_trace_shadow<false>, dim3(g_trace), dim3(kTraceBlock), lds, st, V,
-                                   P.counts + 4 * depth + 1, P.counts + 4 * depth + 3, ssrc, P.gstack, lds_entries,
-                                   refill_min, P.counters, P.err);
+            hipLaunchKernelGGL(k_shadow, dim3(g_trace), dim3(kTraceBlock), lds, st, V, P.counts + 4 * depth + 1,
+                               P.counts + 4 * depth + 3, ssrc, P.gstack, lds_entries, refill_min, P.counters, P.err);
             if (timed) spans.push_back({ c2, mark(), 1 });
         }
         a = timed ? mark() : nullptr;
