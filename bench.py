@@ -48,6 +48,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU work of the baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--wavefront-paths", type=int, default=0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 3 device-to-device copies of MIB MiB (scales the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -63,6 +65,9 @@ def main():
 
     if not torch.cuda.is_available():
         sys.exit("no HIP device visible")
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
+        return
 
     log(f"[bench] generating scene level {args.level} at {args.width}x{args.height}")
     t0 = time.time()
@@ -155,6 +160,17 @@ def main():
         out["cpu_baseline"] = cpu_baseline(sc, cam, params, args, samples_per_step)
         out["speedup_vs_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
     print(json.dumps(out), flush=True)
+
+
+def copy_calibration(mib):
+    """A plain copy of known size: what the HBM byte counters read for it."""
+    import torch
+    a = torch.ones(mib * 1024 * 1024 // 4, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"[bench] copied 3 x {mib} MiB")
 
 
 def cpu_baseline(sc, cam, params, args, samples_full):

@@ -49,18 +49,44 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose: bool = False) -> str:
-    """Compiles every source into libbbgpu.so (always recompiles)."""
-    tmp = LIB + ".tmp"
-    cmd = [hipcc(), *FLAGS, "-I", CSRC, "-o", tmp] + [os.path.join(CSRC, s) for s in SOURCES]
+CPP = os.path.join(HERE, "cpp")
+CPP_LIB = os.path.join(HERE, "libbulbit.so")
+CPP_EXAMPLE = os.path.join(CPP, "_build", "readme_example")
+# host C++ API (bulbit/bulbit.h) over the C ABI: plain g++, same FP rules
+CXX_FLAGS = ["-std=c++17", "-O2", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-pthread"]
+
+
+def _run(cmd, what, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building libbbgpu.so")
+        raise RuntimeError(f"{what} failed")
+
+
+def build(verbose: bool = False) -> str:
+    """Compiles libbbgpu.so, then the C++ API libbulbit.so and its README
+    example (always recompiles)."""
+    tmp = LIB + ".tmp"
+    _run([hipcc(), *FLAGS, "-I", CSRC, "-o", tmp] + [os.path.join(CSRC, s) for s in SOURCES],
+         "hipcc building libbbgpu.so", verbose)
     os.replace(tmp, LIB)
+    build_cpp(verbose)
     return LIB
+
+
+def build_cpp(verbose: bool = False) -> str:
+    cxx = os.environ.get("CXX", "g++")
+    inc = os.path.join(CPP, "include")
+    tmp = CPP_LIB + ".tmp"
+    _run([cxx, *CXX_FLAGS, "-fPIC", "-shared", "-I", inc, os.path.join(CPP, "bulbit.cpp"), "-o", tmp,
+          "-L", HERE, "-lbbgpu", "-Wl,-rpath,$ORIGIN"], "g++ building libbulbit.so", verbose)
+    os.replace(tmp, CPP_LIB)
+    os.makedirs(os.path.dirname(CPP_EXAMPLE), exist_ok=True)
+    _run([cxx, *CXX_FLAGS, "-I", inc, os.path.join(CPP, "examples", "readme.cpp"), "-o", CPP_EXAMPLE,
+          "-L", HERE, "-lbulbit", "-lbbgpu", "-Wl,-rpath,$ORIGIN/../.."], "g++ building the README example", verbose)
+    return CPP_LIB
 
 
 if __name__ == "__main__":

@@ -774,6 +774,8 @@ struct Workspace
     PathBufs P{};
     size_t gstack_entries = 0;
     void* gstack = nullptr;
+    size_t stage_bytes = 0; // device copy of caller buffers that live in host memory
+    void* stage = nullptr;
 };
 
 struct SceneExtra
@@ -814,6 +816,32 @@ void free_workspace(Workspace& w)
     if (w.gstack) hipFree(w.gstack);
     w.gstack = nullptr;
     w.gstack_entries = 0;
+    if (w.stage) hipFree(w.stage);
+    w.stage = nullptr;
+    w.stage_bytes = 0;
+}
+
+bool ensure_stage(Workspace& w, size_t bytes)
+{
+    if (w.stage_bytes >= bytes) return true;
+    if (w.stage) hipFree(w.stage);
+    w.stage = nullptr;
+    w.stage_bytes = 0;
+    if (hipMalloc(&w.stage, bytes) != hipSuccess) return false;
+    w.stage_bytes = bytes;
+    return true;
+}
+
+// true for device and managed memory, which kernels read directly; buffers in
+// host memory (pageable or pinned) are staged through device memory instead
+bool device_accessible(const void* p)
+{
+    hipPointerAttribute_t a{};
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError(); // unregistered host memory on some runtimes
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged || a.isManaged;
 }
 
 // global overflow area of the traversal stack: (cap - lds_entries) entries
@@ -1002,15 +1030,34 @@ int bbgpu_trace(BbScene* scene, const BbRay* rays_dev, int32_t n_rays, int32_t a
         return BBGPU_ERR_NOMEM;
     }
     PathBufs& P = x->ws.P;
+    const BbRay* rays = rays_dev;
+    BbHit* hits = hits_dev;
+    const bool host_rays = n_rays > 0 && !device_accessible(rays_dev);
+    const bool host_hits = n_rays > 0 && !device_accessible(hits_dev);
+    if (host_rays || host_hits) {
+        const size_t rb = host_rays ? sizeof(BbRay) * size_t(n_rays) : 0;
+        const size_t hb = host_hits ? sizeof(BbHit) * size_t(n_rays) : 0;
+        if (!ensure_stage(x->ws, rb + hb)) {
+            bb::set_error("bbgpu_trace: staging allocation failed");
+            return BBGPU_ERR_NOMEM;
+        }
+        char* stage = reinterpret_cast<char*>(x->ws.stage);
+        if (host_rays) {
+            rays = reinterpret_cast<const BbRay*>(stage);
+            HIP_OK(hipMemcpyAsync(stage, rays_dev, rb, hipMemcpyHostToDevice, st));
+        }
+        if (host_hits) hits = reinterpret_cast<BbHit*>(stage + rb);
+    }
     hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, st, P.err, 1);
     hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, st, P.counts, 4);
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, P.counters, 8);
     if (n_rays > 0) {
-        BatchSrc src{ reinterpret_cast<const float4*>(rays_dev), reinterpret_cast<int4*>(hits_dev), V.tris, any_hit };
+        BatchSrc src{ reinterpret_cast<const float4*>(rays), reinterpret_cast<int4*>(hits), V.tris, any_hit };
         hipLaunchKernelGGL(kfn, dim3(grid), dim3(kTraceBlock), lds, st, V, src, n_rays, P.counts, P.gstack, lds_entries,
                            48, P.counters, P.err);
         HIP_OK(hipGetLastError());
     }
+    if (host_hits) HIP_OK(hipMemcpyAsync(hits_dev, hits, sizeof(BbHit) * size_t(n_rays), hipMemcpyDeviceToHost, st));
     int err = 0;
     unsigned long long c[8] = { 0 };
     HIP_OK(hipMemcpyAsync(&err, P.err, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1091,6 +1138,21 @@ int bbgpu_render(BbScene* scene, const BbCameraDesc* camera, const BbRenderParam
     }
     PathBufs& P = x->ws.P;
     const GpuCamera cam = make_camera(*camera);
+    // a film in host memory is copied in, accumulated on the device and copied back
+    float* const film_sum_user = film_sum_dev;
+    int32_t* const film_count_user = film_count_dev;
+    const bool host_film = !device_accessible(film_sum_dev) || !device_accessible(film_count_dev);
+    const size_t sum_bytes = sizeof(float) * 3 * size_t(npix), count_bytes = sizeof(int32_t) * size_t(npix);
+    if (host_film) {
+        if (!ensure_stage(x->ws, sum_bytes + count_bytes)) {
+            bb::set_error("bbgpu_render: staging allocation failed");
+            return BBGPU_ERR_NOMEM;
+        }
+        film_sum_dev = reinterpret_cast<float*>(x->ws.stage);
+        film_count_dev = reinterpret_cast<int32_t*>(reinterpret_cast<char*>(x->ws.stage) + sum_bytes);
+        HIP_OK(hipMemcpyAsync(film_sum_dev, film_sum_user, sum_bytes, hipMemcpyHostToDevice, st));
+        HIP_OK(hipMemcpyAsync(film_count_dev, film_count_user, count_bytes, hipMemcpyHostToDevice, st));
+    }
     if (timed) memset(stats, 0, sizeof(*stats));
 
     EventPool pool;
@@ -1159,6 +1221,10 @@ int bbgpu_render(BbScene* scene, const BbCameraDesc* camera, const BbRenderParam
         }
     }
     hipEvent_t t_end = timed ? mark() : nullptr;
+    if (host_film) {
+        HIP_OK(hipMemcpyAsync(film_sum_user, film_sum_dev, sum_bytes, hipMemcpyDeviceToHost, st));
+        HIP_OK(hipMemcpyAsync(film_count_user, film_count_dev, count_bytes, hipMemcpyDeviceToHost, st));
+    }
     int err = 0;
     unsigned long long c[8] = { 0 };
     HIP_OK(hipMemcpyAsync(&err, P.err, sizeof(int), hipMemcpyDeviceToHost, st));

@@ -473,3 +473,36 @@ def random_rays(bmin, bmax, n, seed, t_max=np.inf):
     else:
         rays["t_max"] = np.asarray(t_max, F32)
     return rays
+
+
+def portable_random_rays(bmin, bmax, n, seed, t_max=np.inf):
+    """Seeded random rays built from IEEE-exact operations only (+, -, *, /,
+    sqrt; no sin/cos), so every machine produces the same bits: origins
+    uniform in the box scaled by 1.2 about its centre, directions uniform on
+    the sphere by rejection from the cube [-1, 1]^3, t in [1e-4, t_max]."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    bmin = np.asarray(bmin, np.float64)
+    bmax = np.asarray(bmax, np.float64)
+    c = 0.5 * (bmin + bmax)
+    h = 0.6 * (bmax - bmin)
+    o = (c + h * rng.uniform(-1.0, 1.0, (n, 3))).astype(F32)
+    d = np.empty((n, 3), np.float64)
+    filled = 0
+    while filled < n:
+        m = max(1024, int(1.95 * (n - filled)))
+        v = rng.uniform(-1.0, 1.0, (m, 3))
+        r2 = v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1] + v[:, 2] * v[:, 2]
+        v = v[(r2 > 1e-6) & (r2 <= 1.0)]
+        r2 = r2[(r2 > 1e-6) & (r2 <= 1.0)]
+        k = min(len(v), n - filled)
+        d[filled:filled + k] = v[:k] / np.sqrt(r2[:k])[:, None]
+        filled += k
+    rays = np.zeros(n, capi.RAY_DTYPE)
+    rays["o"] = o
+    rays["d"] = d.astype(F32)
+    rays["t_min"] = F32(1e-4)
+    if np.isscalar(t_max):
+        rays["t_max"] = F32(t_max)
+    else:
+        rays["t_max"] = np.asarray(t_max, F32)
+    return rays

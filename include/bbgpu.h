@@ -260,9 +260,10 @@ int bbgpu_scene_destroy(BbScene* scene);
 int bbgpu_scene_info(const BbScene* scene, BbSceneInfo* info_out);
 
 /* Replaces BVH::Intersect / BVH::IntersectAny over a ray batch
- * (src/accel/bvh.cpp:299-361): rays and hits are device pointers.
- * counters (host pointer, may be NULL) requests traversal counts; the call
- * then synchronizes the stream. */
+ * (src/accel/bvh.cpp:299-361).  rays_dev and hits_dev are device pointers for
+ * the fast path; buffers in host memory are also accepted and staged through
+ * device memory (PCIe copies included).  counters (host pointer, may be NULL)
+ * requests traversal counts.  The call returns when the batch is traced. */
 int bbgpu_trace(BbScene* scene, const BbRay* rays_dev, int32_t n_rays, int32_t any_hit, BbHit* hits_dev,
                 BbTraceCounters* counters, void* stream);
 
@@ -272,9 +273,11 @@ int bbgpu_trace(BbScene* scene, const BbRay* rays_dev, int32_t n_rays, int32_t a
  * film_sum_dev (3 floats per pixel, row-major, added to the existing contents
  * in sample order) and film_count_dev (1 int32 per pixel), as Film::AddSample
  * does (src/reconstruction/film.cpp:26-37).  NaN/Inf samples are dropped.
- * stats (host pointer, may be NULL) requests timing; the call then
- * synchronizes the stream.  count_traversal = 1 also fills the traversal
- * counters (slower kernels; for measurement only). */
+ * The film buffers are device pointers for the fast path; a film in host
+ * memory is also accepted (copied in, accumulated, copied back).
+ * stats (host pointer, may be NULL) requests timing; count_traversal = 1 also
+ * fills the traversal counters (slower kernels; for measurement only).  The
+ * call returns when the render is complete. */
 int bbgpu_render(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams* params, float* film_sum_dev,
                  int32_t* film_count_dev, void* stream, BbRenderStats* stats, int32_t count_traversal);
 

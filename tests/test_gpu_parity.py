@@ -13,7 +13,7 @@ import os
 import numpy as np
 import pytest
 
-from bulbit_amd import scenes
+from bulbit_amd import capi, scenes
 
 from conftest import pixel_agreement
 
@@ -114,6 +114,71 @@ def test_trace_full_scene_rays(c2):
     sec["t_max"] = np.float32(np.inf)
     _check_trace(g, o, sec)
     _check_trace(g, o, sec, any_hit=True)
+
+
+def test_trace_axis_aligned_rays_on_box_planes(ico6):
+    """Rays with zero direction components whose origins lie exactly on BVH
+    box planes: the slab test then multiplies 0 by an infinite inverse and
+    gets NaN on that axis, which both forms of the test must ignore the same
+    way (bounding_box.h:443-484)."""
+    sc, cam, g, o = ico6
+    nodes, _ = o.bvh()
+    rng = np.random.default_rng(11)
+    n = 200_000
+    pick = nodes[rng.integers(0, len(nodes), n)]
+    corner = np.where(rng.random((n, 3)) < 0.5, pick["bmin"], pick["bmax"]).astype(np.float32)
+    lo, hi = o.world_bounds()
+    free = rng.uniform(lo, hi, (n, 3)).astype(np.float32)
+    keep = rng.random((n, 3)) < 0.5  # per axis: on the plane, or anywhere
+    origin = np.where(keep, corner, free)
+    d = rng.normal(size=(n, 3)).astype(np.float32)
+    zero = rng.integers(0, 3, n)
+    d[np.arange(n), zero] = 0.0  # one zero component per ray
+    two = rng.random(n) < 0.3    # and a second one for 30% of the rays
+    d[np.arange(n)[two], (zero[two] + 1) % 3] = 0.0
+    d = d / np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros(n, capi.RAY_DTYPE)
+    rays["o"] = origin
+    rays["d"] = d.astype(np.float32)
+    rays["t_min"] = np.float32(1e-4)
+    rays["t_max"] = np.float32(np.inf)
+    h = _check_trace(g, o, rays)
+    log(f"plane rays hit fraction {np.mean(h['prim'] >= 0):.3f}")
+    _check_trace(g, o, rays, any_hit=True)
+
+
+def test_trace_accepts_host_buffers(ico6):
+    """bbgpu_trace with rays and hits in host memory (staged by the library)
+    equals the device-buffer call."""
+    import ctypes
+
+    sc, cam, g, o = ico6
+    bmin, bmax = o.world_bounds()
+    rays = np.ascontiguousarray(scenes.random_rays(bmin, bmax, 100_000, seed=12))
+    dev_hits, _ = g.trace(rays)
+    host_hits = np.zeros(len(rays), capi.HIT_DTYPE)
+    capi.check(g.lib.bbgpu_trace(g.h, rays.ctypes.data, len(rays), 0, host_hits.ctypes.data, None, None),
+               "bbgpu_trace", g.lib)
+    assert np.array_equal(host_hits.view(np.int32), dev_hits.view(np.int32))
+
+
+def test_render_accepts_host_film(readme):
+    """bbgpu_render into a film in host memory (copied in, accumulated, copied
+    back) equals the device-film render, and accumulates onto what is there."""
+    import ctypes
+
+    sc, cam, g, o = readme
+    p = scenes.render_params(4)
+    ds, dc, _ = g.render(cam, p)
+    npix = cam.width * cam.height
+    hs = np.zeros(3 * npix, np.float32)
+    hc = np.zeros(npix, np.int32)
+    for begin in (0, 2):
+        q = scenes.render_params(2, sample_begin=begin)
+        capi.check(g.lib.bbgpu_render(g.h, ctypes.byref(cam), ctypes.byref(q), hs.ctypes.data, hc.ctypes.data, None,
+                                      None, 0), "bbgpu_render", g.lib)
+    assert np.array_equal(hc.reshape(dc.shape), dc)
+    assert np.array_equal(hs.reshape(ds.shape).view(np.int32), ds.view(np.int32))
 
 
 def _render_pair(g, o, cam, params):
