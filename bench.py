@@ -48,6 +48,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU work of the baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--wavefront-paths", type=int, default=0)
+    ap.add_argument("--ref-spp", type=int, default=4096, help="spp of the relMSE reference (0: skip the quality check)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (scales the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -156,10 +157,58 @@ def main():
         "relmse_equal_time_cache_vs_plain": None,
     }
 
+    if args.ref_spp > 0:
+        test = s_dev.cpu().numpy().reshape(args.height, args.width, 3) / \
+            c_dev.cpu().numpy().reshape(args.height, args.width, 1).astype(np.float32)
+        out["quality"] = quality(g, sc, cam, args, test)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, cam, params, args, samples_per_step)
         out["speedup_vs_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
     print(json.dumps(out), flush=True)
+
+
+RELMSE_EPS = 1e-2
+
+
+def relmse(x, r):
+    """mean over pixels and channels of (x - r)^2 / (r^2 + eps), eps = 1e-2"""
+    import numpy as np
+    x = x.astype(np.float64)
+    r = r.astype(np.float64)
+    return float(np.mean((x - r) ** 2 / (r * r + RELMSE_EPS)))
+
+
+def quality(g, sc, cam, args, test):
+    """relMSE of the timed render (samples [0, spp)) against references with
+    disjoint samples [spp, spp + ref_spp): the whole frame rendered on the GPU,
+    and a crop rendered by the oracle on the CPU."""
+    import numpy as np
+    from bulbit_amd import scenes
+
+    t0 = time.time()
+    p = scenes.render_params(args.ref_spp, max_bounces=args.max_bounces, sample_begin=args.spp)
+    s, c, _ = g.render(cam, p)
+    ref = s / c[..., None].astype(np.float32)
+    gpu_secs = time.time() - t0
+    out = {"epsilon": RELMSE_EPS, "ref_spp": args.ref_spp, "ref_samples": [args.spp, args.spp + args.ref_spp],
+           "relmse_vs_gpu_reference": round(relmse(test, ref), 6), "gpu_reference_seconds": round(gpu_secs, 2)}
+    log(f"[bench] relMSE of the {args.spp}-spp frame vs a {args.ref_spp}-spp GPU frame: {out['relmse_vs_gpu_reference']}")
+    if not args.no_cpu:
+        from oracle import OracleScene
+        w, h = 96, 54
+        x0, y0 = cam.width // 2 - w // 2, cam.height // 2 - h // 2
+        rect = (x0, y0, x0 + w, y0 + h)
+        threads = max(1, min(16, len(os.sched_getaffinity(0))))
+        o = OracleScene(sc)
+        s2, c2, secs = o.render(cam, p, rect=rect, threads=threads)
+        cpu_ref = (s2 / np.maximum(c2, 1)[..., None])[y0:y0 + h, x0:x0 + w]
+        out["cpu_crop"] = list(rect)
+        out["relmse_vs_cpu_reference_crop"] = round(relmse(test[y0:y0 + h, x0:x0 + w], cpu_ref), 6)
+        out["relmse_gpu_vs_cpu_reference_crop"] = round(relmse(ref[y0:y0 + h, x0:x0 + w], cpu_ref), 8)
+        out["cpu_reference_seconds"] = round(secs, 2)
+        log(f"[bench] crop {rect}: relMSE vs CPU {args.ref_spp}-spp {out['relmse_vs_cpu_reference_crop']}, "
+            f"GPU vs CPU reference {out['relmse_gpu_vs_cpu_reference_crop']}")
+    return out
 
 
 def copy_calibration(mib):

@@ -1,10 +1,17 @@
 // bb_scene.h — device-resident scene layout and the host-side scene object.
 //
 // HBM layout (all arrays 16-byte aligned, indexed as noted):
-//   nodes   [n_nodes]  BbBvhNode, 32 B: the reference's LinearBVHNode
-//                      (include/bulbit/bvh.h:46-58), pre-order, child1 = i + 1
+//   inner   [n_inner]  64 B per internal node of the reference's BVH
+//                      (LinearBVHNode, include/bulbit/bvh.h:46-58), in
+//                      pre-order: the boxes of its two children and their
+//                      references,
+//                        {c1.min.xyz, c1.max.x} {c1.max.yz, c2.min.xy}
+//                        {c2.min.z, c2.max.xyz} {ref1, ref2, split axis, 0}
+//                      ref >= 0: inner record; ref < 0: leaf whose first
+//                      slot is ~ref (its last slot carries kSlotLast)
 //   tris    [n_slots]  3 x float4 = 48 B per BVH slot (leaf order):
-//                      {p0.xyz, prim}, {e1.xyz, sphere index}, {e2.xyz, kind}
+//                      {p0.xyz, prim}, {e1.xyz, sphere index},
+//                      {e2.xyz, kind | kSlotLast on a leaf's last slot}
 //                      with e1 = p1 - p0 and e2 = p2 - p0 computed on the host
 //                      in float32, bit-identical to Triangle::Intersect's
 //                      (src/geometry/triangle.cpp:61-62)
@@ -28,7 +35,9 @@ namespace bb
 enum : int32_t
 {
     KIND_TRIANGLE = 0,
-    KIND_SPHERE = 1
+    KIND_SPHERE = 1,
+    KIND_MASK = 0xff,
+    kSlotLast = 0x100 // on the last slot of a leaf
 };
 
 // BxDF flags, include/bulbit/bxdf.h:7-22
@@ -83,7 +92,7 @@ struct GpuLight
 
 struct SceneView
 {
-    const float4* nodes; // 2 float4 per node
+    const float4* inner; // 4 float4 per internal node
     const float4* tris;  // 3 float4 per slot
     const int4* slot_shade;
     const int32_t* slot_light;
@@ -100,6 +109,8 @@ struct SceneView
     float func_integral;
     int32_t stack_depth;
     int32_t has_null_material;
+    float root_box[6];  // root node box: min.xyz, max.xyz
+    int32_t root_ref;   // the root as a reference (inner record 0, or a leaf)
 };
 
 // Host-side build products

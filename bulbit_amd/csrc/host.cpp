@@ -235,6 +235,20 @@ void flatten(const Builder& B, int32_t root, HostBvh& out)
 
 inline f3 load3(const float* a, int64_t i) { return mk3(a[3 * i], a[3 * i + 1], a[3 * i + 2]); }
 
+inline float i2f(int32_t v)
+{
+    float f;
+    memcpy(&f, &v, 4);
+    return f;
+}
+
+inline int32_t f2i(float f)
+{
+    int32_t v;
+    memcpy(&v, &f, 4);
+    return v;
+}
+
 } // namespace
 
 bool validate_desc(const BbSceneDesc& d, std::string& err)
@@ -528,7 +542,7 @@ bool create_scene(const BbSceneDesc& d, BbScene* s, std::string& err)
             f3 e1 = p1 - p0, e2 = p2 - p0;
             tris[3 * size_t(j) + 0] = make_float4(p0.x, p0.y, p0.z, prim_f);
             tris[3 * size_t(j) + 1] = make_float4(e1.x, e1.y, e1.z, 0.0f);
-            tris[3 * size_t(j) + 2] = make_float4(e2.x, e2.y, e2.z, 0.0f);
+            tris[3 * size_t(j) + 2] = make_float4(e2.x, e2.y, e2.z, i2f(KIND_TRIANGLE));
             slot_shade[j] = make_int4(v[0], v[1], v[2], p.material);
         } else {
             int32_t kind = KIND_SPHERE, sidx = p.shape_index;
@@ -541,11 +555,43 @@ bool create_scene(const BbSceneDesc& d, BbScene* s, std::string& err)
             slot_shade[j] = make_int4(0, 0, 0, p.material);
         }
     }
-    // flag leaves holding a sphere (axis is unused for leaves)
-    for (BbBvhNode& nd : nodes)
-        if (nd.count > 0)
-            for (int32_t k = 0; k < nd.count; ++k)
-                if (d.primitives[bvh.ordered[nd.offset + k]].shape_kind == BB_SHAPE_SPHERE) nd.axis = 1;
+    // flag the last slot of every leaf, and build the 64-byte inner records:
+    // record of internal node i = boxes and references of children i + 1 and
+    // nodes[i].offset (BVH::RayCast's child1 / child2, bvh.h:139-151)
+    std::vector<int32_t> inner_index(nodes.size(), -1);
+    int32_t n_inner = 0;
+    for (size_t i = 0; i < nodes.size(); ++i)
+        if (nodes[i].count == 0) inner_index[i] = n_inner++;
+        else {
+            float4& last = tris[3 * size_t(nodes[i].offset + nodes[i].count - 1) + 2];
+            last.w = i2f(f2i(last.w) | kSlotLast);
+        }
+    auto ref_of = [&](size_t c) { return nodes[c].count == 0 ? inner_index[c] : ~nodes[c].offset; };
+    std::vector<float4> inner(size_t(4) * std::max(1, n_inner));
+    for (size_t i = 0; i < nodes.size(); ++i) {
+        if (nodes[i].count != 0) continue;
+        const BbBvhNode& c1 = nodes[i + 1];
+        const BbBvhNode& c2 = nodes[size_t(nodes[i].offset)];
+        float4* r = &inner[size_t(4) * inner_index[i]];
+        r[0] = make_float4(c1.bmin[0], c1.bmin[1], c1.bmin[2], c1.bmax[0]);
+        r[1] = make_float4(c1.bmax[1], c1.bmax[2], c2.bmin[0], c2.bmin[1]);
+        r[2] = make_float4(c2.bmin[2], c2.bmax[0], c2.bmax[1], c2.bmax[2]);
+        r[3] = make_float4(i2f(ref_of(i + 1)), i2f(ref_of(size_t(nodes[i].offset))), i2f(int32_t(nodes[i].axis)), 0.0f);
+    }
+    // every reference the kernels follow stays inside its array
+    auto ref_ok = [&](int32_t ref) { return ref >= 0 ? ref < n_inner : (~ref >= 0 && ~ref < ns); };
+    for (int32_t k = 0; k < n_inner; ++k) {
+        const int32_t r1 = f2i(inner[size_t(4) * k + 3].x), r2 = f2i(inner[size_t(4) * k + 3].y);
+        const int32_t ax = f2i(inner[size_t(4) * k + 3].z);
+        if (!ref_ok(r1) || !ref_ok(r2) || ax < 0 || ax > 2) {
+            err = "internal error: inner BVH record out of range";
+            return false;
+        }
+    }
+    if (!ref_ok(ref_of(0)) || !(f2i(tris[3 * size_t(ns - 1) + 2].w) & kSlotLast)) {
+        err = "internal error: BVH root or last leaf slot malformed";
+        return false;
+    }
 
     std::vector<float4> normals(std::max(1, d.n_vertices)), tangents(std::max(1, d.n_vertices));
     for (int32_t i = 0; i < d.n_vertices; ++i) {
@@ -555,11 +601,12 @@ bool create_scene(const BbSceneDesc& d, BbScene* s, std::string& err)
 
     SceneView& V = s->view;
     hipGetDevice(&s->device);
-    const float4* nodes_dev = nullptr;
-    const BbBvhNode* nodes_raw = nullptr;
-    if (!upload(s, nodes.data(), nodes.size(), &nodes_raw, err)) return false;
-    nodes_dev = reinterpret_cast<const float4*>(nodes_raw);
-    V.nodes = nodes_dev;
+    if (!upload(s, inner.data(), inner.size(), &V.inner, err)) return false;
+    for (int k = 0; k < 3; ++k) {
+        V.root_box[k] = nodes[0].bmin[k];
+        V.root_box[3 + k] = nodes[0].bmax[k];
+    }
+    V.root_ref = ref_of(0);
     if (!upload(s, tris.data(), tris.size(), &V.tris, err)) return false;
     if (!upload(s, slot_shade.data(), slot_shade.size(), &V.slot_shade, err)) return false;
     if (!upload(s, slot_light.data(), slot_light.size(), &V.slot_light, err)) return false;

@@ -43,8 +43,9 @@ constexpr int kMaxGridBlocks = 256 * 32;
 
 BB_D f3 xyz(float4 v) { return mk3(v.x, v.y, v.z); }
 
-// AABB3::TestRay with precomputed inverse direction (bounding_box.h:443-484).
-// a = (min.x, min.y, min.z, max.x), b = (max.y, max.z, offset, meta)
+// AABB3::TestRay with precomputed inverse direction (bounding_box.h:443-484),
+// split into the part that does not depend on t_max and the box's entry
+// distance.
 //
 // The reference tests x, then y against the x-tightened interval, then z
 // against the x,y-tightened interval, and ignores any NaN slab distance
@@ -54,22 +55,34 @@ BB_D f3 xyz(float4 v) { return mk3(v.x, v.y, v.z); }
 //   t_min_i > t_max_j (i != j)
 // (t_min is never compared with t_max, and t_min_i <= t_max_i always holds
 // for one axis because both are computed with the same sign of inv.)  With
-// NaN-ignoring max/min over the three axes that is
-//   fail = t_min > min_i t_max_i  or  max_i t_min_i > t_max
-//          or  max_i t_min_i > min_i t_max_i,
-// the same predicate with three comparisons and no sequential tightening.
-BB_D bool box_test(float4 a, float4 b, f3 o, float t_min, float t_max, f3 inv, bool n0, bool n1, bool n2)
+// NaN-ignoring max/min over the three axes, lo = max_i t_min_i and
+// hi = min_i t_max_i, that is
+//   fail = t_min > hi  or  lo > hi  or  lo > t_max.
+// The first two do not involve t_max.  The last is monotone in t_max: a box
+// that fails it fails again for any smaller t_max, and whether it passes
+// later needs only lo, which does not change.
+struct BoxHit
 {
-    const float lx = ((n0 ? a.w : a.x) - o.x) * inv.x;
-    const float hx = ((n0 ? a.x : a.w) - o.x) * inv.x;
-    const float ly = ((n1 ? b.x : a.y) - o.y) * inv.y;
-    const float hy = ((n1 ? a.y : b.x) - o.y) * inv.y;
-    const float lz = ((n2 ? b.y : a.z) - o.z) * inv.z;
-    const float hz = ((n2 ? a.z : b.y) - o.z) * inv.z;
-    const float lo = fmaxf(fmaxf(lx, ly), lz);
+    bool ok;  // the t_max-independent part passed
+    float lo; // entry distance: the box passes iff ok and !(lo > t_max)
+};
+
+BB_D BoxHit box_part(f3 bmin, f3 bmax, f3 o, float t_min, f3 inv, bool n0, bool n1, bool n2)
+{
+    const float lx = ((n0 ? bmax.x : bmin.x) - o.x) * inv.x;
+    const float hx = ((n0 ? bmin.x : bmax.x) - o.x) * inv.x;
+    const float ly = ((n1 ? bmax.y : bmin.y) - o.y) * inv.y;
+    const float hy = ((n1 ? bmin.y : bmax.y) - o.y) * inv.y;
+    const float lz = ((n2 ? bmax.z : bmin.z) - o.z) * inv.z;
+    const float hz = ((n2 ? bmin.z : bmax.z) - o.z) * inv.z;
+    BoxHit h;
+    h.lo = fmaxf(fmaxf(lx, ly), lz);
     const float hi = fminf(fminf(hx, hy), hz);
-    return !((t_min > hi) | (lo > t_max) | (lo > hi));
+    h.ok = !((t_min > hi) | (h.lo > hi));
+    return h;
 }
+
+BB_D bool box_pass(BoxHit h, float t_max) { return h.ok && !(h.lo > t_max); }
 
 // Triangle::Intersect's test (src/geometry/triangle.cpp:55-97) with
 // e1 = p1 - p0, e2 = p2 - p0 precomputed (same float32 subtraction) and the
@@ -126,12 +139,16 @@ struct TraceCount
 };
 
 // Per-lane traversal state of BVH::RayCast (include/bulbit/bvh.h:101-155).
+// `cur` is the node to expand next, whose own box test has passed: an inner
+// record (>= 0), a leaf (~first slot), or kNoRef (take the next stack entry).
+constexpr int32_t kNoRef = INT32_MIN;
+
 struct Lane
 {
     f3 o, d, inv, dn;
     float l, t_min, t_max;
     bool n0, n1, n2;
-    int32_t sp, node;
+    int32_t sp, cur;
     int32_t hit_slot;
     float hit_t, hit_u, hit_v;
 };
@@ -149,18 +166,30 @@ BB_D void lane_start(Lane& L, f3 o, f3 d, float t_min, float t_max)
     L.dn = d;
     L.l = normalize_m(L.dn);
     L.sp = 0;
-    L.node = 0;
+    L.cur = kNoRef;
     L.hit_slot = -1;
     L.hit_t = L.hit_u = L.hit_v = 0.0f;
 }
 
-// Traversal stack: entries [0, lds_entries) in LDS ([entry][thread] layout,
-// conflict-free for a wave), deeper entries in a global overflow area
-// ([entry][global thread]); cap = BVH depth + 2 bounds every push.
+// the root is the first node RayCast takes off its stack and box-tests
+template <bool COUNT>
+BB_D void lane_root(const SceneView& S, Lane& L, TraceCount& c)
+{
+    if (COUNT) c.nodes++;
+    const BoxHit h = box_part(mk3(S.root_box[0], S.root_box[1], S.root_box[2]),
+                              mk3(S.root_box[3], S.root_box[4], S.root_box[5]), L.o, L.t_min, L.inv, L.n0, L.n1,
+                              L.n2);
+    L.cur = box_pass(h, L.t_max) ? S.root_ref : kNoRef;
+}
+
+// Traversal stack of (node reference, entry distance) pairs: entries
+// [0, lds_entries) in LDS ([entry][thread] layout, conflict-free for a
+// wave), deeper entries in a global overflow area ([entry][global thread]);
+// cap = BVH depth + 2 bounds every push.
 struct TraceStack
 {
-    int32_t* lds;
-    int32_t* gbl;
+    int2* lds;
+    int2* gbl;
     int32_t lds_entries;
     int32_t gstride;
     int32_t cap;
@@ -168,7 +197,7 @@ struct TraceStack
 
 // FULL: the whole stack fits in LDS (cap <= lds_entries), no overflow branch
 template <bool FULL>
-BB_D void stack_put(const TraceStack& s, int32_t sp, int32_t v)
+BB_D void stack_put(const TraceStack& s, int32_t sp, int2 v)
 {
     if (FULL || sp < s.lds_entries)
         s.lds[sp * kTraceBlock] = v;
@@ -177,46 +206,63 @@ BB_D void stack_put(const TraceStack& s, int32_t sp, int32_t v)
 }
 
 template <bool FULL>
-BB_D int32_t stack_get(const TraceStack& s, int32_t sp)
+BB_D int2 stack_get(const TraceStack& s, int32_t sp)
 {
     return (FULL || sp < s.lds_entries) ? s.lds[sp * kTraceBlock] : s.gbl[(sp - s.lds_entries) * s.gstride];
 }
 
-// One step of RayCast for one lane: the node taken is box-tested with the
-// current t_max; a leaf tests its primitives in stored order (the callbacks of
-// BVH::Intersect / IntersectAny, src/accel/bvh.cpp:299-361); an internal node
-// pushes the far child and takes the near child next (pushing and popping it
-// would test it at the same point with the same t_max).  Returns true when the
-// lane's query is finished.
+// a far child whose t_max-independent test failed, kept on the stack only
+// when counting so that it is counted when RayCast would pop and test it
+// (a signalling-NaN bit pattern: no arithmetic result has it)
+constexpr int32_t kFailedLo = 0x7fbfffff;
+
+// One step of RayCast for one lane.
+//
+// Expanding an inner record tests both children's boxes from the parent's
+// 64 bytes.  RayCast pushes the far child, then tests the near child with
+// the current t_max: the near test here is that same test.  The far child is
+// tested by RayCast when popped, with the t_max of that moment; here its
+// t_max-independent part and entry distance are computed now, a child that
+// fails already is not pushed (it would fail then too), and the pop applies
+// the remaining `lo > t_max` check with the then-current t_max.  So the same
+// nodes are expanded in the same order with the same outcomes, and leaves
+// test their primitives in stored order (the callbacks of BVH::Intersect /
+// IntersectAny, src/accel/bvh.cpp:299-361).  With COUNT, every far child is
+// pushed and counted when popped, which reproduces RayCast's node-test count.
+// Returns true when the lane's query is finished.
 template <bool ANY, bool COUNT, bool FULL>
 BB_D bool trace_step(const SceneView& S, Lane& L, const TraceStack& stk, TraceCount& c, int* err)
 {
-    const float4 a = S.nodes[2 * L.node];
-    const float4 b = S.nodes[2 * L.node + 1];
-    // both halves of the 32-byte node in one round trip
-    asm volatile("" ::"v"(a.x), "v"(a.y), "v"(a.z), "v"(a.w), "v"(b.x), "v"(b.y), "v"(b.z), "v"(b.w));
-    if (COUNT) c.nodes++;
-    const bool pass = box_test(a, b, L.o, L.t_min, L.t_max, L.inv, L.n0, L.n1, L.n2);
-    const int32_t offset = __float_as_int(b.z);
-    const uint32_t meta = __float_as_uint(b.w);
-    const uint32_t count = meta & 0xffffu;
-    const uint32_t axis = (meta >> 16) & 0xffu;
-    if (pass && count == 0) {
-        const int32_t child1 = L.node + 1;
-        const int32_t child2 = offset;
+    if (L.cur >= 0) {
+        const float4* r = S.inner + 4 * L.cur;
+        const float4 a = r[0], b = r[1], e = r[2], m = r[3];
+        // the record's 64 bytes in one round trip
+        asm volatile("" ::"v"(a.x), "v"(a.y), "v"(a.z), "v"(a.w), "v"(b.x), "v"(b.y), "v"(b.z), "v"(b.w), "v"(e.x),
+                     "v"(e.y), "v"(e.z), "v"(e.w), "v"(m.x), "v"(m.y), "v"(m.z));
+        const BoxHit h1 = box_part(mk3(a.x, a.y, a.z), mk3(a.w, b.x, b.y), L.o, L.t_min, L.inv, L.n0, L.n1, L.n2);
+        const BoxHit h2 = box_part(mk3(b.z, b.w, e.x), mk3(e.y, e.z, e.w), L.o, L.t_min, L.inv, L.n0, L.n1, L.n2);
+        const int32_t axis = __float_as_int(m.z);
         const bool neg = axis == 0 ? L.n0 : (axis == 1 ? L.n1 : L.n2);
-        if (L.sp >= stk.cap) {
-            atomicOr(err, 1);
-            return true;
+        const int32_t near_ref = __float_as_int(neg ? m.y : m.x);
+        const int32_t far_ref = __float_as_int(neg ? m.x : m.y);
+        const BoxHit hn = neg ? h2 : h1;
+        const BoxHit hf = neg ? h1 : h2;
+        const bool push = COUNT || box_pass(hf, L.t_max);
+        if (push) {
+            if (L.sp >= stk.cap) {
+                atomicOr(err, 1);
+                return true;
+            }
+            stack_put<FULL>(stk, L.sp, make_int2(far_ref, hf.ok ? __float_as_int(hf.lo) : kFailedLo));
+            ++L.sp;
         }
-        stack_put<FULL>(stk, L.sp, neg ? child1 : child2);
-        ++L.sp;
-        L.node = neg ? child2 : child1;
-        return false;
-    }
-    if (pass) {
-        for (uint32_t i = 0; i < count; ++i) {
-            const int32_t slot = offset + int32_t(i);
+        if (COUNT) c.nodes++;
+        if (box_pass(hn, L.t_max)) {
+            L.cur = near_ref;
+            return false;
+        }
+    } else if (L.cur != kNoRef) {
+        for (int32_t slot = ~L.cur;; ++slot) {
             const float4 t0 = S.tris[3 * slot];
             const float4 t1 = S.tris[3 * slot + 1];
             const float4 t2 = S.tris[3 * slot + 2];
@@ -224,12 +270,13 @@ BB_D bool trace_step(const SceneView& S, Lane& L, const TraceStack& stk, TraceCo
             asm volatile("" ::"v"(t0.x), "v"(t0.y), "v"(t0.z), "v"(t1.x), "v"(t1.y), "v"(t1.z), "v"(t2.x), "v"(t2.y),
                          "v"(t2.z), "v"(t2.w));
             if (COUNT) c.prims++;
+            const int32_t kind = __float_as_int(t2.w);
             float t = 0.0f, u = 0.0f, v = 0.0f;
             // the triangle test runs on every slot so that the three loads
             // above issue together; a sphere slot (zero edges) fails it and
             // takes the sphere test instead
             bool h = mt_test(xyz(t0), xyz(t1), xyz(t2), L.o, L.dn, L.l, L.t_min, L.t_max, t, u, v);
-            if ((axis & 1u) && __float_as_int(t2.w) == KIND_SPHERE) {
+            if ((kind & KIND_MASK) == KIND_SPHERE) {
                 u = v = 0.0f;
                 h = sphere_test(S.spheres[__float_as_int(t1.w)], L.o, L.d, L.t_min, L.t_max, t);
             }
@@ -243,11 +290,17 @@ BB_D bool trace_step(const SceneView& S, Lane& L, const TraceStack& stk, TraceCo
                 if (t <= L.t_min) return true;
                 L.t_max = t;
             }
+            if (kind & kSlotLast) break;
         }
     }
+    // take the next stack entry; its deferred `lo > t_max` check decides
+    // whether it is expanded
     if (L.sp == 0) return true;
     --L.sp;
-    L.node = stack_get<FULL>(stk, L.sp);
+    const int2 e = stack_get<FULL>(stk, L.sp);
+    if (COUNT) c.nodes++;
+    const bool pass = (!COUNT || e.y != kFailedLo) && !(__int_as_float(e.y) > L.t_max);
+    L.cur = pass ? e.x : kNoRef;
     return false;
 }
 
@@ -277,6 +330,7 @@ BB_D void persistent_trace(const SceneView& S, const Src& src, int n, int* fetch
             if (!active && mine < n) {
                 id = mine;
                 src.load(id, L);
+                lane_root<COUNT>(S, L, c);
                 active = true;
             }
             idle = __ballot(active ? 0 : 1);
@@ -349,7 +403,7 @@ struct BatchSrc
     }
 };
 
-BB_D TraceStack make_stack(int32_t* lds_stack, int32_t* gstack, int lds_entries, int cap)
+BB_D TraceStack make_stack(int2* lds_stack, int2* gstack, int lds_entries, int cap)
 {
     TraceStack t;
     t.lds = lds_stack + threadIdx.x;
@@ -367,10 +421,10 @@ constexpr int kTraceSteps = BB_TRACE_STEPS;
 
 template <bool ANY, bool COUNT, bool FULL>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_batch(SceneView S, BatchSrc src, int n, int* fetch,
-                                                               int32_t* gstack, int lds_entries, int refill_min,
+                                                               int2* gstack, int lds_entries, int refill_min,
                                                                unsigned long long* counters, int* err)
 {
-    extern __shared__ int32_t lds_stack[];
+    extern __shared__ int2 lds_stack[];
     const TraceStack stk = make_stack(lds_stack, gstack, lds_entries, S.stack_depth);
     TraceCount c{ 0, 0, 0 };
     persistent_trace<ANY, COUNT, FULL, kTraceSteps>(S, src, n, fetch, stk, c, err, refill_min);
@@ -398,7 +452,7 @@ struct PathBufs
     float4* sray;    // 2 float4 per shadow slot: (o, t_max), (d, path id)
     float4* scontrib;
     int* counts;     // [depth][4]: active paths, shadow rays, closest fetch, shadow fetch
-    int32_t* gstack; // traversal stack overflow, [entry][persistent thread]
+    int2* gstack;    // traversal stack overflow, [entry][persistent thread]
     int* err;
     unsigned long long* counters; // [6]: closest nodes/prims/accepts, shadow nodes/prims/accepts
 };
@@ -491,10 +545,10 @@ struct ShadowSrc
 
 template <bool COUNT, bool FULL>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(SceneView S, const int* __restrict__ n_ptr, int* fetch,
-                                                                 ClosestSrc src, int32_t* gstack, int lds_entries, int refill_min,
+                                                                 ClosestSrc src, int2* gstack, int lds_entries, int refill_min,
                                                                  unsigned long long* counters, int* err)
 {
-    extern __shared__ int32_t lds_stack[];
+    extern __shared__ int2 lds_stack[];
     const TraceStack stk = make_stack(lds_stack, gstack, lds_entries, S.stack_depth);
     TraceCount c{ 0, 0, 0 };
     persistent_trace<false, COUNT, FULL, kTraceSteps>(S, src, *n_ptr, fetch, stk, c, err, refill_min);
@@ -503,10 +557,10 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(SceneView S, cons
 
 template <bool COUNT, bool FULL>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_shadow(SceneView S, const int* __restrict__ n_ptr, int* fetch,
-                                                                ShadowSrc src, int32_t* gstack, int lds_entries, int refill_min,
+                                                                ShadowSrc src, int2* gstack, int lds_entries, int refill_min,
                                                                 unsigned long long* counters, int* err)
 {
-    extern __shared__ int32_t lds_stack[];
+    extern __shared__ int2 lds_stack[];
     const TraceStack stk = make_stack(lds_stack, gstack, lds_entries, S.stack_depth);
     TraceCount c{ 0, 0, 0 };
     persistent_trace<true, COUNT, FULL, kTraceSteps>(S, src, *n_ptr, fetch, stk, c, err, refill_min);
@@ -527,7 +581,7 @@ BB_D SurfaceHit surface_at(const SceneView& S, int32_t slot, float t, float u, f
     SurfaceHit h;
     const float4 t0 = S.tris[3 * slot], t1 = S.tris[3 * slot + 1], t2 = S.tris[3 * slot + 2];
     f3 normal, ns, ts;
-    if (__float_as_int(t2.w) == KIND_SPHERE) {
+    if ((__float_as_int(t2.w) & KIND_MASK) == KIND_SPHERE) {
         const BbSphere& sp = S.spheres[__float_as_int(t1.w)];
         quat q{ sp.q[0], sp.q[1], sp.q[2], sp.q[3] };
         f3 ro, rd;
@@ -844,21 +898,21 @@ bool device_accessible(const void* p)
     return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged || a.isManaged;
 }
 
-// global overflow area of the traversal stack: (cap - lds_entries) entries
+// global overflow area of the traversal stack: (cap - lds_entries) int2 entries
 // for every thread of the persistent grid
 bool ensure_gstack(Workspace& w, size_t entries)
 {
     entries = std::max<size_t>(entries, 64);
     if (w.gstack_entries >= entries) {
-        w.P.gstack = reinterpret_cast<int32_t*>(w.gstack);
+        w.P.gstack = reinterpret_cast<int2*>(w.gstack);
         return true;
     }
     if (w.gstack) hipFree(w.gstack);
     w.gstack = nullptr;
     w.gstack_entries = 0;
-    if (hipMalloc(&w.gstack, entries * sizeof(int32_t)) != hipSuccess) return false;
+    if (hipMalloc(&w.gstack, entries * sizeof(int2)) != hipSuccess) return false;
     w.gstack_entries = entries;
-    w.P.gstack = reinterpret_cast<int32_t*>(w.gstack);
+    w.P.gstack = reinterpret_cast<int2*>(w.gstack);
     return true;
 }
 
@@ -884,7 +938,7 @@ bool ensure_workspace(Workspace& w, size_t paths, int depth_slots)
     }
     w.capacity = paths;
     w.max_depth_slots = depth_slots;
-    w.P.gstack = reinterpret_cast<int32_t*>(w.gstack);
+    w.P.gstack = reinterpret_cast<int2*>(w.gstack);
     return true;
 }
 
@@ -959,7 +1013,7 @@ int lds_entries_for(const SceneView& V, int requested)
     return std::max(1, std::min(e, V.stack_depth));
 }
 
-size_t trace_lds_bytes(int lds_entries) { return size_t(lds_entries) * kTraceBlock * sizeof(int32_t); }
+size_t trace_lds_bytes(int lds_entries) { return size_t(lds_entries) * kTraceBlock * sizeof(int2); }
 
 // persistent grid of a traversal kernel: resident blocks per CU x CUs
 int trace_grid(const void* kernel, size_t lds_bytes)
@@ -977,9 +1031,9 @@ int trace_grid(const void* kernel, size_t lds_bytes)
     return per_cu * cus;
 }
 
-using BatchKernel = void (*)(SceneView, BatchSrc, int, int*, int32_t*, int, int, unsigned long long*, int*);
-using ClosestKernel = void (*)(SceneView, const int*, int*, ClosestSrc, int32_t*, int, int, unsigned long long*, int*);
-using ShadowKernel = void (*)(SceneView, const int*, int*, ShadowSrc, int32_t*, int, int, unsigned long long*, int*);
+using BatchKernel = void (*)(SceneView, BatchSrc, int, int*, int2*, int, int, unsigned long long*, int*);
+using ClosestKernel = void (*)(SceneView, const int*, int*, ClosestSrc, int2*, int, int, unsigned long long*, int*);
+using ShadowKernel = void (*)(SceneView, const int*, int*, ShadowSrc, int2*, int, int, unsigned long long*, int*);
 
 template <bool ANY>
 BatchKernel batch_kernel_any(bool count, bool full)
