@@ -12,6 +12,8 @@ from ctypes import POINTER, c_char_p, c_double, c_float, c_int32, c_size_t, c_ui
 
 import numpy as np
 
+ABI_VERSION = 2  # BBGPU_ABI_VERSION of include/bbgpu.h that these structures mirror
+
 LIB_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(LIB_DIR, "libbbgpu.so")
 
@@ -23,6 +25,10 @@ BB_MAT_DIELECTRIC = 1
 BB_MAT_CONDUCTOR = 2
 BB_MAT_CONDUCTOR_R = 3
 BB_MAT_METALLIC_ROUGHNESS = 4
+
+BB_FILTER_BOX = 0
+BB_FILTER_TENT = 1
+BB_FILTER_GAUSSIAN = 2
 
 BB_LIGHT_SPOT = 0
 BB_LIGHT_UNIFORM_INFINITE = 1
@@ -140,6 +146,8 @@ class BbCameraDesc(ctypes.Structure):
         ("width", c_int32),
         ("height", c_int32),
         ("filter_extent", c_float),
+        ("filter_type", c_int32),
+        ("filter_sigma", c_float),
     ]
 
 
@@ -236,6 +244,8 @@ def load_library(path: str | None = None):
     lib.bbgpu_look_at.argtypes = [POINTER(c_float), POINTER(c_float), POINTER(c_float), POINTER(c_float)]
     for name in EXPORTS:
         getattr(lib, name).restype = c_int32
+    if lib.bbgpu_abi_version() != ABI_VERSION:
+        raise BbError(f"{p} has ABI version {lib.bbgpu_abi_version()}, expected {ABI_VERSION}: rebuild it")
     if path is None:
         _lib = lib
     return lib

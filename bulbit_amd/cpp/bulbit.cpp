@@ -438,8 +438,8 @@ Rendering* PathIntegrator::Render(Allocator& alloc, const Camera* camera)
 {
     auto* pc = dynamic_cast<const PerspectiveCamera*>(camera);
     if (!pc) fatal("PathIntegrator::Render", "a PerspectiveCamera is required");
-    auto* box = dynamic_cast<const BoxFilter*>(pc->GetFilter());
-    if (!box) fatal("PathIntegrator::Render", "the GPU path supports the BoxFilter only (pass one to the camera)");
+    const Filter* filter = pc->GetFilter();
+    if (!filter) fatal("PathIntegrator::Render", "the camera has no filter");
     if (pc->GetMedium()) fatal("PathIntegrator::Render", "participating media are not supported on the GPU path");
 
     BbCameraDesc cam;
@@ -454,7 +454,17 @@ Rendering* PathIntegrator::Render(Allocator& alloc, const Camera* camera)
     cam.focus_distance = pc->focus_distance;
     cam.width = pc->GetScreenResolution().x;
     cam.height = pc->GetScreenResolution().y;
-    cam.filter_extent = box->extent;
+    cam.filter_extent = filter->extent;
+    if (dynamic_cast<const BoxFilter*>(filter)) {
+        cam.filter_type = BB_FILTER_BOX;
+    } else if (dynamic_cast<const TentFilter*>(filter)) {
+        cam.filter_type = BB_FILTER_TENT;
+    } else if (auto* gf = dynamic_cast<const GaussianFilter*>(filter)) {
+        cam.filter_type = BB_FILTER_GAUSSIAN;
+        cam.filter_sigma = gf->sigma;
+    } else {
+        fatal("PathIntegrator::Render", "filter type not supported on the GPU path");
+    }
 
     const auto* is = static_cast<const IndependentSampler*>(sampler);
     BbRenderParams prm;

@@ -8,7 +8,7 @@
 // path tracer.  The covered surface is exactly the path the GPU library
 // implements: constant textures; Diffuse (Lambertian), Dielectric, Conductor
 // and MetallicRoughness materials; spheres and triangle meshes; spot, uniform
-// infinite and diffuse area lights; the box filter.  Anything outside it is
+// infinite and diffuse area lights; box, tent and Gaussian filters.  Anything outside it is
 // reported by the C ABI as unsupported and stops the program with a message
 // (there is no CPU fallback).
 //
@@ -701,6 +701,21 @@ public:
     explicit BoxFilter(Float extent = 1) : Filter(extent) {}
 };
 
+class TentFilter : public Filter
+{
+public:
+    explicit TentFilter(Float extent = 2) : Filter(extent) {}
+};
+
+// sampled from a 32 x 32 table of the Gaussian (gaussian_filter.cpp:8-43),
+// tabulated by the GPU library exactly as the reference does
+class GaussianFilter : public Filter
+{
+public:
+    explicit GaussianFilter(Float sigma = 0.5f, Float extent = 3) : Filter(extent), sigma{ sigma } {}
+    const Float sigma;
+};
+
 // ------------------------------------------------------------------ cameras
 
 class Camera
@@ -712,6 +727,9 @@ public:
     }
     virtual ~Camera() = default;
 
+    // the reference's default camera filter (include/bulbit/camera.h:48)
+    static inline std::unique_ptr<Filter> default_filter = std::make_unique<GaussianFilter>();
+
     const Point2i& GetScreenResolution() const { return resolution; }
     const Filter* GetFilter() const { return filter; }
     const Medium* GetMedium() const { return medium; }
@@ -722,14 +740,13 @@ protected:
     const Filter* filter;
 };
 
-// PerspectiveCamera (include/bulbit/cameras.h:36-48).  The reference's
-// default filter is a GaussianFilter, which the GPU path does not provide:
-// pass a BoxFilter.
+// PerspectiveCamera (include/bulbit/cameras.h:36-48)
 class PerspectiveCamera : public Camera
 {
 public:
     PerspectiveCamera(const Transform& tf, Float vfov, Float aperture_radius, Float focus_distance,
-                      const Point2i& resolution, const Medium* medium = nullptr, const Filter* filter = nullptr)
+                      const Point2i& resolution, const Medium* medium = nullptr,
+                      const Filter* filter = Camera::default_filter.get())
         : Camera(resolution, medium, filter), transform{ tf }, vfov{ vfov }, aperture_radius{ aperture_radius },
           focus_distance{ focus_distance }
     {

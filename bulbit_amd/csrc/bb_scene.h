@@ -9,6 +9,11 @@
 //                        {c2.min.z, c2.max.xyz} {ref1, ref2, split axis, 0}
 //                      ref >= 0: inner record; ref < 0: leaf whose first
 //                      slot is ~ref (its last slot carries kSlotLast)
+//   inner_box [n_inner] 32 B: the record's own node box {min.xyz, max.x}
+//                      {max.yz, 0, 0}, read when the node is taken off the
+//                      traversal stack
+//   leaf_box [n_slots] 32 B, same form: the box of the leaf that starts at
+//                      this slot (entries of other slots unused)
 //   tris    [n_slots]  3 x float4 = 48 B per BVH slot (leaf order):
 //                      {p0.xyz, prim}, {e1.xyz, sphere index},
 //                      {e2.xyz, kind | kSlotLast on a leaf's last slot}
@@ -55,8 +60,23 @@ enum : int32_t
     MK_LAMBERT = 0,
     MK_DIELECTRIC = 1,
     MK_CONDUCTOR = 2,
-    MK_METALLIC_ROUGHNESS = 3
+    MK_METALLIC_ROUGHNESS = 3,
+    MK_CONDUCTOR_MS = 4 // ConductorMultiScatteringBxDF (energy_compensation)
 };
+
+// Reflectance tables of PrepareReflectanceTextures (microfacet.cpp:239-270):
+// 32 x 32 GGX albedo E(cos_theta, alpha) stored [alpha][cos_theta] and its
+// cosine-weighted average E_avg(alpha); computed once per process.
+constexpr int32_t kTableSize = 32;
+constexpr int32_t kTableSamples = 1024;
+
+struct ReflectanceTables
+{
+    float ggx_E[kTableSize * kTableSize];
+    float ggx_E_avg[kTableSize];
+};
+
+const ReflectanceTables& reflectance_tables();
 
 // Material with its BxDF parameters precomputed on the host
 // (the Material::GetBSDF arithmetic that does not depend on the hit).
@@ -92,7 +112,9 @@ struct GpuLight
 
 struct SceneView
 {
-    const float4* inner; // 4 float4 per internal node
+    const float4* inner;     // 4 float4 per internal node
+    const float4* inner_box; // 2 float4 per internal node
+    const float4* leaf_box;  // 2 float4 per slot
     const float4* tris;  // 3 float4 per slot
     const int4* slot_shade;
     const int32_t* slot_light;
@@ -104,6 +126,8 @@ struct SceneView
     const float* light_cdf;
     const float* light_func;
     const int32_t* inf_lights;
+    const float* ggx_E;     // ReflectanceTables::ggx_E
+    const float* ggx_E_avg; // ReflectanceTables::ggx_E_avg
     int32_t n_lights;
     int32_t n_inf;
     float func_integral;

@@ -207,6 +207,8 @@ struct bxdf
     ggx mf;
     spec r, ceta, ck, color;
     float metallic;
+    const float* E;     // GGX albedo table (compensated conductor)
+    const float* E_avg; // its average over directions
 };
 
 struct bsample
@@ -218,9 +220,11 @@ struct bsample
     float eta;
 };
 
-BB_D bxdf make_bxdf(const GpuMaterial& m, bool front_face)
+BB_D bxdf make_bxdf(const GpuMaterial& m, bool front_face, const float* E, const float* E_avg)
 {
     bxdf b;
+    b.E = E;
+    b.E_avg = E_avg;
     b.kind = m.kind;
     b.flags = m.flags_front;
     b.eta = (m.kind == MK_DIELECTRIC) ? (front_face ? m.eta : 1.0f / m.eta) : 1.0f;
@@ -265,6 +269,70 @@ BB_D float conductor_pdf(const bxdf& b, f3 wo, f3 wi)
     normalize_m(wm);
     if (dot_z(wm) < 0.0f) wm = -wm;
     return ggx_pdf(b.mf, wo, wm) / (4.0f * absdot(wo, wm));
+}
+
+// ImageTexture<Float>::Evaluate, bilinear with TexCoordFilter::clamp
+// (textures.h:60-88, texture.h:26-46)
+BB_D float tex_bilinear(const float* img, int32_t w, int32_t h, float u, float v)
+{
+    const float fw = u * float(w) - 0.5f;
+    const float fh = v * float(h) - 0.5f;
+    int32_t i0 = int32_t(floorf(fw)), i1 = i0 + 1;
+    int32_t j0 = int32_t(floorf(fh)), j1 = j0 + 1;
+    i0 = min(max(i0, 0), w - 1);
+    j0 = min(max(j0, 0), h - 1);
+    i1 = min(max(i1, 0), w - 1);
+    j1 = min(max(j1, 0), h - 1);
+    const float fu = fw - floorf(fw);
+    const float fv = fh - floorf(fh);
+    const float v00 = img[i0 + j0 * w], v10 = img[i1 + j0 * w];
+    const float v01 = img[i0 + j1 * w], v11 = img[i1 + j1 * w];
+    return (1.0f - fu) * (1.0f - fv) * v00 + (1.0f - fu) * fv * v01 + fu * (1.0f - fv) * v10 + fu * fv * v11;
+}
+
+// TrowbridgeReitzDistribution::E / E_avg (microfacet.h:133-145) with
+// GetMeanAlpha = sqrt(alpha_x alpha_y) (microfacet.h:157-160)
+BB_D float ggx_E(const bxdf& b, f3 wo)
+{
+    return tex_bilinear(b.E, kTableSize, kTableSize, wo.z, sqrtf(b.mf.ax * b.mf.ay));
+}
+BB_D float ggx_E_avg(const bxdf& b) { return tex_bilinear(b.E_avg, kTableSize, 1, sqrtf(b.mf.ax * b.mf.ay), 0.0f); }
+
+// Conductor with multiple-scattering compensation,
+// conductor_multi_scattering_bxdf.cpp:7-148; FresnelConductorAverage
+// (scattering.h:174-185)
+BB_D spec conductor_ms_lobe(const bxdf& b, f3 wo, f3 wi)
+{
+    const float E_avg = ggx_E_avg(b);
+    const spec F0 = fresnel_complex_s(1.0f, b.ceta, b.ck);
+    const spec F90 = fresnel_complex_s(0.0f, b.ceta, b.ck);
+    const spec fa = (F0 * 20.0f + F90) / 21.0f;
+    const spec fresnel_ms = ((fa * fa) * E_avg) / (mks(1.0f) - fa * (1.0f - E_avg));
+    return (fresnel_ms * ((1.0f - ggx_E(b, wo)) * (1.0f - ggx_E(b, wi)))) / (kPi * (1.0f - E_avg));
+}
+BB_D spec conductor_ms_f(const bxdf& b, f3 wo, f3 wi)
+{
+    if (!same_hemisphere(wo, wi) || ggx_smooth(b.mf)) return mks(0.0f);
+    float co = abs_cos_theta(wo), ci = abs_cos_theta(wi);
+    if (ci == 0.0f || co == 0.0f) return mks(0.0f);
+    f3 wm = wo + wi;
+    if (length2(wm) == 0.0f) return mks(0.0f);
+    normalize_m(wm);
+    spec F = fresnel_complex_s(absdot(wo, wm), b.ceta, b.ck);
+    spec f_ss = ((F * ggx_D(b.mf, wm)) * ggx_G(b.mf, wo, wi)) / (4.0f * ci * co);
+    return b.r * (f_ss + conductor_ms_lobe(b, wo, wi));
+}
+BB_D float conductor_ms_pdf(const bxdf& b, f3 wo, f3 wi)
+{
+    if (!same_hemisphere(wo, wi) || ggx_smooth(b.mf)) return 0.0f;
+    f3 wm = wo + wi;
+    if (length2(wm) == 0.0f) return 0.0f;
+    normalize_m(wm);
+    if (dot_z(wm) < 0.0f) wm = -wm;
+    const float pdf_ss = ggx_pdf(b.mf, wo, wm) / (4.0f * absdot(wo, wm));
+    const float pdf_ms = wi.z * kInvPi; // CosineHemispherePDF(wi.z)
+    const float e = ggx_E(b, wo);
+    return pdf_ms * (1.0f - e) + pdf_ss * e; // Lerp(pdf_ms, pdf_ss, E(wo)), math.h:87-90
 }
 
 // Dielectric, dielectric_bxdf.cpp:9-129
@@ -350,6 +418,7 @@ BB_D spec bxdf_f(const bxdf& b, f3 wo, f3 wi)
     case MK_LAMBERT: return lambert_f(b, wo, wi);
     case MK_DIELECTRIC: return dielectric_f(b, wo, wi);
     case MK_CONDUCTOR: return conductor_f(b, wo, wi);
+    case MK_CONDUCTOR_MS: return conductor_ms_f(b, wo, wi);
     default: return mr_f(b, wo, wi);
     }
 }
@@ -359,6 +428,7 @@ BB_D float bxdf_pdf(const bxdf& b, f3 wo, f3 wi)
     case MK_LAMBERT: return lambert_pdf(wo, wi);
     case MK_DIELECTRIC: return dielectric_pdf(b, wo, wi);
     case MK_CONDUCTOR: return conductor_pdf(b, wo, wi);
+    case MK_CONDUCTOR_MS: return conductor_ms_pdf(b, wo, wi);
     default: return mr_pdf(b, wo, wi);
     }
 }
@@ -378,6 +448,36 @@ BB_D bool bxdf_sample(const bxdf& b, f3 wo, float u0, f2 u12, bsample& s)
         s.flags = BX_DIFFUSE | BX_REFLECTION;
         return true;
     }
+    case MK_CONDUCTOR_MS:
+        if (!ggx_smooth(b.mf)) {
+            if (wo.z == 0.0f) return false;
+            f3 wm, wi;
+            int32_t flags;
+            const float e = ggx_E(b, wo);
+            if (u0 < e) {
+                wm = ggx_sample_wm(b.mf, wo, u12);
+                wi = reflect(wo, wm);
+                flags = BX_GLOSSY | BX_REFLECTION;
+            } else {
+                wi = sample_cosine_hemisphere(u12);
+                wm = normalize(wo + wi);
+                flags = BX_DIFFUSE | BX_REFLECTION;
+            }
+            if (!same_hemisphere(wo, wi)) return false;
+            const float co = abs_cos_theta(wo), ci = abs_cos_theta(wi);
+            if (ci == 0.0f || co == 0.0f) return false;
+            const spec F = fresnel_complex_s(absdot(wo, wm), b.ceta, b.ck);
+            const spec f_ss = ((F * ggx_D(b.mf, wm)) * ggx_G(b.mf, wo, wi)) / (4.0f * ci * co);
+            const spec f_ms = conductor_ms_lobe(b, wo, wi);
+            const float pdf_ss = ggx_pdf(b.mf, wo, wm) / (4.0f * absdot(wo, wm));
+            const float pdf_ms = wi.z * kInvPi;
+            s.f = b.r * (f_ss + f_ms);
+            s.wi = wi;
+            s.pdf = pdf_ms * (1.0f - e) + pdf_ss * e;
+            s.flags = flags;
+            return true;
+        }
+        [[fallthrough]]; // effectively smooth: the perfect specular conductor
     case MK_CONDUCTOR: {
         if (ggx_smooth(b.mf)) {
             f3 wi = mk3(-wo.x, -wo.y, wo.z);

@@ -366,12 +366,9 @@ bool material_to_gpu(const BbMaterial& m, GpuMaterial& g, std::string& err)
     }
     case BB_MAT_CONDUCTOR:
     case BB_MAT_CONDUCTOR_R: {
-        // ConductorMaterial::GetBSDF (conductor_material.cpp:52-99), ConductorBxDF (bxdfs.h:253-277)
-        if (m.energy_compensation) {
-            err = "energy-compensated (multi-scattering) conductor is not supported";
-            return false;
-        }
-        g.kind = MK_CONDUCTOR;
+        // ConductorMaterial::GetBSDF (conductor_material.cpp:52-99): ConductorBxDF (bxdfs.h:253-277),
+        // or ConductorMultiScatteringBxDF (bxdfs.h:303-355) with energy compensation
+        g.kind = m.energy_compensation ? MK_CONDUCTOR_MS : MK_CONDUCTOR;
         g.ax = sqr(m.u_roughness);
         g.ay = sqr(m.v_roughness);
         g.r = mks(m.color[0], m.color[1], m.color[2]);
@@ -568,7 +565,13 @@ bool create_scene(const BbSceneDesc& d, BbScene* s, std::string& err)
         }
     auto ref_of = [&](size_t c) { return nodes[c].count == 0 ? inner_index[c] : ~nodes[c].offset; };
     std::vector<float4> inner(size_t(4) * std::max(1, n_inner));
+    std::vector<float4> inner_box(size_t(2) * std::max(1, n_inner));
+    std::vector<float4> leaf_box(size_t(2) * std::max(1, ns), make_float4(0.0f, 0.0f, 0.0f, 0.0f));
     for (size_t i = 0; i < nodes.size(); ++i) {
+        const BbBvhNode& nd = nodes[i];
+        float4* bx = nd.count == 0 ? &inner_box[size_t(2) * inner_index[i]] : &leaf_box[size_t(2) * nd.offset];
+        bx[0] = make_float4(nd.bmin[0], nd.bmin[1], nd.bmin[2], nd.bmax[0]);
+        bx[1] = make_float4(nd.bmax[1], nd.bmax[2], 0.0f, 0.0f);
         if (nodes[i].count != 0) continue;
         const BbBvhNode& c1 = nodes[i + 1];
         const BbBvhNode& c2 = nodes[size_t(nodes[i].offset)];
@@ -602,6 +605,8 @@ bool create_scene(const BbSceneDesc& d, BbScene* s, std::string& err)
     SceneView& V = s->view;
     hipGetDevice(&s->device);
     if (!upload(s, inner.data(), inner.size(), &V.inner, err)) return false;
+    if (!upload(s, inner_box.data(), inner_box.size(), &V.inner_box, err)) return false;
+    if (!upload(s, leaf_box.data(), leaf_box.size(), &V.leaf_box, err)) return false;
     for (int k = 0; k < 3; ++k) {
         V.root_box[k] = nodes[0].bmin[k];
         V.root_box[3 + k] = nodes[0].bmax[k];
@@ -622,6 +627,9 @@ bool create_scene(const BbSceneDesc& d, BbScene* s, std::string& err)
     if (!upload(s, func.data(), func.size(), &V.light_func, err)) return false;
     inf.resize(std::max<size_t>(1, inf.size()));
     if (!upload(s, inf.data(), inf.size(), &V.inf_lights, err)) return false;
+    const ReflectanceTables& tab = reflectance_tables();
+    if (!upload(s, tab.ggx_E, size_t(kTableSize) * kTableSize, &V.ggx_E, err)) return false;
+    if (!upload(s, tab.ggx_E_avg, size_t(kTableSize), &V.ggx_E_avg, err)) return false;
     V.n_lights = nl;
     V.n_inf = s->n_inf;
     V.func_integral = func_integral;

@@ -139,15 +139,16 @@ struct TraceCount
 };
 
 // Per-lane traversal state of BVH::RayCast (include/bulbit/bvh.h:101-155).
-// `cur` is the node to expand next, whose own box test has passed: an inner
-// record (>= 0), a leaf (~first slot), or kNoRef (take the next stack entry).
+// `cur` is the node to expand next: an inner record (>= 0), a leaf (~first
+// slot), or kNoRef (take the next stack entry).  `pend` is set when `cur`
+// came off the stack and its own box test is still to be done.
 constexpr int32_t kNoRef = INT32_MIN;
 
 struct Lane
 {
     f3 o, d, inv, dn;
     float l, t_min, t_max;
-    bool n0, n1, n2;
+    bool n0, n1, n2, pend;
     int32_t sp, cur;
     int32_t hit_slot;
     float hit_t, hit_u, hit_v;
@@ -167,6 +168,7 @@ BB_D void lane_start(Lane& L, f3 o, f3 d, float t_min, float t_max)
     L.l = normalize_m(L.dn);
     L.sp = 0;
     L.cur = kNoRef;
+    L.pend = false;
     L.hit_slot = -1;
     L.hit_t = L.hit_u = L.hit_v = 0.0f;
 }
@@ -182,14 +184,14 @@ BB_D void lane_root(const SceneView& S, Lane& L, TraceCount& c)
     L.cur = box_pass(h, L.t_max) ? S.root_ref : kNoRef;
 }
 
-// Traversal stack of (node reference, entry distance) pairs: entries
-// [0, lds_entries) in LDS ([entry][thread] layout, conflict-free for a
-// wave), deeper entries in a global overflow area ([entry][global thread]);
-// cap = BVH depth + 2 bounds every push.
+// Traversal stack of node references: entries [0, lds_entries) in LDS
+// ([entry][thread] layout, conflict-free for a wave), deeper entries in a
+// global overflow area ([entry][global thread]); cap = BVH depth + 2 bounds
+// every push.
 struct TraceStack
 {
-    int2* lds;
-    int2* gbl;
+    int32_t* lds;
+    int32_t* gbl;
     int32_t lds_entries;
     int32_t gstride;
     int32_t cap;
@@ -197,7 +199,7 @@ struct TraceStack
 
 // FULL: the whole stack fits in LDS (cap <= lds_entries), no overflow branch
 template <bool FULL>
-BB_D void stack_put(const TraceStack& s, int32_t sp, int2 v)
+BB_D void stack_put(const TraceStack& s, int32_t sp, int32_t v)
 {
     if (FULL || sp < s.lds_entries)
         s.lds[sp * kTraceBlock] = v;
@@ -206,63 +208,81 @@ BB_D void stack_put(const TraceStack& s, int32_t sp, int2 v)
 }
 
 template <bool FULL>
-BB_D int2 stack_get(const TraceStack& s, int32_t sp)
+BB_D int32_t stack_get(const TraceStack& s, int32_t sp)
 {
     return (FULL || sp < s.lds_entries) ? s.lds[sp * kTraceBlock] : s.gbl[(sp - s.lds_entries) * s.gstride];
 }
 
-// a far child whose t_max-independent test failed, kept on the stack only
-// when counting so that it is counted when RayCast would pop and test it
-// (a signalling-NaN bit pattern: no arithmetic result has it)
-constexpr int32_t kFailedLo = 0x7fbfffff;
-
 // One step of RayCast for one lane.
 //
-// Expanding an inner record tests both children's boxes from the parent's
-// 64 bytes.  RayCast pushes the far child, then tests the near child with
-// the current t_max: the near test here is that same test.  The far child is
-// tested by RayCast when popped, with the t_max of that moment; here its
-// t_max-independent part and entry distance are computed now, a child that
-// fails already is not pushed (it would fail then too), and the pop applies
-// the remaining `lo > t_max` check with the then-current t_max.  So the same
-// nodes are expanded in the same order with the same outcomes, and leaves
+// A node taken off the stack is box-tested with the current t_max, as
+// RayCast does at pop time; its own box comes from inner_box / leaf_box in
+// the same round trip as its record.  Expanding an inner record then tests
+// both children's boxes from the parent's 64 bytes: RayCast pushes the far
+// child and tests the near child with the current t_max, and the near test
+// here is that same test.  The far child is pushed only if it passes with
+// the current t_max (a box that fails fails again for any smaller t_max, so
+// RayCast would discard it when popped); with COUNT every far child is
+// pushed so that its pop-time test is counted as RayCast counts it.  Leaves
 // test their primitives in stored order (the callbacks of BVH::Intersect /
-// IntersectAny, src/accel/bvh.cpp:299-361).  With COUNT, every far child is
-// pushed and counted when popped, which reproduces RayCast's node-test count.
-// Returns true when the lane's query is finished.
+// IntersectAny, src/accel/bvh.cpp:299-361).  Returns true when the lane's
+// query is finished.
 template <bool ANY, bool COUNT, bool FULL>
 BB_D bool trace_step(const SceneView& S, Lane& L, const TraceStack& stk, TraceCount& c, int* err)
 {
     if (L.cur >= 0) {
         const float4* r = S.inner + 4 * L.cur;
         const float4 a = r[0], b = r[1], e = r[2], m = r[3];
-        // the record's 64 bytes in one round trip
-        asm volatile("" ::"v"(a.x), "v"(a.y), "v"(a.z), "v"(a.w), "v"(b.x), "v"(b.y), "v"(b.z), "v"(b.w), "v"(e.x),
-                     "v"(e.y), "v"(e.z), "v"(e.w), "v"(m.x), "v"(m.y), "v"(m.z));
-        const BoxHit h1 = box_part(mk3(a.x, a.y, a.z), mk3(a.w, b.x, b.y), L.o, L.t_min, L.inv, L.n0, L.n1, L.n2);
-        const BoxHit h2 = box_part(mk3(b.z, b.w, e.x), mk3(e.y, e.z, e.w), L.o, L.t_min, L.inv, L.n0, L.n1, L.n2);
-        const int32_t axis = __float_as_int(m.z);
-        const bool neg = axis == 0 ? L.n0 : (axis == 1 ? L.n1 : L.n2);
-        const int32_t near_ref = __float_as_int(neg ? m.y : m.x);
-        const int32_t far_ref = __float_as_int(neg ? m.x : m.y);
-        const BoxHit hn = neg ? h2 : h1;
-        const BoxHit hf = neg ? h1 : h2;
-        const bool push = COUNT || box_pass(hf, L.t_max);
-        if (push) {
-            if (L.sp >= stk.cap) {
-                atomicOr(err, 1);
-                return true;
-            }
-            stack_put<FULL>(stk, L.sp, make_int2(far_ref, hf.ok ? __float_as_int(hf.lo) : kFailedLo));
-            ++L.sp;
+        float4 ob0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), ob1 = ob0;
+        if (L.pend) {
+            ob0 = S.inner_box[2 * L.cur];
+            ob1 = S.inner_box[2 * L.cur + 1];
         }
-        if (COUNT) c.nodes++;
-        if (box_pass(hn, L.t_max)) {
-            L.cur = near_ref;
-            return false;
+        // the record's 64 bytes (and the own box) in one round trip
+        asm volatile("" ::"v"(a.x), "v"(a.y), "v"(a.z), "v"(a.w), "v"(b.x), "v"(b.y), "v"(b.z), "v"(b.w), "v"(e.x),
+                     "v"(e.y), "v"(e.z), "v"(e.w), "v"(m.x), "v"(m.y), "v"(m.z), "v"(ob0.x), "v"(ob0.y), "v"(ob0.z),
+                     "v"(ob0.w), "v"(ob1.x), "v"(ob1.y));
+        bool go = true;
+        if (L.pend) {
+            if (COUNT) c.nodes++;
+            go = box_pass(box_part(mk3(ob0.x, ob0.y, ob0.z), mk3(ob0.w, ob1.x, ob1.y), L.o, L.t_min, L.inv, L.n0,
+                                   L.n1, L.n2),
+                          L.t_max);
+        }
+        if (go) {
+            const BoxHit h1 = box_part(mk3(a.x, a.y, a.z), mk3(a.w, b.x, b.y), L.o, L.t_min, L.inv, L.n0, L.n1, L.n2);
+            const BoxHit h2 = box_part(mk3(b.z, b.w, e.x), mk3(e.y, e.z, e.w), L.o, L.t_min, L.inv, L.n0, L.n1, L.n2);
+            const int32_t axis = __float_as_int(m.z);
+            const bool neg = axis == 0 ? L.n0 : (axis == 1 ? L.n1 : L.n2);
+            const int32_t near_ref = __float_as_int(neg ? m.y : m.x);
+            const int32_t far_ref = __float_as_int(neg ? m.x : m.y);
+            const BoxHit hn = neg ? h2 : h1;
+            const BoxHit hf = neg ? h1 : h2;
+            if (COUNT || box_pass(hf, L.t_max)) {
+                if (L.sp >= stk.cap) {
+                    atomicOr(err, 1);
+                    return true;
+                }
+                stack_put<FULL>(stk, L.sp, far_ref);
+                ++L.sp;
+            }
+            if (COUNT) c.nodes++;
+            if (box_pass(hn, L.t_max)) {
+                L.cur = near_ref;
+                L.pend = false;
+                return false;
+            }
         }
     } else if (L.cur != kNoRef) {
-        for (int32_t slot = ~L.cur;; ++slot) {
+        bool go = true;
+        if (L.pend) {
+            const float4 lb0 = S.leaf_box[2 * ~L.cur], lb1 = S.leaf_box[2 * ~L.cur + 1];
+            if (COUNT) c.nodes++;
+            go = box_pass(box_part(mk3(lb0.x, lb0.y, lb0.z), mk3(lb0.w, lb1.x, lb1.y), L.o, L.t_min, L.inv, L.n0,
+                                   L.n1, L.n2),
+                          L.t_max);
+        }
+        for (int32_t slot = ~L.cur; go; ++slot) {
             const float4 t0 = S.tris[3 * slot];
             const float4 t1 = S.tris[3 * slot + 1];
             const float4 t2 = S.tris[3 * slot + 2];
@@ -293,14 +313,11 @@ BB_D bool trace_step(const SceneView& S, Lane& L, const TraceStack& stk, TraceCo
             if (kind & kSlotLast) break;
         }
     }
-    // take the next stack entry; its deferred `lo > t_max` check decides
-    // whether it is expanded
+    // take the next stack entry; its box is tested when it is expanded
     if (L.sp == 0) return true;
     --L.sp;
-    const int2 e = stack_get<FULL>(stk, L.sp);
-    if (COUNT) c.nodes++;
-    const bool pass = (!COUNT || e.y != kFailedLo) && !(__int_as_float(e.y) > L.t_max);
-    L.cur = pass ? e.x : kNoRef;
+    L.cur = stack_get<FULL>(stk, L.sp);
+    L.pend = true;
     return false;
 }
 
@@ -403,7 +420,7 @@ struct BatchSrc
     }
 };
 
-BB_D TraceStack make_stack(int2* lds_stack, int2* gstack, int lds_entries, int cap)
+BB_D TraceStack make_stack(int32_t* lds_stack, int32_t* gstack, int lds_entries, int cap)
 {
     TraceStack t;
     t.lds = lds_stack + threadIdx.x;
@@ -421,10 +438,10 @@ constexpr int kTraceSteps = BB_TRACE_STEPS;
 
 template <bool ANY, bool COUNT, bool FULL>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_batch(SceneView S, BatchSrc src, int n, int* fetch,
-                                                               int2* gstack, int lds_entries, int refill_min,
+                                                               int32_t* gstack, int lds_entries, int refill_min,
                                                                unsigned long long* counters, int* err)
 {
-    extern __shared__ int2 lds_stack[];
+    extern __shared__ int32_t lds_stack[];
     const TraceStack stk = make_stack(lds_stack, gstack, lds_entries, S.stack_depth);
     TraceCount c{ 0, 0, 0 };
     persistent_trace<ANY, COUNT, FULL, kTraceSteps>(S, src, n, fetch, stk, c, err, refill_min);
@@ -433,12 +450,57 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_batch(SceneView S, BatchS
 
 // -------------------------------------------------------- path tracer state
 
+constexpr int kFilterN = 32; // GaussianFilter table size (gaussian_filter.cpp:10)
+
 struct GpuCamera
 {
     f3 origin, lower_left, horizontal, vertical, u, v;
     float lens_radius, filter_extent;
     int32_t width, height;
+    int32_t filter_type;
+    const float* filter_cdf; // GAUSSIAN: marginal CDF, then 32 conditional CDFs (33 floats each)
 };
+
+// Distribution1D::SampleContinuous (sampling.h:381-398) over a CDF of n + 1
+// entries, with FindInterval (math.h:117-135)
+BB_D float cdf_sample(const float* cdf, int32_t n, float u, int32_t& off)
+{
+    int32_t first = 0, len = n + 1;
+    while (len > 0) {
+        const int32_t half = len >> 1, middle = first + half;
+        if (cdf[middle] <= u) {
+            first = middle + 1;
+            len -= half + 1;
+        } else {
+            len = half;
+        }
+    }
+    off = min(max(first - 1, 0), n - 1);
+    const float du = (u - cdf[off]) / (cdf[off + 1] - cdf[off]);
+    return (float(off) + du) / float(n);
+}
+
+// Filter::Sample: box_filter.cpp:17-21, tent_filter.cpp:22-30,
+// GaussianFilter (gaussian_filter.cpp:40-43 over Distribution2D::
+// SampleContinuous, sampling.h:447-461)
+BB_D f2 filter_sample(const GpuCamera& cam, f2 u)
+{
+    const float e = cam.filter_extent;
+    if (cam.filter_type == BB_FILTER_TENT) {
+        const float h = e / 2.0f;
+        const float x = u.x < 0.5f ? h * (sqrtf(2.0f * u.x) - 1.0f) : h * (1.0f - sqrtf(1.0f - 2.0f * (u.x - 0.5f)));
+        const float y = u.y < 0.5f ? h * (sqrtf(2.0f * u.y) - 1.0f) : h * (1.0f - sqrtf(1.0f - 2.0f * (u.y - 0.5f)));
+        return mk2(x, y);
+    }
+    if (cam.filter_type == BB_FILTER_GAUSSIAN) {
+        int32_t v = 0, unused = 0;
+        const float d1 = cdf_sample(cam.filter_cdf, kFilterN, u.y, v);
+        const float d0 = cdf_sample(cam.filter_cdf + (kFilterN + 1) * (1 + v), kFilterN, u.x, unused);
+        const float h = e / 2.0f;
+        return mk2(d0 * e - h, d1 * e - h);
+    }
+    return mk2((u.x * 2.0f - 1.0f) * e / 2.0f, (u.y * 2.0f - 1.0f) * e / 2.0f);
+}
 
 struct PathBufs
 {
@@ -452,7 +514,7 @@ struct PathBufs
     float4* sray;    // 2 float4 per shadow slot: (o, t_max), (d, path id)
     float4* scontrib;
     int* counts;     // [depth][4]: active paths, shadow rays, closest fetch, shadow fetch
-    int2* gstack;    // traversal stack overflow, [entry][persistent thread]
+    int32_t* gstack; // traversal stack overflow, [entry][persistent thread]
     int* err;
     unsigned long long* counters; // [6]: closest nodes/prims/accepts, shadow nodes/prims/accepts
 };
@@ -465,8 +527,8 @@ struct RenderConsts
 // UniDirectionalRayIntegrator::Render per (pixel, sample) start
 // (integrator.cpp:187-190): seed, then SampleRay(Next2D(), Next2D()) with the
 // lens sample u1 drawn first (argument evaluation of the reference's compiler),
-// PerspectiveCamera::SampleRay (perspective_camera.cpp:40-52) with
-// BoxFilter::Sample (box_filter.cpp:17-21).
+// PerspectiveCamera::SampleRay (perspective_camera.cpp:40-52) with the
+// camera's Filter::Sample.
 __global__ __launch_bounds__(kShadeBlock) void k_raygen(GpuCamera cam, RenderConsts rc, PathBufs P)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -484,7 +546,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_raygen(GpuCamera cam, RenderCon
     f2 u0;
     u0.x = pcg_float(st);
     u0.y = pcg_float(st);
-    f2 off = mk2((u0.x * 2.0f - 1.0f) * cam.filter_extent / 2.0f, (u0.y * 2.0f - 1.0f) * cam.filter_extent / 2.0f);
+    f2 off = filter_sample(cam, u0);
     off = mk2(off.x + 0.5f, off.y + 0.5f);
     f3 pixel_center = (cam.lower_left + (cam.horizontal * (float(px) + off.x)) / float(cam.width)) +
                       (cam.vertical * (float(py) + off.y)) / float(cam.height);
@@ -545,10 +607,10 @@ struct ShadowSrc
 
 template <bool COUNT, bool FULL>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(SceneView S, const int* __restrict__ n_ptr, int* fetch,
-                                                                 ClosestSrc src, int2* gstack, int lds_entries, int refill_min,
+                                                                 ClosestSrc src, int32_t* gstack, int lds_entries, int refill_min,
                                                                  unsigned long long* counters, int* err)
 {
-    extern __shared__ int2 lds_stack[];
+    extern __shared__ int32_t lds_stack[];
     const TraceStack stk = make_stack(lds_stack, gstack, lds_entries, S.stack_depth);
     TraceCount c{ 0, 0, 0 };
     persistent_trace<false, COUNT, FULL, kTraceSteps>(S, src, *n_ptr, fetch, stk, c, err, refill_min);
@@ -557,10 +619,10 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(SceneView S, cons
 
 template <bool COUNT, bool FULL>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_shadow(SceneView S, const int* __restrict__ n_ptr, int* fetch,
-                                                                ShadowSrc src, int2* gstack, int lds_entries, int refill_min,
+                                                                ShadowSrc src, int32_t* gstack, int lds_entries, int refill_min,
                                                                 unsigned long long* counters, int* err)
 {
-    extern __shared__ int2 lds_stack[];
+    extern __shared__ int32_t lds_stack[];
     const TraceStack stk = make_stack(lds_stack, gstack, lds_entries, S.stack_depth);
     TraceCount c{ 0, 0, 0 };
     persistent_trace<true, COUNT, FULL, kTraceSteps>(S, src, *n_ptr, fetch, stk, c, err, refill_min);
@@ -688,7 +750,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView S, RenderConsts
                         new_d = -wo;
                         cont = true;
                     } else {
-                        const bxdf b = make_bxdf(S.materials[mi], sh.front);
+                        const bxdf b = make_bxdf(S.materials[mi], sh.front, S.ggx_E, S.ggx_E_avg);
                         const frame3 fr = frame_from_xz(sh.ts, sh.ns);
                         const f3 wo_l = to_local(fr, wo);
                         if (b.flags & (BX_DIFFUSE | BX_GLOSSY)) {
@@ -835,6 +897,8 @@ struct Workspace
 struct SceneExtra
 {
     Workspace ws;
+    float* filter_cdf = nullptr; // device copy of the Gaussian filter's CDFs
+    float filter_sigma = 0.0f, filter_extent = 0.0f;
 };
 
 std::vector<std::pair<BbScene*, SceneExtra*>> g_extras;
@@ -898,21 +962,21 @@ bool device_accessible(const void* p)
     return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged || a.isManaged;
 }
 
-// global overflow area of the traversal stack: (cap - lds_entries) int2 entries
+// global overflow area of the traversal stack: (cap - lds_entries) entries
 // for every thread of the persistent grid
 bool ensure_gstack(Workspace& w, size_t entries)
 {
     entries = std::max<size_t>(entries, 64);
     if (w.gstack_entries >= entries) {
-        w.P.gstack = reinterpret_cast<int2*>(w.gstack);
+        w.P.gstack = reinterpret_cast<int32_t*>(w.gstack);
         return true;
     }
     if (w.gstack) hipFree(w.gstack);
     w.gstack = nullptr;
     w.gstack_entries = 0;
-    if (hipMalloc(&w.gstack, entries * sizeof(int2)) != hipSuccess) return false;
+    if (hipMalloc(&w.gstack, entries * sizeof(int32_t)) != hipSuccess) return false;
     w.gstack_entries = entries;
-    w.P.gstack = reinterpret_cast<int2*>(w.gstack);
+    w.P.gstack = reinterpret_cast<int32_t*>(w.gstack);
     return true;
 }
 
@@ -938,7 +1002,7 @@ bool ensure_workspace(Workspace& w, size_t paths, int depth_slots)
     }
     w.capacity = paths;
     w.max_depth_slots = depth_slots;
-    w.P.gstack = reinterpret_cast<int2*>(w.gstack);
+    w.P.gstack = reinterpret_cast<int32_t*>(w.gstack);
     return true;
 }
 
@@ -974,7 +1038,42 @@ GpuCamera make_camera(const BbCameraDesc& c)
     g.filter_extent = c.filter_extent;
     g.width = c.width;
     g.height = c.height;
+    g.filter_type = c.filter_type;
+    g.filter_cdf = nullptr;
     return g;
+}
+
+// GaussianFilter::GaussianFilter (gaussian_filter.cpp:8-27) and the CDFs of
+// its Distribution2D (sampling.h:343-379, 426-445), computed on the host:
+// out = marginal CDF, then the 32 conditional CDFs, 33 floats each
+void gaussian_filter_cdfs(float sigma, float extent, float* out)
+{
+    const int n = kFilterN;
+    auto gaussian = [](float x, float sg) {
+        // Gaussian(x, 0, sigma) (sampling.h:9-12)
+        const float xm = x - 0.0f;
+        return 1.0f / std::sqrt(2.0f * kPi * sg * sg) * std::exp(-(xm * xm) / (2.0f * sg * sg));
+    };
+    auto dist1d = [n](const float* f, float* cdf) {
+        cdf[0] = 0.0f;
+        for (int i = 1; i < n + 1; ++i) cdf[i] = cdf[i - 1] + f[i - 1] / float(n);
+        const float integral = cdf[n];
+        if (integral == 0.0f)
+            for (int i = 1; i < n + 1; ++i) cdf[i] = float(i) / float(n);
+        else
+            for (int i = 1; i < n + 1; ++i) cdf[i] /= integral;
+        return integral;
+    };
+    std::vector<float> values(size_t(n) * n), marginal(n);
+    const float half_extent = extent / 2.0f;
+    for (int y = 0; y < n; ++y)
+        for (int x = 0; x < n; ++x) {
+            const float px = ((float(x) + 0.5f) / float(n)) * extent - half_extent;
+            const float py = ((float(y) + 0.5f) / float(n)) * extent - half_extent;
+            values[size_t(y) * n + x] = gaussian(px, sigma) * gaussian(py, sigma);
+        }
+    for (int v = 0; v < n; ++v) marginal[v] = dist1d(&values[size_t(v) * n], out + (n + 1) * (1 + v));
+    dist1d(marginal.data(), out);
 }
 
 #define HIP_OK(call)                                                                                                   \
@@ -1013,7 +1112,7 @@ int lds_entries_for(const SceneView& V, int requested)
     return std::max(1, std::min(e, V.stack_depth));
 }
 
-size_t trace_lds_bytes(int lds_entries) { return size_t(lds_entries) * kTraceBlock * sizeof(int2); }
+size_t trace_lds_bytes(int lds_entries) { return size_t(lds_entries) * kTraceBlock * sizeof(int32_t); }
 
 // persistent grid of a traversal kernel: resident blocks per CU x CUs
 int trace_grid(const void* kernel, size_t lds_bytes)
@@ -1031,9 +1130,9 @@ int trace_grid(const void* kernel, size_t lds_bytes)
     return per_cu * cus;
 }
 
-using BatchKernel = void (*)(SceneView, BatchSrc, int, int*, int2*, int, int, unsigned long long*, int*);
-using ClosestKernel = void (*)(SceneView, const int*, int*, ClosestSrc, int2*, int, int, unsigned long long*, int*);
-using ShadowKernel = void (*)(SceneView, const int*, int*, ShadowSrc, int2*, int, int, unsigned long long*, int*);
+using BatchKernel = void (*)(SceneView, BatchSrc, int, int*, int32_t*, int, int, unsigned long long*, int*);
+using ClosestKernel = void (*)(SceneView, const int*, int*, ClosestSrc, int32_t*, int, int, unsigned long long*, int*);
+using ShadowKernel = void (*)(SceneView, const int*, int*, ShadowSrc, int32_t*, int, int, unsigned long long*, int*);
 
 template <bool ANY>
 BatchKernel batch_kernel_any(bool count, bool full)
@@ -1142,6 +1241,13 @@ int bbgpu_render(BbScene* scene, const BbCameraDesc* camera, const BbRenderParam
         bb::set_error("bbgpu_render: invalid camera or parameters");
         return BBGPU_ERR_INVALID;
     }
+    if (camera->filter_type < BB_FILTER_BOX || camera->filter_type > BB_FILTER_GAUSSIAN ||
+        !(camera->filter_extent > 0.0f) || !std::isfinite(camera->filter_extent) ||
+        (camera->filter_type == BB_FILTER_GAUSSIAN && !(camera->filter_sigma > 0.0f))) {
+        bb::set_error("bbgpu_render: invalid filter (type %d, extent %g, sigma %g)", camera->filter_type,
+                      double(camera->filter_extent), double(camera->filter_sigma));
+        return BBGPU_ERR_INVALID;
+    }
     if (p.regularize_bsdf) {
         bb::set_error("bbgpu_render: regularize_bsdf is not supported");
         return BBGPU_ERR_UNSUPPORTED;
@@ -1191,7 +1297,23 @@ int bbgpu_render(BbScene* scene, const BbCameraDesc* camera, const BbRenderParam
         return BBGPU_ERR_NOMEM;
     }
     PathBufs& P = x->ws.P;
-    const GpuCamera cam = make_camera(*camera);
+    GpuCamera cam = make_camera(*camera);
+    if (camera->filter_type == BB_FILTER_GAUSSIAN) {
+        if (!x->filter_cdf || x->filter_sigma != camera->filter_sigma || x->filter_extent != camera->filter_extent) {
+            std::vector<float> cdf(size_t(kFilterN + 1) * (kFilterN + 1));
+            gaussian_filter_cdfs(camera->filter_sigma, camera->filter_extent, cdf.data());
+            if (!x->filter_cdf && hipMalloc(&x->filter_cdf, sizeof(float) * cdf.size()) != hipSuccess) {
+                x->filter_cdf = nullptr;
+                bb::set_error("bbgpu_render: filter table allocation failed");
+                return BBGPU_ERR_NOMEM;
+            }
+            HIP_OK(hipMemcpyAsync(x->filter_cdf, cdf.data(), sizeof(float) * cdf.size(), hipMemcpyHostToDevice, st));
+            HIP_OK(hipStreamSynchronize(st));
+            x->filter_sigma = camera->filter_sigma;
+            x->filter_extent = camera->filter_extent;
+        }
+        cam.filter_cdf = x->filter_cdf;
+    }
     // a film in host memory is copied in, accumulated on the device and copied back
     float* const film_sum_user = film_sum_dev;
     int32_t* const film_count_user = film_count_dev;
@@ -1333,6 +1455,7 @@ void release_scene_workspace(BbScene* s)
     for (size_t i = 0; i < g_extras.size(); ++i)
         if (g_extras[i].first == s) {
             free_workspace(g_extras[i].second->ws);
+            if (g_extras[i].second->filter_cdf) hipFree(g_extras[i].second->filter_cdf);
             delete g_extras[i].second;
             g_extras.erase(g_extras.begin() + long(i));
             return;

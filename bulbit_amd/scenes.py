@@ -89,6 +89,19 @@ def make_camera(position, target, vfov, aperture, focus, width, height, filter_e
     return cam
 
 
+def set_filter(cam, kind: str, extent=None, sigma=0.5):
+    """The camera's filter: "box" (extent 1), "tent" (extent 2) or
+    "gaussian" (sigma 0.5, extent 3), with the reference's defaults
+    (include/bulbit/filters.h)."""
+    types = {"box": (capi.BB_FILTER_BOX, 1.0), "tent": (capi.BB_FILTER_TENT, 2.0),
+             "gaussian": (capi.BB_FILTER_GAUSSIAN, 3.0)}
+    t, default_extent = types[kind]
+    cam.filter_type = t
+    cam.filter_extent = default_extent if extent is None else extent
+    cam.filter_sigma = sigma
+    return cam
+
+
 def diffuse(color):
     m = capi.BbMaterial()
     m.type = capi.BB_MAT_DIFFUSE

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define BBGPU_ABI_VERSION 1
+#define BBGPU_ABI_VERSION 2
 
 /* status codes */
 enum {
@@ -184,10 +184,17 @@ typedef struct BbTraceCounters {
 
 /* ------------------------------------------------------------- render -- */
 
+/* The camera's reconstruction filter (include/bulbit/filters.h) */
+enum {
+    BB_FILTER_BOX = 0,      /* BoxFilter(extent = 1)       src/reconstruction/box_filter.cpp:17-21 */
+    BB_FILTER_TENT = 1,     /* TentFilter(extent = 2)      src/reconstruction/tent_filter.cpp:22-30 */
+    BB_FILTER_GAUSSIAN = 2, /* GaussianFilter(sigma = 0.5, extent = 3)  gaussian_filter.cpp:8-43 */
+};
+
 /* PerspectiveCamera(const Transform&, vfov, aperture_radius, focus_distance,
- * resolution, medium, filter) with a BoxFilter(extent):
- * src/camera/perspective_camera.cpp:8-38, src/reconstruction/box_filter.cpp:17-21.
- * The transform's scale does not enter the camera. */
+ * resolution, medium, filter): src/camera/perspective_camera.cpp:8-52.
+ * The transform's scale does not enter the camera.  A zero-filled tail
+ * (filter_type 0) is the box filter. */
 typedef struct BbCameraDesc {
     float position[3];
     float orientation[4]; /* quaternion x, y, z, w */
@@ -196,7 +203,9 @@ typedef struct BbCameraDesc {
     float focus_distance;
     int32_t width;
     int32_t height;
-    float filter_extent;  /* BoxFilter extent (default 1) */
+    float filter_extent;  /* the filter's extent */
+    int32_t filter_type;  /* BB_FILTER_* */
+    float filter_sigma;   /* GAUSSIAN: sigma */
 } BbCameraDesc;
 
 /* PathIntegrator(accel, lights, sampler, max_bounces, rr_min_bounces,

@@ -45,6 +45,14 @@ def load_oracle():
     lib.orc_fresnel_complex.restype = c_float
     lib.orc_fresnel_complex.argtypes = [c_float, c_float, c_float]
     lib.orc_look_at.argtypes = [POINTER(c_float)] * 4
+    lib.orc_permutation_element.restype = c_int32
+    lib.orc_permutation_element.argtypes = [c_uint32, c_uint32, c_uint32]
+    lib.orc_reflectance_tables.argtypes = [POINTER(c_float)] * 4
+    lib.orc_ggx_E.restype = c_float
+    lib.orc_ggx_E.argtypes = [c_float, c_float, POINTER(c_float)]
+    lib.orc_ggx_E_avg.restype = c_float
+    lib.orc_ggx_E_avg.argtypes = [c_float, c_float]
+    lib.orc_filter_sample.argtypes = [POINTER(capi.BbCameraDesc), POINTER(c_float), POINTER(c_float)]
     lib.orc_camera_ray.argtypes = [POINTER(capi.BbCameraDesc), c_int32, c_int32, POINTER(c_float), POINTER(c_float), c_void_p]
     lib.orc_bsdf_eval.restype = c_int32
     lib.orc_bsdf_eval.argtypes = [POINTER(capi.BbMaterial), POINTER(c_float), POINTER(c_float), c_int32,

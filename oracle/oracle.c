@@ -286,11 +286,112 @@ void orc_look_at(const float position[3], const float target[3], const float up_
 
 /* ================================================================= camera */
 
+/* Distribution1D over n <= ORC_FILTER_N values (sampling.h:343-379) */
+#define ORC_FILTER_N 32
+typedef struct {
+    float func[ORC_FILTER_N], cdf[ORC_FILTER_N + 1], func_integral;
+    int32_t n;
+} dist1d_t;
+
+static void dist1d_init(dist1d_t* d, const float* f, int32_t n)
+{
+    d->n = n;
+    for (int32_t i = 0; i < n; ++i) d->func[i] = f[i];
+    d->cdf[0] = 0.0f;
+    for (int32_t i = 1; i < n + 1; ++i) d->cdf[i] = d->cdf[i - 1] + d->func[i - 1] / (float)n;
+    d->func_integral = d->cdf[n];
+    if (d->func_integral == 0.0f) {
+        for (int32_t i = 1; i < n + 1; ++i) d->cdf[i] = (float)i / (float)n;
+    } else {
+        for (int32_t i = 1; i < n + 1; ++i) d->cdf[i] /= d->func_integral;
+    }
+}
+
+/* Distribution1D::SampleContinuous (sampling.h:381-398) with FindInterval
+ * (math.h:117-135) */
+static float dist1d_sample(const dist1d_t* d, float u, int32_t* off)
+{
+    int32_t size = d->n + 1;
+    int32_t first = 0, len = size;
+    while (len > 0) {
+        int32_t half = len >> 1, middle = first + half;
+        if (d->cdf[middle] <= u) {
+            first = middle + 1;
+            len -= half + 1;
+        } else {
+            len = half;
+        }
+    }
+    int32_t offset = first - 1;
+    if (offset < 0) offset = 0;
+    if (offset > size - 2) offset = size - 2;
+    if (off) *off = offset;
+    float du = (u - d->cdf[offset]) / (d->cdf[offset + 1] - d->cdf[offset]);
+    return ((float)offset + du) / (float)d->n;
+}
+
+/* Distribution2D (sampling.h:426-478) of the Gaussian filter table */
+typedef struct {
+    dist1d_t cond[ORC_FILTER_N];
+    dist1d_t marginal;
+} dist2d_t;
+
 typedef struct {
     v3 origin, lower_left, horizontal, vertical, u, v, w;
-    float lens_radius, filter_extent;
+    float lens_radius, filter_extent, filter_sigma;
+    int32_t filter_type;
     int32_t width, height;
+    dist2d_t gauss;
 } camera_t;
+
+/* Gaussian (sampling.h:9-12) */
+static float gaussian(float x, float mu, float sigma)
+{
+    float xm = x - mu;
+    return 1.0f / sqrtf(orc_two_pi * sigma * sigma) * expf(-(xm * xm) / (2.0f * sigma * sigma));
+}
+
+/* GaussianFilter::GaussianFilter (gaussian_filter.cpp:8-27): 32 x 32 table
+ * of the separable Gaussian over [-extent/2, extent/2]^2 */
+static void gaussian_filter_init(dist2d_t* g, float sigma, float extent)
+{
+    const int32_t n = ORC_FILTER_N;
+    float values[ORC_FILTER_N * ORC_FILTER_N];
+    float half_extent = extent / 2.0f;
+    for (int32_t y = 0; y < n; ++y)
+        for (int32_t x = 0; x < n; ++x) {
+            float px = (((float)x + 0.5f) / (float)n) * extent - half_extent;
+            float py = (((float)y + 0.5f) / (float)n) * extent - half_extent;
+            values[y * n + x] = gaussian(px, 0.0f, sigma) * gaussian(py, 0.0f, sigma);
+        }
+    float marginal[ORC_FILTER_N];
+    for (int32_t v = 0; v < n; ++v) {
+        dist1d_init(&g->cond[v], &values[v * n], n);
+        marginal[v] = g->cond[v].func_integral;
+    }
+    dist1d_init(&g->marginal, marginal, n);
+}
+
+/* Filter::Sample of the three filters: box_filter.cpp:17-21,
+ * tent_filter.cpp:22-30, gaussian_filter.cpp:40-43 */
+static v2 filter_sample(const camera_t* c, v2 u)
+{
+    float e = c->filter_extent;
+    if (c->filter_type == BB_FILTER_TENT) {
+        float h = e / 2.0f;
+        float x = u.x < 0.5f ? h * (sqrtf(2.0f * u.x) - 1.0f) : h * (1.0f - sqrtf(1.0f - 2.0f * (u.x - 0.5f)));
+        float y = u.y < 0.5f ? h * (sqrtf(2.0f * u.y) - 1.0f) : h * (1.0f - sqrtf(1.0f - 2.0f * (u.y - 0.5f)));
+        return V2(x, y);
+    }
+    if (c->filter_type == BB_FILTER_GAUSSIAN) {
+        int32_t v = 0;
+        float d1 = dist1d_sample(&c->gauss.marginal, u.y, &v);
+        float d0 = dist1d_sample(&c->gauss.cond[v], u.x, NULL);
+        float h = e / 2.0f;
+        return V2(d0 * e - h, d1 * e - h);
+    }
+    return V2((u.x * 2.0f - 1.0f) * e / 2.0f, (u.y * 2.0f - 1.0f) * e / 2.0f);
+}
 
 /* src/camera/perspective_camera.cpp:8-38 */
 static void camera_init(camera_t* c, const BbCameraDesc* d)
@@ -311,14 +412,17 @@ static void camera_init(camera_t* c, const BbCameraDesc* d)
                          vmul(c->w, d->focus_distance));
     c->lens_radius = d->aperture_radius;
     c->filter_extent = d->filter_extent;
+    c->filter_type = d->filter_type;
+    c->filter_sigma = d->filter_sigma;
+    if (c->filter_type == BB_FILTER_GAUSSIAN) gaussian_filter_init(&c->gauss, d->filter_sigma, d->filter_extent);
     c->width = d->width;
     c->height = d->height;
 }
 
-/* perspective_camera.cpp:40-52 with BoxFilter::Sample (box_filter.cpp:17-21) */
+/* perspective_camera.cpp:40-52 with the camera's Filter::Sample */
 static void camera_sample_ray(const camera_t* c, int32_t px, int32_t py, v2 u0, v2 u1, v3* o, v3* dir)
 {
-    v2 off = V2((u0.x * 2.0f - 1.0f) * c->filter_extent / 2.0f, (u0.y * 2.0f - 1.0f) * c->filter_extent / 2.0f);
+    v2 off = filter_sample(c, u0);
     off = V2(off.x + 0.5f, off.y + 0.5f);
     v3 pixel_center = vadd(vadd(c->lower_left, vdiv(vmul(c->horizontal, (float)px + off.x), (float)c->width)),
                            vdiv(vmul(c->vertical, (float)py + off.y), (float)c->height));
@@ -327,6 +431,15 @@ static void camera_sample_ray(const camera_t* c, int32_t px, int32_t py, v2 u0, 
     v3 offset = vadd(vmul(c->u, ap.x), vmul(c->v, ap.y));
     *o = vadd(c->origin, offset);
     *dir = vnormalize(vsub(pixel_center, *o));
+}
+
+void orc_filter_sample(const BbCameraDesc* cam, const float u[2], float out[2])
+{
+    camera_t c;
+    camera_init(&c, cam);
+    v2 r = filter_sample(&c, V2(u[0], u[1]));
+    out[0] = r.x;
+    out[1] = r.y;
 }
 
 void orc_camera_ray(const BbCameraDesc* cam, int32_t px, int32_t py, const float u0[2], const float u1[2], BbRay* ray)
@@ -953,9 +1066,169 @@ static v3 ggx_sample_wm(ggx m, v3 wo, v2 u)
     return vnormalize(V3(h.x * m.ax, h.y * m.ay, h.z));
 }
 
+/* ===================================================== reflectance tables */
+
+/* PermutationElement (hash.h:128-159): Kensler's permutation of [0, l) */
+static int32_t permutation_element(uint32_t i, uint32_t l, uint32_t p)
+{
+    uint32_t w = l - 1;
+    w |= w >> 1;
+    w |= w >> 2;
+    w |= w >> 4;
+    w |= w >> 8;
+    w |= w >> 16;
+    do {
+        i ^= p;
+        i *= 0xe170893du;
+        i ^= p >> 16;
+        i ^= (i & w) >> 4;
+        i ^= p >> 8;
+        i *= 0x0929eb3fu;
+        i ^= p >> 23;
+        i ^= (i & w) >> 1;
+        i *= 1u | p >> 27;
+        i *= 0x6935fa69u;
+        i ^= (i & w) >> 11;
+        i *= 0x74dcb303u;
+        i ^= (i & w) >> 2;
+        i *= 0x9e501cc3u;
+        i ^= (i & w) >> 2;
+        i *= 0xc860a3dfu;
+        i &= w;
+        i ^= i >> 5;
+    } while (i >= l);
+    return (int32_t)((i + p) % l);
+}
+
+int32_t orc_permutation_element(uint32_t i, uint32_t l, uint32_t p) { return permutation_element(i, l, p); }
+
+/* PrepareReflectanceTextures (microfacet.cpp:239-270) with texture_size 32
+ * and 1024 samples (integrator.cpp:149-151) */
+#define ORC_TABLE_N 32
+#define ORC_TABLE_SAMPLES 1024
+
+typedef struct {
+    float u0[ORC_TABLE_SAMPLES];
+    v2 u12[ORC_TABLE_SAMPLES];
+    float E[ORC_TABLE_N * ORC_TABLE_N]; /* TrowbridgeReitz E(cos_theta = i, alpha = j) at [j * 32 + i] */
+    float E_avg[ORC_TABLE_N];
+} refl_tables_t;
+
+static refl_tables_t g_tables;
+static pthread_once_t g_tables_once = PTHREAD_ONCE_INIT;
+
+/* TrowbridgeReitzDistribution::rho (microfacet.h:109-131) */
+static float ggx_rho(ggx m, v3 wo, const v2* u, int32_t n)
+{
+    float r = 0.0f;
+    for (int32_t i = 0; i < n; ++i) {
+        v3 wm = ggx_sample_wm(m, wo, u[i]);
+        v3 wi = reflect(wo, wm);
+        if (!same_hemisphere(wo, wi)) continue;
+        float pdf = ggx_pdf(m, wo, wm) / (4.0f * vabsdot(wo, wm));
+        float f = ggx_D(m, wm) * ggx_G(m, wo, wi) / (4.0f * abs_cos_theta(wo) * abs_cos_theta(wi));
+        r += f * abs_cos_theta(wi) / pdf;
+    }
+    return r / (float)n;
+}
+
+static void tables_build(void)
+{
+    refl_tables_t* t = &g_tables;
+    const int32_t samples = ORC_TABLE_SAMPLES, xs = 32, ys = 32;
+    pcg32 rng;
+    pcg_seed(&rng, 3, 1); /* RNG rng(3) */
+    for (int32_t i = 0; i < samples; ++i) {
+        int32_t stratum = permutation_element((uint32_t)i, (uint32_t)samples, 1u);
+        t->u0[i] = ((float)stratum + pcg_float(&rng)) / (float)samples;
+        stratum = permutation_element((uint32_t)i, (uint32_t)samples, 2u);
+        int32_t x = stratum % xs, y = stratum / xs;
+        float ux = ((float)x + pcg_float(&rng)) / (float)xs;
+        float uy = ((float)y + pcg_float(&rng)) / (float)ys;
+        t->u12[i] = V2(ux, uy);
+    }
+    /* TrowbridgeReitzDistribution::PrepareReflectanceTexture (microfacet.cpp:31-75) */
+    const int32_t n = ORC_TABLE_N;
+    const float d = 1.0f / (float)n;
+    for (int32_t j = 0; j < n; ++j) {
+        float a = d / 2.0f + d * (float)j;
+        float r_sum = 0.0f;
+        for (int32_t i = 0; i < n; ++i) {
+            float cos_theta = d / 2.0f + d * (float)i;
+            float sin_theta = sqrtf(1.0f - cos_theta * cos_theta);
+            v3 wo = V3(sin_theta, 0.0f, cos_theta);
+            ggx m = { a, a };
+            float r = ggx_rho(m, wo, t->u12, samples);
+            r_sum += r * cos_theta * d;
+            t->E[j * n + i] = r;
+        }
+        t->E_avg[j] = 2.0f * r_sum;
+    }
+}
+
+static const refl_tables_t* tables(void)
+{
+    pthread_once(&g_tables_once, tables_build);
+    return &g_tables;
+}
+
+void orc_reflectance_tables(float* E, float* E_avg, float* u0, float* u12)
+{
+    const refl_tables_t* t = tables();
+    if (E) memcpy(E, t->E, sizeof(t->E));
+    if (E_avg) memcpy(E_avg, t->E_avg, sizeof(t->E_avg));
+    if (u0) memcpy(u0, t->u0, sizeof(t->u0));
+    if (u12)
+        for (int32_t i = 0; i < ORC_TABLE_SAMPLES; ++i) {
+            u12[2 * i] = t->u12[i].x;
+            u12[2 * i + 1] = t->u12[i].y;
+        }
+}
+
+/* ImageTexture<Float>::Evaluate, bilinear with TexCoordFilter::clamp
+ * (textures.h:60-88, texture.h:26-46) */
+static float tex_bilinear(const float* img, int32_t w, int32_t h, float u, float v)
+{
+    float fw = u * (float)w - 0.5f;
+    float fh = v * (float)h - 0.5f;
+    int32_t i0 = (int32_t)floorf(fw), i1 = i0 + 1;
+    int32_t j0 = (int32_t)floorf(fh), j1 = j0 + 1;
+    i0 = i0 < 0 ? 0 : (i0 > w - 1 ? w - 1 : i0);
+    j0 = j0 < 0 ? 0 : (j0 > h - 1 ? h - 1 : j0);
+    i1 = i1 < 0 ? 0 : (i1 > w - 1 ? w - 1 : i1);
+    j1 = j1 < 0 ? 0 : (j1 > h - 1 ? h - 1 : j1);
+    float fu = fw - floorf(fw);
+    float fv = fh - floorf(fh);
+    float v00 = img[i0 + j0 * w], v10 = img[i1 + j0 * w];
+    float v01 = img[i0 + j1 * w], v11 = img[i1 + j1 * w];
+    return (1.0f - fu) * (1.0f - fv) * v00 + (1.0f - fu) * fv * v01 + fu * (1.0f - fv) * v10 + fu * fv * v11;
+}
+
+/* TrowbridgeReitzDistribution::E / E_avg (microfacet.h:133-145), GetMeanAlpha (:157-160) */
+static float ggx_E(ggx m, v3 wo)
+{
+    return tex_bilinear(tables()->E, ORC_TABLE_N, ORC_TABLE_N, wo.z, sqrtf(m.ax * m.ay));
+}
+static float ggx_E_avg(ggx m)
+{
+    return tex_bilinear(tables()->E_avg, ORC_TABLE_N, 1, sqrtf(m.ax * m.ay), 0.0f);
+}
+
+float orc_ggx_E(float ax, float ay, const float wo[3])
+{
+    ggx m = { ax, ay };
+    return ggx_E(m, V3(wo[0], wo[1], wo[2]));
+}
+
+float orc_ggx_E_avg(float ax, float ay)
+{
+    ggx m = { ax, ay };
+    return ggx_E_avg(m);
+}
+
 /* ================================================================= BxDFs */
 
-enum { K_LAMBERT = 0, K_DIELECTRIC = 1, K_CONDUCTOR = 2, K_MR = 3 };
+enum { K_LAMBERT = 0, K_DIELECTRIC = 1, K_CONDUCTOR = 2, K_MR = 3, K_CONDUCTOR_MS = 4 };
 
 typedef struct {
     int kind;
@@ -984,7 +1257,8 @@ static int bxdf_flags(const bxdf_t* b)
         int f = (b->eta == 1.0f) ? BX_TRANSMISSION : (BX_REFLECTION | BX_TRANSMISSION);
         return f | (ggx_smooth(b->mf) ? BX_SPECULAR : BX_GLOSSY);
     }
-    case K_CONDUCTOR: return ggx_smooth(b->mf) ? (BX_SPECULAR | BX_REFLECTION) : (BX_GLOSSY | BX_REFLECTION);
+    case K_CONDUCTOR:
+    case K_CONDUCTOR_MS: return ggx_smooth(b->mf) ? (BX_SPECULAR | BX_REFLECTION) : (BX_GLOSSY | BX_REFLECTION);
     default: return BX_DIFFUSE | BX_GLOSSY | BX_REFLECTION;
     }
 }
@@ -1062,6 +1336,85 @@ static int conductor_sample(const bxdf_t* b, v3 wo, v2 u12, bsample* s)
     s->wi = wi;
     s->pdf = pdf;
     s->flags = BX_GLOSSY | BX_REFLECTION;
+    s->eta = 1.0f;
+    s->stochastic = 0;
+    return 1;
+}
+
+/* ---- Conductor with multiple-scattering compensation:
+ *      src/bxdf/conductor_multi_scattering_bxdf.cpp:7-148 ---- */
+
+/* FresnelConductorAverage (scattering.h:174-185) */
+static spec fresnel_conductor_average(spec eta, spec k)
+{
+    spec F0 = fresnel_complex_spec(1.0f, eta, k);
+    spec F90 = fresnel_complex_spec(0.0f, eta, k);
+    return sdivf(sadd(smulf(F0, 20.0f), F90), 21.0f);
+}
+
+/* f_ms of the compensated conductor: fresnel_ms (1 - E(wo)) (1 - E(wi)) / (pi (1 - E_avg)) */
+static spec conductor_ms_lobe(const bxdf_t* b, v3 wo, v3 wi)
+{
+    float E_avg = ggx_E_avg(b->mf);
+    spec fa = fresnel_conductor_average(b->ceta, b->ck);
+    spec fresnel_ms = sdiv(smulf(smul(fa, fa), E_avg), ssub(S1(1.0f), smulf(fa, 1.0f - E_avg)));
+    return sdivf(smulf(fresnel_ms, (1.0f - ggx_E(b->mf, wo)) * (1.0f - ggx_E(b->mf, wi))), orc_pi * (1.0f - E_avg));
+}
+
+static spec conductor_ms_f(const bxdf_t* b, v3 wo, v3 wi)
+{
+    if (!same_hemisphere(wo, wi) || ggx_smooth(b->mf)) return S1(0.0f);
+    float cos_o = abs_cos_theta(wo), cos_i = abs_cos_theta(wi);
+    if (cos_i == 0.0f || cos_o == 0.0f) return S1(0.0f);
+    v3 wm = vadd(wo, wi);
+    if (vlen2(wm) == 0.0f) return S1(0.0f);
+    vnormalize_m(&wm);
+    spec F = fresnel_complex_spec(vabsdot(wo, wm), b->ceta, b->ck);
+    spec f_ss = sdivf(smulf(smulf(F, ggx_D(b->mf, wm)), ggx_G(b->mf, wo, wi)), 4.0f * cos_i * cos_o);
+    return smul(b->r, sadd(f_ss, conductor_ms_lobe(b, wo, wi)));
+}
+
+static float conductor_ms_pdf(const bxdf_t* b, v3 wo, v3 wi)
+{
+    if (!same_hemisphere(wo, wi) || ggx_smooth(b->mf)) return 0.0f;
+    v3 wm = vadd(wo, wi);
+    if (vlen2(wm) == 0.0f) return 0.0f;
+    vnormalize_m(&wm);
+    if (dot_z(wm) < 0.0f) wm = vneg(wm);
+    float pdf_ss = ggx_pdf(b->mf, wo, wm) / (4.0f * vabsdot(wo, wm));
+    float pdf_ms = wi.z * orc_inv_pi; /* CosineHemispherePDF(wi.z) */
+    float e = ggx_E(b->mf, wo);
+    return pdf_ms * (1.0f - e) + pdf_ss * e; /* Lerp(pdf_ms, pdf_ss, E(wo)), math.h:87-90 */
+}
+
+static int conductor_ms_sample(const bxdf_t* b, v3 wo, float u0, v2 u12, bsample* s)
+{
+    if (ggx_smooth(b->mf)) return conductor_sample(b, wo, u12, s);
+    if (wo.z == 0.0f) return 0;
+    v3 wm, wi;
+    int flags;
+    float e = ggx_E(b->mf, wo);
+    if (u0 < e) {
+        wm = ggx_sample_wm(b->mf, wo, u12);
+        wi = reflect(wo, wm);
+        flags = BX_GLOSSY | BX_REFLECTION;
+    } else {
+        wi = sample_cosine_hemisphere(u12);
+        wm = vnormalize(vadd(wo, wi));
+        flags = BX_DIFFUSE | BX_REFLECTION;
+    }
+    if (!same_hemisphere(wo, wi)) return 0;
+    float cos_o = abs_cos_theta(wo), cos_i = abs_cos_theta(wi);
+    if (cos_i == 0.0f || cos_o == 0.0f) return 0;
+    spec F = fresnel_complex_spec(vabsdot(wo, wm), b->ceta, b->ck);
+    spec f_ss = sdivf(smulf(smulf(F, ggx_D(b->mf, wm)), ggx_G(b->mf, wo, wi)), 4.0f * cos_i * cos_o);
+    spec f_ms = conductor_ms_lobe(b, wo, wi);
+    float pdf_ss = ggx_pdf(b->mf, wo, wm) / (4.0f * vabsdot(wo, wm));
+    float pdf_ms = wi.z * orc_inv_pi;
+    s->f = smul(b->r, sadd(f_ss, f_ms));
+    s->wi = wi;
+    s->pdf = pdf_ms * (1.0f - e) + pdf_ss * e;
+    s->flags = flags;
     s->eta = 1.0f;
     s->stochastic = 0;
     return 1;
@@ -1282,7 +1635,8 @@ static int make_bsdf(const BbMaterial* m, v3 ns, v3 ts, int front_face, bsdf_t* 
         return 1;
     case BB_MAT_CONDUCTOR:
     case BB_MAT_CONDUCTOR_R:
-        b->kind = K_CONDUCTOR;
+        /* ConductorMultiScatteringBxDF when energy_compensation (conductor_material.cpp:63-95) */
+        b->kind = m->energy_compensation ? K_CONDUCTOR_MS : K_CONDUCTOR;
         b->mf.ax = sqrf(m->u_roughness);
         b->mf.ay = sqrf(m->v_roughness);
         b->r = S3(m->color[0], m->color[1], m->color[2]);
@@ -1322,6 +1676,7 @@ static spec bsdf_f(const bsdf_t* s, v3 wo_w, v3 wi_w)
     case K_LAMBERT: return lambert_f(&s->bx, wo, wi);
     case K_DIELECTRIC: return dielectric_f(&s->bx, wo, wi);
     case K_CONDUCTOR: return conductor_f(&s->bx, wo, wi);
+    case K_CONDUCTOR_MS: return conductor_ms_f(&s->bx, wo, wi);
     default: return mr_f(&s->bx, wo, wi);
     }
 }
@@ -1336,6 +1691,7 @@ static float bsdf_pdf(const bsdf_t* s, v3 wo_w, v3 wi_w)
     case K_LAMBERT: return lambert_pdf(wo, wi);
     case K_DIELECTRIC: return dielectric_pdf(&s->bx, wo, wi);
     case K_CONDUCTOR: return conductor_pdf(&s->bx, wo, wi);
+    case K_CONDUCTOR_MS: return conductor_ms_pdf(&s->bx, wo, wi);
     default: return mr_pdf(&s->bx, wo, wi);
     }
 }
@@ -1350,6 +1706,7 @@ static int bsdf_sample(const bsdf_t* s, v3 wo_w, float u0, v2 u12, bsample* out)
     case K_LAMBERT: ok = lambert_sample(&s->bx, wo, u12, out); break;
     case K_DIELECTRIC: ok = dielectric_sample(&s->bx, wo, u0, u12, out); break;
     case K_CONDUCTOR: ok = conductor_sample(&s->bx, wo, u12, out); break;
+    case K_CONDUCTOR_MS: ok = conductor_ms_sample(&s->bx, wo, u0, u12, out); break;
     default: ok = mr_sample(&s->bx, wo, u0, u12, out); break;
     }
     if (!ok) return 0;
@@ -1367,13 +1724,13 @@ static int material_supported(const BbMaterial* m, char* err, int32_t err_size)
             return 0;
         }
         return 1;
-    case BB_MAT_DIELECTRIC:
     case BB_MAT_CONDUCTOR:
-    case BB_MAT_CONDUCTOR_R:
+    case BB_MAT_CONDUCTOR_R: return 1;
+    case BB_MAT_DIELECTRIC:
         if (m->energy_compensation) {
             int smooth = smax(sqrf(m->u_roughness), sqrf(m->v_roughness)) < 1e-3f;
-            if (!(m->type == BB_MAT_DIELECTRIC && smooth)) {
-                snprintf(err, err_size, "energy-compensated (multi-scattering) BxDFs are not supported");
+            if (!smooth) {
+                snprintf(err, err_size, "energy-compensated (multi-scattering) rough dielectric is not supported");
                 return 0;
             }
         }

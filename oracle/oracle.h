@@ -39,6 +39,17 @@ float orc_fresnel_dielectric(float cos_theta_i, float eta);
 float orc_fresnel_complex(float cos_theta_i, float eta, float k);
 /* Transform::LookAt (transform.h:112-122) */
 void orc_look_at(const float position[3], const float target[3], const float up[3], float q_out[4]);
+/* PermutationElement (hash.h:128-159) */
+int32_t orc_permutation_element(uint32_t i, uint32_t l, uint32_t p);
+/* the reflectance tables of PrepareReflectanceTextures (microfacet.cpp:239-270):
+ * E[32 * 32] (alpha-major), E_avg[32], the 1024 u0 and 1024 (u1, u2) samples;
+ * any pointer may be NULL */
+void orc_reflectance_tables(float* E, float* E_avg, float* u0, float* u12);
+/* TrowbridgeReitzDistribution::E(wo) and E_avg() of a GGX with (ax, ay) */
+float orc_ggx_E(float ax, float ay, const float wo[3]);
+float orc_ggx_E_avg(float ax, float ay);
+/* the camera's Filter::Sample(u) (box, tent or Gaussian, BbCameraDesc.filter_type) */
+void orc_filter_sample(const BbCameraDesc* cam, const float u[2], float out[2]);
 /* PerspectiveCamera::SampleRay with explicit filter sample u0 and lens sample u1 */
 void orc_camera_ray(const BbCameraDesc* cam, int32_t px, int32_t py, const float u0[2], const float u1[2],
                     BbRay* ray_out);

@@ -196,6 +196,25 @@ def test_render_readme_scene_parity(readme):
     assert frac >= 0.999
 
 
+@pytest.mark.parametrize("kind", ["tent", "gaussian"])
+def test_render_with_tent_and_gaussian_filters(readme, kind):
+    """TentFilter and GaussianFilter (the reference's default camera filter):
+    the camera's filter sample moves every primary ray."""
+    sc, cam, g, o = readme
+    import copy
+
+    cam2 = copy.copy(cam)
+    scenes.set_filter(cam2, kind)
+    gi, oi, gc, oc = _render_pair(g, o, cam2, scenes.render_params(8))
+    assert np.array_equal(gc, oc)
+    frac = pixel_agreement(gi, oi)
+    log(f"readme {kind} filter: {frac:.6f} within 1e-3, {np.mean(gi == oi):.4f} bit-equal")
+    assert frac >= 0.999
+    # the filter changes the image (against the box filter)
+    bs, bc, _ = g.render(cam, scenes.render_params(8))
+    assert not np.array_equal(bs / bc[..., None], gi)
+
+
 def test_render_icosphere_parity(ico6):
     sc, cam, g, o = ico6
     gi, oi, gc, oc = _render_pair(g, o, cam, scenes.render_params(8))
@@ -229,6 +248,39 @@ def test_render_all_materials_parity():
     frac = pixel_agreement(gi, oi)
     log(f"all-materials 256x144x8: {frac:.6f} within 1e-3, {np.mean(gi == oi):.4f} bit-equal")
     assert frac >= 0.999
+
+
+def test_render_compensated_conductors_parity():
+    """ConductorMultiScatteringBxDF (energy_compensation, the reference's
+    default for ConductorMaterial): rough gold and a rough R-parameterized
+    conductor, against the oracle; the compensation lobe makes rough metal
+    brighter than the plain conductor."""
+    from bulbit_amd.gpu import GpuScene
+    from oracle import OracleScene
+
+    def build(ec):
+        sc, cam = scenes.icosphere_scene(width=192, height=108, level=5, n_lights=16)
+        r_mat = scenes.conductor(scenes.GOLD_ETA, scenes.GOLD_K, 0.8, energy_compensation=ec)
+        r_mat.type = capi.BB_MAT_CONDUCTOR_R
+        r_mat.k[:] = [0.9, 0.6, 0.3]
+        mats = [scenes.conductor(scenes.GOLD_ETA, scenes.GOLD_K, 0.5, energy_compensation=ec), r_mat,
+                scenes.conductor(scenes.GOLD_ETA, scenes.GOLD_K, 0.0, energy_compensation=ec)]
+        base = len(sc.materials)
+        sc.materials += mats
+        n_ico = 20 * 4 ** 5
+        sc.prims[:n_ico, 2] = base + np.random.default_rng(9).integers(0, len(mats), n_ico // 64).repeat(64)
+        return sc, cam
+
+    sc, cam = build(1)
+    g, o = GpuScene(sc), OracleScene(sc)
+    gi, oi, gc, oc = _render_pair(g, o, cam, scenes.render_params(8))
+    assert np.array_equal(gc, oc)
+    frac = pixel_agreement(gi, oi)
+    log(f"compensated conductors 192x108x8: {frac:.6f} within 1e-3, {np.mean(gi == oi):.4f} bit-equal")
+    assert frac >= 0.999
+    sc0, _ = build(0)
+    s0, c0, _ = GpuScene(sc0).render(cam, scenes.render_params(8))
+    assert gi.mean() > (s0 / c0[..., None]).mean()
 
 
 def test_render_full_scene_crop_parity(c2):

@@ -133,6 +133,58 @@ def test_camera_filter_and_lens_offsets(orc):
     assert np.allclose(list(ray.d), p / np.linalg.norm(p), atol=1e-6)
 
 
+def _filter_sample(orc, cam, u):
+    out = farr(0, 0)
+    orc.orc_filter_sample(ctypes.byref(cam), farr(*u), out)
+    return np.array(out[:], np.float64)
+
+
+def test_box_and_tent_filter_closed_forms(orc):
+    cam = scenes.set_filter(_pinhole(3, 3), "box", extent=1.5)
+    assert np.array_equal(_filter_sample(orc, cam, (0.0, 1.0)), [-0.75, 0.75])
+    assert np.array_equal(_filter_sample(orc, cam, (0.5, 0.25)), [0.0, -0.375])
+    cam = scenes.set_filter(_pinhole(3, 3), "tent")  # extent 2: half extent 1
+    # x = h (sqrt(2u) - 1) below 1/2, h (1 - sqrt(1 - 2 (u - 1/2))) above
+    assert np.array_equal(_filter_sample(orc, cam, (0.0, 0.5)), [-1.0, 0.0])
+    assert np.array_equal(_filter_sample(orc, cam, (0.125, 0.875)), [-0.5, 0.5])
+    # uniform u gives the triangle density (1 - |x|) on [-1, 1]
+    u = (np.arange(20000) + 0.5) / 20000
+    xs = np.array([_filter_sample(orc, cam, (v, 0.5))[0] for v in u])
+    hist, edges = np.histogram(xs, bins=10, range=(-1, 1))
+    c = 0.5 * (edges[:-1] + edges[1:])
+    assert np.allclose(hist / len(xs), (1 - np.abs(c)) * 0.2, atol=2e-3)
+
+
+def test_gaussian_filter_samples_follow_its_table(orc):
+    """GaussianFilter (gaussian_filter.cpp:8-43): a 32 x 32 piecewise-constant
+    Distribution2D of the separable Gaussian over [-e/2, e/2]^2.  Stratified
+    u must put into each cell the probability of the truncated Gaussian's
+    midpoint rule, the sampling is odd-symmetric about u = 1/2, and linear
+    inside a cell."""
+    sigma, e = 0.5, 3.0
+    cam = scenes.set_filter(_pinhole(3, 3), "gaussian", sigma=sigma)
+    n = 8192
+    u = (np.arange(n) + 0.5) / n
+    xs = np.array([_filter_sample(orc, cam, (v, 0.3))[0] for v in u])
+    ys = np.array([_filter_sample(orc, cam, (0.3, v))[1] for v in u])
+    centres = ((np.arange(32) + 0.5) / 32) * e - e / 2
+    g = np.exp(-centres ** 2 / (2 * sigma ** 2))
+    want = g / g.sum()
+    for s_ in (xs, ys):
+        assert s_.min() >= -e / 2 and s_.max() <= e / 2
+        cell = np.clip(((s_ + e / 2) / e * 32).astype(int), 0, 31)
+        got = np.bincount(cell, minlength=32) / n
+        assert np.allclose(got, want, atol=2.0 / n)
+    # odd symmetry and linearity within the central cells
+    a = _filter_sample(orc, cam, (0.3, 0.3))
+    b = _filter_sample(orc, cam, (0.7, 0.7))
+    assert np.allclose(a, -b, atol=1e-6)
+    mid = [_filter_sample(orc, cam, (v, 0.5))[0] for v in (0.49, 0.495, 0.5)]
+    assert abs((mid[1] - mid[0]) - (mid[2] - mid[1])) < 1e-6
+    # the sample variance is that of the truncated Gaussian (~sigma^2 less the tails)
+    assert 0.9 * sigma ** 2 < np.mean(xs ** 2) < sigma ** 2
+
+
 def test_concentric_disk_and_sphere_samples(orc):
     out = farr(0, 0)
     orc.orc_sample_concentric(farr(0.5, 0.5), out)
@@ -334,7 +386,8 @@ def test_lambertian_closed_form(orc):
     (scenes.metallic_roughness((0.7, 0.5, 0.3), 0.0, 0.5), False),
     (scenes.metallic_roughness((0.7, 0.5, 0.3), 1.0, 0.3), False),
     (scenes.diffuse((0.5, 0.6, 0.7)), True),
-], ids=["gold0.3", "gold0.6", "roughglass", "mr_dielectric", "mr_metal", "lambert"])
+    (scenes.conductor(scenes.GOLD_ETA, scenes.GOLD_K, 0.5, energy_compensation=1), True),
+], ids=["gold0.3", "gold0.6", "roughglass", "mr_dielectric", "mr_metal", "lambert", "gold0.5_ms"])
 def test_sampled_pdf_and_value_match_evaluation(orc, mat, pdf_consistent):
     rng = np.random.default_rng(3)
     n_ok = 0
@@ -356,7 +409,8 @@ def test_sampled_pdf_and_value_match_evaluation(orc, mat, pdf_consistent):
     scenes.conductor(scenes.GOLD_ETA, scenes.GOLD_K, 0.4),
     scenes.dielectric(1.5, roughness=0.4),
     scenes.diffuse((0.3, 0.6, 0.9)),
-], ids=["gold", "roughglass", "lambert"])
+    scenes.conductor(scenes.GOLD_ETA, scenes.GOLD_K, 0.4, energy_compensation=1),
+], ids=["gold", "roughglass", "lambert", "gold_ms"])
 def test_bsdf_reciprocity(orc, mat):
     rng = np.random.default_rng(5)
     for _ in range(200):
@@ -452,3 +506,104 @@ def test_readme_render_is_deterministic_and_sample_ordered():
     o.render(cam, scenes.render_params(2), threads=4, film=film)
     o.render(cam, scenes.render_params(2, sample_begin=2), threads=4, film=film)
     assert np.array_equal(film[0], a) and np.array_equal(film[1], ca)
+
+
+# ------------------------------------------- multiple-scattering compensation
+
+def test_permutation_element_is_a_permutation(orc):
+    """PermutationElement (hash.h:128-159, Kensler's correlated multi-jittered
+    permutation) maps [0, l) onto itself one-to-one for any l and pattern."""
+    for l, p in ((1024, 1), (1024, 2), (1000, 7), (37, 123456789)):
+        perm = [orc.orc_permutation_element(i, l, p) for i in range(l)]
+        assert sorted(perm) == list(range(l)), (l, p)
+    assert [orc.orc_permutation_element(i, 1024, 1) for i in range(8)] != list(range(8))
+
+
+def _tables(orc):
+    E = (F * 1024)()
+    E_avg = (F * 32)()
+    u0 = (F * 1024)()
+    u12 = (F * 2048)()
+    orc.orc_reflectance_tables(E, E_avg, u0, u12)
+    return (np.array(E, np.float64).reshape(32, 32), np.array(E_avg, np.float64), np.array(u0, np.float64),
+            np.array(u12, np.float64).reshape(1024, 2))
+
+
+def test_reflectance_table_samples_are_stratified(orc):
+    """PrepareReflectanceTextures (microfacet.cpp:239-270): u0 has one sample
+    in each of 1024 strata and u12 one in each cell of a 32 x 32 grid."""
+    _, _, u0, u12 = _tables(orc)
+    assert np.all((u0 >= 0) & (u0 < 1)) and np.all((u12 >= 0) & (u12 < 1))
+    assert sorted(np.floor(u0 * 1024).astype(int)) == list(range(1024))
+    cells = np.floor(u12[:, 1] * 32).astype(int) * 32 + np.floor(u12[:, 0] * 32).astype(int)
+    assert sorted(cells) == list(range(1024))
+
+
+def _ggx_albedo_quadrature(cos_o, alpha, n=400):
+    """Single-scattering albedo of an isotropic GGX reflector with F = 1,
+    E(wo) = int D(wm) G(wo, wi) / (4 cos_o) dwi, by quadrature over wi in
+    float64 (independent of the oracle's Monte Carlo estimate)."""
+    a2 = alpha * alpha
+    sin_o = np.sqrt(1 - cos_o * cos_o)
+    wo = np.array([sin_o, 0.0, cos_o])
+    t = (np.arange(n) + 0.5) / n * (np.pi / 2)
+    ph = (np.arange(2 * n) + 0.5) / (2 * n) * (2 * np.pi)
+    T, P = np.meshgrid(t, ph, indexing="ij")
+    wi = np.stack([np.sin(T) * np.cos(P), np.sin(T) * np.sin(P), np.cos(T)], -1)
+    wm = wi + wo
+    wm /= np.linalg.norm(wm, axis=-1, keepdims=True)
+    c2 = wm[..., 2] ** 2
+    D = 1.0 / (np.pi * a2 * c2 * c2 * (1 + (1 - c2) / c2 / a2) ** 2)
+
+    def lam(w):
+        c = w[..., 2] ** 2
+        return (np.sqrt(1 + a2 * (1 - c) / c) - 1) / 2
+
+    G = 1.0 / (1 + lam(wo[None, None, :] + 0 * wi) + lam(wi))
+    f = D * G / (4 * cos_o * wi[..., 2])
+    dw = np.sin(T) * (np.pi / 2 / n) * (2 * np.pi / (2 * n))
+    return float(np.sum(f * wi[..., 2] * dw))
+
+
+def test_ggx_reflectance_table_matches_quadrature(orc):
+    """TrowbridgeReitzDistribution::PrepareReflectanceTexture (microfacet.cpp:
+    31-75): E(cos_theta_i, alpha_j) is the albedo of GGX with F = 1 (Monte
+    Carlo over the 1024 stratified samples); E_avg = 2 int E mu dmu."""
+    E, E_avg, _, _ = _tables(orc)
+    d = 1.0 / 32
+    for j in (2, 8, 16, 28):
+        a = d / 2 + d * j
+        for i in (4, 12, 20, 30):
+            c = d / 2 + d * i
+            assert abs(E[j, i] - _ggx_albedo_quadrature(c, a)) < 0.01, (j, i)
+        assert abs(E_avg[j] - 2 * np.sum(E[j] * (d / 2 + d * np.arange(32)) * d)) < 1e-5
+    assert np.all(E <= 1.0 + 1e-3) and np.all(E > 0.3)
+    assert np.all(np.diff(E_avg) < 1e-4)  # rougher loses more energy
+
+
+def test_compensated_conductor_conserves_energy(orc):
+    """ConductorMultiScatteringBxDF (conductor_multi_scattering_bxdf.cpp): for
+    F -> 1 the single-scattering albedo E(wo) plus the compensation lobe
+    (1 - E(wo)) is 1 at every roughness, while the plain ConductorBxDF loses
+    energy as roughness grows.  Albedo by quadrature of f cos over wi."""
+    n = 160
+    t = (np.arange(n) + 0.5) / n * (np.pi / 2)
+    ph = (np.arange(2 * n) + 0.5) / (2 * n) * (2 * np.pi)
+    for rough in (0.3, 0.7, 0.95):
+        albedo = {}
+        for ec in (0, 1):
+            m = capi.BbMaterial()
+            m.type = capi.BB_MAT_CONDUCTOR_R
+            m.k[:] = [1.0, 1.0, 1.0]
+            m.color[:] = [1.0, 1.0, 1.0]
+            m.u_roughness = m.v_roughness = rough
+            m.energy_compensation = ec
+            wo = _dir(0.6, 0.0)
+            acc = 0.0
+            for ti in t:
+                for pj in ph[::4]:
+                    f, _, _ = _eval(orc, m, wo, _dir(ti, pj))
+                    acc += f[0] * np.cos(ti) * np.sin(ti)
+            albedo[ec] = acc * (np.pi / 2 / n) * (2 * np.pi / (2 * n)) * 4
+        assert abs(albedo[1] - 1.0) < 0.03, (rough, albedo)
+        assert albedo[0] < albedo[1] - (0.02 if rough > 0.5 else 0.0), (rough, albedo)
