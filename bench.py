@@ -49,6 +49,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--wavefront-paths", type=int, default=0)
     ap.add_argument("--ref-spp", type=int, default=4096, help="spp of the relMSE reference (0: skip the quality check)")
+    ap.add_argument("--cache-frames", type=int, default=256,
+                    help="1-spp frames of the equal-time radiance-cache comparison (0: skip it)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (scales the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -161,6 +163,10 @@ def main():
         test = s_dev.cpu().numpy().reshape(args.height, args.width, 3) / \
             c_dev.cpu().numpy().reshape(args.height, args.width, 1).astype(np.float32)
         out["quality"] = quality(g, sc, cam, args, test)
+    if args.cache_frames > 0:
+        cq = cache_comparison(args)
+        out["relmse_equal_time_cache_vs_plain"] = cq["ratio"]
+        out["radiance_cache"] = cq
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, cam, params, args, samples_per_step)
         out["speedup_vs_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
@@ -209,6 +215,88 @@ def quality(g, sc, cam, args, test):
         log(f"[bench] crop {rect}: relMSE vs CPU {args.ref_spp}-spp {out['relmse_vs_cpu_reference_crop']}, "
             f"GPU vs CPU reference {out['relmse_gpu_vs_cpu_reference_crop']}")
     return out
+
+
+MLP_FLOPS_QUERY = 2 * (74 * 64 + 3 * 64 * 64 + 64 * 3)  # per query, 74 inputs, 4 x 64 hidden, 3 outputs
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def cache_comparison(args):
+    """Radiance cache vs plain path tracing at equal wall-clock time: the
+    README scene at 1920x1080, one sample per frame, accumulated over frames;
+    relMSE against a plain 4096-spp frame with disjoint samples.  Plain PT
+    renders `cache_frames` frames; the cached tracer (trained online on 1/16
+    of its paths) renders frames until it has used the same time."""
+    import numpy as np
+    import torch
+    from bulbit_amd import scenes
+    from bulbit_amd.gpu import GpuScene
+    from bulbit_amd.nrc import Nrc
+
+    w, h = 1920, 1080
+    sc, cam = scenes.readme_scene(width=w, height=h)
+    g = GpuScene(sc)
+    ref_spp = max(args.ref_spp, 1)
+    t0 = time.time()
+    rs, rc, _ = g.render(cam, scenes.render_params(ref_spp, sample_begin=1 << 20))
+    ref = rs / rc[..., None]
+    log(f"[bench] cache comparison: {ref_spp}-spp reference in {time.time() - t0:.1f}s")
+
+    def run(step, frames, budget=None):
+        film = g.new_film(cam)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        done = 0
+        for f in range(frames):
+            step(f, film)
+            torch.cuda.synchronize()
+            done = f + 1
+            if budget is not None and time.perf_counter() - t >= budget:
+                break
+        secs = time.perf_counter() - t
+        s, c = film[0].cpu().numpy().reshape(h, w, 3), film[1].cpu().numpy().reshape(h, w)
+        return relmse(s / c[..., None], ref), done, secs
+
+    # warm both paths once (kernel loading), outside the timed runs
+    g.render(cam, scenes.render_params(1, sample_begin=1 << 24))
+    net = Nrc(seed=1, learning_rate=1e-2)
+    cp = scenes.cache_params(train_batch=65536, train_steps=2)
+    g.render_cached(net, cam, scenes.render_params(1, sample_begin=1 << 24), cp)
+    net.close()
+
+    e_plain, n_plain, t_plain = run(
+        lambda f, film: g.render_device(cam, scenes.render_params(1, sample_begin=f), film[0].data_ptr(),
+                                        film[1].data_ptr()), args.cache_frames)
+    net = Nrc(seed=1, learning_rate=1e-2)
+    stats = []
+
+    def cached(f, film):
+        _, _, st = g.render_cached(net, cam, scenes.render_params(1, sample_begin=f), cp, film=film)
+        stats.append(st)
+
+    e_cache, n_cache, t_cache = run(cached, 100000, budget=t_plain)
+    net.close()
+    q = sum(st.queries for st in stats)
+    rows = sum(st.train_rows for st in stats)
+    ms_q = sum(st.ms_query for st in stats)
+    ms_t = sum(st.ms_train for st in stats)
+    tflops = (MLP_FLOPS_QUERY * q + 3 * MLP_FLOPS_QUERY * min(rows, len(stats) * 2 * 65536)) / max(ms_q + ms_t, 1e-9) / 1e9
+    log(f"[bench] equal time {t_plain:.2f}s: plain {n_plain} frames relMSE {e_plain:.5f}; cached {n_cache} frames "
+        f"relMSE {e_cache:.5f} ({q} queries, {rows} training rows, query {ms_q:.0f} ms, train {ms_t:.0f} ms)")
+    return {
+        "workload": "README scene 1920x1080, 1 spp per frame, max_bounces 8; reference %d spp" % ref_spp,
+        "seconds": round(t_plain, 3),
+        "plain_frames": n_plain,
+        "plain_relmse": round(e_plain, 6),
+        "cache_frames": n_cache,
+        "cache_relmse": round(e_cache, 6),
+        "ratio": round(e_cache / e_plain, 4),
+        "cache_ms_query": round(ms_q, 1),
+        "cache_ms_train": round(ms_t, 1),
+        "mlp": {"bound": "mfma", "achieved": round(tflops, 3), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(tflops / BF16_PEAK_TFLOPS, 6),
+                "flops_per_query": MLP_FLOPS_QUERY, "flops_per_training_row": 3 * MLP_FLOPS_QUERY},
+    }
 
 
 def copy_calibration(mib):

@@ -19,7 +19,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libbbgpu.so")
 ARCH = os.environ.get("BBGPU_ARCH", "gfx950")
 
-SOURCES = ["host.cpp", "tables.cpp", "kernels.hip"]
+SOURCES = ["host.cpp", "tables.cpp", "kernels.hip", "nrc.hip"]
 HEADERS = ["bb_math.h", "bb_scene.h", "bb_shade.h", os.path.join("..", "..", "include", "bbgpu.h")]
 
 FLAGS = [

@@ -12,7 +12,7 @@ from ctypes import POINTER, c_char_p, c_double, c_float, c_int32, c_size_t, c_ui
 
 import numpy as np
 
-ABI_VERSION = 2  # BBGPU_ABI_VERSION of include/bbgpu.h that these structures mirror
+ABI_VERSION = 4  # BBGPU_ABI_VERSION of include/bbgpu.h that these structures mirror
 
 LIB_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(LIB_DIR, "libbbgpu.so")
@@ -181,6 +181,27 @@ class BbRenderStats(ctypes.Structure):
     ]
 
 
+class BbCacheParams(ctypes.Structure):
+    _fields_ = [
+        ("query_bounce", c_int32),
+        ("train_stride", c_int32),
+        ("train_extra_bounces", c_int32),
+        ("train_batch", c_int32),
+        ("train_steps", c_int32),
+    ]
+
+
+class BbCacheStats(ctypes.Structure):
+    _fields_ = [
+        ("ms_total", c_double),
+        ("ms_query", c_double),
+        ("ms_train", c_double),
+        ("queries", c_int64),
+        ("train_rows", c_int64),
+        ("loss", c_float),
+    ]
+
+
 RAY_DTYPE = np.dtype([("o", "<f4", 3), ("t_min", "<f4"), ("d", "<f4", 3), ("t_max", "<f4")])
 HIT_DTYPE = np.dtype([("prim", "<i4"), ("t", "<f4"), ("u", "<f4"), ("v", "<f4")])
 
@@ -196,6 +217,14 @@ EXPORTS = [
     "bbgpu_trace",
     "bbgpu_render",
     "bbgpu_look_at",
+    "bbgpu_nrc_param_count",
+    "bbgpu_nrc_create",
+    "bbgpu_nrc_destroy",
+    "bbgpu_nrc_get_weights",
+    "bbgpu_nrc_set_weights",
+    "bbgpu_nrc_query",
+    "bbgpu_nrc_train",
+    "bbgpu_render_cached",
 ]
 
 
@@ -242,6 +271,8 @@ def load_library(path: str | None = None):
         c_int32,
     ]
     lib.bbgpu_look_at.argtypes = [POINTER(c_float), POINTER(c_float), POINTER(c_float), POINTER(c_float)]
+    lib.bbgpu_render_cached.argtypes = [c_void_p, c_void_p, POINTER(BbCameraDesc), POINTER(BbRenderParams),
+                                        POINTER(BbCacheParams), c_void_p, c_void_p, c_void_p, POINTER(BbCacheStats)]
     for name in EXPORTS:
         getattr(lib, name).restype = c_int32
     if lib.bbgpu_abi_version() != ABI_VERSION:

@@ -522,6 +522,33 @@ struct PathBufs
 struct RenderConsts
 {
     int32_t n_paths, npix, width, seed, sample0, max_bounces, rr_min_bounces;
+    int32_t train_stride; // cached rendering: every train_stride-th path is a training path (0: none)
+};
+
+// Path flags: bounce (bits 0-15), the last BSDF sample was specular (16),
+// query the cache at the next vertex (17), training path (18), and for a
+// training path the vertex at which it stops (bits 20-27, 255 = not yet known)
+constexpr int32_t kFlagSpecular = 1 << 16;
+constexpr int32_t kFlagQueryNext = 1 << 17;
+constexpr int32_t kFlagTraining = 1 << 18;
+constexpr int32_t kStopShift = 20;
+constexpr int32_t kStopUnknown = 255;
+
+// Buffers of the cached path tracer (radiance cache queries and training
+// vertices); raw feature rows are the cache's 16-float inputs
+struct CacheBufs
+{
+    float* qraw;      // [query][16]
+    float4* qbeta;    // [query] throughput at the query vertex, path id
+    float* qpred;     // [query][3]
+    float* traw;      // [training path][vertex][16]
+    float4* tsnap;    // [training path][vertex][2]: L after the vertex's emission, throughput
+    int32_t* tcount;  // [training path] recorded vertices
+    float* rows;      // [row][16] compacted training rows
+    float* targets;   // [row][3]
+    int32_t* counters; // [0] queries, [1] training rows
+    f3 wmin, winv;    // position normalisation to [0, 1]^3
+    int32_t query_bounce, extra, max_vertices, max_rows;
 };
 
 // UniDirectionalRayIntegrator::Render per (pixel, sample) start
@@ -561,7 +588,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_raygen(GpuCamera cam, RenderCon
     P.L[i] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
     P.beta[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     P.rng[i] = st;
-    P.flags[i] = 0;
+    P.flags[i] = (rc.train_stride > 0 && i % rc.train_stride == 0) ? (kFlagTraining | (kStopUnknown << kStopShift)) : 0;
 }
 
 struct ClosestSrc
@@ -677,7 +704,30 @@ BB_D SurfaceHit surface_at(const SceneView& S, int32_t slot, float t, float u, f
 
 // PathIntegrator::Li, one bounce (path_integrator.cpp:38-147) and the
 // unoccluded part of SampleDirectLight (:152-194).
-__global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView S, RenderConsts rc, PathBufs P, int depth)
+// the cache's input row at a surface vertex: position in [0, 1]^3, direction
+// towards the previous vertex, shading normal, roughness, albedo
+BB_D void cache_features(const CacheBufs& C, const GpuMaterial& m, f3 p, f3 wo, f3 ns, float* r)
+{
+    const f3 q = (p - C.wmin) * C.winv;
+    r[0] = q.x;
+    r[1] = q.y;
+    r[2] = q.z;
+    r[3] = wo.x;
+    r[4] = wo.y;
+    r[5] = wo.z;
+    r[6] = ns.x;
+    r[7] = ns.y;
+    r[8] = ns.z;
+    r[9] = m.kind == MK_LAMBERT ? 1.0f : sqrtf(fmaxf(m.ax, m.ay));
+    const spec a = m.kind == MK_METALLIC_ROUGHNESS ? m.color : m.r;
+    r[10] = a.r;
+    r[11] = a.g;
+    r[12] = a.b;
+    r[13] = r[14] = r[15] = 0.0f;
+}
+
+template <bool CACHE>
+__global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView S, RenderConsts rc, PathBufs P, int depth, CacheBufs C)
 {
     const int in = depth & 1, out = in ^ 1;
     const int n = P.counts[4 * depth];
@@ -700,7 +750,8 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView S, RenderConsts
             uint64_t st = P.rng[pid];
             int32_t fl = P.flags[pid];
             int32_t bounce = fl & 0xffff;
-            bool specular = (fl >> 16) & 1;
+            bool specular = (fl & kFlagSpecular) != 0;
+            bool query_next = false;
             spec L = mks(Lv.x, Lv.y, Lv.z);
             float eta_scale = Lv.w;
             spec beta = mks(Bv.x, Bv.y, Bv.z);
@@ -741,7 +792,38 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView S, RenderConsts
                         }
                     }
                 }
-                if (bounce++ < rc.max_bounces) {
+                bool stop_here = false;
+                if (CACHE) {
+                    // the radiance cache replaces the rest of a rendering path at
+                    // its query vertex; a training path continues `extra` vertices
+                    // further and records every vertex it continues from
+                    const int32_t mi = S.slot_shade[slot].w;
+                    const bool training = (fl & kFlagTraining) != 0;
+                    int32_t stop = (fl >> kStopShift) & 0xff;
+                    if (training) {
+                        if ((fl & kFlagQueryNext) && stop == kStopUnknown) stop = min(depth + C.extra, 254);
+                        fl = (fl & ~(0xff << kStopShift)) | (stop << kStopShift);
+                        if (depth >= stop) {
+                            stop_here = true;
+                        } else if (bounce < rc.max_bounces && mi >= 0) {
+                            const int tp = pid / rc.train_stride;
+                            const int32_t nv = C.tcount[tp];
+                            if (nv < C.max_vertices) {
+                                const size_t rec = size_t(tp) * C.max_vertices + nv;
+                                cache_features(C, S.materials[mi], sh.point, wo, sh.ns, C.traw + 16 * rec);
+                                C.tsnap[2 * rec] = make_float4(L.r, L.g, L.b, 0.0f);
+                                C.tsnap[2 * rec + 1] = make_float4(beta.r, beta.g, beta.b, 0.0f);
+                                C.tcount[tp] = nv + 1;
+                            }
+                        }
+                    } else if ((fl & kFlagQueryNext) && mi >= 0) {
+                        const int q = atomicAdd(&C.counters[0], 1);
+                        cache_features(C, S.materials[mi], sh.point, wo, sh.ns, C.qraw + 16 * size_t(q));
+                        C.qbeta[q] = make_float4(beta.r, beta.g, beta.b, __int_as_float(pid));
+                        stop_here = true;
+                    }
+                }
+                if (!stop_here && bounce++ < rc.max_bounces) {
                     const int32_t mi = S.slot_shade[slot].w;
                     if (mi < 0) {
                         // null material: continue straight (path_integrator.cpp:95-100)
@@ -797,6 +879,9 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView S, RenderConsts
                         if (ok) {
                             const f3 wi = from_local(fr, s.wi);
                             specular = (s.flags & BX_SPECULAR) != 0;
+                            // query after the first non-specular bounce, at vertex
+                            // query_bounce at the latest
+                            if (CACHE) query_next = !specular || depth + 1 >= C.query_bounce;
                             if (s.flags & BX_TRANSMISSION) eta_scale *= sqr(s.eta);
                             prev_pdf = s.pdf;
                             beta = beta * ((s.f * absdot(sh.ns, wi)) / s.pdf);
@@ -820,7 +905,8 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView S, RenderConsts
             P.L[pid] = make_float4(L.r, L.g, L.b, eta_scale);
             P.beta[pid] = make_float4(beta.r, beta.g, beta.b, prev_pdf);
             P.rng[pid] = st;
-            P.flags[pid] = (bounce & 0xffff) | (specular ? (1 << 16) : 0);
+            P.flags[pid] = (bounce & 0xffff) | (specular ? kFlagSpecular : 0) | (query_next ? kFlagQueryNext : 0) |
+                           (fl & (kFlagTraining | (0xff << kStopShift)));
         }
         const int ss = wave_alloc(shadow, shadow_count);
         if (shadow) {
@@ -834,6 +920,48 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView S, RenderConsts
             P.ray[out][2 * ns] = make_float4(new_o.x, new_o.y, new_o.z, 0.0f);
             P.ray[out][2 * ns + 1] = make_float4(new_d.x, new_d.y, new_d.z, 0.0f);
         }
+    }
+}
+
+// cached rendering: L += throughput x max(cache, 0) for every query
+__global__ __launch_bounds__(kShadeBlock) void k_cache_apply(CacheBufs C, float4* __restrict__ L)
+{
+    const int n = C.counters[0];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float4 b = C.qbeta[i];
+        const int pid = __float_as_int(b.w);
+        const float* y = C.qpred + 3 * size_t(i);
+        float4 l = L[pid];
+        l.x = l.x + b.x * fmaxf(y[0], 0.0f);
+        l.y = l.y + b.y * fmaxf(y[1], 0.0f);
+        l.z = l.z + b.z * fmaxf(y[2], 0.0f);
+        L[pid] = l;
+    }
+}
+
+// training rows of the pass: at each recorded vertex of a training path the
+// radiance it reflected towards the previous vertex, (L_final - L_vertex) /
+// throughput_vertex per channel; vertices with a vanishing throughput or a
+// non-finite target are skipped
+__global__ __launch_bounds__(kShadeBlock) void k_cache_targets(CacheBufs C, const float4* __restrict__ L, int n_train,
+                                                                int stride)
+{
+    const int tp = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tp >= n_train) return;
+    const float4 lf = L[tp * stride];
+    const int nv = C.tcount[tp];
+    for (int v = 0; v < nv; ++v) {
+        const size_t rec = size_t(tp) * C.max_vertices + v;
+        const float4 ls = C.tsnap[2 * rec], bs = C.tsnap[2 * rec + 1];
+        if (!(bs.x > 1e-6f && bs.y > 1e-6f && bs.z > 1e-6f)) continue;
+        const float tr = (lf.x - ls.x) / bs.x, tg = (lf.y - ls.y) / bs.y, tb = (lf.z - ls.z) / bs.z;
+        if (!(isfinite(tr) && isfinite(tg) && isfinite(tb))) continue;
+        const int r = atomicAdd(&C.counters[1], 1);
+        if (r >= C.max_rows) continue;
+        for (int k = 0; k < 16; ++k) C.rows[16 * size_t(r) + k] = C.traw[16 * rec + k];
+        C.targets[3 * size_t(r)] = fmaxf(tr, 0.0f);
+        C.targets[3 * size_t(r) + 1] = fmaxf(tg, 0.0f);
+        C.targets[3 * size_t(r) + 2] = fmaxf(tb, 0.0f);
     }
 }
 
@@ -894,9 +1022,18 @@ struct Workspace
     void* stage = nullptr;
 };
 
+struct CacheWork
+{
+    size_t paths = 0, train_paths = 0;
+    int max_vertices = 0;
+    std::vector<void*> allocs;
+    CacheBufs C{};
+};
+
 struct SceneExtra
 {
     Workspace ws;
+    CacheWork cache;
     float* filter_cdf = nullptr; // device copy of the Gaussian filter's CDFs
     float filter_sigma = 0.0f, filter_extent = 0.0f;
 };
@@ -1229,8 +1366,54 @@ int bbgpu_trace(BbScene* scene, const BbRay* rays_dev, int32_t n_rays, int32_t a
     return BBGPU_OK;
 }
 
-int bbgpu_render(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams* params, float* film_sum_dev,
-                 int32_t* film_count_dev, void* stream, BbRenderStats* stats, int32_t count_traversal)
+} // extern "C"
+
+namespace
+{
+
+// the radiance cache of a cached render
+struct CacheCtx
+{
+    BbNrc* nrc;
+    BbCacheParams cp;
+    BbCacheStats* stats;
+};
+
+bool ensure_cache(SceneExtra* x, size_t cap, int stride, int max_vertices)
+{
+    CacheWork& w = x->cache;
+    const size_t ntp = (cap + size_t(stride) - 1) / size_t(stride);
+    if (w.paths >= cap && w.train_paths >= ntp && w.max_vertices >= max_vertices) return true;
+    for (void* q : w.allocs) (void)hipFree(q);
+    w.allocs.clear();
+    w.paths = w.train_paths = 0;
+    w.max_vertices = 0;
+    auto get = [&](void** q, size_t bytes) {
+        if (hipMalloc(q, std::max<size_t>(bytes, 16)) != hipSuccess) return false;
+        w.allocs.push_back(*q);
+        return true;
+    };
+    CacheBufs& C = w.C;
+    const size_t rows = ntp * size_t(max_vertices);
+    const bool ok = get((void**)&C.qraw, cap * 64) && get((void**)&C.qbeta, cap * 16) && get((void**)&C.qpred, cap * 12) &&
+                    get((void**)&C.traw, rows * 64) && get((void**)&C.tsnap, rows * 32) &&
+                    get((void**)&C.tcount, ntp * 4) && get((void**)&C.rows, rows * 64) &&
+                    get((void**)&C.targets, rows * 12) && get((void**)&C.counters, 16);
+    if (!ok) {
+        for (void* q : w.allocs) (void)hipFree(q);
+        w.allocs.clear();
+        return false;
+    }
+    w.paths = cap;
+    w.train_paths = ntp;
+    w.max_vertices = max_vertices;
+    C.max_vertices = max_vertices;
+    C.max_rows = int32_t(rows);
+    return true;
+}
+
+int render_impl(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams* params, float* film_sum_dev,
+                int32_t* film_count_dev, void* stream, BbRenderStats* stats, int32_t count_traversal, CacheCtx* cache)
 {
     if (!scene || !camera || !params || !film_sum_dev || !film_count_dev) {
         bb::set_error("bbgpu_render: null argument");
@@ -1330,6 +1513,25 @@ int bbgpu_render(BbScene* scene, const BbCameraDesc* camera, const BbRenderParam
         HIP_OK(hipMemcpyAsync(film_count_dev, film_count_user, count_bytes, hipMemcpyHostToDevice, st));
     }
     if (timed) memset(stats, 0, sizeof(*stats));
+    const int stride = cache ? std::max(1, cache->cp.train_stride) : 0;
+    CacheBufs* C = nullptr;
+    if (cache) {
+        if (!ensure_cache(x, size_t(cap), stride, p.max_bounces + 1)) {
+            bb::set_error("bbgpu_render_cached: cache workspace allocation failed");
+            return BBGPU_ERR_NOMEM;
+        }
+        C = &x->cache.C;
+        const f3 lo = mk3(V.root_box[0], V.root_box[1], V.root_box[2]);
+        const f3 hi = mk3(V.root_box[3], V.root_box[4], V.root_box[5]);
+        const f3 ext = hi - lo;
+        C->wmin = lo;
+        C->winv = mk3(1.0f / std::max(ext.x, 1e-6f), 1.0f / std::max(ext.y, 1e-6f), 1.0f / std::max(ext.z, 1e-6f));
+        C->query_bounce = std::max(1, cache->cp.query_bounce);
+        C->extra = std::max(0, cache->cp.train_extra_bounces);
+        if (cache->stats) memset(cache->stats, 0, sizeof(*cache->stats));
+    }
+    CacheBufs Cv{};
+    if (C) Cv = *C;
 
     EventPool pool;
     struct Span
@@ -1342,6 +1544,7 @@ int bbgpu_render(BbScene* scene, const BbCameraDesc* camera, const BbRenderParam
 
     hipEvent_t t_begin = nullptr;
     if (timed) t_begin = mark();
+    hipEvent_t t_begin_all = (cache && cache->stats) ? mark() : nullptr;
     hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, st, P.err, 1);
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, P.counters, 8);
     const int nslots = 4 * (p.max_bounces + 2);
@@ -1357,9 +1560,15 @@ int bbgpu_render(BbScene* scene, const BbCameraDesc* camera, const BbRenderParam
         rc.sample0 = int32_t(p.sample_begin + s0);
         rc.max_bounces = p.max_bounces;
         rc.rr_min_bounces = p.rr_min_bounces;
+        rc.train_stride = stride;
         paths_total += n_paths;
         hipEvent_t a = timed ? mark() : nullptr;
         hipLaunchKernelGGL(k_zero_ints, dim3((nslots + 255) / 256), dim3(256), 0, st, P.counts, nslots);
+        const int n_train = cache ? int((n_paths + stride - 1) / stride) : 0;
+        if (cache) {
+            hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, st, Cv.counters, 4);
+            hipLaunchKernelGGL(k_zero_ints, dim3((n_train + 255) / 256), dim3(256), 0, st, Cv.tcount, n_train);
+        }
         hipLaunchKernelGGL(k_raygen, dim3(unsigned((n_paths + kShadeBlock - 1) / kShadeBlock)), dim3(kShadeBlock), 0, st,
                            cam, rc, P);
         if (timed) spans.push_back({ a, mark(), 2 });
@@ -1372,13 +1581,55 @@ int bbgpu_render(BbScene* scene, const BbCameraDesc* camera, const BbRenderParam
                                P.counts + 4 * depth + 2, csrc, P.gstack, lds_entries, refill_min, P.counters, P.err);
             hipEvent_t b = timed ? mark() : nullptr;
             if (timed) spans.push_back({ a, b, 0 });
-            hipLaunchKernelGGL(k_shade, dim3(g_shade), dim3(kShadeBlock), 0, st, V, rc, P, depth);
+            if (cache)
+                hipLaunchKernelGGL(k_shade<true>, dim3(g_shade), dim3(kShadeBlock), 0, st, V, rc, P, depth, Cv);
+            else
+                hipLaunchKernelGGL(k_shade<false>, dim3(g_shade), dim3(kShadeBlock), 0, st, V, rc, P, depth, Cv);
             hipEvent_t c2 = timed ? mark() : nullptr;
             if (timed) spans.push_back({ b, c2, 2 });
             const ShadowSrc ssrc{ P.sray, P.scontrib, P.L };
             hipLaunchKernelGGL(k_shadow, dim3(g_trace), dim3(kTraceBlock), lds, st, V, P.counts + 4 * depth + 1,
                                P.counts + 4 * depth + 3, ssrc, P.gstack, lds_entries, refill_min, P.counters, P.err);
             if (timed) spans.push_back({ c2, mark(), 1 });
+        }
+        if (cache) {
+            // cache queries of the rendering paths, then this pass's training rows
+            int cnt[2] = { 0, 0 };
+            hipEvent_t q0 = mark();
+            HIP_OK(hipMemcpyAsync(cnt, Cv.counters, sizeof(cnt), hipMemcpyDeviceToHost, st));
+            HIP_OK(hipStreamSynchronize(st));
+            if (cnt[0] > 0) {
+                const int rq = bbgpu_nrc_query(cache->nrc, Cv.qraw, cnt[0], Cv.qpred, st);
+                if (rq != BBGPU_OK) return rq;
+                hipLaunchKernelGGL(k_cache_apply, dim3(grid_for(cnt[0], kShadeBlock)), dim3(kShadeBlock), 0, st, Cv, P.L);
+            }
+            hipLaunchKernelGGL(k_cache_targets, dim3((n_train + kShadeBlock - 1) / kShadeBlock), dim3(kShadeBlock), 0,
+                               st, Cv, P.L, n_train, stride);
+            hipEvent_t q1 = mark();
+            HIP_OK(hipMemcpyAsync(cnt, Cv.counters, sizeof(cnt), hipMemcpyDeviceToHost, st));
+            HIP_OK(hipStreamSynchronize(st));
+            const int rows = std::min(cnt[1], Cv.max_rows);
+            const int batch = cache->cp.train_batch > 0 ? cache->cp.train_batch : 65536;
+            float loss = 0.0f;
+            for (int s1 = 0; s1 < cache->cp.train_steps && s1 * batch < rows; ++s1) {
+                const int b = std::min(batch, rows - s1 * batch);
+                const int rt = bbgpu_nrc_train(cache->nrc, Cv.rows + 16 * size_t(s1) * batch,
+                                               Cv.targets + 3 * size_t(s1) * batch, b, 1,
+                                               cache->stats ? &loss : nullptr, nullptr, st);
+                if (rt != BBGPU_OK) return rt;
+            }
+            hipEvent_t q2 = mark();
+            if (cache->stats) {
+                HIP_OK(hipStreamSynchronize(st));
+                float ms_q = 0.0f, ms_t = 0.0f;
+                hipEventElapsedTime(&ms_q, q0, q1);
+                hipEventElapsedTime(&ms_t, q1, q2);
+                cache->stats->ms_query += ms_q;
+                cache->stats->ms_train += ms_t;
+                cache->stats->queries += cnt[0];
+                cache->stats->train_rows += rows;
+                cache->stats->loss = loss;
+            }
         }
         a = timed ? mark() : nullptr;
         hipLaunchKernelGGL(k_film, dim3((npix + kShadeBlock - 1) / kShadeBlock), dim3(kShadeBlock), 0, st, rc, P.L,
@@ -1396,7 +1647,7 @@ int bbgpu_render(BbScene* scene, const BbCameraDesc* camera, const BbRenderParam
             }
         }
     }
-    hipEvent_t t_end = timed ? mark() : nullptr;
+    hipEvent_t t_end = (timed || (cache && cache->stats)) ? mark() : nullptr;
     if (host_film) {
         HIP_OK(hipMemcpyAsync(film_sum_user, film_sum_dev, sum_bytes, hipMemcpyDeviceToHost, st));
         HIP_OK(hipMemcpyAsync(film_count_user, film_count_dev, count_bytes, hipMemcpyDeviceToHost, st));
@@ -1441,7 +1692,39 @@ int bbgpu_render(BbScene* scene, const BbCameraDesc* camera, const BbRenderParam
             stats->shadow.accepts = c[5];
         }
     }
+    if (cache && cache->stats) {
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, t_begin_all, t_end);
+        cache->stats->ms_total = ms;
+    }
     return BBGPU_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int bbgpu_render(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams* params, float* film_sum_dev,
+                 int32_t* film_count_dev, void* stream, BbRenderStats* stats, int32_t count_traversal)
+{
+    return render_impl(scene, camera, params, film_sum_dev, film_count_dev, stream, stats, count_traversal, nullptr);
+}
+
+int bbgpu_render_cached(BbScene* scene, BbNrc* nrc, const BbCameraDesc* camera, const BbRenderParams* params,
+                        const BbCacheParams* cache_params, float* film_sum_dev, int32_t* film_count_dev, void* stream,
+                        BbCacheStats* stats)
+{
+    if (!nrc || !cache_params) {
+        bb::set_error("bbgpu_render_cached: null argument");
+        return BBGPU_ERR_INVALID;
+    }
+    const BbCacheParams& c = *cache_params;
+    if (c.query_bounce < 1 || c.train_stride < 1 || c.train_extra_bounces < 0 || c.train_batch < 0 || c.train_steps < 0) {
+        bb::set_error("bbgpu_render_cached: invalid cache parameters");
+        return BBGPU_ERR_INVALID;
+    }
+    CacheCtx ctx{ nrc, c, stats };
+    return render_impl(scene, camera, params, film_sum_dev, film_count_dev, stream, nullptr, 0, &ctx);
 }
 
 } // extern "C"
@@ -1456,6 +1739,7 @@ void release_scene_workspace(BbScene* s)
         if (g_extras[i].first == s) {
             free_workspace(g_extras[i].second->ws);
             if (g_extras[i].second->filter_cdf) hipFree(g_extras[i].second->filter_cdf);
+            for (void* q : g_extras[i].second->cache.allocs) (void)hipFree(q);
             delete g_extras[i].second;
             g_extras.erase(g_extras.begin() + long(i));
             return;

@@ -79,6 +79,21 @@ class GpuScene:
         )
         return st
 
+    def render_cached(self, nrc, camera, params, cache_params, film=None, stream=None):
+        """bbgpu_render_cached: (sum[h, w, 3], count[h, w], BbCacheStats) as numpy;
+        film (device tensors from new_film) accumulates across calls."""
+        s, c = film if film is not None else self.new_film(camera)
+        st = capi.BbCacheStats()
+        capi.check(
+            self.lib.bbgpu_render_cached(self.h, nrc.h, ctypes.byref(camera), ctypes.byref(params),
+                                         ctypes.byref(cache_params), ctypes.c_void_p(s.data_ptr()),
+                                         ctypes.c_void_p(c.data_ptr()), stream, ctypes.byref(st)),
+            "bbgpu_render_cached",
+            self.lib,
+        )
+        h, w = camera.height, camera.width
+        return s.cpu().numpy().reshape(h, w, 3), c.cpu().numpy().reshape(h, w), st
+
     def new_film(self, camera):
         torch = self.torch
         npix = camera.width * camera.height

@@ -519,3 +519,16 @@ def portable_random_rays(bmin, bmax, n, seed, t_max=np.inf):
     else:
         rays["t_max"] = np.asarray(t_max, F32)
     return rays
+
+
+def cache_params(query_bounce=2, train_stride=16, train_extra_bounces=4, train_batch=65536, train_steps=1):
+    """BbCacheParams of the cached path tracer (defaults: query after the first
+    non-specular bounce or at vertex 2, 1/16 training paths extended by 4
+    vertices, one Adam step of up to 65,536 rows per pass)."""
+    c = capi.BbCacheParams()
+    c.query_bounce = query_bounce
+    c.train_stride = train_stride
+    c.train_extra_bounces = train_extra_bounces
+    c.train_batch = train_batch
+    c.train_steps = train_steps
+    return c

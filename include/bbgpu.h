@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define BBGPU_ABI_VERSION 2
+#define BBGPU_ABI_VERSION 4
 
 /* status codes */
 enum {
@@ -289,6 +289,68 @@ int bbgpu_trace(BbScene* scene, const BbRay* rays_dev, int32_t n_rays, int32_t a
  * call returns when the render is complete. */
 int bbgpu_render(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams* params, float* film_sum_dev,
                  int32_t* film_count_dev, void* stream, BbRenderStats* stats, int32_t count_traversal);
+
+/* ------------------------------------------------------- radiance cache -- */
+
+/* A small MLP radiance cache trained online on path-tail radiance (no
+ * counterpart in Bulbit; the configuration of the cached path tracer):
+ * input = 16 floats per row (position in [0,1]^3, direction, normal,
+ * roughness, albedo, 3 unused), frequency-encoded to 74 values; hidden_layers
+ * hidden layers of width hidden_width with ReLU; 3 linear outputs (RGB).
+ * bf16 weights and activations, fp32 accumulation and master weights, Adam
+ * (beta1 0.9, beta2 0.99), relative L2 loss (y - t)^2 / (y^2 + 0.01). */
+typedef struct BbNrc BbNrc; /* opaque */
+
+typedef struct BbNrcDesc {
+    int32_t hidden_width;   /* 64 */
+    int32_t hidden_layers;  /* 4 */
+    float learning_rate;    /* Adam step size (0: 1e-3) */
+    uint32_t seed;          /* initial weights (Xavier uniform from PCG32) */
+} BbNrcDesc;
+
+/* number of fp32 parameters (the weight matrices, k-major, output padded to 16 columns) */
+int bbgpu_nrc_param_count(void);
+int bbgpu_nrc_create(const BbNrcDesc* desc, BbNrc** nrc_out);
+int bbgpu_nrc_destroy(BbNrc* nrc);
+int bbgpu_nrc_get_weights(BbNrc* nrc, float* host_out, int32_t count);
+int bbgpu_nrc_set_weights(BbNrc* nrc, const float* host_in, int32_t count);
+/* predicted radiance (3 floats per row) of n rows of 16 raw floats; device pointers */
+int bbgpu_nrc_query(BbNrc* nrc, const float* inputs_dev, int32_t n, float* out_dev, void* stream);
+/* one training step on n rows (targets: 3 floats per row, device pointers):
+ * the loss gradient over the batch, then (apply != 0) an Adam update.
+ * loss_out and grad_out (host, may be NULL; grad_out holds
+ * bbgpu_nrc_param_count() floats) make the call wait for the step. */
+int bbgpu_nrc_train(BbNrc* nrc, const float* inputs_dev, const float* targets_dev, int32_t n, int32_t apply,
+                    float* loss_out, float* grad_out, void* stream);
+
+/* The cached path tracer: rendering paths end at their query vertex (the
+ * vertex after the first non-specular bounce, or vertex query_bounce at the
+ * latest) with throughput x cache(vertex); every train_stride-th path is a
+ * training path that continues train_extra_bounces vertices further without
+ * the cache, and the reflected radiance it measured at each vertex trains
+ * the cache after the pass (train_steps Adam steps of train_batch rows). */
+typedef struct BbCacheParams {
+    int32_t query_bounce;         /* 2 */
+    int32_t train_stride;         /* 16 */
+    int32_t train_extra_bounces;  /* 4 */
+    int32_t train_batch;          /* rows per optimizer step (0: 65536) */
+    int32_t train_steps;          /* optimizer steps per pass (0: inference only) */
+} BbCacheParams;
+
+typedef struct BbCacheStats {
+    double ms_total;   /* the whole call on the stream */
+    double ms_query;   /* cache inference and its application */
+    double ms_train;   /* training rows and optimizer steps */
+    int64_t queries;
+    int64_t train_rows;
+    float loss;        /* relative L2 loss of the last step */
+} BbCacheStats;
+
+/* bbgpu_render with the radiance cache (same film and parameter contract;
+ * stats, a host pointer, may be NULL) */
+int bbgpu_render_cached(BbScene* scene, BbNrc* nrc, const BbCameraDesc* camera, const BbRenderParams* params,
+                        const BbCacheParams* cache_params, float* film_sum_dev, int32_t* film_count_dev, void* stream,
+                        BbCacheStats* stats);
 
 /* Transform::LookAt (include/bulbit/transform.h:112-122): position and
  * orientation quaternion (x, y, z, w) of a camera at `position` looking at

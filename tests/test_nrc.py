@@ -1,0 +1,273 @@
+"""The radiance-cache MLP: the CPU restatement (oracle/nrc_ref.py) pinned by
+float64 finite differences, and the bf16 MFMA kernels (bulbit_amd/csrc/nrc.hip)
+checked against it: forward values, the loss gradient over a batch, one Adam
+step, and that training fits a smooth radiance field."""
+import numpy as np
+import pytest
+
+from oracle import nrc_ref as ref
+
+
+def _batch(n, seed):
+    rng = np.random.default_rng(seed)
+    raw = np.zeros((n, 16), np.float32)
+    raw[:, 0:3] = rng.random((n, 3))
+    d = rng.normal(size=(n, 3))
+    raw[:, 3:6] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    nrm = rng.normal(size=(n, 3))
+    raw[:, 6:9] = nrm / np.linalg.norm(nrm, axis=1, keepdims=True)
+    raw[:, 9] = rng.random(n)
+    raw[:, 10:13] = rng.random((n, 3))
+    return raw
+
+
+def _field(raw):
+    """a smooth radiance field of position and normal"""
+    x, y, z = raw[:, 0], raw[:, 1], raw[:, 2]
+    base = 0.6 + 0.3 * np.sin(2 * np.pi * x) * np.cos(np.pi * y) + 0.2 * z
+    return np.stack([base, base * 0.8 + 0.1 * raw[:, 8], base * 0.5 + 0.2 * raw[:, 12]], 1).astype(np.float32)
+
+
+def _init_weights(seed):
+    rng = np.random.default_rng(seed)
+    w = []
+    for l, (k, n) in enumerate(ref.layer_shapes()):
+        a = np.sqrt(6.0 / ((74 if l == 0 else k) + (3 if l == ref.HIDDEN else n)))
+        m = rng.uniform(-a, a, (k, n))
+        if l == 0:
+            m[74:] = 0
+        if l == ref.HIDDEN:
+            m[:, 3:] = 0
+        w.append(m.ravel())
+    return np.concatenate(w).astype(np.float32)
+
+
+# ------------------------------------------------------------------ oracle
+
+def test_bf16_rounds_to_nearest_even():
+    x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, -2.5, 1e-3], np.float32)
+    out = ref.bf16(x)
+    assert out[0] == 1.0
+    assert out[1] == 1.0  # tie -> even
+    assert out[2] == np.float32(1.0 + 4 * 2 ** -8)  # tie -> even (up)
+    assert out[3] == -2.5
+    assert abs(out[4] - 1e-3) < 1e-3 * 2 ** -8
+
+
+def test_encoding_layout():
+    raw = _batch(5, 1).astype(np.float64)
+    e = ref.encode(raw)
+    assert e.shape == (5, 80)
+    assert np.allclose(e[:, 0], np.sin(np.pi * raw[:, 0])) and np.allclose(e[:, 1], np.cos(np.pi * raw[:, 0]))
+    assert np.allclose(e[:, 6], np.sin(2 * np.pi * raw[:, 0]))  # frequency-major: f = 1 starts at 6
+    assert np.allclose(e[:, 62], np.sin(1024 * np.pi * raw[:, 1]))
+    assert np.allclose(e[:, 64:74], raw[:, 3:13]) and np.all(e[:, 74:] == 0)
+
+
+def test_oracle_gradient_matches_finite_differences():
+    """The backward formulas (float64, relative-loss denominator held fixed
+    as NRC's stop-gradient) against central differences of the loss."""
+    raw = _batch(40, 2)
+    t = _field(raw)
+    w = _init_weights(3).astype(np.float64)
+    g = ref.grad64(w, raw, t)
+    y0 = ref.forward64(w, raw)
+
+    def frozen_loss(wv):
+        y = ref.forward64(wv, raw)
+        return float(np.mean((y - t) ** 2 / (y0 * y0 + 0.01)))
+
+    rng = np.random.default_rng(4)
+    offs = np.cumsum([0] + [k * n for k, n in ref.layer_shapes()])
+    for l, (k, n) in enumerate(ref.layer_shapes()):
+        cols = 3 if l == ref.HIDDEN else n
+        rows = 74 if l == 0 else k
+        for _ in range(12):
+            i = offs[l] + rng.integers(0, rows) * n + rng.integers(0, cols)
+            e = 1e-6
+            wp, wm = w.copy(), w.copy()
+            wp[i] += e
+            wm[i] -= e
+            fd = (frozen_loss(wp) - frozen_loss(wm)) / (2 * e)
+            assert abs(fd - g[i]) <= 1e-6 + 1e-4 * abs(g[i]), (l, i, fd, g[i])
+
+
+def test_bf16_oracle_tracks_float64():
+    raw = _batch(256, 5)
+    t = _field(raw)
+    w = _init_weights(6)
+    y, _ = ref.forward(w, raw)
+    y64 = ref.forward64(w.astype(np.float64), raw)
+    assert np.max(np.abs(y - y64)) < 0.05 * max(1.0, np.max(np.abs(y64)))
+    _, g = ref.loss_and_grad(w, raw, t)
+    g64 = ref.grad64(w.astype(np.float64), raw, t)
+    cos = g @ g64 / np.linalg.norm(g) / np.linalg.norm(g64)
+    assert cos > 0.99
+
+
+# --------------------------------------------------------------------- GPU
+
+@pytest.fixture(scope="module")
+def nrc():
+    from bulbit_amd.nrc import Nrc
+
+    return Nrc(seed=7)
+
+
+@pytest.mark.gpu
+def test_nrc_forward_matches_oracle(nrc):
+    w = _init_weights(8)
+    nrc.set_weights(w)
+    raw = _batch(1000, 9)  # not a multiple of the 64-row tile
+    y = nrc.query(raw)
+    yr, _ = ref.forward(w, raw)
+    rel = np.abs(y - yr) / np.maximum(np.abs(yr), 0.05)
+    print(f"[nrc] forward: median rel {np.median(rel):.2e}, p99.5 {np.percentile(rel, 99.5):.2e}", flush=True)
+    assert np.median(rel) < 1e-3
+    assert np.percentile(rel, 99.5) < 2e-2
+
+
+@pytest.mark.gpu
+def test_nrc_gradient_matches_oracle(nrc):
+    w = _init_weights(10)
+    nrc.set_weights(w)
+    raw = _batch(3000, 11)
+    t = _field(raw)
+    loss, g = nrc.train(raw, t, apply=False, want_grad=True)
+    lr, gr = ref.loss_and_grad(w, raw, t)
+    print(f"[nrc] loss gpu {loss:.6f} oracle {lr:.6f}", flush=True)
+    assert abs(loss - lr) <= 2e-3 * lr
+    cos = g @ gr / np.linalg.norm(g) / np.linalg.norm(gr)
+    ratio = np.linalg.norm(g) / np.linalg.norm(gr)
+    print(f"[nrc] gradient cosine {cos:.6f}, norm ratio {ratio:.5f}", flush=True)
+    assert cos > 0.999 and abs(ratio - 1) < 0.01
+    # the padded output columns get no gradient
+    last = np.asarray(g[-64 * 16:]).reshape(64, 16)
+    assert np.all(last[:, 3:] == 0)
+
+
+@pytest.mark.gpu
+def test_nrc_adam_step_matches_oracle(nrc):
+    w = _init_weights(12)
+    nrc.set_weights(w)
+    raw = _batch(2048, 13)
+    t = _field(raw)
+    _, g = nrc.train(raw, t, apply=False, want_grad=True)
+    nrc.set_weights(w)  # resets nothing of Adam's moments: they are still zero
+    from bulbit_amd.nrc import Nrc
+
+    fresh = Nrc(seed=1, learning_rate=1e-3)
+    fresh.set_weights(w)
+    fresh.train(raw, t, apply=True)
+    w1 = fresh.weights()
+    expect, _, _ = ref.adam(w.astype(np.float64), 0.0, 0.0, g.astype(np.float64), 1, 1e-3)
+    mask = np.ones_like(w, bool)
+    mask[-64 * 16:] = (np.arange(64 * 16) % 16) < 3
+    assert np.allclose(w1[mask], expect[mask], rtol=1e-5, atol=1e-6)
+    assert np.all(w1[~mask] == 0)
+    fresh.close()
+
+
+@pytest.mark.gpu
+def test_nrc_learns_a_smooth_field():
+    from bulbit_amd.nrc import Nrc
+
+    net = Nrc(seed=21, learning_rate=1e-2)
+    test_raw = _batch(4096, 99)
+    test_t = _field(test_raw)
+
+    def test_loss():
+        y = net.query(test_raw)
+        return float(np.mean((y - test_t) ** 2 / (y * y + 0.01)))
+
+    before = test_loss()
+    for step in range(300):
+        raw = _batch(8192, 1000 + step)
+        net.train(raw, _field(raw))
+    after = test_loss()
+    print(f"[nrc] held-out relative L2 {before:.4f} -> {after:.5f}", flush=True)
+    assert after < before / 10
+    net.close()
+
+
+# ------------------------------------------------------ cached path tracer
+
+def _relmse(x, r):
+    return float(np.mean((x.astype(np.float64) - r) ** 2 / (r.astype(np.float64) ** 2 + 1e-2)))
+
+
+@pytest.mark.gpu
+def test_cached_render_with_all_paths_training_equals_plain():
+    """Training paths that never reach their stop vertex are plain paths: the
+    cache logic draws no random numbers, so with every path a training path
+    and 8 extra vertices the image is the plain render, bit for bit."""
+    from bulbit_amd import scenes
+    from bulbit_amd.gpu import GpuScene
+    from bulbit_amd.nrc import Nrc
+
+    sc, cam = scenes.icosphere_scene(width=160, height=90, level=4, n_lights=8)
+    g = GpuScene(sc)
+    p = scenes.render_params(4)
+    s0, c0, _ = g.render(cam, p)
+    net = Nrc(seed=3)
+    s1, c1, st = g.render_cached(net, cam, p, scenes.cache_params(train_stride=1, train_extra_bounces=8,
+                                                                   train_steps=0))
+    assert np.array_equal(c0, c1)
+    assert np.array_equal(s0.view(np.int32), s1.view(np.int32))
+    assert st.queries == 0 and st.train_rows > 0
+    net.close()
+
+
+@pytest.mark.gpu
+def test_cached_render_with_a_zero_cache_keeps_the_path_prefix():
+    """With a cache that predicts 0, a rendering path contributes exactly what
+    the plain path gathered before its query vertex, so every pixel is at
+    most the plain render's and the query vertices are where they should be
+    (after the first non-specular bounce)."""
+    from bulbit_amd import scenes
+    from bulbit_amd.gpu import GpuScene
+    from bulbit_amd.nrc import Nrc
+
+    sc, cam = scenes.readme_scene(width=160, height=90)
+    g = GpuScene(sc)
+    p = scenes.render_params(4)
+    s0, c0, _ = g.render(cam, p)
+    net = Nrc(seed=3)
+    net.set_weights(np.zeros(net.n_params, np.float32))
+    n_paths = 160 * 90 * 4
+    s1, c1, st = g.render_cached(net, cam, p, scenes.cache_params(train_stride=n_paths + 1, train_steps=0))
+    assert np.array_equal(c0, c1)
+    assert np.all(s1 <= s0 * (1 + 1e-6) + 1e-7)
+    # the diffuse sphere's primary hits all query at vertex 1
+    assert 0 < st.queries <= n_paths
+    assert s1.mean() < s0.mean()
+    net.close()
+
+
+@pytest.mark.gpu
+def test_cached_render_beats_plain_at_equal_samples_after_training():
+    """README scene: after training online over 48 one-sample frames, 16 more
+    cached frames have lower relMSE against a 1024-spp plain reference than
+    16 plain frames (the cache replaces the noisy tails of the paths)."""
+    from bulbit_amd import scenes
+    from bulbit_amd.gpu import GpuScene
+    from bulbit_amd.nrc import Nrc
+
+    sc, cam = scenes.readme_scene(width=240, height=135)
+    g = GpuScene(sc)
+    rs, rc, _ = g.render(cam, scenes.render_params(1024, sample_begin=100000))
+    ref = rs / rc[..., None]
+    net = Nrc(seed=5, learning_rate=1e-2)
+    cp = scenes.cache_params(train_batch=16384, train_steps=4)
+    for f in range(48):
+        _, _, st = g.render_cached(net, cam, scenes.render_params(1, sample_begin=f), cp)
+    print(f"[nrc] training loss after 48 frames {st.loss:.4f}, rows/frame {st.train_rows}", flush=True)
+    film = g.new_film(cam)
+    for f in range(48, 64):
+        s1, c1, _ = g.render_cached(net, cam, scenes.render_params(1, sample_begin=f), cp, film=film)
+    s0, c0, _ = g.render(cam, scenes.render_params(16, sample_begin=48))
+    e_cache = _relmse(s1 / c1[..., None], ref)
+    e_plain = _relmse(s0 / c0[..., None], ref)
+    print(f"[nrc] relMSE at 16 spp: cached {e_cache:.5f}, plain {e_plain:.5f}", flush=True)
+    assert e_cache < e_plain
+    net.close()
