@@ -159,13 +159,14 @@ def main():
         "relmse_equal_time_cache_vs_plain": None,
     }
 
+    ref_c2 = None
     if args.ref_spp > 0:
         test = s_dev.cpu().numpy().reshape(args.height, args.width, 3) / \
             c_dev.cpu().numpy().reshape(args.height, args.width, 1).astype(np.float32)
-        out["quality"] = quality(g, sc, cam, args, test)
+        out["quality"], ref_c2 = quality(g, sc, cam, args, test)
     if args.cache_frames > 0:
-        cq = cache_comparison(args)
-        out["relmse_equal_time_cache_vs_plain"] = cq["ratio"]
+        cq = cache_comparisons(args, g, cam, ref_c2)
+        out["relmse_equal_time_cache_vs_plain"] = cq["readme"]["ratio"]
         out["radiance_cache"] = cq
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, cam, params, args, samples_per_step)
@@ -214,33 +215,26 @@ def quality(g, sc, cam, args, test):
         out["cpu_reference_seconds"] = round(secs, 2)
         log(f"[bench] crop {rect}: relMSE vs CPU {args.ref_spp}-spp {out['relmse_vs_cpu_reference_crop']}, "
             f"GPU vs CPU reference {out['relmse_gpu_vs_cpu_reference_crop']}")
-    return out
+    return out, ref
 
 
 MLP_FLOPS_QUERY = 2 * (74 * 64 + 3 * 64 * 64 + 64 * 3)  # per query, 74 inputs, 4 x 64 hidden, 3 outputs
 BF16_PEAK_TFLOPS = 2500.0
 
 
-def cache_comparison(args):
-    """Radiance cache vs plain path tracing at equal wall-clock time: the
-    README scene at 1920x1080, one sample per frame, accumulated over frames;
-    relMSE against a plain 4096-spp frame with disjoint samples.  Plain PT
-    renders `cache_frames` frames; the cached tracer (trained online on 1/16
-    of its paths) renders frames until it has used the same time."""
+def cache_comparison(args, g, cam, ref, name, query_primary):
+    """Radiance cache vs plain path tracing at equal wall-clock time: one
+    sample per frame, accumulated over frames, relMSE against `ref` (a plain
+    render with disjoint samples).  Plain PT renders `cache_frames` frames;
+    the cached tracer (trained online on 1/16 of its paths) renders frames
+    until it has used the same time."""
     import numpy as np
     import torch
     from bulbit_amd import scenes
-    from bulbit_amd.gpu import GpuScene
     from bulbit_amd.nrc import Nrc
 
-    w, h = 1920, 1080
-    sc, cam = scenes.readme_scene(width=w, height=h)
-    g = GpuScene(sc)
-    ref_spp = max(args.ref_spp, 1)
-    t0 = time.time()
-    rs, rc, _ = g.render(cam, scenes.render_params(ref_spp, sample_begin=1 << 20))
-    ref = rs / rc[..., None]
-    log(f"[bench] cache comparison: {ref_spp}-spp reference in {time.time() - t0:.1f}s")
+    w, h = cam.width, cam.height
+    base = 1 << 16  # frames use samples base + f, disjoint from the references
 
     def run(step, frames, budget=None):
         film = g.new_film(cam)
@@ -257,39 +251,40 @@ def cache_comparison(args):
         s, c = film[0].cpu().numpy().reshape(h, w, 3), film[1].cpu().numpy().reshape(h, w)
         return relmse(s / c[..., None], ref), done, secs
 
-    # warm both paths once (kernel loading), outside the timed runs
+    cp = scenes.cache_params(train_batch=65536, train_steps=2, query_primary=query_primary)
+    # warm both paths (kernel loading) outside the timed runs
     g.render(cam, scenes.render_params(1, sample_begin=1 << 24))
     net = Nrc(seed=1, learning_rate=1e-2)
-    cp = scenes.cache_params(train_batch=65536, train_steps=2)
     g.render_cached(net, cam, scenes.render_params(1, sample_begin=1 << 24), cp)
     net.close()
 
     e_plain, n_plain, t_plain = run(
-        lambda f, film: g.render_device(cam, scenes.render_params(1, sample_begin=f), film[0].data_ptr(),
+        lambda f, film: g.render_device(cam, scenes.render_params(1, sample_begin=base + f), film[0].data_ptr(),
                                         film[1].data_ptr()), args.cache_frames)
     net = Nrc(seed=1, learning_rate=1e-2)
     stats = []
 
     def cached(f, film):
-        _, _, st = g.render_cached(net, cam, scenes.render_params(1, sample_begin=f), cp, film=film)
+        _, _, st = g.render_cached(net, cam, scenes.render_params(1, sample_begin=base + f), cp, film=film)
         stats.append(st)
 
     e_cache, n_cache, t_cache = run(cached, 100000, budget=t_plain)
     net.close()
     q = sum(st.queries for st in stats)
-    rows = sum(st.train_rows for st in stats)
+    rows = sum(min(st.train_rows, 2 * 65536) for st in stats)
     ms_q = sum(st.ms_query for st in stats)
     ms_t = sum(st.ms_train for st in stats)
-    tflops = (MLP_FLOPS_QUERY * q + 3 * MLP_FLOPS_QUERY * min(rows, len(stats) * 2 * 65536)) / max(ms_q + ms_t, 1e-9) / 1e9
-    log(f"[bench] equal time {t_plain:.2f}s: plain {n_plain} frames relMSE {e_plain:.5f}; cached {n_cache} frames "
-        f"relMSE {e_cache:.5f} ({q} queries, {rows} training rows, query {ms_q:.0f} ms, train {ms_t:.0f} ms)")
+    tflops = (MLP_FLOPS_QUERY * q + 3 * MLP_FLOPS_QUERY * rows) / max(ms_q + ms_t, 1e-9) / 1e9
+    log(f"[bench] {name}: equal time {t_plain:.2f}s: plain {n_plain} frames relMSE {e_plain:.6f}; cached {n_cache} "
+        f"frames relMSE {e_cache:.6f} ({q} queries, {rows} training rows, query {ms_q:.0f} ms, train {ms_t:.0f} ms)")
     return {
-        "workload": "README scene 1920x1080, 1 spp per frame, max_bounces 8; reference %d spp" % ref_spp,
+        "workload": name,
+        "query_at_primary_hit": bool(query_primary),
         "seconds": round(t_plain, 3),
         "plain_frames": n_plain,
-        "plain_relmse": round(e_plain, 6),
+        "plain_relmse": round(e_plain, 7),
         "cache_frames": n_cache,
-        "cache_relmse": round(e_cache, 6),
+        "cache_relmse": round(e_cache, 7),
         "ratio": round(e_cache / e_plain, 4),
         "cache_ms_query": round(ms_q, 1),
         "cache_ms_train": round(ms_t, 1),
@@ -297,6 +292,28 @@ def cache_comparison(args):
                 "frac": round(tflops / BF16_PEAK_TFLOPS, 6),
                 "flops_per_query": MLP_FLOPS_QUERY, "flops_per_training_row": 3 * MLP_FLOPS_QUERY},
     }
+
+
+def cache_comparisons(args, g_c2, cam_c2, ref_c2):
+    """The README scene as the cache configuration states it (query at the
+    first rough hit, which on a lone convex sphere is the primary hit), and
+    the 1080p icosphere scene, where indirect light is what the cache stands
+    in for (query from vertex 1 on)."""
+    from bulbit_amd import scenes
+    from bulbit_amd.gpu import GpuScene
+
+    sc, cam = scenes.readme_scene(width=1920, height=1080)
+    g = GpuScene(sc)
+    t0 = time.time()
+    rs, rc, _ = g.render(cam, scenes.render_params(max(args.ref_spp, 1), sample_begin=1 << 20))
+    log(f"[bench] README {args.ref_spp}-spp reference in {time.time() - t0:.1f}s")
+    out = {"readme": cache_comparison(args, g, cam, rs / rc[..., None],
+                                      "README scene 1920x1080, 1 spp per frame, max_bounces 8", True)}
+    g.close()
+    if ref_c2 is not None:
+        out["icosphere"] = cache_comparison(
+            args, g_c2, cam_c2, ref_c2, "displaced icosphere 1920x1080, 1 spp per frame, max_bounces 8", False)
+    return out
 
 
 def copy_calibration(mib):

@@ -9,11 +9,6 @@
 //                        {c2.min.z, c2.max.xyz} {ref1, ref2, split axis, 0}
 //                      ref >= 0: inner record; ref < 0: leaf whose first
 //                      slot is ~ref (its last slot carries kSlotLast)
-//   inner_box [n_inner] 32 B: the record's own node box {min.xyz, max.x}
-//                      {max.yz, 0, 0}, read when the node is taken off the
-//                      traversal stack
-//   leaf_box [n_slots] 32 B, same form: the box of the leaf that starts at
-//                      this slot (entries of other slots unused)
 //   tris    [n_slots]  3 x float4 = 48 B per BVH slot (leaf order):
 //                      {p0.xyz, prim}, {e1.xyz, sphere index},
 //                      {e2.xyz, kind | kSlotLast on a leaf's last slot}
@@ -112,9 +107,7 @@ struct GpuLight
 
 struct SceneView
 {
-    const float4* inner;     // 4 float4 per internal node
-    const float4* inner_box; // 2 float4 per internal node
-    const float4* leaf_box;  // 2 float4 per slot
+    const float4* inner; // 4 float4 per internal node
     const float4* tris;  // 3 float4 per slot
     const int4* slot_shade;
     const int32_t* slot_light;

@@ -565,13 +565,7 @@ bool create_scene(const BbSceneDesc& d, BbScene* s, std::string& err)
         }
     auto ref_of = [&](size_t c) { return nodes[c].count == 0 ? inner_index[c] : ~nodes[c].offset; };
     std::vector<float4> inner(size_t(4) * std::max(1, n_inner));
-    std::vector<float4> inner_box(size_t(2) * std::max(1, n_inner));
-    std::vector<float4> leaf_box(size_t(2) * std::max(1, ns), make_float4(0.0f, 0.0f, 0.0f, 0.0f));
     for (size_t i = 0; i < nodes.size(); ++i) {
-        const BbBvhNode& nd = nodes[i];
-        float4* bx = nd.count == 0 ? &inner_box[size_t(2) * inner_index[i]] : &leaf_box[size_t(2) * nd.offset];
-        bx[0] = make_float4(nd.bmin[0], nd.bmin[1], nd.bmin[2], nd.bmax[0]);
-        bx[1] = make_float4(nd.bmax[1], nd.bmax[2], 0.0f, 0.0f);
         if (nodes[i].count != 0) continue;
         const BbBvhNode& c1 = nodes[i + 1];
         const BbBvhNode& c2 = nodes[size_t(nodes[i].offset)];
@@ -605,8 +599,6 @@ bool create_scene(const BbSceneDesc& d, BbScene* s, std::string& err)
     SceneView& V = s->view;
     hipGetDevice(&s->device);
     if (!upload(s, inner.data(), inner.size(), &V.inner, err)) return false;
-    if (!upload(s, inner_box.data(), inner_box.size(), &V.inner_box, err)) return false;
-    if (!upload(s, leaf_box.data(), leaf_box.size(), &V.leaf_box, err)) return false;
     for (int k = 0; k < 3; ++k) {
         V.root_box[k] = nodes[0].bmin[k];
         V.root_box[3 + k] = nodes[0].bmax[k];

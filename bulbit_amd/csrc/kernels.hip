@@ -139,16 +139,15 @@ struct TraceCount
 };
 
 // Per-lane traversal state of BVH::RayCast (include/bulbit/bvh.h:101-155).
-// `cur` is the node to expand next: an inner record (>= 0), a leaf (~first
-// slot), or kNoRef (take the next stack entry).  `pend` is set when `cur`
-// came off the stack and its own box test is still to be done.
+// `cur` is the node to expand next, whose own box test has passed: an inner
+// record (>= 0), a leaf (~first slot), or kNoRef (take the next stack entry).
 constexpr int32_t kNoRef = INT32_MIN;
 
 struct Lane
 {
     f3 o, d, inv, dn;
     float l, t_min, t_max;
-    bool n0, n1, n2, pend;
+    bool n0, n1, n2;
     int32_t sp, cur;
     int32_t hit_slot;
     float hit_t, hit_u, hit_v;
@@ -168,7 +167,6 @@ BB_D void lane_start(Lane& L, f3 o, f3 d, float t_min, float t_max)
     L.l = normalize_m(L.dn);
     L.sp = 0;
     L.cur = kNoRef;
-    L.pend = false;
     L.hit_slot = -1;
     L.hit_t = L.hit_u = L.hit_v = 0.0f;
 }
@@ -184,14 +182,14 @@ BB_D void lane_root(const SceneView& S, Lane& L, TraceCount& c)
     L.cur = box_pass(h, L.t_max) ? S.root_ref : kNoRef;
 }
 
-// Traversal stack of node references: entries [0, lds_entries) in LDS
-// ([entry][thread] layout, conflict-free for a wave), deeper entries in a
-// global overflow area ([entry][global thread]); cap = BVH depth + 2 bounds
-// every push.
+// Traversal stack of (node reference, entry distance) pairs: entries
+// [0, lds_entries) in LDS ([entry][thread] layout, conflict-free for a
+// wave), deeper entries in a global overflow area ([entry][global thread]);
+// cap = BVH depth + 2 bounds every push.
 struct TraceStack
 {
-    int32_t* lds;
-    int32_t* gbl;
+    int2* lds;
+    int2* gbl;
     int32_t lds_entries;
     int32_t gstride;
     int32_t cap;
@@ -199,7 +197,7 @@ struct TraceStack
 
 // FULL: the whole stack fits in LDS (cap <= lds_entries), no overflow branch
 template <bool FULL>
-BB_D void stack_put(const TraceStack& s, int32_t sp, int32_t v)
+BB_D void stack_put(const TraceStack& s, int32_t sp, int2 v)
 {
     if (FULL || sp < s.lds_entries)
         s.lds[sp * kTraceBlock] = v;
@@ -208,81 +206,63 @@ BB_D void stack_put(const TraceStack& s, int32_t sp, int32_t v)
 }
 
 template <bool FULL>
-BB_D int32_t stack_get(const TraceStack& s, int32_t sp)
+BB_D int2 stack_get(const TraceStack& s, int32_t sp)
 {
     return (FULL || sp < s.lds_entries) ? s.lds[sp * kTraceBlock] : s.gbl[(sp - s.lds_entries) * s.gstride];
 }
 
+// a far child whose t_max-independent test failed, kept on the stack only
+// when counting so that it is counted when RayCast would pop and test it
+// (a signalling-NaN bit pattern: no arithmetic result has it)
+constexpr int32_t kFailedLo = 0x7fbfffff;
+
 // One step of RayCast for one lane.
 //
-// A node taken off the stack is box-tested with the current t_max, as
-// RayCast does at pop time; its own box comes from inner_box / leaf_box in
-// the same round trip as its record.  Expanding an inner record then tests
-// both children's boxes from the parent's 64 bytes: RayCast pushes the far
-// child and tests the near child with the current t_max, and the near test
-// here is that same test.  The far child is pushed only if it passes with
-// the current t_max (a box that fails fails again for any smaller t_max, so
-// RayCast would discard it when popped); with COUNT every far child is
-// pushed so that its pop-time test is counted as RayCast counts it.  Leaves
+// Expanding an inner record tests both children's boxes from the parent's
+// 64 bytes.  RayCast pushes the far child, then tests the near child with
+// the current t_max: the near test here is that same test.  The far child is
+// tested by RayCast when popped, with the t_max of that moment; here its
+// t_max-independent part and entry distance are computed now, a child that
+// fails already is not pushed (it would fail then too), and the pop applies
+// the remaining `lo > t_max` check with the then-current t_max.  So the same
+// nodes are expanded in the same order with the same outcomes, and leaves
 // test their primitives in stored order (the callbacks of BVH::Intersect /
-// IntersectAny, src/accel/bvh.cpp:299-361).  Returns true when the lane's
-// query is finished.
+// IntersectAny, src/accel/bvh.cpp:299-361).  With COUNT, every far child is
+// pushed and counted when popped, which reproduces RayCast's node-test count.
+// Returns true when the lane's query is finished.
 template <bool ANY, bool COUNT, bool FULL>
 BB_D bool trace_step(const SceneView& S, Lane& L, const TraceStack& stk, TraceCount& c, int* err)
 {
     if (L.cur >= 0) {
         const float4* r = S.inner + 4 * L.cur;
         const float4 a = r[0], b = r[1], e = r[2], m = r[3];
-        float4 ob0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), ob1 = ob0;
-        if (L.pend) {
-            ob0 = S.inner_box[2 * L.cur];
-            ob1 = S.inner_box[2 * L.cur + 1];
-        }
-        // the record's 64 bytes (and the own box) in one round trip
+        // the record's 64 bytes in one round trip
         asm volatile("" ::"v"(a.x), "v"(a.y), "v"(a.z), "v"(a.w), "v"(b.x), "v"(b.y), "v"(b.z), "v"(b.w), "v"(e.x),
-                     "v"(e.y), "v"(e.z), "v"(e.w), "v"(m.x), "v"(m.y), "v"(m.z), "v"(ob0.x), "v"(ob0.y), "v"(ob0.z),
-                     "v"(ob0.w), "v"(ob1.x), "v"(ob1.y));
-        bool go = true;
-        if (L.pend) {
-            if (COUNT) c.nodes++;
-            go = box_pass(box_part(mk3(ob0.x, ob0.y, ob0.z), mk3(ob0.w, ob1.x, ob1.y), L.o, L.t_min, L.inv, L.n0,
-                                   L.n1, L.n2),
-                          L.t_max);
+                     "v"(e.y), "v"(e.z), "v"(e.w), "v"(m.x), "v"(m.y), "v"(m.z));
+        const BoxHit h1 = box_part(mk3(a.x, a.y, a.z), mk3(a.w, b.x, b.y), L.o, L.t_min, L.inv, L.n0, L.n1, L.n2);
+        const BoxHit h2 = box_part(mk3(b.z, b.w, e.x), mk3(e.y, e.z, e.w), L.o, L.t_min, L.inv, L.n0, L.n1, L.n2);
+        const int32_t axis = __float_as_int(m.z);
+        const bool neg = axis == 0 ? L.n0 : (axis == 1 ? L.n1 : L.n2);
+        const int32_t near_ref = __float_as_int(neg ? m.y : m.x);
+        const int32_t far_ref = __float_as_int(neg ? m.x : m.y);
+        const BoxHit hn = neg ? h2 : h1;
+        const BoxHit hf = neg ? h1 : h2;
+        const bool push = COUNT || box_pass(hf, L.t_max);
+        if (push) {
+            if (L.sp >= stk.cap) {
+                atomicOr(err, 1);
+                return true;
+            }
+            stack_put<FULL>(stk, L.sp, make_int2(far_ref, hf.ok ? __float_as_int(hf.lo) : kFailedLo));
+            ++L.sp;
         }
-        if (go) {
-            const BoxHit h1 = box_part(mk3(a.x, a.y, a.z), mk3(a.w, b.x, b.y), L.o, L.t_min, L.inv, L.n0, L.n1, L.n2);
-            const BoxHit h2 = box_part(mk3(b.z, b.w, e.x), mk3(e.y, e.z, e.w), L.o, L.t_min, L.inv, L.n0, L.n1, L.n2);
-            const int32_t axis = __float_as_int(m.z);
-            const bool neg = axis == 0 ? L.n0 : (axis == 1 ? L.n1 : L.n2);
-            const int32_t near_ref = __float_as_int(neg ? m.y : m.x);
-            const int32_t far_ref = __float_as_int(neg ? m.x : m.y);
-            const BoxHit hn = neg ? h2 : h1;
-            const BoxHit hf = neg ? h1 : h2;
-            if (COUNT || box_pass(hf, L.t_max)) {
-                if (L.sp >= stk.cap) {
-                    atomicOr(err, 1);
-                    return true;
-                }
-                stack_put<FULL>(stk, L.sp, far_ref);
-                ++L.sp;
-            }
-            if (COUNT) c.nodes++;
-            if (box_pass(hn, L.t_max)) {
-                L.cur = near_ref;
-                L.pend = false;
-                return false;
-            }
+        if (COUNT) c.nodes++;
+        if (box_pass(hn, L.t_max)) {
+            L.cur = near_ref;
+            return false;
         }
     } else if (L.cur != kNoRef) {
-        bool go = true;
-        if (L.pend) {
-            const float4 lb0 = S.leaf_box[2 * ~L.cur], lb1 = S.leaf_box[2 * ~L.cur + 1];
-            if (COUNT) c.nodes++;
-            go = box_pass(box_part(mk3(lb0.x, lb0.y, lb0.z), mk3(lb0.w, lb1.x, lb1.y), L.o, L.t_min, L.inv, L.n0,
-                                   L.n1, L.n2),
-                          L.t_max);
-        }
-        for (int32_t slot = ~L.cur; go; ++slot) {
+        for (int32_t slot = ~L.cur;; ++slot) {
             const float4 t0 = S.tris[3 * slot];
             const float4 t1 = S.tris[3 * slot + 1];
             const float4 t2 = S.tris[3 * slot + 2];
@@ -313,11 +293,14 @@ BB_D bool trace_step(const SceneView& S, Lane& L, const TraceStack& stk, TraceCo
             if (kind & kSlotLast) break;
         }
     }
-    // take the next stack entry; its box is tested when it is expanded
+    // take the next stack entry; its deferred `lo > t_max` check decides
+    // whether it is expanded
     if (L.sp == 0) return true;
     --L.sp;
-    L.cur = stack_get<FULL>(stk, L.sp);
-    L.pend = true;
+    const int2 e = stack_get<FULL>(stk, L.sp);
+    if (COUNT) c.nodes++;
+    const bool pass = (!COUNT || e.y != kFailedLo) && !(__int_as_float(e.y) > L.t_max);
+    L.cur = pass ? e.x : kNoRef;
     return false;
 }
 
@@ -420,7 +403,7 @@ struct BatchSrc
     }
 };
 
-BB_D TraceStack make_stack(int32_t* lds_stack, int32_t* gstack, int lds_entries, int cap)
+BB_D TraceStack make_stack(int2* lds_stack, int2* gstack, int lds_entries, int cap)
 {
     TraceStack t;
     t.lds = lds_stack + threadIdx.x;
@@ -438,10 +421,10 @@ constexpr int kTraceSteps = BB_TRACE_STEPS;
 
 template <bool ANY, bool COUNT, bool FULL>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_batch(SceneView S, BatchSrc src, int n, int* fetch,
-                                                               int32_t* gstack, int lds_entries, int refill_min,
+                                                               int2* gstack, int lds_entries, int refill_min,
                                                                unsigned long long* counters, int* err)
 {
-    extern __shared__ int32_t lds_stack[];
+    extern __shared__ int2 lds_stack[];
     const TraceStack stk = make_stack(lds_stack, gstack, lds_entries, S.stack_depth);
     TraceCount c{ 0, 0, 0 };
     persistent_trace<ANY, COUNT, FULL, kTraceSteps>(S, src, n, fetch, stk, c, err, refill_min);
@@ -514,7 +497,7 @@ struct PathBufs
     float4* sray;    // 2 float4 per shadow slot: (o, t_max), (d, path id)
     float4* scontrib;
     int* counts;     // [depth][4]: active paths, shadow rays, closest fetch, shadow fetch
-    int32_t* gstack; // traversal stack overflow, [entry][persistent thread]
+    int2* gstack;    // traversal stack overflow, [entry][persistent thread]
     int* err;
     unsigned long long* counters; // [6]: closest nodes/prims/accepts, shadow nodes/prims/accepts
 };
@@ -526,10 +509,9 @@ struct RenderConsts
 };
 
 // Path flags: bounce (bits 0-15), the last BSDF sample was specular (16),
-// query the cache at the next vertex (17), training path (18), and for a
-// training path the vertex at which it stops (bits 20-27, 255 = not yet known)
+// training path (18), and for a training path the vertex at which it stops
+// (bits 20-27, 255 = not yet known)
 constexpr int32_t kFlagSpecular = 1 << 16;
-constexpr int32_t kFlagQueryNext = 1 << 17;
 constexpr int32_t kFlagTraining = 1 << 18;
 constexpr int32_t kStopShift = 20;
 constexpr int32_t kStopUnknown = 255;
@@ -548,7 +530,7 @@ struct CacheBufs
     float* targets;   // [row][3]
     int32_t* counters; // [0] queries, [1] training rows
     f3 wmin, winv;    // position normalisation to [0, 1]^3
-    int32_t query_bounce, extra, max_vertices, max_rows;
+    int32_t query_bounce, query_primary, extra, max_vertices, max_rows;
 };
 
 // UniDirectionalRayIntegrator::Render per (pixel, sample) start
@@ -634,10 +616,10 @@ struct ShadowSrc
 
 template <bool COUNT, bool FULL>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(SceneView S, const int* __restrict__ n_ptr, int* fetch,
-                                                                 ClosestSrc src, int32_t* gstack, int lds_entries, int refill_min,
+                                                                 ClosestSrc src, int2* gstack, int lds_entries, int refill_min,
                                                                  unsigned long long* counters, int* err)
 {
-    extern __shared__ int32_t lds_stack[];
+    extern __shared__ int2 lds_stack[];
     const TraceStack stk = make_stack(lds_stack, gstack, lds_entries, S.stack_depth);
     TraceCount c{ 0, 0, 0 };
     persistent_trace<false, COUNT, FULL, kTraceSteps>(S, src, *n_ptr, fetch, stk, c, err, refill_min);
@@ -646,10 +628,10 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(SceneView S, cons
 
 template <bool COUNT, bool FULL>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_shadow(SceneView S, const int* __restrict__ n_ptr, int* fetch,
-                                                                ShadowSrc src, int32_t* gstack, int lds_entries, int refill_min,
+                                                                ShadowSrc src, int2* gstack, int lds_entries, int refill_min,
                                                                 unsigned long long* counters, int* err)
 {
-    extern __shared__ int32_t lds_stack[];
+    extern __shared__ int2 lds_stack[];
     const TraceStack stk = make_stack(lds_stack, gstack, lds_entries, S.stack_depth);
     TraceCount c{ 0, 0, 0 };
     persistent_trace<true, COUNT, FULL, kTraceSteps>(S, src, *n_ptr, fetch, stk, c, err, refill_min);
@@ -751,7 +733,6 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView S, RenderConsts
             int32_t fl = P.flags[pid];
             int32_t bounce = fl & 0xffff;
             bool specular = (fl & kFlagSpecular) != 0;
-            bool query_next = false;
             spec L = mks(Lv.x, Lv.y, Lv.z);
             float eta_scale = Lv.w;
             spec beta = mks(Bv.x, Bv.y, Bv.z);
@@ -799,9 +780,15 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView S, RenderConsts
                     // further and records every vertex it continues from
                     const int32_t mi = S.slot_shade[slot].w;
                     const bool training = (fl & kFlagTraining) != 0;
+                    // query vertex: the first rough (non-specular) vertex, or
+                    // vertex query_bounce at the latest
+                    const bool query_here =
+                        mi >= 0 && (depth >= C.query_bounce ||
+                                    ((depth >= 1 || C.query_primary) &&
+                                     (S.materials[mi].flags_front & (BX_DIFFUSE | BX_GLOSSY))));
                     int32_t stop = (fl >> kStopShift) & 0xff;
                     if (training) {
-                        if ((fl & kFlagQueryNext) && stop == kStopUnknown) stop = min(depth + C.extra, 254);
+                        if (query_here && stop == kStopUnknown) stop = min(depth + C.extra, 254);
                         fl = (fl & ~(0xff << kStopShift)) | (stop << kStopShift);
                         if (depth >= stop) {
                             stop_here = true;
@@ -816,7 +803,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView S, RenderConsts
                                 C.tcount[tp] = nv + 1;
                             }
                         }
-                    } else if ((fl & kFlagQueryNext) && mi >= 0) {
+                    } else if (query_here) {
                         const int q = atomicAdd(&C.counters[0], 1);
                         cache_features(C, S.materials[mi], sh.point, wo, sh.ns, C.qraw + 16 * size_t(q));
                         C.qbeta[q] = make_float4(beta.r, beta.g, beta.b, __int_as_float(pid));
@@ -879,9 +866,6 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView S, RenderConsts
                         if (ok) {
                             const f3 wi = from_local(fr, s.wi);
                             specular = (s.flags & BX_SPECULAR) != 0;
-                            // query after the first non-specular bounce, at vertex
-                            // query_bounce at the latest
-                            if (CACHE) query_next = !specular || depth + 1 >= C.query_bounce;
                             if (s.flags & BX_TRANSMISSION) eta_scale *= sqr(s.eta);
                             prev_pdf = s.pdf;
                             beta = beta * ((s.f * absdot(sh.ns, wi)) / s.pdf);
@@ -905,8 +889,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView S, RenderConsts
             P.L[pid] = make_float4(L.r, L.g, L.b, eta_scale);
             P.beta[pid] = make_float4(beta.r, beta.g, beta.b, prev_pdf);
             P.rng[pid] = st;
-            P.flags[pid] = (bounce & 0xffff) | (specular ? kFlagSpecular : 0) | (query_next ? kFlagQueryNext : 0) |
-                           (fl & (kFlagTraining | (0xff << kStopShift)));
+            P.flags[pid] = (bounce & 0xffff) | (specular ? kFlagSpecular : 0) | (fl & (kFlagTraining | (0xff << kStopShift)));
         }
         const int ss = wave_alloc(shadow, shadow_count);
         if (shadow) {
@@ -1105,15 +1088,15 @@ bool ensure_gstack(Workspace& w, size_t entries)
 {
     entries = std::max<size_t>(entries, 64);
     if (w.gstack_entries >= entries) {
-        w.P.gstack = reinterpret_cast<int32_t*>(w.gstack);
+        w.P.gstack = reinterpret_cast<int2*>(w.gstack);
         return true;
     }
     if (w.gstack) hipFree(w.gstack);
     w.gstack = nullptr;
     w.gstack_entries = 0;
-    if (hipMalloc(&w.gstack, entries * sizeof(int32_t)) != hipSuccess) return false;
+    if (hipMalloc(&w.gstack, entries * sizeof(int2)) != hipSuccess) return false;
     w.gstack_entries = entries;
-    w.P.gstack = reinterpret_cast<int32_t*>(w.gstack);
+    w.P.gstack = reinterpret_cast<int2*>(w.gstack);
     return true;
 }
 
@@ -1139,7 +1122,7 @@ bool ensure_workspace(Workspace& w, size_t paths, int depth_slots)
     }
     w.capacity = paths;
     w.max_depth_slots = depth_slots;
-    w.P.gstack = reinterpret_cast<int32_t*>(w.gstack);
+    w.P.gstack = reinterpret_cast<int2*>(w.gstack);
     return true;
 }
 
@@ -1249,7 +1232,7 @@ int lds_entries_for(const SceneView& V, int requested)
     return std::max(1, std::min(e, V.stack_depth));
 }
 
-size_t trace_lds_bytes(int lds_entries) { return size_t(lds_entries) * kTraceBlock * sizeof(int32_t); }
+size_t trace_lds_bytes(int lds_entries) { return size_t(lds_entries) * kTraceBlock * sizeof(int2); }
 
 // persistent grid of a traversal kernel: resident blocks per CU x CUs
 int trace_grid(const void* kernel, size_t lds_bytes)
@@ -1267,9 +1250,9 @@ int trace_grid(const void* kernel, size_t lds_bytes)
     return per_cu * cus;
 }
 
-using BatchKernel = void (*)(SceneView, BatchSrc, int, int*, int32_t*, int, int, unsigned long long*, int*);
-using ClosestKernel = void (*)(SceneView, const int*, int*, ClosestSrc, int32_t*, int, int, unsigned long long*, int*);
-using ShadowKernel = void (*)(SceneView, const int*, int*, ShadowSrc, int32_t*, int, int, unsigned long long*, int*);
+using BatchKernel = void (*)(SceneView, BatchSrc, int, int*, int2*, int, int, unsigned long long*, int*);
+using ClosestKernel = void (*)(SceneView, const int*, int*, ClosestSrc, int2*, int, int, unsigned long long*, int*);
+using ShadowKernel = void (*)(SceneView, const int*, int*, ShadowSrc, int2*, int, int, unsigned long long*, int*);
 
 template <bool ANY>
 BatchKernel batch_kernel_any(bool count, bool full)
@@ -1527,6 +1510,7 @@ int render_impl(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams
         C->wmin = lo;
         C->winv = mk3(1.0f / std::max(ext.x, 1e-6f), 1.0f / std::max(ext.y, 1e-6f), 1.0f / std::max(ext.z, 1e-6f));
         C->query_bounce = std::max(1, cache->cp.query_bounce);
+        C->query_primary = cache->cp.query_primary ? 1 : 0;
         C->extra = std::max(0, cache->cp.train_extra_bounces);
         if (cache->stats) memset(cache->stats, 0, sizeof(*cache->stats));
     }

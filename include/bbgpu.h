@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define BBGPU_ABI_VERSION 4
+#define BBGPU_ABI_VERSION 5
 
 /* status codes */
 enum {
@@ -324,8 +324,9 @@ int bbgpu_nrc_train(BbNrc* nrc, const float* inputs_dev, const float* targets_de
                     float* loss_out, float* grad_out, void* stream);
 
 /* The cached path tracer: rendering paths end at their query vertex (the
- * vertex after the first non-specular bounce, or vertex query_bounce at the
- * latest) with throughput x cache(vertex); every train_stride-th path is a
+ * first rough (diffuse or glossy) vertex, from vertex 1 on unless
+ * query_primary, or vertex query_bounce at the latest) with throughput x
+ * cache(vertex); every train_stride-th path is a
  * training path that continues train_extra_bounces vertices further without
  * the cache, and the reflected radiance it measured at each vertex trains
  * the cache after the pass (train_steps Adam steps of train_batch rows). */
@@ -335,6 +336,7 @@ typedef struct BbCacheParams {
     int32_t train_extra_bounces;  /* 4 */
     int32_t train_batch;          /* rows per optimizer step (0: 65536) */
     int32_t train_steps;          /* optimizer steps per pass (0: inference only) */
+    int32_t query_primary;        /* 1: the primary hit may be the query vertex; 0: vertex 1 at the earliest */
 } BbCacheParams;
 
 typedef struct BbCacheStats {
