@@ -49,6 +49,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--wavefront-paths", type=int, default=0)
     ap.add_argument("--ref-spp", type=int, default=4096, help="spp of the relMSE reference (0: skip the quality check)")
+    ap.add_argument("--batch", action="store_true",
+                    help="only the batched workload: 32 seeded scenes of uneven size in one bbgpu_render_batch call")
     ap.add_argument("--cache-frames", type=int, default=256,
                     help="1-spp frames of the equal-time radiance-cache comparison (0: skip it)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -70,6 +72,9 @@ def main():
         sys.exit("no HIP device visible")
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration)
+        return
+    if args.batch:
+        print(json.dumps(batch_workload(args)), flush=True)
         return
 
     log(f"[bench] generating scene level {args.level} at {args.width}x{args.height}")
@@ -313,6 +318,68 @@ def cache_comparisons(args, g_c2, cam_c2, ref_c2):
     if ref_c2 is not None:
         out["icosphere"] = cache_comparison(
             args, g_c2, cam_c2, ref_c2, "displaced icosphere 1920x1080, 1 spp per frame, max_bounces 8", False)
+    return out
+
+
+def batch_workload(args):
+    """32 independent seeded scenes per call: displaced icospheres with
+    triangle counts log-uniform in [1e4, 2e6] (subdivision level 5..8),
+    resolutions from {256x256, 512x512, 1024x576, 1024x1024}, 16 spp,
+    max_bounces uniform in 4..12, 4 to 256 area lights; launch parameters
+    from the autotuner (one probe per resolution); all jobs in one
+    bbgpu_render_batch call on 8 streams, against the same jobs one by one."""
+    import numpy as np
+    import torch
+    from bulbit_amd import scenes
+    from bulbit_amd.gpu import GpuScene, autotune, render_batch
+
+    rng = np.random.Generator(np.random.PCG64(303))
+    res = [(256, 256), (512, 512), (1024, 576), (1024, 1024)]
+    jobs, tuned = [], {}
+    t0 = time.time()
+    for i in range(32):
+        tris = float(np.exp(rng.uniform(np.log(1e4), np.log(2e6))))
+        level = int(np.clip(np.round(np.log(tris / 20.0) / np.log(4.0)), 5, 8))
+        w, h = res[int(rng.integers(0, 4))]
+        n_lights = int(rng.integers(4, 257))
+        sc, cam = scenes.icosphere_scene(width=w, height=h, level=level, seed=1000 + i, n_lights=n_lights)
+        g = GpuScene(sc)
+        p = scenes.render_params(16, max_bounces=int(rng.integers(4, 13)), seed=i)
+        if (w, h) not in tuned:
+            tuned[(w, h)] = autotune(g, cam, p)
+        t = tuned[(w, h)]
+        p.trace_refill, p.trace_lds_stack, p.wavefront_paths = t.trace_refill, t.trace_lds_stack, t.wavefront_paths
+        jobs.append((g, cam, p, g.new_film(cam)))
+        log(f"[bench] job {i}: level {level} ({len(sc.prims)} prims), {w}x{h}, max_bounces {p.max_bounces}, "
+            f"{n_lights} lights")
+    log(f"[bench] 32 scenes built and tuned in {time.time() - t0:.1f}s")
+    render_batch(jobs, streams=8)  # warm-up (workspaces)
+    for j in jobs:
+        j[3][0].zero_()
+        j[3][1].zero_()
+    torch.cuda.synchronize()
+    st = render_batch(jobs, streams=8)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for g, cam, p, film in jobs:
+        g.render_device(cam, p, film[0].data_ptr(), film[1].data_ptr())
+    torch.cuda.synchronize()
+    serial_ms = (time.perf_counter() - t1) * 1e3
+    jm = np.array(st.job_ms_list)
+    out = {
+        "metric": "batched Msamples/s (32 scenes per call)",
+        "value": round(st.samples / st.ms_total / 1e3, 3),
+        "unit": "Msamples/s",
+        "samples": int(st.samples),
+        "ms_total": round(st.ms_total, 2),
+        "streams": int(st.streams),
+        "job_ms_p50": round(float(np.percentile(jm, 50)), 2),
+        "job_ms_p99": round(float(np.percentile(jm, 99)), 2),
+        "serial_ms": round(serial_ms, 2),
+        "serial_msamples_per_s": round(st.samples / serial_ms / 1e3, 3),
+        "tuned": {f"{w}x{h}": [t.trace_refill, t.trace_lds_stack, t.wavefront_paths] for (w, h), t in tuned.items()},
+    }
+    log(f"[bench] batch: {out['value']} Msamples/s in {st.ms_total:.1f} ms; one by one {serial_ms:.1f} ms")
     return out
 
 

@@ -12,7 +12,7 @@ from ctypes import POINTER, c_char_p, c_double, c_float, c_int32, c_size_t, c_ui
 
 import numpy as np
 
-ABI_VERSION = 5  # BBGPU_ABI_VERSION of include/bbgpu.h that these structures mirror
+ABI_VERSION = 6  # BBGPU_ABI_VERSION of include/bbgpu.h that these structures mirror
 
 LIB_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(LIB_DIR, "libbbgpu.so")
@@ -181,6 +181,20 @@ class BbRenderStats(ctypes.Structure):
     ]
 
 
+class BbRenderJob(ctypes.Structure):
+    _fields_ = [
+        ("scene", c_void_p),
+        ("camera", BbCameraDesc),
+        ("params", BbRenderParams),
+        ("film_sum_dev", c_void_p),
+        ("film_count_dev", c_void_p),
+    ]
+
+
+class BbBatchStats(ctypes.Structure):
+    _fields_ = [("ms_total", c_double), ("samples", c_int64), ("streams", c_int32), ("job_ms", POINTER(c_double))]
+
+
 class BbCacheParams(ctypes.Structure):
     _fields_ = [
         ("query_bounce", c_int32),
@@ -226,6 +240,8 @@ EXPORTS = [
     "bbgpu_nrc_query",
     "bbgpu_nrc_train",
     "bbgpu_render_cached",
+    "bbgpu_render_batch",
+    "bbgpu_autotune",
 ]
 
 
@@ -272,6 +288,9 @@ def load_library(path: str | None = None):
         c_int32,
     ]
     lib.bbgpu_look_at.argtypes = [POINTER(c_float), POINTER(c_float), POINTER(c_float), POINTER(c_float)]
+    lib.bbgpu_render_batch.argtypes = [POINTER(BbRenderJob), c_int32, c_int32, POINTER(BbBatchStats)]
+    lib.bbgpu_autotune.argtypes = [c_void_p, POINTER(BbCameraDesc), POINTER(BbRenderParams), c_int32,
+                                   POINTER(BbRenderParams)]
     lib.bbgpu_render_cached.argtypes = [c_void_p, c_void_p, POINTER(BbCameraDesc), POINTER(BbRenderParams),
                                         POINTER(BbCacheParams), c_void_p, c_void_p, c_void_p, POINTER(BbCacheStats)]
     for name in EXPORTS:

@@ -22,6 +22,7 @@
 #include "bb_shade.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -1395,8 +1396,11 @@ bool ensure_cache(SceneExtra* x, size_t cap, int stride, int max_vertices)
     return true;
 }
 
+// wait = false (batched jobs): return once every launch is queued; the
+// caller synchronizes the stream and checks the error flag (check_render)
 int render_impl(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams* params, float* film_sum_dev,
-                int32_t* film_count_dev, void* stream, BbRenderStats* stats, int32_t count_traversal, CacheCtx* cache)
+                int32_t* film_count_dev, void* stream, BbRenderStats* stats, int32_t count_traversal, CacheCtx* cache,
+                bool wait = true)
 {
     if (!scene || !camera || !params || !film_sum_dev || !film_count_dev) {
         bb::set_error("bbgpu_render: null argument");
@@ -1636,6 +1640,7 @@ int render_impl(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams
         HIP_OK(hipMemcpyAsync(film_sum_user, film_sum_dev, sum_bytes, hipMemcpyDeviceToHost, st));
         HIP_OK(hipMemcpyAsync(film_count_user, film_count_dev, count_bytes, hipMemcpyDeviceToHost, st));
     }
+    if (!wait && !timed && !host_film && !cache) return BBGPU_OK;
     int err = 0;
     unsigned long long c[8] = { 0 };
     HIP_OK(hipMemcpyAsync(&err, P.err, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1684,9 +1689,177 @@ int render_impl(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams
     return BBGPU_OK;
 }
 
+// the error flag of a render queued with wait = false (after its stream is synchronized)
+int check_render(BbScene* scene)
+{
+    SceneExtra* x = extra_for(scene);
+    int err = 0;
+    HIP_OK(hipMemcpy(&err, x->ws.P.err, sizeof(int), hipMemcpyDeviceToHost));
+    if (err) {
+        bb::set_error("bbgpu_render_batch: traversal stack overflow");
+        return BBGPU_ERR_DEVICE;
+    }
+    return BBGPU_OK;
+}
+
+std::vector<hipStream_t>& stream_pool()
+{
+    static std::vector<hipStream_t> pool;
+    return pool;
+}
+
+double job_cost(const BbRenderJob& j)
+{
+    return double(j.camera.width) * j.camera.height * std::max(0, j.params.spp) * (j.params.max_bounces + 1);
+}
+
 } // namespace
 
 extern "C" {
+
+int bbgpu_render_batch(BbRenderJob* jobs, int32_t n_jobs, int32_t streams, BbBatchStats* stats)
+{
+    if (n_jobs < 0 || (n_jobs > 0 && !jobs)) {
+        bb::set_error("bbgpu_render_batch: invalid argument");
+        return BBGPU_ERR_INVALID;
+    }
+    for (int32_t i = 0; i < n_jobs; ++i) {
+        const BbRenderJob& j = jobs[i];
+        if (!j.scene || !j.film_sum_dev || !j.film_count_dev) {
+            bb::set_error("bbgpu_render_batch: job %d has a null scene or film", i);
+            return BBGPU_ERR_INVALID;
+        }
+        if (!device_accessible(j.film_sum_dev) || !device_accessible(j.film_count_dev)) {
+            bb::set_error("bbgpu_render_batch: job %d: films must be device memory", i);
+            return BBGPU_ERR_INVALID;
+        }
+    }
+    const int ns = std::max(1, std::min(streams > 0 ? streams : 8, 16));
+    std::vector<hipStream_t>& pool = stream_pool();
+    while (int(pool.size()) < ns) {
+        hipStream_t st = nullptr;
+        HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking)); // default priority
+        pool.push_back(st);
+    }
+    // jobs of one scene share its workspace and run in order on one stream;
+    // scenes go to the least loaded stream, most expensive first
+    std::vector<BbScene*> scenes;
+    std::vector<double> scene_cost;
+    for (int32_t i = 0; i < n_jobs; ++i) {
+        auto it = std::find(scenes.begin(), scenes.end(), jobs[i].scene);
+        if (it == scenes.end()) {
+            scenes.push_back(jobs[i].scene);
+            scene_cost.push_back(job_cost(jobs[i]));
+        } else {
+            scene_cost[size_t(it - scenes.begin())] += job_cost(jobs[i]);
+        }
+    }
+    std::vector<size_t> order(scenes.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return scene_cost[a] > scene_cost[b]; });
+    std::vector<double> load(size_t(ns), 0.0);
+    std::vector<int> scene_stream(scenes.size(), 0);
+    for (size_t oi : order) {
+        const int k = int(std::min_element(load.begin(), load.end()) - load.begin());
+        scene_stream[oi] = k;
+        load[size_t(k)] += scene_cost[oi];
+    }
+    HIP_OK(hipDeviceSynchronize());
+    EventPool evp;
+    std::vector<hipEvent_t> start(static_cast<size_t>(ns)), done(static_cast<size_t>(n_jobs), nullptr);
+    std::vector<int> job_stream(static_cast<size_t>(n_jobs), 0);
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int k = 0; k < ns; ++k) {
+        start[size_t(k)] = evp.get();
+        HIP_OK(hipEventRecord(start[size_t(k)], pool[size_t(k)]));
+    }
+    int64_t samples = 0;
+    for (size_t oi : order)
+        for (int32_t i = 0; i < n_jobs; ++i) {
+            if (jobs[i].scene != scenes[oi]) continue;
+            hipStream_t st = pool[size_t(scene_stream[oi])];
+            const int rc = render_impl(jobs[i].scene, &jobs[i].camera, &jobs[i].params, jobs[i].film_sum_dev,
+                                       jobs[i].film_count_dev, st, nullptr, 0, nullptr, false);
+            if (rc != BBGPU_OK) return rc;
+            done[size_t(i)] = evp.get();
+            job_stream[size_t(i)] = scene_stream[oi];
+            HIP_OK(hipEventRecord(done[size_t(i)], st));
+            samples += int64_t(jobs[i].camera.width) * jobs[i].camera.height * jobs[i].params.spp;
+        }
+    for (int k = 0; k < ns; ++k) HIP_OK(hipStreamSynchronize(pool[size_t(k)]));
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (stats && stats->job_ms)
+        for (int32_t i = 0; i < n_jobs; ++i) {
+            float e = 0.0f;
+            HIP_OK(hipEventElapsedTime(&e, start[size_t(job_stream[size_t(i)])], done[size_t(i)]));
+            stats->job_ms[i] = e;
+        }
+    for (BbScene* sc : scenes) {
+        const int rc = check_render(sc);
+        if (rc != BBGPU_OK) return rc;
+    }
+    if (stats) {
+        stats->ms_total = ms;
+        stats->samples = samples;
+        stats->streams = ns;
+    }
+    return BBGPU_OK;
+}
+
+int bbgpu_autotune(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams* params, int32_t probe_spp,
+                   BbRenderParams* tuned_out)
+{
+    if (!scene || !camera || !params || !tuned_out) {
+        bb::set_error("bbgpu_autotune: null argument");
+        return BBGPU_ERR_INVALID;
+    }
+    // coordinate descent over the knobs that change speed, not results:
+    // refill threshold, LDS stack entries, paths in flight
+    BbRenderParams best = *params;
+    best.spp = std::max(1, std::min(probe_spp > 0 ? probe_spp : 2, std::max(params->spp, 1)));
+    const long long npix = (long long)camera->width * camera->height;
+    float *sum = nullptr;
+    int32_t* cnt = nullptr;
+    HIP_OK(hipMalloc(&sum, sizeof(float) * 3 * size_t(npix)));
+    if (hipMalloc(&cnt, sizeof(int32_t) * size_t(npix)) != hipSuccess) {
+        (void)hipFree(sum);
+        bb::set_error("bbgpu_autotune: film allocation failed");
+        return BBGPU_ERR_NOMEM;
+    }
+    auto time_of = [&](const BbRenderParams& q, double& ms) {
+        BbRenderStats st;
+        const int rc = render_impl(scene, camera, &q, sum, cnt, nullptr, &st, 0, nullptr);
+        ms = st.ms_total;
+        return rc;
+    };
+    double best_ms = 0.0;
+    int rc = time_of(best, best_ms); // warm-up, then the baseline
+    if (rc == BBGPU_OK) rc = time_of(best, best_ms);
+    const int refills[] = { 32, 48, 64 };
+    const int lds[] = { 16, 24, 32 };
+    const long long paths[] = { 1ll << 20, 1ll << 21, 1ll << 22 };
+    for (int knob = 0; knob < 3 && rc == BBGPU_OK; ++knob)
+        for (int v = 0; v < 3 && rc == BBGPU_OK; ++v) {
+            BbRenderParams q = best;
+            if (knob == 0) q.trace_refill = refills[v];
+            if (knob == 1) q.trace_lds_stack = lds[v];
+            if (knob == 2) q.wavefront_paths = int32_t(std::min<long long>(paths[v], 1ll << 30));
+            double ms = 0.0;
+            rc = time_of(q, ms);
+            if (rc == BBGPU_OK && ms < best_ms) {
+                best_ms = ms;
+                best = q;
+            }
+        }
+    (void)hipFree(sum);
+    (void)hipFree(cnt);
+    if (rc != BBGPU_OK) return rc;
+    *tuned_out = *params;
+    tuned_out->trace_refill = best.trace_refill;
+    tuned_out->trace_lds_stack = best.trace_lds_stack;
+    tuned_out->wavefront_paths = best.wavefront_paths;
+    return BBGPU_OK;
+}
 
 int bbgpu_render(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams* params, float* film_sum_dev,
                  int32_t* film_count_dev, void* stream, BbRenderStats* stats, int32_t count_traversal)

@@ -112,3 +112,30 @@ def film_image(film_sum: np.ndarray, film_count: np.ndarray) -> np.ndarray:
     """Film::GetRenderedImage (src/reconstruction/film.cpp:80-100): sum / count."""
     with np.errstate(divide="ignore", invalid="ignore"):
         return (film_sum / film_count[..., None].astype(np.float32)).astype(np.float32)
+
+
+def render_batch(jobs, streams=0):
+    """bbgpu_render_batch over [(GpuScene, camera, params, (sum_dev, count_dev)), ...]
+    (films from GpuScene.new_film); returns BbBatchStats."""
+    lib = capi.load_library()
+    arr = (capi.BbRenderJob * max(1, len(jobs)))()
+    for i, (g, cam, params, film) in enumerate(jobs):
+        arr[i].scene = g.h.value
+        arr[i].camera = cam
+        arr[i].params = params
+        arr[i].film_sum_dev = film[0].data_ptr()
+        arr[i].film_count_dev = film[1].data_ptr()
+    st = capi.BbBatchStats()
+    job_ms = (ctypes.c_double * max(1, len(jobs)))()
+    st.job_ms = job_ms
+    capi.check(lib.bbgpu_render_batch(arr, len(jobs), streams, ctypes.byref(st)), "bbgpu_render_batch", lib)
+    st.job_ms_list = list(job_ms[:len(jobs)])
+    return st
+
+
+def autotune(g, camera, params, probe_spp=2):
+    """bbgpu_autotune: params with the fastest refill / LDS stack / wavefront size"""
+    out = capi.BbRenderParams()
+    capi.check(g.lib.bbgpu_autotune(g.h, ctypes.byref(camera), ctypes.byref(params), probe_spp, ctypes.byref(out)),
+               "bbgpu_autotune", g.lib)
+    return out

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define BBGPU_ABI_VERSION 5
+#define BBGPU_ABI_VERSION 6
 
 /* status codes */
 enum {
@@ -289,6 +289,39 @@ int bbgpu_trace(BbScene* scene, const BbRay* rays_dev, int32_t n_rays, int32_t a
  * call returns when the render is complete. */
 int bbgpu_render(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams* params, float* film_sum_dev,
                  int32_t* film_count_dev, void* stream, BbRenderStats* stats, int32_t count_traversal);
+
+/* ---------------------------------------------------------------- batch -- */
+
+/* One render of a batch: bbgpu_render's arguments (device films only). */
+typedef struct BbRenderJob {
+    BbScene* scene;
+    BbCameraDesc camera;
+    BbRenderParams params;
+    float* film_sum_dev;
+    int32_t* film_count_dev;
+} BbRenderJob;
+
+typedef struct BbBatchStats {
+    double ms_total;  /* host wall time from the first launch to the last stream's end */
+    int64_t samples;  /* camera samples of all jobs */
+    int32_t streams;
+    double* job_ms;   /* in: NULL or room for n_jobs values; out: each job's completion time
+                         from the start of its stream (HIP events) */
+} BbBatchStats;
+
+/* Renders independent jobs (no counterpart in Bulbit: many scenes or camera
+ * jobs of uneven size): each scene's jobs run in order on one of `streams`
+ * HIP streams (0: 8, at most 16; default priority), scenes are assigned
+ * largest first to the least loaded stream, and small jobs overlap on the
+ * GPU.  Every job's film equals what bbgpu_render would accumulate.  Returns
+ * when all jobs are done; stats may be NULL. */
+int bbgpu_render_batch(BbRenderJob* jobs, int32_t n_jobs, int32_t streams, BbBatchStats* stats);
+
+/* Picks trace_refill, trace_lds_stack and wavefront_paths for a job by timing
+ * short renders of probe_spp samples (coordinate descent over 3 values each);
+ * the knobs change speed, never results.  tuned_out = *params with them set. */
+int bbgpu_autotune(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams* params, int32_t probe_spp,
+                   BbRenderParams* tuned_out);
 
 /* ------------------------------------------------------- radiance cache -- */
 

@@ -1,0 +1,62 @@
+"""Batched rendering (bbgpu_render_batch) and the autotuner (bbgpu_autotune).
+
+Jobs of several scenes and cameras, some sharing a scene, run concurrently on
+a stream pool; every job's film must equal, bit for bit, the film of the same
+render alone.  The autotuner only changes speed knobs: images are unchanged.
+"""
+import copy
+
+import numpy as np
+import pytest
+
+from bulbit_amd import capi, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _jobs():
+    from bulbit_amd.gpu import GpuScene
+
+    out = []
+    sc1, cam1 = scenes.icosphere_scene(width=256, height=256, level=5, n_lights=8)
+    sc2, cam2 = scenes.icosphere_scene(width=320, height=180, level=6, n_lights=32, seed=7)
+    sc3, cam3 = scenes.readme_scene(width=200, height=120)
+    g1, g2, g3 = GpuScene(sc1), GpuScene(sc2), GpuScene(sc3)
+    cam1b = copy.copy(cam1)
+    scenes.set_filter(cam1b, "gaussian")
+    specs = [(g1, cam1, scenes.render_params(4, max_bounces=4)), (g1, cam1b, scenes.render_params(2, max_bounces=12)),
+             (g2, cam2, scenes.render_params(3, max_bounces=8, seed=5)), (g3, cam3, scenes.render_params(8)),
+             (g2, cam2, scenes.render_params(2, max_bounces=6, sample_begin=3)), (g3, cam3, scenes.render_params(1))]
+    for g, cam, p in specs:
+        out.append((g, cam, p, g.new_film(cam)))
+    return out, (g1, g2, g3)
+
+
+def test_batch_equals_individual_renders():
+    from bulbit_amd.gpu import render_batch
+
+    jobs, keep = _jobs()
+    st = render_batch(jobs, streams=4)
+    assert st.streams == 4 and st.samples == sum(c.width * c.height * p.spp for _, c, p, _ in jobs)
+    for i, (g, cam, p, film) in enumerate(jobs):
+        s, c, _ = g.render(cam, p)
+        bs = film[0].cpu().numpy().reshape(s.shape)
+        bc = film[1].cpu().numpy().reshape(c.shape)
+        assert np.array_equal(bc, c), i
+        assert np.array_equal(bs.view(np.int32), s.view(np.int32)), i
+    print(f"[batch] {len(jobs)} jobs, {st.samples} samples in {st.ms_total:.1f} ms", flush=True)
+
+
+def test_autotune_keeps_the_image():
+    from bulbit_amd.gpu import GpuScene, autotune
+
+    sc, cam = scenes.icosphere_scene(width=320, height=180, level=6, n_lights=16)
+    g = GpuScene(sc)
+    p = scenes.render_params(4)
+    t = autotune(g, cam, p)
+    assert t.spp == p.spp and t.max_bounces == p.max_bounces and t.seed == p.seed
+    assert t.trace_refill in (0, 32, 48, 64) and t.trace_lds_stack in (0, 16, 24, 32)
+    s0, c0, _ = g.render(cam, p)
+    s1, c1, _ = g.render(cam, t)
+    assert np.array_equal(c0, c1) and np.array_equal(s0.view(np.int32), s1.view(np.int32))
+    print(f"[batch] tuned refill {t.trace_refill}, lds {t.trace_lds_stack}, paths {t.wavefront_paths}", flush=True)
