@@ -228,11 +228,14 @@ BF16_PEAK_TFLOPS = 2500.0
 
 
 def cache_comparison(args, g, cam, ref, name, query_primary):
-    """Radiance cache vs plain path tracing at equal wall-clock time: one
-    sample per frame, accumulated over frames, relMSE against `ref` (a plain
-    render with disjoint samples).  Plain PT renders `cache_frames` frames;
-    the cached tracer (trained online on 1/16 of its paths) renders frames
-    until it has used the same time."""
+    """Radiance cache vs plain path tracing at equal wall-clock time, one
+    sample per frame, against `ref` (a plain render with disjoint samples).
+    Plain PT renders `cache_frames` frames; the cached tracer (trained online
+    on 1/8 of its paths) renders frames until it has used the same time.
+    Two measures: the relMSE of the frames accumulated over the whole time
+    (progressive rendering), and the mean relMSE of single frames from the
+    second half of the run (real-time rendering), with plain PT's per-frame
+    error scaled to the cached tracer's frame time (variance ~ 1 / samples)."""
     import numpy as np
     import torch
     from bulbit_amd import scenes
@@ -242,28 +245,41 @@ def cache_comparison(args, g, cam, ref, name, query_primary):
     base = 1 << 16  # frames use samples base + f, disjoint from the references
 
     def run(step, frames, budget=None):
-        film = g.new_film(cam)
+        acc = g.new_film(cam)
+        frame = g.new_film(cam)
+        keep = []
         torch.cuda.synchronize()
         t = time.perf_counter()
         done = 0
         for f in range(frames):
-            step(f, film)
+            frame[0].zero_()
+            frame[1].zero_()
+            step(f, frame)
+            acc[0].add_(frame[0])
+            acc[1].add_(frame[1])
+            if f % 16 == 15:
+                keep.append((f, frame[0].clone(), frame[1].clone()))
             torch.cuda.synchronize()
             done = f + 1
             if budget is not None and time.perf_counter() - t >= budget:
                 break
         secs = time.perf_counter() - t
-        s, c = film[0].cpu().numpy().reshape(h, w, 3), film[1].cpu().numpy().reshape(h, w)
-        return relmse(s / c[..., None], ref), done, secs
 
-    cp = scenes.cache_params(train_batch=65536, train_steps=2, query_primary=query_primary)
+        def err(sd, cd):
+            sn, cn = sd.cpu().numpy().reshape(h, w, 3), cd.cpu().numpy().reshape(h, w)
+            return relmse(sn / np.maximum(cn, 1)[..., None], ref)
+
+        per_frame = [err(sd, cd) for f, sd, cd in keep if f >= done // 2]
+        return err(acc[0], acc[1]), done, secs, float(np.mean(per_frame)) if per_frame else float("nan")
+
+    cp = scenes.cache_params(train_stride=8, train_batch=65536, train_steps=4, query_primary=query_primary)
     # warm both paths (kernel loading) outside the timed runs
     g.render(cam, scenes.render_params(1, sample_begin=1 << 24))
     net = Nrc(seed=1, learning_rate=1e-2)
     g.render_cached(net, cam, scenes.render_params(1, sample_begin=1 << 24), cp)
     net.close()
 
-    e_plain, n_plain, t_plain = run(
+    e_plain, n_plain, t_plain, f_plain = run(
         lambda f, film: g.render_device(cam, scenes.render_params(1, sample_begin=base + f), film[0].data_ptr(),
                                         film[1].data_ptr()), args.cache_frames)
     net = Nrc(seed=1, learning_rate=1e-2)
@@ -273,15 +289,19 @@ def cache_comparison(args, g, cam, ref, name, query_primary):
         _, _, st = g.render_cached(net, cam, scenes.render_params(1, sample_begin=base + f), cp, film=film)
         stats.append(st)
 
-    e_cache, n_cache, t_cache = run(cached, 100000, budget=t_plain)
+    e_cache, n_cache, t_cache, f_cache = run(cached, 100000, budget=t_plain)
     net.close()
     q = sum(st.queries for st in stats)
-    rows = sum(min(st.train_rows, 2 * 65536) for st in stats)
+    rows = sum(min(st.train_rows, 4 * 65536) for st in stats)
     ms_q = sum(st.ms_query for st in stats)
     ms_t = sum(st.ms_train for st in stats)
     tflops = (MLP_FLOPS_QUERY * q + 3 * MLP_FLOPS_QUERY * rows) / max(ms_q + ms_t, 1e-9) / 1e9
+    frame_plain = t_plain / n_plain
+    frame_cache = t_cache / n_cache
+    f_plain_equal_time = f_plain * frame_plain / frame_cache
     log(f"[bench] {name}: equal time {t_plain:.2f}s: plain {n_plain} frames relMSE {e_plain:.6f}; cached {n_cache} "
-        f"frames relMSE {e_cache:.6f} ({q} queries, {rows} training rows, query {ms_q:.0f} ms, train {ms_t:.0f} ms)")
+        f"frames relMSE {e_cache:.6f}; per frame: plain {f_plain:.5f} ({frame_plain * 1e3:.2f} ms), cached "
+        f"{f_cache:.5f} ({frame_cache * 1e3:.2f} ms); {q} queries, {rows} training rows")
     return {
         "workload": name,
         "query_at_primary_hit": bool(query_primary),
@@ -291,6 +311,9 @@ def cache_comparison(args, g, cam, ref, name, query_primary):
         "cache_frames": n_cache,
         "cache_relmse": round(e_cache, 7),
         "ratio": round(e_cache / e_plain, 4),
+        "per_frame": {"plain_relmse": round(f_plain, 6), "plain_ms": round(frame_plain * 1e3, 3),
+                      "cache_relmse": round(f_cache, 6), "cache_ms": round(frame_cache * 1e3, 3),
+                      "ratio_equal_time": round(f_cache / f_plain_equal_time, 4)},
         "cache_ms_query": round(ms_q, 1),
         "cache_ms_train": round(ms_t, 1),
         "mlp": {"bound": "mfma", "achieved": round(tflops, 3), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",

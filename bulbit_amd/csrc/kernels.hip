@@ -37,6 +37,7 @@ namespace bb
 {
 
 constexpr int kTraceBlock = 128;
+constexpr int kShadowRefillDefault = 48;
 constexpr int kShadeBlock = 256;
 constexpr int kMaxGridBlocks = 256 * 32;
 
@@ -419,6 +420,10 @@ BB_D TraceStack make_stack(int2* lds_stack, int2* gstack, int lds_entries, int c
 #define BB_TRACE_STEPS 2
 #endif
 constexpr int kTraceSteps = BB_TRACE_STEPS;
+#ifndef BB_SHADOW_STEPS
+#define BB_SHADOW_STEPS 2
+#endif
+constexpr int kShadowSteps = BB_SHADOW_STEPS;
 
 template <bool ANY, bool COUNT, bool FULL>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_batch(SceneView S, BatchSrc src, int n, int* fetch,
@@ -635,7 +640,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_shadow(SceneView S, const
     extern __shared__ int2 lds_stack[];
     const TraceStack stk = make_stack(lds_stack, gstack, lds_entries, S.stack_depth);
     TraceCount c{ 0, 0, 0 };
-    persistent_trace<true, COUNT, FULL, kTraceSteps>(S, src, *n_ptr, fetch, stk, c, err, refill_min);
+    persistent_trace<true, COUNT, FULL, kShadowSteps>(S, src, *n_ptr, fetch, stk, c, err, refill_min);
     if (COUNT) wave_count_add(c, counters + 3);
 }
 
@@ -1456,6 +1461,7 @@ int render_impl(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams
     const int lds_entries = lds_entries_for(V, p.trace_lds_stack);
     const size_t lds = trace_lds_bytes(lds_entries);
     const int refill_min = p.trace_refill > 0 ? std::min(p.trace_refill, 64) : 48;
+    const int refill_shadow = p.shadow_refill > 0 ? std::min(p.shadow_refill, 64) : kShadowRefillDefault;
     const bool full = V.stack_depth <= lds_entries;
     const ClosestKernel k_closest = closest_kernel(count, full);
     const ShadowKernel k_shadow = shadow_kernel(count, full);
@@ -1577,7 +1583,7 @@ int render_impl(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams
             if (timed) spans.push_back({ b, c2, 2 });
             const ShadowSrc ssrc{ P.sray, P.scontrib, P.L };
             hipLaunchKernelGGL(k_shadow, dim3(g_trace), dim3(kTraceBlock), lds, st, V, P.counts + 4 * depth + 1,
-                               P.counts + 4 * depth + 3, ssrc, P.gstack, lds_entries, refill_min, P.counters, P.err);
+                               P.counts + 4 * depth + 3, ssrc, P.gstack, lds_entries, refill_shadow, P.counters, P.err);
             if (timed) spans.push_back({ c2, mark(), 1 });
         }
         if (cache) {

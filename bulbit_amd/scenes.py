@@ -447,7 +447,8 @@ def icosphere_scene(width=1920, height=1080, level=8, seed=2026, n_lights=64):
     return sc, cam
 
 
-def render_params(spp, max_bounces=8, seed=0, rr_min_bounces=1, sample_begin=0, wavefront_paths=0, trace_lds_stack=0, trace_refill=0):
+def render_params(spp, max_bounces=8, seed=0, rr_min_bounces=1, sample_begin=0, wavefront_paths=0, trace_lds_stack=0,
+                  trace_refill=0, shadow_refill=0):
     p = capi.BbRenderParams()
     p.spp = spp
     p.seed = seed
@@ -457,6 +458,7 @@ def render_params(spp, max_bounces=8, seed=0, rr_min_bounces=1, sample_begin=0, 
     p.wavefront_paths = wavefront_paths
     p.trace_lds_stack = trace_lds_stack
     p.trace_refill = trace_refill
+    p.shadow_refill = shadow_refill
     p.sample_begin = sample_begin
     return p
 

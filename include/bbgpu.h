@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define BBGPU_ABI_VERSION 6
+#define BBGPU_ABI_VERSION 7
 
 /* status codes */
 enum {
@@ -221,7 +221,8 @@ typedef struct BbRenderParams {
     int32_t wavefront_paths;  /* paths in flight per pass (0 = auto) */
     int32_t trace_lds_stack;  /* traversal stack entries per lane kept in LDS (0 = auto) */
     int32_t sample_begin;     /* render samples [sample_begin, sample_begin+spp) */
-    int32_t trace_refill;     /* a traversal wave takes new rays once this many of its lanes are idle (0 = 48) */
+    int32_t trace_refill;     /* a closest-hit traversal wave takes new rays once this many of its lanes are idle (0 = 48) */
+    int32_t shadow_refill;    /* the same for shadow rays (0 = trace_refill's default) */
 } BbRenderParams;
 
 /* Per-render statistics, filled when a pointer is passed to bbgpu_render.
