@@ -12,7 +12,7 @@ from ctypes import POINTER, c_char_p, c_double, c_float, c_int32, c_size_t, c_ui
 
 import numpy as np
 
-ABI_VERSION = 7  # BBGPU_ABI_VERSION of include/bbgpu.h that these structures mirror
+ABI_VERSION = 8  # BBGPU_ABI_VERSION of include/bbgpu.h that these structures mirror
 
 LIB_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(LIB_DIR, "libbbgpu.so")
@@ -227,6 +227,7 @@ EXPORTS = [
     "bbgpu_last_error",
     "bbgpu_device_count",
     "bbgpu_build_bvh",
+    "bbgpu_reflectance_table",
     "bbgpu_scene_create",
     "bbgpu_scene_destroy",
     "bbgpu_scene_info",
@@ -274,6 +275,7 @@ def load_library(path: str | None = None):
     lib.bbgpu_last_error.argtypes = [c_char_p, c_size_t]
     lib.bbgpu_device_count.argtypes = [POINTER(c_int32)]
     lib.bbgpu_build_bvh.argtypes = [POINTER(BbSceneDesc), c_void_p, c_void_p, POINTER(c_int32)]
+    lib.bbgpu_reflectance_table.argtypes = [c_int32, c_void_p, c_int32, POINTER(c_int32)]
     lib.bbgpu_scene_create.argtypes = [POINTER(BbSceneDesc), POINTER(c_void_p)]
     lib.bbgpu_scene_destroy.argtypes = [c_void_p]
     lib.bbgpu_scene_info.argtypes = [c_void_p, POINTER(BbSceneInfo)]
@@ -301,6 +303,23 @@ def load_library(path: str | None = None):
     if path is None:
         _lib = lib
     return lib
+
+
+TABLE_GGX_E, TABLE_GGX_E_AVG = 0, 1
+TABLE_DIELECTRIC_E, TABLE_DIELECTRIC_E_INV, TABLE_DIELECTRIC_E_AVG, TABLE_DIELECTRIC_E_INV_AVG = 2, 3, 4, 5
+
+
+def reflectance_table(table: int, lib=None):
+    """bbgpu_reflectance_table as a float32 numpy array (host computation)."""
+    import numpy as np
+
+    lib = lib or load_library()
+    n = c_int32(0)
+    check(lib.bbgpu_reflectance_table(int(table), None, 0, ctypes.byref(n)), "bbgpu_reflectance_table", lib)
+    out = np.zeros(n.value, np.float32)
+    check(lib.bbgpu_reflectance_table(int(table), out.ctypes.data, n.value, ctypes.byref(n)),
+          "bbgpu_reflectance_table", lib)
+    return out
 
 
 def last_error(lib=None) -> str:

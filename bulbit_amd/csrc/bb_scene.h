@@ -56,7 +56,8 @@ enum : int32_t
     MK_DIELECTRIC = 1,
     MK_CONDUCTOR = 2,
     MK_METALLIC_ROUGHNESS = 3,
-    MK_CONDUCTOR_MS = 4 // ConductorMultiScatteringBxDF (energy_compensation)
+    MK_CONDUCTOR_MS = 4, // ConductorMultiScatteringBxDF (energy_compensation)
+    MK_DIELECTRIC_MS = 5 // DielectricMultiScatteringBxDF (energy_compensation, rough, eta != 1)
 };
 
 // Reflectance tables of PrepareReflectanceTextures (microfacet.cpp:239-270):
@@ -72,6 +73,21 @@ struct ReflectanceTables
 };
 
 const ReflectanceTables& reflectance_tables();
+
+// DielectricMultiScatteringBxDF's tables (microfacet.cpp:173-236): E and E_inv
+// at (f0 = x, cos_theta = y, alpha = z), [(z * 32 + y) * 32 + x]; E_avg and
+// E_inv_avg at (f0 = x, alpha = y), [y * 32 + x].  On the device they are one
+// array in this order (kDielectricTableFloats floats).
+struct DielectricTables
+{
+    float E[kTableSize * kTableSize * kTableSize];
+    float E_inv[kTableSize * kTableSize * kTableSize];
+    float E_avg[kTableSize * kTableSize];
+    float E_inv_avg[kTableSize * kTableSize];
+};
+constexpr int32_t kDielectricTableFloats = 2 * kTableSize * kTableSize * kTableSize + 2 * kTableSize * kTableSize;
+
+const DielectricTables& dielectric_tables();
 
 // Material with its BxDF parameters precomputed on the host
 // (the Material::GetBSDF arithmetic that does not depend on the hit).
@@ -121,6 +137,7 @@ struct SceneView
     const int32_t* inf_lights;
     const float* ggx_E;     // ReflectanceTables::ggx_E
     const float* ggx_E_avg; // ReflectanceTables::ggx_E_avg
+    const float* dielectric; // DielectricTables (kDielectricTableFloats), or null when unused
     int32_t n_lights;
     int32_t n_inf;
     float func_integral;

@@ -209,6 +209,7 @@ struct bxdf
     float metallic;
     const float* E;     // GGX albedo table (compensated conductor)
     const float* E_avg; // its average over directions
+    const float* dt;    // DielectricTables (compensated dielectric)
 };
 
 struct bsample
@@ -220,14 +221,16 @@ struct bsample
     float eta;
 };
 
-BB_D bxdf make_bxdf(const GpuMaterial& m, bool front_face, const float* E, const float* E_avg)
+BB_D bxdf make_bxdf(const GpuMaterial& m, bool front_face, const SceneView& S)
 {
     bxdf b;
-    b.E = E;
-    b.E_avg = E_avg;
+    b.E = S.ggx_E;
+    b.E_avg = S.ggx_E_avg;
+    b.dt = S.dielectric;
     b.kind = m.kind;
     b.flags = m.flags_front;
-    b.eta = (m.kind == MK_DIELECTRIC) ? (front_face ? m.eta : 1.0f / m.eta) : 1.0f;
+    const bool dielectric = m.kind == MK_DIELECTRIC || m.kind == MK_DIELECTRIC_MS;
+    b.eta = dielectric ? (front_face ? m.eta : 1.0f / m.eta) : 1.0f;
     b.mf = ggx{ m.ax, m.ay };
     b.r = m.r;
     b.ceta = m.ceta;
@@ -411,12 +414,271 @@ BB_D float mr_pdf(const bxdf& b, f3 wo, f3 wi)
     return pr * pdf_r + pt * pdf_d;
 }
 
+// Dielectric with multiple-scattering compensation,
+// dielectric_multi_scattering_bxdf.cpp:9-433, bxdfs.h:145-247.  The host
+// assigns this kind only to a rough interface with eta != 1, so the specular
+// branches (which equal the plain dielectric's) are not needed here.
+
+// ImageTexture3D<Float>::Evaluate, trilinear with TexCoordFilter::clamp
+// (textures3d.h:36-88, texture3d.h:16-43), image(x, y, z) at
+// [(z * n + y) * n + x] (image3d.h:54-62)
+BB_D float tex_trilinear(const float* img, int32_t n, float u, float v, float w)
+{
+    const float fw = u * float(n) - 0.5f;
+    const float fh = v * float(n) - 0.5f;
+    const float fd = w * float(n) - 0.5f;
+    int32_t i0 = int32_t(floorf(fw)), i1 = i0 + 1;
+    int32_t j0 = int32_t(floorf(fh)), j1 = j0 + 1;
+    int32_t k0 = int32_t(floorf(fd)), k1 = k0 + 1;
+    i0 = min(max(i0, 0), n - 1);
+    j0 = min(max(j0, 0), n - 1);
+    k0 = min(max(k0, 0), n - 1);
+    i1 = min(max(i1, 0), n - 1);
+    j1 = min(max(j1, 0), n - 1);
+    k1 = min(max(k1, 0), n - 1);
+    const float fu = fw - floorf(fw);
+    const float fv = fh - floorf(fh);
+    const float fz = fd - floorf(fd);
+    auto at = [&](int32_t i, int32_t j, int32_t k) { return img[(k * n + j) * n + i]; };
+    const float v00 = (1.0f - fu) * at(i0, j0, k0) + fu * at(i1, j0, k0);
+    const float v01 = (1.0f - fu) * at(i0, j0, k1) + fu * at(i1, j0, k1);
+    const float v10 = (1.0f - fu) * at(i0, j1, k0) + fu * at(i1, j1, k0);
+    const float v11 = (1.0f - fu) * at(i0, j1, k1) + fu * at(i1, j1, k1);
+    const float v0 = (1.0f - fv) * v00 + fv * v10;
+    const float v1 = (1.0f - fv) * v01 + fv * v11;
+    return (1.0f - fz) * v0 + fz * v1;
+}
+
+// MapIORtoF0 (scattering.h:205-208)
+BB_D float map_ior_to_f0(float ior) { return sqrtf((ior - 1.0f) / (ior + 1.0f)); }
+
+// E(wo, eta) / E_avg(eta) (bxdfs.h:180-212): tables of the interface seen
+// from outside (eta >= 1) or from inside, at GetMeanAlpha
+BB_D float dms_E(const bxdf& b, f3 w, float eta)
+{
+    constexpr int32_t n = kTableSize;
+    const float a = sqrtf(b.mf.ax * b.mf.ay);
+    if (eta >= 1.0f) return tex_trilinear(b.dt, n, map_ior_to_f0(eta), fabsf(w.z), a);
+    return tex_trilinear(b.dt + n * n * n, n, map_ior_to_f0(1.0f / eta), fabsf(w.z), a);
+}
+BB_D float dms_E_avg(const bxdf& b, float eta)
+{
+    constexpr int32_t n = kTableSize;
+    const float a = sqrtf(b.mf.ax * b.mf.ay);
+    const float* avg = b.dt + 2 * n * n * n;
+    if (eta >= 1.0f) return tex_bilinear(avg, n, n, map_ior_to_f0(eta), a);
+    return tex_bilinear(avg + n * n, n, n, map_ior_to_f0(1.0f / eta), a);
+}
+
+// FresnelDielectricAverage (scattering.h:154-172), eta_p = eta_i / eta_t
+BB_D float fresnel_dielectric_average(float eta_p)
+{
+    if (eta_p < 1.0f) {
+        const float eta2 = eta_p * eta_p;
+        const float eta3 = eta_p * eta2;
+        return 0.997118f + 0.1014f * eta_p - 0.965241f * eta2 - 0.130607f * eta3;
+    }
+    return (eta_p - 1.0f) / (4.08567f + 1.00071f * eta_p);
+}
+
+// ComputeScatteringRatio (bxdfs.h:222-242): the share of the
+// multiple-scattering energy that leaves on wo's side
+BB_D float dms_ratio(const bxdf& b, float eta_o)
+{
+    const float eta_i = b.eta >= 1.0f ? b.eta : 1.0f / b.eta;
+    const float eta_t = 1.0f / eta_i;
+    const float a = (1.0f - fresnel_dielectric_average(eta_i)) / (1.0f - dms_E_avg(b, eta_t));
+    const float c = sqr(eta_i) * (1.0f - fresnel_dielectric_average(eta_t)) / (1.0f - dms_E_avg(b, eta_i));
+    const float x = 1.0f / (a / c + 1.0f);
+    if (eta_o >= 1.0f) return 1.0f - x * (1.0f - fresnel_dielectric_average(eta_o));
+    return 1.0f - (1.0f - x) * (1.0f - fresnel_dielectric_average(eta_o));
+}
+
+// the multiple-scattering lobes of reflection and transmission
+BB_D float dms_fr_ms(const bxdf& b, f3 wo, f3 wi, float ratio, float eta_o)
+{
+    return ratio * (1.0f - dms_E(b, wi, eta_o)) * (1.0f - dms_E(b, wo, eta_o)) /
+           fmaxf(1e-4f, kPi * (1.0f - dms_E_avg(b, eta_o)));
+}
+BB_D float dms_ft_ms(const bxdf& b, f3 wo, f3 wi, float ratio, float eta_o, float eta_i)
+{
+    return (1.0f - ratio) * (1.0f - dms_E(b, wi, eta_i)) * (1.0f - dms_E(b, wo, eta_o)) /
+           fmaxf(1e-4f, kPi * (1.0f - dms_E_avg(b, eta_i)));
+}
+
+// f, TransportDirection::ToLight (dielectric_multi_scattering_bxdf.cpp:9-89)
+BB_D spec dielectric_ms_f(const bxdf& b, f3 wo, f3 wi)
+{
+    const float co = wo.z, ci = wi.z;
+    float eta_p = 1.0f;
+    const bool refl = ci * co > 0.0f;
+    if (!refl) eta_p = co > 0.0f ? b.eta : (1.0f / b.eta);
+    const float eta_o = co > 0.0f ? b.eta : 1.0f / b.eta;
+    const float eta_i = co > 0.0f ? 1.0f / b.eta : b.eta;
+    f3 wm = wi * eta_p + wo;
+    if (ci == 0.0f || co == 0.0f || length2(wm) == 0.0f) return mks(0.0f);
+    normalize_m(wm);
+    if (dot_z(wm) < 0.0f) wm = -wm;
+    if (dot(wm, wi) * ci < 0.0f || dot(wm, wo) * co < 0.0f) return mks(0.0f);
+    const float ratio = dms_ratio(b, eta_o);
+    const float F = fresnel_dielectric(dot(wo, wm), b.eta);
+    if (refl) {
+        const float fr_ss = F * ggx_D(b.mf, wm) * ggx_G(b.mf, wo, wi) / (4.0f * ci * co);
+        return mks(fr_ss + dms_fr_ms(b, wo, wi, ratio, eta_o));
+    }
+    const float denom = sqr(dot(wi, wm) + dot(wo, wm) / eta_p);
+    float ft_ss = (1.0f - F) * ggx_D(b.mf, wm) * ggx_G(b.mf, wo, wi) * fabsf(dot(wi, wm) * dot(wo, wm) / (ci * co * denom));
+    const float ft_ms = dms_ft_ms(b, wo, wi, ratio, eta_o, eta_i);
+    ft_ss /= sqr(eta_p);
+    return b.r * (ft_ss + ft_ms);
+}
+
+// PDF, all sampling flags (dielectric_multi_scattering_bxdf.cpp:91-187)
+BB_D float dielectric_ms_pdf(const bxdf& b, f3 wo, f3 wi)
+{
+    const float co = wo.z, ci = wi.z;
+    float eta_p = 1.0f;
+    const bool refl = ci * co > 0.0f;
+    if (!refl) eta_p = co > 0.0f ? b.eta : (1.0f / b.eta);
+    const float eta_o = co > 0.0f ? b.eta : 1.0f / b.eta;
+    f3 wm = wi * eta_p + wo;
+    if (ci == 0.0f || co == 0.0f || length2(wm) == 0.0f) return 0.0f;
+    normalize_m(wm);
+    if (dot_z(wm) < 0.0f) wm = -wm;
+    if (dot(wm, wi) * ci < 0.0f || dot(wm, wo) * co < 0.0f) return 0.0f;
+    const float ratio = dms_ratio(b, eta_o);
+    const float E_o = clampf_(dms_E(b, wo, eta_o), 1e-4f, 1.0f - 1e-4f);
+    const float R = fresnel_dielectric(dot(wo, wm), b.eta);
+    const float T = 1.0f - R;
+    float pr_ss = R, pt_ss = T, pr_ms = ratio, pt_ms = 1.0f - ratio;
+    if (pr_ss == 0.0f && pt_ss == 0.0f && pr_ms == 0.0f && pt_ms == 0.0f) return 0.0f;
+    const float p_ss_sum = pr_ss + pt_ss, p_ms_sum = pr_ms + pt_ms;
+    pr_ss /= p_ss_sum;
+    pt_ss /= p_ss_sum;
+    pr_ms /= p_ms_sum;
+    pt_ms /= p_ms_sum;
+    float pdf_ss, pdf_ms;
+    if (refl) {
+        pdf_ss = pr_ss * ggx_pdf(b.mf, wo, wm) / (4.0f * absdot(wo, wm));
+        pdf_ms = pr_ms * (abs_cos_theta(wi) * kInvPi);
+    } else {
+        const float dwm_dwi = absdot(wi, wm) / sqr(dot(wi, wm) + dot(wo, wm) / eta_p);
+        pdf_ss = pt_ss * ggx_pdf(b.mf, wo, wm) * dwm_dwi;
+        pdf_ms = pt_ms * (abs_cos_theta(wi) * kInvPi);
+    }
+    return pdf_ms * (1.0f - E_o) + pdf_ss * E_o; // Lerp(pdf_ms, pdf_ss, E_o), math.h:87-90
+}
+
+// Sample_f, rough branch, ToLight, all sampling flags
+// (dielectric_multi_scattering_bxdf.cpp:246-433)
+BB_D bool dielectric_ms_sample(const bxdf& b, f3 wo, float u0, f2 u12, bsample& s)
+{
+    const float eta_o = wo.z > 0.0f ? b.eta : 1.0f / b.eta;
+    const float eta_i = wo.z > 0.0f ? 1.0f / b.eta : b.eta;
+    const float ratio = dms_ratio(b, eta_o);
+    const float E_o = clampf_(dms_E(b, wo, eta_o), 1e-4f, 1.0f - 1e-4f);
+    float pr_ms = ratio, pt_ms = 1.0f - ratio;
+    if (u0 < E_o) {
+        // single scattering at the rough interface
+        f3 wm = ggx_sample_wm(b.mf, wo, u12);
+        normalize_m(wm);
+        if (dot_z(wm) < 0.0f) wm = -wm;
+        const float R = fresnel_dielectric(dot(wo, wm), b.eta);
+        const float T = 1.0f - R;
+        float pr_ss = R, pt_ss = T;
+        if (pr_ss == 0.0f && pt_ss == 0.0f) return false;
+        const float p_ss_sum = pr_ss + pt_ss, p_ms_sum = pr_ms + pt_ms;
+        pr_ss /= p_ss_sum;
+        pt_ss /= p_ss_sum;
+        pr_ms /= p_ms_sum;
+        pt_ms /= p_ms_sum;
+        u0 /= E_o;
+        if (u0 < pr_ss) {
+            const f3 wi = reflect(wo, wm);
+            if (!same_hemisphere(wo, wi)) return false;
+            const float pdf_ss = pr_ss * ggx_pdf(b.mf, wo, wm) / (4.0f * absdot(wo, wm));
+            const float pdf_ms = pr_ms * (abs_cos_theta(wi) * kInvPi);
+            const float fr_ss = R * ggx_D(b.mf, wm) * ggx_G(b.mf, wo, wi) / (4.0f * wi.z * wo.z);
+            s.f = mks(fr_ss + dms_fr_ms(b, wo, wi, ratio, eta_o));
+            s.wi = wi;
+            s.pdf = pdf_ms * (1.0f - E_o) + pdf_ss * E_o;
+            s.flags = BX_GLOSSY | BX_REFLECTION;
+            return true;
+        }
+        f3 wi = mk3(0.0f);
+        float eta_unused = 0.0f;
+        const bool tir = !refract(wi, wo, wm, b.eta, eta_unused);
+        if (same_hemisphere(wo, wi) || wi.z == 0.0f || tir) return false;
+        const float eta_p = wo.z > 0.0f ? b.eta : (1.0f / b.eta);
+        const float denom = sqr(dot(wi, wm) + dot(wo, wm) / eta_p);
+        const float dwm_dwi = absdot(wi, wm) / denom;
+        const float pdf_ss = pt_ss * ggx_pdf(b.mf, wo, wm) * dwm_dwi;
+        const float pdf_ms = pt_ms * (abs_cos_theta(wi) * kInvPi);
+        float ft_ss = T * ggx_D(b.mf, wm) * ggx_G(b.mf, wo, wi) * fabsf(dot(wi, wm) * dot(wo, wm) / (wi.z * wo.z * denom));
+        const float ft_ms = dms_ft_ms(b, wo, wi, ratio, eta_o, eta_i);
+        ft_ss /= sqr(eta_p);
+        s.f = b.r * (ft_ss + ft_ms);
+        s.wi = wi;
+        s.pdf = pdf_ms * (1.0f - E_o) + pdf_ss * E_o;
+        s.flags = BX_GLOSSY | BX_TRANSMISSION;
+        s.eta = eta_p;
+        return true;
+    }
+    // the multiple-scattering lobe: cosine-distributed reflection or transmission
+    if (pr_ms == 0.0f && pt_ms == 0.0f) return false;
+    const float p_ms_sum = pr_ms + pt_ms;
+    pr_ms /= p_ms_sum;
+    pt_ms /= p_ms_sum;
+    u0 = (u0 - E_o) / (1.0f - E_o);
+    f3 wi = sample_cosine_hemisphere(u12);
+    if (u0 < pr_ms) {
+        if (wo.z < 0.0f) wi = -wi;
+        const f3 wm = normalize(wi + wo);
+        const float R = fresnel_dielectric(dot(wo, wm), b.eta);
+        float pr_ss = R, pt_ss = 1.0f - R;
+        const float p_ss_sum = pr_ss + pt_ss;
+        pr_ss /= p_ss_sum;
+        pt_ss /= p_ss_sum;
+        const float pdf_ss = pr_ss * ggx_pdf(b.mf, wo, wm) / (4.0f * absdot(wo, wm));
+        const float pdf_ms = pr_ms * (abs_cos_theta(wi) * kInvPi);
+        const float fr_ss = R * ggx_D(b.mf, wm) * ggx_G(b.mf, wo, wi) / (4.0f * wi.z * wo.z);
+        s.f = mks(fr_ss + dms_fr_ms(b, wo, wi, ratio, eta_o));
+        s.wi = wi;
+        s.pdf = pdf_ms * (1.0f - E_o) + pdf_ss * E_o;
+        s.flags = BX_DIFFUSE | BX_REFLECTION;
+        return true;
+    }
+    if (wo.z > 0.0f) wi = -wi;
+    f3 wm = normalize(wi * eta_o + wo);
+    if (dot_z(wm) < 0.0f) wm = -wm;
+    const float R = fresnel_dielectric(dot(wo, wm), b.eta);
+    const float T = 1.0f - R;
+    float pr_ss = R, pt_ss = 1.0f - R;
+    const float p_ss_sum = pr_ss + pt_ss;
+    pr_ss /= p_ss_sum;
+    pt_ss /= p_ss_sum;
+    const float denom = sqr(dot(wi, wm) + dot(wo, wm) / eta_o);
+    const float dwm_dwi = absdot(wi, wm) / denom;
+    const float pdf_ss = pt_ss * ggx_pdf(b.mf, wo, wm) * dwm_dwi;
+    const float pdf_ms = pt_ms * (abs_cos_theta(wi) * kInvPi);
+    float ft_ss = T * ggx_D(b.mf, wm) * ggx_G(b.mf, wo, wi) * fabsf(dot(wi, wm) * dot(wo, wm) / (wi.z * wo.z * denom));
+    const float ft_ms = dms_ft_ms(b, wo, wi, ratio, eta_o, eta_i);
+    ft_ss /= sqr(eta_o);
+    s.f = b.r * (ft_ss + ft_ms);
+    s.wi = wi;
+    s.pdf = pdf_ms * (1.0f - E_o) + pdf_ss * E_o;
+    s.flags = BX_DIFFUSE | BX_TRANSMISSION;
+    s.eta = eta_o;
+    return true;
+}
+
 // BSDF::f / BSDF::PDF on local directions (bsdf.h:75-99)
 BB_D spec bxdf_f(const bxdf& b, f3 wo, f3 wi)
 {
     switch (b.kind) {
     case MK_LAMBERT: return lambert_f(b, wo, wi);
     case MK_DIELECTRIC: return dielectric_f(b, wo, wi);
+    case MK_DIELECTRIC_MS: return dielectric_ms_f(b, wo, wi);
     case MK_CONDUCTOR: return conductor_f(b, wo, wi);
     case MK_CONDUCTOR_MS: return conductor_ms_f(b, wo, wi);
     default: return mr_f(b, wo, wi);
@@ -427,6 +689,7 @@ BB_D float bxdf_pdf(const bxdf& b, f3 wo, f3 wi)
     switch (b.kind) {
     case MK_LAMBERT: return lambert_pdf(wo, wi);
     case MK_DIELECTRIC: return dielectric_pdf(b, wo, wi);
+    case MK_DIELECTRIC_MS: return dielectric_ms_pdf(b, wo, wi);
     case MK_CONDUCTOR: return conductor_pdf(b, wo, wi);
     case MK_CONDUCTOR_MS: return conductor_ms_pdf(b, wo, wi);
     default: return mr_pdf(b, wo, wi);
@@ -448,6 +711,7 @@ BB_D bool bxdf_sample(const bxdf& b, f3 wo, float u0, f2 u12, bsample& s)
         s.flags = BX_DIFFUSE | BX_REFLECTION;
         return true;
     }
+    case MK_DIELECTRIC_MS: return dielectric_ms_sample(b, wo, u0, u12, s);
     case MK_CONDUCTOR_MS:
         if (!ggx_smooth(b.mf)) {
             if (wo.z == 0.0f) return false;

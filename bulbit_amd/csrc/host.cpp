@@ -356,10 +356,10 @@ bool material_to_gpu(const BbMaterial& m, GpuMaterial& g, std::string& err)
         g.ay = sqr(m.v_roughness);
         g.r = sqrt_s(mks(m.color[0], m.color[1], m.color[2]));
         bool smooth = smax(g.ax, g.ay) < 1e-3f;
-        if (m.energy_compensation && !smooth) {
-            err = "energy-compensated (multi-scattering) rough dielectric is not supported";
-            return false;
-        }
+        // DielectricMultiScatteringBxDF (bxdfs.h:145-247) when compensated; smooth
+        // or with eta 1 it takes the plain BxDF's specular branch
+        // (dielectric_multi_scattering_bxdf.cpp:11-14, 90-93, 196-244)
+        if (m.energy_compensation && !smooth && g.eta != 1.0f) g.kind = MK_DIELECTRIC_MS;
         int f = (g.eta == 1.0f) ? BX_TRANSMISSION : (BX_REFLECTION | BX_TRANSMISSION);
         g.flags_front = f | (smooth ? BX_SPECULAR : BX_GLOSSY);
         return true;
@@ -622,6 +622,13 @@ bool create_scene(const BbSceneDesc& d, BbScene* s, std::string& err)
     const ReflectanceTables& tab = reflectance_tables();
     if (!upload(s, tab.ggx_E, size_t(kTableSize) * kTableSize, &V.ggx_E, err)) return false;
     if (!upload(s, tab.ggx_E_avg, size_t(kTableSize), &V.ggx_E_avg, err)) return false;
+    V.dielectric = nullptr;
+    if (std::any_of(mats.begin(), mats.end(), [](const GpuMaterial& g) { return g.kind == MK_DIELECTRIC_MS; })) {
+        const DielectricTables& dt = dielectric_tables();
+        static_assert(sizeof(DielectricTables) == sizeof(float) * kDielectricTableFloats, "packed tables");
+        if (!upload(s, reinterpret_cast<const float*>(&dt), size_t(kDielectricTableFloats), &V.dielectric, err))
+            return false;
+    }
     V.n_lights = nl;
     V.n_inf = s->n_inf;
     V.func_integral = func_integral;
@@ -678,6 +685,49 @@ int bbgpu_build_bvh(const BbSceneDesc* desc, BbBvhNode* nodes_out, int32_t* orde
     memcpy(nodes_out, bvh.nodes.data(), sizeof(BbBvhNode) * bvh.nodes.size());
     memcpy(ordered_out, bvh.ordered.data(), sizeof(int32_t) * bvh.ordered.size());
     *n_nodes_out = int32_t(bvh.nodes.size());
+    return BBGPU_OK;
+}
+
+int bbgpu_reflectance_table(int32_t table, float* out, int32_t capacity, int32_t* length_out)
+{
+    if (!length_out) {
+        bb::set_error("bbgpu_reflectance_table: null length_out");
+        return BBGPU_ERR_INVALID;
+    }
+    const float* src = nullptr;
+    int32_t len = 0;
+    constexpr int32_t n = bb::kTableSize;
+    switch (table) {
+    case BB_TABLE_GGX_E:
+        src = bb::reflectance_tables().ggx_E;
+        len = n * n;
+        break;
+    case BB_TABLE_GGX_E_AVG:
+        src = bb::reflectance_tables().ggx_E_avg;
+        len = n;
+        break;
+    case BB_TABLE_DIELECTRIC_E:
+        src = bb::dielectric_tables().E;
+        len = n * n * n;
+        break;
+    case BB_TABLE_DIELECTRIC_E_INV:
+        src = bb::dielectric_tables().E_inv;
+        len = n * n * n;
+        break;
+    case BB_TABLE_DIELECTRIC_E_AVG:
+        src = bb::dielectric_tables().E_avg;
+        len = n * n;
+        break;
+    case BB_TABLE_DIELECTRIC_E_INV_AVG:
+        src = bb::dielectric_tables().E_inv_avg;
+        len = n * n;
+        break;
+    default:
+        bb::set_error("bbgpu_reflectance_table: unknown table %d", int(table));
+        return BBGPU_ERR_INVALID;
+    }
+    *length_out = len;
+    if (out && capacity >= len) memcpy(out, src, sizeof(float) * size_t(len));
     return BBGPU_OK;
 }
 

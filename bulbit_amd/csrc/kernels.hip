@@ -825,7 +825,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView S, RenderConsts
                         new_d = -wo;
                         cont = true;
                     } else {
-                        const bxdf b = make_bxdf(S.materials[mi], sh.front, S.ggx_E, S.ggx_E_avg);
+                        const bxdf b = make_bxdf(S.materials[mi], sh.front, S);
                         const frame3 fr = frame_from_xz(sh.ts, sh.ns);
                         const f3 wo_l = to_local(fr, wo);
                         if (b.flags & (BX_DIFFUSE | BX_GLOSSY)) {
@@ -1605,12 +1605,16 @@ int render_impl(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams
             const int rows = std::min(cnt[1], Cv.max_rows);
             const int batch = cache->cp.train_batch > 0 ? cache->cp.train_batch : 65536;
             float loss = 0.0f;
-            for (int s1 = 0; s1 < cache->cp.train_steps && s1 * batch < rows; ++s1) {
-                const int b = std::min(batch, rows - s1 * batch);
-                const int rt = bbgpu_nrc_train(cache->nrc, Cv.rows + 16 * size_t(s1) * batch,
-                                               Cv.targets + 3 * size_t(s1) * batch, b, 1,
+            int64_t trained = 0;
+            // steps take the rows batch by batch and start over when they run out
+            const int n_batches = (rows + batch - 1) / batch;
+            for (int s1 = 0; rows > 0 && s1 < cache->cp.train_steps; ++s1) {
+                const size_t off = size_t(s1 % n_batches) * size_t(batch);
+                const int b = std::min(batch, rows - int(off));
+                const int rt = bbgpu_nrc_train(cache->nrc, Cv.rows + 16 * off, Cv.targets + 3 * off, b, 1,
                                                cache->stats ? &loss : nullptr, nullptr, st);
                 if (rt != BBGPU_OK) return rt;
+                trained += b;
             }
             hipEvent_t q2 = mark();
             if (cache->stats) {
@@ -1621,7 +1625,7 @@ int render_impl(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams
                 cache->stats->ms_query += ms_q;
                 cache->stats->ms_train += ms_t;
                 cache->stats->queries += cnt[0];
-                cache->stats->train_rows += rows;
+                cache->stats->train_rows += trained;
                 cache->stats->loss = loss;
             }
         }

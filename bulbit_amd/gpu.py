@@ -79,18 +79,23 @@ class GpuScene:
         )
         return st
 
+    def render_cached_device(self, nrc, camera, params, cache_params, sum_dev, count_dev, stream=None):
+        """bbgpu_render_cached into device film buffers; returns BbCacheStats."""
+        st = capi.BbCacheStats()
+        capi.check(
+            self.lib.bbgpu_render_cached(self.h, nrc.h, ctypes.byref(camera), ctypes.byref(params),
+                                         ctypes.byref(cache_params), ctypes.c_void_p(sum_dev),
+                                         ctypes.c_void_p(count_dev), stream, ctypes.byref(st)),
+            "bbgpu_render_cached",
+            self.lib,
+        )
+        return st
+
     def render_cached(self, nrc, camera, params, cache_params, film=None, stream=None):
         """bbgpu_render_cached: (sum[h, w, 3], count[h, w], BbCacheStats) as numpy;
         film (device tensors from new_film) accumulates across calls."""
         s, c = film if film is not None else self.new_film(camera)
-        st = capi.BbCacheStats()
-        capi.check(
-            self.lib.bbgpu_render_cached(self.h, nrc.h, ctypes.byref(camera), ctypes.byref(params),
-                                         ctypes.byref(cache_params), ctypes.c_void_p(s.data_ptr()),
-                                         ctypes.c_void_p(c.data_ptr()), stream, ctypes.byref(st)),
-            "bbgpu_render_cached",
-            self.lib,
-        )
+        st = self.render_cached_device(nrc, camera, params, cache_params, s.data_ptr(), c.data_ptr(), stream)
         h, w = camera.height, camera.width
         return s.cpu().numpy().reshape(h, w, 3), c.cpu().numpy().reshape(h, w), st
 

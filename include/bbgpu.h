@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define BBGPU_ABI_VERSION 7
+#define BBGPU_ABI_VERSION 8
 
 /* status codes */
 enum {
@@ -253,6 +253,28 @@ int bbgpu_last_error(char* buf, size_t buf_size);
 /* Number of visible HIP devices (0 when none); selects nothing. */
 int bbgpu_device_count(int32_t* count);
 
+/* The reflectance tables that PrepareReflectanceTextures computes
+ * (src/bxdf/microfacet.cpp:31-75, 173-270) and the library uploads with a
+ * scene that needs them, as computed on the host.  Host-only.  Writes the
+ * table's floats to out when capacity is large enough and always sets
+ * *length_out.  Layouts, index [..] from the slowest axis:
+ *   BB_TABLE_GGX_E             [alpha 32][cos_theta 32]
+ *   BB_TABLE_GGX_E_AVG         [alpha 32]
+ *   BB_TABLE_DIELECTRIC_E      [alpha 32][cos_theta 32][f0 32]  (eta >= 1)
+ *   BB_TABLE_DIELECTRIC_E_INV  same, for the inverted interface
+ *   BB_TABLE_DIELECTRIC_E_AVG  [alpha 32][f0 32], and _E_INV_AVG
+ * (f0 = MapIORtoF0, scattering.h:188-208).  The reference keeps them in
+ * texture objects and has no accessor; this entry point is for inspection. */
+enum {
+    BB_TABLE_GGX_E = 0,
+    BB_TABLE_GGX_E_AVG = 1,
+    BB_TABLE_DIELECTRIC_E = 2,
+    BB_TABLE_DIELECTRIC_E_INV = 3,
+    BB_TABLE_DIELECTRIC_E_AVG = 4,
+    BB_TABLE_DIELECTRIC_E_INV_AVG = 5
+};
+int bbgpu_reflectance_table(int32_t table, float* out, int32_t capacity, int32_t* length_out);
+
 /* Builds the SAH BVH on the host exactly as BVH::BVH
  * (src/accel/bvh.cpp:8-49, 56-271, 273-297) and returns the flattened nodes and
  * the primitive order.  Host-only: makes no HIP call.  nodes_out must hold
@@ -363,7 +385,9 @@ int bbgpu_nrc_train(BbNrc* nrc, const float* inputs_dev, const float* targets_de
  * cache(vertex); every train_stride-th path is a
  * training path that continues train_extra_bounces vertices further without
  * the cache, and the reflected radiance it measured at each vertex trains
- * the cache after the pass (train_steps Adam steps of train_batch rows). */
+ * the cache after the pass: train_steps Adam steps of up to train_batch
+ * rows, taking the pass's rows batch by batch and starting over when they
+ * run out (fewer rows than one batch: every step trains on all of them). */
 typedef struct BbCacheParams {
     int32_t query_bounce;         /* 2 */
     int32_t train_stride;         /* 16 */
@@ -378,7 +402,7 @@ typedef struct BbCacheStats {
     double ms_query;   /* cache inference and its application */
     double ms_train;   /* training rows and optimizer steps */
     int64_t queries;
-    int64_t train_rows;
+    int64_t train_rows; /* rows trained on, counted once per step that used them */
     float loss;        /* relative L2 loss of the last step */
 } BbCacheStats;
 

@@ -48,6 +48,7 @@ def load_oracle():
     lib.orc_permutation_element.restype = c_int32
     lib.orc_permutation_element.argtypes = [c_uint32, c_uint32, c_uint32]
     lib.orc_reflectance_tables.argtypes = [POINTER(c_float)] * 4
+    lib.orc_dielectric_tables.argtypes = [POINTER(c_float)] * 4
     lib.orc_ggx_E.restype = c_float
     lib.orc_ggx_E.argtypes = [c_float, c_float, POINTER(c_float)]
     lib.orc_ggx_E_avg.restype = c_float

@@ -28,6 +28,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
+#include <unistd.h>
 
 /* ================================================================ hashing */
 
@@ -1228,7 +1229,7 @@ float orc_ggx_E_avg(float ax, float ay)
 
 /* ================================================================= BxDFs */
 
-enum { K_LAMBERT = 0, K_DIELECTRIC = 1, K_CONDUCTOR = 2, K_MR = 3, K_CONDUCTOR_MS = 4 };
+enum { K_LAMBERT = 0, K_DIELECTRIC = 1, K_CONDUCTOR = 2, K_MR = 3, K_CONDUCTOR_MS = 4, K_DIELECTRIC_MS = 5 };
 
 typedef struct {
     int kind;
@@ -1253,7 +1254,8 @@ static int bxdf_flags(const bxdf_t* b)
 {
     switch (b->kind) {
     case K_LAMBERT: return !sblack(b->r) ? (BX_DIFFUSE | BX_REFLECTION) : 0;
-    case K_DIELECTRIC: {
+    case K_DIELECTRIC:
+    case K_DIELECTRIC_MS: {
         int f = (b->eta == 1.0f) ? BX_TRANSMISSION : (BX_REFLECTION | BX_TRANSMISSION);
         return f | (ggx_smooth(b->mf) ? BX_SPECULAR : BX_GLOSSY);
     }
@@ -1462,7 +1464,9 @@ static float dielectric_pdf(const bxdf_t* b, v3 wo, v3 wi)
     float dwm_dwi = vabsdot(wi, wm) / sqrf(vdot(wi, wm) + vdot(wo, wm) / eta_p);
     return ggx_pdf(b->mf, wo, wm) * dwm_dwi * pt / (pr + pt);
 }
-static int dielectric_sample(const bxdf_t* b, v3 wo, float u0, v2 u12, bsample* s)
+/* to_light: TransportDirection::ToLight (radiance, the path tracer's
+ * direction); 0 = ToCamera (the reflectance tables) */
+static int dielectric_sample(const bxdf_t* b, v3 wo, float u0, v2 u12, int to_light, bsample* s)
 {
     s->stochastic = 0;
     if (b->eta == 1.0f || ggx_smooth(b->mf)) {
@@ -1482,7 +1486,7 @@ static int dielectric_sample(const bxdf_t* b, v3 wo, float u0, v2 u12, bsample* 
             float eta_p;
             if (!refract(&wi, wo, V3(0.0f, 0.0f, 1.0f), b->eta, &eta_p)) return 0;
             float ft = T / abs_cos_theta(wi);
-            ft /= sqrf(eta_p);
+            if (to_light) ft /= sqrf(eta_p);
             s->f = smulf(b->r, ft);
             s->wi = wi;
             s->pdf = pt / (pr + pt);
@@ -1515,13 +1519,398 @@ static int dielectric_sample(const bxdf_t* b, v3 wo, float u0, v2 u12, bsample* 
         float pdf = ggx_pdf(b->mf, wo, wm) * dwm_dwi * pt / (pr + pt);
         float ft = T * ggx_D(b->mf, wm) * ggx_G(b->mf, wo, wi) *
                    fabsf(vdot(wi, wm) * vdot(wo, wm) / (wi.z * wo.z * denom));
-        ft /= sqrf(eta_p);
+        if (to_light) ft /= sqrf(eta_p);
         s->f = smulf(b->r, ft);
         s->wi = wi;
         s->pdf = pdf;
         s->flags = BX_GLOSSY | BX_TRANSMISSION;
         s->eta = eta_p;
     }
+    return 1;
+}
+
+/* ---- Dielectric with multiple-scattering compensation:
+ *      src/bxdf/dielectric_multi_scattering_bxdf.cpp:9-438, include/bulbit/bxdfs.h:145-247 ---- */
+
+/* DielectricMultiScatteringBxDF::PrepareReflectanceTexture (microfacet.cpp:173-236):
+ * E and E_inv at (f0 = x, cos_theta = y, alpha = z), [(z * 32 + y) * 32 + x]
+ * (image3d.h:54-62), E_avg and E_inv_avg at (f0 = x, alpha = y), [y * 32 + x]
+ * (image.h:55-63); E_inv is the table of the inverted interface (1 / ior) */
+typedef struct {
+    float E[ORC_TABLE_N * ORC_TABLE_N * ORC_TABLE_N];
+    float E_inv[ORC_TABLE_N * ORC_TABLE_N * ORC_TABLE_N];
+    float E_avg[ORC_TABLE_N * ORC_TABLE_N];
+    float E_inv_avg[ORC_TABLE_N * ORC_TABLE_N];
+} dms_tables_t;
+
+static dms_tables_t g_dms;
+static pthread_once_t g_dms_once = PTHREAD_ONCE_INIT;
+
+/* F0toIOR / MapF0toIOR / MapIORtoF0 (scattering.h:188-208) */
+static float f0_to_ior(float f0)
+{
+    float sqrt_f0 = sqrtf(f0);
+    return (1.0f + sqrt_f0) / (1.0f - sqrt_f0);
+}
+static float map_f0_to_ior(float f) { return f0_to_ior(sqrf(sqrf(f))); }
+static float map_ior_to_f0(float ior) { return sqrtf((ior - 1.0f) / (ior + 1.0f)); }
+
+/* BxDF::rho(wo, uc, u, ToCamera) (bxdf.cpp:8-21) of DielectricBxDF(ior, GGX(a, a), 1),
+ * then Spectrum::Average (spectrum.h:59-71) of its three equal channels */
+static float dielectric_rho(float ior, float a, v3 wo, const refl_tables_t* t)
+{
+    bxdf_t b;
+    memset(&b, 0, sizeof(b));
+    b.kind = K_DIELECTRIC;
+    b.eta = ior;
+    b.mf.ax = a;
+    b.mf.ay = a;
+    b.r = S1(1.0f); /* Sqrt(Spectrum(1)) */
+    float r = 0.0f;
+    for (int32_t i = 0; i < ORC_TABLE_SAMPLES; ++i) {
+        bsample smp;
+        memset(&smp, 0, sizeof(smp));
+        if (dielectric_sample(&b, wo, t->u0[i], t->u12[i], 0, &smp)) r += smp.f.r * abs_cos_theta(smp.wi) / smp.pdf;
+    }
+    r = r / (float)ORC_TABLE_SAMPLES;
+    return (r + r + r) / 3.0f;
+}
+
+typedef struct {
+    int32_t first, step;
+} dms_job;
+
+static void* dms_worker(void* arg)
+{
+    const dms_job* job = (const dms_job*)arg;
+    const refl_tables_t* t = tables();
+    dms_tables_t* o = &g_dms;
+    const int32_t n = ORC_TABLE_N;
+    const float d = 1.0f / (float)n;
+    for (int32_t k = job->first; k < n; k += job->step) {
+        float f = d / 2.0f + d * (float)k;
+        float ior = map_f0_to_ior(f);
+        for (int32_t j = 0; j < n; ++j) {
+            float a = d / 2.0f + d * (float)j;
+            float r_sum = 0.0f, r_inv_sum = 0.0f;
+            for (int32_t i = 0; i < n; ++i) {
+                float cos_theta = d / 2.0f + d * (float)i;
+                float sin_theta = sqrtf(1.0f - sqrf(cos_theta));
+                v3 wo = V3(sin_theta, 0.0f, cos_theta);
+                float r = dielectric_rho(ior, a, wo, t);
+                float r_inv = dielectric_rho(1.0f / ior, a, wo, t);
+                r_sum += r * cos_theta * d;
+                r_inv_sum += r_inv * cos_theta * d;
+                o->E[(j * n + i) * n + k] = r;
+                o->E_inv[(j * n + i) * n + k] = r_inv;
+            }
+            o->E_avg[j * n + k] = 2.0f * r_sum;
+            o->E_inv_avg[j * n + k] = 2.0f * r_inv_sum;
+        }
+    }
+    return NULL;
+}
+
+/* the 32 f0 slices are independent (the reference's ParallelFor over k) */
+static void dms_build(void)
+{
+    enum { MAX_THREADS = 16 };
+    long cpus = sysconf(_SC_NPROCESSORS_ONLN);
+    int32_t nt = cpus < 1 ? 1 : (cpus > MAX_THREADS ? MAX_THREADS : (int32_t)cpus);
+    pthread_t th[MAX_THREADS];
+    dms_job jobs[MAX_THREADS];
+    (void)tables();
+    int32_t started = 0;
+    for (int32_t i = 0; i < nt; ++i) {
+        jobs[i].first = i;
+        jobs[i].step = nt;
+        if (pthread_create(&th[i], NULL, dms_worker, &jobs[i]) != 0) break;
+        ++started;
+    }
+    if (started < nt) { /* run the slices of threads that did not start here */
+        for (int32_t i = started; i < nt; ++i) dms_worker(&jobs[i]);
+    }
+    for (int32_t i = 0; i < started; ++i) pthread_join(th[i], NULL);
+}
+
+static const dms_tables_t* dms_tables(void)
+{
+    pthread_once(&g_dms_once, dms_build);
+    return &g_dms;
+}
+
+void orc_dielectric_tables(float* E, float* E_inv, float* E_avg, float* E_inv_avg)
+{
+    const dms_tables_t* t = dms_tables();
+    if (E) memcpy(E, t->E, sizeof(t->E));
+    if (E_inv) memcpy(E_inv, t->E_inv, sizeof(t->E_inv));
+    if (E_avg) memcpy(E_avg, t->E_avg, sizeof(t->E_avg));
+    if (E_inv_avg) memcpy(E_inv_avg, t->E_inv_avg, sizeof(t->E_inv_avg));
+}
+
+/* ImageTexture3D<Float>::Evaluate, trilinear with TexCoordFilter::clamp
+ * (textures3d.h:36-88, texture3d.h:16-43), on an n^3 image */
+static float tex_trilinear(const float* img, int32_t n, float u, float v, float w)
+{
+    float fw = u * (float)n - 0.5f;
+    float fh = v * (float)n - 0.5f;
+    float fd = w * (float)n - 0.5f;
+    int32_t i0 = (int32_t)floorf(fw), i1 = i0 + 1;
+    int32_t j0 = (int32_t)floorf(fh), j1 = j0 + 1;
+    int32_t k0 = (int32_t)floorf(fd), k1 = k0 + 1;
+#define ORC_CLAMPI(x) ((x) < 0 ? 0 : ((x) > n - 1 ? n - 1 : (x)))
+    i0 = ORC_CLAMPI(i0);
+    j0 = ORC_CLAMPI(j0);
+    k0 = ORC_CLAMPI(k0);
+    i1 = ORC_CLAMPI(i1);
+    j1 = ORC_CLAMPI(j1);
+    k1 = ORC_CLAMPI(k1);
+#undef ORC_CLAMPI
+    float fu = fw - floorf(fw);
+    float fv = fh - floorf(fh);
+    float fz = fd - floorf(fd);
+#define ORC_AT(i, j, k) img[((k) * n + (j)) * n + (i)]
+    float v000 = ORC_AT(i0, j0, k0), v001 = ORC_AT(i0, j0, k1);
+    float v010 = ORC_AT(i0, j1, k0), v011 = ORC_AT(i0, j1, k1);
+    float v100 = ORC_AT(i1, j0, k0), v101 = ORC_AT(i1, j0, k1);
+    float v110 = ORC_AT(i1, j1, k0), v111 = ORC_AT(i1, j1, k1);
+#undef ORC_AT
+    float v00 = (1.0f - fu) * v000 + fu * v100;
+    float v01 = (1.0f - fu) * v001 + fu * v101;
+    float v10 = (1.0f - fu) * v010 + fu * v110;
+    float v11 = (1.0f - fu) * v011 + fu * v111;
+    float v0 = (1.0f - fv) * v00 + fv * v10;
+    float v1 = (1.0f - fv) * v01 + fv * v11;
+    return (1.0f - fz) * v0 + fz * v1;
+}
+
+/* DielectricMultiScatteringBxDF::E / E_avg (bxdfs.h:180-212) */
+static float dms_E(const bxdf_t* b, v3 w, float eta)
+{
+    const dms_tables_t* t = dms_tables();
+    float a = sqrtf(b->mf.ax * b->mf.ay);
+    if (eta >= 1.0f) return tex_trilinear(t->E, ORC_TABLE_N, map_ior_to_f0(eta), fabsf(w.z), a);
+    return tex_trilinear(t->E_inv, ORC_TABLE_N, map_ior_to_f0(1.0f / eta), fabsf(w.z), a);
+}
+static float dms_E_avg(const bxdf_t* b, float eta)
+{
+    const dms_tables_t* t = dms_tables();
+    float a = sqrtf(b->mf.ax * b->mf.ay);
+    if (eta >= 1.0f) return tex_bilinear(t->E_avg, ORC_TABLE_N, ORC_TABLE_N, map_ior_to_f0(eta), a);
+    return tex_bilinear(t->E_inv_avg, ORC_TABLE_N, ORC_TABLE_N, map_ior_to_f0(1.0f / eta), a);
+}
+
+/* FresnelDielectricAverage (scattering.h:154-172), eta_p = eta_i / eta_t */
+static float fresnel_dielectric_average(float eta_p)
+{
+    if (eta_p < 1.0f) {
+        const float eta2 = eta_p * eta_p;
+        const float eta3 = eta_p * eta2;
+        return 0.997118f + 0.1014f * eta_p - 0.965241f * eta2 - 0.130607f * eta3;
+    }
+    return (eta_p - 1.0f) / (4.08567f + 1.00071f * eta_p);
+}
+
+/* ComputeScatteringRatio (bxdfs.h:222-242) */
+static float dms_ratio(const bxdf_t* b, float eta_o)
+{
+    float eta_i = b->eta >= 1.0f ? b->eta : 1.0f / b->eta;
+    float eta_t = 1.0f / eta_i;
+    float a = (1.0f - fresnel_dielectric_average(eta_i)) / (1.0f - dms_E_avg(b, eta_t));
+    float bb = sqrf(eta_i) * (1.0f - fresnel_dielectric_average(eta_t)) / (1.0f - dms_E_avg(b, eta_i));
+    float x = 1.0f / (a / bb + 1.0f);
+    if (eta_o >= 1.0f) return 1.0f - x * (1.0f - fresnel_dielectric_average(eta_o));
+    return 1.0f - (1.0f - x) * (1.0f - fresnel_dielectric_average(eta_o));
+}
+
+/* the multiple-scattering lobes: ratio (1 - E(wi)) (1 - E(wo)) / max(1e-4, pi (1 - E_avg))
+ * for reflection, (1 - ratio) ... with eta_i for transmission */
+static float dms_fr_ms(const bxdf_t* b, v3 wo, v3 wi, float ratio, float eta_o)
+{
+    return ratio * (1.0f - dms_E(b, wi, eta_o)) * (1.0f - dms_E(b, wo, eta_o)) /
+           fmaxf(1e-4f, orc_pi * (1.0f - dms_E_avg(b, eta_o)));
+}
+static float dms_ft_ms(const bxdf_t* b, v3 wo, v3 wi, float ratio, float eta_o, float eta_i)
+{
+    return (1.0f - ratio) * (1.0f - dms_E(b, wi, eta_i)) * (1.0f - dms_E(b, wo, eta_o)) /
+           fmaxf(1e-4f, orc_pi * (1.0f - dms_E_avg(b, eta_i)));
+}
+
+/* f (dielectric_multi_scattering_bxdf.cpp:9-89), TransportDirection::ToLight */
+static spec dielectric_ms_f(const bxdf_t* b, v3 wo, v3 wi)
+{
+    if (b->eta == 1.0f || ggx_smooth(b->mf)) return S1(0.0f);
+    float cos_o = wo.z, cos_i = wi.z;
+    float eta_p = 1.0f;
+    int refl = cos_i * cos_o > 0.0f;
+    if (!refl) eta_p = cos_o > 0.0f ? b->eta : (1.0f / b->eta);
+    float eta_o = cos_o > 0.0f ? b->eta : 1.0f / b->eta;
+    float eta_i = cos_o > 0.0f ? 1.0f / b->eta : b->eta;
+    v3 wm = vadd(vmul(wi, eta_p), wo);
+    if (cos_i == 0.0f || cos_o == 0.0f || vlen2(wm) == 0.0f) return S1(0.0f);
+    vnormalize_m(&wm);
+    if (dot_z(wm) < 0.0f) wm = vneg(wm);
+    if (vdot(wm, wi) * cos_i < 0.0f || vdot(wm, wo) * cos_o < 0.0f) return S1(0.0f);
+    float ratio = dms_ratio(b, eta_o);
+    float F = fresnel_dielectric(vdot(wo, wm), b->eta);
+    if (refl) {
+        float fr_ss = F * ggx_D(b->mf, wm) * ggx_G(b->mf, wo, wi) / (4.0f * cos_i * cos_o);
+        float fr_ms = dms_fr_ms(b, wo, wi, ratio, eta_o);
+        return S1(fr_ss + fr_ms);
+    }
+    float denom = sqrf(vdot(wi, wm) + vdot(wo, wm) / eta_p);
+    float ft_ss = (1.0f - F) * ggx_D(b->mf, wm) * ggx_G(b->mf, wo, wi) *
+                  fabsf(vdot(wi, wm) * vdot(wo, wm) / (cos_i * cos_o * denom));
+    float ft_ms = dms_ft_ms(b, wo, wi, ratio, eta_o, eta_i);
+    ft_ss /= sqrf(eta_p);
+    return smulf(b->r, ft_ss + ft_ms);
+}
+
+/* PDF (dielectric_multi_scattering_bxdf.cpp:91-187), all sampling flags */
+static float dielectric_ms_pdf(const bxdf_t* b, v3 wo, v3 wi)
+{
+    if (b->eta == 1.0f || ggx_smooth(b->mf)) return 0.0f;
+    float cos_o = wo.z, cos_i = wi.z;
+    float eta_p = 1.0f;
+    int refl = cos_i * cos_o > 0.0f;
+    if (!refl) eta_p = cos_o > 0.0f ? b->eta : (1.0f / b->eta);
+    float eta_o = cos_o > 0.0f ? b->eta : 1.0f / b->eta;
+    v3 wm = vadd(vmul(wi, eta_p), wo);
+    if (cos_i == 0.0f || cos_o == 0.0f || vlen2(wm) == 0.0f) return 0.0f;
+    vnormalize_m(&wm);
+    if (dot_z(wm) < 0.0f) wm = vneg(wm);
+    if (vdot(wm, wi) * cos_i < 0.0f || vdot(wm, wo) * cos_o < 0.0f) return 0.0f;
+    float ratio = dms_ratio(b, eta_o);
+    float E_o = clampf_(dms_E(b, wo, eta_o), 1e-4f, 1.0f - 1e-4f);
+    float R = fresnel_dielectric(vdot(wo, wm), b->eta);
+    float T = 1.0f - R;
+    float pr_ss = R, pt_ss = T, pr_ms = ratio, pt_ms = 1.0f - ratio;
+    if (pr_ss == 0.0f && pt_ss == 0.0f && pr_ms == 0.0f && pt_ms == 0.0f) return 0.0f;
+    float p_ss_sum = pr_ss + pt_ss, p_ms_sum = pr_ms + pt_ms;
+    pr_ss /= p_ss_sum;
+    pt_ss /= p_ss_sum;
+    pr_ms /= p_ms_sum;
+    pt_ms /= p_ms_sum;
+    float pdf_ss, pdf_ms;
+    if (refl) {
+        pdf_ss = pr_ss * ggx_pdf(b->mf, wo, wm) / (4.0f * vabsdot(wo, wm));
+        pdf_ms = pr_ms * (abs_cos_theta(wi) * orc_inv_pi);
+    } else {
+        float dwm_dwi = vabsdot(wi, wm) / sqrf(vdot(wi, wm) + vdot(wo, wm) / eta_p);
+        pdf_ss = pt_ss * ggx_pdf(b->mf, wo, wm) * dwm_dwi;
+        pdf_ms = pt_ms * (abs_cos_theta(wi) * orc_inv_pi);
+    }
+    return pdf_ms * (1.0f - E_o) + pdf_ss * E_o; /* Lerp(pdf_ms, pdf_ss, E_o) */
+}
+
+/* Sample_f (dielectric_multi_scattering_bxdf.cpp:189-433), rough branch,
+ * TransportDirection::ToLight, all sampling flags */
+static int dielectric_ms_sample(const bxdf_t* b, v3 wo, float u0, v2 u12, bsample* s)
+{
+    s->stochastic = 0;
+    if (b->eta == 1.0f || ggx_smooth(b->mf)) return dielectric_sample(b, wo, u0, u12, 1, s);
+    float eta_o = wo.z > 0.0f ? b->eta : 1.0f / b->eta;
+    float eta_i = wo.z > 0.0f ? 1.0f / b->eta : b->eta;
+    float ratio = dms_ratio(b, eta_o);
+    float E_o = clampf_(dms_E(b, wo, eta_o), 1e-4f, 1.0f - 1e-4f);
+    float pr_ms = ratio, pt_ms = 1.0f - ratio;
+    if (u0 < E_o) {
+        /* single scattering at the rough interface */
+        v3 wm = ggx_sample_wm(b->mf, wo, u12);
+        vnormalize_m(&wm);
+        if (dot_z(wm) < 0.0f) wm = vneg(wm);
+        float R = fresnel_dielectric(vdot(wo, wm), b->eta);
+        float T = 1.0f - R;
+        float pr_ss = R, pt_ss = T;
+        if (pr_ss == 0.0f && pt_ss == 0.0f) return 0;
+        float p_ss_sum = pr_ss + pt_ss, p_ms_sum = pr_ms + pt_ms;
+        pr_ss /= p_ss_sum;
+        pt_ss /= p_ss_sum;
+        pr_ms /= p_ms_sum;
+        pt_ms /= p_ms_sum;
+        u0 /= E_o;
+        if (u0 < pr_ss) {
+            v3 wi = reflect(wo, wm);
+            if (!same_hemisphere(wo, wi)) return 0;
+            float pdf_ss = pr_ss * ggx_pdf(b->mf, wo, wm) / (4.0f * vabsdot(wo, wm));
+            float pdf_ms = pr_ms * (abs_cos_theta(wi) * orc_inv_pi);
+            float fr_ss = R * ggx_D(b->mf, wm) * ggx_G(b->mf, wo, wi) / (4.0f * wi.z * wo.z);
+            float fr_ms = dms_fr_ms(b, wo, wi, ratio, eta_o);
+            s->f = S1(fr_ss + fr_ms);
+            s->wi = wi;
+            s->pdf = pdf_ms * (1.0f - E_o) + pdf_ss * E_o;
+            s->flags = BX_GLOSSY | BX_REFLECTION;
+            s->eta = 1.0f;
+            return 1;
+        }
+        v3 wi = V3(0.0f, 0.0f, 0.0f);
+        int tir = !refract(&wi, wo, wm, b->eta, NULL);
+        if (same_hemisphere(wo, wi) || wi.z == 0.0f || tir) return 0;
+        float eta_p = wo.z > 0.0f ? b->eta : (1.0f / b->eta);
+        float denom = sqrf(vdot(wi, wm) + vdot(wo, wm) / eta_p);
+        float dwm_dwi = vabsdot(wi, wm) / denom;
+        float pdf_ss = pt_ss * ggx_pdf(b->mf, wo, wm) * dwm_dwi;
+        float pdf_ms = pt_ms * (abs_cos_theta(wi) * orc_inv_pi);
+        float ft_ss = T * ggx_D(b->mf, wm) * ggx_G(b->mf, wo, wi) *
+                      fabsf(vdot(wi, wm) * vdot(wo, wm) / (wi.z * wo.z * denom));
+        float ft_ms = dms_ft_ms(b, wo, wi, ratio, eta_o, eta_i);
+        ft_ss /= sqrf(eta_p);
+        s->f = smulf(b->r, ft_ss + ft_ms);
+        s->wi = wi;
+        s->pdf = pdf_ms * (1.0f - E_o) + pdf_ss * E_o;
+        s->flags = BX_GLOSSY | BX_TRANSMISSION;
+        s->eta = eta_p;
+        return 1;
+    }
+    /* the multiple-scattering lobe: cosine-distributed reflection or transmission */
+    if (pr_ms == 0.0f && pt_ms == 0.0f) return 0;
+    float p_ms_sum = pr_ms + pt_ms;
+    pr_ms /= p_ms_sum;
+    pt_ms /= p_ms_sum;
+    u0 = (u0 - E_o) / (1.0f - E_o);
+    if (u0 < pr_ms) {
+        v3 wi = sample_cosine_hemisphere(u12);
+        if (wo.z < 0.0f) wi = vneg(wi);
+        v3 wm = vnormalize(vadd(wi, wo));
+        float R = fresnel_dielectric(vdot(wo, wm), b->eta);
+        float pr_ss = R, pt_ss = 1.0f - R;
+        float p_ss_sum = pr_ss + pt_ss;
+        pr_ss /= p_ss_sum;
+        pt_ss /= p_ss_sum;
+        float pdf_ss = pr_ss * ggx_pdf(b->mf, wo, wm) / (4.0f * vabsdot(wo, wm));
+        float pdf_ms = pr_ms * (abs_cos_theta(wi) * orc_inv_pi);
+        float fr_ss = R * ggx_D(b->mf, wm) * ggx_G(b->mf, wo, wi) / (4.0f * wi.z * wo.z);
+        float fr_ms = dms_fr_ms(b, wo, wi, ratio, eta_o);
+        s->f = S1(fr_ss + fr_ms);
+        s->wi = wi;
+        s->pdf = pdf_ms * (1.0f - E_o) + pdf_ss * E_o;
+        s->flags = BX_DIFFUSE | BX_REFLECTION;
+        s->eta = 1.0f;
+        return 1;
+    }
+    v3 wi = sample_cosine_hemisphere(u12);
+    if (wo.z > 0.0f) wi = vneg(wi);
+    v3 wm = vnormalize(vadd(vmul(wi, eta_o), wo));
+    if (dot_z(wm) < 0.0f) wm = vneg(wm);
+    float R = fresnel_dielectric(vdot(wo, wm), b->eta);
+    float T = 1.0f - R;
+    float pr_ss = R, pt_ss = 1.0f - R;
+    float p_ss_sum = pr_ss + pt_ss;
+    pr_ss /= p_ss_sum;
+    pt_ss /= p_ss_sum;
+    float denom = sqrf(vdot(wi, wm) + vdot(wo, wm) / eta_o);
+    float dwm_dwi = vabsdot(wi, wm) / denom;
+    float pdf_ss = pt_ss * ggx_pdf(b->mf, wo, wm) * dwm_dwi;
+    float pdf_ms = pt_ms * (abs_cos_theta(wi) * orc_inv_pi);
+    float ft_ss = T * ggx_D(b->mf, wm) * ggx_G(b->mf, wo, wi) *
+                  fabsf(vdot(wi, wm) * vdot(wo, wm) / (wi.z * wo.z * denom));
+    float ft_ms = dms_ft_ms(b, wo, wi, ratio, eta_o, eta_i);
+    ft_ss /= sqrf(eta_o);
+    s->f = smulf(b->r, ft_ss + ft_ms);
+    s->wi = wi;
+    s->pdf = pdf_ms * (1.0f - E_o) + pdf_ss * E_o;
+    s->flags = BX_DIFFUSE | BX_TRANSMISSION;
+    s->eta = eta_o;
     return 1;
 }
 
@@ -1627,11 +2016,14 @@ static int make_bsdf(const BbMaterial* m, v3 ns, v3 ts, int front_face, bsdf_t* 
         b->r = S3(m->color[0], m->color[1], m->color[2]);
         return 1;
     case BB_MAT_DIELECTRIC:
-        b->kind = K_DIELECTRIC;
+        /* DielectricMultiScatteringBxDF when energy_compensation (dielectric_material.cpp:36-42);
+         * effectively smooth, or with eta 1, it is the plain BxDF's specular branch
+         * (dielectric_multi_scattering_bxdf.cpp:11-14, 90-93, 196-244) */
         b->eta = front_face ? m->eta[0] : 1.0f / m->eta[0];
         b->r = ssqrt(S3(m->color[0], m->color[1], m->color[2]));
         b->mf.ax = sqrf(m->u_roughness);
         b->mf.ay = sqrf(m->v_roughness);
+        b->kind = (m->energy_compensation && !ggx_smooth(b->mf) && b->eta != 1.0f) ? K_DIELECTRIC_MS : K_DIELECTRIC;
         return 1;
     case BB_MAT_CONDUCTOR:
     case BB_MAT_CONDUCTOR_R:
@@ -1675,6 +2067,7 @@ static spec bsdf_f(const bsdf_t* s, v3 wo_w, v3 wi_w)
     switch (s->bx.kind) {
     case K_LAMBERT: return lambert_f(&s->bx, wo, wi);
     case K_DIELECTRIC: return dielectric_f(&s->bx, wo, wi);
+    case K_DIELECTRIC_MS: return dielectric_ms_f(&s->bx, wo, wi);
     case K_CONDUCTOR: return conductor_f(&s->bx, wo, wi);
     case K_CONDUCTOR_MS: return conductor_ms_f(&s->bx, wo, wi);
     default: return mr_f(&s->bx, wo, wi);
@@ -1690,6 +2083,7 @@ static float bsdf_pdf(const bsdf_t* s, v3 wo_w, v3 wi_w)
     switch (s->bx.kind) {
     case K_LAMBERT: return lambert_pdf(wo, wi);
     case K_DIELECTRIC: return dielectric_pdf(&s->bx, wo, wi);
+    case K_DIELECTRIC_MS: return dielectric_ms_pdf(&s->bx, wo, wi);
     case K_CONDUCTOR: return conductor_pdf(&s->bx, wo, wi);
     case K_CONDUCTOR_MS: return conductor_ms_pdf(&s->bx, wo, wi);
     default: return mr_pdf(&s->bx, wo, wi);
@@ -1704,7 +2098,8 @@ static int bsdf_sample(const bsdf_t* s, v3 wo_w, float u0, v2 u12, bsample* out)
     int ok;
     switch (s->bx.kind) {
     case K_LAMBERT: ok = lambert_sample(&s->bx, wo, u12, out); break;
-    case K_DIELECTRIC: ok = dielectric_sample(&s->bx, wo, u0, u12, out); break;
+    case K_DIELECTRIC: ok = dielectric_sample(&s->bx, wo, u0, u12, 1, out); break;
+    case K_DIELECTRIC_MS: ok = dielectric_ms_sample(&s->bx, wo, u0, u12, out); break;
     case K_CONDUCTOR: ok = conductor_sample(&s->bx, wo, u12, out); break;
     case K_CONDUCTOR_MS: ok = conductor_ms_sample(&s->bx, wo, u0, u12, out); break;
     default: ok = mr_sample(&s->bx, wo, u0, u12, out); break;
@@ -1726,15 +2121,7 @@ static int material_supported(const BbMaterial* m, char* err, int32_t err_size)
         return 1;
     case BB_MAT_CONDUCTOR:
     case BB_MAT_CONDUCTOR_R: return 1;
-    case BB_MAT_DIELECTRIC:
-        if (m->energy_compensation) {
-            int smooth = smax(sqrf(m->u_roughness), sqrf(m->v_roughness)) < 1e-3f;
-            if (!smooth) {
-                snprintf(err, err_size, "energy-compensated (multi-scattering) rough dielectric is not supported");
-                return 0;
-            }
-        }
-        return 1;
+    case BB_MAT_DIELECTRIC: return 1;
     case BB_MAT_METALLIC_ROUGHNESS: return 1;
     }
     snprintf(err, err_size, "unknown material type %d", m->type);

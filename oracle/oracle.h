@@ -45,6 +45,11 @@ int32_t orc_permutation_element(uint32_t i, uint32_t l, uint32_t p);
  * E[32 * 32] (alpha-major), E_avg[32], the 1024 u0 and 1024 (u1, u2) samples;
  * any pointer may be NULL */
 void orc_reflectance_tables(float* E, float* E_avg, float* u0, float* u12);
+/* the compensated dielectric's tables (DielectricMultiScatteringBxDF::
+ * PrepareReflectanceTexture, microfacet.cpp:173-236): E and E_inv[32^3] at
+ * [(alpha * 32 + cos_theta) * 32 + f0], E_avg and E_inv_avg[32^2] at
+ * [alpha * 32 + f0]; any pointer may be NULL */
+void orc_dielectric_tables(float* E, float* E_inv, float* E_avg, float* E_inv_avg);
 /* TrowbridgeReitzDistribution::E(wo) and E_avg() of a GGX with (ax, ay) */
 float orc_ggx_E(float ax, float ay, const float wo[3]);
 float orc_ggx_E_avg(float ax, float ay);

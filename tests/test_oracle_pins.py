@@ -375,29 +375,57 @@ def test_lambertian_closed_form(orc):
     assert np.all(f == 0) and pdf == 0
 
 
-@pytest.mark.parametrize("mat,pdf_consistent", [
-    (scenes.conductor(scenes.GOLD_ETA, scenes.GOLD_K, 0.3), True),
-    (scenes.conductor(scenes.GOLD_ETA, scenes.GOLD_K, 0.6), True),
-    (scenes.dielectric(1.5, roughness=0.4), True),
+# inside: also draw wo below the shading frame.  The plain DielectricBxDF
+# samples the VNDF cap of wo's own hemisphere (sampling.h:206-232) and uses
+# that normal unflipped (dielectric_bxdf.cpp:192-194), while PDF() flips the
+# half vector up (:104-108); from below the two pdfs differ by construction
+# in the reference.  The compensated variant flips the sampled normal
+# (dielectric_multi_scattering_bxdf.cpp:255-259) and stays consistent.
+# MetallicRoughnessBxDF's diffuse branch never flips wi below the frame
+# (metallic_roughness_bxdf.cpp:133-137), so its f() is 0 there.  The
+# compensated dielectric's Sample_f has no backfacing test on transmission:
+# its VNDF normal can face wo from below (then Refract bends with the
+# unflipped eta while eta' follows wo's side, :285-292), and its cosine-lobe
+# transmission can yield a generalized half vector that f() and PDF() reject
+# as backfacing (:44-47, :136-139); so it is compared for wo above the frame
+# and without the cosine-lobe transmission samples (flags 6).
+SKIP_NONE = ()
+SKIP_DIFFUSE_TRANSMISSION = (4 | 2,)
+
+
+@pytest.mark.parametrize("mat,pdf_consistent,inside,skip", [
+    (scenes.conductor(scenes.GOLD_ETA, scenes.GOLD_K, 0.3), True, True, SKIP_NONE),
+    (scenes.conductor(scenes.GOLD_ETA, scenes.GOLD_K, 0.6), True, True, SKIP_NONE),
+    (scenes.dielectric(1.5, roughness=0.4), True, False, SKIP_NONE),
     # MetallicRoughnessBxDF::Sample_f weighs its lobes with the VNDF-sampled
     # half vector even when it then samples the diffuse lobe
     # (metallic_roughness_bxdf.cpp:106-151), while PDF() uses the half vector
     # of (wo, wi): the two pdfs differ by design, so only f is compared.
-    (scenes.metallic_roughness((0.7, 0.5, 0.3), 0.0, 0.5), False),
-    (scenes.metallic_roughness((0.7, 0.5, 0.3), 1.0, 0.3), False),
-    (scenes.diffuse((0.5, 0.6, 0.7)), True),
-    (scenes.conductor(scenes.GOLD_ETA, scenes.GOLD_K, 0.5, energy_compensation=1), True),
-], ids=["gold0.3", "gold0.6", "roughglass", "mr_dielectric", "mr_metal", "lambert", "gold0.5_ms"])
-def test_sampled_pdf_and_value_match_evaluation(orc, mat, pdf_consistent):
+    (scenes.metallic_roughness((0.7, 0.5, 0.3), 0.0, 0.5), False, False, SKIP_NONE),
+    (scenes.metallic_roughness((0.7, 0.5, 0.3), 1.0, 0.3), False, False, SKIP_NONE),
+    (scenes.diffuse((0.5, 0.6, 0.7)), True, True, SKIP_NONE),
+    (scenes.conductor(scenes.GOLD_ETA, scenes.GOLD_K, 0.5, energy_compensation=1), True, True, SKIP_NONE),
+    (scenes.dielectric(1.5, roughness=0.5, energy_compensation=1), True, False, SKIP_DIFFUSE_TRANSMISSION),
+    (scenes.dielectric(1.33, roughness=0.85, color=(0.9, 0.7, 0.5), energy_compensation=1), True, False,
+     SKIP_DIFFUSE_TRANSMISSION),
+], ids=["gold0.3", "gold0.6", "roughglass", "mr_dielectric", "mr_metal", "lambert", "gold0.5_ms", "roughglass0.5_ms",
+        "water0.85_ms"])
+def test_sampled_pdf_and_value_match_evaluation(orc, mat, pdf_consistent, inside, skip):
+    """Sample_f's value and pdf equal f() and PDF() at the sampled direction,
+    on both faces and from both sides of the shading frame."""
     rng = np.random.default_rng(3)
     n_ok = 0
-    for _ in range(300):
-        wo = _dir(rng.uniform(0.05, 1.4), rng.uniform(0, 2 * np.pi))
-        ok, f, wi, pdf, fl, eta = _sample(orc, mat, wo, rng.random(), rng.random(2))
-        if not ok:
+    for it in range(300):
+        theta = rng.uniform(0.05, 1.4)
+        if inside and it % 3 == 2:
+            theta = np.pi - theta  # wo below the shading frame
+        front = 0 if it % 4 == 3 else 1
+        wo = _dir(theta, rng.uniform(0, 2 * np.pi))
+        ok, f, wi, pdf, fl, eta = _sample(orc, mat, wo, rng.random(), rng.random(2), front=front)
+        if not ok or fl in skip:
             continue
         n_ok += 1
-        f2, pdf2, _ = _eval(orc, mat, wo, wi)
+        f2, pdf2, _ = _eval(orc, mat, wo, wi, front=front)
         assert pdf > 0
         if pdf_consistent:
             assert np.isclose(pdf, pdf2, rtol=2e-4, atol=1e-7)
@@ -607,3 +635,166 @@ def test_compensated_conductor_conserves_energy(orc):
             albedo[ec] = acc * (np.pi / 2 / n) * (2 * np.pi / (2 * n)) * 4
         assert abs(albedo[1] - 1.0) < 0.03, (rough, albedo)
         assert albedo[0] < albedo[1] - (0.02 if rough > 0.5 else 0.0), (rough, albedo)
+
+
+# ------------------------------------------------ compensated rough dielectric
+
+def _dielectric_tables(orc):
+    n = 32
+    bufs = [(F * (n ** 3))(), (F * (n ** 3))(), (F * (n * n))(), (F * (n * n))()]
+    orc.orc_dielectric_tables(*bufs)
+    E, E_inv = (np.array(b, np.float32).reshape(n, n, n) for b in bufs[:2])  # [alpha][cos][f0]
+    A, A_inv = (np.array(b, np.float32).reshape(n, n) for b in bufs[2:])     # [alpha][f0]
+    return E, E_inv, A, A_inv
+
+
+def test_library_tables_equal_oracle_tables(orc):
+    """libbbgpu computes the reflectance tables on the host with its own code
+    and uploads them; they equal the oracle's bit for bit (the GPU renders
+    read them), in the layouts that bbgpu_reflectance_table documents."""
+    E, E_inv, A, A_inv = _dielectric_tables(orc)
+    ggx_E, ggx_avg, _, _ = _tables(orc)
+    pairs = [(capi.TABLE_GGX_E, ggx_E.astype(np.float32)), (capi.TABLE_GGX_E_AVG, ggx_avg.astype(np.float32)),
+             (capi.TABLE_DIELECTRIC_E, E), (capi.TABLE_DIELECTRIC_E_INV, E_inv),
+             (capi.TABLE_DIELECTRIC_E_AVG, A), (capi.TABLE_DIELECTRIC_E_INV_AVG, A_inv)]
+    for table, ref in pairs:
+        got = capi.reflectance_table(table)
+        assert got.size == ref.size, table
+        assert np.array_equal(got.view(np.int32), ref.reshape(-1).view(np.int32)), table
+
+
+def _fresnel64(c, eta):
+    """FresnelDielectric (scattering.h:51-77) in float64, c = cos_i >= 0"""
+    s2t = (1 - c * c) / (eta * eta)
+    if s2t >= 1:
+        return 1.0
+    ct = np.sqrt(1 - s2t)
+    rp = (eta * c - ct) / (eta * c + ct)
+    rs = (c - eta * ct) / (c + eta * ct)
+    return (rp * rp + rs * rs) / 2
+
+
+def _rough_dielectric_albedo_quadrature(eta, cos_o, alpha, n=700):
+    """Directional albedo of the rough dielectric interface (reflection plus
+    transmission, importance transport: no 1 / eta^2), by deterministic
+    quadrature over the microfacet normal in float64.  A VNDF sample of wm
+    has weight G(wo, wi) / G1(wo) whether it reflects (probability F) or
+    refracts (1 - F), so E(wo) = int (wo.wm) D(wm) / cos_o [F G(wo, wr) +
+    (1 - F) G(wo, wt)] dwm over visible normals, each term only where its
+    direction lies on its side."""
+    a2 = alpha * alpha
+    wo = np.array([np.sqrt(1 - cos_o * cos_o), 0.0, cos_o])
+    t = (np.arange(n) + 0.5) / n * (np.pi / 2)
+    ph = (np.arange(2 * n) + 0.5) / (2 * n) * (2 * np.pi)
+    T, P = np.meshgrid(t, ph, indexing="ij")
+    wm = np.stack([np.sin(T) * np.cos(P), np.sin(T) * np.sin(P), np.cos(T)], -1)
+    c2 = wm[..., 2] ** 2
+    D = 1.0 / (np.pi * a2 * c2 * c2 * (1 + (1 - c2) / c2 / a2) ** 2)
+
+    def lam(w):
+        c = np.clip(w[..., 2] ** 2, 1e-300, None)
+        return (np.sqrt(1 + a2 * (1 - c) / c) - 1) / 2
+
+    ci = wm @ wo  # wo . wm
+    vis = ci > 0
+    wr = -wo + 2 * ci[..., None] * wm
+    s2t = (1 - ci * ci) / (eta * eta)
+    tir = s2t >= 1
+    ct = np.sqrt(np.clip(1 - s2t, 0, None))
+    wt = -wo / eta + (ci / eta - ct)[..., None] * wm
+    Fr = np.where(tir, 1.0, 0.0)
+    ok = ~tir
+    rp = (eta * ci - ct) / (eta * ci + ct)
+    rs = (ci - eta * ct) / (ci + eta * ct)
+    Fr = np.where(ok, (rp * rp + rs * rs) / 2, Fr)
+    lo = lam(wo[None, None, :] + 0 * wm)
+    Gr = 1.0 / (1 + lo + lam(wr))
+    Gt = 1.0 / (1 + lo + lam(wt))
+    term = Fr * Gr * (wr[..., 2] > 0) + (1 - Fr) * Gt * (ok & (wt[..., 2] < 0))
+    dens = np.where(vis, ci * D / cos_o, 0.0)
+    dw = np.sin(T) * (np.pi / 2 / n) * (2 * np.pi / (2 * n))
+    return float(np.sum(dens * term * dw))
+
+
+def test_dielectric_tables_match_quadrature(orc):
+    """DielectricMultiScatteringBxDF::PrepareReflectanceTexture (microfacet.cpp:
+    173-236): E(f0_k, cos_i, alpha_j) is BxDF::rho of DielectricBxDF(ior, GGX(a, a))
+    in the ToCamera direction with ior = MapF0toIOR(f0) (scattering.h:188-208),
+    and E_inv the same with 1 / ior; each agrees with an independent float64
+    quadrature.  The nodes are asymmetric in (f0, cos, alpha), so a transposed
+    layout fails; E_avg = 2 sum E mu dmu over the cos axis."""
+    E, E_inv, A, A_inv = _dielectric_tables(orc)
+    d = 1.0 / 32
+    cen = d / 2 + d * np.arange(32)
+    for k, i, j in ((14, 5, 9), (4, 20, 26), (25, 28, 14), (20, 10, 3)):
+        f = cen[k]
+        ior = (1 + f * f) / (1 - f * f)
+        for table, eta in ((E, ior), (E_inv, 1 / ior)):
+            q = _rough_dielectric_albedo_quadrature(eta, cen[i], cen[j])
+            assert abs(table[j, i, k] - q) < 0.015, (k, i, j, eta, table[j, i, k], q)
+    for table, avg in ((E, A), (E_inv, A_inv)):
+        assert np.allclose(avg, 2 * np.einsum("jik,i->jk", table.astype(np.float64), cen * d), atol=2e-5)
+    assert np.all(E <= 1.0 + 1e-3) and np.all(E_inv <= 1.0 + 1e-3) and np.all(E_inv > 0.05)
+
+
+def _fresnel_average64(eta, n=20000):
+    mu = (np.arange(n) + 0.5) / n
+    return float(2 * np.sum([_fresnel64(m, eta) * m for m in mu]) / n)
+
+
+def test_compensated_dielectric_lobes(orc):
+    """DielectricMultiScatteringBxDF::f (dielectric_multi_scattering_bxdf.cpp:
+    9-89): f minus the plain DielectricBxDF's f is the compensation lobe.
+    (1) On wo's side it holds ratio (1 - E(wo)) of energy (quadrature over wi;
+    E at a table node), which measures ComputeScatteringRatio (bxdfs.h:
+    222-242) from outside and from inside.  (2) The two ratios satisfy the
+    balance that defines them, (1 - ratio_out) / (1 - Favg(eta)) +
+    (1 - ratio_in) / (1 - Favg(1 / eta)) = 1, with Favg = 2 int F mu dmu by
+    quadrature (the reference uses the Imageworks fit, scattering.h:154-172).
+    (3) Its transmission part obeys ft(o -> i) = eta^2 ft(i -> o) for wo
+    outside (radiance transport, no 1 / eta'^2 on this lobe): the tables and
+    the fit cancel, so this holds to rounding.  On the transmitted side f()
+    also drops directions whose generalized half vector is backfacing
+    (:44-47), so that side's energy is not (1 - ratio)(1 - E) in the
+    reference and is not checked."""
+    E, E_inv, _, _ = _dielectric_tables(orc)
+    d = 1.0 / 32
+    cen = d / 2 + d * np.arange(32)
+    k, i, j = 14, 18, 24
+    f0 = np.float32(cen[k])
+    ior = float((1 + f0 * f0) / (1 - f0 * f0))
+    rough = float(np.sqrt(cen[j]))
+    mats = {ec: scenes.dielectric(ior, roughness=rough, energy_compensation=ec) for ec in (0, 1)}
+
+    def lobe(wo, wi):
+        f1, _, _ = _eval(orc, mats[1], wo, wi)
+        f0_, _, _ = _eval(orc, mats[0], wo, wi)
+        return f1[0] - f0_[0]
+
+    n = 90
+    t = (np.arange(n) + 0.5) / n * (np.pi / 2)
+    ph = (np.arange(2 * n) + 0.5) / (2 * n) * (2 * np.pi)
+    dw = (np.pi / 2 / n) * (2 * np.pi / (2 * n))
+    ratios = {}
+    for side, table in ((1, E), (-1, E_inv)):
+        wo = (np.sqrt(1 - cen[i] ** 2), 0.0, side * cen[i])
+        acc = 0.0
+        for ti in t:
+            for pj in ph:
+                wi = _dir(ti, pj)
+                acc += lobe(wo, (wi[0], wi[1], side * wi[2])) * np.cos(ti) * np.sin(ti)
+        ratios[side] = acc * dw / (1.0 - float(table[j, i, k]))
+        assert 0.0 < ratios[side] < 1.0, ratios
+    balance = (1 - ratios[1]) / (1 - _fresnel_average64(ior)) + (1 - ratios[-1]) / (1 - _fresnel_average64(1 / ior))
+    assert abs(balance - 1.0) < 0.02, (ratios, balance)
+    rng = np.random.default_rng(17)
+    checked = 0
+    for _ in range(60):
+        wo = _dir(rng.uniform(0.1, 1.4), rng.uniform(0, 2 * np.pi))
+        wi = _dir(np.pi - rng.uniform(0.1, 1.4), rng.uniform(0, 2 * np.pi))
+        fo, fi = lobe(wo, wi), lobe(wi, wo)
+        if fo == 0.0 or fi == 0.0:  # a backfacing generalized half vector
+            continue
+        checked += 1
+        assert np.isclose(fo, ior * ior * fi, rtol=2e-3), (wo, wi, fo, fi)
+    assert checked > 20
