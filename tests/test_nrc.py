@@ -210,8 +210,10 @@ def test_cached_render_with_all_paths_training_equals_plain():
     p = scenes.render_params(4)
     s0, c0, _ = g.render(cam, p)
     net = Nrc(seed=3)
+    # one Adam step per pass: no path is ever queried, so training cannot
+    # change the image, and train_rows counts the rows it trained on
     s1, c1, st = g.render_cached(net, cam, p, scenes.cache_params(train_stride=1, train_extra_bounces=8,
-                                                                   train_steps=0))
+                                                                   train_steps=1))
     assert np.array_equal(c0, c1)
     assert np.array_equal(s0.view(np.int32), s1.view(np.int32))
     assert st.queries == 0 and st.train_rows > 0
