@@ -506,11 +506,14 @@ struct PathBufs
     int2* gstack;    // traversal stack overflow, [entry][persistent thread]
     int* err;
     unsigned long long* counters; // [6]: closest nodes/prims/accepts, shadow nodes/prims/accepts
+    int* sample_begin;            // the current call's sample_begin
 };
 
 struct RenderConsts
 {
-    int32_t n_paths, npix, width, seed, sample0, max_bounces, rr_min_bounces;
+    int32_t n_paths, npix, width, seed, max_bounces, rr_min_bounces;
+    int32_t sample0;              // the pass's first sample relative to the call's sample_begin
+    const int32_t* sample_begin;  // device copy of the call's sample_begin (set per call, read by k_raygen)
     int32_t train_stride; // cached rendering: every train_stride-th path is a training path (0: none)
 };
 
@@ -552,7 +555,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_raygen(GpuCamera cam, RenderCon
     }
     if (i >= rc.n_paths) return;
     const int pix = i % rc.npix;
-    const int sample = rc.sample0 + i / rc.npix;
+    const int sample = *rc.sample_begin + rc.sample0 + i / rc.npix;
     const int px = pix % rc.width, py = pix / rc.width;
     uint64_t st = pcg_seed(pixel_sample_hash(px, py, rc.seed, sample));
     f2 u1;
@@ -984,6 +987,11 @@ __global__ void k_zero_ints(int* p, int n)
     if (i < n) p[i] = 0;
 }
 
+__global__ void k_set_int(int* p, int v)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) *p = v;
+}
+
 __global__ void k_zero_u64(unsigned long long* p, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1009,7 +1017,20 @@ struct Workspace
     void* gstack = nullptr;
     size_t stage_bytes = 0; // device copy of caller buffers that live in host memory
     void* stage = nullptr;
+    uint64_t generation = 0; // bumped whenever a buffer above moves (captured graphs hold the pointers)
 };
+
+// A render captured as a HIP graph.  The key holds everything the launch
+// sequence depends on except sample_begin, which k_raygen reads from the
+// workspace: the camera, the parameters, the film pointers and the
+// workspace generation.
+struct GraphEntry
+{
+    std::vector<unsigned char> key;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+constexpr size_t kMaxGraphs = 8; // per scene, oldest dropped first
 
 struct CacheWork
 {
@@ -1025,7 +1046,17 @@ struct SceneExtra
     CacheWork cache;
     float* filter_cdf = nullptr; // device copy of the Gaussian filter's CDFs
     float filter_sigma = 0.0f, filter_extent = 0.0f;
+    std::vector<GraphEntry> graphs;
+    hipStream_t capture = nullptr; // stream the graphs are captured on (default priority)
 };
+
+void drop_graph(GraphEntry& g)
+{
+    if (g.exec) hipGraphExecDestroy(g.exec);
+    if (g.graph) hipGraphDestroy(g.graph);
+    g.exec = nullptr;
+    g.graph = nullptr;
+}
 
 std::vector<std::pair<BbScene*, SceneExtra*>> g_extras;
 std::mutex g_extras_lock;
@@ -1071,6 +1102,7 @@ bool ensure_stage(Workspace& w, size_t bytes)
     if (w.stage) hipFree(w.stage);
     w.stage = nullptr;
     w.stage_bytes = 0;
+    ++w.generation;
     if (hipMalloc(&w.stage, bytes) != hipSuccess) return false;
     w.stage_bytes = bytes;
     return true;
@@ -1100,6 +1132,7 @@ bool ensure_gstack(Workspace& w, size_t entries)
     if (w.gstack) hipFree(w.gstack);
     w.gstack = nullptr;
     w.gstack_entries = 0;
+    ++w.generation;
     if (hipMalloc(&w.gstack, entries * sizeof(int2)) != hipSuccess) return false;
     w.gstack_entries = entries;
     w.P.gstack = reinterpret_cast<int2*>(w.gstack);
@@ -1115,12 +1148,14 @@ bool ensure_workspace(Workspace& w, size_t paths, int depth_slots)
     w.allocs.clear();
     w.capacity = 0;
     w.max_depth_slots = 0;
+    ++w.generation;
     PathBufs& P = w.P;
     bool ok = dalloc(w, &P.L, paths) && dalloc(w, &P.beta, paths) && dalloc(w, &P.rng, paths) &&
               dalloc(w, &P.flags, paths) && dalloc(w, &P.queue[0], paths) && dalloc(w, &P.queue[1], paths) &&
               dalloc(w, &P.ray[0], 2 * paths) && dalloc(w, &P.ray[1], 2 * paths) && dalloc(w, &P.hit, paths) &&
               dalloc(w, &P.sray, 2 * paths) && dalloc(w, &P.scontrib, paths) &&
-              dalloc(w, &P.counts, size_t(4) * (depth_slots + 2)) && dalloc(w, &P.err, 4) && dalloc(w, &P.counters, 8);
+              dalloc(w, &P.counts, size_t(4) * (depth_slots + 2)) && dalloc(w, &P.err, 4) && dalloc(w, &P.counters, 8) &&
+              dalloc(w, &P.sample_begin, 1);
     if (!ok) {
         for (void* p : w.allocs) hipFree(p);
         w.allocs.clear();
@@ -1539,11 +1574,50 @@ int render_impl(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams
     hipEvent_t t_begin = nullptr;
     if (timed) t_begin = mark();
     hipEvent_t t_begin_all = (cache && cache->stats) ? mark() : nullptr;
-    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, st, P.err, 1);
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, P.counters, 8);
+    // the one value that differs between calls that share a captured graph
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, st, P.sample_begin, p.sample_begin);
+
+    // Graph mode: the launch sequence below has no host synchronization when
+    // neither stats nor the cache are on; it is captured once on the scene's
+    // capture stream and replayed on the caller's stream
+    const bool use_graph = p.graph != 0 && !timed && !cache;
+    GraphEntry* graph = nullptr;
+    bool capturing = false;
+    std::vector<unsigned char> key;
+    hipStream_t ls = st; // the stream the launch sequence goes to
+    if (use_graph) {
+        BbRenderParams kp = p;
+        kp.sample_begin = 0;
+        const void* films[2] = { film_sum_dev, film_count_dev };
+        const uint64_t gen = x->ws.generation;
+        auto put = [&](const void* d, size_t n) {
+            key.insert(key.end(), static_cast<const unsigned char*>(d), static_cast<const unsigned char*>(d) + n);
+        };
+        put(camera, sizeof(*camera));
+        put(&kp, sizeof(kp));
+        put(films, sizeof(films));
+        put(&gen, sizeof(gen));
+        for (GraphEntry& ge : x->graphs)
+            if (ge.key == key) graph = &ge;
+        if (!graph) {
+            if (!x->capture && hipStreamCreateWithFlags(&x->capture, hipStreamNonBlocking) != hipSuccess) {
+                x->capture = nullptr;
+                bb::set_error("bbgpu_render: capture stream creation failed");
+                return BBGPU_ERR_DEVICE;
+            }
+            HIP_OK(hipStreamBeginCapture(x->capture, hipStreamCaptureModeThreadLocal));
+            ls = x->capture;
+            capturing = true;
+        }
+    }
     const int nslots = 4 * (p.max_bounces + 2);
     int64_t paths_total = 0;
-    for (long long s0 = 0; s0 < p.spp; s0 += per_pass) {
+    const bool replay = graph != nullptr; // a captured graph stands in for the launches below
+    if (!replay) {
+        hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, ls, P.err, 1);
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ls, P.counters, 8);
+    }
+    for (long long s0 = 0; !replay && s0 < p.spp; s0 += per_pass) {
         const long long k = std::min<long long>(per_pass, p.spp - s0);
         const long long n_paths = k * npix;
         RenderConsts rc;
@@ -1551,19 +1625,20 @@ int render_impl(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams
         rc.npix = int32_t(npix);
         rc.width = camera->width;
         rc.seed = p.seed;
-        rc.sample0 = int32_t(p.sample_begin + s0);
+        rc.sample0 = int32_t(s0);
+        rc.sample_begin = P.sample_begin;
         rc.max_bounces = p.max_bounces;
         rc.rr_min_bounces = p.rr_min_bounces;
         rc.train_stride = stride;
         paths_total += n_paths;
         hipEvent_t a = timed ? mark() : nullptr;
-        hipLaunchKernelGGL(k_zero_ints, dim3((nslots + 255) / 256), dim3(256), 0, st, P.counts, nslots);
+        hipLaunchKernelGGL(k_zero_ints, dim3((nslots + 255) / 256), dim3(256), 0, ls, P.counts, nslots);
         const int n_train = cache ? int((n_paths + stride - 1) / stride) : 0;
         if (cache) {
-            hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, st, Cv.counters, 4);
-            hipLaunchKernelGGL(k_zero_ints, dim3((n_train + 255) / 256), dim3(256), 0, st, Cv.tcount, n_train);
+            hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, ls, Cv.counters, 4);
+            hipLaunchKernelGGL(k_zero_ints, dim3((n_train + 255) / 256), dim3(256), 0, ls, Cv.tcount, n_train);
         }
-        hipLaunchKernelGGL(k_raygen, dim3(unsigned((n_paths + kShadeBlock - 1) / kShadeBlock)), dim3(kShadeBlock), 0, st,
+        hipLaunchKernelGGL(k_raygen, dim3(unsigned((n_paths + kShadeBlock - 1) / kShadeBlock)), dim3(kShadeBlock), 0, ls,
                            cam, rc, P);
         if (timed) spans.push_back({ a, mark(), 2 });
         const int g_shade = grid_for(n_paths, kShadeBlock);
@@ -1571,18 +1646,18 @@ int render_impl(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams
             const int in = depth & 1;
             a = timed ? mark() : nullptr;
             const ClosestSrc csrc{ P.ray[in], P.hit };
-            hipLaunchKernelGGL(k_closest, dim3(g_trace), dim3(kTraceBlock), lds, st, V, P.counts + 4 * depth,
+            hipLaunchKernelGGL(k_closest, dim3(g_trace), dim3(kTraceBlock), lds, ls, V, P.counts + 4 * depth,
                                P.counts + 4 * depth + 2, csrc, P.gstack, lds_entries, refill_min, P.counters, P.err);
             hipEvent_t b = timed ? mark() : nullptr;
             if (timed) spans.push_back({ a, b, 0 });
             if (cache)
-                hipLaunchKernelGGL(k_shade<true>, dim3(g_shade), dim3(kShadeBlock), 0, st, V, rc, P, depth, Cv);
+                hipLaunchKernelGGL(k_shade<true>, dim3(g_shade), dim3(kShadeBlock), 0, ls, V, rc, P, depth, Cv);
             else
-                hipLaunchKernelGGL(k_shade<false>, dim3(g_shade), dim3(kShadeBlock), 0, st, V, rc, P, depth, Cv);
+                hipLaunchKernelGGL(k_shade<false>, dim3(g_shade), dim3(kShadeBlock), 0, ls, V, rc, P, depth, Cv);
             hipEvent_t c2 = timed ? mark() : nullptr;
             if (timed) spans.push_back({ b, c2, 2 });
             const ShadowSrc ssrc{ P.sray, P.scontrib, P.L };
-            hipLaunchKernelGGL(k_shadow, dim3(g_trace), dim3(kTraceBlock), lds, st, V, P.counts + 4 * depth + 1,
+            hipLaunchKernelGGL(k_shadow, dim3(g_trace), dim3(kTraceBlock), lds, ls, V, P.counts + 4 * depth + 1,
                                P.counts + 4 * depth + 3, ssrc, P.gstack, lds_entries, refill_shadow, P.counters, P.err);
             if (timed) spans.push_back({ c2, mark(), 1 });
         }
@@ -1595,7 +1670,7 @@ int render_impl(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams
             if (cnt[0] > 0) {
                 const int rq = bbgpu_nrc_query(cache->nrc, Cv.qraw, cnt[0], Cv.qpred, st);
                 if (rq != BBGPU_OK) return rq;
-                hipLaunchKernelGGL(k_cache_apply, dim3(grid_for(cnt[0], kShadeBlock)), dim3(kShadeBlock), 0, st, Cv, P.L);
+                hipLaunchKernelGGL(k_cache_apply, dim3(grid_for(cnt[0], kShadeBlock)), dim3(kShadeBlock), 0, ls, Cv, P.L);
             }
             hipLaunchKernelGGL(k_cache_targets, dim3((n_train + kShadeBlock - 1) / kShadeBlock), dim3(kShadeBlock), 0,
                                st, Cv, P.L, n_train, stride);
@@ -1630,10 +1705,10 @@ int render_impl(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams
             }
         }
         a = timed ? mark() : nullptr;
-        hipLaunchKernelGGL(k_film, dim3((npix + kShadeBlock - 1) / kShadeBlock), dim3(kShadeBlock), 0, st, rc, P.L,
+        hipLaunchKernelGGL(k_film, dim3((npix + kShadeBlock - 1) / kShadeBlock), dim3(kShadeBlock), 0, ls, rc, P.L,
                            film_sum_dev, film_count_dev);
         if (timed) spans.push_back({ a, mark(), 2 });
-        HIP_OK(hipGetLastError());
+        if (!capturing) HIP_OK(hipGetLastError());
         if (timed) {
             // ray counts of this pass (counts are reused by the next pass)
             std::vector<int> h(nslots);
@@ -1645,6 +1720,24 @@ int render_impl(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams
             }
         }
     }
+    if (capturing) {
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(x->capture, &g);
+        hipGraphExec_t e = nullptr;
+        if (ec != hipSuccess || !g || hipGetLastError() != hipSuccess ||
+            hipGraphInstantiate(&e, g, nullptr, nullptr, 0) != hipSuccess) {
+            if (g) hipGraphDestroy(g);
+            bb::set_error("bbgpu_render: graph capture failed");
+            return BBGPU_ERR_DEVICE;
+        }
+        if (x->graphs.size() >= kMaxGraphs) {
+            drop_graph(x->graphs.front());
+            x->graphs.erase(x->graphs.begin());
+        }
+        x->graphs.push_back(GraphEntry{ std::move(key), g, e });
+        graph = &x->graphs.back();
+    }
+    if (graph) HIP_OK(hipGraphLaunch(graph->exec, st));
     hipEvent_t t_end = (timed || (cache && cache->stats)) ? mark() : nullptr;
     if (host_film) {
         HIP_OK(hipMemcpyAsync(film_sum_user, film_sum_dev, sum_bytes, hipMemcpyDeviceToHost, st));
@@ -1904,6 +1997,8 @@ void release_scene_workspace(BbScene* s)
     std::lock_guard<std::mutex> g(g_extras_lock);
     for (size_t i = 0; i < g_extras.size(); ++i)
         if (g_extras[i].first == s) {
+            for (GraphEntry& ge : g_extras[i].second->graphs) drop_graph(ge);
+            if (g_extras[i].second->capture) hipStreamDestroy(g_extras[i].second->capture);
             free_workspace(g_extras[i].second->ws);
             if (g_extras[i].second->filter_cdf) hipFree(g_extras[i].second->filter_cdf);
             for (void* q : g_extras[i].second->cache.allocs) (void)hipFree(q);

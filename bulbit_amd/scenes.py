@@ -448,7 +448,7 @@ def icosphere_scene(width=1920, height=1080, level=8, seed=2026, n_lights=64):
 
 
 def render_params(spp, max_bounces=8, seed=0, rr_min_bounces=1, sample_begin=0, wavefront_paths=0, trace_lds_stack=0,
-                  trace_refill=0, shadow_refill=0):
+                  trace_refill=0, shadow_refill=0, graph=0):
     p = capi.BbRenderParams()
     p.spp = spp
     p.seed = seed
@@ -460,6 +460,7 @@ def render_params(spp, max_bounces=8, seed=0, rr_min_bounces=1, sample_begin=0, 
     p.trace_refill = trace_refill
     p.shadow_refill = shadow_refill
     p.sample_begin = sample_begin
+    p.graph = graph
     return p
 
 

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define BBGPU_ABI_VERSION 8
+#define BBGPU_ABI_VERSION 9
 
 /* status codes */
 enum {
@@ -223,6 +223,9 @@ typedef struct BbRenderParams {
     int32_t sample_begin;     /* render samples [sample_begin, sample_begin+spp) */
     int32_t trace_refill;     /* a closest-hit traversal wave takes new rays once this many of its lanes are idle (0 = 48) */
     int32_t shadow_refill;    /* the same for shadow rays (0 = trace_refill's default) */
+    int32_t graph;            /* 1: launch the render as a HIP graph, captured on first use and kept with the
+                                 scene for calls that differ only in sample_begin (speed only; not with stats
+                                 or the radiance cache) */
 } BbRenderParams;
 
 /* Per-render statistics, filled when a pointer is passed to bbgpu_render.
