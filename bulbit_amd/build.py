@@ -52,6 +52,7 @@ def hipcc() -> str:
 CPP = os.path.join(HERE, "cpp")
 CPP_LIB = os.path.join(HERE, "libbulbit.so")
 CPP_EXAMPLE = os.path.join(CPP, "_build", "readme_example")
+CPP_WRITE_IMAGES = os.path.join(CPP, "_build", "write_images")
 # host C++ API (bulbit/bulbit.h) over the C ABI: plain g++, same FP rules
 CXX_FLAGS = ["-std=c++17", "-O2", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-pthread"]
 
@@ -80,12 +81,15 @@ def build_cpp(verbose: bool = False) -> str:
     cxx = os.environ.get("CXX", "g++")
     inc = os.path.join(CPP, "include")
     tmp = CPP_LIB + ".tmp"
-    _run([cxx, *CXX_FLAGS, "-fPIC", "-shared", "-I", inc, os.path.join(CPP, "bulbit.cpp"), "-o", tmp,
+    _run([cxx, *CXX_FLAGS, "-fPIC", "-shared", "-I", inc, os.path.join(CPP, "bulbit.cpp"),
+          os.path.join(CPP, "image_io.cpp"), "-o", tmp,
           "-L", HERE, "-lbbgpu", "-Wl,-rpath,$ORIGIN"], "g++ building libbulbit.so", verbose)
     os.replace(tmp, CPP_LIB)
     os.makedirs(os.path.dirname(CPP_EXAMPLE), exist_ok=True)
     _run([cxx, *CXX_FLAGS, "-I", inc, os.path.join(CPP, "examples", "readme.cpp"), "-o", CPP_EXAMPLE,
           "-L", HERE, "-lbulbit", "-lbbgpu", "-Wl,-rpath,$ORIGIN/../.."], "g++ building the README example", verbose)
+    _run([cxx, *CXX_FLAGS, "-I", inc, os.path.join(CPP, "examples", "write_images.cpp"), "-o", CPP_WRITE_IMAGES,
+          "-L", HERE, "-lbulbit", "-lbbgpu", "-Wl,-rpath,$ORIGIN/../.."], "g++ building write_images", verbose)
     return CPP_LIB
 
 

@@ -494,51 +494,4 @@ Rendering* PathIntegrator::Render(Allocator& alloc, const Camera* camera)
     return r;
 }
 
-// ------------------------------------------------------------------ image
-
-bool WriteImage(const Image3& image, const std::string& filename)
-{
-    auto ends_with = [&](const char* ext) {
-        const size_t n = std::strlen(ext);
-        return filename.size() >= n && filename.compare(filename.size() - n, n, ext) == 0;
-    };
-    const int32 w = image.width, h = image.height;
-    if (ends_with(".pfm")) {
-        FILE* f = std::fopen(filename.c_str(), "wb");
-        if (!f) return false;
-        std::fprintf(f, "PF\n%d %d\n-1\n", w, h);
-        // PFM stores rows bottom to top, little-endian floats
-        for (int32 y = h - 1; y >= 0; --y)
-            for (int32 x = 0; x < w; ++x) {
-                const Spectrum& s = image(x, y);
-                const float v[3] = { s.r, s.g, s.b };
-                std::fwrite(v, sizeof(float), 3, f);
-            }
-        return std::fclose(f) == 0;
-    }
-    if (ends_with(".hdr")) {
-        FILE* f = std::fopen(filename.c_str(), "wb");
-        if (!f) return false;
-        std::fprintf(f, "#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y %d +X %d\n", h, w);
-        for (int32 y = 0; y < h; ++y)
-            for (int32 x = 0; x < w; ++x) {
-                const Spectrum& s = image(x, y);
-                const float m = std::max(s.r, std::max(s.g, s.b));
-                unsigned char rgbe[4] = { 0, 0, 0, 0 };
-                if (m > 1e-32f) {
-                    int e = 0;
-                    const float scale = std::frexp(m, &e) * 256.0f / m;
-                    rgbe[0] = (unsigned char)(std::max(0.0f, s.r) * scale);
-                    rgbe[1] = (unsigned char)(std::max(0.0f, s.g) * scale);
-                    rgbe[2] = (unsigned char)(std::max(0.0f, s.b) * scale);
-                    rgbe[3] = (unsigned char)(e + 128);
-                }
-                std::fwrite(rgbe, 1, 4, f);
-            }
-        return std::fclose(f) == 0;
-    }
-    std::fprintf(stderr, "bulbit_amd: WriteImage: %s: only .pfm and .hdr are written\n", filename.c_str());
-    return false;
-}
-
 } // namespace bulbit

@@ -779,9 +779,19 @@ public:
 using Image3 = Image<Spectrum>;
 using Image3f = Image3;
 
-// Writes linear radiance: .hdr (Radiance RGBE) or .pfm; false for any other
-// format or an I/O error.
-bool WriteImage(const Image3& image, const std::string& filename);
+// sRGB_from_RGB (include/bulbit/color.h:31-50): the sRGB transfer curve
+Vec3 sRGB_from_RGB(const Vec3& rgb);
+
+// Tone mapping applied when writing an 8-bit image (include/bulbit/image.h:89-99)
+using ToneMappingCallback = Vec3(const Vec3&);
+inline ToneMappingCallback* default_tonemapping_callback = [](const Vec3& rgb) -> Vec3 { return sRGB_from_RGB(rgb); };
+
+// WriteImage (src/util/image.cpp:219-266): .png and .jpg tone-mapped to 8
+// bits (callback, then clamp to [0, 1] and scale by 256), linear .hdr
+// (Radiance RGBE) and .pfm; false for any other extension or an I/O error.
+// The film's row 0 is the bottom of the picture.
+bool WriteImage(const Image3& image, const std::string& filename,
+                ToneMappingCallback* callback = default_tonemapping_callback);
 
 // --------------------------------------------------------- film, rendering
 

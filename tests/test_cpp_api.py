@@ -31,7 +31,8 @@ def read_pfm(path):
         w, h = map(int, f.readline().split())
         scale = float(f.readline())
         data = np.frombuffer(f.read(), dtype="<f4" if scale < 0 else ">f4")
-    return data.reshape(h, w, 3)[::-1]  # rows are stored bottom to top
+    # PFM rows run bottom to top, like the film's rows (row 0 is the bottom)
+    return data.reshape(h, w, 3)
 
 
 def test_cpp_api_library_and_example_are_built():
@@ -75,3 +76,15 @@ def test_readme_example_equals_c_abi_render_and_oracle(tmp_path):
     frac = pixel_agreement(img, os_ / oc[..., None])
     print(f"[cpp] README example vs oracle: {frac:.6f} within 1e-3", flush=True)
     assert frac >= 0.999
+
+    # the same render written as an sRGB PNG: top row of the picture first
+    from PIL import Image
+    from test_image_io import _srgb8
+
+    png_out = tmp_path / "readme.png"
+    r = subprocess.run([_example(), "640", "360", "16", str(png_out)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    png = np.asarray(Image.open(png_out).convert("RGB")).astype(int)
+    want = _srgb8(ref.astype(np.float32))[::-1].astype(int)
+    d = np.abs(png - want)
+    assert d.max() <= 1 and np.mean(d == 0) > 0.99
