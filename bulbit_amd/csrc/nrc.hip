@@ -47,6 +47,13 @@ constexpr int layer_n(int l) { return l == kHidden ? kOutCols : kWidth; }
 constexpr int layer_off(int l) { return l == 0 ? 0 : layer_off(l - 1) + layer_k(l - 1) * layer_n(l - 1); }
 constexpr int kParams = layer_off(kHidden + 1);
 
+// Layer inputs saved by the training forward pass, per chunk of kRows rows,
+// feature-major ([feature][row in chunk]): h_0 (kIn features), then
+// h_1..h_kHidden (kWidth each).  This is the operand layout the backward
+// MFMAs read (four consecutive rows of one feature in 8 bytes).
+constexpr int act_off(int l) { return l == 0 ? 0 : kRows * kIn + (l - 1) * kRows * kWidth; }
+constexpr int kChunkActs = act_off(kHidden + 1);
+
 typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 
@@ -149,8 +156,8 @@ __global__ __launch_bounds__(kNrcBlock) void k_mlp_forward(const float* __restri
             else
                 for (int i = 0; i < kIn; ++i) e[i] = 0;
             for (int i = 0; i < kIn; ++i) act[lane * kIn + i] = e[i];
-            if (SAVE && row < n)
-                for (int i = 0; i < kIn; ++i) acts[size_t(row) * kIn + i] = e[i];
+            if (SAVE) // every row of the chunk, zeros beyond n
+                for (int i = 0; i < kIn; ++i) acts[size_t(ch) * kChunkActs + act_off(0) + i * kRows + lane] = e[i];
         }
         // hidden layers
 #pragma unroll
@@ -163,15 +170,20 @@ __global__ __launch_bounds__(kNrcBlock) void k_mlp_forward(const float* __restri
 #pragma unroll
             for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
-                for (int ct = 0; ct < kWidth / 16; ++ct)
+                for (int ct = 0; ct < kWidth / 16; ++ct) {
+                    s4 hv;
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         const int r = rt * 16 + q * 4 + j, c = ct * 16 + r16;
                         const uint16_t h = to_bf16(fmaxf(acc[rt][ct][j], 0.0f));
                         act[r * kIn + c] = h;
-                        if (SAVE && row0 + r < n)
-                            acts[size_t(n) * kIn + (size_t(l) * n + size_t(row0 + r)) * kWidth + c] = h;
+                        hv[j] = short(h);
                     }
+                    // rows rt*16 + 4q .. +3 of feature ct*16 + r16 of h_{l+1}
+                    if (SAVE)
+                        *reinterpret_cast<s4*>(acts + size_t(ch) * kChunkActs + act_off(l + 1) +
+                                               (ct * 16 + r16) * kRows + rt * 16 + q * 4) = hv;
+                }
         }
         // output layer
         f4v acc[4][1];
@@ -198,65 +210,66 @@ BB_D float loss_grad(float y, float t, float inv_count, float& loss)
     return 2.0f * d / den * inv_count;
 }
 
-// Gradient of the loss for n rows whose layer inputs were saved by the
-// forward pass: dW_l = h_l^T dz_{l+1}, dh_l = dz_{l+1} W_l^T masked by
-// h_l > 0.  Each block accumulates its rows' dW in LDS and adds it to grad.
+// Gradient of the loss for n rows whose layer inputs the forward pass saved
+// (feature-major per chunk): dW_l = h_l^T dz_{l+1}, dh_l = dz_{l+1} W_l^T
+// masked by h_l > 0.  Per chunk of 64 rows a wave keeps dz in registers in
+// the MFMA result layout, which is exactly the B operand of dW = h^T dz (the
+// lane's own tile), and a row-major bf16 copy in LDS for the A operand of
+// dz W^T; h^T and the ReLU mask come from the saved layout with one 8-byte
+// load each.  Each block sums its waves' dW in LDS and stores the sum to
+// partials[block] (k_adam adds the blocks in order).
 __global__ __launch_bounds__(kNrcBlock) void k_mlp_backward(const float* __restrict__ y,
                                                              const float* __restrict__ target, int n,
                                                              NrcWeights Wt, const uint16_t* __restrict__ acts,
-                                                             float* __restrict__ grad, float* __restrict__ loss_sum)
+                                                             float* __restrict__ partials, float* __restrict__ loss_sum)
 {
     __shared__ float sG[kParams];
-    __shared__ uint16_t sD[kNrcBlock / 64][kRows * kWidth]; // dz of the current layer output
-    __shared__ uint16_t sH[kNrcBlock / 64][kRows * kIn];    // the layer's input h_l
+    __shared__ uint16_t sD[kNrcBlock / 64][kRows * kWidth]; // dz of the current layer output, row-major
     for (int i = threadIdx.x; i < kParams; i += blockDim.x) sG[i] = 0.0f;
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int r16 = lane & 15, q = lane >> 4;
-    uint16_t* dz = sD[wave];
-    uint16_t* hb = sH[wave];
+    uint16_t* dzl = sD[wave];
     const float inv_count = 1.0f / (3.0f * float(n));
     float loss = 0.0f;
     const int chunks = (n + kRows - 1) / kRows;
     for (int ch = blockIdx.x * (kNrcBlock / 64) + wave; ch < chunks; ch += gridDim.x * (kNrcBlock / 64)) {
         const int row0 = ch * kRows;
-        {
-            const int row = row0 + lane;
-            for (int c = 0; c < kOutCols; ++c) {
+        const uint16_t* hc = acts + size_t(ch) * kChunkActs;
+        // dz of the output: dL/dy of rows rt*16 + 4q + j, column r16 (3 used)
+        s4 dzb[4][kWidth / 16];
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int r = rt * 16 + q * 4 + j, row = row0 + r;
                 float g = 0.0f;
-                if (c < 3 && row < n) g = loss_grad(y[size_t(row) * 3 + c], target[size_t(row) * 3 + c], inv_count, loss);
-                dz[lane * kWidth + c] = to_bf16(g);
+                if (r16 < 3 && row < n) g = loss_grad(y[size_t(row) * 3 + r16], target[size_t(row) * 3 + r16], inv_count, loss);
+                const uint16_t gb = to_bf16(g);
+                dzb[rt][0][j] = short(gb);
+                dzl[r * kWidth + r16] = gb;
             }
         }
         // layers from the output down
 #pragma unroll
         for (int l = kHidden; l >= 0; --l) {
             const int K = layer_k(l), N = layer_n(l);
-            // h_l into LDS (rows beyond n are zero)
-            const uint16_t* src =
-                l == 0 ? acts : acts + size_t(n) * kIn + size_t(l - 1) * n * kWidth;
-            for (int i = lane; i < kRows * K; i += 64) {
-                const int r = i / K, c = i % K;
-                hb[r * kIn + c] = (row0 + r < n) ? src[size_t(row0 + r) * K + c] : uint16_t(0);
-            }
-            // dW_l += h_l^T dz: (K x rows) (rows x N), k-steps over the rows
-            for (int mt = 0; mt < K / 16; ++mt)
+            const uint16_t* h = hc + act_off(l);
+            // dW_l += h_l^T dz: A = h^T (features x rows), B = dz (rows x N)
+            for (int mt = 0; mt < K / 16; ++mt) {
+                s4 a[4];
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) a[kk] = load4(h + (mt * 16 + r16) * kRows + kk * 16 + q * 4);
                 for (int nt = 0; nt < N / 16; ++nt) {
                     f4v acc = f4v{ 0.0f, 0.0f, 0.0f, 0.0f };
 #pragma unroll
-                    for (int kk = 0; kk < kRows / 16; ++kk) {
-                        const s4 a = gather4(hb + (kk * 16 + q * 4) * kIn + mt * 16 + r16, kIn);
-                        const s4 b = gather4(dz + (kk * 16 + q * 4) * kWidth + nt * 16 + r16, kWidth);
-                        acc = mfma(a, b, acc);
-                    }
+                    for (int kk = 0; kk < 4; ++kk) acc = mfma(a[kk], dzb[kk][nt], acc);
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        const int m = mt * 16 + q * 4 + j, c = nt * 16 + r16;
-                        atomicAdd(&sG[layer_off(l) + m * N + c], acc[j]);
-                    }
+                    for (int j = 0; j < 4; ++j) atomicAdd(&sG[layer_off(l) + (mt * 16 + q * 4 + j) * N + nt * 16 + r16], acc[j]);
                 }
+            }
             if (l == 0) break;
-            // dh_l = dz W_l^T, masked by h_l > 0: (rows x N) (N x K)
+            // dh_l = dz W_l^T: (rows x N) (N x K), K = kWidth here
             f4v acc[4][kWidth / 16];
 #pragma unroll
             for (int rt = 0; rt < 4; ++rt)
@@ -269,36 +282,43 @@ __global__ __launch_bounds__(kNrcBlock) void k_mlp_backward(const float* __restr
                     b[ct] = load4(Wt.W + layer_off(l) + (ct * 16 + r16) * N + kk * 16 + q * 4);
 #pragma unroll
                 for (int rt = 0; rt < 4; ++rt) {
-                    const s4 a = load4(dz + (rt * 16 + r16) * kWidth + kk * 16 + q * 4);
+                    const s4 av = load4(dzl + (rt * 16 + r16) * kWidth + kk * 16 + q * 4);
 #pragma unroll
-                    for (int ct = 0; ct < kWidth / 16; ++ct) acc[rt][ct] = mfma(a, b[ct], acc[rt][ct]);
+                    for (int ct = 0; ct < kWidth / 16; ++ct) acc[rt][ct] = mfma(av, b[ct], acc[rt][ct]);
                 }
             }
+            // mask by h_l > 0 and keep dz_l in both forms
 #pragma unroll
             for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
-                for (int ct = 0; ct < kWidth / 16; ++ct)
+                for (int ct = 0; ct < kWidth / 16; ++ct) {
+                    const s4 hv = load4(h + (ct * 16 + r16) * kRows + rt * 16 + q * 4);
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
-                        const int r = rt * 16 + q * 4 + j, c = ct * 16 + r16;
-                        const bool on = hb[r * kIn + c] != 0 && !(hb[r * kIn + c] & 0x8000u);
-                        dz[r * kWidth + c] = on ? to_bf16(acc[rt][ct][j]) : uint16_t(0);
+                        const uint16_t hj = uint16_t(hv[j]);
+                        const bool on = hj != 0 && !(hj & 0x8000u);
+                        const uint16_t d = on ? to_bf16(acc[rt][ct][j]) : uint16_t(0);
+                        dzb[rt][ct][j] = short(d);
+                        dzl[(rt * 16 + q * 4 + j) * kWidth + ct * 16 + r16] = d;
                     }
+                }
         }
     }
     // wave loss -> global
     for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
     if (lane == 0) atomicAdd(loss_sum, loss);
     __syncthreads();
-    for (int i = threadIdx.x; i < kParams; i += blockDim.x)
-        if (sG[i] != 0.0f) atomicAdd(&grad[i], sG[i]);
+    float* out = partials + size_t(blockIdx.x) * kParams;
+    for (int i = threadIdx.x; i < kParams; i += blockDim.x) out[i] = sG[i];
 }
 
-// Adam (beta1 0.9, beta2 0.99, eps 1e-8) on the fp32 master weights, then
-// the bf16 copies in both layouts; the unused output columns stay zero
-__global__ void k_adam(float* __restrict__ w, float* __restrict__ m, float* __restrict__ v,
-                       const float* __restrict__ g, float lr, float c1, float c2, int apply, uint16_t* __restrict__ WT,
-                       uint16_t* __restrict__ W)
+// The gradient as the sum of the backward blocks' partial sums (in block
+// order), kept in g; then Adam (beta1 0.9, beta2 0.99, eps 1e-8) on the fp32
+// master weights when apply, and the bf16 copies in both layouts; the unused
+// output columns stay zero
+__global__ void k_adam(float* __restrict__ w, float* __restrict__ m, float* __restrict__ v, float* __restrict__ g,
+                       const float* __restrict__ partials, int n_partials, float lr, float c1, float c2, int apply,
+                       uint16_t* __restrict__ WT, uint16_t* __restrict__ W)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= kParams) return;
@@ -306,9 +326,11 @@ __global__ void k_adam(float* __restrict__ w, float* __restrict__ m, float* __re
     while (l < kHidden && i >= layer_off(l + 1)) ++l;
     const int K = layer_k(l), N = layer_n(l);
     const int k = (i - layer_off(l)) / N, nn = (i - layer_off(l)) % N;
+    float gi = 0.0f;
+    for (int b = 0; b < n_partials; ++b) gi += partials[size_t(b) * kParams + i];
+    if (n_partials > 0) g[i] = gi;
     float wi = w[i];
     if (apply && !(l == kHidden && nn >= 3)) {
-        const float gi = g[i];
         const float mi = 0.9f * m[i] + 0.1f * gi;
         const float vi = 0.99f * v[i] + 0.01f * gi * gi;
         m[i] = mi;
@@ -340,6 +362,7 @@ struct BbNrc
     float* m = nullptr;
     float* v = nullptr;
     float* g = nullptr;
+    float* partials = nullptr; // [backward block][kParams]
     float* loss = nullptr;
     uint16_t* WT = nullptr;
     uint16_t* W = nullptr;
@@ -363,11 +386,14 @@ int nrc_grid(int n)
     const int waves = (n + bb::kRows - 1) / bb::kRows;
     return std::max(1, std::min((waves + 3) / 4, 512));
 }
+static_assert(bb::kNrcBlock / 64 == 4, "nrc_grid assumes four waves per block");
 
-bool nrc_sync_bf16(BbNrc* x, hipStream_t st, bool apply, float c1, float c2)
+constexpr int kMaxBackwardBlocks = 512;
+
+bool nrc_sync_bf16(BbNrc* x, hipStream_t st, bool apply, float c1, float c2, int n_partials = 0)
 {
-    hipLaunchKernelGGL(bb::k_adam, dim3((bb::kParams + 255) / 256), dim3(256), 0, st, x->w, x->m, x->v, x->g, x->lr,
-                       c1, c2, apply ? 1 : 0, x->WT, x->W);
+    hipLaunchKernelGGL(bb::k_adam, dim3((bb::kParams + 255) / 256), dim3(256), 0, st, x->w, x->m, x->v, x->g,
+                       x->partials, n_partials, x->lr, c1, c2, apply ? 1 : 0, x->WT, x->W);
     return nrc_ok(hipGetLastError(), "k_adam");
 }
 
@@ -394,13 +420,14 @@ int bbgpu_nrc_create(const BbNrcDesc* desc, BbNrc** out)
         return BBGPU_ERR_DEVICE;
     }
     BbNrc* x = new BbNrc();
-    hipGetDevice(&x->device);
+    if (hipGetDevice(&x->device) != hipSuccess) x->device = 0;
     x->lr = desc->learning_rate > 0.0f ? desc->learning_rate : 1e-3f;
     const size_t P = bb::kParams;
     bool ok = nrc_ok(hipMalloc(&x->w, P * 4), "hipMalloc") && nrc_ok(hipMalloc(&x->m, P * 4), "hipMalloc") &&
               nrc_ok(hipMalloc(&x->v, P * 4), "hipMalloc") && nrc_ok(hipMalloc(&x->g, P * 4), "hipMalloc") &&
               nrc_ok(hipMalloc(&x->loss, 4), "hipMalloc") && nrc_ok(hipMalloc(&x->WT, P * 2), "hipMalloc") &&
-              nrc_ok(hipMalloc(&x->W, P * 2), "hipMalloc");
+              nrc_ok(hipMalloc(&x->W, P * 2), "hipMalloc") &&
+              nrc_ok(hipMalloc(&x->partials, size_t(kMaxBackwardBlocks) * P * 4), "hipMalloc");
     if (!ok) {
         bbgpu_nrc_destroy(x);
         return BBGPU_ERR_NOMEM;
@@ -434,8 +461,8 @@ int bbgpu_nrc_create(const BbNrcDesc* desc, BbNrc** out)
 int bbgpu_nrc_destroy(BbNrc* x)
 {
     if (!x) return BBGPU_OK;
-    for (void* p : { (void*)x->w, (void*)x->m, (void*)x->v, (void*)x->g, (void*)x->loss, (void*)x->WT, (void*)x->W,
-                     (void*)x->acts, (void*)x->y })
+    for (void* p : { (void*)x->w, (void*)x->m, (void*)x->v, (void*)x->g, (void*)x->partials, (void*)x->loss,
+                     (void*)x->WT, (void*)x->W, (void*)x->acts, (void*)x->y })
         if (p) (void)hipFree(p);
     delete x;
     return BBGPU_OK;
@@ -485,7 +512,8 @@ int bbgpu_nrc_train(BbNrc* x, const float* inputs_dev, const float* targets_dev,
         return BBGPU_ERR_INVALID;
     }
     hipStream_t st = (hipStream_t)stream;
-    const size_t need = size_t(n) * (bb::kIn + bb::kHidden * bb::kWidth);
+    const size_t chunks = (size_t(n) + bb::kRows - 1) / bb::kRows;
+    const size_t need = chunks * size_t(bb::kChunkActs);
     if (x->acts_rows < size_t(n)) {
         if (x->acts) (void)hipFree(x->acts);
         if (x->y) (void)hipFree(x->y);
@@ -497,19 +525,21 @@ int bbgpu_nrc_train(BbNrc* x, const float* inputs_dev, const float* targets_dev,
         x->acts_rows = size_t(n);
     }
     const bb::NrcWeights Wt{ x->WT, x->W };
-    hipLaunchKernelGGL(bb::k_fill0, dim3((bb::kParams + 255) / 256), dim3(256), 0, st, x->g, bb::kParams);
+    const int grid = nrc_grid(n);
     hipLaunchKernelGGL(bb::k_fill0, dim3(1), dim3(64), 0, st, x->loss, 1);
-    hipLaunchKernelGGL(bb::k_mlp_forward<true>, dim3(nrc_grid(n)), dim3(bb::kNrcBlock), 0, st, inputs_dev, n, Wt, x->y,
+    hipLaunchKernelGGL(bb::k_mlp_forward<true>, dim3(grid), dim3(bb::kNrcBlock), 0, st, inputs_dev, n, Wt, x->y,
                        x->acts);
-    hipLaunchKernelGGL(bb::k_mlp_backward, dim3(nrc_grid(n)), dim3(bb::kNrcBlock), 0, st, x->y, targets_dev, n, Wt,
-                       x->acts, x->g, x->loss);
+    hipLaunchKernelGGL(bb::k_mlp_backward, dim3(grid), dim3(bb::kNrcBlock), 0, st, x->y, targets_dev, n, Wt, x->acts,
+                       x->partials, x->loss);
     if (!nrc_ok(hipGetLastError(), "k_mlp_backward")) return BBGPU_ERR_DEVICE;
+    // the gradient (partial sums of the grid's blocks), and the step when apply
+    float c1 = 1.0f, c2 = 1.0f;
     if (apply) {
         x->step++;
-        const float c1 = 1.0f - std::pow(0.9f, float(x->step));
-        const float c2 = 1.0f - std::pow(0.99f, float(x->step));
-        if (!nrc_sync_bf16(x, st, true, c1, c2)) return BBGPU_ERR_DEVICE;
+        c1 = 1.0f - std::pow(0.9f, float(x->step));
+        c2 = 1.0f - std::pow(0.99f, float(x->step));
     }
+    if (!nrc_sync_bf16(x, st, apply != 0, c1, c2, grid)) return BBGPU_ERR_DEVICE;
     if (loss_out && !nrc_ok(hipMemcpyAsync(loss_out, x->loss, 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync"))
         return BBGPU_ERR_DEVICE;
     if (grad_out &&
