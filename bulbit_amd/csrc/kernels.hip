@@ -26,6 +26,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 namespace bb
@@ -184,22 +185,29 @@ BB_D void lane_root(const SceneView& S, Lane& L, TraceCount& c)
     L.cur = box_pass(h, L.t_max) ? S.root_ref : kNoRef;
 }
 
-// Traversal stack of (node reference, entry distance) pairs: entries
-// [0, lds_entries) in LDS ([entry][thread] layout, conflict-free for a
-// wave), deeper entries in a global overflow area ([entry][global thread]);
-// cap = BVH depth + 2 bounds every push.
+// Traversal stack: entries [0, lds_entries) in LDS ([entry][thread] layout,
+// conflict-free for a wave), deeper entries in a global overflow area
+// ([entry][global thread]); cap = BVH depth + 2 bounds every push.  An entry
+// is a (node reference, entry distance) pair (int2), or the reference alone
+// (int) for any-hit queries without counters, whose t_max never changes
+// while they run, so the pop-time `lo > t_max` check cannot fail.
+template <typename E>
 struct TraceStack
 {
-    int2* lds;
-    int2* gbl;
+    E* lds;
+    E* gbl;
     int32_t lds_entries;
     int32_t gstride;
     int32_t cap;
 };
 
+// stack entries of a query kind
+template <bool ANY, bool COUNT>
+using StackEntry = typename std::conditional<ANY && !COUNT, int, int2>::type;
+
 // FULL: the whole stack fits in LDS (cap <= lds_entries), no overflow branch
-template <bool FULL>
-BB_D void stack_put(const TraceStack& s, int32_t sp, int2 v)
+template <bool FULL, typename E>
+BB_D void stack_put(const TraceStack<E>& s, int32_t sp, E v)
 {
     if (FULL || sp < s.lds_entries)
         s.lds[sp * kTraceBlock] = v;
@@ -207,8 +215,8 @@ BB_D void stack_put(const TraceStack& s, int32_t sp, int2 v)
         s.gbl[(sp - s.lds_entries) * s.gstride] = v;
 }
 
-template <bool FULL>
-BB_D int2 stack_get(const TraceStack& s, int32_t sp)
+template <bool FULL, typename E>
+BB_D E stack_get(const TraceStack<E>& s, int32_t sp)
 {
     return (FULL || sp < s.lds_entries) ? s.lds[sp * kTraceBlock] : s.gbl[(sp - s.lds_entries) * s.gstride];
 }
@@ -233,8 +241,10 @@ constexpr int32_t kFailedLo = 0x7fbfffff;
 // pushed and counted when popped, which reproduces RayCast's node-test count.
 // Returns true when the lane's query is finished.
 template <bool ANY, bool COUNT, bool FULL>
-BB_D bool trace_step(const SceneView& S, Lane& L, const TraceStack& stk, TraceCount& c, int* err)
+BB_D bool trace_step(const SceneView& S, Lane& L, const TraceStack<StackEntry<ANY, COUNT>>& stk, TraceCount& c,
+                     int* err)
 {
+    constexpr bool kSlim = std::is_same<StackEntry<ANY, COUNT>, int>::value;
     if (L.cur >= 0) {
         const float4* r = S.inner + 4 * L.cur;
         const float4 a = r[0], b = r[1], e = r[2], m = r[3];
@@ -255,7 +265,10 @@ BB_D bool trace_step(const SceneView& S, Lane& L, const TraceStack& stk, TraceCo
                 atomicOr(err, 1);
                 return true;
             }
-            stack_put<FULL>(stk, L.sp, make_int2(far_ref, hf.ok ? __float_as_int(hf.lo) : kFailedLo));
+            if constexpr (kSlim)
+                stack_put<FULL>(stk, L.sp, far_ref);
+            else
+                stack_put<FULL>(stk, L.sp, make_int2(far_ref, hf.ok ? __float_as_int(hf.lo) : kFailedLo));
             ++L.sp;
         }
         if (COUNT) c.nodes++;
@@ -299,10 +312,14 @@ BB_D bool trace_step(const SceneView& S, Lane& L, const TraceStack& stk, TraceCo
     // whether it is expanded
     if (L.sp == 0) return true;
     --L.sp;
-    const int2 e = stack_get<FULL>(stk, L.sp);
-    if (COUNT) c.nodes++;
-    const bool pass = (!COUNT || e.y != kFailedLo) && !(__int_as_float(e.y) > L.t_max);
-    L.cur = pass ? e.x : kNoRef;
+    if constexpr (kSlim) {
+        L.cur = stack_get<FULL>(stk, L.sp); // pushed only if it passed with this same t_max
+    } else {
+        const int2 e = stack_get<FULL>(stk, L.sp);
+        if (COUNT) c.nodes++;
+        const bool pass = (!COUNT || e.y != kFailedLo) && !(__int_as_float(e.y) > L.t_max);
+        L.cur = pass ? e.x : kNoRef;
+    }
     return false;
 }
 
@@ -312,8 +329,8 @@ BB_D bool trace_step(const SceneView& S, Lane& L, const TraceStack& stk, TraceCo
 // per 64 queries).  Between refill checks every lane takes STEPS traversal
 // steps.  Src provides load(index, Lane&) and finish(index, const Lane&).
 template <bool ANY, bool COUNT, bool FULL, int STEPS, typename Src>
-BB_D void persistent_trace(const SceneView& S, const Src& src, int n, int* fetch, const TraceStack& stk, TraceCount& c,
-                           int* err, int refill_min)
+BB_D void persistent_trace(const SceneView& S, const Src& src, int n, int* fetch,
+                           const TraceStack<StackEntry<ANY, COUNT>>& stk, TraceCount& c, int* err, int refill_min)
 {
     const int lane = __lane_id();
     const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
@@ -405,12 +422,15 @@ struct BatchSrc
     }
 };
 
-BB_D TraceStack make_stack(int2* lds_stack, int2* gstack, int lds_entries, int cap)
+// the calling thread's stack in the block's dynamic LDS and in the overflow
+// area (both hold entries of type E)
+template <typename E>
+BB_D TraceStack<E> make_stack(void* lds_stack, int2* gstack, int lds_entries, int cap)
 {
-    TraceStack t;
-    t.lds = lds_stack + threadIdx.x;
+    TraceStack<E> t;
+    t.lds = reinterpret_cast<E*>(lds_stack) + threadIdx.x;
     t.gstride = gridDim.x * blockDim.x;
-    t.gbl = gstack + blockIdx.x * blockDim.x + threadIdx.x;
+    t.gbl = reinterpret_cast<E*>(gstack) + blockIdx.x * blockDim.x + threadIdx.x;
     t.lds_entries = lds_entries;
     t.cap = cap;
     return t;
@@ -431,7 +451,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_batch(SceneView S, BatchS
                                                                unsigned long long* counters, int* err)
 {
     extern __shared__ int2 lds_stack[];
-    const TraceStack stk = make_stack(lds_stack, gstack, lds_entries, S.stack_depth);
+    const auto stk = make_stack<StackEntry<ANY, COUNT>>(lds_stack, gstack, lds_entries, S.stack_depth);
     TraceCount c{ 0, 0, 0 };
     persistent_trace<ANY, COUNT, FULL, kTraceSteps>(S, src, n, fetch, stk, c, err, refill_min);
     if (COUNT) wave_count_add(c, counters);
@@ -629,7 +649,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(SceneView S, cons
                                                                  unsigned long long* counters, int* err)
 {
     extern __shared__ int2 lds_stack[];
-    const TraceStack stk = make_stack(lds_stack, gstack, lds_entries, S.stack_depth);
+    const auto stk = make_stack<StackEntry<false, COUNT>>(lds_stack, gstack, lds_entries, S.stack_depth);
     TraceCount c{ 0, 0, 0 };
     persistent_trace<false, COUNT, FULL, kTraceSteps>(S, src, *n_ptr, fetch, stk, c, err, refill_min);
     if (COUNT) wave_count_add(c, counters);
@@ -641,7 +661,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_shadow(SceneView S, const
                                                                 unsigned long long* counters, int* err)
 {
     extern __shared__ int2 lds_stack[];
-    const TraceStack stk = make_stack(lds_stack, gstack, lds_entries, S.stack_depth);
+    const auto stk = make_stack<StackEntry<true, COUNT>>(lds_stack, gstack, lds_entries, S.stack_depth);
     TraceCount c{ 0, 0, 0 };
     persistent_trace<true, COUNT, FULL, kShadowSteps>(S, src, *n_ptr, fetch, stk, c, err, refill_min);
     if (COUNT) wave_count_add(c, counters + 3);
@@ -1273,7 +1293,12 @@ int lds_entries_for(const SceneView& V, int requested)
     return std::max(1, std::min(e, V.stack_depth));
 }
 
-size_t trace_lds_bytes(int lds_entries) { return size_t(lds_entries) * kTraceBlock * sizeof(int2); }
+// dynamic LDS of a traversal kernel: 4-byte entries for any-hit queries
+// without counters, 8-byte ones otherwise
+size_t trace_lds_bytes(int lds_entries, bool slim = false)
+{
+    return size_t(lds_entries) * kTraceBlock * (slim ? sizeof(int) : sizeof(int2));
+}
 
 // persistent grid of a traversal kernel: resident blocks per CU x CUs
 int trace_grid(const void* kernel, size_t lds_bytes)
@@ -1334,7 +1359,7 @@ int bbgpu_trace(BbScene* scene, const BbRay* rays_dev, int32_t n_rays, int32_t a
     SceneExtra* x = extra_for(scene);
     const SceneView& V = scene->view;
     const int lds_entries = lds_entries_for(V, 0);
-    const size_t lds = trace_lds_bytes(lds_entries);
+    const size_t lds = trace_lds_bytes(lds_entries, any_hit != 0 && counters == nullptr);
     const bool full = V.stack_depth <= lds_entries;
     const BatchKernel kfn = batch_kernel(any_hit != 0, counters != nullptr, full);
     const int grid = std::min(trace_grid(reinterpret_cast<const void*>(kfn), lds), grid_for(std::max(n_rays, 1), kTraceBlock));
@@ -1495,15 +1520,18 @@ int render_impl(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams
     const bool count = timed && count_traversal;
     const int lds_entries = lds_entries_for(V, p.trace_lds_stack);
     const size_t lds = trace_lds_bytes(lds_entries);
+    const size_t lds_shadow = trace_lds_bytes(lds_entries, !count); // reference-only entries without counters
     const int refill_min = p.trace_refill > 0 ? std::min(p.trace_refill, 64) : 48;
     const int refill_shadow = p.shadow_refill > 0 ? std::min(p.shadow_refill, 64) : kShadowRefillDefault;
     const bool full = V.stack_depth <= lds_entries;
     const ClosestKernel k_closest = closest_kernel(count, full);
     const ShadowKernel k_shadow = shadow_kernel(count, full);
-    const int g_trace = std::min(std::max(trace_grid(reinterpret_cast<const void*>(k_closest), lds),
-                                          trace_grid(reinterpret_cast<const void*>(k_shadow), lds)),
-                                 grid_for(cap, kTraceBlock));
-    if (!ensure_gstack(x->ws, size_t(std::max(0, V.stack_depth - lds_entries)) * size_t(g_trace) * kTraceBlock)) {
+    const int g_trace = std::min(trace_grid(reinterpret_cast<const void*>(k_closest), lds), grid_for(cap, kTraceBlock));
+    const int g_shadow =
+        std::min(trace_grid(reinterpret_cast<const void*>(k_shadow), lds_shadow), grid_for(cap, kTraceBlock));
+    // the overflow area serves both grids (int2-sized entries hold the 4-byte ones too)
+    if (!ensure_gstack(x->ws, size_t(std::max(0, V.stack_depth - lds_entries)) * size_t(std::max(g_trace, g_shadow)) *
+                                  kTraceBlock)) {
         bb::set_error("bbgpu_render: traversal stack allocation failed");
         return BBGPU_ERR_NOMEM;
     }
@@ -1657,7 +1685,7 @@ int render_impl(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams
             hipEvent_t c2 = timed ? mark() : nullptr;
             if (timed) spans.push_back({ b, c2, 2 });
             const ShadowSrc ssrc{ P.sray, P.scontrib, P.L };
-            hipLaunchKernelGGL(k_shadow, dim3(g_trace), dim3(kTraceBlock), lds, ls, V, P.counts + 4 * depth + 1,
+            hipLaunchKernelGGL(k_shadow, dim3(g_shadow), dim3(kTraceBlock), lds_shadow, ls, V, P.counts + 4 * depth + 1,
                                P.counts + 4 * depth + 3, ssrc, P.gstack, lds_entries, refill_shadow, P.counters, P.err);
             if (timed) spans.push_back({ c2, mark(), 1 });
         }
