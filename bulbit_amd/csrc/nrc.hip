@@ -138,9 +138,11 @@ template <bool SAVE>
 __global__ __launch_bounds__(kNrcBlock) void k_mlp_forward(const float* __restrict__ raw, int n, NrcWeights Wt,
                                                             float* __restrict__ out, uint16_t* __restrict__ acts)
 {
-    __shared__ uint16_t sW[kParams];
+    __shared__ __align__(16) uint16_t sW[kParams];
     __shared__ uint16_t sAct[kNrcBlock / 64][kRows * kIn];
-    for (int i = threadIdx.x; i < kParams; i += blockDim.x) sW[i] = Wt.WT[i];
+    static_assert(kParams % 8 == 0, "weights staged 16 bytes at a time");
+    for (int i = threadIdx.x; i < kParams / 8; i += blockDim.x)
+        reinterpret_cast<uint4*>(sW)[i] = reinterpret_cast<const uint4*>(Wt.WT)[i];
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     uint16_t* act = sAct[wave];
@@ -149,18 +151,20 @@ __global__ __launch_bounds__(kNrcBlock) void k_mlp_forward(const float* __restri
     for (int ch = blockIdx.x * (kNrcBlock / 64) + wave; ch < chunks; ch += gridDim.x * (kNrcBlock / 64)) {
         const int row0 = ch * kRows;
         {
+            // each lane encodes its row straight into the wave's LDS rows
             const int row = row0 + lane;
-            uint16_t e[kIn];
+            uint16_t* e = act + lane * kIn;
             if (row < n)
                 encode_row(raw + size_t(row) * kRawStride, e);
             else
                 for (int i = 0; i < kIn; ++i) e[i] = 0;
-            for (int i = 0; i < kIn; ++i) act[lane * kIn + i] = e[i];
             if (SAVE) // every row of the chunk, zeros beyond n
                 for (int i = 0; i < kIn; ++i) acts[size_t(ch) * kChunkActs + act_off(0) + i * kRows + lane] = e[i];
         }
-        // hidden layers
-#pragma unroll
+        // hidden layers: layer 0 (kIn inputs), then the kWidth-input ones in a
+        // rolled loop (one copy of the code: small kernels stay in the
+        // instruction cache when every wave runs a single chunk)
+#pragma unroll 1
         for (int l = 0; l < kHidden; ++l) {
             f4v acc[4][kWidth / 16];
             if (l == 0)
@@ -210,6 +214,80 @@ BB_D float loss_grad(float y, float t, float inv_count, float& loss)
     return 2.0f * d / den * inv_count;
 }
 
+// One layer of the backward pass for a wave's chunk: dW_l += h_l^T dz (K x N)
+// with dz in registers (dzb, the MFMA result layout) and row-major in dzl;
+// with DH, dz becomes dh_l = dz W_l^T masked by h_l > 0 (kWidth columns).
+template <int K, int N, bool DH>
+BB_D void bwd_layer(int l, const uint16_t* hc, const NrcWeights& Wt, float* sG, uint16_t* dzl,
+                    s4 (&dzb)[4][kWidth / 16], int lane)
+{
+    const int r16 = lane & 15, q = lane >> 4;
+    const uint16_t* h = hc + act_off(l);
+    float* g = sG + layer_off(l);
+    // every global operand of the layer first, so that the wave (alone on
+    // its SIMD) waits for memory once per layer: h^T tiles, the ReLU mask
+    // (h_l itself) and W_l
+    s4 a[K / 16][4];
+#pragma unroll
+    for (int mt = 0; mt < K / 16; ++mt)
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) a[mt][kk] = load4(h + (mt * 16 + r16) * kRows + kk * 16 + q * 4);
+    s4 hv[DH ? 4 : 1][kWidth / 16];
+    s4 b[DH ? N / 16 : 1][kWidth / 16];
+    if constexpr (DH) {
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+            for (int ct = 0; ct < kWidth / 16; ++ct) hv[rt][ct] = load4(h + (ct * 16 + r16) * kRows + rt * 16 + q * 4);
+#pragma unroll
+        for (int kk = 0; kk < N / 16; ++kk)
+#pragma unroll
+            for (int ct = 0; ct < kWidth / 16; ++ct)
+                b[kk][ct] = load4(Wt.W + layer_off(l) + (ct * 16 + r16) * N + kk * 16 + q * 4);
+    }
+    // dW_l += h_l^T dz: A = h^T (features x rows), B = dz (rows x N)
+#pragma unroll
+    for (int mt = 0; mt < K / 16; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < N / 16; ++nt) {
+            f4v acc = f4v{ 0.0f, 0.0f, 0.0f, 0.0f };
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) acc = mfma(a[mt][kk], dzb[kk][nt], acc);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) atomicAdd(&g[(mt * 16 + q * 4 + j) * N + nt * 16 + r16], acc[j]);
+        }
+    if constexpr (!DH) return;
+    // dh_l = dz W_l^T: (rows x N) (N x kWidth)
+    f4v acc[4][kWidth / 16];
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < kWidth / 16; ++ct) acc[rt][ct] = f4v{ 0.0f, 0.0f, 0.0f, 0.0f };
+#pragma unroll
+    for (int kk = 0; kk < N / 16; ++kk) {
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt) {
+            const s4 av = load4(dzl + (rt * 16 + r16) * kWidth + kk * 16 + q * 4);
+#pragma unroll
+            for (int ct = 0; ct < kWidth / 16; ++ct) acc[rt][ct] = mfma(av, b[kk][ct], acc[rt][ct]);
+        }
+    }
+    // mask by h_l > 0 and keep dz_l in both forms
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < kWidth / 16; ++ct) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint16_t hj = uint16_t(hv[rt][ct][j]);
+                const bool on = hj != 0 && !(hj & 0x8000u);
+                const uint16_t d = on ? to_bf16(acc[rt][ct][j]) : uint16_t(0);
+                dzb[rt][ct][j] = short(d);
+                dzl[(rt * 16 + q * 4 + j) * kWidth + ct * 16 + r16] = d;
+            }
+        }
+}
+
 // Gradient of the loss for n rows whose layer inputs the forward pass saved
 // (feature-major per chunk): dW_l = h_l^T dz_{l+1}, dh_l = dz_{l+1} W_l^T
 // masked by h_l > 0.  Per chunk of 64 rows a wave keeps dz in registers in
@@ -250,59 +328,11 @@ __global__ __launch_bounds__(kNrcBlock) void k_mlp_backward(const float* __restr
                 dzl[r * kWidth + r16] = gb;
             }
         }
-        // layers from the output down
-#pragma unroll
-        for (int l = kHidden; l >= 0; --l) {
-            const int K = layer_k(l), N = layer_n(l);
-            const uint16_t* h = hc + act_off(l);
-            // dW_l += h_l^T dz: A = h^T (features x rows), B = dz (rows x N)
-            for (int mt = 0; mt < K / 16; ++mt) {
-                s4 a[4];
-#pragma unroll
-                for (int kk = 0; kk < 4; ++kk) a[kk] = load4(h + (mt * 16 + r16) * kRows + kk * 16 + q * 4);
-                for (int nt = 0; nt < N / 16; ++nt) {
-                    f4v acc = f4v{ 0.0f, 0.0f, 0.0f, 0.0f };
-#pragma unroll
-                    for (int kk = 0; kk < 4; ++kk) acc = mfma(a[kk], dzb[kk][nt], acc);
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) atomicAdd(&sG[layer_off(l) + (mt * 16 + q * 4 + j) * N + nt * 16 + r16], acc[j]);
-                }
-            }
-            if (l == 0) break;
-            // dh_l = dz W_l^T: (rows x N) (N x K), K = kWidth here
-            f4v acc[4][kWidth / 16];
-#pragma unroll
-            for (int rt = 0; rt < 4; ++rt)
-#pragma unroll
-                for (int ct = 0; ct < kWidth / 16; ++ct) acc[rt][ct] = f4v{ 0.0f, 0.0f, 0.0f, 0.0f };
-            for (int kk = 0; kk < N / 16; ++kk) {
-                s4 b[kWidth / 16];
-#pragma unroll
-                for (int ct = 0; ct < kWidth / 16; ++ct)
-                    b[ct] = load4(Wt.W + layer_off(l) + (ct * 16 + r16) * N + kk * 16 + q * 4);
-#pragma unroll
-                for (int rt = 0; rt < 4; ++rt) {
-                    const s4 av = load4(dzl + (rt * 16 + r16) * kWidth + kk * 16 + q * 4);
-#pragma unroll
-                    for (int ct = 0; ct < kWidth / 16; ++ct) acc[rt][ct] = mfma(av, b[ct], acc[rt][ct]);
-                }
-            }
-            // mask by h_l > 0 and keep dz_l in both forms
-#pragma unroll
-            for (int rt = 0; rt < 4; ++rt)
-#pragma unroll
-                for (int ct = 0; ct < kWidth / 16; ++ct) {
-                    const s4 hv = load4(h + (ct * 16 + r16) * kRows + rt * 16 + q * 4);
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        const uint16_t hj = uint16_t(hv[j]);
-                        const bool on = hj != 0 && !(hj & 0x8000u);
-                        const uint16_t d = on ? to_bf16(acc[rt][ct][j]) : uint16_t(0);
-                        dzb[rt][ct][j] = short(d);
-                        dzl[(rt * 16 + q * 4 + j) * kWidth + ct * 16 + r16] = d;
-                    }
-                }
-        }
+        // layers from the output down; the kWidth x kWidth ones in a rolled loop
+        bwd_layer<kWidth, kOutCols, true>(kHidden, hc, Wt, sG, dzl, dzb, lane);
+#pragma unroll 1
+        for (int l = kHidden - 1; l >= 1; --l) bwd_layer<kWidth, kWidth, true>(l, hc, Wt, sG, dzl, dzb, lane);
+        bwd_layer<kIn, kWidth, false>(0, hc, Wt, sG, dzl, dzb, lane);
     }
     // wave loss -> global
     for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
