@@ -2,9 +2,10 @@
 // MFMA on gfx950 (fp32 accumulation, fp32 master weights, Adam).
 //
 // Network (no biases): the 74-wide input encoding padded to kIn = 80,
-// kHidden hidden layers of width kWidth with ReLU, and a linear output of 3
-// (stored as 16 columns, the last 13 held at zero):
-//   h0 = enc(x);  h_{l+1} = relu(h_l W_l) for l < kHidden;  y = h_kHidden W_kHidden
+// L hidden layers of width W with ReLU, and a linear output of 3 (stored as
+// 16 columns, the last 13 held at zero):
+//   h0 = enc(x);  h_{l+1} = relu(h_l W_l) for l < L;  y = h_L W_L
+// Two shapes are built: W = 64, L = 4 and W = 128, L = 5.
 // Input encoding of a raw row (16 floats: position in [0,1]^3, direction,
 // normal, roughness, albedo): sin/cos(2^f pi p_a) frequency-major for
 // f = 0..11 and axis a (72 values, the first 64 kept), then direction (3),
@@ -13,6 +14,14 @@
 // Matrix products use v_mfma_f32_16x16x16_bf16: a 16x16 tile of A (rows x k)
 // comes from lane l as A[l % 16][4 (l / 16) + j], j = 0..3; B (k x n) as
 // B[4 (l / 16) + j][l % 16]; the f32 result D[4 (l / 16) + j][l % 16].
+//
+// Training step: the forward pass saves every layer's bf16 input per chunk of
+// 64 rows, feature-major; k_mlp_dz runs the gradient chain dz_{l-1} =
+// (dz_l W_l^T) masked by h_l > 0 per chunk and saves each dz_l, also
+// feature-major; k_mlp_dw forms dW_l = h_l^T dz_l as a GEMM over the rows
+// (whose operands are exactly those saved layouts), split over row ranges
+// into partial sums; k_reduce_adam adds the partial sums in a fixed order
+// and takes the Adam step.
 #include "bb_math.h"
 #include "../../include/bbgpu.h"
 
@@ -33,26 +42,45 @@ namespace bb
 {
 
 constexpr int kIn = 80;      // encoded input width (74 used)
-constexpr int kWidth = 64;   // hidden width
-constexpr int kHidden = 4;   // hidden layers
 constexpr int kOutCols = 16; // output columns (3 used)
 constexpr int kRawStride = 16;
 constexpr int kRows = 64;    // rows per wave per chunk (4 row tiles of 16)
 constexpr int kNrcBlock = 256;
 
-// weight matrices: W_0 [kIn][kWidth], W_1..W_{kHidden-1} [kWidth][kWidth],
-// W_kHidden [kWidth][kOutCols]; fp32 master copies k-major ([k][n])
-constexpr int layer_k(int l) { return l == 0 ? kIn : kWidth; }
-constexpr int layer_n(int l) { return l == kHidden ? kOutCols : kWidth; }
-constexpr int layer_off(int l) { return l == 0 ? 0 : layer_off(l - 1) + layer_k(l - 1) * layer_n(l - 1); }
-constexpr int kParams = layer_off(kHidden + 1);
+// One network shape.  Weight matrices: W_0 [kIn][W], W_1..W_{L-1} [W][W],
+// W_L [W][kOutCols]; fp32 master copies k-major ([k][n]).
+template <int W_, int L_>
+struct Net
+{
+    static constexpr int W = W_, L = L_;
+    static constexpr int k(int l) { return l == 0 ? kIn : W; }
+    static constexpr int n(int l) { return l == L ? kOutCols : W; }
+    static constexpr int off(int l) { return l == 0 ? 0 : off(l - 1) + k(l - 1) * n(l - 1); }
+    static constexpr int params = off(L + 1);
+    static constexpr int max_layer = (kIn * W > W * W ? kIn * W : W * W);
+    // row stride (bf16) of a wave's activation tile in the forward pass
+    static constexpr int ld = W > kIn ? W : kIn;
+    // the whole bf16 network stays in LDS when it fits beside four
+    // activation tiles; otherwise each layer is staged in turn
+    static constexpr bool resident = params * 2 + 4 * kRows * ld * 2 <= 96 * 1024;
+    // Layer inputs saved by the training forward pass, per chunk of kRows
+    // rows, feature-major ([feature][row in chunk]): h_0 (kIn features), then
+    // h_1..h_L (W each).  Four consecutive rows of one feature are one 8-byte
+    // MFMA operand (k = rows) of dW = h^T dz, and the ReLU mask of the dz
+    // chain in its result layout.
+    static constexpr int act_off(int l) { return l == 0 ? 0 : kRows * kIn + (l - 1) * kRows * W; }
+    static constexpr int chunk_acts = act_off(L + 1);
+    // dz_l (gradient of layer l's output, n(l) features) per chunk, feature-major
+    static constexpr int dz_off(int l) { return l == 0 ? 0 : dz_off(l - 1) + n(l - 1) * kRows; }
+    static constexpr int chunk_dz = dz_off(L + 1);
+    // dW work groups (one wave each over a range of rows): layer 0 in
+    // 80 x 64 blocks (5 x 4 tiles), the hidden layers in 64 x 64 blocks,
+    // the output layer in 64 x 16 blocks
+    static constexpr int groups = W / 64 + (L - 1) * (W / 64) * (W / 64) + W / 64;
+};
 
-// Layer inputs saved by the training forward pass, per chunk of kRows rows,
-// feature-major ([feature][row in chunk]): h_0 (kIn features), then
-// h_1..h_kHidden (kWidth each).  This is the operand layout the backward
-// MFMAs read (four consecutive rows of one feature in 8 bytes).
-constexpr int act_off(int l) { return l == 0 ? 0 : kRows * kIn + (l - 1) * kRows * kWidth; }
-constexpr int kChunkActs = act_off(kHidden + 1);
+using NetC = Net<64, 4>;  // the C4 cache
+using NetL = Net<128, 5>; // the wide cache
 
 typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f4v __attribute__((ext_vector_type(4)));
@@ -88,20 +116,10 @@ BB_D void encode_row(const float* r, uint16_t* out)
 
 BB_D s4 load4(const uint16_t* p) { return *reinterpret_cast<const s4*>(p); }
 
-BB_D s4 gather4(const uint16_t* p, int stride)
-{
-    s4 v;
-    v.x = short(p[0]);
-    v.y = short(p[stride]);
-    v.z = short(p[2 * stride]);
-    v.w = short(p[3 * stride]);
-    return v;
-}
-
 BB_D f4v mfma(s4 a, s4 b, f4v c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
 
 // one layer for a wave's kRows rows: act [kRows][ld_in] (bf16, LDS) times
-// W_l (WT: [n][k] bf16) -> acc[row tile][col tile]
+// W_l (WT: [n][k] bf16, LDS) -> acc[row tile][col tile]
 template <int K, int N>
 BB_D void layer_forward(const uint16_t* act, int ld_in, const uint16_t* WT, f4v (&acc)[4][N / 16], int lane)
 {
@@ -124,74 +142,100 @@ BB_D void layer_forward(const uint16_t* act, int ld_in, const uint16_t* WT, f4v 
     }
 }
 
+// bf16 copy of count values (a multiple of 8) into LDS by the whole block
+BB_D void stage16(uint16_t* dst, const uint16_t* src, int count)
+{
+    for (int i = threadIdx.x; i < count / 8; i += blockDim.x)
+        reinterpret_cast<uint4*>(dst)[i] = reinterpret_cast<const uint4*>(src)[i];
+}
+
 // ---------------------------------------------------------------- forward
 
 struct NrcWeights
 {
-    const uint16_t* WT; // bf16 [n][k] per layer (at layer_off)
+    const uint16_t* WT; // bf16 [n][k] per layer (at off(l))
     const uint16_t* W;  // bf16 [k][n] per layer
 };
 
 // y = f(enc(raw)) for n rows; with SAVE the bf16 input of every layer is
-// stored to acts[l][row][0..K_l) (layer l's input h_l) for the backward pass
-template <bool SAVE>
+// stored per chunk, feature-major (Net::act_off), for the backward pass.
+// The block's four waves take four consecutive chunks per iteration; when
+// the weights are staged layer by layer every wave joins every barrier.
+template <class NT, bool SAVE>
 __global__ __launch_bounds__(kNrcBlock) void k_mlp_forward(const float* __restrict__ raw, int n, NrcWeights Wt,
                                                             float* __restrict__ out, uint16_t* __restrict__ acts)
 {
-    __shared__ __align__(16) uint16_t sW[kParams];
-    __shared__ uint16_t sAct[kNrcBlock / 64][kRows * kIn];
-    static_assert(kParams % 8 == 0, "weights staged 16 bytes at a time");
-    for (int i = threadIdx.x; i < kParams / 8; i += blockDim.x)
-        reinterpret_cast<uint4*>(sW)[i] = reinterpret_cast<const uint4*>(Wt.WT)[i];
-    __syncthreads();
+    constexpr int W = NT::W, L = NT::L, LD = NT::ld;
+    __shared__ __align__(16) uint16_t sW[NT::resident ? NT::params : NT::max_layer];
+    __shared__ __align__(16) uint16_t sAct[kNrcBlock / 64][kRows * LD];
+    static_assert(NT::params % 8 == 0 && NT::max_layer % 8 == 0, "weights staged 16 bytes at a time");
+    if constexpr (NT::resident) {
+        stage16(sW, Wt.WT, NT::params);
+        __syncthreads();
+    }
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     uint16_t* act = sAct[wave];
     const int r16 = lane & 15, q = lane >> 4;
     const int chunks = (n + kRows - 1) / kRows;
-    for (int ch = blockIdx.x * (kNrcBlock / 64) + wave; ch < chunks; ch += gridDim.x * (kNrcBlock / 64)) {
+    for (int base = blockIdx.x * (kNrcBlock / 64); base < chunks; base += gridDim.x * (kNrcBlock / 64)) {
+        const int ch = base + wave;
+        const bool active = ch < chunks;
         const int row0 = ch * kRows;
-        {
+        if (active) {
             // each lane encodes its row straight into the wave's LDS rows
             const int row = row0 + lane;
-            uint16_t* e = act + lane * kIn;
+            uint16_t* e = act + lane * LD;
             if (row < n)
                 encode_row(raw + size_t(row) * kRawStride, e);
             else
                 for (int i = 0; i < kIn; ++i) e[i] = 0;
             if (SAVE) // every row of the chunk, zeros beyond n
-                for (int i = 0; i < kIn; ++i) acts[size_t(ch) * kChunkActs + act_off(0) + i * kRows + lane] = e[i];
+                for (int i = 0; i < kIn; ++i) acts[size_t(ch) * NT::chunk_acts + NT::act_off(0) + i * kRows + lane] = e[i];
         }
-        // hidden layers: layer 0 (kIn inputs), then the kWidth-input ones in a
-        // rolled loop (one copy of the code: small kernels stay in the
-        // instruction cache when every wave runs a single chunk)
+        // hidden layers in a rolled loop (one copy of the code: small kernels
+        // stay in the instruction cache)
 #pragma unroll 1
-        for (int l = 0; l < kHidden; ++l) {
-            f4v acc[4][kWidth / 16];
+        for (int l = 0; l < L; ++l) {
+            const uint16_t* Wl = sW + (NT::resident ? NT::off(l) : 0);
+            if constexpr (!NT::resident) {
+                __syncthreads(); // every wave is done with the previous layer's weights
+                stage16(sW, Wt.WT + NT::off(l), NT::k(l) * NT::n(l));
+                __syncthreads();
+            }
+            if (!active) continue;
+            f4v acc[4][W / 16];
             if (l == 0)
-                layer_forward<kIn, kWidth>(act, kIn, sW + layer_off(0), acc, lane);
+                layer_forward<kIn, W>(act, LD, Wl, acc, lane);
             else
-                layer_forward<kWidth, kWidth>(act, kIn, sW + layer_off(l), acc, lane);
+                layer_forward<W, W>(act, LD, Wl, acc, lane);
 #pragma unroll
             for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
-                for (int ct = 0; ct < kWidth / 16; ++ct) {
+                for (int ct = 0; ct < W / 16; ++ct) {
                     s4 hv;
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         const int r = rt * 16 + q * 4 + j, c = ct * 16 + r16;
                         const uint16_t h = to_bf16(fmaxf(acc[rt][ct][j], 0.0f));
-                        act[r * kIn + c] = h;
+                        act[r * LD + c] = h;
                         hv[j] = short(h);
                     }
                     // rows rt*16 + 4q .. +3 of feature ct*16 + r16 of h_{l+1}
                     if (SAVE)
-                        *reinterpret_cast<s4*>(acts + size_t(ch) * kChunkActs + act_off(l + 1) +
+                        *reinterpret_cast<s4*>(acts + size_t(ch) * NT::chunk_acts + NT::act_off(l + 1) +
                                                (ct * 16 + r16) * kRows + rt * 16 + q * 4) = hv;
                 }
         }
         // output layer
+        const uint16_t* Wo = sW + (NT::resident ? NT::off(L) : 0);
+        if constexpr (!NT::resident) {
+            __syncthreads();
+            stage16(sW, Wt.WT + NT::off(L), NT::k(L) * NT::n(L));
+            __syncthreads();
+        }
+        if (!active) continue;
         f4v acc[4][1];
-        layer_forward<kWidth, kOutCols>(act, kIn, sW + layer_off(kHidden), acc, lane);
+        layer_forward<W, kOutCols>(act, LD, Wo, acc, lane);
 #pragma unroll
         for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
@@ -214,97 +258,70 @@ BB_D float loss_grad(float y, float t, float inv_count, float& loss)
     return 2.0f * d / den * inv_count;
 }
 
-// One layer of the backward pass for a wave's chunk: dW_l += h_l^T dz (K x N)
-// with dz in registers (dzb, the MFMA result layout) and row-major in dzl;
-// with DH, dz becomes dh_l = dz W_l^T masked by h_l > 0 (kWidth columns).
-template <int K, int N, bool DH>
-BB_D void bwd_layer(int l, const uint16_t* hc, const NrcWeights& Wt, float* sG, uint16_t* dzl,
-                    s4 (&dzb)[4][kWidth / 16], int lane)
+// dz_{l-1} = (dz_l W_l^T) masked by h_l > 0 for a wave's chunk: dz_l
+// (kRows x N) row-major in dzl (LDS), W_l k-major from global memory, h_l
+// (the mask) feature-major from the saved inputs; the result replaces dzl
+// (row-major, stride W) and is stored feature-major to dzc.
+template <class NT, int N>
+BB_D void dz_layer(int l, const uint16_t* hc, const uint16_t* Wk, uint16_t* dzl, uint16_t* dzc, int lane)
 {
+    constexpr int W = NT::W;
     const int r16 = lane & 15, q = lane >> 4;
-    const uint16_t* h = hc + act_off(l);
-    float* g = sG + layer_off(l);
-    // every global operand of the layer first, so that the wave (alone on
-    // its SIMD) waits for memory once per layer: h^T tiles, the ReLU mask
-    // (h_l itself) and W_l
-    s4 a[K / 16][4];
-#pragma unroll
-    for (int mt = 0; mt < K / 16; ++mt)
-#pragma unroll
-        for (int kk = 0; kk < 4; ++kk) a[mt][kk] = load4(h + (mt * 16 + r16) * kRows + kk * 16 + q * 4);
-    s4 hv[DH ? 4 : 1][kWidth / 16];
-    s4 b[DH ? N / 16 : 1][kWidth / 16];
-    if constexpr (DH) {
-#pragma unroll
-        for (int rt = 0; rt < 4; ++rt)
-#pragma unroll
-            for (int ct = 0; ct < kWidth / 16; ++ct) hv[rt][ct] = load4(h + (ct * 16 + r16) * kRows + rt * 16 + q * 4);
-#pragma unroll
-        for (int kk = 0; kk < N / 16; ++kk)
-#pragma unroll
-            for (int ct = 0; ct < kWidth / 16; ++ct)
-                b[kk][ct] = load4(Wt.W + layer_off(l) + (ct * 16 + r16) * N + kk * 16 + q * 4);
-    }
-    // dW_l += h_l^T dz: A = h^T (features x rows), B = dz (rows x N)
-#pragma unroll
-    for (int mt = 0; mt < K / 16; ++mt)
-#pragma unroll
-        for (int nt = 0; nt < N / 16; ++nt) {
-            f4v acc = f4v{ 0.0f, 0.0f, 0.0f, 0.0f };
-#pragma unroll
-            for (int kk = 0; kk < 4; ++kk) acc = mfma(a[mt][kk], dzb[kk][nt], acc);
-#pragma unroll
-            for (int j = 0; j < 4; ++j) atomicAdd(&g[(mt * 16 + q * 4 + j) * N + nt * 16 + r16], acc[j]);
-        }
-    if constexpr (!DH) return;
-    // dh_l = dz W_l^T: (rows x N) (N x kWidth)
-    f4v acc[4][kWidth / 16];
+    const uint16_t* h = hc + NT::act_off(l);
+    const uint16_t* Wl = Wk + NT::off(l);
+    // the mask first, so that its loads overlap the products
+    s4 hv[4][W / 16];
 #pragma unroll
     for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
-        for (int ct = 0; ct < kWidth / 16; ++ct) acc[rt][ct] = f4v{ 0.0f, 0.0f, 0.0f, 0.0f };
+        for (int ct = 0; ct < W / 16; ++ct) hv[rt][ct] = load4(h + (ct * 16 + r16) * kRows + rt * 16 + q * 4);
+    f4v acc[4][W / 16];
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < W / 16; ++ct) acc[rt][ct] = f4v{ 0.0f, 0.0f, 0.0f, 0.0f };
+    // (rows x N) (N x W): A = dz rows, B = W_l^T (B[n][k] = W_l[k][n])
 #pragma unroll
     for (int kk = 0; kk < N / 16; ++kk) {
+        s4 b[W / 16];
+#pragma unroll
+        for (int ct = 0; ct < W / 16; ++ct) b[ct] = load4(Wl + (ct * 16 + r16) * N + kk * 16 + q * 4);
 #pragma unroll
         for (int rt = 0; rt < 4; ++rt) {
-            const s4 av = load4(dzl + (rt * 16 + r16) * kWidth + kk * 16 + q * 4);
+            const s4 av = load4(dzl + (rt * 16 + r16) * W + kk * 16 + q * 4);
 #pragma unroll
-            for (int ct = 0; ct < kWidth / 16; ++ct) acc[rt][ct] = mfma(av, b[kk][ct], acc[rt][ct]);
+            for (int ct = 0; ct < W / 16; ++ct) acc[rt][ct] = mfma(av, b[ct], acc[rt][ct]);
         }
     }
-    // mask by h_l > 0 and keep dz_l in both forms
+    uint16_t* out = dzc + NT::dz_off(l - 1);
 #pragma unroll
     for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
-        for (int ct = 0; ct < kWidth / 16; ++ct) {
+        for (int ct = 0; ct < W / 16; ++ct) {
+            s4 dv;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const uint16_t hj = uint16_t(hv[rt][ct][j]);
                 const bool on = hj != 0 && !(hj & 0x8000u);
                 const uint16_t d = on ? to_bf16(acc[rt][ct][j]) : uint16_t(0);
-                dzb[rt][ct][j] = short(d);
-                dzl[(rt * 16 + q * 4 + j) * kWidth + ct * 16 + r16] = d;
+                dv[j] = short(d);
+                dzl[(rt * 16 + q * 4 + j) * W + ct * 16 + r16] = d;
             }
+            *reinterpret_cast<s4*>(out + (ct * 16 + r16) * kRows + rt * 16 + q * 4) = dv;
         }
 }
 
-// Gradient of the loss for n rows whose layer inputs the forward pass saved
-// (feature-major per chunk): dW_l = h_l^T dz_{l+1}, dh_l = dz_{l+1} W_l^T
-// masked by h_l > 0.  Per chunk of 64 rows a wave keeps dz in registers in
-// the MFMA result layout, which is exactly the B operand of dW = h^T dz (the
-// lane's own tile), and a row-major bf16 copy in LDS for the A operand of
-// dz W^T; h^T and the ReLU mask come from the saved layout with one 8-byte
-// load each.  Each block sums its waves' dW in LDS and stores the sum to
-// partials[block] (k_adam adds the blocks in order).
-__global__ __launch_bounds__(kNrcBlock) void k_mlp_backward(const float* __restrict__ y,
-                                                             const float* __restrict__ target, int n,
-                                                             NrcWeights Wt, const uint16_t* __restrict__ acts,
-                                                             float* __restrict__ partials, float* __restrict__ loss_sum)
+// The gradient chain for n rows whose layer inputs the forward pass saved:
+// dz_L = dL/dy, then dz_{l-1} for l = L..1, every dz_l stored per chunk
+// (Net::dz_off).  One wave per chunk; the loss is summed into loss_sum.
+template <class NT>
+__global__ __launch_bounds__(kNrcBlock) void k_mlp_dz(const float* __restrict__ y, const float* __restrict__ target,
+                                                      int n, const uint16_t* __restrict__ Wk,
+                                                      const uint16_t* __restrict__ acts, uint16_t* __restrict__ dzs,
+                                                      float* __restrict__ loss_sum)
 {
-    __shared__ float sG[kParams];
-    __shared__ uint16_t sD[kNrcBlock / 64][kRows * kWidth]; // dz of the current layer output, row-major
-    for (int i = threadIdx.x; i < kParams; i += blockDim.x) sG[i] = 0.0f;
-    __syncthreads();
+    constexpr int W = NT::W, L = NT::L;
+    __shared__ __align__(16) uint16_t sD[kNrcBlock / 64][kRows * W]; // dz of the current layer output, row-major
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int r16 = lane & 15, q = lane >> 4;
     uint16_t* dzl = sD[wave];
@@ -313,54 +330,127 @@ __global__ __launch_bounds__(kNrcBlock) void k_mlp_backward(const float* __restr
     const int chunks = (n + kRows - 1) / kRows;
     for (int ch = blockIdx.x * (kNrcBlock / 64) + wave; ch < chunks; ch += gridDim.x * (kNrcBlock / 64)) {
         const int row0 = ch * kRows;
-        const uint16_t* hc = acts + size_t(ch) * kChunkActs;
-        // dz of the output: dL/dy of rows rt*16 + 4q + j, column r16 (3 used)
-        s4 dzb[4][kWidth / 16];
+        const uint16_t* hc = acts + size_t(ch) * NT::chunk_acts;
+        uint16_t* dzc = dzs + size_t(ch) * NT::chunk_dz;
+        // dz_L: dL/dy of rows rt*16 + 4q + j, column r16 (3 used)
 #pragma unroll
         for (int rt = 0; rt < 4; ++rt) {
+            s4 dv;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int r = rt * 16 + q * 4 + j, row = row0 + r;
                 float g = 0.0f;
-                if (r16 < 3 && row < n) g = loss_grad(y[size_t(row) * 3 + r16], target[size_t(row) * 3 + r16], inv_count, loss);
+                if (r16 < 3 && row < n)
+                    g = loss_grad(y[size_t(row) * 3 + r16], target[size_t(row) * 3 + r16], inv_count, loss);
                 const uint16_t gb = to_bf16(g);
-                dzb[rt][0][j] = short(gb);
-                dzl[r * kWidth + r16] = gb;
+                dv[j] = short(gb);
+                dzl[r * W + r16] = gb;
             }
+            *reinterpret_cast<s4*>(dzc + NT::dz_off(L) + r16 * kRows + rt * 16 + q * 4) = dv;
         }
-        // layers from the output down; the kWidth x kWidth ones in a rolled loop
-        bwd_layer<kWidth, kOutCols, true>(kHidden, hc, Wt, sG, dzl, dzb, lane);
+        dz_layer<NT, kOutCols>(L, hc, Wk, dzl, dzc, lane);
 #pragma unroll 1
-        for (int l = kHidden - 1; l >= 1; --l) bwd_layer<kWidth, kWidth, true>(l, hc, Wt, sG, dzl, dzb, lane);
-        bwd_layer<kIn, kWidth, false>(0, hc, Wt, sG, dzl, dzb, lane);
+        for (int l = L - 1; l >= 1; --l) dz_layer<NT, W>(l, hc, Wk, dzl, dzc, lane);
     }
-    // wave loss -> global
     for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
     if (lane == 0) atomicAdd(loss_sum, loss);
-    __syncthreads();
-    float* out = partials + size_t(blockIdx.x) * kParams;
-    for (int i = threadIdx.x; i < kParams; i += blockDim.x) out[i] = sG[i];
 }
 
-// The gradient as the sum of the backward blocks' partial sums (in block
-// order), kept in g; then Adam (beta1 0.9, beta2 0.99, eps 1e-8) on the fp32
-// master weights when apply, and the bf16 copies in both layouts; the unused
-// output columns stay zero
-__global__ void k_adam(float* __restrict__ w, float* __restrict__ m, float* __restrict__ v, float* __restrict__ g,
-                       const float* __restrict__ partials, int n_partials, float lr, float c1, float c2, int apply,
-                       uint16_t* __restrict__ WT, uint16_t* __restrict__ W)
+// dW_l rows [mt0, mt0 + 16 TM) x columns [nt0, nt0 + 16 TN) summed over
+// chunks [c0, c1): A = h_l^T (features x rows), B = dz_l (rows x features),
+// both straight from their feature-major chunks; out is the layer's block of
+// one partial sum ([K][N])
+template <class NT, int TM, int TN>
+BB_D void dw_tile(int l, int mt0, int nt0, int c0, int c1, const uint16_t* acts, const uint16_t* dzs, float* out,
+                  int lane)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= kParams) return;
+    const int r16 = lane & 15, q = lane >> 4;
+    const int N = NT::n(l);
+    f4v acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[i][j] = f4v{ 0.0f, 0.0f, 0.0f, 0.0f };
+    for (int c = c0; c < c1; ++c) {
+        const uint16_t* h = acts + size_t(c) * NT::chunk_acts + NT::act_off(l) + (mt0 * 16 + r16) * kRows + q * 4;
+        const uint16_t* d = dzs + size_t(c) * NT::chunk_dz + NT::dz_off(l) + (nt0 * 16 + r16) * kRows + q * 4;
+#pragma unroll
+        for (int st = 0; st < kRows / 16; ++st) {
+            s4 a[TM], b[TN];
+#pragma unroll
+            for (int i = 0; i < TM; ++i) a[i] = load4(h + i * 16 * kRows + st * 16);
+#pragma unroll
+            for (int j = 0; j < TN; ++j) b[j] = load4(d + j * 16 * kRows + st * 16);
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+#pragma unroll
+                for (int j = 0; j < TN; ++j) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                out[((mt0 + i) * 16 + q * 4 + e) * N + (nt0 + j) * 16 + r16] = acc[i][j][e];
+}
+
+// dW of every layer as Net::groups work groups times `splits` row ranges,
+// one wave each: block b does group b % groups over chunks
+// [chunks s / splits, chunks (s + 1) / splits) with s = b / groups, into
+// partials[s][params]
+template <class NT>
+__global__ __launch_bounds__(64) void k_mlp_dw(const uint16_t* __restrict__ acts, const uint16_t* __restrict__ dzs,
+                                               int chunks, int splits, float* __restrict__ partials)
+{
+    constexpr int W = NT::W, L = NT::L, B = W / 64; // 64-wide blocks per hidden dimension
+    const int g = blockIdx.x % NT::groups, s = blockIdx.x / NT::groups;
+    if (s >= splits) return;
+    const int c0 = int(int64_t(chunks) * s / splits), c1 = int(int64_t(chunks) * (s + 1) / splits);
+    const int lane = threadIdx.x;
+    float* out = partials + size_t(s) * NT::params;
+    if (g < B) { // layer 0: all 80 rows, columns 64 g ..
+        dw_tile<NT, 5, 4>(0, 0, 4 * g, c0, c1, acts, dzs, out + NT::off(0), lane);
+    } else if (g < B + (L - 1) * B * B) {
+        const int i = g - B, l = 1 + i / (B * B), blk = i % (B * B);
+        dw_tile<NT, 4, 4>(l, 4 * (blk / B), 4 * (blk % B), c0, c1, acts, dzs, out + NT::off(l), lane);
+    } else { // output layer: rows 64 (g - ..) .., its 16 columns
+        dw_tile<NT, 4, 1>(L, 4 * (g - B - (L - 1) * B * B), 0, c0, c1, acts, dzs, out + NT::off(L), lane);
+    }
+}
+
+// The gradient as the sum of the row ranges' partial sums, kept in g: wave
+// w of a block adds ranges w, w + 4, ... in order, then the four sums are
+// added in wave order.  Then Adam (beta1 0.9, beta2 0.99, eps 1e-8) on the
+// fp32 master weights when apply, and the bf16 copies in both layouts; the
+// unused output columns stay zero.  splits = 0 only refreshes the copies.
+template <class NT>
+__global__ __launch_bounds__(256) void k_reduce_adam(float* __restrict__ w, float* __restrict__ m,
+                                                     float* __restrict__ v, float* __restrict__ g,
+                                                     const float* __restrict__ partials, int splits, float lr,
+                                                     float c1, float c2, int apply, uint16_t* __restrict__ WT,
+                                                     uint16_t* __restrict__ W)
+{
+    __shared__ float part[4][64];
+    const int t = threadIdx.x, wv = t >> 6;
+    const int i = blockIdx.x * 64 + (t & 63);
+    float s = 0.0f;
+    if (i < NT::params) {
+#pragma unroll 8
+        for (int b = wv; b < splits; b += 4) s += partials[size_t(b) * NT::params + i];
+    }
+    part[wv][t & 63] = s;
+    __syncthreads();
+    if (wv != 0 || i >= NT::params) return;
+    const float gi = ((part[0][t] + part[1][t]) + part[2][t]) + part[3][t];
     int l = 0;
-    while (l < kHidden && i >= layer_off(l + 1)) ++l;
-    const int K = layer_k(l), N = layer_n(l);
-    const int k = (i - layer_off(l)) / N, nn = (i - layer_off(l)) % N;
-    float gi = 0.0f;
-    for (int b = 0; b < n_partials; ++b) gi += partials[size_t(b) * kParams + i];
-    if (n_partials > 0) g[i] = gi;
+    while (l < NT::L && i >= NT::off(l + 1)) ++l;
+    const int K = NT::k(l), N = NT::n(l);
+    const int k = (i - NT::off(l)) / N, nn = (i - NT::off(l)) % N;
+    if (splits > 0) g[i] = gi;
     float wi = w[i];
-    if (apply && !(l == kHidden && nn >= 3)) {
+    if (apply && !(l == NT::L && nn >= 3)) {
         const float mi = 0.9f * m[i] + 0.1f * gi;
         const float vi = 0.99f * v[i] + 0.01f * gi * gi;
         m[i] = mi;
@@ -370,7 +460,7 @@ __global__ void k_adam(float* __restrict__ w, float* __restrict__ m, float* __re
     }
     const uint16_t h = to_bf16(wi);
     W[i] = h;
-    WT[layer_off(l) + nn * K + k] = h;
+    WT[NT::off(l) + nn * K + k] = h;
 }
 
 __global__ void k_fill0(float* p, int n)
@@ -386,17 +476,21 @@ __global__ void k_fill0(float* p, int n)
 struct BbNrc
 {
     int device = 0;
+    int width = 64;
+    int hidden = 4;
+    int params = 0;
     float lr = 1e-3f;
     int64_t step = 0;
     float* w = nullptr; // fp32 master
     float* m = nullptr;
     float* v = nullptr;
     float* g = nullptr;
-    float* partials = nullptr; // [backward block][kParams]
+    float* partials = nullptr; // [row range][params]
     float* loss = nullptr;
     uint16_t* WT = nullptr;
     uint16_t* W = nullptr;
     uint16_t* acts = nullptr; // saved layer inputs of the last training batch
+    uint16_t* dzs = nullptr;  // saved layer-output gradients of the last training batch
     float* y = nullptr;
     size_t acts_rows = 0;
 };
@@ -418,20 +512,87 @@ int nrc_grid(int n)
 }
 static_assert(bb::kNrcBlock / 64 == 4, "nrc_grid assumes four waves per block");
 
-constexpr int kMaxBackwardBlocks = 512;
+// most row ranges of the dW GEMM (partial sums held)
+constexpr int kMaxSplits = 512;
 
-bool nrc_sync_bf16(BbNrc* x, hipStream_t st, bool apply, float c1, float c2, int n_partials = 0)
+// row ranges for the dW GEMM: about 2048 waves in all, at least four chunks each
+int dw_splits(int chunks, int groups)
 {
-    hipLaunchKernelGGL(bb::k_adam, dim3((bb::kParams + 255) / 256), dim3(256), 0, st, x->w, x->m, x->v, x->g,
-                       x->partials, n_partials, x->lr, c1, c2, apply ? 1 : 0, x->WT, x->W);
-    return nrc_ok(hipGetLastError(), "k_adam");
+    const int want = (2048 + groups - 1) / groups;
+    return std::max(1, std::min({ want, chunks / 4, kMaxSplits }));
+}
+
+int params_of(int width, int hidden)
+{
+    if (width == bb::NetC::W && hidden == bb::NetC::L) return bb::NetC::params;
+    if (width == bb::NetL::W && hidden == bb::NetL::L) return bb::NetL::params;
+    return -1;
+}
+
+template <class NT>
+bool nrc_sync_bf16_t(BbNrc* x, hipStream_t st, bool apply, float c1, float c2, int splits)
+{
+    hipLaunchKernelGGL(bb::k_reduce_adam<NT>, dim3((NT::params + 63) / 64), dim3(256), 0, st, x->w, x->m, x->v, x->g,
+                       x->partials, splits, x->lr, c1, c2, apply ? 1 : 0, x->WT, x->W);
+    return nrc_ok(hipGetLastError(), "k_reduce_adam");
+}
+
+bool nrc_sync_bf16(BbNrc* x, hipStream_t st, bool apply, float c1, float c2, int splits = 0)
+{
+    return x->width == bb::NetC::W ? nrc_sync_bf16_t<bb::NetC>(x, st, apply, c1, c2, splits)
+                                   : nrc_sync_bf16_t<bb::NetL>(x, st, apply, c1, c2, splits);
+}
+
+template <class NT>
+void launch_query(BbNrc* x, const float* inputs_dev, int n, float* out_dev, hipStream_t st)
+{
+    const bb::NrcWeights Wt{ x->WT, x->W };
+    hipLaunchKernelGGL((bb::k_mlp_forward<NT, false>), dim3(nrc_grid(n)), dim3(bb::kNrcBlock), 0, st, inputs_dev, n,
+                       Wt, out_dev, nullptr);
+}
+
+// forward, gradient chain and dW partial sums of one step; *splits = the
+// number of row ranges for the reduction
+template <class NT>
+int train_step(BbNrc* x, const float* inputs_dev, const float* targets_dev, int n, hipStream_t st, int* splits_out)
+{
+    const size_t chunks = (size_t(n) + bb::kRows - 1) / bb::kRows;
+    if (x->acts_rows < size_t(n)) {
+        for (void* p : { (void*)x->acts, (void*)x->dzs, (void*)x->y })
+            if (p) (void)hipFree(p);
+        x->acts = nullptr;
+        x->dzs = nullptr;
+        x->y = nullptr;
+        x->acts_rows = 0;
+        if (!nrc_ok(hipMalloc(&x->acts, chunks * size_t(NT::chunk_acts) * 2), "hipMalloc") ||
+            !nrc_ok(hipMalloc(&x->dzs, chunks * size_t(NT::chunk_dz) * 2), "hipMalloc") ||
+            !nrc_ok(hipMalloc(&x->y, size_t(n) * 12), "hipMalloc"))
+            return BBGPU_ERR_NOMEM;
+        x->acts_rows = size_t(n);
+    }
+    const bb::NrcWeights Wt{ x->WT, x->W };
+    const int grid = nrc_grid(n);
+    const int splits = dw_splits(int(chunks), NT::groups);
+    hipLaunchKernelGGL(bb::k_fill0, dim3(1), dim3(64), 0, st, x->loss, 1);
+    hipLaunchKernelGGL((bb::k_mlp_forward<NT, true>), dim3(grid), dim3(bb::kNrcBlock), 0, st, inputs_dev, n, Wt, x->y,
+                       x->acts);
+    hipLaunchKernelGGL(bb::k_mlp_dz<NT>, dim3(grid), dim3(bb::kNrcBlock), 0, st, x->y, targets_dev, n, x->W, x->acts,
+                       x->dzs, x->loss);
+    hipLaunchKernelGGL(bb::k_mlp_dw<NT>, dim3(NT::groups * splits), dim3(64), 0, st, x->acts, x->dzs, int(chunks),
+                       splits, x->partials);
+    if (!nrc_ok(hipGetLastError(), "k_mlp_dw")) return BBGPU_ERR_DEVICE;
+    *splits_out = splits;
+    return BBGPU_OK;
 }
 
 } // namespace
 
 extern "C" {
 
-int bbgpu_nrc_param_count(void) { return bb::kParams; }
+int bbgpu_nrc_param_count(const BbNrcDesc* desc)
+{
+    return desc ? params_of(desc->hidden_width, desc->hidden_layers) : bb::NetC::params;
+}
 
 int bbgpu_nrc_create(const BbNrcDesc* desc, BbNrc** out)
 {
@@ -439,8 +600,10 @@ int bbgpu_nrc_create(const BbNrcDesc* desc, BbNrc** out)
         bb::set_error("bbgpu_nrc_create: null argument");
         return BBGPU_ERR_INVALID;
     }
-    if (desc->hidden_width != bb::kWidth || desc->hidden_layers != bb::kHidden) {
-        bb::set_error("bbgpu_nrc_create: only %d hidden layers of width %d are built", bb::kHidden, bb::kWidth);
+    const int P = params_of(desc->hidden_width, desc->hidden_layers);
+    if (P < 0) {
+        bb::set_error("bbgpu_nrc_create: %d hidden layers of width %d are not built (%d x %d and %d x %d are)",
+                      desc->hidden_layers, desc->hidden_width, bb::NetC::L, bb::NetC::W, bb::NetL::L, bb::NetL::W);
         return BBGPU_ERR_UNSUPPORTED;
     }
     int ndev = 0;
@@ -450,35 +613,44 @@ int bbgpu_nrc_create(const BbNrcDesc* desc, BbNrc** out)
         return BBGPU_ERR_DEVICE;
     }
     BbNrc* x = new BbNrc();
-    if (hipGetDevice(&x->device) != hipSuccess) x->device = 0;
+    if (!nrc_ok(hipGetDevice(&x->device), "hipGetDevice")) {
+        delete x;
+        return BBGPU_ERR_DEVICE;
+    }
+    x->width = desc->hidden_width;
+    x->hidden = desc->hidden_layers;
+    x->params = P;
     x->lr = desc->learning_rate > 0.0f ? desc->learning_rate : 1e-3f;
-    const size_t P = bb::kParams;
-    bool ok = nrc_ok(hipMalloc(&x->w, P * 4), "hipMalloc") && nrc_ok(hipMalloc(&x->m, P * 4), "hipMalloc") &&
-              nrc_ok(hipMalloc(&x->v, P * 4), "hipMalloc") && nrc_ok(hipMalloc(&x->g, P * 4), "hipMalloc") &&
-              nrc_ok(hipMalloc(&x->loss, 4), "hipMalloc") && nrc_ok(hipMalloc(&x->WT, P * 2), "hipMalloc") &&
-              nrc_ok(hipMalloc(&x->W, P * 2), "hipMalloc") &&
-              nrc_ok(hipMalloc(&x->partials, size_t(kMaxBackwardBlocks) * P * 4), "hipMalloc");
+    const size_t PS = size_t(P);
+    bool ok = nrc_ok(hipMalloc(&x->w, PS * 4), "hipMalloc") && nrc_ok(hipMalloc(&x->m, PS * 4), "hipMalloc") &&
+              nrc_ok(hipMalloc(&x->v, PS * 4), "hipMalloc") && nrc_ok(hipMalloc(&x->g, PS * 4), "hipMalloc") &&
+              nrc_ok(hipMalloc(&x->loss, 4), "hipMalloc") && nrc_ok(hipMalloc(&x->WT, PS * 2), "hipMalloc") &&
+              nrc_ok(hipMalloc(&x->W, PS * 2), "hipMalloc") &&
+              nrc_ok(hipMalloc(&x->partials, size_t(kMaxSplits) * PS * 4), "hipMalloc");
     if (!ok) {
         bbgpu_nrc_destroy(x);
         return BBGPU_ERR_NOMEM;
     }
     // Xavier-uniform initial weights from PCG32 seeded with desc->seed
-    std::vector<float> w(P, 0.0f);
+    const int L = x->hidden, Wd = x->width;
+    std::vector<float> w(PS, 0.0f);
     uint64_t st = bb::pcg_seed(uint64_t(desc->seed));
-    for (int l = 0; l <= bb::kHidden; ++l) {
-        const int K = bb::layer_k(l), N = bb::layer_n(l);
-        const int fan_in = l == 0 ? 74 : K, fan_out = l == bb::kHidden ? 3 : N;
+    size_t off = 0;
+    for (int l = 0; l <= L; ++l) {
+        const int K = l == 0 ? bb::kIn : Wd, N = l == L ? bb::kOutCols : Wd;
+        const int fan_in = l == 0 ? 74 : K, fan_out = l == L ? 3 : N;
         const float a = std::sqrt(6.0f / float(fan_in + fan_out));
         for (int k = 0; k < K; ++k)
             for (int n = 0; n < N; ++n) {
                 const float u = bb::pcg_float(st);
-                const bool used = (l != bb::kHidden || n < 3) && (l != 0 || k < 74);
-                w[size_t(bb::layer_off(l) + k * N + n)] = used ? (2.0f * u - 1.0f) * a : 0.0f;
+                const bool used = (l != L || n < 3) && (l != 0 || k < 74);
+                w[off + size_t(k * N + n)] = used ? (2.0f * u - 1.0f) * a : 0.0f;
             }
+        off += size_t(K) * N;
     }
-    ok = nrc_ok(hipMemcpy(x->w, w.data(), P * 4, hipMemcpyHostToDevice), "hipMemcpy") &&
-         nrc_ok(hipMemset(x->m, 0, P * 4), "hipMemset") && nrc_ok(hipMemset(x->v, 0, P * 4), "hipMemset") &&
-         nrc_ok(hipMemset(x->g, 0, P * 4), "hipMemset") && nrc_sync_bf16(x, nullptr, false, 1.0f, 1.0f) &&
+    ok = nrc_ok(hipMemcpy(x->w, w.data(), PS * 4, hipMemcpyHostToDevice), "hipMemcpy") &&
+         nrc_ok(hipMemset(x->m, 0, PS * 4), "hipMemset") && nrc_ok(hipMemset(x->v, 0, PS * 4), "hipMemset") &&
+         nrc_ok(hipMemset(x->g, 0, PS * 4), "hipMemset") && nrc_sync_bf16(x, nullptr, false, 1.0f, 1.0f) &&
          nrc_ok(hipDeviceSynchronize(), "hipDeviceSynchronize");
     if (!ok) {
         bbgpu_nrc_destroy(x);
@@ -492,7 +664,7 @@ int bbgpu_nrc_destroy(BbNrc* x)
 {
     if (!x) return BBGPU_OK;
     for (void* p : { (void*)x->w, (void*)x->m, (void*)x->v, (void*)x->g, (void*)x->partials, (void*)x->loss,
-                     (void*)x->WT, (void*)x->W, (void*)x->acts, (void*)x->y })
+                     (void*)x->WT, (void*)x->W, (void*)x->acts, (void*)x->dzs, (void*)x->y })
         if (p) (void)hipFree(p);
     delete x;
     return BBGPU_OK;
@@ -500,8 +672,8 @@ int bbgpu_nrc_destroy(BbNrc* x)
 
 int bbgpu_nrc_get_weights(BbNrc* x, float* out, int32_t count)
 {
-    if (!x || !out || count != bb::kParams) {
-        bb::set_error("bbgpu_nrc_get_weights: invalid argument (count must be %d)", bb::kParams);
+    if (!x || !out || count != x->params) {
+        bb::set_error("bbgpu_nrc_get_weights: invalid argument (count must be %d)", x ? x->params : 0);
         return BBGPU_ERR_INVALID;
     }
     if (!nrc_ok(hipMemcpy(out, x->w, size_t(count) * 4, hipMemcpyDeviceToHost), "hipMemcpy")) return BBGPU_ERR_DEVICE;
@@ -510,8 +682,8 @@ int bbgpu_nrc_get_weights(BbNrc* x, float* out, int32_t count)
 
 int bbgpu_nrc_set_weights(BbNrc* x, const float* in, int32_t count)
 {
-    if (!x || !in || count != bb::kParams) {
-        bb::set_error("bbgpu_nrc_set_weights: invalid argument (count must be %d)", bb::kParams);
+    if (!x || !in || count != x->params) {
+        bb::set_error("bbgpu_nrc_set_weights: invalid argument (count must be %d)", x ? x->params : 0);
         return BBGPU_ERR_INVALID;
     }
     if (!nrc_ok(hipMemcpy(x->w, in, size_t(count) * 4, hipMemcpyHostToDevice), "hipMemcpy") ||
@@ -528,9 +700,10 @@ int bbgpu_nrc_query(BbNrc* x, const float* inputs_dev, int32_t n, float* out_dev
     }
     if (n == 0) return BBGPU_OK;
     hipStream_t st = (hipStream_t)stream;
-    const bb::NrcWeights Wt{ x->WT, x->W };
-    hipLaunchKernelGGL(bb::k_mlp_forward<false>, dim3(nrc_grid(n)), dim3(bb::kNrcBlock), 0, st, inputs_dev, n, Wt,
-                       out_dev, nullptr);
+    if (x->width == bb::NetC::W)
+        launch_query<bb::NetC>(x, inputs_dev, n, out_dev, st);
+    else
+        launch_query<bb::NetL>(x, inputs_dev, n, out_dev, st);
     return nrc_ok(hipGetLastError(), "k_mlp_forward") ? BBGPU_OK : BBGPU_ERR_DEVICE;
 }
 
@@ -542,38 +715,22 @@ int bbgpu_nrc_train(BbNrc* x, const float* inputs_dev, const float* targets_dev,
         return BBGPU_ERR_INVALID;
     }
     hipStream_t st = (hipStream_t)stream;
-    const size_t chunks = (size_t(n) + bb::kRows - 1) / bb::kRows;
-    const size_t need = chunks * size_t(bb::kChunkActs);
-    if (x->acts_rows < size_t(n)) {
-        if (x->acts) (void)hipFree(x->acts);
-        if (x->y) (void)hipFree(x->y);
-        x->acts = nullptr;
-        x->y = nullptr;
-        x->acts_rows = 0;
-        if (!nrc_ok(hipMalloc(&x->acts, need * 2), "hipMalloc") || !nrc_ok(hipMalloc(&x->y, size_t(n) * 12), "hipMalloc"))
-            return BBGPU_ERR_NOMEM;
-        x->acts_rows = size_t(n);
-    }
-    const bb::NrcWeights Wt{ x->WT, x->W };
-    const int grid = nrc_grid(n);
-    hipLaunchKernelGGL(bb::k_fill0, dim3(1), dim3(64), 0, st, x->loss, 1);
-    hipLaunchKernelGGL(bb::k_mlp_forward<true>, dim3(grid), dim3(bb::kNrcBlock), 0, st, inputs_dev, n, Wt, x->y,
-                       x->acts);
-    hipLaunchKernelGGL(bb::k_mlp_backward, dim3(grid), dim3(bb::kNrcBlock), 0, st, x->y, targets_dev, n, Wt, x->acts,
-                       x->partials, x->loss);
-    if (!nrc_ok(hipGetLastError(), "k_mlp_backward")) return BBGPU_ERR_DEVICE;
-    // the gradient (partial sums of the grid's blocks), and the step when apply
+    int splits = 0;
+    const int r = x->width == bb::NetC::W ? train_step<bb::NetC>(x, inputs_dev, targets_dev, n, st, &splits)
+                                          : train_step<bb::NetL>(x, inputs_dev, targets_dev, n, st, &splits);
+    if (r != BBGPU_OK) return r;
+    // the gradient (the row ranges' partial sums), and the step when apply
     float c1 = 1.0f, c2 = 1.0f;
     if (apply) {
         x->step++;
         c1 = 1.0f - std::pow(0.9f, float(x->step));
         c2 = 1.0f - std::pow(0.99f, float(x->step));
     }
-    if (!nrc_sync_bf16(x, st, apply != 0, c1, c2, grid)) return BBGPU_ERR_DEVICE;
+    if (!nrc_sync_bf16(x, st, apply != 0, c1, c2, splits)) return BBGPU_ERR_DEVICE;
     if (loss_out && !nrc_ok(hipMemcpyAsync(loss_out, x->loss, 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync"))
         return BBGPU_ERR_DEVICE;
     if (grad_out &&
-        !nrc_ok(hipMemcpyAsync(grad_out, x->g, size_t(bb::kParams) * 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync"))
+        !nrc_ok(hipMemcpyAsync(grad_out, x->g, size_t(x->params) * 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync"))
         return BBGPU_ERR_DEVICE;
     if ((loss_out || grad_out) && !nrc_ok(hipStreamSynchronize(st), "hipStreamSynchronize")) return BBGPU_ERR_DEVICE;
     return BBGPU_OK;

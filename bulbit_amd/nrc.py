@@ -19,6 +19,7 @@ def _declare(lib):
     if getattr(lib, "_nrc_declared", False):
         return lib
     lib.bbgpu_nrc_param_count.restype = c_int32
+    lib.bbgpu_nrc_param_count.argtypes = [POINTER(BbNrcDesc)]
     lib.bbgpu_nrc_create.argtypes = [POINTER(BbNrcDesc), POINTER(c_void_p)]
     lib.bbgpu_nrc_destroy.argtypes = [c_void_p]
     lib.bbgpu_nrc_get_weights.argtypes = [c_void_p, POINTER(c_float), c_int32]
@@ -45,7 +46,8 @@ class Nrc:
         h = c_void_p()
         capi.check(self.lib.bbgpu_nrc_create(ctypes.byref(d), ctypes.byref(h)), "bbgpu_nrc_create", self.lib)
         self.h = h
-        self.n_params = self.lib.bbgpu_nrc_param_count()
+        self.width, self.layers = width, layers
+        self.n_params = self.lib.bbgpu_nrc_param_count(ctypes.byref(d))
 
     def close(self):
         if getattr(self, "h", None) and self.h.value:

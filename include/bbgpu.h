@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define BBGPU_ABI_VERSION 9
+#define BBGPU_ABI_VERSION 10
 
 /* status codes */
 enum {
@@ -361,14 +361,16 @@ int bbgpu_autotune(BbScene* scene, const BbCameraDesc* camera, const BbRenderPar
 typedef struct BbNrc BbNrc; /* opaque */
 
 typedef struct BbNrcDesc {
-    int32_t hidden_width;   /* 64 */
-    int32_t hidden_layers;  /* 4 */
+    int32_t hidden_width;   /* 64 (with 4 layers) or 128 (with 5 layers) */
+    int32_t hidden_layers;  /* 4 or 5, as above */
     float learning_rate;    /* Adam step size (0: 1e-3) */
     uint32_t seed;          /* initial weights (Xavier uniform from PCG32) */
 } BbNrcDesc;
 
-/* number of fp32 parameters (the weight matrices, k-major, output padded to 16 columns) */
-int bbgpu_nrc_param_count(void);
+/* number of fp32 parameters of the network desc describes (the weight
+ * matrices, k-major, input padded to 80 rows and output to 16 columns);
+ * NULL: the 64 x 4 network; -1: a shape that is not built */
+int bbgpu_nrc_param_count(const BbNrcDesc* desc);
 int bbgpu_nrc_create(const BbNrcDesc* desc, BbNrc** nrc_out);
 int bbgpu_nrc_destroy(BbNrc* nrc);
 int bbgpu_nrc_get_weights(BbNrc* nrc, float* host_out, int32_t count);
@@ -378,7 +380,7 @@ int bbgpu_nrc_query(BbNrc* nrc, const float* inputs_dev, int32_t n, float* out_d
 /* one training step on n rows (targets: 3 floats per row, device pointers):
  * the loss gradient over the batch, then (apply != 0) an Adam update.
  * loss_out and grad_out (host, may be NULL; grad_out holds
- * bbgpu_nrc_param_count() floats) make the call wait for the step. */
+ * bbgpu_nrc_param_count(desc) floats) make the call wait for the step. */
 int bbgpu_nrc_train(BbNrc* nrc, const float* inputs_dev, const float* targets_dev, int32_t n, int32_t apply,
                     float* loss_out, float* grad_out, void* stream);
 

@@ -8,27 +8,39 @@ loss (y - t)^2 / (y^2 + 0.01) averaged over rows and channels, and Adam
 (beta1 0.9, beta2 0.99, eps 1e-8).  The cache has no counterpart in Bulbit;
 `forward64`/`loss64` are the same network in float64 without rounding, for
 finite-difference checks of the gradient formulas.
+
+Two shapes are built, 4 hidden layers of width 64 and 5 of width 128; every
+function that takes flat weights reads the shape from their count.
 """
 from __future__ import annotations
 
 import numpy as np
 
 K_IN, WIDTH, HIDDEN, OUT_COLS = 80, 64, 4, 16
+SHAPES = ((64, 4), (128, 5))  # (hidden width, hidden layers)
 
 
-def layer_shapes():
-    return [(K_IN if l == 0 else WIDTH, OUT_COLS if l == HIDDEN else WIDTH) for l in range(HIDDEN + 1)]
+def layer_shapes(width=WIDTH, hidden=HIDDEN):
+    return [(K_IN if l == 0 else width, OUT_COLS if l == hidden else width) for l in range(hidden + 1)]
 
 
-def param_count():
-    return sum(k * n for k, n in layer_shapes())
+def param_count(width=WIDTH, hidden=HIDDEN):
+    return sum(k * n for k, n in layer_shapes(width, hidden))
+
+
+def shape_of(w):
+    """(width, hidden) of a flat parameter vector"""
+    for width, hidden in SHAPES:
+        if len(w) == param_count(width, hidden):
+            return width, hidden
+    raise ValueError(f"no network shape has {len(w)} parameters")
 
 
 def split(w):
     """flat k-major parameters -> list of [k][n] matrices (dtype kept)"""
     w = np.asarray(w)
     out, off = [], 0
-    for k, n in layer_shapes():
+    for k, n in layer_shapes(*shape_of(w)):
         out.append(w[off:off + k * n].reshape(k, n))
         off += k * n
     return out
@@ -62,12 +74,13 @@ def encode(raw):
 def forward(w, raw):
     """y [n][3] and the bf16 inputs of every layer, as the GPU computes them"""
     W = [bf16(m) for m in split(np.asarray(w, np.float32))]
+    hidden = len(W) - 1
     h = bf16(encode(raw).astype(np.float32))
     hs = [h]
-    for l in range(HIDDEN):
+    for l in range(hidden):
         h = bf16(np.maximum(h @ W[l], 0.0).astype(np.float32))
         hs.append(h)
-    y = (h @ W[HIDDEN])[:, :3].astype(np.float32)
+    y = (h @ W[hidden])[:, :3].astype(np.float32)
     return y, hs
 
 
@@ -75,14 +88,15 @@ def loss_and_grad(w, raw, target):
     """relative L2 loss and its gradient with respect to the flat weights"""
     y, hs = forward(w, raw)
     W = [bf16(m) for m in split(np.asarray(w, np.float32))]
+    hidden = len(W) - 1
     n = len(raw)
     d = y - target
     den = y * y + 0.01
     loss = float(np.sum(d * d / den) / (3 * n))
     dz = np.zeros((n, OUT_COLS), np.float32)
     dz[:, :3] = bf16((2.0 * d / den / (3 * n)).astype(np.float32))
-    grads = [None] * (HIDDEN + 1)
-    for l in range(HIDDEN, -1, -1):
+    grads = [None] * (hidden + 1)
+    for l in range(hidden, -1, -1):
         grads[l] = (hs[l].T.astype(np.float64) @ dz.astype(np.float64)).astype(np.float32)
         if l == 0:
             break
@@ -93,10 +107,11 @@ def loss_and_grad(w, raw, target):
 
 def forward64(w, raw):
     W = split(np.asarray(w, np.float64))
+    hidden = len(W) - 1
     h = encode(raw)
-    for l in range(HIDDEN):
+    for l in range(hidden):
         h = np.maximum(h @ W[l].astype(np.float64), 0.0)
-    return (h @ W[HIDDEN].astype(np.float64))[:, :3]
+    return (h @ W[hidden].astype(np.float64))[:, :3]
 
 
 def loss64(w, raw, target):
@@ -108,17 +123,18 @@ def grad64(w, raw, target):
     """analytic gradient in float64 with the loss denominator held constant
     (the stop-gradient of NRC's relative loss)"""
     W = split(np.asarray(w, np.float64))
+    hidden = len(W) - 1
     h = encode(raw)
     hs = [h]
-    for l in range(HIDDEN):
+    for l in range(hidden):
         h = np.maximum(h @ W[l], 0.0)
         hs.append(h)
-    y = (h @ W[HIDDEN])[:, :3]
+    y = (h @ W[hidden])[:, :3]
     n = len(raw)
     dz = np.zeros((n, OUT_COLS))
     dz[:, :3] = 2.0 * (y - target) / (y * y + 0.01) / (3 * n)
-    grads = [None] * (HIDDEN + 1)
-    for l in range(HIDDEN, -1, -1):
+    grads = [None] * (hidden + 1)
+    for l in range(hidden, -1, -1):
         grads[l] = hs[l].T @ dz
         if l == 0:
             break

@@ -22,7 +22,7 @@ def test_library_exports_every_declared_entry_point(bblib):
     assert names and sorted(capi.EXPORTS) == names
     for n in names:
         assert hasattr(bblib, n), n
-    assert bblib.bbgpu_abi_version() == capi.ABI_VERSION == 9
+    assert bblib.bbgpu_abi_version() == capi.ABI_VERSION == 10
 
 
 def _build(bblib, sc):
@@ -103,3 +103,14 @@ def test_scene_create_reports_missing_device_or_unsupported(bblib):
     else:
         assert st == 2 and "not supported" in capi.last_error(bblib)
     assert not h.value
+
+
+def test_nrc_parameter_counts(bblib):
+    from bulbit_amd.nrc import BbNrcDesc, _declare
+    from oracle import nrc_ref
+
+    _declare(bblib)
+    for (w, l) in nrc_ref.SHAPES:
+        assert bblib.bbgpu_nrc_param_count(ctypes.byref(BbNrcDesc(w, l, 1e-3, 1))) == nrc_ref.param_count(w, l)
+    assert bblib.bbgpu_nrc_param_count(None) == nrc_ref.param_count(64, 4)
+    assert bblib.bbgpu_nrc_param_count(ctypes.byref(BbNrcDesc(96, 4, 1e-3, 1))) == -1

@@ -28,15 +28,19 @@ def _field(raw):
     return np.stack([base, base * 0.8 + 0.1 * raw[:, 8], base * 0.5 + 0.2 * raw[:, 12]], 1).astype(np.float32)
 
 
-def _init_weights(seed):
+SHAPES = [pytest.param(s, id=f"{s[0]}x{s[1]}") for s in ref.SHAPES]
+
+
+def _init_weights(seed, shape=(ref.WIDTH, ref.HIDDEN)):
     rng = np.random.default_rng(seed)
+    hidden = shape[1]
     w = []
-    for l, (k, n) in enumerate(ref.layer_shapes()):
-        a = np.sqrt(6.0 / ((74 if l == 0 else k) + (3 if l == ref.HIDDEN else n)))
+    for l, (k, n) in enumerate(ref.layer_shapes(*shape)):
+        a = np.sqrt(6.0 / ((74 if l == 0 else k) + (3 if l == hidden else n)))
         m = rng.uniform(-a, a, (k, n))
         if l == 0:
             m[74:] = 0
-        if l == ref.HIDDEN:
+        if l == hidden:
             m[:, 3:] = 0
         w.append(m.ravel())
     return np.concatenate(w).astype(np.float32)
@@ -64,12 +68,13 @@ def test_encoding_layout():
     assert np.allclose(e[:, 64:74], raw[:, 3:13]) and np.all(e[:, 74:] == 0)
 
 
-def test_oracle_gradient_matches_finite_differences():
+@pytest.mark.parametrize("shape", SHAPES)
+def test_oracle_gradient_matches_finite_differences(shape):
     """The backward formulas (float64, relative-loss denominator held fixed
     as NRC's stop-gradient) against central differences of the loss."""
     raw = _batch(40, 2)
     t = _field(raw)
-    w = _init_weights(3).astype(np.float64)
+    w = _init_weights(3, shape).astype(np.float64)
     g = ref.grad64(w, raw, t)
     y0 = ref.forward64(w, raw)
 
@@ -78,9 +83,9 @@ def test_oracle_gradient_matches_finite_differences():
         return float(np.mean((y - t) ** 2 / (y0 * y0 + 0.01)))
 
     rng = np.random.default_rng(4)
-    offs = np.cumsum([0] + [k * n for k, n in ref.layer_shapes()])
-    for l, (k, n) in enumerate(ref.layer_shapes()):
-        cols = 3 if l == ref.HIDDEN else n
+    offs = np.cumsum([0] + [k * n for k, n in ref.layer_shapes(*shape)])
+    for l, (k, n) in enumerate(ref.layer_shapes(*shape)):
+        cols = 3 if l == shape[1] else n
         rows = 74 if l == 0 else k
         for _ in range(12):
             i = offs[l] + rng.integers(0, rows) * n + rng.integers(0, cols)
@@ -92,10 +97,11 @@ def test_oracle_gradient_matches_finite_differences():
             assert abs(fd - g[i]) <= 1e-6 + 1e-4 * abs(g[i]), (l, i, fd, g[i])
 
 
-def test_bf16_oracle_tracks_float64():
+@pytest.mark.parametrize("shape", SHAPES)
+def test_bf16_oracle_tracks_float64(shape):
     raw = _batch(256, 5)
     t = _field(raw)
-    w = _init_weights(6)
+    w = _init_weights(6, shape)
     y, _ = ref.forward(w, raw)
     y64 = ref.forward64(w.astype(np.float64), raw)
     assert np.max(np.abs(y - y64)) < 0.05 * max(1.0, np.max(np.abs(y64)))
@@ -107,16 +113,18 @@ def test_bf16_oracle_tracks_float64():
 
 # --------------------------------------------------------------------- GPU
 
-@pytest.fixture(scope="module")
-def nrc():
+@pytest.fixture(scope="module", params=SHAPES)
+def nrc(request):
     from bulbit_amd.nrc import Nrc
 
-    return Nrc(seed=7)
+    net = Nrc(seed=7, width=request.param[0], layers=request.param[1])
+    yield net
+    net.close()
 
 
 @pytest.mark.gpu
 def test_nrc_forward_matches_oracle(nrc):
-    w = _init_weights(8)
+    w = _init_weights(8, (nrc.width, nrc.layers))
     nrc.set_weights(w)
     raw = _batch(1000, 9)  # not a multiple of the 64-row tile
     y = nrc.query(raw)
@@ -129,7 +137,7 @@ def test_nrc_forward_matches_oracle(nrc):
 
 @pytest.mark.gpu
 def test_nrc_gradient_matches_oracle(nrc):
-    w = _init_weights(10)
+    w = _init_weights(10, (nrc.width, nrc.layers))
     nrc.set_weights(w)
     raw = _batch(3000, 11)
     t = _field(raw)
@@ -142,13 +150,13 @@ def test_nrc_gradient_matches_oracle(nrc):
     print(f"[nrc] gradient cosine {cos:.6f}, norm ratio {ratio:.5f}", flush=True)
     assert cos > 0.999 and abs(ratio - 1) < 0.01
     # the padded output columns get no gradient
-    last = np.asarray(g[-64 * 16:]).reshape(64, 16)
+    last = np.asarray(g[-nrc.width * 16:]).reshape(nrc.width, 16)
     assert np.all(last[:, 3:] == 0)
 
 
 @pytest.mark.gpu
 def test_nrc_adam_step_matches_oracle(nrc):
-    w = _init_weights(12)
+    w = _init_weights(12, (nrc.width, nrc.layers))
     nrc.set_weights(w)
     raw = _batch(2048, 13)
     t = _field(raw)
@@ -156,23 +164,24 @@ def test_nrc_adam_step_matches_oracle(nrc):
     nrc.set_weights(w)  # resets nothing of Adam's moments: they are still zero
     from bulbit_amd.nrc import Nrc
 
-    fresh = Nrc(seed=1, learning_rate=1e-3)
+    fresh = Nrc(seed=1, learning_rate=1e-3, width=nrc.width, layers=nrc.layers)
     fresh.set_weights(w)
     fresh.train(raw, t, apply=True)
     w1 = fresh.weights()
     expect, _, _ = ref.adam(w.astype(np.float64), 0.0, 0.0, g.astype(np.float64), 1, 1e-3)
     mask = np.ones_like(w, bool)
-    mask[-64 * 16:] = (np.arange(64 * 16) % 16) < 3
+    mask[-nrc.width * 16:] = (np.arange(nrc.width * 16) % 16) < 3
     assert np.allclose(w1[mask], expect[mask], rtol=1e-5, atol=1e-6)
     assert np.all(w1[~mask] == 0)
     fresh.close()
 
 
 @pytest.mark.gpu
-def test_nrc_learns_a_smooth_field():
+@pytest.mark.parametrize("shape", SHAPES)
+def test_nrc_learns_a_smooth_field(shape):
     from bulbit_amd.nrc import Nrc
 
-    net = Nrc(seed=21, learning_rate=1e-2)
+    net = Nrc(seed=21, learning_rate=1e-2, width=shape[0], layers=shape[1])
     test_raw = _batch(4096, 99)
     test_t = _field(test_raw)
 
