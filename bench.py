@@ -53,6 +53,11 @@ def main():
                     help="only the batched workload: 32 seeded scenes of uneven size in one bbgpu_render_batch call")
     ap.add_argument("--cache-frames", type=int, default=256,
                     help="1-spp frames of the equal-time radiance-cache comparison (0: skip it)")
+    ap.add_argument("--large", action="store_true",
+                    help="the large-scene workload instead (4096 displaced level-5 icospheres, 3840x2160, 4 spp per "
+                         "frame, max_bounces 12, the 128 x 5 radiance cache)")
+    ap.add_argument("--large-spheres", type=int, default=4096)
+    ap.add_argument("--large-frames", type=int, default=64)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (scales the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -75,6 +80,9 @@ def main():
         return
     if args.batch:
         print(json.dumps(batch_workload(args)), flush=True)
+        return
+    if args.large:
+        print(json.dumps(large_workload(args)), flush=True)
         return
 
     log(f"[bench] generating scene level {args.level} at {args.width}x{args.height}")
@@ -226,16 +234,23 @@ def quality(g, sc, cam, args, test):
     return out, ref
 
 
-MLP_FLOPS_QUERY = 2 * (74 * 64 + 3 * 64 * 64 + 64 * 3)  # per query, 74 inputs, 4 x 64 hidden, 3 outputs
+def mlp_flops_query(width=64, layers=4):
+    """FLOPs of one query: 74 inputs, `layers` hidden layers of `width`, 3 outputs"""
+    return 2 * (74 * width + (layers - 1) * width * width + width * 3)
+
+
+MLP_FLOPS_QUERY = mlp_flops_query()
 BF16_PEAK_TFLOPS = 2500.0
 
 
-def cache_comparison(args, g, cam, ref, name, query_primary, train_stride=16, learning_rate=1e-3, train_steps=4):
-    """Radiance cache vs plain path tracing at equal wall-clock time, one
-    sample per frame, against `ref` (a plain render with disjoint samples).
-    Plain PT renders `cache_frames` frames; the cached tracer (trained online
-    on 1/train_stride of its paths, train_steps Adam steps per frame) renders
-    frames until it has used the same time.
+def cache_comparison(args, g, cam, ref, name, query_primary, train_stride=16, learning_rate=1e-3, train_steps=4,
+                     spp=1, max_bounces=8, shape=(64, 4), frames=None, keep_every=16):
+    """Radiance cache vs plain path tracing at equal wall-clock time, `spp`
+    samples per frame, against `ref` (a plain render with disjoint samples).
+    Plain PT renders `frames` (default `cache_frames`) frames; the cached
+    tracer (a network of shape[1] hidden layers of width shape[0], trained
+    online on 1/train_stride of its paths, train_steps Adam steps per frame)
+    renders frames until it has used the same time.
     Two measures: the relMSE of the frames accumulated over the whole time
     (progressive rendering), and the mean relMSE of single frames from the
     second half of the run (real-time rendering), with plain PT's per-frame
@@ -261,7 +276,7 @@ def cache_comparison(args, g, cam, ref, name, query_primary, train_stride=16, le
             step(f, frame)
             acc[0].add_(frame[0])
             acc[1].add_(frame[1])
-            if f % 16 == 15:
+            if f % keep_every == keep_every - 1:
                 keep.append((f, frame[0].clone(), frame[1].clone()))
             torch.cuda.synchronize()
             done = f + 1
@@ -278,22 +293,27 @@ def cache_comparison(args, g, cam, ref, name, query_primary, train_stride=16, le
 
     cp = scenes.cache_params(train_stride=train_stride, train_batch=65536, train_steps=train_steps,
                              query_primary=query_primary)
+    frames = args.cache_frames if frames is None else frames
+    flops_q = mlp_flops_query(*shape)
+
+    def params(f):
+        return scenes.render_params(spp, max_bounces=max_bounces, sample_begin=base + f * spp)
+
     # warm both paths (kernel loading) outside the timed runs
-    g.render(cam, scenes.render_params(1, sample_begin=1 << 24))
-    net = Nrc(seed=1, learning_rate=learning_rate)
-    g.render_cached(net, cam, scenes.render_params(1, sample_begin=1 << 24), cp)
+    warm = scenes.render_params(spp, max_bounces=max_bounces, sample_begin=1 << 24)
+    g.render_device(cam, warm, *(t.data_ptr() for t in g.new_film(cam)))
+    net = Nrc(seed=1, learning_rate=learning_rate, width=shape[0], layers=shape[1])
+    g.render_cached_device(net, cam, warm, cp, *(t.data_ptr() for t in g.new_film(cam)))
     net.close()
 
     e_plain, n_plain, t_plain, f_plain = run(
-        lambda f, film: g.render_device(cam, scenes.render_params(1, sample_begin=base + f), film[0].data_ptr(),
-                                        film[1].data_ptr()), args.cache_frames)
-    net = Nrc(seed=1, learning_rate=learning_rate)
+        lambda f, film: g.render_device(cam, params(f), film[0].data_ptr(), film[1].data_ptr()), frames)
+    net = Nrc(seed=1, learning_rate=learning_rate, width=shape[0], layers=shape[1])
     stats = []
 
     def cached(f, film):
         # device film, no host copy: the same as the plain leg
-        stats.append(g.render_cached_device(net, cam, scenes.render_params(1, sample_begin=base + f), cp,
-                                            film[0].data_ptr(), film[1].data_ptr()))
+        stats.append(g.render_cached_device(net, cam, params(f), cp, film[0].data_ptr(), film[1].data_ptr()))
 
     e_cache, n_cache, t_cache, f_cache = run(cached, 100000, budget=t_plain)
     net.close()
@@ -301,7 +321,7 @@ def cache_comparison(args, g, cam, ref, name, query_primary, train_stride=16, le
     rows = sum(st.train_rows for st in stats)
     ms_q = sum(st.ms_query for st in stats)
     ms_t = sum(st.ms_train for st in stats)
-    tflops = (MLP_FLOPS_QUERY * q + 3 * MLP_FLOPS_QUERY * rows) / max(ms_q + ms_t, 1e-9) / 1e9
+    tflops = (flops_q * q + 3 * flops_q * rows) / max(ms_q + ms_t, 1e-9) / 1e9
     frame_plain = t_plain / n_plain
     frame_cache = t_cache / n_cache
     f_plain_equal_time = f_plain * frame_plain / frame_cache
@@ -310,6 +330,8 @@ def cache_comparison(args, g, cam, ref, name, query_primary, train_stride=16, le
         f"{f_cache:.5f} ({frame_cache * 1e3:.2f} ms); {q} queries, {rows} training rows")
     return {
         "workload": name,
+        "network": {"hidden_width": shape[0], "hidden_layers": shape[1]},
+        "spp_per_frame": spp,
         "query_at_primary_hit": bool(query_primary),
         "train_stride": train_stride,
         "learning_rate": learning_rate,
@@ -327,7 +349,7 @@ def cache_comparison(args, g, cam, ref, name, query_primary, train_stride=16, le
         "cache_ms_train": round(ms_t, 1),
         "mlp": {"bound": "mfma", "achieved": round(tflops, 3), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(tflops / BF16_PEAK_TFLOPS, 6),
-                "flops_per_query": MLP_FLOPS_QUERY, "flops_per_training_row": 3 * MLP_FLOPS_QUERY},
+                "flops_per_query": flops_q, "flops_per_training_row": 3 * flops_q},
     }
 
 
@@ -356,6 +378,103 @@ def cache_comparisons(args, g_c2, cam_c2, ref_c2):
         out["icosphere"] = cache_comparison(args, g_c2, cam_c2, ref_c2,
                                             "displaced icosphere 1920x1080, 1 spp per frame, max_bounces 8", False,
                                             train_stride=16, learning_rate=1e-3)
+    return out
+
+
+def large_workload(args):
+    """The large scene: `--large-spheres` displaced level-5 icospheres (4096:
+    83,886,080 triangles) Poisson-disk placed on a 100 x 100 plane under 1024
+    emissive triangles of log-normal power, metallic-roughness materials;
+    3840x2160, 4 spp per frame, max_bounces 12.  Reports plain-PT Msamples/s,
+    rays/s and the closest-hit kernel's algorithmic bytes/s (with the
+    traversal counters of one frame), then the 128 x 5 radiance cache against
+    plain PT at equal time over `--large-frames` frames, against a
+    `--ref-spp` plain render with disjoint samples."""
+    import numpy as np
+    import torch
+    from bulbit_amd import scenes
+    from bulbit_amd.gpu import GpuScene
+
+    W, H, SPP, MB = 3840, 2160, 4, 12
+    t0 = time.time()
+    sc, cam = scenes.large_scene(width=W, height=H, n_spheres=args.large_spheres)
+    gen_s = time.time() - t0
+    log(f"[bench] large scene generated in {gen_s:.1f}s: {len(sc.prims)} primitives")
+    g = GpuScene(sc)
+    log(f"[bench] BVH {g.info.n_nodes} nodes built in {g.info.build_seconds:.1f}s, "
+        f"{g.info.device_bytes / 1e9:.2f} GB on device")
+    n_prims = len(sc.prims)
+    del sc
+    params = scenes.render_params(SPP, max_bounces=MB)
+    s_dev, c_dev = g.new_film(cam)
+    for i in range(args.warmup):
+        g.render_device(cam, params, s_dev.data_ptr(), c_dev.data_ptr())
+        log(f"[bench] warmup {i + 1}/{args.warmup} done")
+    cst = g.render_device(cam, params, s_dev.data_ptr(), c_dev.data_ptr(), stats=True, count_traversal=True)
+    cl = cst.closest
+    log(f"[bench] counted frame: {cl.rays} closest rays, {cl.nodes / max(cl.rays, 1):.1f} nodes/ray, "
+        f"{cl.prims / max(cl.rays, 1):.2f} prims/ray")
+    torch.cuda.synchronize()
+    stats = []
+    t = time.perf_counter()
+    for i in range(args.steps):
+        s_dev.zero_()
+        c_dev.zero_()
+        stats.append(g.render_device(cam, params, s_dev.data_ptr(), c_dev.data_ptr(), stats=True))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t
+    value = W * H * SPP * args.steps / elapsed / 1e6
+    ms_closest = float(np.mean([s.ms_trace_closest for s in stats]))
+    bytes_frame = BYTES_NODE * cl.nodes + BYTES_PRIM * cl.prims + BYTES_ACCEPT * cl.accepts + BYTES_RAY * cl.rays
+    achieved = bytes_frame / (ms_closest / 1e3) / 1e9
+    rays = stats[0].rays_closest + stats[0].rays_shadow
+    log(f"[bench] plain: {value:.1f} Msamples/s, {elapsed / args.steps * 1e3:.1f} ms per frame (closest "
+        f"{ms_closest:.1f} ms, {achieved:.0f} GB/s algorithmic)")
+    out = {
+        "metric": "Msamples/s at 2160p, 12 bounces, large scene; relMSE at equal time, cache vs plain PT",
+        "value": round(value, 3),
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "%d displaced level-5 icospheres on a plane, 1024 area lights (%d primitives), "
+                               "%dx%d, %d spp, max_bounces %d" % (args.large_spheres, n_prims, W, H, SPP, MB),
+                   "primitives": n_prims, "scene_seconds": round(gen_s, 1),
+                   "bvh_seconds": round(g.info.build_seconds, 1), "device_bytes": int(g.info.device_bytes)},
+        "traversal": {"kernel": "k_trace_closest", "algorithmic_gbs": round(achieved, 1),
+                      "frac_of_hbm_peak": round(achieved / HBM_PEAK_GBS, 4), "ms_per_frame": round(ms_closest, 2),
+                      "nodes_per_ray": round(cl.nodes / max(cl.rays, 1), 2),
+                      "prims_per_ray": round(cl.prims / max(cl.rays, 1), 3)},
+        "rays_per_s": round(rays * args.steps / elapsed, 1),
+    }
+    if args.ref_spp > 0:
+        # the reference in chunks of 64 spp, samples disjoint from every frame
+        rs, rc = g.new_film(cam)
+        t0 = time.time()
+        chunk = 64
+        for c0 in range(0, args.ref_spp, chunk):
+            p = scenes.render_params(min(chunk, args.ref_spp - c0), max_bounces=MB, sample_begin=(1 << 20) + c0)
+            g.render_device(cam, p, rs.data_ptr(), rc.data_ptr())
+            torch.cuda.synchronize()
+            log(f"[bench] reference {c0 + p.spp}/{args.ref_spp} spp ({time.time() - t0:.0f}s)")
+        ref = rs.cpu().numpy().reshape(H, W, 3) / rc.cpu().numpy().reshape(H, W, 1).astype(np.float32)
+        del rs, rc
+        test = s_dev.cpu().numpy().reshape(H, W, 3) / c_dev.cpu().numpy().reshape(H, W, 1).astype(np.float32)
+        out["quality"] = {"epsilon": RELMSE_EPS, "ref_spp": args.ref_spp, "ref_seconds": round(time.time() - t0, 1),
+                          "relmse_4spp_frame_vs_reference": round(relmse(test, ref), 6)}
+        if args.large_frames > 0:
+            name = "large scene %dx%d, %d spp per frame, max_bounces %d" % (W, H, SPP, MB)
+            cq = cache_comparison(args, g, cam, ref, name, False, train_stride=16, learning_rate=1e-3,
+                                  train_steps=4, spp=SPP, max_bounces=MB, shape=(128, 5), frames=args.large_frames,
+                                  keep_every=8)
+            out["relmse_equal_time_cache_vs_plain"] = cq["per_frame"]["ratio_equal_time"]
+            out["relmse_equal_time_cache_vs_plain_accumulated"] = cq["ratio"]
+            out["radiance_cache"] = cq
+    g.close()
     return out
 
 

@@ -447,6 +447,180 @@ def icosphere_scene(width=1920, height=1080, level=8, seed=2026, n_lights=64):
     return sc, cam
 
 
+def poisson_disk(n, half_extent, min_dist, seed, k=30):
+    """n points in [-half_extent, half_extent]^2, pairwise at least min_dist
+    apart: Bridson's dart throwing (annulus candidates around active points,
+    grid of cell size min_dist / sqrt(2)), seeded.  Raises if the square is
+    full before n points."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    size = 2.0 * half_extent
+    cell = min_dist / np.sqrt(2.0)
+    gn = int(np.ceil(size / cell))
+    grid = -np.ones((gn, gn), np.int64)
+    pts = []
+
+    def cell_of(p):
+        return min(int((p[0] + half_extent) / cell), gn - 1), min(int((p[1] + half_extent) / cell), gn - 1)
+
+    def fits(p):
+        if not (-half_extent <= p[0] < half_extent and -half_extent <= p[1] < half_extent):
+            return False
+        cx, cy = cell_of(p)
+        for ix in range(max(cx - 2, 0), min(cx + 3, gn)):
+            for iy in range(max(cy - 2, 0), min(cy + 3, gn)):
+                j = grid[ix, iy]
+                if j >= 0 and (pts[j][0] - p[0]) ** 2 + (pts[j][1] - p[1]) ** 2 < min_dist * min_dist:
+                    return False
+        return True
+
+    def add(p):
+        grid[cell_of(p)] = len(pts)
+        pts.append(p)
+        active.append(len(pts) - 1)
+
+    active = []
+    add(rng.uniform(-half_extent, half_extent, 2))
+    while active and len(pts) < n:
+        a = int(rng.integers(0, len(active)))
+        base = pts[active[a]]
+        for _ in range(k):
+            r = min_dist * np.sqrt(rng.uniform(1.0, 4.0))
+            t = rng.uniform(0.0, 2.0 * np.pi)
+            p = base + r * np.array([np.cos(t), np.sin(t)])
+            if fits(p):
+                add(p)
+                break
+        else:
+            active[a] = active[-1]
+            active.pop()
+    if len(pts) < n:
+        raise ValueError(f"only {len(pts)} points of spacing {min_dist} fit in the square")
+    return np.array(pts[:n])
+
+
+LARGE_DEFAULTS = dict(
+    n_spheres=4096,
+    level=5,
+    half_extent=50.0,
+    min_dist=1.2,
+    radius=(0.3, 0.55),
+    amplitude=0.05,
+    noise_frequency=3.0,
+    n_lights=1024,
+    light_height=(8.0, 14.0),
+    light_size=0.8,
+    power_sigma=1.5,
+    radiance_scale=8.0,
+    seed=5150,
+)
+
+
+def large_scene(width=3840, height=2160, n_spheres=4096, level=5, n_lights=1024, seed=5150, batch=64):
+    """Many displaced icospheres on a plane under many emissive triangles.
+
+    * n_spheres icospheres of subdivision `level` (4096 at level 5: 83,886,080
+      triangles), centres on the plane y = 0 placed by seeded Poisson-disk
+      sampling over [-50, 50]^2 with spacing 1.2, radius U[0.3, 0.55], each
+      displaced radially by 1 + 0.05 * noise(3 p + offset) (value noise, a
+      random offset per sphere) and raised by radius * 1.05;
+    * smooth vertex normals (area-weighted face normals), tangents orthogonal
+      to them;
+    * one metallic-roughness material per sphere: metallic 0 or 1 (even odds),
+      roughness U[0.05, 1], base colour U[0.2, 0.9] per channel;
+    * ground: 2 triangles at y = 0 over [-55, 55]^2, metallic-roughness
+      (grey 0.5, metallic 0, roughness 0.9);
+    * n_lights one-sided emissive triangles facing down, centres uniform over
+      the plane at heights U[8, 14], circumradius 0.8; power log-normal with
+      sigma 1.5 (equal areas, so radiance = 8 exp(N(0, 1.5^2)));
+    * camera at (0, 10, 58) looking at (0, 0, 10), vfov 40, pinhole.
+    Primitive order: spheres (sphere by sphere), ground, lights.  The
+    spheres' vertex arrays are built in batches of `batch` spheres.
+    """
+    import scipy.sparse
+
+    cfg = dict(LARGE_DEFAULTS)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    amp = cfg["amplitude"]
+    v, f = icosphere(level)
+    V, F = len(v), len(f)
+    centres = poisson_disk(n_spheres, cfg["half_extent"], cfg["min_dist"], seed + 1)
+    radii = rng.uniform(*cfg["radius"], n_spheres)
+    offsets = rng.uniform(0.0, 100.0, (n_spheres, 3))
+    metallic = rng.integers(0, 2, n_spheres).astype(np.float64)
+    rough = rng.uniform(0.05, 1.0, n_spheres)
+    colours = rng.uniform(0.2, 0.9, (n_spheres, 3))
+    # vertex-face incidence: smooth normal = sum of the adjacent face normals
+    inc = scipy.sparse.csr_matrix((np.ones(3 * F), (f.T.ravel(), np.tile(np.arange(F), 3))), shape=(V, F))
+
+    pos = np.empty((n_spheres * V, 3), F32)
+    nrm = np.empty((n_spheres * V, 3), F32)
+    tan = np.empty((n_spheres * V, 3), F32)
+
+    def cross(a, b):
+        return np.stack([a[..., 1] * b[..., 2] - a[..., 2] * b[..., 1], a[..., 2] * b[..., 0] - a[..., 0] * b[..., 2],
+                         a[..., 0] * b[..., 1] - a[..., 1] * b[..., 0]], -1)
+
+    def build(s0):
+        s1 = min(s0 + batch, n_spheres)
+        S = s1 - s0
+        p = (v[None] * cfg["noise_frequency"] + offsets[s0:s1, None, :]).reshape(-1, 3)
+        disp = (1.0 + amp * value_noise3(p, seed + 2)).reshape(S, V)
+        P = v[None] * (radii[s0:s1, None, None] * disp[..., None])
+        P[..., 0] += centres[s0:s1, 0, None]
+        P[..., 1] += (radii[s0:s1] * (1.0 + amp))[:, None]
+        P[..., 2] += centres[s0:s1, 1, None]
+        fn = cross(P[:, f[:, 1]] - P[:, f[:, 0]], P[:, f[:, 2]] - P[:, f[:, 0]])  # (S, F, 3)
+        vn = (inc @ fn.transpose(1, 0, 2).reshape(F, 3 * S)).reshape(V, S, 3).transpose(1, 0, 2).reshape(-1, 3)
+        vn /= np.linalg.norm(vn, axis=1, keepdims=True)
+        pos[s0 * V:s1 * V] = P.reshape(-1, 3)
+        nrm[s0 * V:s1 * V] = vn
+        tan[s0 * V:s1 * V] = _tangents_for(vn)
+
+    # numpy releases the GIL in the array work: batches run on threads
+    import concurrent.futures
+    import os
+
+    with concurrent.futures.ThreadPoolExecutor(min(16, os.cpu_count() or 1)) as ex:
+        list(ex.map(build, range(0, n_spheres, batch)))
+    idx = (f[None].astype(np.int64) + (np.arange(n_spheres, dtype=np.int64) * V)[:, None, None]).reshape(-1, 3)
+
+    sc = SceneData()
+    mats = np.array([sc.add_material(metallic_roughness(tuple(colours[i]), float(metallic[i]), float(rough[i])))
+                     for i in range(n_spheres)], np.int32)
+    sc.add_mesh(pos, nrm, tan, idx, np.repeat(mats, F))
+    del pos, nrm, tan, idx
+
+    e = cfg["half_extent"] + 5.0
+    gpos = np.array([[-e, 0, -e], [e, 0, -e], [e, 0, e], [-e, 0, e]], np.float64)
+    gmat = sc.add_material(metallic_roughness((0.5, 0.5, 0.5), 0.0, 0.9))
+    sc.add_mesh(gpos, np.tile([0.0, 1.0, 0.0], (4, 1)), np.tile([1.0, 0.0, 0.0], (4, 1)),
+                np.array([[0, 2, 1], [0, 3, 2]]), gmat)
+
+    black = sc.add_material(diffuse((0.0, 0.0, 0.0)))
+    lc = np.stack([rng.uniform(-cfg["half_extent"], cfg["half_extent"], n_lights),
+                   rng.uniform(*cfg["light_height"], n_lights),
+                   rng.uniform(-cfg["half_extent"], cfg["half_extent"], n_lights)], 1)
+    ang = rng.uniform(0.0, 2.0 * np.pi, n_lights)
+    power = np.exp(rng.normal(0.0, cfg["power_sigma"], n_lights))
+    s = cfg["light_size"]
+    tri = np.empty((n_lights, 3, 3))
+    for c in range(3):
+        a = ang + c * 2.0 * np.pi / 3.0
+        # wound so that Cross(e1, e2) points down (-y)
+        tri[:, c] = lc + s * np.stack([np.cos(a), np.zeros(n_lights), np.sin(a)], 1)
+    down = np.tile([0.0, -1.0, 0.0], (3 * n_lights, 1))
+    side = np.tile([1.0, 0.0, 0.0], (3 * n_lights, 1))
+    prims = sc.add_mesh(tri.reshape(-1, 3), down, side, np.arange(3 * n_lights).reshape(-1, 3), black)
+    for i in range(n_lights):
+        L = cfg["radiance_scale"] * float(power[i])
+        sc.add_area_light(prims[i], (L, L, L))
+
+    cam_pos = (0.0, 10.0, 58.0)
+    target = (0.0, 0.0, 10.0)
+    cam = make_camera(cam_pos, target, 40.0, 0.0, dist32(cam_pos, target), width, height)
+    return sc, cam
+
+
 def render_params(spp, max_bounces=8, seed=0, rr_min_bounces=1, sample_begin=0, wavefront_paths=0, trace_lds_stack=0,
                   trace_refill=0, shadow_refill=0, graph=0):
     p = capi.BbRenderParams()

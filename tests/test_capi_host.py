@@ -51,7 +51,8 @@ def _mixed_scene():
     lambda: scenes.icosphere_scene(32, 18, level=4)[0],
     lambda: scenes.icosphere_scene(32, 18, level=7)[0],
     _mixed_scene,
-], ids=["readme", "ico4", "ico7", "mixed"])
+    lambda: scenes.large_scene(64, 36, n_spheres=128, level=2, n_lights=64)[0],
+], ids=["readme", "ico4", "ico7", "mixed", "large"])
 def test_host_bvh_identical_to_oracle(bblib, make):
     from oracle import OracleScene
 
@@ -114,3 +115,25 @@ def test_nrc_parameter_counts(bblib):
         assert bblib.bbgpu_nrc_param_count(ctypes.byref(BbNrcDesc(w, l, 1e-3, 1))) == nrc_ref.param_count(w, l)
     assert bblib.bbgpu_nrc_param_count(None) == nrc_ref.param_count(64, 4)
     assert bblib.bbgpu_nrc_param_count(ctypes.byref(BbNrcDesc(96, 4, 1e-3, 1))) == -1
+
+
+def test_large_scene_generator():
+    """Counts, placement (no two spheres overlap, all above the plane),
+    one-sided lights facing down, log-normal light powers."""
+    n, level, nl = 200, 2, 400
+    sc, cam = scenes.large_scene(64, 36, n_spheres=n, level=level, n_lights=nl)
+    tris = 20 * 4 ** level
+    assert len(sc.prims) == n * tris + 2 + nl and len(sc.lights) == nl
+    verts = sc.positions[: n * (10 * 4 ** level + 2)].reshape(n, -1, 3).astype(np.float64)
+    centre = verts.mean(axis=1)
+    radius = np.linalg.norm(verts - centre[:, None], axis=2).max(axis=1)
+    d = np.linalg.norm(centre[:, None, [0, 2]] - centre[None, :, [0, 2]], axis=2) + np.eye(n) * 1e9
+    assert np.all(d >= radius[:, None] + radius[None, :])
+    assert verts[..., 1].min() > 0.0
+    li = sc.indices[-nl:]
+    P = sc.positions.astype(np.float64)
+    nrm = np.cross(P[li[:, 1]] - P[li[:, 0]], P[li[:, 2]] - P[li[:, 0]])
+    assert np.all(nrm[:, 1] < 0) and np.all(np.abs(nrm[:, [0, 2]]) < 1e-6 * np.abs(nrm[:, 1:2]))
+    logp = np.log(np.array([L.spectrum[0] for L in sc.lights]) / scenes.LARGE_DEFAULTS["radiance_scale"])
+    assert abs(logp.mean()) < 0.25 and abs(logp.std() - 1.5) < 0.15
+    assert all(m.type == capi.BB_MAT_METALLIC_ROUGHNESS for m in sc.materials[: n + 1])

@@ -319,6 +319,30 @@ def test_render_compensated_dielectrics_parity():
     assert np.all(np.isfinite(gi)) and gi.mean() > (s0 / c0[..., None]).mean()
 
 
+def test_large_scene_generator_parity():
+    """The large-scene generator at a reduced size (256 level-3 spheres, 128
+    log-normal lights): hit ids bit-exact on random and bounded rays (closest
+    and any-hit), and a 12-bounce render of the metallic-roughness spheres
+    within 1e-3 of the oracle."""
+    from bulbit_amd.gpu import GpuScene
+    from oracle import OracleScene
+
+    sc, cam = scenes.large_scene(width=200, height=112, n_spheres=256, level=3, n_lights=128)
+    g, o = GpuScene(sc), OracleScene(sc)
+    bmin, bmax = o.world_bounds()
+    _check_trace(g, o, scenes.random_rays(bmin, bmax, 300_000, seed=31))
+    rng = np.random.default_rng(32)
+    bounded = scenes.random_rays(bmin, bmax, 300_000, seed=33, t_max=rng.uniform(0, 20, 300_000))
+    _check_trace(g, o, bounded)
+    _check_trace(g, o, bounded, any_hit=True)
+    gi, oi, gc, oc = _render_pair(g, o, cam, scenes.render_params(8, max_bounces=12))
+    assert np.array_equal(gc, oc)
+    frac = pixel_agreement(gi, oi)
+    log(f"large-scene generator 200x112x8, 12 bounces: {frac:.6f} within 1e-3, {np.mean(gi == oi):.4f} bit-equal")
+    assert frac >= 0.999
+    assert gi.mean() > 0.01  # the lights reach the camera
+
+
 def test_render_full_scene_crop_parity(c2):
     """The benchmark frame itself: GPU image rows against the oracle on a
     strided set of rows at 2 spp."""
