@@ -53,6 +53,7 @@ CPP = os.path.join(HERE, "cpp")
 CPP_LIB = os.path.join(HERE, "libbulbit.so")
 CPP_EXAMPLE = os.path.join(CPP, "_build", "readme_example")
 CPP_WRITE_IMAGES = os.path.join(CPP, "_build", "write_images")
+CPP_TEXTURED = os.path.join(CPP, "_build", "textured_example")
 # host C++ API (bulbit/bulbit.h) over the C ABI: plain g++, same FP rules
 CXX_FLAGS = ["-std=c++17", "-O2", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-pthread"]
 
@@ -90,6 +91,8 @@ def build_cpp(verbose: bool = False) -> str:
           "-L", HERE, "-lbulbit", "-lbbgpu", "-Wl,-rpath,$ORIGIN/../.."], "g++ building the README example", verbose)
     _run([cxx, *CXX_FLAGS, "-I", inc, os.path.join(CPP, "examples", "write_images.cpp"), "-o", CPP_WRITE_IMAGES,
           "-L", HERE, "-lbulbit", "-lbbgpu", "-Wl,-rpath,$ORIGIN/../.."], "g++ building write_images", verbose)
+    _run([cxx, *CXX_FLAGS, "-I", inc, os.path.join(CPP, "examples", "textured.cpp"), "-o", CPP_TEXTURED,
+          "-L", HERE, "-lbulbit", "-lbbgpu", "-Wl,-rpath,$ORIGIN/../.."], "g++ building the textured example", verbose)
     return CPP_LIB
 
 

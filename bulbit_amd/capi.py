@@ -12,7 +12,7 @@ from ctypes import POINTER, c_char_p, c_double, c_float, c_int32, c_size_t, c_ui
 
 import numpy as np
 
-ABI_VERSION = 10  # BBGPU_ABI_VERSION of include/bbgpu.h that these structures mirror
+ABI_VERSION = 11  # BBGPU_ABI_VERSION of include/bbgpu.h that these structures mirror
 
 LIB_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(LIB_DIR, "libbbgpu.so")
@@ -53,6 +53,30 @@ class BbMaterial(ctypes.Structure):
         ("u_roughness", c_float),
         ("v_roughness", c_float),
         ("metallic", c_float),
+        ("tex_color", c_int32),
+        ("tex_u_roughness", c_int32),
+        ("tex_v_roughness", c_int32),
+        ("tex_metallic", c_int32),
+        ("tex_normal", c_int32),
+        ("tex_alpha", c_int32),
+    ]
+
+
+BB_TEX_CONSTANT, BB_TEX_IMAGE, BB_TEX_CHECKER = 0, 1, 2
+BB_TEXCOORD_REPEAT, BB_TEXCOORD_CLAMP = 0, 1
+
+
+class BbTexture(ctypes.Structure):
+    _fields_ = [
+        ("type", c_int32),
+        ("channels", c_int32),
+        ("width", c_int32),
+        ("height", c_int32),
+        ("filter", c_int32),
+        ("checker", c_int32 * 2),
+        ("resolution", c_float * 2),
+        ("value", c_float * 3),
+        ("texels", POINTER(c_float)),
     ]
 
 
@@ -88,6 +112,8 @@ class BbSceneDesc(ctypes.Structure):
         ("materials", POINTER(BbMaterial)),
         ("n_lights", c_int32),
         ("lights", POINTER(BbLight)),
+        ("n_textures", c_int32),
+        ("textures", POINTER(BbTexture)),
     ]
 
 

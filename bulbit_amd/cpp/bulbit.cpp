@@ -14,6 +14,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <memory>
+#include <type_traits>
 
 namespace bulbit
 {
@@ -45,6 +47,9 @@ T constant_of(const Texture<T>* tex, T fallback, const char* what)
     return v;
 }
 
+float component(Float v, int) { return v; }
+float component(const Spectrum& v, int k) { return v[k]; }
+
 void put3(float* dst, const Spectrum& s)
 {
     dst[0] = s.r;
@@ -71,7 +76,67 @@ struct SceneExport
     std::vector<BbPrimitive> prims;
     std::vector<BbMaterial> materials;
     std::vector<BbLight> lights;
+    std::vector<BbTexture> textures;
+    std::vector<std::unique_ptr<std::vector<float>>> texels; // stable storage of image texels
+    std::map<const void*, int32_t> texture_slot;             // texture -> index + 1
     std::map<const Mesh*, int32_t> mesh_base;
+
+    // a texture as a BbTexture; returns its slot (index + 1)
+    template <typename T>
+    int32_t add_texture(const Texture<T>* t)
+    {
+        auto it = texture_slot.find(t);
+        if (it != texture_slot.end()) return it->second;
+        constexpr int32_t channels = std::is_same<T, Float>::value ? 1 : 3;
+        BbTexture b;
+        std::memset(&b, 0, sizeof(b));
+        b.channels = channels;
+        T v{};
+        if (t->Constant(&v)) {
+            b.type = BB_TEX_CONSTANT;
+            for (int k = 0; k < channels; ++k) b.value[k] = component(v, k);
+        } else if (auto* im = dynamic_cast<const ImageTexture<T>*>(t)) {
+            if (!im->image) fatal("ImageTexture", "empty image");
+            b.type = BB_TEX_IMAGE;
+            b.width = im->image.width;
+            b.height = im->image.height;
+            b.filter = im->texcoord_filter == TexCoordFilter::clamp ? BB_TEXCOORD_CLAMP : BB_TEXCOORD_REPEAT;
+            auto store = std::make_unique<std::vector<float>>();
+            store->reserve(im->image.data.size() * channels);
+            for (const T& x : im->image.data)
+                for (int k = 0; k < channels; ++k) store->push_back(component(x, k));
+            b.texels = store->data();
+            texels.push_back(std::move(store));
+        } else if (auto* ch = dynamic_cast<const CheckerTexture<T>*>(t)) {
+            b.type = BB_TEX_CHECKER;
+            b.checker[0] = add_texture(ch->a);
+            b.checker[1] = add_texture(ch->b);
+            b.resolution[0] = ch->resolution.x;
+            b.resolution[1] = ch->resolution.y;
+        } else {
+            fatal("scene export", "texture type not supported on the GPU path");
+        }
+        textures.push_back(b);
+        const int32_t slot = int32_t(textures.size());
+        texture_slot[t] = slot;
+        return slot;
+    }
+
+    // a material parameter: its constant value (slot 0), or its texture's slot
+    template <typename T>
+    int32_t param(const Texture<T>* t, T fallback, T& value)
+    {
+        value = fallback;
+        if (!t || t->Constant(&value)) return 0;
+        return add_texture(t);
+    }
+
+    void param3(const SpectrumTexture* t, Spectrum fallback, float* dst, int32_t& slot)
+    {
+        Spectrum v;
+        slot = param(t, fallback, v);
+        put3(dst, v);
+    }
     std::map<const Material*, int32_t> material_index;
     std::map<const Primitive*, int32_t> prim_index;
 
@@ -84,18 +149,20 @@ struct SceneExport
         std::memset(&b, 0, sizeof(b));
         if (auto* d = dynamic_cast<const DiffuseMaterial*>(m)) {
             b.type = BB_MAT_DIFFUSE;
-            put3(b.color, constant_of<Spectrum>(d->reflectance, Spectrum(0), "DiffuseMaterial reflectance"));
+            param3(d->reflectance, Spectrum(0), b.color, b.tex_color);
+            // a roughness texture selects the EON BxDF, outside the path
             b.u_roughness = b.v_roughness = constant_of<Float>(d->roughness, 0.0f, "DiffuseMaterial roughness");
-            if (d->normal || d->alpha) fatal("DiffuseMaterial", "normal and alpha textures are not supported");
+            if (d->normal) b.tex_normal = add_texture(d->normal);
+            if (d->alpha) b.tex_alpha = add_texture(d->alpha);
         } else if (auto* e = dynamic_cast<const DielectricMaterial*>(m)) {
             b.type = BB_MAT_DIELECTRIC;
             b.energy_compensation = e->energy_compensation ? 1 : 0;
             b.eta[0] = b.eta[1] = b.eta[2] = e->eta;
             if (!e->reflectance) fatal("DielectricMaterial", "reflectance texture is required");
-            put3(b.color, constant_of<Spectrum>(e->reflectance, Spectrum(1), "DielectricMaterial reflectance"));
-            b.u_roughness = constant_of<Float>(e->u_roughness, 0.0f, "DielectricMaterial u_roughness");
-            b.v_roughness = constant_of<Float>(e->v_roughness, 0.0f, "DielectricMaterial v_roughness");
-            if (e->normal) fatal("DielectricMaterial", "normal textures are not supported");
+            param3(e->reflectance, Spectrum(1), b.color, b.tex_color);
+            b.tex_u_roughness = param(e->u_roughness, 0.0f, b.u_roughness);
+            b.tex_v_roughness = param(e->v_roughness, 0.0f, b.v_roughness);
+            if (e->normal) b.tex_normal = add_texture(e->normal);
         } else if (auto* c = dynamic_cast<const ConductorMaterial*>(m)) {
             b.energy_compensation = c->energy_compensation ? 1 : 0;
             if (c->eta) {
@@ -106,18 +173,19 @@ struct SceneExport
                 b.type = BB_MAT_CONDUCTOR_R;
                 put3(b.k, constant_of<Spectrum>(c->k, Spectrum(0), "ConductorMaterial R"));
             }
-            put3(b.color, constant_of<Spectrum>(c->reflectance, Spectrum(1), "ConductorMaterial reflectance"));
-            b.u_roughness = constant_of<Float>(c->u_roughness, 0.0f, "ConductorMaterial u_roughness");
-            b.v_roughness = constant_of<Float>(c->v_roughness, 0.0f, "ConductorMaterial v_roughness");
-            if (c->normal || c->alpha) fatal("ConductorMaterial", "normal and alpha textures are not supported");
+            param3(c->reflectance, Spectrum(1), b.color, b.tex_color);
+            b.tex_u_roughness = param(c->u_roughness, 0.0f, b.u_roughness);
+            b.tex_v_roughness = param(c->v_roughness, 0.0f, b.v_roughness);
+            if (c->normal) b.tex_normal = add_texture(c->normal);
+            if (c->alpha) b.tex_alpha = add_texture(c->alpha);
         } else if (auto* r = dynamic_cast<const MetallicRoughnessMaterial*>(m)) {
             b.type = BB_MAT_METALLIC_ROUGHNESS;
-            put3(b.color, constant_of<Spectrum>(r->basecolor, Spectrum(0), "MetallicRoughnessMaterial basecolor"));
-            b.metallic = constant_of<Float>(r->metallic, 0.0f, "MetallicRoughnessMaterial metallic");
-            b.u_roughness = constant_of<Float>(r->u_roughness, 0.0f, "MetallicRoughnessMaterial u_roughness");
-            b.v_roughness = constant_of<Float>(r->v_roughness, 0.0f, "MetallicRoughnessMaterial v_roughness");
-            if (r->normal || r->alpha)
-                fatal("MetallicRoughnessMaterial", "normal and alpha textures are not supported");
+            param3(r->basecolor, Spectrum(0), b.color, b.tex_color);
+            b.tex_metallic = param(r->metallic, 0.0f, b.metallic);
+            b.tex_u_roughness = param(r->u_roughness, 0.0f, b.u_roughness);
+            b.tex_v_roughness = param(r->v_roughness, 0.0f, b.v_roughness);
+            if (r->normal) b.tex_normal = add_texture(r->normal);
+            if (r->alpha) b.tex_alpha = add_texture(r->alpha);
         } else {
             fatal("scene export", "material type not supported on the GPU path");
         }
@@ -231,6 +299,8 @@ struct SceneExport
         d.materials = materials.data();
         d.n_lights = int32_t(lights.size());
         d.lights = lights.data();
+        d.n_textures = int32_t(textures.size());
+        d.textures = textures.data();
         return d;
     }
 };

@@ -6,8 +6,9 @@
 // IndependentSampler / PerspectiveCamera / BoxFilter / Film / Rendering API
 // compiles against this header unchanged and renders with the HIP wavefront
 // path tracer.  The covered surface is exactly the path the GPU library
-// implements: constant textures; Diffuse (Lambertian), Dielectric, Conductor
-// and MetallicRoughness materials; spheres and triangle meshes; spot, uniform
+// implements: constant, image (bilinear) and checker textures in every
+// material parameter the path reads, normal maps and alpha tests; Diffuse
+// (Lambertian), Dielectric, Conductor and MetallicRoughness materials; spheres and triangle meshes; spot, uniform
 // infinite and diffuse area lights; box, tent and Gaussian filters.  Anything outside it is
 // reported by the C ABI as unsupported and stops the program with a message
 // (there is no CPU fallback).
@@ -207,7 +208,34 @@ struct Spectrum
     Float& operator[](int i) { return i == 0 ? r : (i == 1 ? g : b); }
 };
 
+// -------------------------------------------------------------------- image
+
+template <typename T>
+class Image
+{
+public:
+    using Type = T;
+    Image() = default;
+    Image(int32 width, int32 height) : width{ width }, height{ height }, data(size_t(width) * size_t(height)) {}
+
+    T& operator[](int32 i) { return data[size_t(i)]; }
+    const T& operator[](int32 i) const { return data[size_t(i)]; }
+    T& operator()(int32 x, int32 y) { return data[size_t(y) * size_t(width) + size_t(x)]; }
+    const T& operator()(int32 x, int32 y) const { return data[size_t(y) * size_t(width) + size_t(x)]; }
+    explicit operator bool() const { return !data.empty(); }
+
+    int32 width = 0, height = 0;
+    std::vector<T> data;
+};
+
 // ----------------------------------------------------------------- textures
+
+// TexCoordFilter (include/bulbit/texture.h:20-24)
+enum TexCoordFilter
+{
+    repeat,
+    clamp,
+};
 
 template <typename T>
 class Texture
@@ -216,6 +244,42 @@ public:
     virtual ~Texture() = default;
     // the value of a constant texture; false for any other kind
     virtual bool Constant(T* value) const = 0;
+};
+
+// ImageTexture (include/bulbit/textures.h:35-107): bilinear lookups of the
+// image, texel (x, y) at x + y * width; the scene's texture pool creates it
+// with TexCoordFilter::repeat (textures.h:238-251)
+template <typename T>
+class ImageTexture : public Texture<T>
+{
+public:
+    ImageTexture(Image<T> image, TexCoordFilter texcoord_filter = TexCoordFilter::repeat)
+        : image{ std::move(image) }, texcoord_filter{ texcoord_filter }
+    {
+    }
+    int32 GetWidth() const { return image.width; }
+    int32 GetHeight() const { return image.height; }
+    bool Constant(T*) const override { return false; }
+
+    Image<T> image;
+    TexCoordFilter texcoord_filter;
+};
+
+// CheckerTexture (include/bulbit/textures.h:109-142): a where the integer
+// parts of uv * resolution sum to an odd number, else b
+template <typename T>
+class CheckerTexture : public Texture<T>
+{
+public:
+    CheckerTexture(const Texture<T>* a, const Texture<T>* b, const Point2& resolution)
+        : a{ a }, b{ b }, resolution{ resolution }
+    {
+    }
+    bool Constant(T*) const override { return false; }
+
+    const Texture<T>* a;
+    const Texture<T>* b;
+    Point2 resolution;
 };
 
 template <typename T>
@@ -754,26 +818,6 @@ public:
 
     Transform transform;
     Float vfov, aperture_radius, focus_distance;
-};
-
-// -------------------------------------------------------------------- image
-
-template <typename T>
-class Image
-{
-public:
-    using Type = T;
-    Image() = default;
-    Image(int32 width, int32 height) : width{ width }, height{ height }, data(size_t(width) * size_t(height)) {}
-
-    T& operator[](int32 i) { return data[size_t(i)]; }
-    const T& operator[](int32 i) const { return data[size_t(i)]; }
-    T& operator()(int32 x, int32 y) { return data[size_t(y) * size_t(width) + size_t(x)]; }
-    const T& operator()(int32 x, int32 y) const { return data[size_t(y) * size_t(width) + size_t(x)]; }
-    explicit operator bool() const { return !data.empty(); }
-
-    int32 width = 0, height = 0;
-    std::vector<T> data;
 };
 
 using Image3 = Image<Spectrum>;

@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
+#include <cstring>
 #include <cmath>
 #include <cstdint>
 
@@ -31,6 +32,7 @@ constexpr float kPi = 3.14159265358979323846f;
 constexpr float kTwoPi = 2.0f * kPi;
 constexpr float kFourPi = 4.0f * kPi;
 constexpr float kInvPi = 1.0f / kPi;
+constexpr float kInvTwoPi = 1.0f / (2.0f * kPi);
 constexpr float kInvFourPi = 1.0f / (4.0f * kPi);
 constexpr float kEps = FLT_EPSILON;
 constexpr float kRayEps = 1e-4f; // Ray::epsilon, include/bulbit/ray.h:19
@@ -217,6 +219,32 @@ BB_HD uint64_t pixel_sample_hash(int32_t px, int32_t py, int32_t seed, int32_t s
     h *= m;
     h ^= h >> r;
     return h;
+}
+
+// HashFloat(ray, point) (hash.h:111-126): MurmurHash64A, seed 0, over the
+// 36 bytes of Ray{o, d} followed by the point, low 32 bits times 2^-32
+BB_HD float hash_float_ray_point(f3 o, f3 d, f3 p)
+{
+    const uint64_t m = 0xc6a4a7935bd1e995ull;
+    const int r = 47;
+    const float w[9] = { o.x, o.y, o.z, d.x, d.y, d.z, p.x, p.y, p.z };
+    uint32_t b[9];
+    for (int i = 0; i < 9; ++i) memcpy(&b[i], &w[i], 4);
+    uint64_t h = 0ull ^ (36ull * m);
+    for (int i = 0; i < 4; ++i) {
+        uint64_t k = uint64_t(b[2 * i]) | (uint64_t(b[2 * i + 1]) << 32);
+        k *= m;
+        k ^= k >> r;
+        k *= m;
+        h ^= k;
+        h *= m;
+    }
+    h ^= uint64_t(b[8]); // the 4-byte tail, little-endian
+    h *= m;
+    h ^= h >> r;
+    h *= m;
+    h ^= h >> r;
+    return float(uint32_t(h)) * 0x1p-32f;
 }
 
 // PCG32 (random.h:27-46, 87-90); inc = (1 << 1) | 1 = 3 for initseq 1

@@ -37,7 +37,8 @@ enum : int32_t
     KIND_TRIANGLE = 0,
     KIND_SPHERE = 1,
     KIND_MASK = 0xff,
-    kSlotLast = 0x100 // on the last slot of a leaf
+    kSlotLast = 0x100, // on the last slot of a leaf
+    kSlotAlpha = 0x200 // the slot's material has an alpha texture
 };
 
 // BxDF flags, include/bulbit/bxdf.h:7-22
@@ -102,7 +103,23 @@ struct GpuMaterial
     spec color;  // metallic-roughness base colour
     float metallic;
     int32_t flags_front; // BxDF flags (front and back face give the same flags)
-    float pad[2];
+    int32_t type;        // BB_MAT_* of the description
+    int32_t energy_compensation;
+    // texture slots (index into SceneView::textures, -1: the constants above)
+    int32_t tex_color, tex_u, tex_v, tex_metallic, tex_normal, tex_alpha;
+    int32_t textured;    // any of tex_color, tex_u, tex_v, tex_metallic
+    int32_t pad;
+};
+
+// A texture (BbTexture): constant value, bilinear image, or checker of two
+// non-checker textures a and b
+struct GpuTexture
+{
+    int32_t type, channels, width, height;
+    int32_t filter, a, b, pad;
+    float res_x, res_y;
+    float value[3];
+    int32_t texel_offset; // first float in SceneView::texels
 };
 
 struct GpuLight
@@ -138,6 +155,11 @@ struct SceneView
     const float* ggx_E;     // ReflectanceTables::ggx_E
     const float* ggx_E_avg; // ReflectanceTables::ggx_E_avg
     const float* dielectric; // DielectricTables (kDielectricTableFloats), or null when unused
+    const GpuTexture* textures;
+    const float* texels;
+    const float2* uvs;       // per vertex, null when the scene has none (uv = (0, 0))
+    int32_t has_alpha;       // some triangle's material has an alpha texture
+    int32_t has_textures;    // some material has a textured parameter or a normal map
     int32_t n_lights;
     int32_t n_inf;
     float func_integral;

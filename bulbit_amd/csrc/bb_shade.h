@@ -240,6 +240,140 @@ BB_D bxdf make_bxdf(const GpuMaterial& m, bool front_face, const SceneView& S)
     return b;
 }
 
+// ------------------------------------------------------------- textures
+
+// FilterTexCoord (include/bulbit/texture.h:26-46); repeat maps -k to
+// width - (k % width) - 1, as the reference writes it
+BB_D int32_t filter_coord(int32_t i, int32_t n, int32_t filter)
+{
+    if (filter == BB_TEXCOORD_CLAMP) return min(max(i, 0), n - 1);
+    return i >= 0 ? i % n : n - ((-i) % n) - 1;
+}
+
+// ImageTexture::Evaluate, bilinear (include/bulbit/textures.h:60-88): channel c of the image
+BB_D void image_eval(const GpuTexture& t, const float* texels, f2 uv, float* out)
+{
+    const float w = uv.x * float(t.width) - 0.5f;
+    const float h = uv.y * float(t.height) - 0.5f;
+    int32_t i0 = int32_t(floorf(w)), i1 = i0 + 1;
+    int32_t j0 = int32_t(floorf(h)), j1 = j0 + 1;
+    i0 = filter_coord(i0, t.width, t.filter);
+    j0 = filter_coord(j0, t.height, t.filter);
+    i1 = filter_coord(i1, t.width, t.filter);
+    j1 = filter_coord(j1, t.height, t.filter);
+    const float fu = w - floorf(w);
+    const float fv = h - floorf(h);
+    const float* img = texels + t.texel_offset;
+    const int c = t.channels;
+    const float* v00 = img + size_t(i0 + j0 * t.width) * c;
+    const float* v10 = img + size_t(i1 + j0 * t.width) * c;
+    const float* v01 = img + size_t(i0 + j1 * t.width) * c;
+    const float* v11 = img + size_t(i1 + j1 * t.width) * c;
+    const float w00 = (1.0f - fu) * (1.0f - fv), w01 = (1.0f - fu) * fv, w10 = fu * (1.0f - fv), w11 = fu * fv;
+    for (int k = 0; k < c; ++k) out[k] = w00 * v00[k] + w01 * v01[k] + w10 * v10[k] + w11 * v11[k];
+}
+
+// Texture<T>::Evaluate of a constant or image texture
+BB_D void leaf_texture_eval(const GpuTexture& t, const float* texels, f2 uv, float* out)
+{
+    if (t.type == BB_TEX_IMAGE) {
+        image_eval(t, texels, uv, out);
+    } else {
+        out[0] = t.value[0];
+        out[1] = t.value[1];
+        out[2] = t.value[2];
+    }
+}
+
+// Texture<T>::Evaluate; CheckerTexture (textures.h:109-130): Point2i of
+// uv * resolution (truncation), a when the coordinate sum is odd, else b
+BB_D void texture_eval(const SceneView& S, int32_t idx, f2 uv, float* out)
+{
+    const GpuTexture& t = S.textures[idx];
+    if (t.type != BB_TEX_CHECKER) {
+        leaf_texture_eval(t, S.texels, uv, out);
+        return;
+    }
+    const int32_t sx = int32_t(uv.x * t.res_x), sy = int32_t(uv.y * t.res_y);
+    const int32_t c = sx + sy;
+    leaf_texture_eval(S.textures[(c % 2) ? t.a : t.b], S.texels, uv, out);
+}
+
+BB_D float texture1(const SceneView& S, int32_t idx, f2 uv)
+{
+    float v[3];
+    texture_eval(S, idx, uv, v);
+    return v[0];
+}
+
+BB_D spec texture3(const SceneView& S, int32_t idx, f2 uv)
+{
+    float v[3];
+    texture_eval(S, idx, uv, v);
+    return mks(v[0], v[1], v[2]);
+}
+
+// The material's constants with its textured parameters evaluated at uv:
+// the per-hit part of the *Material::GetBSDF functions
+// (diffuse_material.cpp:21-36, dielectric_material.cpp:28-52,
+// conductor_material.cpp:52-99, metallic_roughness_material.cpp:28-39),
+// with the same derived values as the constant set-up (kind, alphas, flags)
+BB_D GpuMaterial material_at(const GpuMaterial& m0, const SceneView& S, f2 uv)
+{
+    GpuMaterial m = m0;
+    switch (m.type) {
+    case BB_MAT_DIFFUSE:
+        if (m.tex_color >= 0) {
+            m.r = texture3(S, m.tex_color, uv);
+            m.flags_front = !is_black(m.r) ? (BX_DIFFUSE | BX_REFLECTION) : 0;
+        }
+        break;
+    case BB_MAT_DIELECTRIC: {
+        if (m.tex_color >= 0) m.r = sqrt_s(texture3(S, m.tex_color, uv));
+        if (m.tex_u >= 0) m.ax = sqr(texture1(S, m.tex_u, uv));
+        if (m.tex_v >= 0) m.ay = sqr(texture1(S, m.tex_v, uv));
+        const bool smooth = smax(m.ax, m.ay) < 1e-3f;
+        m.kind = (m.energy_compensation && !smooth && m.eta != 1.0f) ? MK_DIELECTRIC_MS : MK_DIELECTRIC;
+        const int32_t f = (m.eta == 1.0f) ? BX_TRANSMISSION : (BX_REFLECTION | BX_TRANSMISSION);
+        m.flags_front = f | (smooth ? BX_SPECULAR : BX_GLOSSY);
+        break;
+    }
+    case BB_MAT_CONDUCTOR:
+    case BB_MAT_CONDUCTOR_R: {
+        if (m.tex_color >= 0) m.r = texture3(S, m.tex_color, uv);
+        if (m.tex_u >= 0) m.ax = sqr(texture1(S, m.tex_u, uv));
+        if (m.tex_v >= 0) m.ay = sqr(texture1(S, m.tex_v, uv));
+        const bool smooth = smax(m.ax, m.ay) < 1e-3f;
+        m.flags_front = smooth ? (BX_SPECULAR | BX_REFLECTION) : (BX_GLOSSY | BX_REFLECTION);
+        break;
+    }
+    case BB_MAT_METALLIC_ROUGHNESS:
+        if (m.tex_color >= 0) m.color = texture3(S, m.tex_color, uv);
+        if (m.tex_metallic >= 0) m.metallic = texture1(S, m.tex_metallic, uv);
+        if (m.tex_u >= 0) m.ax = fmaxf(sqr(texture1(S, m.tex_u, uv)), 1e-3f);
+        if (m.tex_v >= 0) m.ay = fmaxf(sqr(texture1(S, m.tex_v, uv)), 1e-3f);
+        break;
+    }
+    return m;
+}
+
+// NormalMapping (src/geometry/intersection.cpp:9-30): the shading normal
+// from the map in the (tangent, normal) frame, reflected about the
+// geometric normal when it faces away from wo; the tangent projected
+// (not renormalized)
+BB_D void normal_map(const SceneView& S, int32_t idx, f2 uv, f3 wo, f3 normal, f3& ns, f3& ts)
+{
+    const spec sp = texture3(S, idx, uv);
+    f3 sn = mk3(sp.r * 2.0f - 1.0f, sp.g * 2.0f - 1.0f, sp.b * 2.0f - 1.0f);
+    normalize_m(sn);
+    const frame3 tbn = frame_from_xz(ts, ns);
+    f3 n = from_local(tbn, sn);
+    if (dot(n, wo) < 0.0f) n = -n + normal * (2.0f * dot(n, normal));
+    const f3 t = ts - n * dot(ts, n);
+    ns = n;
+    ts = t;
+}
+
 // Lambertian, lambertian_bxdf.cpp:8-53
 BB_D spec lambert_f(const bxdf& b, f3 wo, f3 wi)
 {

@@ -334,9 +334,11 @@ bool build_bvh(const BbSceneDesc& d, HostBvh& out, std::string& err)
 namespace
 {
 
-bool material_to_gpu(const BbMaterial& m, GpuMaterial& g, std::string& err)
+bool material_constants(const BbMaterial& m, GpuMaterial& g, std::string& err)
 {
     memset(&g, 0, sizeof(g));
+    g.type = m.type;
+    g.energy_compensation = m.energy_compensation;
     switch (m.type) {
     case BB_MAT_DIFFUSE:
         // DiffuseMaterial::GetBSDF: Lambertian when roughness <= 0 (diffuse_material.cpp:21-36)
@@ -420,6 +422,114 @@ bool upload(BbScene* s, const T* host, size_t count, const T** dev_out, std::str
     return true;
 }
 
+// a texture slot of a material: 1 + index (0: none) -> index or -1, with
+// the channel count the slot needs
+bool texture_slot(const BbSceneDesc& d, int32_t slot, int32_t channels, const char* what, int32_t& out,
+                  std::string& err)
+{
+    out = -1;
+    if (slot == 0) return true;
+    if (slot < 0 || slot > d.n_textures || !d.textures) {
+        err = std::string("material ") + what + " texture index out of range";
+        return false;
+    }
+    if (d.textures[slot - 1].channels != channels) {
+        err = std::string("material ") + what + " texture must have " + std::to_string(channels) + " channel(s)";
+        return false;
+    }
+    out = slot - 1;
+    return true;
+}
+
+// the constants of material_constants plus the texture slots that the
+// reference's material evaluates (the *Material::GetBSDF functions,
+// Intersection::GetBSDF for the normal map, Primitive::Intersect for alpha)
+bool material_to_gpu(const BbSceneDesc& d, const BbMaterial& m, GpuMaterial& g, std::string& err)
+{
+    if (!material_constants(m, g, err)) return false;
+    const bool mr = m.type == BB_MAT_METALLIC_ROUGHNESS;
+    if (m.type == BB_MAT_DIFFUSE && m.tex_u_roughness != 0) {
+        err = "a diffuse roughness texture (EON BxDF) is not supported";
+        return false;
+    }
+    if (!mr && m.tex_metallic != 0) {
+        err = "a metallic texture belongs to a metallic-roughness material";
+        return false;
+    }
+    bool ok = texture_slot(d, m.tex_color, 3, "colour", g.tex_color, err) &&
+              texture_slot(d, m.tex_u_roughness, 1, "u roughness", g.tex_u, err) &&
+              texture_slot(d, m.tex_v_roughness, 1, "v roughness", g.tex_v, err) &&
+              texture_slot(d, m.tex_metallic, 1, "metallic", g.tex_metallic, err) &&
+              texture_slot(d, m.tex_normal, 3, "normal", g.tex_normal, err) &&
+              texture_slot(d, m.tex_alpha, 1, "alpha", g.tex_alpha, err);
+    if (!ok) return false;
+    // DielectricMaterial has no alpha texture (GetAlphaTexture returns null)
+    if (m.type == BB_MAT_DIELECTRIC) g.tex_alpha = -1;
+    // DiffuseMaterial's v roughness does not exist
+    if (m.type == BB_MAT_DIFFUSE) g.tex_v = -1;
+    g.textured = (g.tex_color >= 0 || g.tex_u >= 0 || g.tex_v >= 0 || g.tex_metallic >= 0) ? 1 : 0;
+    return true;
+}
+
+// BbTexture -> GpuTexture, texels appended to `texels`
+bool textures_to_gpu(const BbSceneDesc& d, std::vector<GpuTexture>& out, std::vector<float>& texels,
+                     std::string& err)
+{
+    out.assign(std::max(1, d.n_textures), GpuTexture{});
+    if (d.n_textures < 0 || (d.n_textures > 0 && !d.textures)) {
+        err = "texture array missing";
+        return false;
+    }
+    for (int32_t i = 0; i < d.n_textures; ++i) {
+        const BbTexture& t = d.textures[i];
+        GpuTexture& g = out[i];
+        if (t.channels != 1 && t.channels != 3) {
+            err = "texture channels must be 1 or 3";
+            return false;
+        }
+        g.type = t.type;
+        g.channels = t.channels;
+        g.a = g.b = -1;
+        for (int k = 0; k < 3; ++k) g.value[k] = t.channels == 1 ? t.value[0] : t.value[k];
+        if (t.type == BB_TEX_CONSTANT) continue;
+        if (t.type == BB_TEX_IMAGE) {
+            if (t.width <= 0 || t.height <= 0 || !t.texels ||
+                int64_t(t.width) * t.height * t.channels > (int64_t(1) << 31) - int64_t(texels.size())) {
+                err = "image texture needs a positive size and texels";
+                return false;
+            }
+            if (t.filter != BB_TEXCOORD_REPEAT && t.filter != BB_TEXCOORD_CLAMP) {
+                err = "unknown texture coordinate filter";
+                return false;
+            }
+            g.width = t.width;
+            g.height = t.height;
+            g.filter = t.filter;
+            g.texel_offset = int32_t(texels.size());
+            texels.insert(texels.end(), t.texels, t.texels + size_t(t.width) * t.height * t.channels);
+            continue;
+        }
+        if (t.type == BB_TEX_CHECKER) {
+            for (int k = 0; k < 2; ++k) {
+                const int32_t c = t.checker[k] - 1;
+                if (c < 0 || c >= d.n_textures || d.textures[c].type == BB_TEX_CHECKER ||
+                    d.textures[c].channels != t.channels) {
+                    err = "checker textures take two constant or image textures of their channel count";
+                    return false;
+                }
+            }
+            g.a = t.checker[0] - 1;
+            g.b = t.checker[1] - 1;
+            g.res_x = t.resolution[0];
+            g.res_y = t.resolution[1];
+            continue;
+        }
+        err = "unknown texture type";
+        return false;
+    }
+    return true;
+}
+
 } // namespace
 
 bool create_scene(const BbSceneDesc& d, BbScene* s, std::string& err)
@@ -439,7 +549,14 @@ bool create_scene(const BbSceneDesc& d, BbScene* s, std::string& err)
 
     std::vector<GpuMaterial> mats(std::max(1, d.n_materials));
     for (int32_t i = 0; i < d.n_materials; ++i)
-        if (!material_to_gpu(d.materials[i], mats[i], err)) return false;
+        if (!material_to_gpu(d, d.materials[i], mats[i], err)) return false;
+    std::vector<GpuTexture> texs;
+    std::vector<float> texels;
+    if (!textures_to_gpu(d, texs, texels, err)) return false;
+    if (d.n_primitives >= (1 << 30)) { // hit records keep bit 30 for alpha restarts
+        err = "at most 2^30 - 1 primitives";
+        return false;
+    }
 
     // lights: Integrator::Integrator (integrator.cpp:107-147)
     std::vector<int32_t> prim_light(d.n_primitives, -1);
@@ -539,9 +656,14 @@ bool create_scene(const BbSceneDesc& d, BbScene* s, std::string& err)
             f3 e1 = p1 - p0, e2 = p2 - p0;
             tris[3 * size_t(j) + 0] = make_float4(p0.x, p0.y, p0.z, prim_f);
             tris[3 * size_t(j) + 1] = make_float4(e1.x, e1.y, e1.z, 0.0f);
-            tris[3 * size_t(j) + 2] = make_float4(e2.x, e2.y, e2.z, i2f(KIND_TRIANGLE));
+            const bool alpha = p.material >= 0 && mats[p.material].tex_alpha >= 0;
+            tris[3 * size_t(j) + 2] = make_float4(e2.x, e2.y, e2.z, i2f(KIND_TRIANGLE | (alpha ? kSlotAlpha : 0)));
             slot_shade[j] = make_int4(v[0], v[1], v[2], p.material);
         } else {
+            if (p.material >= 0 && mats[p.material].tex_alpha >= 0) {
+                err = "alpha textures on spheres are not supported";
+                return false;
+            }
             int32_t kind = KIND_SPHERE, sidx = p.shape_index;
             float kind_f, sidx_f;
             memcpy(&kind_f, &kind, 4);
@@ -622,13 +744,32 @@ bool create_scene(const BbSceneDesc& d, BbScene* s, std::string& err)
     const ReflectanceTables& tab = reflectance_tables();
     if (!upload(s, tab.ggx_E, size_t(kTableSize) * kTableSize, &V.ggx_E, err)) return false;
     if (!upload(s, tab.ggx_E_avg, size_t(kTableSize), &V.ggx_E_avg, err)) return false;
+    texels.resize(std::max<size_t>(1, texels.size()));
+    if (!upload(s, texs.data(), texs.size(), &V.textures, err)) return false;
+    if (!upload(s, texels.data(), texels.size(), &V.texels, err)) return false;
+    V.uvs = nullptr;
+    if (d.uvs && d.n_vertices > 0) {
+        std::vector<float2> uvs(d.n_vertices);
+        for (int32_t i = 0; i < d.n_vertices; ++i) uvs[i] = make_float2(d.uvs[2 * i], d.uvs[2 * i + 1]);
+        if (!upload(s, uvs.data(), uvs.size(), &V.uvs, err)) return false;
+    }
     V.dielectric = nullptr;
-    if (std::any_of(mats.begin(), mats.end(), [](const GpuMaterial& g) { return g.kind == MK_DIELECTRIC_MS; })) {
+    // compensated dielectrics need the tables; a textured roughness can make
+    // any compensated dielectric rough at some hit
+    if (std::any_of(mats.begin(), mats.end(), [](const GpuMaterial& g) {
+            return g.kind == MK_DIELECTRIC_MS ||
+                   (g.type == BB_MAT_DIELECTRIC && g.energy_compensation && (g.tex_u >= 0 || g.tex_v >= 0));
+        })) {
         const DielectricTables& dt = dielectric_tables();
         static_assert(sizeof(DielectricTables) == sizeof(float) * kDielectricTableFloats, "packed tables");
         if (!upload(s, reinterpret_cast<const float*>(&dt), size_t(kDielectricTableFloats), &V.dielectric, err))
             return false;
     }
+    V.has_alpha = 0;
+    for (int32_t j = 0; j < ns; ++j)
+        if (f2i(tris[3 * size_t(j) + 2].w) & kSlotAlpha) V.has_alpha = 1;
+    V.has_textures = std::any_of(mats.begin(), mats.end(),
+                                 [](const GpuMaterial& g) { return g.textured || g.tex_normal >= 0; }) ? 1 : 0;
     V.n_lights = nl;
     V.n_inf = s->n_inf;
     V.func_integral = func_integral;

@@ -156,6 +156,52 @@ struct Lane
     float hit_t, hit_u, hit_v;
 };
 
+// Triangle::GetTexCoord (src/geometry/triangle.cpp:31-38): w uv0 + u uv1 + v uv2
+BB_D f2 tri_uv(const SceneView& S, int4 vi, float u, float v)
+{
+    if (!S.uvs) return mk2(0.0f, 0.0f);
+    const float w = 1.0f - u - v;
+    const float2 a = S.uvs[vi.x], b = S.uvs[vi.y], c = S.uvs[vi.z];
+    return mk2(w * a.x + u * b.x + v * c.x, w * a.y + u * b.y + v * c.y);
+}
+
+// a hit whose candidate was accepted on a restarted ray (alpha_chain): the
+// slot carries this bit in the hit record
+constexpr int32_t kHitRestart = 1 << 30;
+constexpr int kAlphaChain = 16; // re-intersections followed before giving up (a miss)
+
+// Primitive::Intersect's alpha test (include/bulbit/primitive.h:59-75) for
+// the triangle candidate (t, u, v) found on the ray (o, d): kept when
+// alpha >= 1 or when HashFloat(ray, point) does not exceed alpha (alpha <= 0:
+// never kept).  A rejected candidate re-intersects the same triangle on the
+// ray (point, Normalize(d)) over [Ray::epsilon, t_max - t]: for a flat
+// triangle that ray meets the plane again only through the rounding of the
+// point, at grazing angles, and the test repeats there.  On a keep after a
+// restart, (o, d, t, u, v) describe the restarted ray and its hit and
+// `restarted` is set, as the reference's Intersection holds them.
+BB_D bool alpha_chain(const SceneView& S, int32_t slot, f3& o, f3& d, float t_max, float& t, float& u, float& v,
+                      bool& restarted)
+{
+    const int4 vi = S.slot_shade[slot];
+    const GpuMaterial& m = S.materials[vi.w];
+    const f3 p0 = xyz(S.tris[3 * slot]), e1 = xyz(S.tris[3 * slot + 1]), e2 = xyz(S.tris[3 * slot + 2]);
+    for (int k = 0; k < kAlphaChain; ++k) {
+        const float alpha = texture1(S, m.tex_alpha, tri_uv(S, vi, u, v));
+        if (!(alpha < 1.0f)) return true;
+        const f3 point = o + d * t;
+        const float p = alpha <= 0.0f ? 1.0f : hash_float_ray_point(o, d, point);
+        if (!(p > alpha)) return true;
+        t_max = t_max - t;
+        d = normalize(d);
+        o = point;
+        f3 dn = d;
+        const float l = normalize_m(dn);
+        if (!mt_test(p0, e1, e2, o, dn, l, kRayEps, t_max, t, u, v)) return false;
+        restarted = true;
+    }
+    return false;
+}
+
 BB_D void lane_start(Lane& L, f3 o, f3 d, float t_min, float t_max)
 {
     L.o = o;
@@ -240,7 +286,7 @@ constexpr int32_t kFailedLo = 0x7fbfffff;
 // IntersectAny, src/accel/bvh.cpp:299-361).  With COUNT, every far child is
 // pushed and counted when popped, which reproduces RayCast's node-test count.
 // Returns true when the lane's query is finished.
-template <bool ANY, bool COUNT, bool FULL>
+template <bool ANY, bool COUNT, bool FULL, bool ALPHA>
 BB_D bool trace_step(const SceneView& S, Lane& L, const TraceStack<StackEntry<ANY, COUNT>>& stk, TraceCount& c,
                      int* err)
 {
@@ -295,9 +341,16 @@ BB_D bool trace_step(const SceneView& S, Lane& L, const TraceStack<StackEntry<AN
                 u = v = 0.0f;
                 h = sphere_test(S.spheres[__float_as_int(t1.w)], L.o, L.d, L.t_min, L.t_max, t);
             }
+            bool restarted = false;
+            if constexpr (ALPHA) { // scenes with an alpha texture only
+                if (h && (kind & kSlotAlpha)) {
+                    f3 ao = L.o, ad = L.d;
+                    h = alpha_chain(S, slot, ao, ad, L.t_max, t, u, v, restarted);
+                }
+            }
             if (h) {
                 if (COUNT) c.accepts++;
-                L.hit_slot = slot;
+                L.hit_slot = slot | (restarted ? kHitRestart : 0);
                 if (ANY) return true;
                 L.hit_t = t;
                 L.hit_u = u;
@@ -328,7 +381,7 @@ BB_D bool trace_step(const SceneView& S, Lane& L, const TraceStack<StackEntry<AN
 // `refill_min` of its 64 lanes are idle (64: whole-wave batches, one atomic
 // per 64 queries).  Between refill checks every lane takes STEPS traversal
 // steps.  Src provides load(index, Lane&) and finish(index, const Lane&).
-template <bool ANY, bool COUNT, bool FULL, int STEPS, typename Src>
+template <bool ANY, bool COUNT, bool FULL, bool ALPHA, int STEPS, typename Src>
 BB_D void persistent_trace(const SceneView& S, const Src& src, int n, int* fetch,
                            const TraceStack<StackEntry<ANY, COUNT>>& stk, TraceCount& c, int* err, int refill_min)
 {
@@ -357,7 +410,7 @@ BB_D void persistent_trace(const SceneView& S, const Src& src, int n, int* fetch
         if (idle == ~0ull) break;
 #pragma unroll
         for (int k = 0; k < STEPS; ++k) {
-            if (active && trace_step<ANY, COUNT, FULL>(S, L, stk, c, err)) {
+            if (active && trace_step<ANY, COUNT, FULL, ALPHA>(S, L, stk, c, err)) {
                 src.finish(id, L);
                 active = false;
             }
@@ -414,7 +467,8 @@ struct BatchSrc
         if (any_hit)
             out = make_int4(L.hit_slot >= 0 ? 1 : -1, 0, 0, 0);
         else if (L.hit_slot >= 0)
-            out = make_int4(__float_as_int(tris[3 * L.hit_slot].w), __float_as_int(L.hit_t), __float_as_int(L.hit_u),
+            out = make_int4(__float_as_int(tris[3 * (L.hit_slot & ~kHitRestart)].w), __float_as_int(L.hit_t),
+                            __float_as_int(L.hit_u),
                             __float_as_int(L.hit_v));
         else
             out = make_int4(-1, 0, 0, 0);
@@ -445,7 +499,7 @@ constexpr int kTraceSteps = BB_TRACE_STEPS;
 #endif
 constexpr int kShadowSteps = BB_SHADOW_STEPS;
 
-template <bool ANY, bool COUNT, bool FULL>
+template <bool ANY, bool COUNT, bool FULL, bool ALPHA>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_batch(SceneView S, BatchSrc src, int n, int* fetch,
                                                                int2* gstack, int lds_entries, int refill_min,
                                                                unsigned long long* counters, int* err)
@@ -453,7 +507,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_batch(SceneView S, BatchS
     extern __shared__ int2 lds_stack[];
     const auto stk = make_stack<StackEntry<ANY, COUNT>>(lds_stack, gstack, lds_entries, S.stack_depth);
     TraceCount c{ 0, 0, 0 };
-    persistent_trace<ANY, COUNT, FULL, kTraceSteps>(S, src, n, fetch, stk, c, err, refill_min);
+    persistent_trace<ANY, COUNT, FULL, ALPHA, kTraceSteps>(S, src, n, fetch, stk, c, err, refill_min);
     if (COUNT) wave_count_add(c, counters);
 }
 
@@ -643,7 +697,7 @@ struct ShadowSrc
     }
 };
 
-template <bool COUNT, bool FULL>
+template <bool COUNT, bool FULL, bool ALPHA>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(SceneView S, const int* __restrict__ n_ptr, int* fetch,
                                                                  ClosestSrc src, int2* gstack, int lds_entries, int refill_min,
                                                                  unsigned long long* counters, int* err)
@@ -651,11 +705,11 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(SceneView S, cons
     extern __shared__ int2 lds_stack[];
     const auto stk = make_stack<StackEntry<false, COUNT>>(lds_stack, gstack, lds_entries, S.stack_depth);
     TraceCount c{ 0, 0, 0 };
-    persistent_trace<false, COUNT, FULL, kTraceSteps>(S, src, *n_ptr, fetch, stk, c, err, refill_min);
+    persistent_trace<false, COUNT, FULL, ALPHA, kTraceSteps>(S, src, *n_ptr, fetch, stk, c, err, refill_min);
     if (COUNT) wave_count_add(c, counters);
 }
 
-template <bool COUNT, bool FULL>
+template <bool COUNT, bool FULL, bool ALPHA>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_shadow(SceneView S, const int* __restrict__ n_ptr, int* fetch,
                                                                 ShadowSrc src, int2* gstack, int lds_entries, int refill_min,
                                                                 unsigned long long* counters, int* err)
@@ -663,7 +717,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_shadow(SceneView S, const
     extern __shared__ int2 lds_stack[];
     const auto stk = make_stack<StackEntry<true, COUNT>>(lds_stack, gstack, lds_entries, S.stack_depth);
     TraceCount c{ 0, 0, 0 };
-    persistent_trace<true, COUNT, FULL, kShadowSteps>(S, src, *n_ptr, fetch, stk, c, err, refill_min);
+    persistent_trace<true, COUNT, FULL, ALPHA, kShadowSteps>(S, src, *n_ptr, fetch, stk, c, err, refill_min);
     if (COUNT) wave_count_add(c, counters + 3);
 }
 
@@ -673,7 +727,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_shadow(SceneView S, const
 struct SurfaceHit
 {
     f3 point, normal, ns, ts;
-    bool front;
+    f3 nl; // sphere: the object-space normal
+    bool front, sphere;
 };
 
 BB_D SurfaceHit surface_at(const SceneView& S, int32_t slot, float t, float u, float v, f3 o, f3 d)
@@ -688,6 +743,8 @@ BB_D SurfaceHit surface_at(const SceneView& S, int32_t slot, float t, float u, f
         sphere_local_ray(sp, o, d, ro, rd);
         f3 pl = ro + rd * t;
         f3 nl = normalize(pl / sp.radius);
+        h.nl = nl;
+        h.sphere = true;
         h.point = quat_rotate(q, mk3(sp.s[0], sp.s[1], sp.s[2]) * pl) + mk3(sp.p[0], sp.p[1], sp.p[2]);
         f3 n = quat_rotate(q, nl);
         f3 tt = (n.y < 1.0f - 1e-4f) ? normalize(cross(mk3(0.0f, 1.0f, 0.0f), n)) : normalize(cross(n, mk3(1.0f, 0.0f, 0.0f)));
@@ -695,6 +752,8 @@ BB_D SurfaceHit surface_at(const SceneView& S, int32_t slot, float t, float u, f
         ns = n;
         ts = tt;
     } else {
+        h.sphere = false;
+        h.nl = mk3(0.0f);
         const int4 vi = S.slot_shade[slot];
         const f3 e1 = xyz(t1), e2 = xyz(t2);
         const float w = 1.0f - u - v;
@@ -711,6 +770,29 @@ BB_D SurfaceHit surface_at(const SceneView& S, int32_t slot, float t, float u, f
     h.ns = ns * sign;
     h.ts = ts * sign;
     return h;
+}
+
+// the hit's texture coordinates: Triangle::GetTexCoord, or Sphere's
+// ComputeTexCoord of the object-space normal (src/geometry/sphere.cpp:9-16)
+BB_D f2 hit_uv(const SceneView& S, int32_t slot, const SurfaceHit& h, float u, float v)
+{
+    if (h.sphere) {
+        const float theta = (float)::acos((double)clampf_(h.nl.y, -1.0f, 1.0f));
+        const float r = (float)::atan2((double)h.nl.z, (double)h.nl.x);
+        const float phi = r < 0.0f ? r + kTwoPi : r;
+        return mk2(phi * kInvTwoPi, 1.0f - theta * kInvPi);
+    }
+    return tri_uv(S, S.slot_shade[slot], u, v);
+}
+
+// The hit's material with its textures evaluated at the hit's texture
+// coordinates, and the normal map applied to the shading frame
+BB_D void textured_hit(const SceneView& S, int32_t slot, float u, float v, f3 wo, GpuMaterial& mat,
+                                          SurfaceHit& sh)
+{
+    const f2 uv = hit_uv(S, slot, sh, u, v);
+    if (mat.textured) mat = material_at(mat, S, uv);
+    if (mat.tex_normal >= 0) normal_map(S, mat.tex_normal, uv, wo, sh.normal, sh.ns, sh.ts);
 }
 
 // PathIntegrator::Li, one bounce (path_integrator.cpp:38-147) and the
@@ -737,7 +819,7 @@ BB_D void cache_features(const CacheBufs& C, const GpuMaterial& m, f3 p, f3 wo, 
     r[13] = r[14] = r[15] = 0.0f;
 }
 
-template <bool CACHE>
+template <bool CACHE, bool TEX>
 __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView S, RenderConsts rc, PathBufs P, int depth, CacheBufs C)
 {
     const int in = depth & 1, out = in ^ 1;
@@ -766,7 +848,19 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView S, RenderConsts
             float eta_scale = Lv.w;
             spec beta = mks(Bv.x, Bv.y, Bv.z);
             float prev_pdf = Bv.w;
-            const int32_t slot = hh.x;
+            // a restarted alpha hit lies on the restarted ray: replay the
+            // chain (deterministic) from the original ray to recover it
+            const int32_t slot = hh.x < 0 ? hh.x : (hh.x & ~kHitRestart);
+            f3 so = o, sd = d;
+            if (hh.x >= 0 && (hh.x & kHitRestart)) {
+                const float4 q0 = S.tris[3 * slot], q1 = S.tris[3 * slot + 1], q2 = S.tris[3 * slot + 2];
+                float rt = 0.0f, ru = 0.0f, rv = 0.0f;
+                bool rs = false;
+                f3 dn = d;
+                const float l = normalize_m(dn);
+                if (mt_test(xyz(q0), xyz(q1), xyz(q2), o, dn, l, kRayEps, INFINITY, rt, ru, rv))
+                    alpha_chain(S, slot, so, sd, INFINITY, rt, ru, rv, rs);
+            }
             if (slot < 0) {
                 // escaped: infinite lights (path_integrator.cpp:41-62)
                 for (int32_t k2 = 0; k2 < S.n_inf; ++k2) {
@@ -781,8 +875,19 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView S, RenderConsts
                 }
             } else {
                 const float t = __int_as_float(hh.y), u = __int_as_float(hh.z), v = __int_as_float(hh.w);
-                const SurfaceHit sh = surface_at(S, slot, t, u, v, o, d);
+                SurfaceHit sh = surface_at(S, slot, t, u, v, so, sd);
                 const f3 wo = normalize(-d);
+                // the hit's material with its textures evaluated at the hit,
+                // and the normal map applied to the shading frame (the
+                // per-hit part of Intersection::GetBSDF, intersection.cpp:32-52)
+                const int32_t mat_index = S.slot_shade[slot].w;
+                GpuMaterial mat;
+                if (mat_index >= 0) {
+                    mat = S.materials[mat_index];
+                    if constexpr (TEX) { // scenes with textured or normal-mapped materials only
+                        if (mat.textured || mat.tex_normal >= 0) textured_hit(S, slot, u, v, wo, mat, sh);
+                    }
+                }
                 const int32_t al = S.slot_light[slot];
                 if (al >= 0) {
                     // area light emission (path_integrator.cpp:67-84)
@@ -814,7 +919,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView S, RenderConsts
                     const bool query_here =
                         mi >= 0 && (depth >= C.query_bounce ||
                                     ((depth >= 1 || C.query_primary) &&
-                                     (S.materials[mi].flags_front & (BX_DIFFUSE | BX_GLOSSY))));
+                                     (mat.flags_front & (BX_DIFFUSE | BX_GLOSSY))));
                     int32_t stop = (fl >> kStopShift) & 0xff;
                     if (training) {
                         if (query_here && stop == kStopUnknown) stop = min(depth + C.extra, 254);
@@ -826,7 +931,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView S, RenderConsts
                             const int32_t nv = C.tcount[tp];
                             if (nv < C.max_vertices) {
                                 const size_t rec = size_t(tp) * C.max_vertices + nv;
-                                cache_features(C, S.materials[mi], sh.point, wo, sh.ns, C.traw + 16 * rec);
+                                cache_features(C, mat, sh.point, wo, sh.ns, C.traw + 16 * rec);
                                 C.tsnap[2 * rec] = make_float4(L.r, L.g, L.b, 0.0f);
                                 C.tsnap[2 * rec + 1] = make_float4(beta.r, beta.g, beta.b, 0.0f);
                                 C.tcount[tp] = nv + 1;
@@ -834,7 +939,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView S, RenderConsts
                         }
                     } else if (query_here) {
                         const int q = atomicAdd(&C.counters[0], 1);
-                        cache_features(C, S.materials[mi], sh.point, wo, sh.ns, C.qraw + 16 * size_t(q));
+                        cache_features(C, mat, sh.point, wo, sh.ns, C.qraw + 16 * size_t(q));
                         C.qbeta[q] = make_float4(beta.r, beta.g, beta.b, __int_as_float(pid));
                         stop_here = true;
                     }
@@ -848,7 +953,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView S, RenderConsts
                         new_d = -wo;
                         cont = true;
                     } else {
-                        const bxdf b = make_bxdf(S.materials[mi], sh.front, S);
+                        const bxdf b = make_bxdf(mat, sh.front, S);
                         const frame3 fr = frame_from_xz(sh.ts, sh.ns);
                         const f3 wo_l = to_local(fr, wo);
                         if (b.flags & (BX_DIFFUSE | BX_GLOSSY)) {
@@ -1320,28 +1425,43 @@ using BatchKernel = void (*)(SceneView, BatchSrc, int, int*, int2*, int, int, un
 using ClosestKernel = void (*)(SceneView, const int*, int*, ClosestSrc, int2*, int, int, unsigned long long*, int*);
 using ShadowKernel = void (*)(SceneView, const int*, int*, ShadowSrc, int2*, int, int, unsigned long long*, int*);
 
-template <bool ANY>
-BatchKernel batch_kernel_any(bool count, bool full)
+// the traversal variants: any-hit or closest, counters, whole stack in LDS,
+// and the alpha test compiled in only for scenes that have alpha textures
+template <bool ANY, bool ALPHA>
+BatchKernel batch_kernel_t(bool count, bool full)
 {
-    if (count) return full ? k_trace_batch<ANY, true, true> : k_trace_batch<ANY, true, false>;
-    return full ? k_trace_batch<ANY, false, true> : k_trace_batch<ANY, false, false>;
+    if (count) return full ? k_trace_batch<ANY, true, true, ALPHA> : k_trace_batch<ANY, true, false, ALPHA>;
+    return full ? k_trace_batch<ANY, false, true, ALPHA> : k_trace_batch<ANY, false, false, ALPHA>;
 }
 
-BatchKernel batch_kernel(bool any, bool count, bool full)
+BatchKernel batch_kernel(bool any, bool count, bool full, bool alpha)
 {
-    return any ? batch_kernel_any<true>(count, full) : batch_kernel_any<false>(count, full);
+    if (alpha) return any ? batch_kernel_t<true, true>(count, full) : batch_kernel_t<false, true>(count, full);
+    return any ? batch_kernel_t<true, false>(count, full) : batch_kernel_t<false, false>(count, full);
 }
 
-ClosestKernel closest_kernel(bool count, bool full)
+template <bool ALPHA>
+ClosestKernel closest_kernel_t(bool count, bool full)
 {
-    if (count) return full ? k_trace_closest<true, true> : k_trace_closest<true, false>;
-    return full ? k_trace_closest<false, true> : k_trace_closest<false, false>;
+    if (count) return full ? k_trace_closest<true, true, ALPHA> : k_trace_closest<true, false, ALPHA>;
+    return full ? k_trace_closest<false, true, ALPHA> : k_trace_closest<false, false, ALPHA>;
 }
 
-ShadowKernel shadow_kernel(bool count, bool full)
+ClosestKernel closest_kernel(bool count, bool full, bool alpha)
 {
-    if (count) return full ? k_trace_shadow<true, true> : k_trace_shadow<true, false>;
-    return full ? k_trace_shadow<false, true> : k_trace_shadow<false, false>;
+    return alpha ? closest_kernel_t<true>(count, full) : closest_kernel_t<false>(count, full);
+}
+
+template <bool ALPHA>
+ShadowKernel shadow_kernel_t(bool count, bool full)
+{
+    if (count) return full ? k_trace_shadow<true, true, ALPHA> : k_trace_shadow<true, false, ALPHA>;
+    return full ? k_trace_shadow<false, true, ALPHA> : k_trace_shadow<false, false, ALPHA>;
+}
+
+ShadowKernel shadow_kernel(bool count, bool full, bool alpha)
+{
+    return alpha ? shadow_kernel_t<true>(count, full) : shadow_kernel_t<false>(count, full);
 }
 
 } // namespace
@@ -1361,7 +1481,7 @@ int bbgpu_trace(BbScene* scene, const BbRay* rays_dev, int32_t n_rays, int32_t a
     const int lds_entries = lds_entries_for(V, 0);
     const size_t lds = trace_lds_bytes(lds_entries, any_hit != 0 && counters == nullptr);
     const bool full = V.stack_depth <= lds_entries;
-    const BatchKernel kfn = batch_kernel(any_hit != 0, counters != nullptr, full);
+    const BatchKernel kfn = batch_kernel(any_hit != 0, counters != nullptr, full, V.has_alpha != 0);
     const int grid = std::min(trace_grid(reinterpret_cast<const void*>(kfn), lds), grid_for(std::max(n_rays, 1), kTraceBlock));
     if (!ensure_workspace(x->ws, 1, 16) ||
         !ensure_gstack(x->ws, size_t(std::max(0, V.stack_depth - lds_entries)) * size_t(grid) * kTraceBlock)) {
@@ -1524,8 +1644,12 @@ int render_impl(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams
     const int refill_min = p.trace_refill > 0 ? std::min(p.trace_refill, 64) : 48;
     const int refill_shadow = p.shadow_refill > 0 ? std::min(p.shadow_refill, 64) : kShadowRefillDefault;
     const bool full = V.stack_depth <= lds_entries;
-    const ClosestKernel k_closest = closest_kernel(count, full);
-    const ShadowKernel k_shadow = shadow_kernel(count, full);
+    const ClosestKernel k_closest = closest_kernel(count, full, V.has_alpha != 0);
+    // shading with the texture code only for scenes that use textures
+    void (*const k_shade_fn)(SceneView, RenderConsts, PathBufs, int, CacheBufs) =
+        cache ? (V.has_textures ? k_shade<true, true> : k_shade<true, false>)
+              : (V.has_textures ? k_shade<false, true> : k_shade<false, false>);
+    const ShadowKernel k_shadow = shadow_kernel(count, full, V.has_alpha != 0);
     const int g_trace = std::min(trace_grid(reinterpret_cast<const void*>(k_closest), lds), grid_for(cap, kTraceBlock));
     const int g_shadow =
         std::min(trace_grid(reinterpret_cast<const void*>(k_shadow), lds_shadow), grid_for(cap, kTraceBlock));
@@ -1678,10 +1802,7 @@ int render_impl(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams
                                P.counts + 4 * depth + 2, csrc, P.gstack, lds_entries, refill_min, P.counters, P.err);
             hipEvent_t b = timed ? mark() : nullptr;
             if (timed) spans.push_back({ a, b, 0 });
-            if (cache)
-                hipLaunchKernelGGL(k_shade<true>, dim3(g_shade), dim3(kShadeBlock), 0, ls, V, rc, P, depth, Cv);
-            else
-                hipLaunchKernelGGL(k_shade<false>, dim3(g_shade), dim3(kShadeBlock), 0, ls, V, rc, P, depth, Cv);
+            hipLaunchKernelGGL(k_shade_fn, dim3(g_shade), dim3(kShadeBlock), 0, ls, V, rc, P, depth, Cv);
             hipEvent_t c2 = timed ? mark() : nullptr;
             if (timed) spans.push_back({ b, c2, 2 });
             const ShadowSrc ssrc{ P.sray, P.scontrib, P.L };

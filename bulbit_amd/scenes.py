@@ -142,6 +142,41 @@ def metallic_roughness(color, metallic, roughness):
     return m
 
 
+def constant_texture(value):
+    """ConstantTexture: a float (1 channel) or an RGB triple (3 channels)"""
+    t = capi.BbTexture()
+    t.type = capi.BB_TEX_CONSTANT
+    v = np.atleast_1d(np.asarray(value, np.float32))
+    t.channels = 1 if v.size == 1 else 3
+    t.value[:] = [float(v[0])] * 3 if v.size == 1 else [float(x) for x in v[:3]]
+    return t
+
+
+def image_texture(texels, filter=capi.BB_TEXCOORD_REPEAT):
+    """ImageTexture (bilinear) over texels[height][width] (1 channel) or
+    [height][width][3]; texel (x, y) is texels[y][x].  The array is kept by
+    the returned structure."""
+    a = np.ascontiguousarray(texels, np.float32)
+    t = capi.BbTexture()
+    t.type = capi.BB_TEX_IMAGE
+    t.channels = 1 if a.ndim == 2 else a.shape[2]
+    t.height, t.width = a.shape[0], a.shape[1]
+    t.filter = filter
+    t.texels = a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    t._texels = a  # noqa: keeps the array alive with the structure
+    return t
+
+
+def checker_texture(a_slot, b_slot, resolution, channels):
+    """CheckerTexture(a, b, resolution); a_slot, b_slot are add_texture values"""
+    t = capi.BbTexture()
+    t.type = capi.BB_TEX_CHECKER
+    t.channels = channels
+    t.checker[:] = [a_slot, b_slot]
+    t.resolution[:] = list(resolution)
+    return t
+
+
 @dataclass
 class SceneData:
     positions: np.ndarray = field(default_factory=lambda: np.zeros((0, 3), F32))
@@ -152,16 +187,31 @@ class SceneData:
     prims: np.ndarray = field(default_factory=lambda: np.zeros((0, 4), np.int32))
     materials: list = field(default_factory=list)
     lights: list = field(default_factory=list)
+    uvs: np.ndarray | None = None  # [n_vertices][2] once a mesh brings texture coordinates
+    textures: list = field(default_factory=list)  # BbTexture, texel arrays kept in _texels
+
+    def add_texture(self, tex) -> int:
+        """Appends a texture (see constant_texture, image_texture,
+        checker_texture); returns its material-slot value, index + 1."""
+        self.textures.append(tex)
+        return len(self.textures)
 
     def add_material(self, m) -> int:
         self.materials.append(m)
         return len(self.materials) - 1
 
-    def add_mesh(self, positions, normals, tangents, indices, materials) -> np.ndarray:
+    def add_mesh(self, positions, normals, tangents, indices, materials, uvs=None) -> np.ndarray:
         """Mesh + one primitive per triangle (cli/primitive_builder.cpp:70-101).
         Normals and tangents are normalised as the Mesh constructor does with
-        an identity transform (src/geometry/mesh.cpp:8-39).  Returns the new
+        an identity transform (src/geometry/mesh.cpp:8-39).  uvs: texture
+        coordinates per vertex (zeros when absent).  Returns the new
         primitive indices."""
+        nv_before = len(self.positions)
+        n_new = len(positions)
+        if uvs is not None or self.uvs is not None:
+            old = self.uvs if self.uvs is not None else np.zeros((nv_before, 2), F32)
+            new = np.asarray(uvs, F32).reshape(-1, 2) if uvs is not None else np.zeros((n_new, 2), F32)
+            self.uvs = np.concatenate([old, new])
         positions = np.asarray(positions, F32) + F32(0)
         normals = normalize_member32(np.asarray(normals, F32) + F32(0))
         tangents = normalize_member32(np.asarray(tangents, F32) + F32(0))
@@ -232,7 +282,7 @@ class SceneData:
         d.positions = arr(self.positions.astype(F32), ctypes.c_float)
         d.normals = arr(self.normals.astype(F32), ctypes.c_float)
         d.tangents = arr(self.tangents.astype(F32), ctypes.c_float)
-        d.uvs = None
+        d.uvs = arr(self.uvs.astype(F32), ctypes.c_float) if self.uvs is not None else None
         d.n_triangles = len(self.indices)
         d.indices = arr(self.indices.astype(np.int32), ctypes.c_int32)
         sph = (capi.BbSphere * max(1, len(self.spheres)))(*self.spheres)
@@ -248,6 +298,10 @@ class SceneData:
         d.materials = ctypes.cast(mats, ctypes.POINTER(capi.BbMaterial))
         d.n_lights = len(self.lights)
         d.lights = ctypes.cast(lights, ctypes.POINTER(capi.BbLight))
+        texs = (capi.BbTexture * max(1, len(self.textures)))(*self.textures)
+        self._keep.append(texs)
+        d.n_textures = len(self.textures)
+        d.textures = ctypes.cast(texs, ctypes.POINTER(capi.BbTexture))
         return d
 
 

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define BBGPU_ABI_VERSION 10
+#define BBGPU_ABI_VERSION 11
 
 /* status codes */
 enum {
@@ -66,15 +66,22 @@ enum {
     BB_MAT_METALLIC_ROUGHNESS = 4, /* MetallicRoughnessMaterial  metallic_roughness_material.cpp:28-39 */
 };
 
-/* Materials with constant textures.  Field meaning per type:
+/* Materials.  Field meaning per type:
  *   DIFFUSE:      color = reflectance, u_roughness = roughness (only 0 is supported: Lambertian)
  *   DIELECTRIC:   eta[0] = eta, color = reflectance, u/v_roughness
  *   CONDUCTOR:    eta, k, color = lobe reflectance (1,1,1 when absent), u/v_roughness
  *   CONDUCTOR_R:  k = physical reflectance R, color = lobe reflectance, u/v_roughness
  *   METALLIC_ROUGHNESS: color = basecolor, metallic, u/v_roughness
- * energy_compensation selects the *MultiScattering BxDF variants, which are
- * supported only where they coincide with the single-scattering BxDF
- * (smooth dielectric). */
+ * energy_compensation selects the *MultiScattering BxDF variants.
+ * Texture slots hold 1 + an index into BbSceneDesc.textures, or 0 for the
+ * constant above (so a zeroed struct is a constant material):
+ *   tex_color       the reflectance / basecolor texture (3 channels)
+ *   tex_u_roughness, tex_v_roughness, tex_metallic   (1 channel; not on DIFFUSE)
+ *   tex_normal      normal map, Intersection::GetBSDF's NormalMapping
+ *                   (src/geometry/intersection.cpp:9-30, 3 channels)
+ *   tex_alpha       alpha, Primitive::Intersect's stochastic alpha test
+ *                   (include/bulbit/primitive.h:45-77, 1 channel; triangles
+ *                   only, DielectricMaterial has none) */
 typedef struct BbMaterial {
     int32_t type;
     int32_t energy_compensation;
@@ -84,7 +91,38 @@ typedef struct BbMaterial {
     float u_roughness;
     float v_roughness;
     float metallic;
+    int32_t tex_color;
+    int32_t tex_u_roughness;
+    int32_t tex_v_roughness;
+    int32_t tex_metallic;
+    int32_t tex_normal;
+    int32_t tex_alpha;
 } BbMaterial;
+
+enum {
+    BB_TEX_CONSTANT = 0, /* ConstantTexture  include/bulbit/textures.h:11-33 */
+    BB_TEX_IMAGE = 1,    /* ImageTexture (bilinear)  textures.h:35-107 */
+    BB_TEX_CHECKER = 2,  /* CheckerTexture  textures.h:109-142 */
+};
+enum { BB_TEXCOORD_REPEAT = 0, BB_TEXCOORD_CLAMP = 1 }; /* TexCoordFilter, include/bulbit/texture.h:20-46 */
+
+/* A Float (channels 1) or Spectrum (channels 3) texture.
+ *   CONSTANT: value (value[0] for one channel)
+ *   IMAGE:    width x height texels of `channels` floats, texel (x, y) at
+ *             x + y * width (Image<T>::operator[]), linear values; filter
+ *   CHECKER:  checker = 1 + the indices of textures a and b (constant or
+ *             image, with this texture's channel count), resolution */
+typedef struct BbTexture {
+    int32_t type;
+    int32_t channels;
+    int32_t width;
+    int32_t height;
+    int32_t filter;
+    int32_t checker[2];
+    float resolution[2];
+    float value[3];
+    const float* texels;
+} BbTexture;
 
 enum {
     BB_LIGHT_SPOT = 0,              /* SpotLight            src/light/spot_light.cpp:7-24 */
@@ -110,7 +148,7 @@ typedef struct BbLight {
 /* Host-side scene description.  Vertex data is in world space (the Mesh
  * constructor's transform already applied, src/geometry/mesh.cpp:8-39).
  * normals and tangents are required when triangles are present; uvs may be
- * NULL (they only feed texture lookups, and all textures are constant). */
+ * NULL (every texture coordinate is then (0, 0)); they feed texture lookups. */
 typedef struct BbSceneDesc {
     int32_t n_vertices;
     const float* positions;  /* 3 * n_vertices */
@@ -127,6 +165,8 @@ typedef struct BbSceneDesc {
     const BbMaterial* materials;
     int32_t n_lights;
     const BbLight* lights;
+    int32_t n_textures;
+    const BbTexture* textures;
 } BbSceneDesc;
 
 /* LinearBVHNode, include/bulbit/bvh.h:46-58: 32 bytes, same field meaning. */

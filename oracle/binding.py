@@ -74,6 +74,8 @@ def load_oracle():
     lib.orc_scene_build_seconds.argtypes = [c_void_p]
     lib.orc_trace.argtypes = [c_void_p, c_void_p, c_int32, c_int32, c_void_p, c_int32, POINTER(capi.BbTraceCounters)]
     lib.orc_trace_brute.argtypes = [c_void_p, c_void_p, c_int32, c_void_p]
+    lib.orc_texture_eval.argtypes = [c_void_p, c_int32, POINTER(c_float), POINTER(c_float)]
+    lib.orc_set_cr_libm.argtypes = [c_int32]
     lib.orc_render.restype = c_double
     lib.orc_render.argtypes = [c_void_p, POINTER(capi.BbCameraDesc), POINTER(capi.BbRenderParams), c_int32, c_int32,
                                c_int32, c_int32, c_void_p, c_void_p, c_int32]
@@ -85,6 +87,14 @@ def load_oracle():
 
 def fptr(a):
     return np.ascontiguousarray(a, dtype=np.float32).ctypes.data_as(POINTER(c_float))
+
+
+def correctly_rounded_libm(on=True):
+    """Switch the oracle's sampling transcendentals (sin, cos, acos, atan2,
+    pow) between glibc's float functions (the reference's, the default) and
+    double evaluation rounded to float (the GPU's).  For tests that separate
+    the libm difference from everything else."""
+    load_oracle().orc_set_cr_libm(1 if on else 0)
 
 
 class OracleScene:
@@ -113,6 +123,12 @@ class OracleScene:
     @property
     def build_seconds(self):
         return self.lib.orc_scene_build_seconds(self.h)
+
+    def texture(self, index, uv):
+        """Texture<T>::Evaluate of texture `index` (0-based) at uv -> 3 floats"""
+        out = (c_float * 3)()
+        self.lib.orc_texture_eval(self.h, int(index), (c_float * 2)(*uv), out)
+        return np.array(out, np.float32)
 
     def bvh(self):
         nodes = np.zeros(self.n_nodes, capi.BVH_NODE_DTYPE)

@@ -32,6 +32,17 @@
 
 /* ================================================================ hashing */
 
+/* Transcendentals on the sampling path.  The reference calls glibc's float
+ * functions, and so does the oracle by default.  orc_set_cr_libm(1) (tests
+ * only) evaluates them in double and rounds, as the GPU path does, so that a
+ * comparison can separate what the libm difference causes from the rest. */
+static int orc_cr_libm = 0;
+static inline float o_sinf(float x) { return orc_cr_libm ? (float)sin((double)x) : sinf(x); }
+static inline float o_cosf(float x) { return orc_cr_libm ? (float)cos((double)x) : cosf(x); }
+static inline float o_acosf(float x) { return orc_cr_libm ? (float)acos((double)x) : acosf(x); }
+static inline float o_atan2f(float y, float x) { return orc_cr_libm ? (float)atan2((double)y, (double)x) : atan2f(y, x); }
+static inline float o_powf(float x, float y) { return orc_cr_libm ? (float)pow((double)x, (double)y) : powf(x, y); }
+
 /* include/bulbit/hash.h:17-70 (MurmurHash64A) */
 uint64_t orc_murmur64a(const void* key_, int32_t len, uint64_t seed)
 {
@@ -155,7 +166,7 @@ static v2 sample_concentric(v2 u)
         r = uo.y;
         theta = (orc_pi / 2.0f) - (orc_pi / 4.0f) * (uo.x / uo.y);
     }
-    return V2(cosf(theta) * r, sinf(theta) * r);
+    return V2(o_cosf(theta) * r, o_sinf(theta) * r);
 }
 
 /* sampling.h:64-74 */
@@ -164,7 +175,7 @@ static v3 sample_uniform_sphere(v2 u)
     float z = 1.0f - 2.0f * u.x;
     float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
     float phi = orc_two_pi * u.y;
-    return V3(r * cosf(phi), r * sinf(phi), z);
+    return V3(r * o_cosf(phi), r * o_sinf(phi), z);
 }
 
 /* sampling.h:151-169 (Malley's method) */
@@ -484,6 +495,9 @@ struct OrcScene {
     BbMaterial* mats;
     BbLight* lights;
     light_extra* lx;
+    float* uvs;        /* 2 per vertex, NULL: every texture coordinate is (0, 0) */
+    int32_t n_textures;
+    BbTexture* texs;   /* texels point at owned copies */
     /* BVH */
     BbBvhNode* nodes;
     int32_t n_nodes;
@@ -594,7 +608,9 @@ static inline int sphere_intersect(const BbSphere* sp, v3 o, v3 d, float t_min, 
 typedef struct {
     int32_t prim;
     float t, u, v;
+    v3 ro, rd; /* the ray the accepted candidate was found on (a restarted one after an alpha rejection) */
     v3 point, normal, ns, ts;
+    v2 uv;
     int front_face;
 } isect_t;
 
@@ -609,11 +625,30 @@ static void set_face_normal(isect_t* is, v3 wi, v3 outward, v3 shading_n, v3 sha
     is->ts = vmul(shading_t, sign);
 }
 
+/* Sphere's ComputeTexCoord (src/geometry/sphere.cpp:9-16) */
+static v2 sphere_texcoord(v3 n)
+{
+    float theta = o_acosf(clampf_(n.y, -1.0f, 1.0f));
+    float r = o_atan2f(n.z, n.x);
+    float phi = r < 0.0f ? r + orc_two_pi : r;
+    return V2(phi * orc_inv_two_pi, 1.0f - theta * orc_inv_pi);
+}
+
+/* Triangle::GetTexCoord (triangle.cpp:31-38) */
+static v2 triangle_texcoord(const OrcScene* s, const int32_t* v, float u, float vv, float w)
+{
+    if (!s->uvs) return V2(0.0f, 0.0f);
+    const float* a = s->uvs + 2 * v[0];
+    const float* b = s->uvs + 2 * v[1];
+    const float* c = s->uvs + 2 * v[2];
+    return V2(w * a[0] + u * b[0] + vv * c[0], w * a[1] + u * b[1] + vv * c[1]);
+}
+
 /* The shading part of Triangle::Intersect (triangle.cpp:99-107) and
  * Sphere::Intersect (sphere.cpp:63-82), evaluated once for the final closest
- * candidate; the reference evaluates it for every accepted candidate with the
- * same inputs, so the last evaluation is identical.  uv is not computed: it
- * feeds only texture lookups, and every texture here is constant. */
+ * candidate on the ray it was found on; the reference evaluates it for every
+ * accepted candidate with the same inputs, so the last evaluation is
+ * identical. */
 static void finish_isect(const OrcScene* s, v3 o, v3 d, isect_t* is)
 {
     const BbPrimitive* pr = &s->prims[is->prim];
@@ -624,6 +659,7 @@ static void finish_isect(const OrcScene* s, v3 o, v3 d, isect_t* is)
         float u = is->u, vv = is->v;
         float w = 1.0f - u - vv;
         is->point = vadd(o, vmul(d, is->t));
+        is->uv = triangle_texcoord(s, v, u, vv, w);
         v3 normal = vnormalize(vcross(e1, e2));
         /* Triangle::GetNormal / GetTangent (triangle.cpp:13-29) */
         v3 n0 = vtx(s, s->nrm, v[0]), n1 = vtx(s, s->nrm, v[1]), n2 = vtx(s, s->nrm, v[2]);
@@ -641,6 +677,7 @@ static void finish_isect(const OrcScene* s, v3 o, v3 d, isect_t* is)
         v3 point = vadd(ro, vmul(rd, is->t));
         v3 normal = vnormalize(vdiv(point, sp->radius));
         is->point = vadd(quat_rotate(q, vmulv(sc, point)), p);
+        is->uv = sphere_texcoord(normal);
         v3 n = quat_rotate(q, normal);
         v3 t;
         if (n.y < 1.0f - 1e-4f)
@@ -860,8 +897,80 @@ static inline int box_test(const BbBvhNode* nd, v3 o, float t_min, float t_max, 
     return 1;
 }
 
-static inline int prim_intersect(const OrcScene* s, int32_t prim, v3 o, v3 d, float t_min, float t_max, float* t,
-                                 float* u, float* v)
+/* ------------------------------------------------------------ textures */
+
+/* FilterTexCoord (include/bulbit/texture.h:26-46) */
+static int32_t filter_coord(int32_t i, int32_t n, int32_t filter)
+{
+    if (filter == BB_TEXCOORD_CLAMP) return i < 0 ? 0 : (i > n - 1 ? n - 1 : i);
+    return i >= 0 ? i % n : n - (-i % n) - 1;
+}
+
+/* Texture<T>::Evaluate: ConstantTexture, ImageTexture (bilinear,
+ * textures.h:60-88), CheckerTexture (textures.h:109-130) */
+static void texture_eval(const OrcScene* s, int32_t idx, v2 uv, float out[3])
+{
+    const BbTexture* t = &s->texs[idx];
+    int c = t->channels;
+    if (t->type == BB_TEX_CONSTANT) {
+        for (int k = 0; k < 3; ++k) out[k] = c == 1 ? t->value[0] : t->value[k];
+        return;
+    }
+    if (t->type == BB_TEX_CHECKER) {
+        int32_t sx = (int32_t)(uv.x * t->resolution[0]);
+        int32_t sy = (int32_t)(uv.y * t->resolution[1]);
+        int32_t cs = sx + sy;
+        texture_eval(s, (cs % 2) ? t->checker[0] - 1 : t->checker[1] - 1, uv, out);
+        return;
+    }
+    float w = uv.x * (float)t->width - 0.5f;
+    float h = uv.y * (float)t->height - 0.5f;
+    int32_t i0 = (int32_t)floorf(w), i1 = i0 + 1;
+    int32_t j0 = (int32_t)floorf(h), j1 = j0 + 1;
+    i0 = filter_coord(i0, t->width, t->filter);
+    j0 = filter_coord(j0, t->height, t->filter);
+    i1 = filter_coord(i1, t->width, t->filter);
+    j1 = filter_coord(j1, t->height, t->filter);
+    float fu = w - floorf(w);
+    float fv = h - floorf(h);
+    const float* v00 = t->texels + (size_t)(i0 + j0 * t->width) * c;
+    const float* v10 = t->texels + (size_t)(i1 + j0 * t->width) * c;
+    const float* v01 = t->texels + (size_t)(i0 + j1 * t->width) * c;
+    const float* v11 = t->texels + (size_t)(i1 + j1 * t->width) * c;
+    for (int k = 0; k < c; ++k)
+        out[k] = (1 - fu) * (1 - fv) * v00[k] + (1 - fu) * (fv) * v01[k] + (fu) * (1 - fv) * v10[k] + (fu) * (fv) * v11[k];
+    for (int k = c; k < 3; ++k) out[k] = out[0];
+}
+
+static float texture1(const OrcScene* s, int32_t idx, v2 uv)
+{
+    float v[3];
+    texture_eval(s, idx, uv, v);
+    return v[0];
+}
+
+static spec texture3(const OrcScene* s, int32_t idx, v2 uv)
+{
+    float v[3];
+    texture_eval(s, idx, uv, v);
+    return S3(v[0], v[1], v[2]);
+}
+
+void orc_texture_eval(const OrcScene* s, int32_t idx, const float uv[2], float out[3])
+{
+    texture_eval(s, idx, V2(uv[0], uv[1]), out);
+}
+
+/* HashFloat(ray, point) (hash.h:111-126): MurmurHash64A over Ray{o, d} and the point */
+static float hash_float_ray_point(v3 o, v3 d, v3 p)
+{
+    float buf[9] = { o.x, o.y, o.z, d.x, d.y, d.z, p.x, p.y, p.z };
+    return (float)(uint32_t)orc_murmur64a(buf, 36, 0) * 0x1p-32f;
+}
+
+/* Shape::Intersect of a primitive's shape */
+static inline int shape_intersect(const OrcScene* s, int32_t prim, v3 o, v3 d, float t_min, float t_max, float* t,
+                                  float* u, float* v)
 {
     const BbPrimitive* pr = &s->prims[prim];
     if (pr->shape_kind == BB_SHAPE_TRIANGLE) {
@@ -872,6 +981,39 @@ static inline int prim_intersect(const OrcScene* s, int32_t prim, v3 o, v3 d, fl
     *u = 0.0f;
     *v = 0.0f;
     return sphere_intersect(&s->spheres[pr->shape_index], o, d, t_min, t_max, t);
+}
+
+/* Primitive::Intersect (include/bulbit/primitive.h:45-77): the shape test,
+ * then the material's alpha test; a rejected hit re-intersects the same
+ * primitive from the hit point along Normalize(d) over
+ * [Ray::epsilon, t_max - t].  *o and *d become the ray of the accepted hit. */
+static int prim_intersect(const OrcScene* s, int32_t prim, v3* o, v3* d, float t_min, float t_max, float* t, float* u,
+                          float* v)
+{
+    if (!shape_intersect(s, prim, *o, *d, t_min, t_max, t, u, v)) return 0;
+    const BbPrimitive* pr = &s->prims[prim];
+    if (pr->material < 0) return 1;
+    const BbMaterial* m = &s->mats[pr->material];
+    if (m->tex_alpha == 0 || m->type == BB_MAT_DIELECTRIC) return 1;
+    for (;;) {
+        isect_t is;
+        memset(&is, 0, sizeof(is));
+        is.prim = prim;
+        is.t = *t;
+        is.u = *u;
+        is.v = *v;
+        finish_isect(s, *o, *d, &is);
+        float alpha = texture1(s, m->tex_alpha - 1, is.uv);
+        if (!(alpha < 1.0f)) return 1;
+        float p = alpha <= 0.0f ? 1.0f : hash_float_ray_point(*o, *d, is.point);
+        if (!(p > alpha)) return 1;
+        float t_max2 = t_max - *t;
+        *d = vnormalize(*d);
+        *o = is.point;
+        t_min = 1e-4f;
+        t_max = t_max2;
+        if (!shape_intersect(s, prim, *o, *d, t_min, t_max, t, u, v)) return 0;
+    }
 }
 
 #define ORC_STACK 1024
@@ -897,8 +1039,9 @@ static int bvh_raycast(const OrcScene* s, v3 o, v3 d, float t_min, float t_max, 
             for (int32_t i = 0; i < nd->count; ++i) {
                 int32_t prim = s->ordered[nd->offset + i];
                 float t, u, v;
+                v3 po = o, pd = d;
                 if (cnt) cnt->prims++;
-                if (prim_intersect(s, prim, o, d, t_min, t_max, &t, &u, &v)) {
+                if (prim_intersect(s, prim, &po, &pd, t_min, t_max, &t, &u, &v)) {
                     if (cnt) cnt->accepts++;
                     hit = 1;
                     if (any_hit) return 1;
@@ -906,6 +1049,8 @@ static int bvh_raycast(const OrcScene* s, v3 o, v3 d, float t_min, float t_max, 
                     out->t = t;
                     out->u = u;
                     out->v = v;
+                    out->ro = po;
+                    out->rd = pd;
                     if (t <= t_min) return 1;
                     t_max = t;
                 }
@@ -1061,8 +1206,8 @@ static v3 ggx_sample_wm(ggx m, v3 wo, v2 u)
     float phi = orc_two_pi * u.x;
     float z = fmaf((1.0f - u.y), (1.0f + wo_std.z), -wo_std.z);
     float sin_t = sqrtf(clampf_(1.0f - z * z, 0.0f, 1.0f));
-    float x = sin_t * cosf(phi);
-    float y = sin_t * sinf(phi);
+    float x = sin_t * o_cosf(phi);
+    float y = sin_t * o_sinf(phi);
     v3 h = vadd(V3(x, y, z), wo_std);
     return vnormalize(V3(h.x * m.ax, h.y * m.ay, h.z));
 }
@@ -1633,6 +1778,17 @@ static void dms_build(void)
     for (int32_t i = 0; i < started; ++i) pthread_join(th[i], NULL);
 }
 
+static const dms_tables_t* dms_tables(void);
+
+/* the tables are built first, in the default mode, so that they stay equal
+ * to the library's whatever the mode */
+void orc_set_cr_libm(int32_t on)
+{
+    (void)tables();
+    (void)dms_tables();
+    orc_cr_libm = on;
+}
+
 static const dms_tables_t* dms_tables(void)
 {
     pthread_once(&g_dms_once, dms_build);
@@ -1923,7 +2079,7 @@ static spec mr_F0(const bxdf_t* b)
 }
 static spec mr_schlick(spec f0, float c)
 {
-    float p = powf(1.0f - c, 5.0f);
+    float p = o_powf(1.0f - c, 5.0f);
     return sadd(f0, smulf(ssub(S1(1.0f), f0), p));
 }
 static spec mr_f(const bxdf_t* b, v3 wo, v3 wi)
@@ -2005,7 +2161,7 @@ typedef struct {
  * diffuse_material.cpp:21-36, dielectric_material.cpp:28-52,
  * conductor_material.cpp:52-99, metallic_roughness_material.cpp:28-39).
  * BSDF(n, t, bxdf) builds Frame::FromXZ(t, n) (bsdf.h:64-68). */
-static int make_bsdf(const BbMaterial* m, v3 ns, v3 ts, int front_face, bsdf_t* out)
+static int make_bsdf_c(const BbMaterial* m, v3 ns, v3 ts, int front_face, bsdf_t* out)
 {
     bxdf_t* b = &out->bx;
     memset(b, 0, sizeof(*b));
@@ -2056,6 +2212,40 @@ static int make_bsdf(const BbMaterial* m, v3 ns, v3 ts, int front_face, bsdf_t* 
         return 1;
     }
     return 0;
+}
+
+/* make_bsdf_c with the material's textures evaluated at uv: each texture
+ * slot replaces the constant the material would otherwise pass (the
+ * *Material::GetBSDF functions evaluate every parameter texture at isect.uv) */
+static int make_bsdf(const OrcScene* s, const BbMaterial* m0, v2 uv, v3 ns, v3 ts, int front_face, bsdf_t* out)
+{
+    BbMaterial m = *m0;
+    if (s) {
+        if (m.tex_color) {
+            spec c = texture3(s, m.tex_color - 1, uv);
+            m.color[0] = c.r;
+            m.color[1] = c.g;
+            m.color[2] = c.b;
+        }
+        if (m.tex_u_roughness && m.type != BB_MAT_DIFFUSE) m.u_roughness = texture1(s, m.tex_u_roughness - 1, uv);
+        if (m.tex_v_roughness && m.type != BB_MAT_DIFFUSE) m.v_roughness = texture1(s, m.tex_v_roughness - 1, uv);
+        if (m.tex_metallic && m.type == BB_MAT_METALLIC_ROUGHNESS) m.metallic = texture1(s, m.tex_metallic - 1, uv);
+    }
+    return make_bsdf_c(&m, ns, ts, front_face, out);
+}
+
+/* NormalMapping (src/geometry/intersection.cpp:9-30) */
+static void normal_mapping(const OrcScene* s, int32_t idx, v3 wo, isect_t* is)
+{
+    spec sp = texture3(s, idx, is->uv);
+    v3 sn = V3(sp.r * 2 - 1, sp.g * 2 - 1, sp.b * 2 - 1);
+    vnormalize_m(&sn);
+    frame3 tbn = frame_from_xz(is->ts, is->ns);
+    v3 n = frame_from_local(&tbn, sn);
+    if (vdot(n, wo) < 0.0f) n = reflect(n, is->normal);
+    v3 t = vsub(is->ts, vmul(n, vdot(is->ts, n)));
+    is->ns = n;
+    is->ts = t;
 }
 
 /* BSDF::f (bsdf.h:75-86) */
@@ -2114,7 +2304,7 @@ static int material_supported(const BbMaterial* m, char* err, int32_t err_size)
 {
     switch (m->type) {
     case BB_MAT_DIFFUSE:
-        if (m->u_roughness > 0.0f) {
+        if (m->u_roughness > 0.0f || m->tex_u_roughness) {
             snprintf(err, err_size, "diffuse roughness > 0 (EON BxDF) is not supported");
             return 0;
         }
@@ -2132,7 +2322,7 @@ int32_t orc_bsdf_eval(const BbMaterial* mat, const float n[3], const float t[3],
                       const float wi[3], float f_out[3], float* pdf_out)
 {
     bsdf_t b;
-    make_bsdf(mat, V3(n[0], n[1], n[2]), V3(t[0], t[1], t[2]), front_face, &b);
+    make_bsdf(NULL, mat, V2(0.0f, 0.0f), V3(n[0], n[1], n[2]), V3(t[0], t[1], t[2]), front_face, &b);
     spec f = bsdf_f(&b, V3(wo[0], wo[1], wo[2]), V3(wi[0], wi[1], wi[2]));
     f_out[0] = f.r;
     f_out[1] = f.g;
@@ -2146,7 +2336,7 @@ int32_t orc_bsdf_sample(const BbMaterial* mat, const float n[3], const float t[3
                         float* pdf_out, int32_t* flags_out, float* eta_out)
 {
     bsdf_t b;
-    make_bsdf(mat, V3(n[0], n[1], n[2]), V3(t[0], t[1], t[2]), front_face, &b);
+    make_bsdf(NULL, mat, V2(0.0f, 0.0f), V3(n[0], n[1], n[2]), V3(t[0], t[1], t[2]), front_face, &b);
     bsample s;
     memset(&s, 0, sizeof(s));
     int ok = bsdf_sample(&b, V3(wo[0], wo[1], wo[2]), u0, V2(u12[0], u12[1]), &s);
@@ -2323,6 +2513,10 @@ void orc_scene_destroy(OrcScene* s)
     free(s->prims);
     free(s->mats);
     free(s->lights);
+    free(s->uvs);
+    if (s->texs)
+        for (int32_t i = 0; i < s->n_textures; ++i) free((void*)s->texs[i].texels);
+    free(s->texs);
     free(s->lx);
     free(s->nodes);
     free(s->ordered);
@@ -2362,6 +2556,18 @@ OrcScene* orc_scene_create(const BbSceneDesc* d, char* err, int32_t err_size)
     s->prims = (BbPrimitive*)dup_bytes(d->primitives, sizeof(BbPrimitive) * (size_t)d->n_primitives);
     s->mats = (BbMaterial*)dup_bytes(d->materials, sizeof(BbMaterial) * (size_t)d->n_materials);
     s->lights = (BbLight*)dup_bytes(d->lights, sizeof(BbLight) * (size_t)d->n_lights);
+    s->uvs = d->uvs ? (float*)dup_bytes(d->uvs, sizeof(float) * 2 * (size_t)d->n_vertices) : NULL;
+    s->n_textures = d->n_textures;
+    s->texs = (BbTexture*)calloc((size_t)(d->n_textures + 1), sizeof(BbTexture));
+    for (int32_t i = 0; i < d->n_textures; ++i) {
+        s->texs[i] = d->textures[i];
+        if (d->textures[i].type == BB_TEX_IMAGE)
+            s->texs[i].texels = (float*)dup_bytes(d->textures[i].texels, sizeof(float) * (size_t)d->textures[i].width *
+                                                                                 (size_t)d->textures[i].height *
+                                                                                 (size_t)d->textures[i].channels);
+        else
+            s->texs[i].texels = NULL;
+    }
     if (d->n_triangles > 0 && (!d->positions || !d->normals || !d->tangents || !d->indices))
         FAIL("triangles need positions, normals, tangents and indices");
     for (int32_t i = 0; i < 3 * d->n_triangles; ++i)
@@ -2522,7 +2728,7 @@ static spec path_li(const OrcScene* s, const BbRenderParams* P, v3 ro, v3 rd, pc
             }
             break;
         }
-        finish_isect(s, ro, rd, &is);
+        finish_isect(s, is.ro, is.rd, &is);
         v3 wo = vnormalize(vneg(rd));
 
         int32_t al = s->prim_light[is.prim];
@@ -2551,8 +2757,10 @@ static spec path_li(const OrcScene* s, const BbRenderParams* P, v3 ro, v3 rd, pc
             --bounce;
             continue;
         }
+        /* Intersection::GetBSDF (intersection.cpp:32-52): normal map, then the material */
+        if (s->mats[mi].tex_normal) normal_mapping(s, s->mats[mi].tex_normal - 1, wo, &is);
         bsdf_t bsdf;
-        make_bsdf(&s->mats[mi], is.ns, is.ts, is.front_face, &bsdf);
+        make_bsdf(s, &s->mats[mi], is.uv, is.ns, is.ts, is.front_face, &bsdf);
         int flags = bxdf_flags(&bsdf.bx);
         if (flags & (BX_DIFFUSE | BX_GLOSSY)) L = sadd(L, sample_direct_light(s, wo, &is, &bsdf, g, beta));
 
@@ -2760,7 +2968,8 @@ void orc_trace_brute(const OrcScene* s, const BbRay* rays, int32_t n, BbHit* hit
         hits[i].t = hits[i].u = hits[i].v = 0.0f;
         for (int32_t p = 0; p < s->n_prims; ++p) {
             float t, u, v;
-            if (prim_intersect(s, p, o, d, r->t_min, t_max, &t, &u, &v)) {
+            v3 po = o, pd = d;
+            if (prim_intersect(s, p, &po, &pd, r->t_min, t_max, &t, &u, &v)) {
                 hits[i].prim = p;
                 hits[i].t = t;
                 hits[i].u = u;

@@ -31,7 +31,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
-from bulbit_amd import scenes  # noqa: E402
+from bulbit_amd import capi, scenes  # noqa: E402
 
 C2_RAY_SEED = 20261202
 C2_RAYS = 10_000_000
@@ -48,7 +48,10 @@ def sha(*arrays) -> str:
 
 
 def scene_digest(sc) -> str:
-    mats = np.frombuffer(b"".join(bytes(m) for m in sc.materials), np.uint8)
+    # the constant material fields (the texture slots that follow them are
+    # zero in these scenes)
+    const = capi.BbMaterial.tex_color.offset
+    mats = np.frombuffer(b"".join(bytes(m)[:const] for m in sc.materials), np.uint8)
     lights = np.frombuffer(b"".join(bytes(l) for l in sc.lights), np.uint8)
     return sha(sc.positions.astype(np.float32), sc.normals.astype(np.float32), sc.tangents.astype(np.float32),
                sc.indices.astype(np.int32), sc.prims.astype(np.int32), mats, lights)

@@ -343,6 +343,34 @@ def test_large_scene_generator_parity():
     assert gi.mean() > 0.01  # the lights reach the camera
 
 
+@pytest.mark.parametrize("which", ["readme", "ico6", "large"])
+def test_render_bit_exact_with_correctly_rounded_libm(readme, ico6, which):
+    """The only difference between the GPU path and the oracle is libm: with
+    the oracle's sampling transcendentals evaluated in double and rounded
+    (as the GPU does), film sums and counts are bit-equal."""
+    from oracle import correctly_rounded_libm
+
+    if which == "large":
+        from bulbit_amd.gpu import GpuScene
+        from oracle import OracleScene
+
+        sc, cam = scenes.large_scene(width=160, height=90, n_spheres=128, level=3, n_lights=64)
+        g, o = GpuScene(sc), OracleScene(sc)
+        p = scenes.render_params(4, max_bounces=12)
+    else:
+        sc, cam, g, o = readme if which == "readme" else ico6
+        p = scenes.render_params(4)
+    gs, gc, _ = g.render(cam, p)
+    try:
+        correctly_rounded_libm(True)
+        os_, oc, _ = o.render(cam, p, threads=THREADS)
+    finally:
+        correctly_rounded_libm(False)
+    assert np.array_equal(gc, oc)
+    log(f"{which}: {np.mean(gs == os_):.6f} of film values bit-equal with correctly rounded libm")
+    assert np.array_equal(gs.view(np.int32), os_.view(np.int32))
+
+
 def test_render_full_scene_crop_parity(c2):
     """The benchmark frame itself: GPU image rows against the oracle on a
     strided set of rows at 2 spp."""
