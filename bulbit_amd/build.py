@@ -54,6 +54,8 @@ CPP_LIB = os.path.join(HERE, "libbulbit.so")
 CPP_EXAMPLE = os.path.join(CPP, "_build", "readme_example")
 CPP_WRITE_IMAGES = os.path.join(CPP, "_build", "write_images")
 CPP_TEXTURED = os.path.join(CPP, "_build", "textured_example")
+CPP_LOAD_OBJ = os.path.join(CPP, "_build", "load_obj")
+CPP_CONVERT_IMAGE = os.path.join(CPP, "_build", "convert_image")
 # host C++ API (bulbit/bulbit.h) over the C ABI: plain g++, same FP rules
 CXX_FLAGS = ["-std=c++17", "-O2", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-pthread"]
 
@@ -83,7 +85,7 @@ def build_cpp(verbose: bool = False) -> str:
     inc = os.path.join(CPP, "include")
     tmp = CPP_LIB + ".tmp"
     _run([cxx, *CXX_FLAGS, "-fPIC", "-shared", "-I", inc, os.path.join(CPP, "bulbit.cpp"),
-          os.path.join(CPP, "image_io.cpp"), "-o", tmp,
+          os.path.join(CPP, "image_io.cpp"), os.path.join(CPP, "model_loader.cpp"), "-o", tmp,
           "-L", HERE, "-lbbgpu", "-Wl,-rpath,$ORIGIN"], "g++ building libbulbit.so", verbose)
     os.replace(tmp, CPP_LIB)
     os.makedirs(os.path.dirname(CPP_EXAMPLE), exist_ok=True)
@@ -93,6 +95,10 @@ def build_cpp(verbose: bool = False) -> str:
           "-L", HERE, "-lbulbit", "-lbbgpu", "-Wl,-rpath,$ORIGIN/../.."], "g++ building write_images", verbose)
     _run([cxx, *CXX_FLAGS, "-I", inc, os.path.join(CPP, "examples", "textured.cpp"), "-o", CPP_TEXTURED,
           "-L", HERE, "-lbulbit", "-lbbgpu", "-Wl,-rpath,$ORIGIN/../.."], "g++ building the textured example", verbose)
+    _run([cxx, *CXX_FLAGS, "-I", inc, os.path.join(CPP, "examples", "load_obj.cpp"), "-o", CPP_LOAD_OBJ,
+          "-L", HERE, "-lbulbit", "-lbbgpu", "-Wl,-rpath,$ORIGIN/../.."], "g++ building load_obj", verbose)
+    _run([cxx, *CXX_FLAGS, "-I", inc, os.path.join(CPP, "examples", "convert_image.cpp"), "-o", CPP_CONVERT_IMAGE,
+          "-L", HERE, "-lbulbit", "-lbbgpu", "-Wl,-rpath,$ORIGIN/../.."], "g++ building convert_image", verbose)
     return CPP_LIB
 
 

@@ -12,6 +12,9 @@
 #include "bulbit/bulbit.h"
 
 #include <algorithm>
+#include <cctype>
+#include <cstdlib>
+#include <functional>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -397,6 +400,225 @@ bool WriteImage(const Image3& image, const std::string& filename, ToneMappingCal
     std::fprintf(stderr, "bulbit_amd: WriteImage: %s: supported extensions are .png .jpg .hdr .pfm\n",
                  filename.c_str());
     return false;
+}
+
+// ------------------------------------------------------------ readers
+//
+// ReadImage1 / ReadImage3 (src/util/image.cpp:13-130) decode through
+// stb_image (pinned at ae721c50, not vendored in the reference) with
+// stbi_set_flip_vertically_on_load(true) and stbi_loadf: 8-bit samples c
+// become pow(c / 255, gamma) with gamma 2.2 for colour and 1 for non-colour
+// data (stbi__ldr_to_hdr), an alpha channel becomes c / 255, Radiance .hdr
+// RGBE becomes r * 2^(e - 136) (stbi__hdr_convert), and every value is
+// clamped at 0.  Decoded here: binary PNM (P5 grey, P6 RGB, 8 or 16 bits,
+// 16-bit samples reduced to their high byte as stb does), Radiance .hdr
+// (flat and RLE scanlines), and PFM (beyond stb: linear floats, already
+// bottom row first).
+
+namespace
+{
+
+struct Decoded
+{
+    int32 width = 0, height = 0, comps = 0;
+    std::vector<float> data; // comps per pixel, row 0 at the bottom
+};
+
+float ldr_to_float(unsigned char c, bool non_color)
+{
+    const float x = c / 255.0f;
+    return non_color ? float(std::pow(double(x), 1.0)) : float(std::pow(double(x), double(2.2f)));
+}
+
+std::string pnm_token(std::FILE* f)
+{
+    std::string t;
+    int c = std::fgetc(f);
+    while (c != EOF) {
+        if (c == '#') {
+            while (c != EOF && c != '\n') c = std::fgetc(f);
+        } else if (!std::isspace(c)) {
+            break;
+        }
+        c = std::fgetc(f);
+    }
+    while (c != EOF && !std::isspace(c)) {
+        t.push_back(char(c));
+        c = std::fgetc(f);
+    }
+    return t;
+}
+
+bool read_pnm(std::FILE* f, bool non_color, Decoded& out)
+{
+    const std::string magic = pnm_token(f);
+    if (magic != "P5" && magic != "P6") return false;
+    const int32 w = std::atoi(pnm_token(f).c_str()), h = std::atoi(pnm_token(f).c_str());
+    const int32 maxval = std::atoi(pnm_token(f).c_str());
+    if (w <= 0 || h <= 0 || maxval <= 0 || maxval > 65535) return false;
+    const int32 comps = magic == "P6" ? 3 : 1;
+    const int32 bytes = maxval > 255 ? 2 : 1;
+    std::vector<unsigned char> raw(size_t(w) * h * comps * bytes);
+    if (std::fread(raw.data(), 1, raw.size(), f) != raw.size()) return false;
+    out.width = w;
+    out.height = h;
+    out.comps = comps;
+    out.data.resize(size_t(w) * h * comps);
+    for (int32 y = 0; y < h; ++y)
+        for (int32 i = 0; i < w * comps; ++i) {
+            const size_t src = (size_t(y) * w * comps + i) * bytes; // first byte: the high byte of 16-bit samples
+            out.data[size_t(h - 1 - y) * w * comps + i] = ldr_to_float(raw[src], non_color);
+        }
+    return true;
+}
+
+bool read_hdr(std::FILE* f, Decoded& out)
+{
+    char line[256];
+    if (!std::fgets(line, sizeof(line), f) || std::strncmp(line, "#?", 2) != 0) return false;
+    for (;;) { // header lines up to the blank line
+        if (!std::fgets(line, sizeof(line), f)) return false;
+        if (line[0] == '\n' || line[0] == '\r') break;
+        if (std::strncmp(line, "FORMAT=", 7) == 0 && !std::strstr(line, "32-bit_rle_rgbe")) return false;
+    }
+    int32 w = 0, h = 0;
+    if (!std::fgets(line, sizeof(line), f) || std::sscanf(line, "-Y %d +X %d", &h, &w) != 2 || w <= 0 || h <= 0)
+        return false;
+    std::vector<unsigned char> rgbe(size_t(w) * h * 4);
+    std::vector<unsigned char> row(size_t(w) * 4);
+    for (int32 y = 0; y < h; ++y) {
+        unsigned char* dst = rgbe.data() + size_t(y) * w * 4;
+        int c0 = std::fgetc(f), c1 = std::fgetc(f), c2 = std::fgetc(f), c3 = std::fgetc(f);
+        if (c3 == EOF) return false;
+        if (w < 8 || w >= 32768 || c0 != 2 || c1 != 2 || (c2 & 0x80)) { // flat scanline
+            dst[0] = (unsigned char)c0;
+            dst[1] = (unsigned char)c1;
+            dst[2] = (unsigned char)c2;
+            dst[3] = (unsigned char)c3;
+            if (w > 1 && std::fread(dst + 4, 1, size_t(w - 1) * 4, f) != size_t(w - 1) * 4) return false;
+            continue;
+        }
+        if (((c2 << 8) | c3) != w) return false;
+        for (int k = 0; k < 4; ++k) { // RLE: each component in turn
+            int32 x = 0;
+            while (x < w) {
+                int count = std::fgetc(f);
+                if (count == EOF) return false;
+                if (count > 128) {
+                    count -= 128;
+                    const int v = std::fgetc(f);
+                    if (v == EOF || x + count > w) return false;
+                    for (int j = 0; j < count; ++j) row[size_t(x++) * 4 + k] = (unsigned char)v;
+                } else {
+                    if (count == 0 || x + count > w) return false;
+                    for (int j = 0; j < count; ++j) {
+                        const int v = std::fgetc(f);
+                        if (v == EOF) return false;
+                        row[size_t(x++) * 4 + k] = (unsigned char)v;
+                    }
+                }
+            }
+        }
+        std::memcpy(dst, row.data(), row.size());
+    }
+    out.width = w;
+    out.height = h;
+    out.comps = 3;
+    out.data.resize(size_t(w) * h * 3);
+    for (int32 y = 0; y < h; ++y)
+        for (int32 x = 0; x < w; ++x) {
+            const unsigned char* p = rgbe.data() + (size_t(y) * w + x) * 4;
+            float* o = out.data.data() + (size_t(h - 1 - y) * w + x) * 3;
+            if (p[3]) {
+                const float f1 = float(std::ldexp(1.0f, int(p[3]) - (128 + 8)));
+                o[0] = p[0] * f1;
+                o[1] = p[1] * f1;
+                o[2] = p[2] * f1;
+            } else {
+                o[0] = o[1] = o[2] = 0.0f;
+            }
+        }
+    return true;
+}
+
+bool read_pfm(std::FILE* f, Decoded& out)
+{
+    char magic[3] = { 0, 0, 0 };
+    if (std::fscanf(f, "%2s", magic) != 1) return false;
+    const int32 comps = std::strcmp(magic, "PF") == 0 ? 3 : (std::strcmp(magic, "Pf") == 0 ? 1 : 0);
+    int32 w = 0, h = 0;
+    float scale = 0.0f;
+    if (!comps || std::fscanf(f, "%d %d %f", &w, &h, &scale) != 3 || w <= 0 || h <= 0) return false;
+    std::fgetc(f); // the single whitespace before the data
+    out.width = w;
+    out.height = h;
+    out.comps = comps;
+    out.data.resize(size_t(w) * h * comps);
+    if (std::fread(out.data.data(), 4, out.data.size(), f) != out.data.size()) return false;
+    if (scale > 0) // big-endian data
+        for (float& v : out.data) {
+            uint32_t u;
+            std::memcpy(&u, &v, 4);
+            u = (u >> 24) | ((u >> 8) & 0xff00u) | ((u << 8) & 0xff0000u) | (u << 24);
+            std::memcpy(&v, &u, 4);
+        }
+    return true;
+}
+
+bool decode(const std::string& filename, bool non_color, Decoded& out)
+{
+    std::FILE* f = std::fopen(filename.c_str(), "rb");
+    if (!f) return false;
+    bool ok = false;
+    const std::string ext = filename.size() >= 4 ? filename.substr(filename.size() - 4) : "";
+    if (ext == ".hdr")
+        ok = read_hdr(f, out);
+    else if (ext == ".pfm")
+        ok = read_pfm(f, out);
+    else if (ext == ".ppm" || ext == ".pgm" || ext == ".pnm")
+        ok = read_pnm(f, non_color, out);
+    std::fclose(f);
+    return ok;
+}
+
+} // namespace
+
+Image1 ReadImage1(const std::string& filename, int32 channel, bool non_color, std::function<Float(Float)> transform)
+{
+    Decoded d;
+    if (!decode(filename, non_color, d)) {
+        std::fprintf(stderr, "Failed to read image1: %s\n", filename.c_str());
+        return {};
+    }
+    // a channel beyond the file's components (an alpha channel the file
+    // does not have, for one) gives an empty image, as in the reference
+    if (channel < 0 || channel >= d.comps) return {};
+    Image1 image(d.width, d.height);
+    for (size_t i = 0; i < image.data.size(); ++i) {
+        const float v = d.comps == 1 ? d.data[i] : d.data[i * size_t(d.comps) + size_t(channel)];
+        const Float c = Float(std::fmax(0.0f, v));
+        image.data[i] = transform ? transform(c) : c;
+    }
+    return image;
+}
+
+Image3 ReadImage3(const std::string& filename, bool non_color, std::function<Spectrum(Spectrum)> transform)
+{
+    Decoded d;
+    if (!decode(filename, non_color, d)) {
+        std::fprintf(stderr, "Failed to read image3: %s\n", filename.c_str());
+        return {};
+    }
+    Image3 image(d.width, d.height);
+    for (size_t i = 0; i < image.data.size(); ++i) {
+        Spectrum s;
+        for (int k = 0; k < 3; ++k) {
+            const float v = d.comps == 1 ? d.data[i] : d.data[i * size_t(d.comps) + size_t(k)];
+            s[k] = Float(std::fmax(0.0f, v));
+        }
+        image.data[i] = transform ? transform(s) : s;
+    }
+    return image;
 }
 
 } // namespace bulbit

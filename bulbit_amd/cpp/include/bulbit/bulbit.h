@@ -24,6 +24,10 @@
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
+#include <filesystem>
+#include <functional>
+#include <optional>
+#include <variant>
 #include <limits>
 #include <memory>
 #include <string>
@@ -155,6 +159,7 @@ struct Mat4
     constexpr Mat4(Identity) : ex{ 1, 0, 0, 0 }, ey{ 0, 1, 0, 0 }, ez{ 0, 0, 1, 0 }, ew{ 0, 0, 0, 1 } {}
     explicit constexpr Mat4(Float d) : ex{ d, 0, 0, 0 }, ey{ 0, d, 0, 0 }, ez{ 0, 0, d, 0 }, ew{ 0, 0, 0, d } {}
     constexpr Mat4(const Vec4& x, const Vec4& y, const Vec4& z, const Vec4& w) : ex{ x }, ey{ y }, ez{ z }, ew{ w } {}
+    Mat4(const struct Transform& t); // src/util/math.cpp:32-38
 };
 
 // M * v, summed left to right per row
@@ -182,6 +187,7 @@ struct Transform
     Quat q;
     Vec3 s{ 1, 1, 1 };
 
+
     constexpr Transform() = default;
     constexpr Transform(Identity) {}
     constexpr Transform(const Vec3& position) : p{ position } {}
@@ -195,6 +201,22 @@ struct Transform
     // camera transform at `position` looking at `target` (transform.h:117-122)
     static Transform LookAt(const Vec3& position, const Vec3& target, const Vec3& up);
 };
+
+// Mat4(Mat3(q), p) with the columns scaled by s (src/util/math.cpp:7-38)
+inline Mat4::Mat4(const Transform& t)
+{
+    const Quat& q = t.q;
+    const Float xx = q.x * q.x, yy = q.y * q.y, zz = q.z * q.z;
+    const Float xz = q.x * q.z, xy = q.x * q.y, yz = q.y * q.z;
+    const Float wx = q.w * q.x, wy = q.w * q.y, wz = q.w * q.z;
+    ex = Vec4(1 - 2 * (yy + zz), 2 * (xy + wz), 2 * (xz - wy), 0);
+    ey = Vec4(2 * (xy - wz), 1 - 2 * (xx + zz), 2 * (yz + wx), 0);
+    ez = Vec4(2 * (xz + wy), 2 * (yz - wx), 1 - 2 * (xx + yy), 0);
+    ew = Vec4(t.p, 1);
+    ex = Vec4(ex.x * t.s.x, ex.y * t.s.x, ex.z * t.s.x, ex.w * t.s.x);
+    ey = Vec4(ey.x * t.s.y, ey.y * t.s.y, ey.z * t.s.y, ey.w * t.s.y);
+    ez = Vec4(ez.x * t.s.z, ez.y * t.s.z, ez.z * t.s.z, ez.w * t.s.z);
+}
 
 // ----------------------------------------------------------------- spectrum
 
@@ -645,6 +667,39 @@ private:
     std::vector<std::unique_ptr<Material>> materials;
 };
 
+// ---------------------------------------------------------- model loading
+
+// AreaLightInfo (cli/primitive_builder.h:23-34), diffuse area lights only
+struct AreaLightInfo
+{
+    bool two_sided = false;
+    std::variant<Float, Spectrum, const SpectrumTexture*> emission = 0.0f;
+};
+
+// ModelLoaderOptions (cli/model_loader.h:8-19)
+struct ModelLoaderOptions
+{
+    bool flip_normal = false;
+    bool gen_smooth_normal = false;
+    bool flip_tex_coord = false;
+    bool use_fallback_material = false;
+
+    const Material* fallback_material = nullptr;
+    MediumInterface fallback_medium_interface = {};
+
+    std::optional<AreaLightInfo> area_light = {};
+};
+
+// LoadOBJ (cli/model_loader.cpp:803-1043): meshes per shape and material,
+// MetallicRoughness materials from the MTL file (CreateOBJMaterial,
+// :742-790), every triangle of a material group an area light with the
+// material's emission.  LoadModel dispatches on the extension; .gltf and
+// .glb are reported as unsupported.
+void LoadOBJ(Scene& scene, std::filesystem::path filename, const Transform& transform,
+             const ModelLoaderOptions& options = {});
+void LoadModel(Scene& scene, std::filesystem::path filename, const Transform& transform,
+               const ModelLoaderOptions& options = {});
+
 // --------------------------------------------------------------- threading
 
 // The GPU render does not use host worker threads; the pool is kept so that
@@ -836,6 +891,17 @@ inline ToneMappingCallback* default_tonemapping_callback = [](const Vec3& rgb) -
 // The film's row 0 is the bottom of the picture.
 bool WriteImage(const Image3& image, const std::string& filename,
                 ToneMappingCallback* callback = default_tonemapping_callback);
+
+using Image1 = Image<Float>;
+
+// ReadImage1 / ReadImage3 (include/bulbit/image.h:76-87): channel `channel`,
+// or RGB, of an image file as linear floats, bottom row first; 8-bit colour
+// data decoded with gamma 2.2 unless non_color.  Reads binary PNM (.ppm,
+// .pgm, .pnm), Radiance .hdr and .pfm; an empty image on failure.
+Image1 ReadImage1(const std::string& filename, int32 channel, bool non_color = false,
+                  std::function<Float(Float)> transform = {});
+Image3 ReadImage3(const std::string& filename, bool non_color = false,
+                  std::function<Spectrum(Spectrum)> transform = {});
 
 // --------------------------------------------------------- film, rendering
 
