@@ -150,8 +150,9 @@ struct SceneExport
         if (auto* d = dynamic_cast<const DiffuseMaterial*>(m)) {
             b.type = BB_MAT_DIFFUSE;
             param3(d->reflectance, Spectrum(0), b.color, b.tex_color);
-            // a roughness texture selects the EON BxDF, outside the path
-            b.u_roughness = b.v_roughness = constant_of<Float>(d->roughness, 0.0f, "DiffuseMaterial roughness");
+            // roughness > 0 selects the EON BxDF (per hit for a texture)
+            b.tex_u_roughness = param(d->roughness, 0.0f, b.u_roughness);
+            b.v_roughness = b.u_roughness;
             if (d->normal) b.tex_normal = add_texture(d->normal);
             if (d->alpha) b.tex_alpha = add_texture(d->alpha);
         } else if (auto* e = dynamic_cast<const DielectricMaterial*>(m)) {

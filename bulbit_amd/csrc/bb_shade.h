@@ -24,6 +24,7 @@ namespace bb
 BB_D float dsin(float x) { return (float)::sin((double)x); }
 BB_D float dcos(float x) { return (float)::cos((double)x); }
 BB_D float dpow(float x, float y) { return (float)::pow((double)x, (double)y); }
+BB_D float dacos(float x) { return (float)::acos((double)x); }
 
 // ------------------------------------------------------------------ sampling
 
@@ -323,10 +324,14 @@ BB_D GpuMaterial material_at(const GpuMaterial& m0, const SceneView& S, f2 uv)
     GpuMaterial m = m0;
     switch (m.type) {
     case BB_MAT_DIFFUSE:
-        if (m.tex_color >= 0) {
-            m.r = texture3(S, m.tex_color, uv);
-            m.flags_front = !is_black(m.r) ? (BX_DIFFUSE | BX_REFLECTION) : 0;
+        if (m.tex_color >= 0) m.r = texture3(S, m.tex_color, uv);
+        if (m.tex_u >= 0) {
+            // DiffuseMaterial::GetBSDF: EON when the roughness is > 0, else Lambertian
+            const float r = texture1(S, m.tex_u, uv);
+            m.kind = r > 0.0f ? MK_EON : MK_LAMBERT;
+            m.ax = m.ay = r;
         }
+        m.flags_front = !is_black(m.r) ? (BX_DIFFUSE | BX_REFLECTION) : 0;
         break;
     case BB_MAT_DIELECTRIC: {
         if (m.tex_color >= 0) m.r = sqrt_s(texture3(S, m.tex_color, uv));
@@ -372,6 +377,141 @@ BB_D void normal_map(const SceneView& S, int32_t idx, f2 uv, f3 wo, f3 normal, f
     const f3 t = ts - n * dot(ts, n);
     ns = n;
     ts = t;
+}
+
+// --------------------------------------------------------------- EON
+// EONBxDF (src/bxdf/eon_bxdf.cpp), the energy-preserving Oren-Nayar BRDF of
+// Portsmouth et al., "EON: A Practical Energy-Preserving Rough Diffuse BRDF"
+// (JCGT 14(1), 2025), exact variant: a single-scattering FON lobe plus a
+// multiple-scattering lobe that restores the energy FON loses, sampled by
+// mixing a clipped linearly transformed cosine (CLTC) with the uniform
+// hemisphere.  The fitted constants are the paper's.  b.r = rho, b.mf.ax = r.
+
+constexpr float kFON1 = 0.5f - 2.0f / (3.0f * kPi);           // constant1_FON
+constexpr float kFON2 = 2.0f / 3.0f - 28.0f / (15.0f * kPi);  // constant2_FON
+
+// E_FON_exact: the FON lobe's directional albedo
+BB_D float eon_E(float mu, float r)
+{
+    const float AF = 1.0f / (1.0f + kFON1 * r);
+    const float BF = r * AF;
+    const float Si = sqrtf(1.0f - (mu * mu));
+    const float G = Si * (dacos(mu) - Si * mu) + (2.0f / 3.0f) * ((Si / mu) * (1.0f - (Si * Si * Si)) - Si);
+    return AF + (BF * kInvPi) * G;
+}
+
+// ltc_coeffs (the paper's Listing 2)
+BB_D void eon_ltc(float mu, float r, float& a, float& b, float& c, float& d)
+{
+    a = 1.0f + r * (0.303392f + (-0.518982f + 0.111709f * mu) * mu + (-0.276266f + 0.335918f * mu) * r);
+    b = r * (-1.16407f + 1.15859f * mu + (0.150815f - 0.150105f * mu) * r) / (mu * mu * mu - 1.43545f);
+    c = 1.0f + (0.20013f + (-0.506373f + 0.261777f * mu) * mu) * r;
+    d = ((0.540852f + (-1.01625f + 0.475392f * mu) * mu) * r) / (-1.0743f + mu * (0.0725628f + mu));
+}
+
+// orthonormal_basis_ltc: X = wo's direction in the tangent plane, Y = Z x X, Z = z
+BB_D void eon_basis(f3 w, f3& X, f3& Y)
+{
+    const float len2 = w.x * w.x + w.y * w.y;
+    X = len2 > 0.0f ? mk3(w.x, w.y, 0.0f) / sqrtf(len2) : mk3(1.0f, 0.0f, 0.0f);
+    Y = mk3(-X.y, X.x, 0.0f);
+}
+
+BB_D float eon_cltc_pdf(f3 wo, f3 wi_local, float r)
+{
+    f3 X, Y;
+    eon_basis(wo, X, Y);
+    // Mul(transpose(X, Y, Z), wi)
+    const f3 wi = mk3(X.x * wi_local.x + X.y * wi_local.y + X.z * wi_local.z,
+                      Y.x * wi_local.x + Y.y * wi_local.y + Y.z * wi_local.z,
+                      0.0f * wi_local.x + 0.0f * wi_local.y + 1.0f * wi_local.z);
+    float a, b, c, d;
+    eon_ltc(wo.z, r, a, b, c, d);
+    const float detM = c * (a - b * d);
+    const f3 wh = mk3(c * (wi.x - b * wi.z), (a - b * d) * wi.y, -c * (d * wi.x - a * wi.z));
+    const float len2 = dot(wh, wh);
+    const float vz = 1.0f / sqrtf(d * d + 1.0f);
+    const float s = 0.5f * (1.0f + vz);
+    return detM * detM / (len2 * len2) * fmaxf(wh.z, 0.0f) / (kPi * s);
+}
+
+BB_D f3 eon_cltc_sample(f3 wo, float r, float u1, float u2, float& pdf)
+{
+    float a, b, c, d;
+    eon_ltc(wo.z, r, a, b, c, d);
+    const float R = sqrtf(u1);
+    const float phi = kTwoPi * u2;
+    float x = R * dcos(phi);
+    const float y = R * dsin(phi);
+    const float vz = 1.0f / sqrtf(d * d + 1.0f);
+    const float s = 0.5f * (1.0f + vz);
+    x = -(sqrtf(1.0f - y * y) * (1.0f - s) + x * s); // -Lerp(sqrt(1 - y^2), x, s)
+    const f3 wh = mk3(x, y, sqrtf(fmaxf(1.0f - (x * x + y * y), 0.0f)));
+    const float pdf_wh = wh.z / (kPi * s);
+    const f3 wi = mk3(a * wh.x + b * wh.z, c * wh.y, d * wh.x + wh.z);
+    const float len = length(wi);
+    const float detM = c * (a - b * d);
+    pdf = pdf_wh * len * len * len / detM;
+    f3 X, Y;
+    eon_basis(wo, X, Y);
+    // Normalize(Mul((X, Y, Z), wi))
+    return normalize(mk3(X.x * wi.x + Y.x * wi.y + 0.0f * wi.z, X.y * wi.x + Y.y * wi.y + 0.0f * wi.z,
+                         X.z * wi.x + Y.z * wi.y + 1.0f * wi.z));
+}
+
+BB_D spec eon_f(const bxdf& b, f3 wo, f3 wi)
+{
+    if (!same_hemisphere(wo, wi)) return mks(0.0f);
+    const float r = b.mf.ax;
+    const float mu_i = wi.z, mu_o = wo.z;
+    const float s = dot(wi, wo) - mu_i * mu_o;
+    const float sovertF = s > 0.0f ? s / smax(mu_i, mu_o) : s;
+    const float AF = 1.0f / (1.0f + kFON1 * r);
+    const spec f_ss = ((b.r * kInvPi) * AF) * (1.0f + r * sovertF);
+    const float EFo = eon_E(mu_o, r), EFi = eon_E(mu_i, r);
+    const float avgEF = AF * (1.0f + kFON2 * r);
+    const spec rho_ms = ((b.r * b.r) * avgEF) / (mks(1.0f) - b.r * (1.0f - avgEF));
+    const float eps = 1.0e-7f;
+    const spec f_ms = (((rho_ms * kInvPi) * smax(eps, 1.0f - EFo)) * smax(eps, 1.0f - EFi)) / smax(eps, 1.0f - avgEF);
+    return f_ss + f_ms;
+}
+
+// probability of the uniform lobe
+BB_D float eon_P_u(float mu, float r)
+{
+    return dpow(r, 0.1f) * (0.162925f + mu * (-0.372058f + (0.538233f - 0.290822f * mu) * mu));
+}
+
+BB_D float eon_pdf(const bxdf& b, f3 wo, f3 wi)
+{
+    if (!same_hemisphere(wo, wi)) return 0.0f;
+    const float P_u = eon_P_u(wo.z, b.mf.ax);
+    const float P_c = 1.0f - P_u;
+    return P_u * kInvTwoPi + P_c * eon_cltc_pdf(wo, wi, b.mf.ax);
+}
+
+BB_D bool eon_sample(const bxdf& b, f3 wo, float u0, f2 u12, bsample& out)
+{
+    const float r = b.mf.ax;
+    const float P_u = eon_P_u(wo.z, r);
+    const float P_c = 1.0f - P_u;
+    f3 wi;
+    float pdf_c;
+    if (u0 <= P_u) {
+        // SampleUniformHemisphere (include/bulbit/sampling.h:50-57)
+        const float z = u12.x;
+        const float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+        const float phi = kTwoPi * u12.y;
+        wi = mk3(rr * dcos(phi), rr * dsin(phi), z);
+        pdf_c = eon_cltc_pdf(wo, wi, r);
+    } else {
+        wi = eon_cltc_sample(wo, r, u12.x, u12.y, pdf_c);
+    }
+    out.pdf = P_u * kInvTwoPi + P_c * pdf_c;
+    out.f = eon_f(b, wo, wi);
+    out.wi = wi;
+    out.flags = BX_DIFFUSE | BX_REFLECTION;
+    return true;
 }
 
 // Lambertian, lambertian_bxdf.cpp:8-53
@@ -811,6 +951,7 @@ BB_D spec bxdf_f(const bxdf& b, f3 wo, f3 wi)
 {
     switch (b.kind) {
     case MK_LAMBERT: return lambert_f(b, wo, wi);
+    case MK_EON: return eon_f(b, wo, wi);
     case MK_DIELECTRIC: return dielectric_f(b, wo, wi);
     case MK_DIELECTRIC_MS: return dielectric_ms_f(b, wo, wi);
     case MK_CONDUCTOR: return conductor_f(b, wo, wi);
@@ -822,6 +963,7 @@ BB_D float bxdf_pdf(const bxdf& b, f3 wo, f3 wi)
 {
     switch (b.kind) {
     case MK_LAMBERT: return lambert_pdf(wo, wi);
+    case MK_EON: return eon_pdf(b, wo, wi);
     case MK_DIELECTRIC: return dielectric_pdf(b, wo, wi);
     case MK_DIELECTRIC_MS: return dielectric_ms_pdf(b, wo, wi);
     case MK_CONDUCTOR: return conductor_pdf(b, wo, wi);
@@ -845,6 +987,7 @@ BB_D bool bxdf_sample(const bxdf& b, f3 wo, float u0, f2 u12, bsample& s)
         s.flags = BX_DIFFUSE | BX_REFLECTION;
         return true;
     }
+    case MK_EON: return eon_sample(b, wo, u0, u12, s);
     case MK_DIELECTRIC_MS: return dielectric_ms_sample(b, wo, u0, u12, s);
     case MK_CONDUCTOR_MS:
         if (!ggx_smooth(b.mf)) {

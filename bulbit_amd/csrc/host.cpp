@@ -341,12 +341,10 @@ bool material_constants(const BbMaterial& m, GpuMaterial& g, std::string& err)
     g.energy_compensation = m.energy_compensation;
     switch (m.type) {
     case BB_MAT_DIFFUSE:
-        // DiffuseMaterial::GetBSDF: Lambertian when roughness <= 0 (diffuse_material.cpp:21-36)
-        if (m.u_roughness > 0.0f) {
-            err = "diffuse roughness > 0 (EON BxDF) is not supported";
-            return false;
-        }
-        g.kind = MK_LAMBERT;
+        // DiffuseMaterial::GetBSDF: EONBxDF when roughness > 0, else Lambertian
+        // (diffuse_material.cpp:21-36); the EON roughness is kept in ax and ay
+        g.kind = m.u_roughness > 0.0f ? MK_EON : MK_LAMBERT;
+        g.ax = g.ay = m.u_roughness > 0.0f ? m.u_roughness : 0.0f;
         g.r = mks(m.color[0], m.color[1], m.color[2]);
         g.flags_front = !is_black(g.r) ? (BX_DIFFUSE | BX_REFLECTION) : 0;
         return true;
@@ -448,10 +446,6 @@ bool material_to_gpu(const BbSceneDesc& d, const BbMaterial& m, GpuMaterial& g, 
 {
     if (!material_constants(m, g, err)) return false;
     const bool mr = m.type == BB_MAT_METALLIC_ROUGHNESS;
-    if (m.type == BB_MAT_DIFFUSE && m.tex_u_roughness != 0) {
-        err = "a diffuse roughness texture (EON BxDF) is not supported";
-        return false;
-    }
     if (!mr && m.tex_metallic != 0) {
         err = "a metallic texture belongs to a metallic-roughness material";
         return false;

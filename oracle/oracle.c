@@ -1374,7 +1374,7 @@ float orc_ggx_E_avg(float ax, float ay)
 
 /* ================================================================= BxDFs */
 
-enum { K_LAMBERT = 0, K_DIELECTRIC = 1, K_CONDUCTOR = 2, K_MR = 3, K_CONDUCTOR_MS = 4, K_DIELECTRIC_MS = 5 };
+enum { K_LAMBERT = 0, K_DIELECTRIC = 1, K_CONDUCTOR = 2, K_MR = 3, K_CONDUCTOR_MS = 4, K_DIELECTRIC_MS = 5, K_EON = 6 };
 
 typedef struct {
     int kind;
@@ -1398,7 +1398,8 @@ typedef struct {
 static int bxdf_flags(const bxdf_t* b)
 {
     switch (b->kind) {
-    case K_LAMBERT: return !sblack(b->r) ? (BX_DIFFUSE | BX_REFLECTION) : 0;
+    case K_LAMBERT:
+    case K_EON: return !sblack(b->r) ? (BX_DIFFUSE | BX_REFLECTION) : 0;
     case K_DIELECTRIC:
     case K_DIELECTRIC_MS: {
         int f = (b->eta == 1.0f) ? BX_TRANSMISSION : (BX_REFLECTION | BX_TRANSMISSION);
@@ -1432,6 +1433,132 @@ static int lambert_sample(const bxdf_t* b, v3 wo, v2 u12, bsample* s)
     s->flags = BX_DIFFUSE | BX_REFLECTION;
     s->eta = 1.0f;
     s->stochastic = 0;
+    return 1;
+}
+
+/* ---- EON: src/bxdf/eon_bxdf.cpp (Portsmouth et al., JCGT 14(1), 2025),
+ * exact variant; b->r = rho, b->mf.ax = roughness ---- */
+static const float fon1 = 0.5f - 2.0f / (3.0f * ORC_PI);
+static const float fon2 = 2.0f / 3.0f - 28.0f / (15.0f * ORC_PI);
+
+static float eon_E(float mu, float r)
+{
+    float AF = 1.0f / (1.0f + fon1 * r);
+    float BF = r * AF;
+    float Si = sqrtf(1.0f - (mu * mu));
+    float G = Si * (o_acosf(mu) - Si * mu) + (2.0f / 3.0f) * ((Si / mu) * (1.0f - (Si * Si * Si)) - Si);
+    return AF + (BF * orc_inv_pi) * G;
+}
+
+static void eon_ltc(float mu, float r, float* a, float* b, float* c, float* d)
+{
+    *a = 1.0f + r * (0.303392f + (-0.518982f + 0.111709f * mu) * mu + (-0.276266f + 0.335918f * mu) * r);
+    *b = r * (-1.16407f + 1.15859f * mu + (0.150815f - 0.150105f * mu) * r) / (mu * mu * mu - 1.43545f);
+    *c = 1.0f + (0.20013f + (-0.506373f + 0.261777f * mu) * mu) * r;
+    *d = ((0.540852f + (-1.01625f + 0.475392f * mu) * mu) * r) / (-1.0743f + mu * (0.0725628f + mu));
+}
+
+/* orthonormal_basis_ltc: columns X, Y = (-X.y, X.x, 0), Z = (0, 0, 1) */
+static void eon_basis(v3 w, v3* X, v3* Y)
+{
+    float len2 = w.x * w.x + w.y * w.y;
+    *X = len2 > 0.0f ? V3(w.x / sqrtf(len2), w.y / sqrtf(len2), 0.0f / sqrtf(len2)) : V3(1.0f, 0.0f, 0.0f);
+    *Y = V3(-X->y, X->x, 0.0f);
+}
+
+static float eon_cltc_pdf(v3 wo, v3 wl, float r)
+{
+    v3 X, Y;
+    eon_basis(wo, &X, &Y);
+    v3 wi = V3(X.x * wl.x + X.y * wl.y + X.z * wl.z, Y.x * wl.x + Y.y * wl.y + Y.z * wl.z,
+               0.0f * wl.x + 0.0f * wl.y + 1.0f * wl.z);
+    float a, b, c, d;
+    eon_ltc(wo.z, r, &a, &b, &c, &d);
+    float detM = c * (a - b * d);
+    v3 wh = V3(c * (wi.x - b * wi.z), (a - b * d) * wi.y, -c * (d * wi.x - a * wi.z));
+    float len2 = vdot(wh, wh);
+    float vz = 1.0f / sqrtf(d * d + 1.0f);
+    float s = 0.5f * (1.0f + vz);
+    return detM * detM / (len2 * len2) * smax(wh.z, 0.0f) / (ORC_PI * s);
+}
+
+static v3 eon_cltc_sample(v3 wo, float r, float u1, float u2, float* pdf)
+{
+    float a, b, c, d;
+    eon_ltc(wo.z, r, &a, &b, &c, &d);
+    float R = sqrtf(u1);
+    float phi = orc_two_pi * u2;
+    float x = R * o_cosf(phi);
+    float y = R * o_sinf(phi);
+    float vz = 1.0f / sqrtf(d * d + 1.0f);
+    float s = 0.5f * (1.0f + vz);
+    x = -(sqrtf(1.0f - y * y) * (1.0f - s) + x * s);
+    v3 wh = V3(x, y, sqrtf(smax(1.0f - (x * x + y * y), 0.0f)));
+    float pdf_wh = wh.z / (ORC_PI * s);
+    v3 wi = V3(a * wh.x + b * wh.z, c * wh.y, d * wh.x + wh.z);
+    float len = vlen(wi);
+    float detM = c * (a - b * d);
+    *pdf = pdf_wh * len * len * len / detM;
+    v3 X, Y;
+    eon_basis(wo, &X, &Y);
+    return vnormalize(V3(X.x * wi.x + Y.x * wi.y + 0.0f * wi.z, X.y * wi.x + Y.y * wi.y + 0.0f * wi.z,
+                         X.z * wi.x + Y.z * wi.y + 1.0f * wi.z));
+}
+
+static spec eon_f(const bxdf_t* b, v3 wo, v3 wi)
+{
+    if (!same_hemisphere(wo, wi)) return S1(0.0f);
+    float r = b->mf.ax;
+    float mu_i = wi.z, mu_o = wo.z;
+    float s = vdot(wi, wo) - mu_i * mu_o;
+    float sovertF = s > 0.0f ? s / smax(mu_i, mu_o) : s;
+    float AF = 1.0f / (1.0f + fon1 * r);
+    spec f_ss = smulf(smulf(smulf(b->r, orc_inv_pi), AF), 1.0f + r * sovertF);
+    float EFo = eon_E(mu_o, r), EFi = eon_E(mu_i, r);
+    float avgEF = AF * (1.0f + fon2 * r);
+    spec rho_ms = sdiv(smulf(smul(b->r, b->r), avgEF), ssub(S1(1.0f), smulf(b->r, 1.0f - avgEF)));
+    const float eps = 1.0e-7f;
+    spec f_ms = sdivf(smulf(smulf(smulf(rho_ms, orc_inv_pi), smax(eps, 1.0f - EFo)), smax(eps, 1.0f - EFi)),
+                      smax(eps, 1.0f - avgEF));
+    return sadd(f_ss, f_ms);
+}
+
+static float eon_P_u(float mu, float r)
+{
+    return o_powf(r, 0.1f) * (0.162925f + mu * (-0.372058f + (0.538233f - 0.290822f * mu) * mu));
+}
+
+static float eon_pdf(const bxdf_t* b, v3 wo, v3 wi)
+{
+    if (!same_hemisphere(wo, wi)) return 0.0f;
+    float P_u = eon_P_u(wo.z, b->mf.ax);
+    float P_c = 1.0f - P_u;
+    return P_u * orc_inv_two_pi + P_c * eon_cltc_pdf(wo, wi, b->mf.ax);
+}
+
+static int eon_sample(const bxdf_t* b, v3 wo, float u0, v2 u12, bsample* out)
+{
+    float r = b->mf.ax;
+    float P_u = eon_P_u(wo.z, r);
+    float P_c = 1.0f - P_u;
+    v3 wi;
+    float pdf_c;
+    if (u0 <= P_u) {
+        /* SampleUniformHemisphere (sampling.h:50-57) */
+        float z = u12.x;
+        float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+        float phi = orc_two_pi * u12.y;
+        wi = V3(rr * o_cosf(phi), rr * o_sinf(phi), z);
+        pdf_c = eon_cltc_pdf(wo, wi, r);
+    } else {
+        wi = eon_cltc_sample(wo, r, u12.x, u12.y, &pdf_c);
+    }
+    out->pdf = P_u * orc_inv_two_pi + P_c * pdf_c;
+    out->f = eon_f(b, wo, wi);
+    out->wi = wi;
+    out->flags = BX_DIFFUSE | BX_REFLECTION;
+    out->eta = 1.0f;
+    out->stochastic = 0;
     return 1;
 }
 
@@ -2168,8 +2295,10 @@ static int make_bsdf_c(const BbMaterial* m, v3 ns, v3 ts, int front_face, bsdf_t
     out->frame = frame_from_xz(ts, ns);
     switch (m->type) {
     case BB_MAT_DIFFUSE:
-        b->kind = K_LAMBERT;
+        /* EONBxDF(rho, r) when the roughness is > 0, else Lambertian */
+        b->kind = m->u_roughness > 0.0f ? K_EON : K_LAMBERT;
         b->r = S3(m->color[0], m->color[1], m->color[2]);
+        b->mf.ax = b->mf.ay = m->u_roughness;
         return 1;
     case BB_MAT_DIELECTRIC:
         /* DielectricMultiScatteringBxDF when energy_compensation (dielectric_material.cpp:36-42);
@@ -2227,7 +2356,7 @@ static int make_bsdf(const OrcScene* s, const BbMaterial* m0, v2 uv, v3 ns, v3 t
             m.color[1] = c.g;
             m.color[2] = c.b;
         }
-        if (m.tex_u_roughness && m.type != BB_MAT_DIFFUSE) m.u_roughness = texture1(s, m.tex_u_roughness - 1, uv);
+        if (m.tex_u_roughness) m.u_roughness = texture1(s, m.tex_u_roughness - 1, uv);
         if (m.tex_v_roughness && m.type != BB_MAT_DIFFUSE) m.v_roughness = texture1(s, m.tex_v_roughness - 1, uv);
         if (m.tex_metallic && m.type == BB_MAT_METALLIC_ROUGHNESS) m.metallic = texture1(s, m.tex_metallic - 1, uv);
     }
@@ -2256,6 +2385,7 @@ static spec bsdf_f(const bsdf_t* s, v3 wo_w, v3 wi_w)
     if (wo.z == 0.0f) return S1(0.0f);
     switch (s->bx.kind) {
     case K_LAMBERT: return lambert_f(&s->bx, wo, wi);
+    case K_EON: return eon_f(&s->bx, wo, wi);
     case K_DIELECTRIC: return dielectric_f(&s->bx, wo, wi);
     case K_DIELECTRIC_MS: return dielectric_ms_f(&s->bx, wo, wi);
     case K_CONDUCTOR: return conductor_f(&s->bx, wo, wi);
@@ -2272,6 +2402,7 @@ static float bsdf_pdf(const bsdf_t* s, v3 wo_w, v3 wi_w)
     if (wo.z == 0.0f) return 0.0f;
     switch (s->bx.kind) {
     case K_LAMBERT: return lambert_pdf(wo, wi);
+    case K_EON: return eon_pdf(&s->bx, wo, wi);
     case K_DIELECTRIC: return dielectric_pdf(&s->bx, wo, wi);
     case K_DIELECTRIC_MS: return dielectric_ms_pdf(&s->bx, wo, wi);
     case K_CONDUCTOR: return conductor_pdf(&s->bx, wo, wi);
@@ -2288,6 +2419,7 @@ static int bsdf_sample(const bsdf_t* s, v3 wo_w, float u0, v2 u12, bsample* out)
     int ok;
     switch (s->bx.kind) {
     case K_LAMBERT: ok = lambert_sample(&s->bx, wo, u12, out); break;
+    case K_EON: ok = eon_sample(&s->bx, wo, u0, u12, out); break;
     case K_DIELECTRIC: ok = dielectric_sample(&s->bx, wo, u0, u12, 1, out); break;
     case K_DIELECTRIC_MS: ok = dielectric_ms_sample(&s->bx, wo, u0, u12, out); break;
     case K_CONDUCTOR: ok = conductor_sample(&s->bx, wo, u12, out); break;
@@ -2303,12 +2435,7 @@ static int bsdf_sample(const bsdf_t* s, v3 wo_w, float u0, v2 u12, bsample* out)
 static int material_supported(const BbMaterial* m, char* err, int32_t err_size)
 {
     switch (m->type) {
-    case BB_MAT_DIFFUSE:
-        if (m->u_roughness > 0.0f || m->tex_u_roughness) {
-            snprintf(err, err_size, "diffuse roughness > 0 (EON BxDF) is not supported");
-            return 0;
-        }
-        return 1;
+    case BB_MAT_DIFFUSE: return 1;
     case BB_MAT_CONDUCTOR:
     case BB_MAT_CONDUCTOR_R: return 1;
     case BB_MAT_DIELECTRIC: return 1;

@@ -94,9 +94,9 @@ def test_scene_create_reports_missing_device_or_unsupported(bblib):
     n = ctypes.c_int32(-1)
     assert bblib.bbgpu_device_count(ctypes.byref(n)) == 0
     sc, _ = scenes.readme_scene(16, 9)
-    rough_diffuse = scenes.diffuse((0.5, 0.5, 0.5))
-    rough_diffuse.u_roughness = 0.5  # DiffuseMaterial's EON BxDF: outside the path
-    sc.materials[0] = rough_diffuse
+    # an alpha texture on a sphere: outside the path (the reference's
+    # re-intersection can hit the sphere's far side)
+    sc.materials[0].tex_alpha = sc.add_texture(scenes.constant_texture(0.5))
     h = ctypes.c_void_p()
     st = bblib.bbgpu_scene_create(ctypes.byref(sc.desc()), ctypes.byref(h))
     if n.value == 0:

@@ -343,6 +343,42 @@ def test_large_scene_generator_parity():
     assert gi.mean() > 0.01  # the lights reach the camera
 
 
+def test_render_eon_parity():
+    """DiffuseMaterial with roughness > 0 (the EON BxDF) on the icosphere and
+    the ground, beside Lambertian groups: within 1e-3 of the oracle, and
+    bit-equal with the oracle's transcendentals correctly rounded."""
+    from bulbit_amd.gpu import GpuScene
+    from oracle import OracleScene, correctly_rounded_libm
+
+    sc, cam = scenes.icosphere_scene(width=240, height=136, level=5, n_lights=16)
+    mats = []
+    for r, col in ((0.2, (0.8, 0.5, 0.3)), (0.6, (0.3, 0.7, 0.4)), (1.0, (0.9, 0.9, 0.9)), (0.0, (0.5, 0.5, 0.8))):
+        m = scenes.diffuse(col)
+        m.u_roughness = m.v_roughness = r
+        mats.append(m)
+    base = len(sc.materials)
+    sc.materials += mats
+    n_ico = 20 * 4 ** 5
+    sc.prims[:n_ico, 2] = base + (np.arange(n_ico) // 512) % len(mats)
+    sc.prims[n_ico:n_ico + 2, 2] = base + 1  # rough ground
+    sc.add_infinite_light((0.05, 0.05, 0.05))
+    g, o = GpuScene(sc), OracleScene(sc)
+    p = scenes.render_params(8)
+    gs, gc, _ = g.render(cam, p)
+    os_, oc, _ = o.render(cam, p, threads=THREADS)
+    gi, oi = gs / gc[..., None], os_ / oc[..., None]
+    assert np.array_equal(gc, oc)
+    frac = pixel_agreement(gi, oi)
+    log(f"EON 240x136x8: {frac:.6f} within 1e-3, {np.mean(gi == oi):.4f} bit-equal")
+    assert frac >= 0.999
+    try:
+        correctly_rounded_libm(True)
+        cs, cc, _ = o.render(cam, p, threads=THREADS)
+    finally:
+        correctly_rounded_libm(False)
+    assert np.array_equal(cs.view(np.int32), gs.view(np.int32))
+
+
 @pytest.mark.parametrize("which", ["readme", "ico6", "large"])
 def test_render_bit_exact_with_correctly_rounded_libm(readme, ico6, which):
     """The only difference between the GPU path and the oracle is libm: with

@@ -393,6 +393,13 @@ SKIP_NONE = ()
 SKIP_DIFFUSE_TRANSMISSION = (4 | 2,)
 
 
+def rough_diffuse(color, r):
+    """DiffuseMaterial with roughness r > 0: the EON BxDF"""
+    m = scenes.diffuse(color)
+    m.u_roughness = m.v_roughness = r
+    return m
+
+
 @pytest.mark.parametrize("mat,pdf_consistent,inside,skip", [
     (scenes.conductor(scenes.GOLD_ETA, scenes.GOLD_K, 0.3), True, True, SKIP_NONE),
     (scenes.conductor(scenes.GOLD_ETA, scenes.GOLD_K, 0.6), True, True, SKIP_NONE),
@@ -408,8 +415,10 @@ SKIP_DIFFUSE_TRANSMISSION = (4 | 2,)
     (scenes.dielectric(1.5, roughness=0.5, energy_compensation=1), True, False, SKIP_DIFFUSE_TRANSMISSION),
     (scenes.dielectric(1.33, roughness=0.85, color=(0.9, 0.7, 0.5), energy_compensation=1), True, False,
      SKIP_DIFFUSE_TRANSMISSION),
+    (rough_diffuse((0.6, 0.5, 0.4), 0.3), True, False, SKIP_NONE),
+    (rough_diffuse((0.9, 0.9, 0.9), 1.0), True, False, SKIP_NONE),
 ], ids=["gold0.3", "gold0.6", "roughglass", "mr_dielectric", "mr_metal", "lambert", "gold0.5_ms", "roughglass0.5_ms",
-        "water0.85_ms"])
+        "water0.85_ms", "eon0.3", "eon1.0"])
 def test_sampled_pdf_and_value_match_evaluation(orc, mat, pdf_consistent, inside, skip):
     """Sample_f's value and pdf equal f() and PDF() at the sampled direction,
     on both faces and from both sides of the shading frame."""
@@ -438,7 +447,8 @@ def test_sampled_pdf_and_value_match_evaluation(orc, mat, pdf_consistent, inside
     scenes.dielectric(1.5, roughness=0.4),
     scenes.diffuse((0.3, 0.6, 0.9)),
     scenes.conductor(scenes.GOLD_ETA, scenes.GOLD_K, 0.4, energy_compensation=1),
-], ids=["gold", "roughglass", "lambert", "gold_ms"])
+    rough_diffuse((0.3, 0.6, 0.9), 0.7),
+], ids=["gold", "roughglass", "lambert", "gold_ms", "eon"])
 def test_bsdf_reciprocity(orc, mat):
     rng = np.random.default_rng(5)
     for _ in range(200):
@@ -466,6 +476,36 @@ def test_ggx_albedo_near_one_for_perfect_reflector(orc):
             acc.append(f[0] * abs(wi[2]) / pdf if ok else 0.0)
         e = float(np.mean(acc))
         assert lo < e <= 1.0 + 3 * np.std(acc) / np.sqrt(len(acc)), (rough, e)
+
+
+@pytest.mark.parametrize("r", [0.2, 0.6, 1.0])
+def test_eon_white_furnace(orc, r):
+    """EON is energy-preserving: with rho = 1 its multiple-scattering lobe
+    returns exactly the energy the FON lobe loses, so the directional albedo
+    E[f cos / pdf] is 1 for every wo (Portsmouth et al. 2025, eq. for
+    rho_ms); a dropped or mis-weighted term moves it off 1."""
+    m = rough_diffuse((1.0, 1.0, 1.0), r)
+    rng = np.random.default_rng(21)
+    for theta in (0.2, 0.8, 1.3):
+        wo = _dir(theta, 0.4)
+        acc = []
+        for u in rng.random((20000, 3)):
+            ok, f, wi, pdf, _, _ = _sample(orc, m, wo, u[0], u[1:])
+            acc.append(f[0] * abs(wi[2]) / pdf if ok else 0.0)
+        acc = np.array(acc)
+        err = 4 * acc.std() / np.sqrt(len(acc))
+        assert abs(acc.mean() - 1.0) < max(err, 2e-3), (r, theta, acc.mean(), err)
+
+
+def test_eon_tends_to_lambert(orc):
+    """With a vanishing roughness EON is the Lambertian lobe rho / pi."""
+    m = rough_diffuse((0.5, 0.3, 0.8), 1e-6)
+    rng = np.random.default_rng(22)
+    for _ in range(50):
+        wo = _dir(rng.uniform(0.05, 1.4), rng.uniform(0, 2 * np.pi))
+        wi = _dir(rng.uniform(0.05, 1.4), rng.uniform(0, 2 * np.pi))
+        f, _, _ = _eval(orc, m, wo, wi)
+        assert np.allclose(f, np.array([0.5, 0.3, 0.8]) / np.pi, rtol=1e-4)
 
 
 def test_smooth_dielectric_throughput(orc):

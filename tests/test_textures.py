@@ -160,6 +160,7 @@ def textured_scene(width=240, height=136, level=5, alpha=True):
     m = scenes.dielectric(1.5, roughness=0.3, energy_compensation=1); m.tex_u_roughness = m.tex_v_roughness = t_rough; mats.append(m)
     m = scenes.diffuse((0.7, 0.7, 0.7)); m.tex_normal = t_normal; mats.append(m)
     m = scenes.conductor(scenes.GOLD_ETA, scenes.GOLD_K, 0.2, energy_compensation=1); m.tex_normal = t_normal; mats.append(m)
+    m = scenes.diffuse((0.8, 0.7, 0.6)); m.tex_u_roughness = t_check_r; mats.append(m)  # EON where the roughness > 0
     if alpha:
         m = scenes.diffuse((0.6, 0.9, 0.6)); m.tex_alpha = t_alpha; mats.append(m)
         m = scenes.metallic_roughness((0.5, 0.5, 0.9), 1.0, 0.3); m.tex_alpha = t_alpha; m.tex_normal = t_normal; mats.append(m)
