@@ -56,6 +56,8 @@ CPP_WRITE_IMAGES = os.path.join(CPP, "_build", "write_images")
 CPP_TEXTURED = os.path.join(CPP, "_build", "textured_example")
 CPP_LOAD_OBJ = os.path.join(CPP, "_build", "load_obj")
 CPP_CONVERT_IMAGE = os.path.join(CPP, "_build", "convert_image")
+CPP_DESCRIBE_SCENE = os.path.join(CPP, "_build", "describe_scene")
+CPP_RENDER_SCENE = os.path.join(CPP, "_build", "render_scene")
 # host C++ API (bulbit/bulbit.h) over the C ABI: plain g++, same FP rules
 CXX_FLAGS = ["-std=c++17", "-O2", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-pthread"]
 
@@ -85,7 +87,7 @@ def build_cpp(verbose: bool = False) -> str:
     inc = os.path.join(CPP, "include")
     tmp = CPP_LIB + ".tmp"
     _run([cxx, *CXX_FLAGS, "-fPIC", "-shared", "-I", inc, os.path.join(CPP, "bulbit.cpp"),
-          os.path.join(CPP, "image_io.cpp"), os.path.join(CPP, "model_loader.cpp"), "-o", tmp,
+          os.path.join(CPP, "image_io.cpp"), os.path.join(CPP, "model_loader.cpp"), os.path.join(CPP, "scene_loader.cpp"), "-o", tmp,
           "-L", HERE, "-lbbgpu", "-Wl,-rpath,$ORIGIN"], "g++ building libbulbit.so", verbose)
     os.replace(tmp, CPP_LIB)
     os.makedirs(os.path.dirname(CPP_EXAMPLE), exist_ok=True)
@@ -99,6 +101,9 @@ def build_cpp(verbose: bool = False) -> str:
           "-L", HERE, "-lbulbit", "-lbbgpu", "-Wl,-rpath,$ORIGIN/../.."], "g++ building load_obj", verbose)
     _run([cxx, *CXX_FLAGS, "-I", inc, os.path.join(CPP, "examples", "convert_image.cpp"), "-o", CPP_CONVERT_IMAGE,
           "-L", HERE, "-lbulbit", "-lbbgpu", "-Wl,-rpath,$ORIGIN/../.."], "g++ building convert_image", verbose)
+    for src, exe in (("describe_scene.cpp", CPP_DESCRIBE_SCENE), ("render_scene.cpp", CPP_RENDER_SCENE)):
+        _run([cxx, *CXX_FLAGS, "-I", inc, os.path.join(CPP, "examples", src), "-o", exe,
+              "-L", HERE, "-lbulbit", "-lbbgpu", "-Wl,-rpath,$ORIGIN/../.."], "g++ building " + src, verbose)
     return CPP_LIB
 
 

@@ -91,6 +91,7 @@ constexpr Vec3 operator-(const Vec3& a) { return Vec3(-a.x, -a.y, -a.z); }
 constexpr Vec3 operator*(const Vec3& a, Float s) { return Vec3(a.x * s, a.y * s, a.z * s); }
 constexpr Vec3 operator*(Float s, const Vec3& a) { return Vec3(a.x * s, a.y * s, a.z * s); }
 constexpr Vec3 operator/(const Vec3& a, Float s) { return Vec3(a.x / s, a.y / s, a.z / s); }
+constexpr Vec3 operator*(const Vec3& a, const Vec3& b) { return Vec3(a.x * b.x, a.y * b.y, a.z * b.z); }
 constexpr bool operator==(const Vec3& a, const Vec3& b) { return a.x == b.x && a.y == b.y && a.z == b.z; }
 
 inline Float Dot(const Vec3& a, const Vec3& b)
@@ -171,6 +172,14 @@ inline Vec4 Mul(const Mat4& m, const Vec4& v)
                 m.ex.w * v.x + m.ey.w * v.y + m.ez.w * v.z + m.ew.w * v.w);
 }
 
+// 3x3 matrix; ex, ey, ez are the columns
+struct Mat3
+{
+    Vec3 ex, ey, ez;
+    constexpr Mat3() = default;
+    constexpr Mat3(const Vec3& x, const Vec3& y, const Vec3& z) : ex{ x }, ey{ y }, ez{ z } {}
+};
+
 struct Quat
 {
     Float x = 0, y = 0, z = 0, w = 1;
@@ -178,7 +187,86 @@ struct Quat
     constexpr Quat(Identity) {}
     constexpr Quat(Float x, Float y, Float z, Float w) : x{ x }, y{ y }, z{ z }, w{ w } {}
     explicit constexpr Quat(Float w) : x{ 0 }, y{ 0 }, z{ 0 }, w{ w } {}
+    // the rotation of an orthonormal basis (include/bulbit/quaternion.h:34-62)
+    Quat(const Mat3& m);
+    // the basis whose z is `front`, x = normalized up x front (quaternion.h:64-72)
+    Quat(const Vec3& front, const Vec3& up);
+    // `angle` radians about `unit_axis` (quaternion.h:75-83)
+    Quat(Float angle, const Vec3& unit_axis);
+
+    Vec3 Rotate(const Vec3& v) const; // quaternion.h:142-155
+    Vec3 GetBasisZ() const;           // quaternion.h:199-205
 };
+
+constexpr Quat operator*(const Quat& q, Float s) { return Quat(q.x * s, q.y * s, q.z * s, q.w * s); }
+
+// Hamilton product, terms in the order of include/bulbit/quaternion.h:273-281
+constexpr Quat operator*(const Quat& a, const Quat& b)
+{
+    return Quat(a.w * b.x + b.w * a.x + a.y * b.z - b.y * a.z, a.w * b.y + b.w * a.y + a.z * b.x - b.z * a.x,
+                a.w * b.z + b.w * a.z + a.x * b.y - b.x * a.y, a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z);
+}
+
+inline Quat::Quat(const Mat3& m)
+{
+    // the largest of 4w^2, 4x^2, 4y^2, 4z^2 picked by the signs of the
+    // diagonal, then the other three components from the off-diagonal sums
+    Float t;
+    if (m.ez.z < 0) {
+        if (m.ex.x > m.ey.y) {
+            t = 1 + m.ex.x - m.ey.y - m.ez.z;
+            *this = Quat(t, m.ex.y + m.ey.x, m.ez.x + m.ex.z, m.ey.z - m.ez.y);
+        } else {
+            t = 1 - m.ex.x + m.ey.y - m.ez.z;
+            *this = Quat(m.ex.y + m.ey.x, t, m.ey.z + m.ez.y, m.ez.x - m.ex.z);
+        }
+    } else {
+        if (m.ex.x < -m.ey.y) {
+            t = 1 - m.ex.x - m.ey.y + m.ez.z;
+            *this = Quat(m.ez.x + m.ex.z, m.ey.z + m.ez.y, t, m.ex.y - m.ey.x);
+        } else {
+            t = 1 + m.ex.x + m.ey.y + m.ez.z;
+            *this = Quat(m.ey.z - m.ez.y, m.ez.x - m.ex.z, m.ex.y - m.ey.x, t);
+        }
+    }
+    *this = *this * (0.5f / std::sqrt(t));
+}
+
+inline Quat::Quat(const Vec3& front, const Vec3& up)
+{
+    Mat3 basis;
+    basis.ez = front;
+    basis.ex = Cross(up, basis.ez);
+    basis.ex.Normalize();
+    basis.ey = Cross(basis.ez, basis.ex);
+    *this = Quat(basis);
+}
+
+inline Quat::Quat(Float angle, const Vec3& unit_axis)
+{
+    const Float half = angle * 0.5f;
+    const Float s = std::sin(half);
+    x = unit_axis.x * s;
+    y = unit_axis.y * s;
+    z = unit_axis.z * s;
+    w = std::cos(half);
+}
+
+// q v q*, in the arrangement of include/bulbit/quaternion.h:142-155
+inline Vec3 Quat::Rotate(const Vec3& v) const
+{
+    const Float vx = 2 * v.x, vy = 2 * v.y, vz = 2 * v.z;
+    const Float w2 = w * w - 0.5f;
+    const Float dot2 = x * vx + y * vy + z * vz;
+    return Vec3(vx * w2 + (y * vz - z * vy) * w + x * dot2, vy * w2 + (z * vx - x * vz) * w + y * dot2,
+                vz * w2 + (x * vy - y * vx) * w + z * dot2);
+}
+
+inline Vec3 Quat::GetBasisZ() const
+{
+    const Float z2 = z * 2, w2 = w * 2;
+    return Vec3((y * w2) + x * z2, (-x * w2) + y * z2, (w * w2) - 1 + z * z2);
+}
 
 // rigid transform with scale: position, orientation, scale (include/bulbit/transform.h)
 struct Transform
@@ -197,10 +285,26 @@ struct Transform
         : p{ position }, q{ orientation }, s{ scale }
     {
     }
+    constexpr Transform(Float x, Float y, Float z, const Quat& orientation = Quat(1), const Vec3& scale = Vec3(1))
+        : p{ x, y, z }, q{ orientation }, s{ scale }
+    {
+    }
+    // scale = column lengths, rotation from the normalized columns,
+    // position = last column (include/bulbit/transform.h:59-69)
+    Transform(const Mat4& m);
 
     // camera transform at `position` looking at `target` (transform.h:117-122)
     static Transform LookAt(const Vec3& position, const Vec3& target, const Vec3& up);
 };
+
+// A * B: B's position moved by A, rotations and scales multiplied
+// (include/bulbit/transform.h:147-157)
+inline Transform Mul(const Transform& a, const Transform& b)
+{
+    return Transform(a.q.Rotate(a.s * b.p) + a.p, a.q * b.q, a.s * b.s);
+}
+inline Transform operator*(const Transform& a, const Transform& b) { return Mul(a, b); }
+inline Vec3 Mul(const Transform& t, const Vec3& v) { return t.q.Rotate(t.s * v) + t.p; }
 
 // Mat4(Mat3(q), p) with the columns scaled by s (src/util/math.cpp:7-38)
 inline Mat4::Mat4(const Transform& t)
@@ -216,6 +320,16 @@ inline Mat4::Mat4(const Transform& t)
     ex = Vec4(ex.x * t.s.x, ex.y * t.s.x, ex.z * t.s.x, ex.w * t.s.x);
     ey = Vec4(ey.x * t.s.y, ey.y * t.s.y, ey.z * t.s.y, ey.w * t.s.y);
     ez = Vec4(ez.x * t.s.z, ez.y * t.s.z, ez.z * t.s.z, ez.w * t.s.z);
+}
+
+inline Transform::Transform(const Mat4& m)
+{
+    s.x = std::sqrt(m.ex.x * m.ex.x + m.ex.y * m.ex.y + m.ex.z * m.ex.z);
+    s.y = std::sqrt(m.ey.x * m.ey.x + m.ey.y * m.ey.y + m.ey.z * m.ey.z);
+    s.z = std::sqrt(m.ez.x * m.ez.x + m.ez.y * m.ez.y + m.ez.z * m.ez.z);
+    q = Quat(Mat3(Vec3(m.ex.x, m.ex.y, m.ex.z) / s.x, Vec3(m.ey.x, m.ey.y, m.ey.z) / s.y,
+                  Vec3(m.ez.x, m.ez.y, m.ez.z) / s.z));
+    p = Vec3(m.ew.x, m.ew.y, m.ew.z);
 }
 
 // ----------------------------------------------------------------- spectrum
@@ -786,6 +900,95 @@ private:
     AABB bounds;
 };
 
+// ------------------------------------------------------------ renderer info
+
+// Render settings read from a scene file (include/bulbit/renderer_info.h:8-120)
+enum class FilterType
+{
+    box,
+    tent,
+    gaussian,
+};
+
+struct FilterInfo
+{
+    FilterType type = FilterType::gaussian;
+    Float extent = 3;
+    Float gaussian_stddev = 0.5f;
+};
+
+struct FilmInfo
+{
+    std::string filename = "";
+    Point2i resolution = { 1280, 720 };
+    FilterInfo filter_info;
+};
+
+enum class SamplerType
+{
+    independent,
+    stratified,
+};
+
+struct SamplerInfo
+{
+    SamplerType type = SamplerType::independent;
+    int32 spp = 64;
+};
+
+enum class CameraType
+{
+    perspective,
+    orthographic,
+    spherical,
+};
+
+struct CameraInfo
+{
+    CameraType type = CameraType::perspective;
+    Transform transform = identity;
+    Float fov = 35.0f;
+    Float aperture_radius = 0.0f;
+    Float focus_distance = 1.0f;
+    Point2 viewport_size = { 1, 1 };
+    FilmInfo film_info;
+    SamplerInfo sampler_info;
+    const Medium* medium = nullptr;
+};
+
+// the reference's integrators; only `path` runs here
+enum class IntegratorType
+{
+    path,
+    vol_path,
+    light_path,
+    light_vol_path,
+    bdpt,
+    vol_bdpt,
+    pm,
+    vol_pm,
+    sppm,
+    vol_sppm,
+    vcm,
+    restir_di,
+    restir_pt,
+    naive_path,
+    naive_vol_path,
+    random_walk,
+    ao,
+    albedo,
+    debug,
+    count,
+};
+
+struct IntegratorInfo
+{
+    IntegratorType type = IntegratorType::path;
+    int32 max_bounces = 16;
+    int32 rr_min_bounces = 1;
+    bool regularize_bsdf = false;
+};
+
 // ----------------------------------------------------------------- sampling
 
 class Sampler
@@ -793,6 +996,9 @@ class Sampler
 public:
     explicit Sampler(int32 samples_per_pixel) : samples_per_pixel{ samples_per_pixel } {}
     virtual ~Sampler() = default;
+    // IndependentSampler for SamplerType::independent, else null (the
+    // stratified sampler is not on the GPU path) (src/sampler/sampler.cpp:7-23)
+    static Sampler* Create(Allocator& alloc, const SamplerInfo& sampler_info);
     int32 samples_per_pixel;
 };
 
@@ -811,6 +1017,8 @@ class Filter
 public:
     explicit Filter(Float extent) : extent{ extent } {}
     virtual ~Filter() = default;
+    // src/reconstruction/filter.cpp:7-21
+    static Filter* Create(Allocator& alloc, const FilterInfo& filter_info);
     Float extent;
 };
 
@@ -845,6 +1053,8 @@ public:
     {
     }
     virtual ~Camera() = default;
+    // a PerspectiveCamera for CameraType::perspective, else null (src/camera/camera.cpp:7-23)
+    static Camera* Create(Allocator& alloc, const CameraInfo& camera_info, const Filter* filter);
 
     // the reference's default camera filter (include/bulbit/camera.h:48)
     static inline std::unique_ptr<Filter> default_filter = std::make_unique<GaussianFilter>();
@@ -950,6 +1160,9 @@ class Integrator
 public:
     virtual ~Integrator() = default;
     virtual Rendering* Render(Allocator& alloc, const Camera* camera) = 0;
+    // a PathIntegrator for IntegratorType::path, else null (src/integrator/integrator.cpp:15-30)
+    static Integrator* Create(Allocator& alloc, const IntegratorInfo& integrator_info, const Intersectable* accel,
+                              const std::vector<Light*>& lights, const Sampler* sampler);
 };
 
 // PathIntegrator (include/bulbit/integrators.h:92-114): unidirectional path
@@ -975,5 +1188,25 @@ private:
     bool regularize_bsdf;
     BbScene* device = nullptr;
 };
+
+// ------------------------------------------------------------ scene files
+
+// RendererInfo (include/bulbit/renderer_info.h:114-120)
+struct RendererInfo
+{
+    Scene scene;
+    CameraInfo camera_info;
+    IntegratorInfo integrator_info;
+};
+
+// LoadScene (cli/scene_loader.cpp:2147-2164): a Mitsuba 3 XML scene into
+// `render_info`, with the reference loader's element rules.  Relative file
+// names resolve against the scene file's folder.  Elements the reference
+// reads but the GPU path does not render (media, point, directional and
+// envmap emitters, mirror, principled, plastic, layered, blend, thin
+// dielectric and subsurface BSDFs, other integrators) make it print
+// `bulbit_amd: LoadScene: ...` and return false; elements the reference
+// itself skips with a message are skipped with a message.
+bool LoadScene(RendererInfo* render_info, std::filesystem::path filename);
 
 } // namespace bulbit
