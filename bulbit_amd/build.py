@@ -88,7 +88,7 @@ def build_cpp(verbose: bool = False) -> str:
     tmp = CPP_LIB + ".tmp"
     _run([cxx, *CXX_FLAGS, "-fPIC", "-shared", "-I", inc, os.path.join(CPP, "bulbit.cpp"),
           os.path.join(CPP, "image_io.cpp"), os.path.join(CPP, "model_loader.cpp"), os.path.join(CPP, "scene_loader.cpp"), "-o", tmp,
-          "-L", HERE, "-lbbgpu", "-Wl,-rpath,$ORIGIN"], "g++ building libbulbit.so", verbose)
+          "-L", HERE, "-lbbgpu", "-lz", "-Wl,-rpath,$ORIGIN"], "g++ building libbulbit.so", verbose)
     os.replace(tmp, CPP_LIB)
     os.makedirs(os.path.dirname(CPP_EXAMPLE), exist_ok=True)
     _run([cxx, *CXX_FLAGS, "-I", inc, os.path.join(CPP, "examples", "readme.cpp"), "-o", CPP_EXAMPLE,

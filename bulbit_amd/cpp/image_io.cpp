@@ -21,6 +21,8 @@
 #include <string>
 #include <vector>
 
+#include <zlib.h>
+
 namespace bulbit
 {
 
@@ -411,7 +413,7 @@ bool WriteImage(const Image3& image, const std::string& filename, ToneMappingCal
 // data (stbi__ldr_to_hdr), an alpha channel becomes c / 255, Radiance .hdr
 // RGBE becomes r * 2^(e - 136) (stbi__hdr_convert), and every value is
 // clamped at 0.  Decoded here: binary PNM (P5 grey, P6 RGB, 8 or 16 bits,
-// 16-bit samples reduced to their high byte as stb does), Radiance .hdr
+// 16-bit samples reduced to their high byte as stb does), PNG (below), Radiance .hdr
 // (flat and RLE scanlines), and PFM (beyond stb: linear floats, already
 // bottom row first).
 
@@ -421,6 +423,7 @@ namespace
 struct Decoded
 {
     int32 width = 0, height = 0, comps = 0;
+    int32 file_comps = 0;    // the file's own channel count (stb's *comp); 0: comps
     std::vector<float> data; // comps per pixel, row 0 at the bottom
 };
 
@@ -565,6 +568,190 @@ bool read_pfm(std::FILE* f, Decoded& out)
     return true;
 }
 
+// PNG (stb_image's stbi__parse_png_file / stbi__create_png_image /
+// stbi__convert_format): every image is expanded to RGBA as stbi_loadf's
+// STBI_rgb_alpha request does (grey replicated, alpha 255 where the file has
+// none, palette entries looked up, tRNS colour keys turned into alpha 0),
+// sub-byte greys scaled to 8 bits (x 0xff, 0x55, 0x11), 16-bit samples
+// reduced to their high byte, Adam7 interlacing undone.  The reported
+// channel count is the file's: 1 grey, 2 grey + alpha, 3 RGB or palette,
+// 4 RGBA or palette with tRNS.  The zlib stream is inflated by zlib.
+bool read_png(std::FILE* f, bool non_color, Decoded& out)
+{
+    std::vector<unsigned char> file;
+    unsigned char buf[65536];
+    for (size_t n; (n = std::fread(buf, 1, sizeof buf, f)) > 0;) file.insert(file.end(), buf, buf + n);
+    static const unsigned char sig[8] = { 137, 80, 78, 71, 13, 10, 26, 10 };
+    if (file.size() < 8 || std::memcmp(file.data(), sig, 8) != 0) return false;
+    auto be32 = [](const unsigned char* p) {
+        return (uint32_t(p[0]) << 24) | (uint32_t(p[1]) << 16) | (uint32_t(p[2]) << 8) | uint32_t(p[3]);
+    };
+    uint32_t w = 0, h = 0;
+    int depth = 0, color = -1, interlace = 0;
+    std::vector<unsigned char> idat, palette; // palette: RGBA entries
+    unsigned char trns_key[6] = { 0, 0, 0, 0, 0, 0 };
+    bool has_key = false, pal_alpha = false;
+    size_t pos = 8;
+    for (;;) {
+        if (pos + 8 > file.size()) return false;
+        const uint32_t len = be32(&file[pos]);
+        const std::string type(reinterpret_cast<const char*>(&file[pos + 4]), 4);
+        if (pos + 12 + size_t(len) > file.size()) return false;
+        const unsigned char* d = &file[pos + 8];
+        if (type == "IHDR") {
+            if (len != 13) return false;
+            w = be32(d);
+            h = be32(d + 4);
+            depth = d[8];
+            color = d[9];
+            interlace = d[12];
+            if (w == 0 || h == 0 || w > (1u << 24) || h > (1u << 24) || d[10] != 0 || d[11] != 0 || interlace > 1)
+                return false;
+        } else if (type == "PLTE") {
+            if (len % 3 || len > 768) return false;
+            palette.assign(size_t(len / 3) * 4, 255);
+            for (uint32_t i = 0; i < len / 3; ++i)
+                for (int k = 0; k < 3; ++k) palette[i * 4 + k] = d[i * 3 + k];
+        } else if (type == "tRNS") {
+            if (color == 3) {
+                if (palette.empty() || len > palette.size() / 4) return false;
+                for (uint32_t i = 0; i < len; ++i) palette[i * 4 + 3] = d[i];
+                pal_alpha = true;
+            } else if (color == 0 || color == 2) {
+                if (len != (color == 0 ? 2u : 6u)) return false;
+                std::memcpy(trns_key, d, len);
+                has_key = true;
+            } else {
+                return false;
+            }
+        } else if (type == "IDAT") {
+            idat.insert(idat.end(), d, d + len);
+        } else if (type == "IEND") {
+            break;
+        } else if (!(type[0] & 0x20)) {
+            return false; // an unknown critical chunk
+        }
+        pos += 12 + size_t(len);
+    }
+    const int samples = color == 0 ? 1 : color == 2 ? 3 : color == 3 ? 1 : color == 4 ? 2 : color == 6 ? 4 : 0;
+    if (!samples || idat.empty()) return false;
+    const bool depth_ok = color == 0 ? (depth == 1 || depth == 2 || depth == 4 || depth == 8 || depth == 16)
+                          : color == 3 ? (depth == 1 || depth == 2 || depth == 4 || depth == 8)
+                                       : (depth == 8 || depth == 16);
+    if (!depth_ok || (color == 3 && palette.empty())) return false;
+
+    // the 7 Adam7 passes (or the whole image): origin and step in x and y
+    static const int ax0[7] = { 0, 4, 0, 2, 0, 1, 0 }, ay0[7] = { 0, 0, 4, 0, 2, 0, 1 };
+    static const int adx[7] = { 8, 8, 4, 4, 2, 2, 1 }, ady[7] = { 8, 8, 8, 4, 4, 2, 2 };
+    const int passes = interlace ? 7 : 1;
+    const size_t bits_pp = size_t(samples) * size_t(depth);
+    size_t raw_size = 0;
+    uint32_t pw[7], ph[7];
+    for (int p = 0; p < passes; ++p) {
+        pw[p] = interlace ? (w > uint32_t(ax0[p]) ? (w - ax0[p] + adx[p] - 1) / adx[p] : 0) : w;
+        ph[p] = interlace ? (h > uint32_t(ay0[p]) ? (h - ay0[p] + ady[p] - 1) / ady[p] : 0) : h;
+        if (pw[p] && ph[p]) raw_size += size_t(ph[p]) * (1 + (size_t(pw[p]) * bits_pp + 7) / 8);
+    }
+    std::vector<unsigned char> raw(raw_size);
+    z_stream zs;
+    std::memset(&zs, 0, sizeof zs);
+    if (inflateInit(&zs) != Z_OK) return false;
+    zs.next_in = idat.data();
+    zs.avail_in = uInt(idat.size());
+    zs.next_out = raw.data();
+    zs.avail_out = uInt(raw.size());
+    const int zr = inflate(&zs, Z_FINISH);
+    const size_t produced = raw.size() - zs.avail_out;
+    inflateEnd(&zs);
+    if ((zr != Z_STREAM_END && zr != Z_BUF_ERROR) || produced != raw.size()) return false;
+
+    // RGBA 8-bit, top row first
+    std::vector<unsigned char> rgba(size_t(w) * h * 4);
+    const size_t bpp = std::max<size_t>(1, bits_pp / 8); // filter byte distance
+    static const unsigned char scale[5] = { 0, 0xff, 0x55, 0, 0x11 };
+    size_t off = 0;
+    for (int p = 0; p < passes; ++p) {
+        if (!pw[p] || !ph[p]) continue;
+        const size_t stride = (size_t(pw[p]) * bits_pp + 7) / 8;
+        std::vector<unsigned char> prev(stride, 0), cur(stride);
+        for (uint32_t y = 0; y < ph[p]; ++y) {
+            const int filter = raw[off];
+            const unsigned char* src = &raw[off + 1];
+            off += 1 + stride;
+            for (size_t i = 0; i < stride; ++i) { // undo the scanline filter (PNG spec section 9)
+                const int a = i >= bpp ? cur[i - bpp] : 0, b = prev[i], c = i >= bpp ? prev[i - bpp] : 0;
+                int v = src[i];
+                switch (filter) {
+                case 0: break;
+                case 1: v += a; break;
+                case 2: v += b; break;
+                case 3: v += (a + b) >> 1; break;
+                case 4: {
+                    const int pa = std::abs(b - c), pb = std::abs(a - c), pc = std::abs(a + b - 2 * c);
+                    v += (pa <= pb && pa <= pc) ? a : (pb <= pc ? b : c);
+                    break;
+                }
+                default: return false;
+                }
+                cur[i] = (unsigned char)v;
+            }
+            for (uint32_t x = 0; x < pw[p]; ++x) {
+                unsigned char px[4] = { 0, 0, 0, 255 };
+                uint16_t s16[4] = { 0, 0, 0, 0 };
+                for (int k = 0; k < samples; ++k) {
+                    const size_t bit = (size_t(x) * samples + k) * size_t(depth);
+                    if (depth == 16) {
+                        s16[k] = uint16_t((cur[bit / 8] << 8) | cur[bit / 8 + 1]);
+                        px[k] = (unsigned char)(s16[k] >> 8);
+                    } else if (depth == 8) {
+                        px[k] = s16[k] = cur[bit / 8];
+                    } else {
+                        const int v = (cur[bit / 8] >> (8 - depth - int(bit % 8))) & ((1 << depth) - 1);
+                        s16[k] = uint16_t(v);
+                        px[k] = (unsigned char)(color == 3 ? v : v * scale[depth]);
+                    }
+                }
+                unsigned char o[4];
+                if (color == 3) {
+                    if (size_t(px[0]) * 4 >= palette.size()) return false;
+                    std::memcpy(o, &palette[size_t(px[0]) * 4], 4);
+                } else if (color == 0 || color == 4) {
+                    o[0] = o[1] = o[2] = px[0];
+                    o[3] = color == 4 ? px[1] : 255;
+                } else {
+                    o[0] = px[0];
+                    o[1] = px[1];
+                    o[2] = px[2];
+                    o[3] = color == 6 ? px[3] : 255;
+                }
+                if (has_key) { // a colour key compares the file's samples at their own depth
+                    bool match = true;
+                    for (int k = 0; k < samples; ++k)
+                        match &= s16[k] == (depth == 16 ? uint16_t((trns_key[2 * k] << 8) | trns_key[2 * k + 1])
+                                                        : uint16_t(trns_key[2 * k + 1]));
+                    if (match) o[3] = 0;
+                }
+                const size_t X = interlace ? size_t(ax0[p]) + size_t(x) * adx[p] : x;
+                const size_t Y = interlace ? size_t(ay0[p]) + size_t(y) * ady[p] : y;
+                std::memcpy(&rgba[(Y * w + X) * 4], o, 4);
+            }
+            std::swap(prev, cur);
+        }
+    }
+    out.width = int32(w);
+    out.height = int32(h);
+    out.comps = 4;
+    out.file_comps = color == 3 ? (pal_alpha ? 4 : 3) : samples;
+    out.data.resize(size_t(w) * h * 4);
+    for (uint32_t y = 0; y < h; ++y)
+        for (uint32_t x = 0; x < w; ++x)
+            for (int k = 0; k < 4; ++k) {
+                const unsigned char c = rgba[(size_t(y) * w + x) * 4 + k];
+                out.data[(size_t(h - 1 - y) * w + x) * 4 + k] = k == 3 ? c / 255.0f : ldr_to_float(c, non_color);
+            }
+    return true;
+}
+
 bool decode(const std::string& filename, bool non_color, Decoded& out)
 {
     std::FILE* f = std::fopen(filename.c_str(), "rb");
@@ -577,6 +764,8 @@ bool decode(const std::string& filename, bool non_color, Decoded& out)
         ok = read_pfm(f, out);
     else if (ext == ".ppm" || ext == ".pgm" || ext == ".pnm")
         ok = read_pnm(f, non_color, out);
+    else if (ext == ".png" || ext == ".PNG")
+        ok = read_png(f, non_color, out);
     std::fclose(f);
     return ok;
 }
@@ -592,7 +781,7 @@ Image1 ReadImage1(const std::string& filename, int32 channel, bool non_color, st
     }
     // a channel beyond the file's components (an alpha channel the file
     // does not have, for one) gives an empty image, as in the reference
-    if (channel < 0 || channel >= d.comps) return {};
+    if (channel < 0 || channel >= (d.file_comps ? d.file_comps : d.comps)) return {};
     Image1 image(d.width, d.height);
     for (size_t i = 0; i < image.data.size(); ++i) {
         const float v = d.comps == 1 ? d.data[i] : d.data[i * size_t(d.comps) + size_t(channel)];
