@@ -264,6 +264,11 @@ struct SceneExport
             put3(b.spectrum, s->intensity);
             b.angle_max = s->angle_max;
             b.angle_falloff_start = s->angle_falloff_start;
+        } else if (auto* p = dynamic_cast<const PointLight*>(l)) {
+            if (p->medium) fatal("PointLight", "participating media are not supported on the GPU path");
+            b.type = BB_LIGHT_POINT;
+            put3(b.position, p->position);
+            put3(b.spectrum, p->intensity);
         } else if (auto* u = dynamic_cast<const UniformInfiniteLight*>(l)) {
             b.type = BB_LIGHT_UNIFORM_INFINITE;
             put3(b.spectrum, u->l);

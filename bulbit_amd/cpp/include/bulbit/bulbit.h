@@ -653,6 +653,20 @@ public:
     const Medium* medium;
 };
 
+// PointLight (include/bulbit/lights.h:15-40)
+class PointLight : public Light
+{
+public:
+    PointLight(const Point3& position, const Spectrum& intensity, const Medium* medium)
+        : position{ position }, intensity{ intensity }, medium{ medium }
+    {
+    }
+
+    Point3 position;
+    Spectrum intensity;
+    const Medium* medium;
+};
+
 // UniformInfiniteLight (include/bulbit/lights.h:236-260)
 class UniformInfiniteLight : public Light
 {
@@ -1202,7 +1216,7 @@ struct RendererInfo
 // LoadScene (cli/scene_loader.cpp:2147-2164): a Mitsuba 3 XML scene into
 // `render_info`, with the reference loader's element rules.  Relative file
 // names resolve against the scene file's folder.  Elements the reference
-// reads but the GPU path does not render (media, point, directional and
+// reads but the GPU path does not render (media, directional and
 // envmap emitters, mirror, principled, plastic, layered, blend, thin
 // dielectric and subsurface BSDFs, other integrators) make it print
 // `bulbit_amd: LoadScene: ...` and return false; elements the reference

@@ -1016,7 +1016,21 @@ private:
             for (const XmlNode& c : node.children)
                 if (c.value("name") == "radiance") radiance = Intensity(c);
             scene->CreateLight<UniformInfiniteLight>(radiance);
-        } else if (type == "point" || type == "directional" || type == "envmap") {
+        } else if (type == "point") {
+            Vec3 position(0);
+            Spectrum intensity(1);
+            Transform to_world = identity;
+            for (const XmlNode& c : node.children) {
+                const std::string name = c.value("name");
+                if (name == "position")
+                    position = ToLowercase(c.name) == "point" ? ParsePoint(c) : ParseVec3(c.attribute("value"));
+                else if (name == "intensity")
+                    intensity = Intensity(c);
+                else if (name == "to_world")
+                    to_world = ParseTransform(c);
+            }
+            scene->CreateLight<PointLight>(Mul(to_world, position), intensity, nullptr);
+        } else if (type == "directional" || type == "envmap") {
             throw Unsupported("the " + type + " emitter");
         } else {
             warn("Light not supported: " + type);

@@ -1168,9 +1168,19 @@ struct light_sample
     float pdf;
 };
 
-// Light::Sample_Li for spot, uniform infinite and diffuse area (triangle) lights
+// Light::Sample_Li for spot, point, uniform infinite and diffuse area (triangle) lights
 BB_D bool light_sample_li(const GpuLight& L, f3 ref, f2 u, light_sample& ls)
 {
+    if (L.type == BB_LIGHT_POINT) {
+        // point_light.cpp:30-42
+        f3 wi = L.p - ref;
+        const float distance = normalize_m(wi);
+        ls.wi = wi;
+        ls.visibility = distance;
+        ls.pdf = 1.0f;
+        ls.Li = L.L / (distance * distance);
+        return true;
+    }
     if (L.type == BB_LIGHT_SPOT) {
         // spot_light.cpp:46-68
         f3 wi = L.p - ref;

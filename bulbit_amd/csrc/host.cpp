@@ -605,6 +605,11 @@ bool create_scene(const BbSceneDesc& d, BbScene* s, std::string& err)
             // DiffuseAreaLight::Phi (diffuse_area_light.cpp:144-148), Triangle::Area (triangle.cpp:255-264)
             float area = 0.5f * length(cross(g.e1, g.e2));
             phi = ((c * area) * kPi) * float(L.two_sided ? 2 : 1);
+        } else if (L.type == BB_LIGHT_POINT) {
+            // PointLight ctor (point_light.cpp:7-13), Phi (:93-96)
+            g.p = mk3(L.position[0], L.position[1], L.position[2]);
+            g.L = c;
+            phi = c * kFourPi;
         } else {
             err = "unknown light type";
             return false;

@@ -253,6 +253,13 @@ class SceneData:
         L.angle_falloff_start = angle_falloff_start
         self.lights.append(L)
 
+    def add_point_light(self, position, intensity):
+        L = capi.BbLight()
+        L.type = capi.BB_LIGHT_POINT
+        L.position[:] = list(position)
+        L.spectrum[:] = list(intensity)
+        self.lights.append(L)
+
     def add_infinite_light(self, l, scale=1.0):
         L = capi.BbLight()
         L.type = capi.BB_LIGHT_UNIFORM_INFINITE

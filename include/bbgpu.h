@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define BBGPU_ABI_VERSION 11
+#define BBGPU_ABI_VERSION 12
 
 /* status codes */
 enum {
@@ -128,18 +128,19 @@ enum {
     BB_LIGHT_SPOT = 0,              /* SpotLight            src/light/spot_light.cpp:7-24 */
     BB_LIGHT_UNIFORM_INFINITE = 1,  /* UniformInfiniteLight src/light/uniform_infinite_light.cpp:7-12 */
     BB_LIGHT_DIFFUSE_AREA = 2,      /* DiffuseAreaLight     src/light/diffuse_area_light.cpp:9-15 */
+    BB_LIGHT_POINT = 3,             /* PointLight           src/light/point_light.cpp:7-13 */
 };
 
-/* Constructor arguments of the three light types, in Scene::GetLights() order
+/* Constructor arguments of the light types, in Scene::GetLights() order
  * (the order fixes the power-sampler CDF, power_light_sampler.cpp:8-23). */
 typedef struct BbLight {
     int32_t type;
     int32_t primitive;          /* DIFFUSE_AREA: index into primitives (a triangle) */
     int32_t two_sided;          /* DIFFUSE_AREA */
     int32_t reserved;
-    float position[3];          /* SPOT */
+    float position[3];          /* SPOT, POINT */
     float direction[3];         /* SPOT: the constructor's direction argument */
-    float spectrum[3];          /* SPOT intensity / INFINITE l / AREA emission */
+    float spectrum[3];          /* SPOT, POINT intensity / INFINITE l / AREA emission */
     float scale;                /* INFINITE scale */
     float angle_max;            /* SPOT, degrees */
     float angle_falloff_start;  /* SPOT, degrees */

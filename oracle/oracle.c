@@ -2501,6 +2501,8 @@ static spec light_phi(const OrcScene* s, int32_t li)
         return smulf(smulf(c, orc_two_pi), ((1.0f - x->cos_min) + (x->cos_min - x->cos_max) / 2.0f));
     case BB_LIGHT_UNIFORM_INFINITE:
         return smulf(smulf(smulf(smulf(c, L->scale), orc_four_pi), orc_pi), sqrf(x->world_radius));
+    case BB_LIGHT_POINT: /* point_light.cpp:93-96 */
+        return smulf(c, orc_four_pi);
     default: {
         int32_t tri = s->prims[L->primitive].shape_index;
         return smulf(smulf(smulf(c, triangle_area(s, tri)), orc_pi), (float)(L->two_sided ? 2 : 1));
@@ -2533,6 +2535,15 @@ static int light_sample_li(const OrcScene* s, int32_t li, v3 ref, v2 u, light_sa
         ls->visibility = distance;
         ls->pdf = 1.0f;
         ls->Li = sdivf(l, distance * distance);
+        return 1;
+    }
+    if (L->type == BB_LIGHT_POINT) { /* point_light.cpp:30-42 */
+        v3 wi = vsub(V3(L->position[0], L->position[1], L->position[2]), ref);
+        float distance = vnormalize_m(&wi);
+        ls->wi = wi;
+        ls->visibility = distance;
+        ls->pdf = 1.0f;
+        ls->Li = sdivf(c, distance * distance);
         return 1;
     }
     if (L->type == BB_LIGHT_UNIFORM_INFINITE) {
@@ -2767,7 +2778,7 @@ OrcScene* orc_scene_create(const BbSceneDesc* d, char* err, int32_t err_size)
             if (L->primitive < 0 || L->primitive >= s->n_prims) FAIL("light %d: bad primitive", i);
             if (s->prims[L->primitive].shape_kind != BB_SHAPE_TRIANGLE) FAIL("light %d: area lights must be triangles", i);
             s->prim_light[L->primitive] = i;
-        } else {
+        } else if (L->type != BB_LIGHT_POINT) {
             FAIL("light %d: unknown type", i);
         }
     }
@@ -2823,7 +2834,8 @@ static spec sample_direct_light(const OrcScene* s, v3 wo, const isect_t* is, con
     if (bvh_raycast(s, is->point, ls.wi, 1e-4f, ls.visibility, 1, NULL, NULL)) return S1(0.0f);
     float light_pdf = pmf * ls.pdf;
     spec f_cos = smulf(bsdf_f(bsdf, wo, ls.wi), vabsdot(is->ns, ls.wi));
-    if (s->lights[li].type == BB_LIGHT_SPOT) return sdivf(smul(smul(beta, ls.Li), f_cos), light_pdf);
+    if (s->lights[li].type == BB_LIGHT_SPOT || s->lights[li].type == BB_LIGHT_POINT) /* delta lights */
+        return sdivf(smul(smul(beta, ls.Li), f_cos), light_pdf);
     float mis = power_heuristic(light_pdf, bsdf_pdf_v);
     return sdivf(smul(smul(smulf(beta, mis), ls.Li), f_cos), light_pdf);
 }

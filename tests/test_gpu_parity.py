@@ -200,6 +200,31 @@ def test_render_readme_scene_parity(readme):
     assert frac >= 0.999
 
 
+def test_render_point_lights_parity():
+    """PointLight (delta, Li = I / d^2) beside the README scene's lights:
+    within 1e-3 of the oracle, bit-equal with its libm correctly rounded."""
+    from bulbit_amd.gpu import GpuScene
+    from oracle import OracleScene, correctly_rounded_libm
+
+    sc, cam = scenes.readme_scene(320, 180)
+    sc.add_point_light((1.5, 2.0, 1.0), (3.0, 2.5, 2.0))
+    sc.add_point_light((-2.0, 0.6, 0.5), (0.5, 1.0, 2.0))
+    g, o = GpuScene(sc), OracleScene(sc)
+    p = scenes.render_params(16)
+    gi, oi, gc, oc = _render_pair(g, o, cam, p)
+    assert np.array_equal(gc, oc)
+    frac = pixel_agreement(gi, oi)
+    log(f"point lights 320x180x16: {frac:.6f} within 1e-3, {np.mean(gi == oi):.4f} bit-equal")
+    assert frac >= 0.999
+    gs, _, _ = g.render(cam, p)
+    try:
+        correctly_rounded_libm(True)
+        cs, _, _ = o.render(cam, p, threads=THREADS)
+    finally:
+        correctly_rounded_libm(False)
+    assert np.array_equal(cs.view(np.int32), gs.view(np.int32))
+
+
 @pytest.mark.parametrize("kind", ["tent", "gaussian"])
 def test_render_with_tent_and_gaussian_filters(readme, kind):
     """TentFilter and GaussianFilter (the reference's default camera filter):

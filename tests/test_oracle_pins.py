@@ -539,6 +539,37 @@ def test_power_light_pmf_closed_form():
     assert np.allclose(pmf, power / power.sum(), rtol=1e-5)
 
 
+def test_point_light_direct_lighting_closed_form():
+    """A point light of intensity I at height h over a Lambertian plane of
+    albedo rho, direct light only (max_bounces 1): straight below the light
+    the radiance is rho / pi * I / h^2 (PointLight::Sample_Li,
+    point_light.cpp:30-42; a delta light, so no MIS weight).  A dropped
+    square, an MIS weight or a pdf other than 1 moves it.  The power-sampler
+    PMF of a point light is 4 pi I's luminance over the total."""
+    from oracle import OracleScene
+
+    sc = scenes.SceneData()
+    m = sc.add_material(scenes.diffuse((0.5, 0.25, 0.125)))
+    q = np.array([[-50, 0, -50], [50, 0, -50], [50, 0, 50], [-50, 0, 50]], np.float32)
+    sc.add_mesh(q, [[0, 1, 0]] * 4, [[1, 0, 0]] * 4, [[0, 2, 1], [0, 3, 2]], m)
+    h, I = 2.0, np.array([3.0, 2.0, 1.0])
+    sc.add_point_light((0.0, h, 0.0), I)
+    sc.add_spot_light((5.0, 5.0, 5.0), (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0), 30.0, 20.0)
+    cam = scenes.make_camera((0.0, 1.0, 0.0), (0.0, 0.0, 0.0), 0.05, 0.0, 1.0, 4, 4, up=(0.0, 0.0, -1.0))
+    o = OracleScene(sc)
+    pmf = o.light_pmf()
+    lum = lambda c: 0.2126 * c[0] + 0.7152 * c[1] + 0.0722 * c[2]
+    spot_phi = 2 * np.pi * ((1 - np.cos(np.radians(20))) + (np.cos(np.radians(20)) - np.cos(np.radians(30))) / 2)
+    powers = np.array([4 * np.pi * lum(I), spot_phi * lum((1, 1, 1))])
+    assert np.allclose(pmf, powers / powers.sum(), rtol=1e-5)
+    # the point light alone: NEE then has no light-selection noise
+    sc.lights = sc.lights[:1]
+    s, c, _ = OracleScene(sc).render(cam, scenes.render_params(16, max_bounces=1), threads=1)
+    img = s / c[..., None]
+    want = np.array([0.5, 0.25, 0.125]) / np.pi * I / h ** 2
+    assert np.allclose(img.reshape(-1, 3), want, rtol=1e-4), (img.reshape(-1, 3)[0], want)
+
+
 # ------------------------------------------------------------ integrator
 
 def test_furnace_convex_lambertian_sphere_in_uniform_light():

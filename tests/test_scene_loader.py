@@ -123,6 +123,11 @@ XML = """<?xml version="1.0" encoding="utf-8"?>
         <float name="cutoff_angle" value="30"/>
         <transform name="to_world"><lookat origin="2 3 2" target="0 0 0" up="0 1 0"/></transform>
     </emitter>
+    <emitter type="point">
+        <point name="position" x="0.5" y="1.5" z="1"/>
+        <rgb name="intensity" value="2 1.5 1"/>
+        <transform name="to_world"><translate x="0.25"/></transform>
+    </emitter>
     <emitter type="constant"><rgb name="radiance" value="0.1"/></emitter>
 </scene>
 """
@@ -292,7 +297,7 @@ def test_shapes_are_placed_by_their_transforms(desc):
 
 def test_lights(desc):
     L = desc["lights"]
-    assert [list(x)[0] for x in L] == ["area", "area", "spot", "infinite"]
+    assert [list(x)[0] for x in L] == ["area", "area", "spot", "point", "infinite"]
     assert [x["area"]["primitive"] for x in L[:2]] == [30, 31]
     assert all(x["area"]["emission"]["constant"] == [10, 9, 8] and not x["area"]["two_sided"] for x in L[:2])
     spot = L[2]["spot"]
@@ -300,7 +305,9 @@ def test_lights(desc):
     assert spot["position"] == [2, 3, 2] and np.allclose(spot["direction"], d, atol=1e-6)
     # beam_width defaults to 3/4 of the default cutoff (20), not of the given one
     assert spot["angle_max"] == 30 and spot["angle_falloff_start"] == 15 and spot["intensity"] == [4, 4, 4]
-    assert L[3]["infinite"] == {"l": pytest.approx([0.1, 0.1, 0.1]), "scale": 1}
+    # the point's position moved by to_world
+    assert L[3]["point"] == {"position": [0.75, 1.5, 1], "intensity": [2, 1.5, 1]}
+    assert L[4]["infinite"] == {"l": pytest.approx([0.1, 0.1, 0.1]), "scale": 1}
 
 
 def test_transform_math_against_rotation_matrices(tmp_path):
@@ -345,7 +352,7 @@ def test_filters_and_defaults(tmp_path):
 
 
 @pytest.mark.parametrize("xml, what", [
-    ('<emitter type="point"/>', "point emitter"),
+    ('<emitter type="directional"/>', "directional emitter"),
     ('<emitter type="envmap"><string name="filename" value="sky.hdr"/></emitter>', "envmap emitter"),
     ('<bsdf type="principled" id="p"/>', "principled BSDF"),
     ('<bsdf type="plastic" id="p"/>', "plastic BSDF"),
@@ -463,6 +470,8 @@ def scene_from_description(d):
         (kind, v), = L.items()
         if kind == "area":
             sc.add_area_light(v["primitive"], v["emission"]["constant"], v["two_sided"])
+        elif kind == "point":
+            sc.add_point_light(v["position"], v["intensity"])
         elif kind == "spot":
             sc.add_spot_light(v["position"], v["direction"], v["intensity"], v["angle_max"], v["angle_falloff_start"])
         else:
