@@ -12,7 +12,7 @@ from ctypes import POINTER, c_char_p, c_double, c_float, c_int32, c_size_t, c_ui
 
 import numpy as np
 
-ABI_VERSION = 12  # BBGPU_ABI_VERSION of include/bbgpu.h that these structures mirror
+ABI_VERSION = 13  # BBGPU_ABI_VERSION of include/bbgpu.h that these structures mirror
 
 LIB_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(LIB_DIR, "libbbgpu.so")
@@ -25,6 +25,7 @@ BB_MAT_DIELECTRIC = 1
 BB_MAT_CONDUCTOR = 2
 BB_MAT_CONDUCTOR_R = 3
 BB_MAT_METALLIC_ROUGHNESS = 4
+BB_MAT_MIRROR = 5
 
 BB_FILTER_BOX = 0
 BB_FILTER_TENT = 1

@@ -179,6 +179,12 @@ struct SceneExport
             b.tex_v_roughness = param(c->v_roughness, 0.0f, b.v_roughness);
             if (c->normal) b.tex_normal = add_texture(c->normal);
             if (c->alpha) b.tex_alpha = add_texture(c->alpha);
+        } else if (auto* mi = dynamic_cast<const MirrorMaterial*>(m)) {
+            b.type = BB_MAT_MIRROR;
+            if (!mi->reflectance) fatal("MirrorMaterial", "reflectance texture is required");
+            param3(mi->reflectance, Spectrum(1), b.color, b.tex_color);
+            if (mi->normal) b.tex_normal = add_texture(mi->normal);
+            if (mi->alpha) b.tex_alpha = add_texture(mi->alpha);
         } else if (auto* r = dynamic_cast<const MetallicRoughnessMaterial*>(m)) {
             b.type = BB_MAT_METALLIC_ROUGHNESS;
             param3(r->basecolor, Spectrum(0), b.color, b.tex_color);

@@ -71,6 +71,9 @@ std::string material(const Material* m)
                ",\"reflectance\":" + texture(c->reflectance) + ",\"energy_compensation\":" +
                (c->energy_compensation ? "true" : "false") + ",\"normal\":" + texture(c->normal) +
                ",\"alpha\":" + texture(c->alpha) + "}";
+    if (auto* r = dynamic_cast<const MirrorMaterial*>(m))
+        return "{\"type\":\"mirror\",\"reflectance\":" + texture(r->reflectance) + ",\"normal\":" +
+               texture(r->normal) + ",\"alpha\":" + texture(r->alpha) + "}";
     if (auto* r = dynamic_cast<const MetallicRoughnessMaterial*>(m))
         return "{\"type\":\"metallic_roughness\",\"basecolor\":" + texture(r->basecolor) + ",\"metallic\":" +
                texture(r->metallic) + ",\"u_roughness\":" + texture(r->u_roughness) + ",\"v_roughness\":" +

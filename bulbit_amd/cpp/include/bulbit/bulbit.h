@@ -460,6 +460,21 @@ public:
     const FloatTexture* alpha;
 };
 
+// MirrorMaterial (include/bulbit/materials.h:33-51): specular reflection scaled by reflectance
+class MirrorMaterial : public Material
+{
+public:
+    MirrorMaterial(const SpectrumTexture* reflectance, const SpectrumTexture* normal = nullptr,
+                   const FloatTexture* alpha = nullptr)
+        : reflectance{ reflectance }, normal{ normal }, alpha{ alpha }
+    {
+    }
+
+    const SpectrumTexture* reflectance;
+    const SpectrumTexture* normal;
+    const FloatTexture* alpha;
+};
+
 // DielectricMaterial (include/bulbit/materials.h:52-84)
 class DielectricMaterial : public Material
 {
@@ -1217,7 +1232,7 @@ struct RendererInfo
 // `render_info`, with the reference loader's element rules.  Relative file
 // names resolve against the scene file's folder.  Elements the reference
 // reads but the GPU path does not render (media, directional and
-// envmap emitters, mirror, principled, plastic, layered, blend, thin
+// envmap emitters, principled, plastic, layered, blend, thin
 // dielectric and subsurface BSDFs, other integrators) make it print
 // `bulbit_amd: LoadScene: ...` and return false; elements the reference
 // itself skips with a message are skipped with a message.

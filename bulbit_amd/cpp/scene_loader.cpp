@@ -803,10 +803,13 @@ private:
                     warn("Unsupported parameter for (rough)conductor material: " + name);
                 }
             }
-            if (eta == nullptr && k == nullptr) throw Unsupported("the mirror BSDF (a conductor without eta and k)");
-            if (eta == nullptr || k == nullptr) throw std::runtime_error("a conductor needs both eta and k");
-            mat = scene->CreateMaterial<ConductorMaterial>(eta, k, u_roughness, v_roughness, reflectance, true,
-                                                           mi.normal, mi.alpha);
+            if (eta == nullptr && k == nullptr) {
+                mat = scene->CreateMaterial<MirrorMaterial>(SpectrumConstant(Spectrum(1)), mi.normal, mi.alpha);
+            } else {
+                if (eta == nullptr || k == nullptr) throw std::runtime_error("a conductor needs both eta and k");
+                mat = scene->CreateMaterial<ConductorMaterial>(eta, k, u_roughness, v_roughness, reflectance, true,
+                                                               mi.normal, mi.alpha);
+            }
         } else if (type == "dielectric" || type == "roughdielectric" || type == "rough_dielectric") {
             Float int_ior = 1.5046f, ext_ior = 1.000277f;
             const SpectrumTexture* reflectance = SpectrumConstant(Spectrum(1));

@@ -59,7 +59,8 @@ enum : int32_t
     MK_METALLIC_ROUGHNESS = 3,
     MK_CONDUCTOR_MS = 4, // ConductorMultiScatteringBxDF (energy_compensation)
     MK_DIELECTRIC_MS = 5, // DielectricMultiScatteringBxDF (energy_compensation, rough, eta != 1)
-    MK_EON = 6            // EONBxDF: DiffuseMaterial with roughness > 0 (roughness kept in ax and ay)
+    MK_EON = 6,           // EONBxDF: DiffuseMaterial with roughness > 0 (roughness kept in ax and ay)
+    MK_MIRROR = 7         // SpecularReflectionBxDF: MirrorMaterial
 };
 
 // Reflectance tables of PrepareReflectanceTextures (microfacet.cpp:239-270):

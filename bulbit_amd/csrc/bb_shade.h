@@ -352,6 +352,10 @@ BB_D GpuMaterial material_at(const GpuMaterial& m0, const SceneView& S, f2 uv)
         m.flags_front = smooth ? (BX_SPECULAR | BX_REFLECTION) : (BX_GLOSSY | BX_REFLECTION);
         break;
     }
+    case BB_MAT_MIRROR:
+        if (m.tex_color >= 0) m.r = texture3(S, m.tex_color, uv);
+        m.flags_front = !is_black(m.r) ? (BX_SPECULAR | BX_REFLECTION) : 0;
+        break;
     case BB_MAT_METALLIC_ROUGHNESS:
         if (m.tex_color >= 0) m.color = texture3(S, m.tex_color, uv);
         if (m.tex_metallic >= 0) m.metallic = texture1(S, m.tex_metallic, uv);
@@ -956,6 +960,7 @@ BB_D spec bxdf_f(const bxdf& b, f3 wo, f3 wi)
     case MK_DIELECTRIC_MS: return dielectric_ms_f(b, wo, wi);
     case MK_CONDUCTOR: return conductor_f(b, wo, wi);
     case MK_CONDUCTOR_MS: return conductor_ms_f(b, wo, wi);
+    case MK_MIRROR: return mks(0.0f); // specular_reflection_bxdf.cpp:7-13
     default: return mr_f(b, wo, wi);
     }
 }
@@ -968,6 +973,7 @@ BB_D float bxdf_pdf(const bxdf& b, f3 wo, f3 wi)
     case MK_DIELECTRIC_MS: return dielectric_ms_pdf(b, wo, wi);
     case MK_CONDUCTOR: return conductor_pdf(b, wo, wi);
     case MK_CONDUCTOR_MS: return conductor_ms_pdf(b, wo, wi);
+    case MK_MIRROR: return 0.0f;
     default: return mr_pdf(b, wo, wi);
     }
 }
@@ -988,6 +994,15 @@ BB_D bool bxdf_sample(const bxdf& b, f3 wo, float u0, f2 u12, bsample& s)
         return true;
     }
     case MK_EON: return eon_sample(b, wo, u0, u12, s);
+    case MK_MIRROR: {
+        // specular_reflection_bxdf.cpp:24-38: wo mirrored about the normal, weight r / |cos|
+        const f3 wi = mk3(-wo.x, -wo.y, wo.z);
+        s.f = b.r / fabsf(wi.z);
+        s.wi = wi;
+        s.pdf = 1.0f;
+        s.flags = BX_SPECULAR | BX_REFLECTION;
+        return true;
+    }
     case MK_DIELECTRIC_MS: return dielectric_ms_sample(b, wo, u0, u12, s);
     case MK_CONDUCTOR_MS:
         if (!ggx_smooth(b.mf)) {

@@ -387,6 +387,12 @@ bool material_constants(const BbMaterial& m, GpuMaterial& g, std::string& err)
         g.flags_front = smooth ? (BX_SPECULAR | BX_REFLECTION) : (BX_GLOSSY | BX_REFLECTION);
         return true;
     }
+    case BB_MAT_MIRROR:
+        // MirrorMaterial::GetBSDF (mirror_material.cpp:18-23): SpecularReflectionBxDF (bxdfs.h:48-73)
+        g.kind = MK_MIRROR;
+        g.r = mks(m.color[0], m.color[1], m.color[2]);
+        g.flags_front = !is_black(g.r) ? (BX_SPECULAR | BX_REFLECTION) : 0;
+        return true;
     case BB_MAT_METALLIC_ROUGHNESS:
         // MetallicRoughnessMaterial::GetBSDF (metallic_roughness_material.cpp:28-39)
         g.kind = MK_METALLIC_ROUGHNESS;

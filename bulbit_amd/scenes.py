@@ -132,6 +132,13 @@ def conductor(eta, k, roughness, color=(1.0, 1.0, 1.0), energy_compensation=0):
     return m
 
 
+def mirror(color=(1.0, 1.0, 1.0)):
+    m = capi.BbMaterial()
+    m.type = capi.BB_MAT_MIRROR
+    m.color[:] = list(color)
+    return m
+
+
 def metallic_roughness(color, metallic, roughness):
     m = capi.BbMaterial()
     m.type = capi.BB_MAT_METALLIC_ROUGHNESS

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define BBGPU_ABI_VERSION 12
+#define BBGPU_ABI_VERSION 13
 
 /* status codes */
 enum {
@@ -64,6 +64,7 @@ enum {
     BB_MAT_CONDUCTOR = 2,          /* ConductorMaterial(eta, k, ...)  conductor_material.cpp:52-99 */
     BB_MAT_CONDUCTOR_R = 3,        /* ConductorMaterial(R, ...)       conductor_material.cpp:33-50 */
     BB_MAT_METALLIC_ROUGHNESS = 4, /* MetallicRoughnessMaterial  metallic_roughness_material.cpp:28-39 */
+    BB_MAT_MIRROR = 5,             /* MirrorMaterial(R)   src/material/mirror_material.cpp:18-23; R in color */
 };
 
 /* Materials.  Field meaning per type:

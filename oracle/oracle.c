@@ -1374,7 +1374,7 @@ float orc_ggx_E_avg(float ax, float ay)
 
 /* ================================================================= BxDFs */
 
-enum { K_LAMBERT = 0, K_DIELECTRIC = 1, K_CONDUCTOR = 2, K_MR = 3, K_CONDUCTOR_MS = 4, K_DIELECTRIC_MS = 5, K_EON = 6 };
+enum { K_LAMBERT = 0, K_DIELECTRIC = 1, K_CONDUCTOR = 2, K_MR = 3, K_CONDUCTOR_MS = 4, K_DIELECTRIC_MS = 5, K_EON = 6, K_MIRROR = 7 };
 
 typedef struct {
     int kind;
@@ -1400,6 +1400,7 @@ static int bxdf_flags(const bxdf_t* b)
     switch (b->kind) {
     case K_LAMBERT:
     case K_EON: return !sblack(b->r) ? (BX_DIFFUSE | BX_REFLECTION) : 0;
+    case K_MIRROR: return !sblack(b->r) ? (BX_SPECULAR | BX_REFLECTION) : 0; /* bxdfs.h:55-58 */
     case K_DIELECTRIC:
     case K_DIELECTRIC_MS: {
         int f = (b->eta == 1.0f) ? BX_TRANSMISSION : (BX_REFLECTION | BX_TRANSMISSION);
@@ -2332,6 +2333,10 @@ static int make_bsdf_c(const BbMaterial* m, v3 ns, v3 ts, int front_face, bsdf_t
             b->ck = sdiv(smulf(ssqrt(r), 2.0f), ssqrt(mx));
         }
         return 1;
+    case BB_MAT_MIRROR: /* mirror_material.cpp:18-23: SpecularReflectionBxDF(r) */
+        b->kind = K_MIRROR;
+        b->r = S3(m->color[0], m->color[1], m->color[2]);
+        return 1;
     case BB_MAT_METALLIC_ROUGHNESS:
         b->kind = K_MR;
         b->color = S3(m->color[0], m->color[1], m->color[2]);
@@ -2390,6 +2395,7 @@ static spec bsdf_f(const bsdf_t* s, v3 wo_w, v3 wi_w)
     case K_DIELECTRIC_MS: return dielectric_ms_f(&s->bx, wo, wi);
     case K_CONDUCTOR: return conductor_f(&s->bx, wo, wi);
     case K_CONDUCTOR_MS: return conductor_ms_f(&s->bx, wo, wi);
+    case K_MIRROR: return S1(0.0f); /* specular_reflection_bxdf.cpp:7-13 */
     default: return mr_f(&s->bx, wo, wi);
     }
 }
@@ -2407,6 +2413,7 @@ static float bsdf_pdf(const bsdf_t* s, v3 wo_w, v3 wi_w)
     case K_DIELECTRIC_MS: return dielectric_ms_pdf(&s->bx, wo, wi);
     case K_CONDUCTOR: return conductor_pdf(&s->bx, wo, wi);
     case K_CONDUCTOR_MS: return conductor_ms_pdf(&s->bx, wo, wi);
+    case K_MIRROR: return 0.0f;
     default: return mr_pdf(&s->bx, wo, wi);
     }
 }
@@ -2424,6 +2431,17 @@ static int bsdf_sample(const bsdf_t* s, v3 wo_w, float u0, v2 u12, bsample* out)
     case K_DIELECTRIC_MS: ok = dielectric_ms_sample(&s->bx, wo, u0, u12, out); break;
     case K_CONDUCTOR: ok = conductor_sample(&s->bx, wo, u12, out); break;
     case K_CONDUCTOR_MS: ok = conductor_ms_sample(&s->bx, wo, u0, u12, out); break;
+    case K_MIRROR: { /* specular_reflection_bxdf.cpp:24-38 */
+        v3 wi = V3(-wo.x, -wo.y, wo.z);
+        out->f = sdivf(s->bx.r, fabsf(wi.z));
+        out->wi = wi;
+        out->pdf = 1.0f;
+        out->flags = BX_SPECULAR | BX_REFLECTION;
+        out->eta = 1.0f;
+        out->stochastic = 0;
+        ok = 1;
+        break;
+    }
     default: ok = mr_sample(&s->bx, wo, u0, u12, out); break;
     }
     if (!ok) return 0;
@@ -2440,6 +2458,7 @@ static int material_supported(const BbMaterial* m, char* err, int32_t err_size)
     case BB_MAT_CONDUCTOR_R: return 1;
     case BB_MAT_DIELECTRIC: return 1;
     case BB_MAT_METALLIC_ROUGHNESS: return 1;
+    case BB_MAT_MIRROR: return 1;
     }
     snprintf(err, err_size, "unknown material type %d", m->type);
     return 0;

@@ -200,6 +200,31 @@ def test_render_readme_scene_parity(readme):
     assert frac >= 0.999
 
 
+def test_render_mirror_parity():
+    """MirrorMaterial on the README scene's spheres:
+    within 1e-3 of the oracle, bit-equal with its libm correctly rounded."""
+    from bulbit_amd.gpu import GpuScene
+    from oracle import OracleScene, correctly_rounded_libm
+
+    sc, cam = scenes.readme_scene(320, 180)
+    mid = sc.add_material(scenes.mirror((0.9, 0.8, 0.7)))
+    sc.prims[:, 2] = np.where(sc.prims[:, 0] == capi.BB_SHAPE_SPHERE, mid, sc.prims[:, 2])
+    g, o = GpuScene(sc), OracleScene(sc)
+    p = scenes.render_params(16)
+    gi, oi, gc, oc = _render_pair(g, o, cam, p)
+    assert np.array_equal(gc, oc)
+    frac = pixel_agreement(gi, oi)
+    log(f"mirror 320x180x16: {frac:.6f} within 1e-3, {np.mean(gi == oi):.4f} bit-equal")
+    assert frac >= 0.999
+    gs, _, _ = g.render(cam, p)
+    try:
+        correctly_rounded_libm(True)
+        cs, _, _ = o.render(cam, p, threads=THREADS)
+    finally:
+        correctly_rounded_libm(False)
+    assert np.array_equal(cs.view(np.int32), gs.view(np.int32))
+
+
 def test_render_point_lights_parity():
     """PointLight (delta, Li = I / d^2) beside the README scene's lights:
     within 1e-3 of the oracle, bit-equal with its libm correctly rounded."""

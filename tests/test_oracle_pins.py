@@ -539,6 +539,22 @@ def test_power_light_pmf_closed_form():
     assert np.allclose(pmf, power / power.sum(), rtol=1e-5)
 
 
+def test_mirror_reflects_with_weight_r(orc):
+    """MirrorMaterial (SpecularReflectionBxDF, specular_reflection_bxdf.cpp):
+    the sample is wo mirrored about the normal with f |cos| / pdf = R on
+    both sides, and f and pdf of any pair are 0."""
+    R = (0.9, 0.5, 0.25)
+    m = scenes.mirror(R)
+    for theta, phi in ((0.3, 0.2), (1.2, 2.5), (2.6, -1.0)):
+        wo = _dir(theta, phi)
+        ok, f, wi, pdf, flags, eta = _sample(orc, m, wo, 0.37, (0.1, 0.9))
+        assert ok and pdf == 1.0 and eta == 1.0
+        assert np.allclose(wi, [-wo[0], -wo[1], wo[2]], atol=1e-7)
+        assert np.allclose(np.asarray(f) * abs(wi[2]) / pdf, R, rtol=1e-6)
+        fe, pe, _ = _eval(orc, m, wo, wi)
+        assert np.all(np.asarray(fe) == 0) and pe == 0
+
+
 def test_point_light_direct_lighting_closed_form():
     """A point light of intensity I at height h over a Lambertian plane of
     albedo rho, direct light only (max_bounces 1): straight below the light

@@ -118,6 +118,11 @@ XML = """<?xml version="1.0" encoding="utf-8"?>
         <bsdf type="diffuse"><rgb name="reflectance" value="0.5"/></bsdf>
         <emitter type="area"><rgb name="radiance" value="10 9 8"/></emitter>
     </shape>
+    <shape type="sphere">
+        <point name="center" x="-0.3" y="0.25" z="0.8"/>
+        <float name="radius" value="0.25"/>
+        <bsdf type="conductor"><string name="material" value="none"/><rgb name="eta" value="1"/></bsdf>
+    </shape>
     <emitter type="spot">
         <rgb name="intensity" value="4"/>
         <float name="cutoff_angle" value="30"/>
@@ -218,7 +223,8 @@ def test_camera_film_sampler_and_integrator(desc):
 
 def test_materials_follow_the_reference_rules(desc):
     m = desc["materials"]
-    assert [x["type"] for x in m] == ["diffuse", "conductor", "dielectric", "diffuse", "diffuse", "diffuse", "diffuse"]
+    assert [x["type"] for x in m] == ["diffuse", "conductor", "dielectric", "diffuse", "diffuse", "diffuse", "diffuse",
+                                      "mirror"]
     floor = m[0]["reflectance"]
     # color0 is the checker's second value, color1 its first; resolution = 2 x to_uv's scale
     assert floor["checker"][0]["constant"] == pytest.approx([0.1, 0.1, 0.8])
@@ -249,11 +255,13 @@ def test_materials_follow_the_reference_rules(desc):
     assert cutout["alpha"]["checker"][1]["constant"] == 1
     lamp = m[6]
     assert lamp["reflectance"]["constant"] == [0.5, 0.5, 0.5]
+    # material="none": a mirror; the parameters after it are not read
+    assert m[7] == {"type": "mirror", "reflectance": {"constant": [1, 1, 1]}, "normal": None, "alpha": None}
 
 
 def test_shapes_are_placed_by_their_transforms(desc):
     p = desc["primitives"]
-    assert len(p) == 2 + 12 + 1 + 1 + 12 + 2 + 2
+    assert len(p) == 2 + 12 + 1 + 1 + 12 + 2 + 2 + 1 and p[32]["material"] == 7
     # floor: Rx(-90) * scale 3 * (scale 2 of the unit quad)
     R = _rot((1, 0, 0), -90)
     for i, tri in enumerate(_quad_tris(QUAD * 6)):
@@ -356,7 +364,6 @@ def test_filters_and_defaults(tmp_path):
     ('<emitter type="envmap"><string name="filename" value="sky.hdr"/></emitter>', "envmap emitter"),
     ('<bsdf type="principled" id="p"/>', "principled BSDF"),
     ('<bsdf type="plastic" id="p"/>', "plastic BSDF"),
-    ('<bsdf type="conductor" id="m"/>', "mirror BSDF"),
     ('<medium type="homogeneous" id="fog"/>', "participating media"),
     ('<integrator type="bdpt"/>', "bdpt integrator"),
     ('<sensor type="perspective"><sampler type="stratified"/></sensor>', "stratified sampler"),
@@ -436,6 +443,10 @@ def scene_from_description(d):
             b.color[:] = c
             b.u_roughness, b.tex_u_roughness = _param(sc, m["u_roughness"], 0.0, 1)
             b.v_roughness, b.tex_v_roughness = _param(sc, m["v_roughness"], 0.0, 1)
+        elif kind == "mirror":
+            b.type = capi.BB_MAT_MIRROR
+            c, b.tex_color = _param(sc, m["reflectance"], [1, 1, 1], 3)
+            b.color[:] = c
         elif kind == "conductor":
             b.type = capi.BB_MAT_CONDUCTOR
             b.energy_compensation = int(m["energy_compensation"])
