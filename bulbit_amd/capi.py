@@ -12,7 +12,7 @@ from ctypes import POINTER, c_char_p, c_double, c_float, c_int32, c_size_t, c_ui
 
 import numpy as np
 
-ABI_VERSION = 13  # BBGPU_ABI_VERSION of include/bbgpu.h that these structures mirror
+ABI_VERSION = 14  # BBGPU_ABI_VERSION of include/bbgpu.h that these structures mirror
 
 LIB_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(LIB_DIR, "libbbgpu.so")
@@ -35,6 +35,7 @@ BB_LIGHT_SPOT = 0
 BB_LIGHT_UNIFORM_INFINITE = 1
 BB_LIGHT_DIFFUSE_AREA = 2
 BB_LIGHT_POINT = 3
+BB_LIGHT_DIRECTIONAL = 4
 
 
 class BbPrimitive(ctypes.Structure):

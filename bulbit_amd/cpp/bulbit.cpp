@@ -275,6 +275,10 @@ struct SceneExport
             b.type = BB_LIGHT_POINT;
             put3(b.position, p->position);
             put3(b.spectrum, p->intensity);
+        } else if (auto* dl = dynamic_cast<const DirectionalLight*>(l)) {
+            b.type = BB_LIGHT_DIRECTIONAL;
+            put3(b.direction, dl->direction);
+            put3(b.spectrum, dl->intensity);
         } else if (auto* u = dynamic_cast<const UniformInfiniteLight*>(l)) {
             b.type = BB_LIGHT_UNIFORM_INFINITE;
             put3(b.spectrum, u->l);

@@ -155,6 +155,8 @@ int main(int argc, char** argv)
             s = "{\"spot\":{\"position\":" + vec(l->position) + ",\"direction\":" + vec(l->direction) +
                 ",\"intensity\":" + spec(l->intensity) + ",\"angle_max\":" + num(l->angle_max) +
                 ",\"angle_falloff_start\":" + num(l->angle_falloff_start) + "}}";
+        else if (auto* l = dynamic_cast<const DirectionalLight*>(lights[i]))
+            s = "{\"directional\":{\"direction\":" + vec(l->direction) + ",\"intensity\":" + spec(l->intensity) + "}}";
         else if (auto* l = dynamic_cast<const PointLight*>(lights[i]))
             s = "{\"point\":{\"position\":" + vec(l->position) + ",\"intensity\":" + spec(l->intensity) + "}}";
         else if (auto* l = dynamic_cast<const UniformInfiniteLight*>(lights[i]))

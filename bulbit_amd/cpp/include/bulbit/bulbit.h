@@ -682,6 +682,16 @@ public:
     const Medium* medium;
 };
 
+// DirectionalLight (include/bulbit/lights.h:65-95): light arriving from `direction`
+class DirectionalLight : public Light
+{
+public:
+    DirectionalLight(const Vec3& direction, const Spectrum& intensity) : direction{ direction }, intensity{ intensity } {}
+
+    Vec3 direction;
+    Spectrum intensity;
+};
+
 // UniformInfiniteLight (include/bulbit/lights.h:236-260)
 class UniformInfiniteLight : public Light
 {
@@ -1231,8 +1241,7 @@ struct RendererInfo
 // LoadScene (cli/scene_loader.cpp:2147-2164): a Mitsuba 3 XML scene into
 // `render_info`, with the reference loader's element rules.  Relative file
 // names resolve against the scene file's folder.  Elements the reference
-// reads but the GPU path does not render (media, directional and
-// envmap emitters, principled, plastic, layered, blend, thin
+// reads but the GPU path does not render (media, envmap emitters, principled, plastic, layered, blend, thin
 // dielectric and subsurface BSDFs, other integrators) make it print
 // `bulbit_amd: LoadScene: ...` and return false; elements the reference
 // itself skips with a message are skipped with a message.

@@ -1033,7 +1033,18 @@ private:
                     to_world = ParseTransform(c);
             }
             scene->CreateLight<PointLight>(Mul(to_world, position), intensity, nullptr);
-        } else if (type == "directional" || type == "envmap") {
+        } else if (type == "directional") {
+            Spectrum irradiance(1);
+            Transform to_world = identity;
+            Vec3 direction(0, 0, -1);
+            for (const XmlNode& c : node.children) {
+                const std::string name = c.value("name");
+                if (name == "irradiance") irradiance = Intensity(c);
+                else if (name == "to_world") to_world = ParseTransform(c);
+                else if (name == "direction") direction = ParseVec3(c.attribute("value"));
+            }
+            scene->CreateLight<DirectionalLight>(to_world.q.Rotate(direction), irradiance);
+        } else if (type == "envmap") {
             throw Unsupported("the " + type + " emitter");
         } else {
             warn("Light not supported: " + type);

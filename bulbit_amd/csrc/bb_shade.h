@@ -1186,6 +1186,14 @@ struct light_sample
 // Light::Sample_Li for spot, point, uniform infinite and diffuse area (triangle) lights
 BB_D bool light_sample_li(const GpuLight& L, f3 ref, f2 u, light_sample& ls)
 {
+    if (L.type == BB_LIGHT_DIRECTIONAL) {
+        // directional_light.cpp:29-38
+        ls.wi = L.z;
+        ls.visibility = 2.0f * L.world_radius;
+        ls.pdf = 1.0f;
+        ls.Li = L.L;
+        return true;
+    }
     if (L.type == BB_LIGHT_POINT) {
         // point_light.cpp:30-42
         f3 wi = L.p - ref;

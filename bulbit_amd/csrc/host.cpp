@@ -611,6 +611,15 @@ bool create_scene(const BbSceneDesc& d, BbScene* s, std::string& err)
             // DiffuseAreaLight::Phi (diffuse_area_light.cpp:144-148), Triangle::Area (triangle.cpp:255-264)
             float area = 0.5f * length(cross(g.e1, g.e2));
             phi = ((c * area) * kPi) * float(L.two_sided ? 2 : 1);
+        } else if (L.type == BB_LIGHT_DIRECTIONAL) {
+            // DirectionalLight ctor and Preprocess (directional_light.cpp:7-17), Phi (:92-95)
+            g.z = normalize(mk3(L.direction[0], L.direction[1], L.direction[2]));
+            f3 ctr = (wmin + wmax) * 0.5f;
+            bool inside = !(wmin.x > ctr.x || wmax.x < ctr.x) && !(wmin.y > ctr.y || wmax.y < ctr.y) &&
+                          !(wmin.z > ctr.z || wmax.z < ctr.z);
+            g.world_radius = inside ? length(wmax - ctr) : 0.0f;
+            g.L = c;
+            phi = (c * kPi) * sqr(g.world_radius);
         } else if (L.type == BB_LIGHT_POINT) {
             // PointLight ctor (point_light.cpp:7-13), Phi (:93-96)
             g.p = mk3(L.position[0], L.position[1], L.position[2]);

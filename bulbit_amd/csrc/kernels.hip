@@ -974,7 +974,8 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView S, RenderConsts
                                         const spec f = (wo_l.z == 0.0f) ? mks(0.0f) : bxdf_f(b, wo_l, wi_l);
                                         const float light_pdf = pmf * ls.pdf;
                                         const spec f_cos = f * absdot(sh.ns, ls.wi);
-                                        if (Lt.type == BB_LIGHT_SPOT || Lt.type == BB_LIGHT_POINT) { // delta lights
+                                        if (Lt.type == BB_LIGHT_SPOT || Lt.type == BB_LIGHT_POINT ||
+                                            Lt.type == BB_LIGHT_DIRECTIONAL) { // delta lights
                                             sh_c = ((beta * ls.Li) * f_cos) / light_pdf;
                                         } else {
                                             const float mis = power_heuristic(light_pdf, bsdf_pdf);

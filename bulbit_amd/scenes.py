@@ -267,6 +267,13 @@ class SceneData:
         L.spectrum[:] = list(intensity)
         self.lights.append(L)
 
+    def add_directional_light(self, direction, intensity):
+        L = capi.BbLight()
+        L.type = capi.BB_LIGHT_DIRECTIONAL
+        L.direction[:] = [float(x) for x in direction]
+        L.spectrum[:] = list(intensity)
+        self.lights.append(L)
+
     def add_infinite_light(self, l, scale=1.0):
         L = capi.BbLight()
         L.type = capi.BB_LIGHT_UNIFORM_INFINITE

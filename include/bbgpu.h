@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define BBGPU_ABI_VERSION 13
+#define BBGPU_ABI_VERSION 14
 
 /* status codes */
 enum {
@@ -130,6 +130,7 @@ enum {
     BB_LIGHT_UNIFORM_INFINITE = 1,  /* UniformInfiniteLight src/light/uniform_infinite_light.cpp:7-12 */
     BB_LIGHT_DIFFUSE_AREA = 2,      /* DiffuseAreaLight     src/light/diffuse_area_light.cpp:9-15 */
     BB_LIGHT_POINT = 3,             /* PointLight           src/light/point_light.cpp:7-13 */
+    BB_LIGHT_DIRECTIONAL = 4,       /* DirectionalLight     src/light/directional_light.cpp:7-17 */
 };
 
 /* Constructor arguments of the light types, in Scene::GetLights() order
@@ -140,8 +141,8 @@ typedef struct BbLight {
     int32_t two_sided;          /* DIFFUSE_AREA */
     int32_t reserved;
     float position[3];          /* SPOT, POINT */
-    float direction[3];         /* SPOT: the constructor's direction argument */
-    float spectrum[3];          /* SPOT, POINT intensity / INFINITE l / AREA emission */
+    float direction[3];         /* SPOT, DIRECTIONAL: the constructor's direction argument */
+    float spectrum[3];          /* SPOT, POINT, DIRECTIONAL intensity / INFINITE l / AREA emission */
     float scale;                /* INFINITE scale */
     float angle_max;            /* SPOT, degrees */
     float angle_falloff_start;  /* SPOT, degrees */

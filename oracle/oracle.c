@@ -2522,6 +2522,8 @@ static spec light_phi(const OrcScene* s, int32_t li)
         return smulf(smulf(smulf(smulf(c, L->scale), orc_four_pi), orc_pi), sqrf(x->world_radius));
     case BB_LIGHT_POINT: /* point_light.cpp:93-96 */
         return smulf(c, orc_four_pi);
+    case BB_LIGHT_DIRECTIONAL: /* directional_light.cpp:92-95 */
+        return smulf(smulf(c, orc_pi), sqrf(x->world_radius));
     default: {
         int32_t tri = s->prims[L->primitive].shape_index;
         return smulf(smulf(smulf(c, triangle_area(s, tri)), orc_pi), (float)(L->two_sided ? 2 : 1));
@@ -2554,6 +2556,13 @@ static int light_sample_li(const OrcScene* s, int32_t li, v3 ref, v2 u, light_sa
         ls->visibility = distance;
         ls->pdf = 1.0f;
         ls->Li = sdivf(l, distance * distance);
+        return 1;
+    }
+    if (L->type == BB_LIGHT_DIRECTIONAL) { /* directional_light.cpp:29-38 */
+        ls->wi = x->z;
+        ls->visibility = 2.0f * x->world_radius;
+        ls->pdf = 1.0f;
+        ls->Li = c;
         return 1;
     }
     if (L->type == BB_LIGHT_POINT) { /* point_light.cpp:30-42 */
@@ -2797,6 +2806,13 @@ OrcScene* orc_scene_create(const BbSceneDesc* d, char* err, int32_t err_size)
             if (L->primitive < 0 || L->primitive >= s->n_prims) FAIL("light %d: bad primitive", i);
             if (s->prims[L->primitive].shape_kind != BB_SHAPE_TRIANGLE) FAIL("light %d: area lights must be triangles", i);
             s->prim_light[L->primitive] = i;
+        } else if (L->type == BB_LIGHT_DIRECTIONAL) {
+            /* directional_light.cpp:7-17: w = Normalize(direction), the world's bounding sphere */
+            x->z = vnormalize(V3(L->direction[0], L->direction[1], L->direction[2]));
+            v3 c = vmul(vadd(s->world_min, s->world_max), 0.5f);
+            int inside = !(s->world_min.x > c.x || s->world_max.x < c.x) && !(s->world_min.y > c.y || s->world_max.y < c.y) &&
+                         !(s->world_min.z > c.z || s->world_max.z < c.z);
+            x->world_radius = inside ? vlen(vsub(s->world_max, c)) : 0.0f;
         } else if (L->type != BB_LIGHT_POINT) {
             FAIL("light %d: unknown type", i);
         }
@@ -2853,7 +2869,8 @@ static spec sample_direct_light(const OrcScene* s, v3 wo, const isect_t* is, con
     if (bvh_raycast(s, is->point, ls.wi, 1e-4f, ls.visibility, 1, NULL, NULL)) return S1(0.0f);
     float light_pdf = pmf * ls.pdf;
     spec f_cos = smulf(bsdf_f(bsdf, wo, ls.wi), vabsdot(is->ns, ls.wi));
-    if (s->lights[li].type == BB_LIGHT_SPOT || s->lights[li].type == BB_LIGHT_POINT) /* delta lights */
+    if (s->lights[li].type == BB_LIGHT_SPOT || s->lights[li].type == BB_LIGHT_POINT ||
+        s->lights[li].type == BB_LIGHT_DIRECTIONAL) /* delta lights */
         return sdivf(smul(smul(beta, ls.Li), f_cos), light_pdf);
     float mis = power_heuristic(light_pdf, bsdf_pdf_v);
     return sdivf(smul(smul(smulf(beta, mis), ls.Li), f_cos), light_pdf);

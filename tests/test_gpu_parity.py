@@ -226,7 +226,7 @@ def test_render_mirror_parity():
 
 
 def test_render_point_lights_parity():
-    """PointLight (delta, Li = I / d^2) beside the README scene's lights:
+    """PointLight (delta, Li = I / d^2) and DirectionalLight beside the README scene's lights:
     within 1e-3 of the oracle, bit-equal with its libm correctly rounded."""
     from bulbit_amd.gpu import GpuScene
     from oracle import OracleScene, correctly_rounded_libm
@@ -234,6 +234,7 @@ def test_render_point_lights_parity():
     sc, cam = scenes.readme_scene(320, 180)
     sc.add_point_light((1.5, 2.0, 1.0), (3.0, 2.5, 2.0))
     sc.add_point_light((-2.0, 0.6, 0.5), (0.5, 1.0, 2.0))
+    sc.add_directional_light((0.3, 1.0, 0.4), (0.6, 0.5, 0.4))
     g, o = GpuScene(sc), OracleScene(sc)
     p = scenes.render_params(16)
     gi, oi, gc, oc = _render_pair(g, o, cam, p)

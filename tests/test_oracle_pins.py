@@ -555,6 +555,26 @@ def test_mirror_reflects_with_weight_r(orc):
         assert np.all(np.asarray(fe) == 0) and pe == 0
 
 
+def test_directional_light_closed_form():
+    """DirectionalLight (directional_light.cpp:29-38, a delta light): light
+    arriving along w = Normalize(direction) with intensity E on a Lambertian
+    plane gives rho / pi * E * cos(theta); its power is E pi R^2 with R the
+    world's bounding-sphere radius."""
+    from oracle import OracleScene
+
+    sc = scenes.SceneData()
+    m = sc.add_material(scenes.diffuse((0.6, 0.3, 0.15)))
+    q = np.array([[-4, 0, -4], [4, 0, -4], [4, 0, 4], [-4, 0, 4]], np.float32)
+    sc.add_mesh(q, [[0, 1, 0]] * 4, [[1, 0, 0]] * 4, [[0, 2, 1], [0, 3, 2]], m)
+    E = np.array([2.0, 1.0, 0.5])
+    sc.add_directional_light((0.0, 2.0, 2.0 * np.sqrt(3.0)), E)  # 60 degrees from the normal
+    cam = scenes.make_camera((0.0, 1.0, 0.0), (0.0, 0.0, 0.0), 0.05, 0.0, 1.0, 4, 4, up=(0.0, 0.0, -1.0))
+    s, c, _ = OracleScene(sc).render(cam, scenes.render_params(16, max_bounces=1), threads=1)
+    img = s / c[..., None]
+    want = np.array([0.6, 0.3, 0.15]) / np.pi * E * 0.5
+    assert np.allclose(img.reshape(-1, 3), want, rtol=1e-4), (img.reshape(-1, 3)[0], want)
+
+
 def test_point_light_direct_lighting_closed_form():
     """A point light of intensity I at height h over a Lambertian plane of
     albedo rho, direct light only (max_bounces 1): straight below the light

@@ -133,6 +133,11 @@ XML = """<?xml version="1.0" encoding="utf-8"?>
         <rgb name="intensity" value="2 1.5 1"/>
         <transform name="to_world"><translate x="0.25"/></transform>
     </emitter>
+    <emitter type="directional">
+        <vector name="direction" value="0, 0, 1"/>
+        <rgb name="irradiance" value="0.5 0.4 0.3"/>
+        <transform name="to_world"><rotate x="1" angle="-60"/></transform>
+    </emitter>
     <emitter type="constant"><rgb name="radiance" value="0.1"/></emitter>
 </scene>
 """
@@ -305,7 +310,7 @@ def test_shapes_are_placed_by_their_transforms(desc):
 
 def test_lights(desc):
     L = desc["lights"]
-    assert [list(x)[0] for x in L] == ["area", "area", "spot", "point", "infinite"]
+    assert [list(x)[0] for x in L] == ["area", "area", "spot", "point", "directional", "infinite"]
     assert [x["area"]["primitive"] for x in L[:2]] == [30, 31]
     assert all(x["area"]["emission"]["constant"] == [10, 9, 8] and not x["area"]["two_sided"] for x in L[:2])
     spot = L[2]["spot"]
@@ -315,7 +320,10 @@ def test_lights(desc):
     assert spot["angle_max"] == 30 and spot["angle_falloff_start"] == 15 and spot["intensity"] == [4, 4, 4]
     # the point's position moved by to_world
     assert L[3]["point"] == {"position": [0.75, 1.5, 1], "intensity": [2, 1.5, 1]}
-    assert L[4]["infinite"] == {"l": pytest.approx([0.1, 0.1, 0.1]), "scale": 1}
+    # the direction rotated by to_world's rotation
+    assert np.allclose(L[4]["directional"]["direction"], _rot((1, 0, 0), -60) @ [0, 0, 1], atol=1e-6)
+    assert L[4]["directional"]["intensity"] == pytest.approx([0.5, 0.4, 0.3])
+    assert L[5]["infinite"] == {"l": pytest.approx([0.1, 0.1, 0.1]), "scale": 1}
 
 
 def test_transform_math_against_rotation_matrices(tmp_path):
@@ -360,7 +368,6 @@ def test_filters_and_defaults(tmp_path):
 
 
 @pytest.mark.parametrize("xml, what", [
-    ('<emitter type="directional"/>', "directional emitter"),
     ('<emitter type="envmap"><string name="filename" value="sky.hdr"/></emitter>', "envmap emitter"),
     ('<bsdf type="principled" id="p"/>', "principled BSDF"),
     ('<bsdf type="plastic" id="p"/>', "plastic BSDF"),
@@ -481,6 +488,8 @@ def scene_from_description(d):
         (kind, v), = L.items()
         if kind == "area":
             sc.add_area_light(v["primitive"], v["emission"]["constant"], v["two_sided"])
+        elif kind == "directional":
+            sc.add_directional_light(v["direction"], v["intensity"])
         elif kind == "point":
             sc.add_point_light(v["position"], v["intensity"])
         elif kind == "spot":
