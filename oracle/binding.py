@@ -24,12 +24,20 @@ def build_oracle(force: bool = False) -> str:
 
 
 def load_oracle():
+    """The committed oracle, liboracle.so (built with the Makefile when missing)."""
     global _lib
     if _lib is not None:
         return _lib
     if not os.path.exists(LIB):
         build_oracle()
-    lib = ctypes.CDLL(LIB)
+    _lib = load_oracle_from(LIB)
+    return _lib
+
+
+def load_oracle_from(path):
+    """An oracle library at `path` with its entry points typed (tests load
+    builds of oracle.c with a planted mistake this way)."""
+    lib = ctypes.CDLL(path)
     lib.orc_murmur64a.restype = c_uint64
     lib.orc_murmur64a.argtypes = [c_void_p, c_int32, c_uint64]
     lib.orc_pixel_sample_hash.restype = c_uint64
@@ -81,7 +89,9 @@ def load_oracle():
                                c_int32, c_int32, c_void_p, c_void_p, c_int32]
     lib.orc_sample_li.argtypes = [c_void_p, POINTER(capi.BbCameraDesc), POINTER(capi.BbRenderParams), c_int32, c_int32,
                                   c_int32, POINTER(c_float), POINTER(c_int32)]
-    _lib = lib
+    lib.orc_light_sample.argtypes = [c_void_p, c_int32, POINTER(c_float), c_int32, c_void_p, c_void_p, c_void_p,
+                                     c_void_p, c_void_p, c_void_p]
+    lib.orc_light_hit_pdf.argtypes = [c_void_p, c_int32, POINTER(c_float), c_int32, c_void_p, c_void_p]
     return lib
 
 
@@ -100,8 +110,8 @@ def correctly_rounded_libm(on=True):
 class OracleScene:
     """A scene built by the oracle (its own BVH and light set-up)."""
 
-    def __init__(self, scene_data):
-        self.lib = load_oracle()
+    def __init__(self, scene_data, lib=None):
+        self.lib = lib if lib is not None else load_oracle()
         self.data = scene_data
         desc = scene_data.desc()
         err = ctypes.create_string_buffer(512)
@@ -174,6 +184,26 @@ class OracleScene:
         secs = self.lib.orc_render(self.h, ctypes.byref(camera), ctypes.byref(params), x0, y0, x1, y1,
                                    s.ctypes.data, c.ctypes.data, threads or os.cpu_count() or 1)
         return s, c, secs
+
+    def light_sample(self, light, ref, u):
+        """Light::Sample_Li of light `light` at the point ref for the sample
+        pairs u[n, 2] -> dict of wi[n, 3], Li[n, 3], pdf[n], visibility[n], ok[n]"""
+        u = np.ascontiguousarray(u, np.float32).reshape(-1, 2)
+        n = len(u)
+        out = {"wi": np.zeros((n, 3), np.float32), "Li": np.zeros((n, 3), np.float32), "pdf": np.zeros(n, np.float32),
+               "visibility": np.zeros(n, np.float32), "ok": np.zeros(n, np.int32)}
+        self.lib.orc_light_sample(self.h, int(light), (c_float * 3)(*ref), n, u.ctypes.data, out["wi"].ctypes.data,
+                                  out["Li"].ctypes.data, out["pdf"].ctypes.data, out["visibility"].ctypes.data,
+                                  out["ok"].ctypes.data)
+        return out
+
+    def light_hit_pdf(self, light, origin, d):
+        """Triangle::PDF(isect, ray) for rays from origin along d[n, 3] whose
+        closest hit is area light `light`'s triangle, 0 for the others"""
+        d = np.ascontiguousarray(d, np.float32).reshape(-1, 3)
+        pdf = np.zeros(len(d), np.float32)
+        self.lib.orc_light_hit_pdf(self.h, int(light), (c_float * 3)(*origin), len(d), d.ctypes.data, pdf.ctypes.data)
+        return pdf
 
     def sample_li(self, camera, params, px, py, sample):
         L = (c_float * 3)()

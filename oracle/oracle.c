@@ -2651,6 +2651,38 @@ static float triangle_pdf_solid_angle(const OrcScene* s, int32_t tri, const isec
     return distance_squared / (cosine * triangle_area(s, tri));
 }
 
+void orc_light_sample(const OrcScene* s, int32_t li, const float ref[3], int32_t n, const float* u, float* wi,
+                      float* Li, float* pdf, float* visibility, int32_t* ok)
+{
+    for (int32_t i = 0; i < n; ++i) {
+        light_sample ls;
+        memset(&ls, 0, sizeof(ls));
+        ok[i] = light_sample_li(s, li, V3(ref[0], ref[1], ref[2]), V2(u[2 * i], u[2 * i + 1]), &ls);
+        wi[3 * i] = ls.wi.x;
+        wi[3 * i + 1] = ls.wi.y;
+        wi[3 * i + 2] = ls.wi.z;
+        Li[3 * i] = ls.Li.r;
+        Li[3 * i + 1] = ls.Li.g;
+        Li[3 * i + 2] = ls.Li.b;
+        pdf[i] = ls.pdf;
+        visibility[i] = ls.visibility;
+    }
+}
+
+void orc_light_hit_pdf(const OrcScene* s, int32_t li, const float o[3], int32_t n, const float* d, float* pdf)
+{
+    for (int32_t i = 0; i < n; ++i) {
+        v3 ro = V3(o[0], o[1], o[2]), rd = V3(d[3 * i], d[3 * i + 1], d[3 * i + 2]);
+        isect_t is;
+        pdf[i] = 0.0f;
+        if (!bvh_raycast(s, ro, rd, 1e-4f, INFINITY, 0, &is, NULL)) continue;
+        finish_isect(s, is.ro, is.rd, &is);
+        if (s->prim_light[is.prim] != li) continue;
+        /* path_li's emitter-hit density before the light PMF */
+        pdf[i] = triangle_pdf_solid_angle(s, s->prims[is.prim].shape_index, &is, rd);
+    }
+}
+
 /* ================================================================ scenes */
 
 static double now_s(void)

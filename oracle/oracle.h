@@ -95,6 +95,18 @@ void orc_trace_brute(const OrcScene* scene, const BbRay* rays, int32_t n, BbHit*
 double orc_render(const OrcScene* scene, const BbCameraDesc* cam, const BbRenderParams* params, int32_t x0, int32_t y0,
                   int32_t x1, int32_t y1, float* film_sum, int32_t* film_count, int32_t threads);
 
+/* Light::Sample_Li of light li (Scene light order) at the point ref for n
+ * sample pairs u[2n] (spot_light.cpp:46-68, diffuse_area_light.cpp:42-72 with
+ * Triangle::Sample, triangle.cpp:159-229, ...): wi[3n], Li[3n], pdf[n],
+ * visibility[n] and ok[n] (the boolean the reference returns). */
+void orc_light_sample(const OrcScene* scene, int32_t li, const float ref[3], int32_t n, const float* u, float* wi,
+                      float* Li, float* pdf, float* visibility, int32_t* ok);
+/* For n rays from o along d[3n]: Triangle::PDF(isect, ray) (triangle.cpp:
+ * 247-253), the solid-angle density PathIntegrator::Li weighs an emitter hit
+ * with (path_integrator.cpp:67-84) before the light PMF, when the ray's
+ * closest hit is area light li's triangle; 0 otherwise. */
+void orc_light_hit_pdf(const OrcScene* scene, int32_t li, const float o[3], int32_t n, const float* d, float* pdf);
+
 /* One path sample, returning L (NaN/Inf kept) and the number of draws used. */
 void orc_sample_li(const OrcScene* scene, const BbCameraDesc* cam, const BbRenderParams* params, int32_t px, int32_t py,
                    int32_t sample_index, float L_out[3], int32_t* draws_out);
