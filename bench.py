@@ -221,7 +221,7 @@ def quality(g, sc, cam, args, test):
         w, h = 96, 54
         x0, y0 = cam.width // 2 - w // 2, cam.height // 2 - h // 2
         rect = (x0, y0, x0 + w, y0 + h)
-        threads = max(1, min(16, len(os.sched_getaffinity(0))))
+        threads = host_cores()["threads"]
         o = OracleScene(sc)
         s2, c2, secs = o.render(cam, p, rect=rect, threads=threads)
         cpu_ref = (s2 / np.maximum(c2, 1)[..., None])[y0:y0 + h, x0:x0 + w]
@@ -551,12 +551,31 @@ def copy_calibration(mib):
     log(f"[bench] copied 3 x {mib} MiB")
 
 
+def host_cores():
+    """The cores this process may use: its CPU affinity, lowered to the
+    cgroup's CPU quota (cpu.max) when one is set.  Every one of the oracle's
+    threads renders tiles (none only polls, unlike the reference's pool of
+    n - 1 workers), so `threads` = that core count occupies every core."""
+    affinity = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()[:2]
+        if q != "max":
+            quota = max(1, int(float(q) / float(period) + 0.5))
+    except (OSError, ValueError):
+        pass
+    threads = max(1, min(affinity, quota) if quota else affinity)
+    return {"threads": threads, "os_cpu_count": os.cpu_count(), "affinity": affinity, "cgroup_quota": quota}
+
+
 def cpu_baseline(sc, cam, params, args, samples_full):
     """The oracle on the host cores: full frames of the same scene, fewer spp."""
     from oracle import OracleScene
     from bulbit_amd import scenes
 
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    hc = host_cores()
+    threads = hc["threads"]
     t0 = time.time()
     o = OracleScene(sc)
     log(f"[bench] oracle scene built in {time.time() - t0:.2f}s; calibrating the CPU sample on {threads} threads")
@@ -573,9 +592,14 @@ def cpu_baseline(sc, cam, params, args, samples_full):
         "value": round(value, 4),
         "unit": "Msamples/s",
         "cores": threads,
+        "os_cpu_count": hc["os_cpu_count"],
+        "affinity": hc["affinity"],
+        "cgroup_quota": hc["cgroup_quota"],
         "kind": "port",
         "sample": f"oracle (CPU restatement of PathIntegrator) rendering the same {cam.width}x{cam.height} frame at "
-                  f"{spp} spp (samples 1..{spp}), {n} samples in {secs:.1f}s on {threads} threads; BVH build excluded",
+                  f"{spp} spp (samples 1..{spp}), {n} samples in {secs:.1f}s on {threads} threads (every core the process may use: "
+                  f"affinity {hc['affinity']}, cgroup quota {hc['cgroup_quota']}, os.cpu_count {hc['os_cpu_count']}); "
+                  f"BVH build excluded",
     }
 
 

@@ -20,7 +20,7 @@ LIB = os.path.join(HERE, "libbbgpu.so")
 ARCH = os.environ.get("BBGPU_ARCH", "gfx950")
 
 SOURCES = ["host.cpp", "tables.cpp", "kernels.hip", "nrc.hip"]
-HEADERS = ["bb_math.h", "bb_scene.h", "bb_shade.h", os.path.join("..", "..", "include", "bbgpu.h")]
+HEADERS = ["bb_math.h", "bb_scene.h", "bb_shade.h", "bb_texcoord.h", os.path.join("..", "..", "include", "bbgpu.h")]
 
 FLAGS = [
     f"--offload-arch={ARCH}",

@@ -19,6 +19,8 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <new>
+#include <stdexcept>
 #include <vector>
 
 #include <zlib.h>
@@ -652,6 +654,10 @@ bool read_png(std::FILE* f, bool non_color, Decoded& out)
         ph[p] = interlace ? (h > uint32_t(ay0[p]) ? (h - ay0[p] + ady[p] - 1) / ady[p] : 0) : h;
         if (pw[p] && ph[p]) raw_size += size_t(ph[p]) * (1 + (size_t(pw[p]) * bits_pp + 7) / 8);
     }
+    // deflate expands at most 1032:1, so IHDR sizes that the IDAT data cannot
+    // fill are rejected before anything is allocated for them; zlib counts
+    // its output in a 32-bit uInt
+    if (raw_size == 0 || raw_size / 1032 > idat.size() + 64 || raw_size > size_t(UINT32_MAX)) return false;
     std::vector<unsigned char> raw(raw_size);
     z_stream zs;
     std::memset(&zs, 0, sizeof zs);
@@ -758,14 +764,22 @@ bool decode(const std::string& filename, bool non_color, Decoded& out)
     if (!f) return false;
     bool ok = false;
     const std::string ext = filename.size() >= 4 ? filename.substr(filename.size() - 4) : "";
-    if (ext == ".hdr")
-        ok = read_hdr(f, out);
-    else if (ext == ".pfm")
-        ok = read_pfm(f, out);
-    else if (ext == ".ppm" || ext == ".pgm" || ext == ".pnm")
-        ok = read_pnm(f, non_color, out);
-    else if (ext == ".png" || ext == ".PNG")
-        ok = read_png(f, non_color, out);
+    // a header whose sizes cannot be allocated fails the read (stb_image
+    // returns NULL for it) instead of throwing out of ReadImage1/ReadImage3
+    try {
+        if (ext == ".hdr")
+            ok = read_hdr(f, out);
+        else if (ext == ".pfm")
+            ok = read_pfm(f, out);
+        else if (ext == ".ppm" || ext == ".pgm" || ext == ".pnm")
+            ok = read_pnm(f, non_color, out);
+        else if (ext == ".png" || ext == ".PNG")
+            ok = read_png(f, non_color, out);
+    } catch (const std::bad_alloc&) {
+        ok = false;
+    } catch (const std::length_error&) {
+        ok = false;
+    }
     std::fclose(f);
     return ok;
 }

@@ -17,6 +17,7 @@
 #pragma once
 
 #include "bb_scene.h"
+#include "bb_texcoord.h"
 
 namespace bb
 {
@@ -243,21 +244,14 @@ BB_D bxdf make_bxdf(const GpuMaterial& m, bool front_face, const SceneView& S)
 
 // ------------------------------------------------------------- textures
 
-// FilterTexCoord (include/bulbit/texture.h:26-46); repeat maps -k to
-// width - (k % width) - 1, as the reference writes it
-BB_D int32_t filter_coord(int32_t i, int32_t n, int32_t filter)
-{
-    if (filter == BB_TEXCOORD_CLAMP) return min(max(i, 0), n - 1);
-    return i >= 0 ? i % n : n - ((-i) % n) - 1;
-}
-
-// ImageTexture::Evaluate, bilinear (include/bulbit/textures.h:60-88): channel c of the image
+// ImageTexture::Evaluate, bilinear (include/bulbit/textures.h:60-88): channel c of the image;
+// filter_coord and tex_floor_index (bb_texcoord.h) keep every tap inside the image
 BB_D void image_eval(const GpuTexture& t, const float* texels, f2 uv, float* out)
 {
     const float w = uv.x * float(t.width) - 0.5f;
     const float h = uv.y * float(t.height) - 0.5f;
-    int32_t i0 = int32_t(floorf(w)), i1 = i0 + 1;
-    int32_t j0 = int32_t(floorf(h)), j1 = j0 + 1;
+    int32_t i0 = tex_floor_index(w, t.width, t.filter), i1 = i0 + 1;
+    int32_t j0 = tex_floor_index(h, t.height, t.filter), j1 = j0 + 1;
     i0 = filter_coord(i0, t.width, t.filter);
     j0 = filter_coord(j0, t.height, t.filter);
     i1 = filter_coord(i1, t.width, t.filter);

@@ -197,3 +197,18 @@ def test_corrupt_png_is_rejected(tmp_path):
     data = bytearray((tmp_path / "a.png").read_bytes())
     (tmp_path / "b.png").write_bytes(bytes(data[:-30]))  # IDAT cut short, no IEND
     assert convert(tmp_path, tmp_path / "b.png")[0] == 1
+
+
+@pytest.mark.parametrize("size", [(1 << 24, 1 << 24), (1 << 31, 3), (70000, 70000)])
+def test_oversized_header_is_rejected_without_allocating(tmp_path, size):
+    """An IHDR far larger than its IDAT can inflate to: the read fails (as
+    stb_image returns NULL) instead of allocating the header's size."""
+    w, h = size
+
+    def chunk(t, d):
+        return struct.pack(">I", len(d)) + t + d + struct.pack(">I", zlib.crc32(t + d) & 0xFFFFFFFF)
+
+    data = (b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, 8, 6, 0, 0, 0)) +
+            chunk(b"IDAT", zlib.compress(bytes(64), 9)) + chunk(b"IEND", b""))
+    (tmp_path / "big.png").write_bytes(data)
+    assert convert(tmp_path, tmp_path / "big.png")[0] == 1
