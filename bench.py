@@ -490,18 +490,12 @@ def batch_workload(args):
     from bulbit_amd import scenes
     from bulbit_amd.gpu import GpuScene, autotune, render_batch
 
-    rng = np.random.Generator(np.random.PCG64(303))
-    res = [(256, 256), (512, 512), (1024, 576), (1024, 1024)]
     jobs, tuned = [], {}
     t0 = time.time()
-    for i in range(32):
-        tris = float(np.exp(rng.uniform(np.log(1e4), np.log(2e6))))
-        level = int(np.clip(np.round(np.log(tris / 20.0) / np.log(4.0)), 5, 8))
-        w, h = res[int(rng.integers(0, 4))]
-        n_lights = int(rng.integers(4, 257))
-        sc, cam = scenes.icosphere_scene(width=w, height=h, level=level, seed=1000 + i, n_lights=n_lights)
+    for i, spec in enumerate(scenes.batch_workload_specs(32, seed=303)):
+        sc, cam, p = scenes.batch_workload_job(spec)
+        w, h, level, n_lights = spec["width"], spec["height"], spec["level"], spec["n_lights"]
         g = GpuScene(sc)
-        p = scenes.render_params(16, max_bounces=int(rng.integers(4, 13)), seed=i)
         if (w, h) not in tuned:
             tuned[(w, h)] = autotune(g, cam, p)
         t = tuned[(w, h)]

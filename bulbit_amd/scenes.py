@@ -522,6 +522,36 @@ def icosphere_scene(width=1920, height=1080, level=8, seed=2026, n_lights=64):
     return sc, cam
 
 
+BATCH_RESOLUTIONS = [(256, 256), (512, 512), (1024, 576), (1024, 1024)]
+
+
+def batch_workload_specs(n_jobs=32, seed=303):
+    """The batched workload: n_jobs independent seeded scenes of one call.
+    Each is the displaced-icosphere scene with a triangle count drawn
+    log-uniform in [1e4, 2e6] (realized as the nearest subdivision level,
+    5..8, of 20 * 4^level faces), a resolution from BATCH_RESOLUTIONS, 16 spp,
+    max_bounces uniform in 4..12 and 4 to 256 area lights.  Returns a list of
+    dicts: level, width, height, n_lights, max_bounces, scene_seed, seed."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for i in range(n_jobs):
+        tris = float(np.exp(rng.uniform(np.log(1e4), np.log(2e6))))
+        level = int(np.clip(np.round(np.log(tris / 20.0) / np.log(4.0)), 5, 8))
+        w, h = BATCH_RESOLUTIONS[int(rng.integers(0, 4))]
+        n_lights = int(rng.integers(4, 257))
+        max_bounces = int(rng.integers(4, 13))
+        out.append(dict(level=level, width=w, height=h, n_lights=n_lights, max_bounces=max_bounces,
+                        scene_seed=1000 + i, seed=i))
+    return out
+
+
+def batch_workload_job(spec, spp=16):
+    """(SceneData, camera, BbRenderParams) of one batch_workload_specs entry"""
+    sc, cam = icosphere_scene(width=spec["width"], height=spec["height"], level=spec["level"],
+                              seed=spec["scene_seed"], n_lights=spec["n_lights"])
+    return sc, cam, render_params(spp, max_bounces=spec["max_bounces"], seed=spec["seed"])
+
+
 def poisson_disk(n, half_extent, min_dist, seed, k=30):
     """n points in [-half_extent, half_extent]^2, pairwise at least min_dist
     apart: Bridson's dart throwing (annulus candidates around active points,
