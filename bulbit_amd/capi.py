@@ -12,7 +12,7 @@ from ctypes import POINTER, c_char_p, c_double, c_float, c_int32, c_size_t, c_ui
 
 import numpy as np
 
-ABI_VERSION = 14  # BBGPU_ABI_VERSION of include/bbgpu.h that these structures mirror
+ABI_VERSION = 15  # BBGPU_ABI_VERSION of include/bbgpu.h that these structures mirror
 
 LIB_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(LIB_DIR, "libbbgpu.so")
@@ -262,6 +262,7 @@ EXPORTS = [
     "bbgpu_scene_destroy",
     "bbgpu_scene_info",
     "bbgpu_trace",
+    "bbgpu_trace_noalloc",
     "bbgpu_render",
     "bbgpu_look_at",
     "bbgpu_nrc_param_count",
@@ -310,6 +311,7 @@ def load_library(path: str | None = None):
     lib.bbgpu_scene_destroy.argtypes = [c_void_p]
     lib.bbgpu_scene_info.argtypes = [c_void_p, POINTER(BbSceneInfo)]
     lib.bbgpu_trace.argtypes = [c_void_p, c_void_p, c_int32, c_int32, c_void_p, POINTER(BbTraceCounters), c_void_p]
+    lib.bbgpu_trace_noalloc.argtypes = lib.bbgpu_trace.argtypes
     lib.bbgpu_render.argtypes = [
         c_void_p,
         POINTER(BbCameraDesc),

@@ -1467,6 +1467,115 @@ ShadowKernel shadow_kernel(bool count, bool full, bool alpha)
 
 } // namespace
 
+namespace
+{
+
+// What a traversal batch launches: the kernel variant, its LDS, the number
+// of stack entries kept in LDS and the persistent grid
+struct TracePlan
+{
+    BatchKernel kfn;
+    size_t lds;
+    int lds_entries;
+    int grid;
+    size_t overflow_per_lane; // stack entries beyond LDS per lane of the grid
+};
+
+TracePlan trace_plan(const SceneView& V, int32_t n_rays, int32_t any_hit, bool count)
+{
+    TracePlan t{};
+    t.lds_entries = lds_entries_for(V, 0);
+    t.lds = trace_lds_bytes(t.lds_entries, any_hit != 0 && !count);
+    const bool full = V.stack_depth <= t.lds_entries;
+    t.kfn = batch_kernel(any_hit != 0, count, full, V.has_alpha != 0);
+    t.grid = std::min(trace_grid(reinterpret_cast<const void*>(t.kfn), t.lds), grid_for(std::max(n_rays, 1), kTraceBlock));
+    t.overflow_per_lane = size_t(std::max(0, V.stack_depth - t.lds_entries));
+    return t;
+}
+
+// The device memory a traversal batch uses beyond its rays and hits
+struct TraceScratch
+{
+    int* err;                     // 1: a traversal stack overflowed
+    int* counts;                  // [4]: the work-queue fetch counter at [0]
+    unsigned long long* counters; // [8]: nodes, prims, accepts
+    int2* gstack;                 // the stack's overflow area, plan.overflow_per_lane entries per lane
+};
+
+// Traces a batch whose rays and hits are in device memory; returns once it is done
+int trace_run(const char* who, const SceneView& V, const BbRay* rays, int32_t n_rays, int32_t any_hit, BbHit* hits,
+              BbTraceCounters* counters, hipStream_t st, const TracePlan& t, const TraceScratch& s)
+{
+    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, st, s.err, 1);
+    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, st, s.counts, 4);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, s.counters, 8);
+    if (n_rays > 0) {
+        BatchSrc src{ reinterpret_cast<const float4*>(rays), reinterpret_cast<int4*>(hits), V.tris, any_hit };
+        hipLaunchKernelGGL(t.kfn, dim3(t.grid), dim3(kTraceBlock), t.lds, st, V, src, n_rays, s.counts, s.gstack,
+                           t.lds_entries, 48, s.counters, s.err);
+        HIP_OK(hipGetLastError());
+    }
+    int err = 0;
+    unsigned long long c[8] = { 0 };
+    HIP_OK(hipMemcpyAsync(&err, s.err, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(c, s.counters, sizeof(c), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    if (err) {
+        bb::set_error("%s: traversal stack overflow", who);
+        return BBGPU_ERR_DEVICE;
+    }
+    if (counters) {
+        counters->rays = uint64_t(n_rays);
+        counters->nodes = c[0];
+        counters->prims = c[1];
+        counters->accepts = c[2];
+    }
+    return BBGPU_OK;
+}
+
+// Scratch of bbgpu_trace_noalloc, one copy per device (it lives as long as
+// the process): a 256-byte header (counters at 0, the error flag at 64, the
+// work-queue counts at 128), then the traversal stacks' overflow area.  Host
+// code never passes `scratch` itself to a kernel (its host-side name is not a
+// device address); it uses the address hipGetSymbolAddress returns for the
+// current device.
+constexpr size_t kNoAllocScratchBytes = size_t(16) << 20;
+constexpr size_t kNoAllocHeaderBytes = 256;
+
+} // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace
+{
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_lock;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// the device address of `scratch` on the current device (looked up once per device)
+unsigned char* scratch_of_current_device()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> g(g_scratch_lock);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = static_cast<unsigned char*>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+
+} // namespace
+
 extern "C" {
 
 int bbgpu_trace(BbScene* scene, const BbRay* rays_dev, int32_t n_rays, int32_t any_hit, BbHit* hits_dev,
@@ -1479,13 +1588,8 @@ int bbgpu_trace(BbScene* scene, const BbRay* rays_dev, int32_t n_rays, int32_t a
     hipStream_t st = (hipStream_t)stream;
     SceneExtra* x = extra_for(scene);
     const SceneView& V = scene->view;
-    const int lds_entries = lds_entries_for(V, 0);
-    const size_t lds = trace_lds_bytes(lds_entries, any_hit != 0 && counters == nullptr);
-    const bool full = V.stack_depth <= lds_entries;
-    const BatchKernel kfn = batch_kernel(any_hit != 0, counters != nullptr, full, V.has_alpha != 0);
-    const int grid = std::min(trace_grid(reinterpret_cast<const void*>(kfn), lds), grid_for(std::max(n_rays, 1), kTraceBlock));
-    if (!ensure_workspace(x->ws, 1, 16) ||
-        !ensure_gstack(x->ws, size_t(std::max(0, V.stack_depth - lds_entries)) * size_t(grid) * kTraceBlock)) {
+    const TracePlan t = trace_plan(V, n_rays, any_hit, counters != nullptr);
+    if (!ensure_workspace(x->ws, 1, 16) || !ensure_gstack(x->ws, t.overflow_per_lane * size_t(t.grid) * kTraceBlock)) {
         bb::set_error("bbgpu_trace: workspace allocation failed");
         return BBGPU_ERR_NOMEM;
     }
@@ -1508,32 +1612,48 @@ int bbgpu_trace(BbScene* scene, const BbRay* rays_dev, int32_t n_rays, int32_t a
         }
         if (host_hits) hits = reinterpret_cast<BbHit*>(stage + rb);
     }
-    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, st, P.err, 1);
-    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, st, P.counts, 4);
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, P.counters, 8);
-    if (n_rays > 0) {
-        BatchSrc src{ reinterpret_cast<const float4*>(rays), reinterpret_cast<int4*>(hits), V.tris, any_hit };
-        hipLaunchKernelGGL(kfn, dim3(grid), dim3(kTraceBlock), lds, st, V, src, n_rays, P.counts, P.gstack, lds_entries,
-                           48, P.counters, P.err);
-        HIP_OK(hipGetLastError());
+    const TraceScratch s{ P.err, P.counts, P.counters, P.gstack };
+    const int r = trace_run("bbgpu_trace", V, rays, n_rays, any_hit, hits, counters, st, t, s);
+    if (r == BBGPU_OK && host_hits)
+        HIP_OK(hipMemcpy(hits_dev, hits, sizeof(BbHit) * size_t(n_rays), hipMemcpyDeviceToHost));
+    return r;
+}
+
+int bbgpu_trace_noalloc(BbScene* scene, const BbRay* rays_dev, int32_t n_rays, int32_t any_hit, BbHit* hits_dev,
+                        BbTraceCounters* counters, void* stream)
+{
+    if (!scene || n_rays < 0 || (n_rays > 0 && (!rays_dev || !hits_dev))) {
+        bb::set_error("bbgpu_trace_noalloc: invalid argument");
+        return BBGPU_ERR_INVALID;
     }
-    if (host_hits) HIP_OK(hipMemcpyAsync(hits_dev, hits, sizeof(BbHit) * size_t(n_rays), hipMemcpyDeviceToHost, st));
-    int err = 0;
-    unsigned long long c[8] = { 0 };
-    HIP_OK(hipMemcpyAsync(&err, P.err, sizeof(int), hipMemcpyDeviceToHost, st));
-    HIP_OK(hipMemcpyAsync(c, P.counters, sizeof(c), hipMemcpyDeviceToHost, st));
-    HIP_OK(hipStreamSynchronize(st));
-    if (err) {
-        bb::set_error("bbgpu_trace: traversal stack overflow");
+    if (n_rays > 0 && (!device_accessible(rays_dev) || !device_accessible(hits_dev))) {
+        bb::set_error("bbgpu_trace_noalloc: rays and hits must be in device memory");
+        return BBGPU_ERR_INVALID;
+    }
+    unsigned char* base = scratch_of_current_device();
+    if (!base) {
+        bb::set_error("bbgpu_trace_noalloc: the scratch array has no address on the current device");
         return BBGPU_ERR_DEVICE;
     }
-    if (counters) {
-        counters->rays = uint64_t(n_rays);
-        counters->nodes = c[0];
-        counters->prims = c[1];
-        counters->accepts = c[2];
+    const SceneView& V = scene->view;
+    TracePlan t = trace_plan(V, n_rays, any_hit, counters != nullptr);
+    // a smaller persistent grid when the stacks' overflow area would not fit
+    const size_t area = kNoAllocScratchBytes - kNoAllocHeaderBytes;
+    if (t.overflow_per_lane > 0) {
+        const size_t per_block = t.overflow_per_lane * size_t(kTraceBlock) * sizeof(int2);
+        const size_t fit = area / per_block;
+        if (fit < 1) {
+            bb::set_error("bbgpu_trace_noalloc: the traversal stack (%d entries, %d in LDS) needs %zu bytes of "
+                          "scratch per block, more than the %zu bytes available",
+                          V.stack_depth, t.lds_entries, per_block, area);
+            return BBGPU_ERR_INVALID;
+        }
+        t.grid = int(std::min<size_t>(size_t(t.grid), fit));
     }
-    return BBGPU_OK;
+    const TraceScratch s{ reinterpret_cast<int*>(base + 64), reinterpret_cast<int*>(base + 128),
+                          reinterpret_cast<unsigned long long*>(base),
+                          reinterpret_cast<int2*>(base + kNoAllocHeaderBytes) };
+    return trace_run("bbgpu_trace_noalloc", V, rays_dev, n_rays, any_hit, hits_dev, counters, (hipStream_t)stream, t, s);
 }
 
 } // extern "C"

@@ -57,6 +57,18 @@ class GpuScene:
         )
         return cnt
 
+    def trace_noalloc_device(self, rays_dev, n, hits_dev, any_hit=False, count=False, stream=None):
+        """bbgpu_trace_noalloc: the same batch traced with the library's static scratch array."""
+        cnt = capi.BbTraceCounters() if count else None
+        capi.check(
+            self.lib.bbgpu_trace_noalloc(self.h, ctypes.c_void_p(rays_dev), int(n), int(any_hit),
+                                         ctypes.c_void_p(hits_dev), ctypes.byref(cnt) if cnt is not None else None,
+                                         stream),
+            "bbgpu_trace_noalloc",
+            self.lib,
+        )
+        return cnt
+
     def trace(self, rays: np.ndarray, any_hit=False, count=False):
         """rays: RAY_DTYPE array -> (hits HIT_DTYPE array, counters or None)."""
         torch = self.torch

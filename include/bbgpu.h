@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define BBGPU_ABI_VERSION 14
+#define BBGPU_ABI_VERSION 15
 
 /* status codes */
 enum {
@@ -344,6 +344,19 @@ int bbgpu_scene_info(const BbScene* scene, BbSceneInfo* info_out);
  * requests traversal counts.  The call returns when the batch is traced. */
 int bbgpu_trace(BbScene* scene, const BbRay* rays_dev, int32_t n_rays, int32_t any_hit, BbHit* hits_dev,
                 BbTraceCounters* counters, void* stream);
+
+/* bbgpu_trace without any device allocation.  It has no counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, results and counters as bbgpu_trace, but rays_dev and
+ * hits_dev must be device memory (host buffers are rejected).  The kernels'
+ * scratch (error flag, work-queue counter, traversal counters and the
+ * stacks' overflow area) is one static 16 MiB device array per GPU, held
+ * until the process ends; a scene whose stack overflow needs more than that
+ * for one block is rejected.  Every call on a GPU shares that GPU's array:
+ * calls that overlap on one GPU (other streams, other threads) race. */
+int bbgpu_trace_noalloc(BbScene* scene, const BbRay* rays_dev, int32_t n_rays, int32_t any_hit, BbHit* hits_dev,
+                        BbTraceCounters* counters, void* stream);
 
 /* Replaces UniDirectionalRayIntegrator::Render with PathIntegrator::Li
  * (src/integrator/integrator.cpp:162-210, path_integrator.cpp:26-194):
