@@ -12,7 +12,7 @@ from ctypes import POINTER, c_char_p, c_double, c_float, c_int32, c_size_t, c_ui
 
 import numpy as np
 
-ABI_VERSION = 15  # BBGPU_ABI_VERSION of include/bbgpu.h that these structures mirror
+ABI_VERSION = 16  # BBGPU_ABI_VERSION of include/bbgpu.h that these structures mirror
 
 LIB_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(LIB_DIR, "libbbgpu.so")
@@ -234,6 +234,7 @@ class BbCacheParams(ctypes.Structure):
         ("train_batch", c_int32),
         ("train_steps", c_int32),
         ("query_primary", c_int32),
+        ("query_direct", c_int32),
     ]
 
 
@@ -245,6 +246,8 @@ class BbCacheStats(ctypes.Structure):
         ("queries", c_int64),
         ("train_rows", c_int64),
         ("loss", c_float),
+        ("diverged", c_int32),
+        ("passes_redone", c_int32),
     ]
 
 
@@ -272,6 +275,7 @@ EXPORTS = [
     "bbgpu_nrc_set_weights",
     "bbgpu_nrc_query",
     "bbgpu_nrc_train",
+    "bbgpu_nrc_diverged",
     "bbgpu_render_cached",
     "bbgpu_render_batch",
     "bbgpu_autotune",

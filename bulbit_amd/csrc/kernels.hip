@@ -612,8 +612,12 @@ struct CacheBufs
     float* rows;      // [row][16] compacted training rows
     float* targets;   // [row][3]
     int32_t* counters; // [0] queries, [1] training rows
+    int32_t* diverged; // the network's divergence flag (sticky)
+    int32_t* state;    // [0] diverged when the pass began, [1] a prediction of this pass was not finite,
+                       // [2] 1 + the pass dropped for it (0: none)
+    unsigned long long* tally; // [0] queries, [1] rows trained (summed over the call's passes)
     f3 wmin, winv;    // position normalisation to [0, 1]^3
-    int32_t query_bounce, query_primary, extra, max_vertices, max_rows;
+    int32_t query_bounce, query_primary, query_nee, extra, max_vertices, max_rows, max_queries;
 };
 
 // UniDirectionalRayIntegrator::Render per (pixel, sample) start
@@ -848,6 +852,16 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView S, RenderConsts
             float eta_scale = Lv.w;
             spec beta = mks(Bv.x, Bv.y, Bv.z);
             float prev_pdf = Bv.w;
+            if (CACHE && C.query_nee && (fl & kFlagTraining)) {
+                // the radiance gathered up to and including the previous
+                // vertex's direct light: its indirect part starts here
+                const int tp = pid / rc.train_stride;
+                const int32_t nv = C.tcount[tp];
+                if (nv > 0) {
+                    const size_t rec = size_t(tp) * C.max_vertices + nv - 1;
+                    if (C.tsnap[2 * rec].w == 0.0f) C.tsnap[2 * rec] = make_float4(L.r, L.g, L.b, 1.0f);
+                }
+            }
             // a restarted alpha hit lies on the restarted ray: replay the
             // chain (deterministic) from the original ray to recover it
             const int32_t slot = hh.x < 0 ? hh.x : (hh.x & ~kHitRestart);
@@ -908,6 +922,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView S, RenderConsts
                     }
                 }
                 bool stop_here = false;
+                bool nee_only = false; // a query vertex that takes its direct light before the cache
                 if (CACHE) {
                     // the radiance cache replaces the rest of a rendering path at
                     // its query vertex; a training path continues `extra` vertices
@@ -917,9 +932,9 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView S, RenderConsts
                     // query vertex: the first rough (non-specular) vertex, or
                     // vertex query_bounce at the latest
                     const bool query_here =
-                        mi >= 0 && (depth >= C.query_bounce ||
-                                    ((depth >= 1 || C.query_primary) &&
-                                     (mat.flags_front & (BX_DIFFUSE | BX_GLOSSY))));
+                        mi >= 0 && C.state[0] == 0 &&
+                        (depth >= C.query_bounce ||
+                         ((depth >= 1 || C.query_primary) && (mat.flags_front & (BX_DIFFUSE | BX_GLOSSY))));
                     int32_t stop = (fl >> kStopShift) & 0xff;
                     if (training) {
                         if (query_here && stop == kStopUnknown) stop = min(depth + C.extra, 254);
@@ -932,7 +947,10 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView S, RenderConsts
                             if (nv < C.max_vertices) {
                                 const size_t rec = size_t(tp) * C.max_vertices + nv;
                                 cache_features(C, mat, sh.point, wo, sh.ns, C.traw + 16 * rec);
-                                C.tsnap[2 * rec] = make_float4(L.r, L.g, L.b, 0.0f);
+                                // with direct light at the query vertex the cache learns the
+                                // indirect part: the snapshot is taken at the next vertex
+                                C.tsnap[2 * rec] = C.query_nee ? make_float4(0.0f, 0.0f, 0.0f, 0.0f)
+                                                               : make_float4(L.r, L.g, L.b, 0.0f);
                                 C.tsnap[2 * rec + 1] = make_float4(beta.r, beta.g, beta.b, 0.0f);
                                 C.tcount[tp] = nv + 1;
                             }
@@ -941,7 +959,10 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView S, RenderConsts
                         const int q = atomicAdd(&C.counters[0], 1);
                         cache_features(C, mat, sh.point, wo, sh.ns, C.qraw + 16 * size_t(q));
                         C.qbeta[q] = make_float4(beta.r, beta.g, beta.b, __int_as_float(pid));
-                        stop_here = true;
+                        if (C.query_nee && bounce < rc.max_bounces)
+                            nee_only = true;
+                        else
+                            stop_here = true;
                     }
                 }
                 if (!stop_here && bounce++ < rc.max_bounces) {
@@ -995,7 +1016,8 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView S, RenderConsts
                         u12.y = pcg_float(st);
                         const float u0 = pcg_float(st);
                         bsample s;
-                        bool ok = wo_l.z != 0.0f && (b.flags & (BX_REFLECTION | BX_TRANSMISSION)) &&
+                        // at a query vertex with direct light the cache stands in for the continuation
+                        bool ok = !nee_only && wo_l.z != 0.0f && (b.flags & (BX_REFLECTION | BX_TRANSMISSION)) &&
                                   bxdf_sample(b, wo_l, u0, u12, s);
                         if (ok) ok = !(is_black(s.f) || s.pdf == 0.0f || s.wi.z == 0.0f);
                         if (ok) {
@@ -1041,20 +1063,52 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView S, RenderConsts
     }
 }
 
-// cached rendering: L += throughput x max(cache, 0) for every query
-__global__ __launch_bounds__(kShadeBlock) void k_cache_apply(CacheBufs C, float4* __restrict__ L)
+// the cache state at the start of a pass: queries only while the network has not diverged
+__global__ void k_cache_pass_begin(CacheBufs C)
 {
-    const int n = C.counters[0];
+    if (threadIdx.x == 0) {
+        C.state[0] = *C.diverged;
+        C.state[1] = 0;
+    }
+}
+
+// cached rendering: L += throughput x max(cache, 0) for every query.  A
+// prediction that is not finite marks the network diverged and the pass
+// (`pass`, counted from 0) for dropping: its query paths lost their tails.
+__global__ __launch_bounds__(kShadeBlock) void k_cache_apply(CacheBufs C, float4* __restrict__ L, int pass)
+{
+    const int n = min(C.counters[0], C.max_queries);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const float4 b = C.qbeta[i];
         const int pid = __float_as_int(b.w);
         const float* y = C.qpred + 3 * size_t(i);
+        const float y0 = y[0], y1 = y[1], y2 = y[2];
+        if (!(isfinite(y0) && isfinite(y1) && isfinite(y2))) {
+            atomicOr(&C.state[1], 1);
+            atomicOr(C.diverged, 1);
+            atomicCAS(&C.state[2], 0, pass + 1);
+            continue;
+        }
         float4 l = L[pid];
-        l.x = l.x + b.x * fmaxf(y[0], 0.0f);
-        l.y = l.y + b.y * fmaxf(y[1], 0.0f);
-        l.z = l.z + b.z * fmaxf(y[2], 0.0f);
+        l.x = l.x + b.x * fmaxf(y0, 0.0f);
+        l.y = l.y + b.y * fmaxf(y1, 0.0f);
+        l.z = l.z + b.z * fmaxf(y2, 0.0f);
         L[pid] = l;
     }
+}
+
+// counts of the pass for the call's statistics: its queries and the rows its
+// `steps` optimizer steps of `batch` rows trained on
+__global__ void k_cache_tally(CacheBufs C, int steps, int batch)
+{
+    if (threadIdx.x != 0) return;
+    C.tally[0] += (unsigned long long)min(max(C.counters[0], 0), C.max_queries);
+    const int rows = min(max(C.counters[1], 0), C.max_rows);
+    if (rows == 0 || batch <= 0) return;
+    const int nb = (rows + batch - 1) / batch;
+    unsigned long long t = 0;
+    for (int s = 0; s < steps; ++s) t += (unsigned long long)min(batch, rows - (s % nb) * batch);
+    C.tally[1] += t;
 }
 
 // training rows of the pass: at each recorded vertex of a training path the
@@ -1065,13 +1119,16 @@ __global__ __launch_bounds__(kShadeBlock) void k_cache_targets(CacheBufs C, cons
                                                                 int stride)
 {
     const int tp = blockIdx.x * blockDim.x + threadIdx.x;
-    if (tp >= n_train) return;
+    if (tp >= n_train || C.state[0]) return; // no training once the network diverged
     const float4 lf = L[tp * stride];
     const int nv = C.tcount[tp];
     for (int v = 0; v < nv; ++v) {
         const size_t rec = size_t(tp) * C.max_vertices + v;
-        const float4 ls = C.tsnap[2 * rec], bs = C.tsnap[2 * rec + 1];
+        float4 ls = C.tsnap[2 * rec];
+        const float4 bs = C.tsnap[2 * rec + 1];
         if (!(bs.x > 1e-6f && bs.y > 1e-6f && bs.z > 1e-6f)) continue;
+        // indirect-only targets: a vertex the path did not continue from saw none
+        if (C.query_nee && ls.w == 0.0f) ls = lf;
         const float tr = (lf.x - ls.x) / bs.x, tg = (lf.y - ls.y) / bs.y, tb = (lf.z - ls.z) / bs.z;
         if (!(isfinite(tr) && isfinite(tg) && isfinite(tb))) continue;
         const int r = atomicAdd(&C.counters[1], 1);
@@ -1085,11 +1142,12 @@ __global__ __launch_bounds__(kShadeBlock) void k_cache_targets(CacheBufs C, cons
 
 // Film::AddSample in sample order (film.cpp:26-37); NaN/Inf samples dropped
 // (integrator.cpp:193-196).  Path i of a pass is pixel i % npix, sample i / npix.
+// skip (cached renders): non-zero drops the pass (it is rendered again without the cache)
 __global__ __launch_bounds__(kShadeBlock) void k_film(RenderConsts rc, const float4* __restrict__ L, float* film_sum,
-                                                       int32_t* film_count)
+                                                       int32_t* film_count, const int32_t* skip)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= rc.npix) return;
+    if (p >= rc.npix || (skip && *skip)) return;
     float sr = film_sum[3 * p], sg = film_sum[3 * p + 1], sb = film_sum[3 * p + 2];
     int32_t cnt = film_count[p];
     for (int pid = p; pid < rc.n_paths; pid += rc.npix) {
@@ -1658,6 +1716,17 @@ int bbgpu_trace_noalloc(BbScene* scene, const BbRay* rays_dev, int32_t n_rays, i
 
 } // extern "C"
 
+namespace bb
+{
+int nrc_query_counted(BbNrc* x, const float* inputs_dev, const int* count_dev, int cap, float* out_dev, hipStream_t st);
+int nrc_train_counted(BbNrc* x, const float* rows_dev, const float* targets_dev, const int* count_dev, int cap,
+                      int batch, int step, hipStream_t st);
+int* nrc_status(BbNrc* x);
+bool nrc_diverged(const BbNrc* x);
+void nrc_note_diverged(BbNrc* x, bool d);
+float* nrc_loss(BbNrc* x);
+} // namespace bb
+
 namespace
 {
 
@@ -1688,7 +1757,8 @@ bool ensure_cache(SceneExtra* x, size_t cap, int stride, int max_vertices)
     const bool ok = get((void**)&C.qraw, cap * 64) && get((void**)&C.qbeta, cap * 16) && get((void**)&C.qpred, cap * 12) &&
                     get((void**)&C.traw, rows * 64) && get((void**)&C.tsnap, rows * 32) &&
                     get((void**)&C.tcount, ntp * 4) && get((void**)&C.rows, rows * 64) &&
-                    get((void**)&C.targets, rows * 12) && get((void**)&C.counters, 16);
+                    get((void**)&C.targets, rows * 12) && get((void**)&C.counters, 16) &&
+                    get((void**)&C.state, 16) && get((void**)&C.tally, 16);
     if (!ok) {
         for (void* q : w.allocs) (void)hipFree(q);
         w.allocs.clear();
@@ -1699,6 +1769,7 @@ bool ensure_cache(SceneExtra* x, size_t cap, int stride, int max_vertices)
     w.max_vertices = max_vertices;
     C.max_vertices = max_vertices;
     C.max_rows = int32_t(rows);
+    C.max_queries = int32_t(cap);
     return true;
 }
 
@@ -1829,8 +1900,12 @@ int render_impl(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams
         C->winv = mk3(1.0f / std::max(ext.x, 1e-6f), 1.0f / std::max(ext.y, 1e-6f), 1.0f / std::max(ext.z, 1e-6f));
         C->query_bounce = std::max(1, cache->cp.query_bounce);
         C->query_primary = cache->cp.query_primary ? 1 : 0;
+        C->query_nee = cache->cp.query_direct ? 1 : 0;
         C->extra = std::max(0, cache->cp.train_extra_bounces);
+        C->diverged = nrc_status(cache->nrc);
         if (cache->stats) memset(cache->stats, 0, sizeof(*cache->stats));
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, C->tally, 2);
+        hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, st, C->state, 4);
     }
     CacheBufs Cv{};
     if (C) Cv = *C;
@@ -1839,9 +1914,10 @@ int render_impl(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams
     struct Span
     {
         hipEvent_t a, b;
-        int kind; // 0 closest, 1 shadow, 2 shade
+        int kind; // 0 closest, 1 shadow, 2 shade, 3 cache query, 4 cache training
     };
     std::vector<Span> spans;
+    const bool cache_timed = cache && cache->stats;
     auto mark = [&]() { hipEvent_t e = pool.get(); hipEventRecord(e, st); return e; };
 
     hipEvent_t t_begin = nullptr;
@@ -1885,6 +1961,7 @@ int render_impl(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams
     }
     const int nslots = 4 * (p.max_bounces + 2);
     int64_t paths_total = 0;
+    int pass = 0;
     const bool replay = graph != nullptr; // a captured graph stands in for the launches below
     if (!replay) {
         hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, ls, P.err, 1);
@@ -1908,6 +1985,7 @@ int render_impl(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams
         hipLaunchKernelGGL(k_zero_ints, dim3((nslots + 255) / 256), dim3(256), 0, ls, P.counts, nslots);
         const int n_train = cache ? int((n_paths + stride - 1) / stride) : 0;
         if (cache) {
+            hipLaunchKernelGGL(k_cache_pass_begin, dim3(1), dim3(64), 0, ls, Cv);
             hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, ls, Cv.counters, 4);
             hipLaunchKernelGGL(k_zero_ints, dim3((n_train + 255) / 256), dim3(256), 0, ls, Cv.tcount, n_train);
         }
@@ -1932,51 +2010,32 @@ int render_impl(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams
             if (timed) spans.push_back({ c2, mark(), 1 });
         }
         if (cache) {
-            // cache queries of the rendering paths, then this pass's training rows
-            int cnt[2] = { 0, 0 };
-            hipEvent_t q0 = mark();
-            HIP_OK(hipMemcpyAsync(cnt, Cv.counters, sizeof(cnt), hipMemcpyDeviceToHost, st));
-            HIP_OK(hipStreamSynchronize(st));
-            if (cnt[0] > 0) {
-                const int rq = bbgpu_nrc_query(cache->nrc, Cv.qraw, cnt[0], Cv.qpred, st);
-                if (rq != BBGPU_OK) return rq;
-                hipLaunchKernelGGL(k_cache_apply, dim3(grid_for(cnt[0], kShadeBlock)), dim3(kShadeBlock), 0, ls, Cv, P.L);
-            }
+            // cache queries of the rendering paths, then this pass's training
+            // rows; every count stays on the device
+            hipEvent_t q0 = cache_timed ? mark() : nullptr;
+            const int rq = nrc_query_counted(cache->nrc, Cv.qraw, Cv.counters, Cv.max_queries, Cv.qpred, st);
+            if (rq != BBGPU_OK) return rq;
+            hipLaunchKernelGGL(k_cache_apply, dim3(std::min(grid_for(n_paths, kShadeBlock), 4096)), dim3(kShadeBlock), 0,
+                               st, Cv, P.L, pass);
             hipLaunchKernelGGL(k_cache_targets, dim3((n_train + kShadeBlock - 1) / kShadeBlock), dim3(kShadeBlock), 0,
                                st, Cv, P.L, n_train, stride);
-            hipEvent_t q1 = mark();
-            HIP_OK(hipMemcpyAsync(cnt, Cv.counters, sizeof(cnt), hipMemcpyDeviceToHost, st));
-            HIP_OK(hipStreamSynchronize(st));
-            const int rows = std::min(cnt[1], Cv.max_rows);
+            hipEvent_t q1 = cache_timed ? mark() : nullptr;
             const int batch = cache->cp.train_batch > 0 ? cache->cp.train_batch : 65536;
-            float loss = 0.0f;
-            int64_t trained = 0;
-            // steps take the rows batch by batch and start over when they run out
-            const int n_batches = (rows + batch - 1) / batch;
-            for (int s1 = 0; rows > 0 && s1 < cache->cp.train_steps; ++s1) {
-                const size_t off = size_t(s1 % n_batches) * size_t(batch);
-                const int b = std::min(batch, rows - int(off));
-                const int rt = bbgpu_nrc_train(cache->nrc, Cv.rows + 16 * off, Cv.targets + 3 * off, b, 1,
-                                               cache->stats ? &loss : nullptr, nullptr, st);
+            for (int s1 = 0; s1 < cache->cp.train_steps; ++s1) {
+                const int rt = nrc_train_counted(cache->nrc, Cv.rows, Cv.targets, Cv.counters + 1, Cv.max_rows, batch,
+                                                 s1, st);
                 if (rt != BBGPU_OK) return rt;
-                trained += b;
             }
-            hipEvent_t q2 = mark();
-            if (cache->stats) {
-                HIP_OK(hipStreamSynchronize(st));
-                float ms_q = 0.0f, ms_t = 0.0f;
-                hipEventElapsedTime(&ms_q, q0, q1);
-                hipEventElapsedTime(&ms_t, q1, q2);
-                cache->stats->ms_query += ms_q;
-                cache->stats->ms_train += ms_t;
-                cache->stats->queries += cnt[0];
-                cache->stats->train_rows += trained;
-                cache->stats->loss = loss;
+            hipLaunchKernelGGL(k_cache_tally, dim3(1), dim3(64), 0, st, Cv, cache->cp.train_steps, batch);
+            if (cache_timed) {
+                spans.push_back({ q0, q1, 3 });
+                spans.push_back({ q1, mark(), 4 });
             }
         }
         a = timed ? mark() : nullptr;
         hipLaunchKernelGGL(k_film, dim3((npix + kShadeBlock - 1) / kShadeBlock), dim3(kShadeBlock), 0, ls, rc, P.L,
-                           film_sum_dev, film_count_dev);
+                           film_sum_dev, film_count_dev, cache ? Cv.state + 1 : nullptr);
+        ++pass;
         if (timed) spans.push_back({ a, mark(), 2 });
         if (!capturing) HIP_OK(hipGetLastError());
         if (timed) {
@@ -2008,6 +2067,43 @@ int render_impl(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams
         graph = &x->graphs.back();
     }
     if (graph) HIP_OK(hipGraphLaunch(graph->exec, st));
+    if (cache) {
+        // one synchronization per call: the pass dropped for non-finite
+        // predictions (rendered again without the cache, into the same film),
+        // the divergence flag and the statistics
+        int stv[4] = { 0, 0, 0, 0 };
+        int flag = 0;
+        unsigned long long tally[2] = { 0, 0 };
+        float loss = 0.0f;
+        HIP_OK(hipMemcpyAsync(stv, Cv.state, sizeof(stv), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipMemcpyAsync(&flag, Cv.diverged, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipMemcpyAsync(tally, Cv.tally, sizeof(tally), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipMemcpyAsync(&loss, nrc_loss(cache->nrc), sizeof(float), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        nrc_note_diverged(cache->nrc, flag != 0);
+        if (stv[2] > 0) {
+            const long long s0 = (long long)(stv[2] - 1) * per_pass;
+            BbRenderParams rp = p;
+            rp.sample_begin = p.sample_begin + int32_t(s0);
+            rp.spp = int32_t(std::min<long long>(per_pass, p.spp - s0));
+            rp.graph = 0;
+            const int rr = render_impl(scene, camera, &rp, film_sum_dev, film_count_dev, stream, nullptr, 0, nullptr);
+            if (rr != BBGPU_OK) return rr;
+        }
+        if (cache->stats) {
+            cache->stats->queries = int64_t(tally[0]);
+            cache->stats->train_rows = int64_t(tally[1]);
+            cache->stats->loss = loss;
+            cache->stats->diverged = flag != 0 ? 1 : 0;
+            cache->stats->passes_redone = stv[2] > 0 ? 1 : 0;
+            for (const Span& sp : spans) {
+                float d = 0.0f;
+                hipEventElapsedTime(&d, sp.a, sp.b);
+                if (sp.kind == 3) cache->stats->ms_query += d;
+                if (sp.kind == 4) cache->stats->ms_train += d;
+            }
+        }
+    }
     hipEvent_t t_end = (timed || (cache && cache->stats)) ? mark() : nullptr;
     if (host_film) {
         HIP_OK(hipMemcpyAsync(film_sum_user, film_sum_dev, sum_bytes, hipMemcpyDeviceToHost, st));
@@ -2038,7 +2134,7 @@ int render_impl(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams
             } else if (sp.kind == 1) {
                 stats->ms_trace_shadow += d;
                 stats->launches_shadow++;
-            } else {
+            } else if (sp.kind == 2) {
                 stats->ms_shade += d;
             }
         }
@@ -2252,6 +2348,14 @@ int bbgpu_render_cached(BbScene* scene, BbNrc* nrc, const BbCameraDesc* camera, 
     if (c.query_bounce < 1 || c.train_stride < 1 || c.train_extra_bounces < 0 || c.train_batch < 0 || c.train_steps < 0) {
         bb::set_error("bbgpu_render_cached: invalid cache parameters");
         return BBGPU_ERR_INVALID;
+    }
+    if (nrc_diverged(nrc)) {
+        // the plain tracer is the fallback of a diverged cache
+        if (stats) {
+            memset(stats, 0, sizeof(*stats));
+            stats->diverged = 1;
+        }
+        return render_impl(scene, camera, params, film_sum_dev, film_count_dev, stream, nullptr, 0, nullptr);
     }
     CacheCtx ctx{ nrc, c, stats };
     return render_impl(scene, camera, params, film_sum_dev, film_count_dev, stream, nullptr, 0, &ctx);

@@ -82,6 +82,44 @@ struct Net
 using NetC = Net<64, 4>;  // the C4 cache
 using NetL = Net<128, 5>; // the wide cache
 
+// The rows a kernel works on.  total == nullptr: rows [0, cap).  Otherwise
+// min(*total, cap) rows sit in the buffer (a count the device wrote) and,
+// with batch > 0, one optimizer step takes batch number `step` of them,
+// starting over when they run out: rows [off, off + len) with
+// off = (step mod ceil(rows / batch)) batch.
+struct RowSpan
+{
+    const int* total;
+    int cap;
+    int batch;
+    int step;
+};
+
+BB_D void span_rows(const RowSpan& s, int& off, int& len)
+{
+    off = 0;
+    len = s.cap;
+    if (!s.total) return;
+    const int rows = min(max(*s.total, 0), s.cap);
+    len = rows;
+    if (s.batch <= 0 || rows == 0) return;
+    const int nb = (rows + s.batch - 1) / s.batch;
+    off = (s.step % nb) * s.batch;
+    len = min(s.batch, rows - off);
+}
+
+// most row ranges of the dW GEMM (partial sums held)
+constexpr int kMaxSplits = 512;
+
+// row ranges for the dW GEMM: about 2048 waves in all, at least four chunks each
+BB_HD int dw_splits(int chunks, int groups)
+{
+    const int want = (2048 + groups - 1) / groups;
+    const int a = want < chunks / 4 ? want : chunks / 4;
+    const int b = a < kMaxSplits ? a : kMaxSplits;
+    return b > 1 ? b : 1;
+}
+
 typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 
@@ -162,10 +200,13 @@ struct NrcWeights
 // The block's four waves take four consecutive chunks per iteration; when
 // the weights are staged layer by layer every wave joins every barrier.
 template <class NT, bool SAVE>
-__global__ __launch_bounds__(kNrcBlock) void k_mlp_forward(const float* __restrict__ raw, int n, NrcWeights Wt,
+__global__ __launch_bounds__(kNrcBlock) void k_mlp_forward(const float* __restrict__ raw, RowSpan span, NrcWeights Wt,
                                                             float* __restrict__ out, uint16_t* __restrict__ acts)
 {
     constexpr int W = NT::W, L = NT::L, LD = NT::ld;
+    int off = 0, n = 0;
+    span_rows(span, off, n);
+    raw += size_t(off) * kRawStride;
     __shared__ __align__(16) uint16_t sW[NT::resident ? NT::params : NT::max_layer];
     __shared__ __align__(16) uint16_t sAct[kNrcBlock / 64][kRows * LD];
     static_assert(NT::params % 8 == 0 && NT::max_layer % 8 == 0, "weights staged 16 bytes at a time");
@@ -316,11 +357,15 @@ BB_D void dz_layer(int l, const uint16_t* hc, const uint16_t* Wk, uint16_t* dzl,
 // (Net::dz_off).  One wave per chunk; the loss is summed into loss_sum.
 template <class NT>
 __global__ __launch_bounds__(kNrcBlock) void k_mlp_dz(const float* __restrict__ y, const float* __restrict__ target,
-                                                      int n, const uint16_t* __restrict__ Wk,
+                                                      RowSpan span, const uint16_t* __restrict__ Wk,
                                                       const uint16_t* __restrict__ acts, uint16_t* __restrict__ dzs,
                                                       float* __restrict__ loss_sum)
 {
     constexpr int W = NT::W, L = NT::L;
+    int off = 0, n = 0;
+    span_rows(span, off, n);
+    if (n == 0) return;
+    target += size_t(off) * 3;
     __shared__ __align__(16) uint16_t sD[kNrcBlock / 64][kRows * W]; // dz of the current layer output, row-major
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int r16 = lane & 15, q = lane >> 4;
@@ -402,11 +447,15 @@ BB_D void dw_tile(int l, int mt0, int nt0, int c0, int c1, const uint16_t* acts,
 // partials[s][params]
 template <class NT>
 __global__ __launch_bounds__(64) void k_mlp_dw(const uint16_t* __restrict__ acts, const uint16_t* __restrict__ dzs,
-                                               int chunks, int splits, float* __restrict__ partials)
+                                               RowSpan span, float* __restrict__ partials)
 {
     constexpr int W = NT::W, L = NT::L, B = W / 64; // 64-wide blocks per hidden dimension
+    int off = 0, n = 0;
+    span_rows(span, off, n);
+    const int chunks = (n + kRows - 1) / kRows;
+    const int splits = dw_splits(chunks, NT::groups);
     const int g = blockIdx.x % NT::groups, s = blockIdx.x / NT::groups;
-    if (s >= splits) return;
+    if (n == 0 || s >= splits) return;
     const int c0 = int(int64_t(chunks) * s / splits), c1 = int(int64_t(chunks) * (s + 1) / splits);
     const int lane = threadIdx.x;
     float* out = partials + size_t(s) * NT::params;
@@ -428,11 +477,17 @@ __global__ __launch_bounds__(64) void k_mlp_dw(const uint16_t* __restrict__ acts
 template <class NT>
 __global__ __launch_bounds__(256) void k_reduce_adam(float* __restrict__ w, float* __restrict__ m,
                                                      float* __restrict__ v, float* __restrict__ g,
-                                                     const float* __restrict__ partials, int splits, float lr,
-                                                     float c1, float c2, int apply, uint16_t* __restrict__ WT,
-                                                     uint16_t* __restrict__ W)
+                                                     const float* __restrict__ partials, RowSpan span, int reduce,
+                                                     float lr, float c1, float c2, int apply,
+                                                     uint16_t* __restrict__ WT, uint16_t* __restrict__ W,
+                                                     const float* __restrict__ loss, int* __restrict__ diverged)
 {
     __shared__ float part[4][64];
+    int off = 0, n = 0;
+    span_rows(span, off, n);
+    // no rows: nothing to reduce and no step
+    const int splits = (reduce && n > 0) ? dw_splits((n + kRows - 1) / kRows, NT::groups) : 0;
+    if (reduce && n == 0) apply = 0;
     const int t = threadIdx.x, wv = t >> 6;
     const int i = blockIdx.x * 64 + (t & 63);
     float s = 0.0f;
@@ -458,6 +513,8 @@ __global__ __launch_bounds__(256) void k_reduce_adam(float* __restrict__ w, floa
         wi -= lr * (mi / c1) / (sqrtf(vi / c2) + 1e-8f);
         w[i] = wi;
     }
+    // a diverged network (non-finite loss or weight) is flagged for good
+    if (!isfinite(wi) || (apply && !isfinite(*loss))) *diverged = 1;
     const uint16_t h = to_bf16(wi);
     W[i] = h;
     WT[NT::off(l) + nn * K + k] = h;
@@ -487,12 +544,14 @@ struct BbNrc
     float* g = nullptr;
     float* partials = nullptr; // [row range][params]
     float* loss = nullptr;
+    int* status = nullptr; // [0]: diverged (non-finite loss, weight or prediction), sticky until new weights
     uint16_t* WT = nullptr;
     uint16_t* W = nullptr;
     uint16_t* acts = nullptr; // saved layer inputs of the last training batch
     uint16_t* dzs = nullptr;  // saved layer-output gradients of the last training batch
     float* y = nullptr;
     size_t acts_rows = 0;
+    bool diverged = false; // host copy of status[0] as of the last synchronizing call
 };
 
 namespace
@@ -505,22 +564,13 @@ bool nrc_ok(hipError_t e, const char* what)
     return false;
 }
 
-int nrc_grid(int n)
+// blocks of a forward or gradient-chain launch over at most cap rows
+int nrc_grid(int cap)
 {
-    const int waves = (n + bb::kRows - 1) / bb::kRows;
+    const int waves = (cap + bb::kRows - 1) / bb::kRows;
     return std::max(1, std::min((waves + 3) / 4, 512));
 }
 static_assert(bb::kNrcBlock / 64 == 4, "nrc_grid assumes four waves per block");
-
-// most row ranges of the dW GEMM (partial sums held)
-constexpr int kMaxSplits = 512;
-
-// row ranges for the dW GEMM: about 2048 waves in all, at least four chunks each
-int dw_splits(int chunks, int groups)
-{
-    const int want = (2048 + groups - 1) / groups;
-    return std::max(1, std::min({ want, chunks / 4, kMaxSplits }));
-}
 
 int params_of(int width, int hidden)
 {
@@ -529,63 +579,127 @@ int params_of(int width, int hidden)
     return -1;
 }
 
+// the Adam step (apply), or only the bf16 copies (reduce = 0: no gradient)
 template <class NT>
-bool nrc_sync_bf16_t(BbNrc* x, hipStream_t st, bool apply, float c1, float c2, int splits)
+bool nrc_reduce_t(BbNrc* x, hipStream_t st, bb::RowSpan span, bool reduce, bool apply, float c1, float c2)
 {
     hipLaunchKernelGGL(bb::k_reduce_adam<NT>, dim3((NT::params + 63) / 64), dim3(256), 0, st, x->w, x->m, x->v, x->g,
-                       x->partials, splits, x->lr, c1, c2, apply ? 1 : 0, x->WT, x->W);
+                       x->partials, span, reduce ? 1 : 0, x->lr, c1, c2, apply ? 1 : 0, x->WT, x->W, x->loss,
+                       x->status);
     return nrc_ok(hipGetLastError(), "k_reduce_adam");
 }
 
-bool nrc_sync_bf16(BbNrc* x, hipStream_t st, bool apply, float c1, float c2, int splits = 0)
+bool nrc_reduce(BbNrc* x, hipStream_t st, bb::RowSpan span, bool reduce, bool apply, float c1, float c2)
 {
-    return x->width == bb::NetC::W ? nrc_sync_bf16_t<bb::NetC>(x, st, apply, c1, c2, splits)
-                                   : nrc_sync_bf16_t<bb::NetL>(x, st, apply, c1, c2, splits);
+    return x->width == bb::NetC::W ? nrc_reduce_t<bb::NetC>(x, st, span, reduce, apply, c1, c2)
+                                   : nrc_reduce_t<bb::NetL>(x, st, span, reduce, apply, c1, c2);
 }
 
+// refreshes the bf16 copies from the fp32 master weights
+bool nrc_refresh(BbNrc* x, hipStream_t st) { return nrc_reduce(x, st, bb::RowSpan{ nullptr, 0, 0, 0 }, false, false, 1.0f, 1.0f); }
+
 template <class NT>
-void launch_query(BbNrc* x, const float* inputs_dev, int n, float* out_dev, hipStream_t st)
+void launch_query(BbNrc* x, const float* inputs_dev, bb::RowSpan span, float* out_dev, hipStream_t st)
 {
     const bb::NrcWeights Wt{ x->WT, x->W };
-    hipLaunchKernelGGL((bb::k_mlp_forward<NT, false>), dim3(nrc_grid(n)), dim3(bb::kNrcBlock), 0, st, inputs_dev, n,
-                       Wt, out_dev, nullptr);
+    hipLaunchKernelGGL((bb::k_mlp_forward<NT, false>), dim3(nrc_grid(span.cap)), dim3(bb::kNrcBlock), 0, st, inputs_dev,
+                       span, Wt, out_dev, nullptr);
 }
 
-// forward, gradient chain and dW partial sums of one step; *splits = the
-// number of row ranges for the reduction
-template <class NT>
-int train_step(BbNrc* x, const float* inputs_dev, const float* targets_dev, int n, hipStream_t st, int* splits_out)
+int query_span(BbNrc* x, const float* inputs_dev, bb::RowSpan span, float* out_dev, hipStream_t st)
 {
-    const size_t chunks = (size_t(n) + bb::kRows - 1) / bb::kRows;
-    if (x->acts_rows < size_t(n)) {
-        for (void* p : { (void*)x->acts, (void*)x->dzs, (void*)x->y })
-            if (p) (void)hipFree(p);
-        x->acts = nullptr;
-        x->dzs = nullptr;
-        x->y = nullptr;
-        x->acts_rows = 0;
-        if (!nrc_ok(hipMalloc(&x->acts, chunks * size_t(NT::chunk_acts) * 2), "hipMalloc") ||
-            !nrc_ok(hipMalloc(&x->dzs, chunks * size_t(NT::chunk_dz) * 2), "hipMalloc") ||
-            !nrc_ok(hipMalloc(&x->y, size_t(n) * 12), "hipMalloc"))
-            return BBGPU_ERR_NOMEM;
-        x->acts_rows = size_t(n);
-    }
+    if (x->width == bb::NetC::W)
+        launch_query<bb::NetC>(x, inputs_dev, span, out_dev, st);
+    else
+        launch_query<bb::NetL>(x, inputs_dev, span, out_dev, st);
+    return nrc_ok(hipGetLastError(), "k_mlp_forward") ? BBGPU_OK : BBGPU_ERR_DEVICE;
+}
+
+// the saved activations, gradients and outputs of a step on up to rows rows
+template <class NT>
+bool ensure_train_buffers(BbNrc* x, int rows)
+{
+    if (x->acts_rows >= size_t(rows)) return true;
+    const size_t chunks = (size_t(rows) + bb::kRows - 1) / bb::kRows;
+    for (void* p : { (void*)x->acts, (void*)x->dzs, (void*)x->y })
+        if (p) (void)hipFree(p);
+    x->acts = nullptr;
+    x->dzs = nullptr;
+    x->y = nullptr;
+    x->acts_rows = 0;
+    if (!nrc_ok(hipMalloc(&x->acts, chunks * size_t(NT::chunk_acts) * 2), "hipMalloc") ||
+        !nrc_ok(hipMalloc(&x->dzs, chunks * size_t(NT::chunk_dz) * 2), "hipMalloc") ||
+        !nrc_ok(hipMalloc(&x->y, chunks * bb::kRows * 12), "hipMalloc"))
+        return false;
+    x->acts_rows = chunks * bb::kRows;
+    return true;
+}
+
+// forward, gradient chain, dW partial sums and the reduction (with the Adam
+// step when apply) of one step over the rows span selects; at most
+// max_len rows (which sizes the launches)
+template <class NT>
+int train_step(BbNrc* x, const float* inputs_dev, const float* targets_dev, bb::RowSpan span, int max_len, bool apply,
+               hipStream_t st)
+{
+    if (!ensure_train_buffers<NT>(x, max_len)) return BBGPU_ERR_NOMEM;
+    const int chunks = (max_len + bb::kRows - 1) / bb::kRows;
     const bb::NrcWeights Wt{ x->WT, x->W };
-    const int grid = nrc_grid(n);
-    const int splits = dw_splits(int(chunks), NT::groups);
+    const int grid = nrc_grid(max_len);
     hipLaunchKernelGGL(bb::k_fill0, dim3(1), dim3(64), 0, st, x->loss, 1);
-    hipLaunchKernelGGL((bb::k_mlp_forward<NT, true>), dim3(grid), dim3(bb::kNrcBlock), 0, st, inputs_dev, n, Wt, x->y,
+    hipLaunchKernelGGL((bb::k_mlp_forward<NT, true>), dim3(grid), dim3(bb::kNrcBlock), 0, st, inputs_dev, span, Wt, x->y,
                        x->acts);
-    hipLaunchKernelGGL(bb::k_mlp_dz<NT>, dim3(grid), dim3(bb::kNrcBlock), 0, st, x->y, targets_dev, n, x->W, x->acts,
+    hipLaunchKernelGGL(bb::k_mlp_dz<NT>, dim3(grid), dim3(bb::kNrcBlock), 0, st, x->y, targets_dev, span, x->W, x->acts,
                        x->dzs, x->loss);
-    hipLaunchKernelGGL(bb::k_mlp_dw<NT>, dim3(NT::groups * splits), dim3(64), 0, st, x->acts, x->dzs, int(chunks),
-                       splits, x->partials);
+    hipLaunchKernelGGL(bb::k_mlp_dw<NT>, dim3(NT::groups * bb::dw_splits(chunks, NT::groups)), dim3(64), 0, st, x->acts,
+                       x->dzs, span, x->partials);
     if (!nrc_ok(hipGetLastError(), "k_mlp_dw")) return BBGPU_ERR_DEVICE;
-    *splits_out = splits;
+    // Adam bias corrections of the step about to be taken (every launched
+    // step counts, also one that finds no rows and leaves the weights alone)
+    float c1 = 1.0f, c2 = 1.0f;
+    if (apply) {
+        x->step++;
+        c1 = 1.0f - std::pow(0.9f, float(x->step));
+        c2 = 1.0f - std::pow(0.99f, float(x->step));
+    }
+    if (!nrc_reduce(x, st, span, true, apply, c1, c2)) return BBGPU_ERR_DEVICE;
     return BBGPU_OK;
 }
 
+int train_span(BbNrc* x, const float* inputs_dev, const float* targets_dev, bb::RowSpan span, int max_len, bool apply,
+               hipStream_t st)
+{
+    return x->width == bb::NetC::W ? train_step<bb::NetC>(x, inputs_dev, targets_dev, span, max_len, apply, st)
+                                   : train_step<bb::NetL>(x, inputs_dev, targets_dev, span, max_len, apply, st);
+}
+
 } // namespace
+
+// Entry points for the cached renderer, whose row counts stay on the device
+namespace bb
+{
+
+// predictions of min(*count, cap) rows
+int nrc_query_counted(BbNrc* x, const float* inputs_dev, const int* count_dev, int cap, float* out_dev, hipStream_t st)
+{
+    return query_span(x, inputs_dev, RowSpan{ count_dev, cap, 0, 0 }, out_dev, st);
+}
+
+// Adam step number `step` of a pass over min(*count, cap) rows in batches of `batch`
+int nrc_train_counted(BbNrc* x, const float* rows_dev, const float* targets_dev, const int* count_dev, int cap,
+                      int batch, int step, hipStream_t st)
+{
+    const int max_len = std::max(1, std::min(batch, cap));
+    return train_span(x, rows_dev, targets_dev, RowSpan{ count_dev, cap, batch, step }, max_len, true, st);
+}
+
+// device flag: the network diverged (set by the step, or by whoever applies its predictions)
+int* nrc_status(BbNrc* x) { return x->status; }
+bool nrc_diverged(const BbNrc* x) { return x->diverged; }
+void nrc_note_diverged(BbNrc* x, bool d) { x->diverged = d; }
+float* nrc_loss(BbNrc* x) { return x->loss; }
+
+} // namespace bb
 
 extern "C" {
 
@@ -624,9 +738,9 @@ int bbgpu_nrc_create(const BbNrcDesc* desc, BbNrc** out)
     const size_t PS = size_t(P);
     bool ok = nrc_ok(hipMalloc(&x->w, PS * 4), "hipMalloc") && nrc_ok(hipMalloc(&x->m, PS * 4), "hipMalloc") &&
               nrc_ok(hipMalloc(&x->v, PS * 4), "hipMalloc") && nrc_ok(hipMalloc(&x->g, PS * 4), "hipMalloc") &&
-              nrc_ok(hipMalloc(&x->loss, 4), "hipMalloc") && nrc_ok(hipMalloc(&x->WT, PS * 2), "hipMalloc") &&
-              nrc_ok(hipMalloc(&x->W, PS * 2), "hipMalloc") &&
-              nrc_ok(hipMalloc(&x->partials, size_t(kMaxSplits) * PS * 4), "hipMalloc");
+              nrc_ok(hipMalloc(&x->loss, 4), "hipMalloc") && nrc_ok(hipMalloc(&x->status, 16), "hipMalloc") &&
+              nrc_ok(hipMalloc(&x->WT, PS * 2), "hipMalloc") && nrc_ok(hipMalloc(&x->W, PS * 2), "hipMalloc") &&
+              nrc_ok(hipMalloc(&x->partials, size_t(bb::kMaxSplits) * PS * 4), "hipMalloc");
     if (!ok) {
         bbgpu_nrc_destroy(x);
         return BBGPU_ERR_NOMEM;
@@ -650,7 +764,8 @@ int bbgpu_nrc_create(const BbNrcDesc* desc, BbNrc** out)
     }
     ok = nrc_ok(hipMemcpy(x->w, w.data(), PS * 4, hipMemcpyHostToDevice), "hipMemcpy") &&
          nrc_ok(hipMemset(x->m, 0, PS * 4), "hipMemset") && nrc_ok(hipMemset(x->v, 0, PS * 4), "hipMemset") &&
-         nrc_ok(hipMemset(x->g, 0, PS * 4), "hipMemset") && nrc_sync_bf16(x, nullptr, false, 1.0f, 1.0f) &&
+         nrc_ok(hipMemset(x->g, 0, PS * 4), "hipMemset") && nrc_ok(hipMemset(x->status, 0, 16), "hipMemset") &&
+         nrc_ok(hipMemset(x->loss, 0, 4), "hipMemset") && nrc_refresh(x, nullptr) &&
          nrc_ok(hipDeviceSynchronize(), "hipDeviceSynchronize");
     if (!ok) {
         bbgpu_nrc_destroy(x);
@@ -664,7 +779,7 @@ int bbgpu_nrc_destroy(BbNrc* x)
 {
     if (!x) return BBGPU_OK;
     for (void* p : { (void*)x->w, (void*)x->m, (void*)x->v, (void*)x->g, (void*)x->partials, (void*)x->loss,
-                     (void*)x->WT, (void*)x->W, (void*)x->acts, (void*)x->dzs, (void*)x->y })
+                     (void*)x->status, (void*)x->WT, (void*)x->W, (void*)x->acts, (void*)x->dzs, (void*)x->y })
         if (p) (void)hipFree(p);
     delete x;
     return BBGPU_OK;
@@ -686,9 +801,26 @@ int bbgpu_nrc_set_weights(BbNrc* x, const float* in, int32_t count)
         bb::set_error("bbgpu_nrc_set_weights: invalid argument (count must be %d)", x ? x->params : 0);
         return BBGPU_ERR_INVALID;
     }
+    // new weights clear the divergence flag (non-finite ones set it again)
+    int flag = 0;
     if (!nrc_ok(hipMemcpy(x->w, in, size_t(count) * 4, hipMemcpyHostToDevice), "hipMemcpy") ||
-        !nrc_sync_bf16(x, nullptr, false, 1.0f, 1.0f) || !nrc_ok(hipDeviceSynchronize(), "hipDeviceSynchronize"))
+        !nrc_ok(hipMemset(x->status, 0, 16), "hipMemset") || !nrc_refresh(x, nullptr) ||
+        !nrc_ok(hipMemcpy(&flag, x->status, sizeof(int), hipMemcpyDeviceToHost), "hipMemcpy"))
         return BBGPU_ERR_DEVICE;
+    x->diverged = flag != 0;
+    return BBGPU_OK;
+}
+
+int bbgpu_nrc_diverged(BbNrc* x, int32_t* diverged_out)
+{
+    if (!x || !diverged_out) {
+        bb::set_error("bbgpu_nrc_diverged: null argument");
+        return BBGPU_ERR_INVALID;
+    }
+    int flag = 0;
+    if (!nrc_ok(hipMemcpy(&flag, x->status, sizeof(int), hipMemcpyDeviceToHost), "hipMemcpy")) return BBGPU_ERR_DEVICE;
+    x->diverged = flag != 0;
+    *diverged_out = flag != 0 ? 1 : 0;
     return BBGPU_OK;
 }
 
@@ -699,12 +831,7 @@ int bbgpu_nrc_query(BbNrc* x, const float* inputs_dev, int32_t n, float* out_dev
         return BBGPU_ERR_INVALID;
     }
     if (n == 0) return BBGPU_OK;
-    hipStream_t st = (hipStream_t)stream;
-    if (x->width == bb::NetC::W)
-        launch_query<bb::NetC>(x, inputs_dev, n, out_dev, st);
-    else
-        launch_query<bb::NetL>(x, inputs_dev, n, out_dev, st);
-    return nrc_ok(hipGetLastError(), "k_mlp_forward") ? BBGPU_OK : BBGPU_ERR_DEVICE;
+    return query_span(x, inputs_dev, bb::RowSpan{ nullptr, n, 0, 0 }, out_dev, (hipStream_t)stream);
 }
 
 int bbgpu_nrc_train(BbNrc* x, const float* inputs_dev, const float* targets_dev, int32_t n, int32_t apply,
@@ -715,18 +842,9 @@ int bbgpu_nrc_train(BbNrc* x, const float* inputs_dev, const float* targets_dev,
         return BBGPU_ERR_INVALID;
     }
     hipStream_t st = (hipStream_t)stream;
-    int splits = 0;
-    const int r = x->width == bb::NetC::W ? train_step<bb::NetC>(x, inputs_dev, targets_dev, n, st, &splits)
-                                          : train_step<bb::NetL>(x, inputs_dev, targets_dev, n, st, &splits);
-    if (r != BBGPU_OK) return r;
     // the gradient (the row ranges' partial sums), and the step when apply
-    float c1 = 1.0f, c2 = 1.0f;
-    if (apply) {
-        x->step++;
-        c1 = 1.0f - std::pow(0.9f, float(x->step));
-        c2 = 1.0f - std::pow(0.99f, float(x->step));
-    }
-    if (!nrc_sync_bf16(x, st, apply != 0, c1, c2, splits)) return BBGPU_ERR_DEVICE;
+    const int r = train_span(x, inputs_dev, targets_dev, bb::RowSpan{ nullptr, n, 0, 0 }, n, apply != 0, st);
+    if (r != BBGPU_OK) return r;
     if (loss_out && !nrc_ok(hipMemcpyAsync(loss_out, x->loss, 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync"))
         return BBGPU_ERR_DEVICE;
     if (grad_out &&

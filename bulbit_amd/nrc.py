@@ -27,8 +27,9 @@ def _declare(lib):
     lib.bbgpu_nrc_query.argtypes = [c_void_p, c_void_p, c_int32, c_void_p, c_void_p]
     lib.bbgpu_nrc_train.argtypes = [c_void_p, c_void_p, c_void_p, c_int32, c_int32, POINTER(c_float), POINTER(c_float),
                                     c_void_p]
+    lib.bbgpu_nrc_diverged.argtypes = [c_void_p, POINTER(c_int32)]
     for name in ("bbgpu_nrc_create", "bbgpu_nrc_destroy", "bbgpu_nrc_get_weights", "bbgpu_nrc_set_weights",
-                 "bbgpu_nrc_query", "bbgpu_nrc_train"):
+                 "bbgpu_nrc_query", "bbgpu_nrc_train", "bbgpu_nrc_diverged"):
         getattr(lib, name).restype = c_int32
     lib._nrc_declared = True
     return lib
@@ -70,6 +71,12 @@ class Nrc:
         w = np.ascontiguousarray(w, np.float32)
         capi.check(self.lib.bbgpu_nrc_set_weights(self.h, w.ctypes.data_as(POINTER(c_float)), self.n_params),
                    "bbgpu_nrc_set_weights", self.lib)
+
+    def diverged(self) -> bool:
+        """True once a training step or a cached render found the network diverged."""
+        d = c_int32()
+        capi.check(self.lib.bbgpu_nrc_diverged(self.h, ctypes.byref(d)), "bbgpu_nrc_diverged", self.lib)
+        return bool(d.value)
 
     def _dev(self, a, cols):
         a = np.ascontiguousarray(a, np.float32).reshape(-1, cols)

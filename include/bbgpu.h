@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define BBGPU_ABI_VERSION 15
+#define BBGPU_ABI_VERSION 16
 
 /* status codes */
 enum {
@@ -439,6 +439,11 @@ int bbgpu_nrc_query(BbNrc* nrc, const float* inputs_dev, int32_t n, float* out_d
  * bbgpu_nrc_param_count(desc) floats) make the call wait for the step. */
 int bbgpu_nrc_train(BbNrc* nrc, const float* inputs_dev, const float* targets_dev, int32_t n, int32_t apply,
                     float* loss_out, float* grad_out, void* stream);
+/* 1 in *diverged_out when the network has diverged (a training step met a
+ * non-finite loss or produced a non-finite weight, or a cached render met a
+ * non-finite prediction); new weights (bbgpu_nrc_set_weights) clear it.
+ * Waits for the device. */
+int bbgpu_nrc_diverged(BbNrc* nrc, int32_t* diverged_out);
 
 /* The cached path tracer: rendering paths end at their query vertex (the
  * first rough (diffuse or glossy) vertex, from vertex 1 on unless
@@ -456,6 +461,9 @@ typedef struct BbCacheParams {
     int32_t train_batch;          /* rows per optimizer step (0: 65536) */
     int32_t train_steps;          /* optimizer steps per pass (0: inference only) */
     int32_t query_primary;        /* 1: the primary hit may be the query vertex; 0: vertex 1 at the earliest */
+    int32_t query_direct;         /* 1: the query vertex takes its direct light (NEE) as the plain tracer does
+                                     and the cache stands in for the indirect rest only; 0: the cache stands in
+                                     for everything the vertex reflects */
 } BbCacheParams;
 
 typedef struct BbCacheStats {
@@ -465,10 +473,19 @@ typedef struct BbCacheStats {
     int64_t queries;
     int64_t train_rows; /* rows trained on, counted once per step that used them */
     float loss;        /* relative L2 loss of the last step */
+    int32_t diverged;  /* 1: the network is diverged (non-finite loss, weight or prediction); the passes from the
+                          detection on were rendered by the plain tracer, and so are later calls until new
+                          weights are set (bbgpu_nrc_set_weights) */
+    int32_t passes_redone; /* passes whose cached samples were dropped and rendered again without the cache */
 } BbCacheStats;
 
 /* bbgpu_render with the radiance cache (same film and parameter contract;
- * stats, a host pointer, may be NULL) */
+ * stats, a host pointer, may be NULL).  Row counts stay on the device: the
+ * call synchronizes its stream once, at the end.  Divergence: a non-finite
+ * loss or weight after a training step, or a non-finite prediction, marks
+ * the network diverged.  A pass whose predictions were not finite is dropped
+ * from the film and rendered again by the plain tracer; later passes, and
+ * later calls until new weights are set, are plain renders (the fallback). */
 int bbgpu_render_cached(BbScene* scene, BbNrc* nrc, const BbCameraDesc* camera, const BbRenderParams* params,
                         const BbCacheParams* cache_params, float* film_sum_dev, int32_t* film_count_dev, void* stream,
                         BbCacheStats* stats);

@@ -257,11 +257,13 @@ def test_cached_render_with_a_zero_cache_keeps_the_path_prefix():
 
 
 @pytest.mark.gpu
-def test_cached_render_beats_plain_at_equal_samples_after_training():
+@pytest.mark.parametrize("direct", [False, True], ids=["cache-all", "direct+cache"])
+def test_cached_render_beats_plain_at_equal_samples_after_training(direct):
     """Displaced icosphere under area lights (indirect light matters): after
     training online over 48 one-sample frames, 16 more cached frames have
     lower relMSE against a 1024-spp plain reference than 16 plain frames (the
-    cache replaces the noisy tails of the paths)."""
+    cache replaces the noisy tails of the paths), whether the cache stands in
+    for all the query vertex reflects or only for its indirect part."""
     from bulbit_amd import scenes
     from bulbit_amd.gpu import GpuScene
     from bulbit_amd.nrc import Nrc
@@ -271,7 +273,7 @@ def test_cached_render_beats_plain_at_equal_samples_after_training():
     rs, rc, _ = g.render(cam, scenes.render_params(1024, sample_begin=100000))
     ref = rs / rc[..., None]
     net = Nrc(seed=5, learning_rate=1e-2)
-    cp = scenes.cache_params(train_stride=4, train_batch=16384, train_steps=4)
+    cp = scenes.cache_params(train_stride=4, train_batch=16384, train_steps=4, query_direct=direct)
     for f in range(48):
         _, _, st = g.render_cached(net, cam, scenes.render_params(1, sample_begin=f), cp)
     print(f"[nrc] training loss after 48 frames {st.loss:.4f}, rows/frame {st.train_rows}", flush=True)
@@ -281,6 +283,114 @@ def test_cached_render_beats_plain_at_equal_samples_after_training():
     s0, c0, _ = g.render(cam, scenes.render_params(16, sample_begin=48))
     e_cache = _relmse(s1 / c1[..., None], ref)
     e_plain = _relmse(s0 / c0[..., None], ref)
-    print(f"[nrc] relMSE at 16 spp: cached {e_cache:.5f}, plain {e_plain:.5f}", flush=True)
+    print(f"[nrc] relMSE at 16 spp (direct light at the query vertex: {direct}): cached {e_cache:.5f}, "
+          f"plain {e_plain:.5f}", flush=True)
     assert e_cache < e_plain
     net.close()
+
+
+# ------------------------------------------------ divergence and the fallback
+
+def _readme_small():
+    from bulbit_amd import scenes
+    from bulbit_amd.gpu import GpuScene
+
+    sc, cam = scenes.readme_scene(width=160, height=90)
+    return GpuScene(sc), cam
+
+
+@pytest.mark.gpu
+def test_diverged_cache_falls_back_to_the_plain_tracer():
+    """Non-finite weights (set from the host) mark the network diverged at
+    once; a cached render is then the plain render, bit for bit, with no
+    query, and says so.  New finite weights clear the flag."""
+    from bulbit_amd import scenes
+    from bulbit_amd.nrc import Nrc
+
+    g, cam = _readme_small()
+    p = scenes.render_params(4)
+    s0, c0, _ = g.render(cam, p)
+    net = Nrc(seed=3)
+    w = net.weights()
+    w[5] = np.nan
+    net.set_weights(w)
+    assert net.diverged()
+    cp = scenes.cache_params(train_stride=16, train_steps=2, query_primary=True)
+    s1, c1, st = g.render_cached(net, cam, p, cp)
+    assert st.diverged == 1 and st.passes_redone == 0 and st.queries == 0
+    assert np.array_equal(c0, c1)
+    assert np.array_equal(s0.view(np.int32), s1.view(np.int32))
+    net.set_weights(np.zeros(net.n_params, np.float32))
+    assert not net.diverged()
+    _, _, st = g.render_cached(net, cam, p, cp)
+    assert st.diverged == 0 and st.queries > 0
+    net.close()
+    g.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("passes", [1, 4])
+def test_non_finite_predictions_drop_and_redo_the_pass(passes):
+    """Weights of 1e20 are finite, so the network starts out healthy, but its
+    predictions overflow.  The pass that met them is dropped from the film
+    and rendered again by the plain tracer, the network is marked diverged,
+    and the rest of the call and later calls run plain.  With one pass per
+    call the film is the plain render bit for bit; with four, the same
+    samples are added in another order (counts equal, sums within rounding)."""
+    from bulbit_amd import scenes
+    from bulbit_amd.nrc import Nrc
+
+    g, cam = _readme_small()
+    npix = 160 * 90
+    p = scenes.render_params(4, wavefront_paths=npix if passes == 4 else 0)
+    s0, c0, _ = g.render(cam, p)
+    net = Nrc(seed=3)
+    net.set_weights(np.full(net.n_params, 1e20, np.float32))
+    assert not net.diverged()
+    cp = scenes.cache_params(train_stride=4 * npix + 1, train_steps=0, query_primary=True)
+    print(f"[nrc] divergence with {passes} pass(es) per call", flush=True)
+    s1, c1, st = g.render_cached(net, cam, p, cp)
+    assert st.diverged == 1 and st.passes_redone == 1
+    assert net.diverged()
+    assert np.array_equal(c0, c1)
+    if passes == 1:
+        assert np.array_equal(s0.view(np.int32), s1.view(np.int32))
+    else:
+        assert np.allclose(s1, s0, rtol=1e-5, atol=1e-6)
+    s2, c2, st2 = g.render_cached(net, cam, p, cp)
+    assert st2.diverged == 1 and st2.queries == 0 and st2.passes_redone == 0
+    assert np.array_equal(c0, c2)
+    assert np.array_equal(s0.view(np.int32), s2.view(np.int32))
+    net.close()
+    g.close()
+
+
+@pytest.mark.gpu
+def test_direct_light_at_the_query_vertex():
+    """query_direct: the query vertex takes its direct light as the plain
+    tracer does and the cache stands in for the indirect rest.  With every
+    path training the image is the plain render bit for bit; with a zero
+    cache every pixel lies between the render without the query vertex's
+    direct light and the plain render."""
+    from bulbit_amd import scenes
+    from bulbit_amd.nrc import Nrc
+
+    g, cam = _readme_small()
+    p = scenes.render_params(4)
+    s0, c0, _ = g.render(cam, p)
+    net = Nrc(seed=3)
+    s1, c1, st = g.render_cached(net, cam, p, scenes.cache_params(train_stride=1, train_extra_bounces=8,
+                                                                   train_steps=1, query_direct=True))
+    assert np.array_equal(s0.view(np.int32), s1.view(np.int32)) and st.queries == 0 and st.train_rows > 0
+    net.set_weights(np.zeros(net.n_params, np.float32))
+    n_paths = 160 * 90 * 4
+    cp = dict(train_stride=n_paths + 1, train_steps=0, query_primary=True)
+    s_all, c_all, _ = g.render_cached(net, cam, p, scenes.cache_params(**cp))
+    s_dir, c_dir, st = g.render_cached(net, cam, p, scenes.cache_params(query_direct=True, **cp))
+    assert np.array_equal(c0, c_dir) and st.queries > 0
+    assert np.all(s_dir <= s0 * (1 + 1e-6) + 1e-7)
+    assert np.all(s_dir >= s_all * (1 - 1e-6) - 1e-7)
+    # the spot light reaches the sphere: direct light is most of what the sphere reflects
+    assert (s_dir - s_all).sum() > 0.5 * (s0 - s_all).sum()
+    net.close()
+    g.close()
