@@ -169,6 +169,20 @@ struct SceneView
     int32_t has_null_material;
     float root_box[6];  // root node box: min.xyz, max.xyz
     int32_t root_ref;   // the root as a reference (inner record 0, or a leaf)
+    // Two-level records (8 float4 each) for the traversal kernels that keep
+    // no node counters: record of internal node N holds the entries of N's
+    // children, an inner child replaced by its own two children (so up to 4
+    // entries: node N's grandchildren, or a child that is a leaf):
+    //   {b0.min.xyz, b0.max.x} {b0.max.yz, b1.min.xy} {b1.min.z, b1.max.xyz}
+    //   {b2.min.xyz, b2.max.x} {b2.max.yz, b3.min.xy} {b3.min.z, b3.max.xyz}
+    //   {ref0, ref1, ref2, ref3} {meta, 0, 0, 0}
+    // entries 0, 1 come from child1 (N + 1), entries 2, 3 from child2; an
+    // entry is a two-level record (>= 0) or a leaf (~first slot).  meta:
+    // bits 0-1 N's split axis, 2-3 child1's, 4-5 child2's, bit 6: entry 1
+    // exists (child1 is inner), bit 7: entry 3 exists (child2 is inner).
+    const float4* quads;
+    int32_t quad_root;        // the root as a two-level reference (record 0, or a leaf)
+    int32_t quad_stack_depth; // most stack entries a two-level traversal holds (+ 1)
 };
 
 // Host-side build products

@@ -711,8 +711,94 @@ bool create_scene(const BbSceneDesc& d, BbScene* s, std::string& err)
         r[2] = make_float4(c2.bmin[2], c2.bmax[0], c2.bmax[1], c2.bmax[2]);
         r[3] = make_float4(i2f(ref_of(i + 1)), i2f(ref_of(size_t(nodes[i].offset))), i2f(int32_t(nodes[i].axis)), 0.0f);
     }
+    // two-level records: one for the root and one for every inner node that
+    // is an entry of another record (an inner grandchild), numbered in
+    // depth-first order; quad_levels = the most records on a root-to-leaf walk
+    std::vector<int32_t> quad_index(nodes.size(), -1);
+    std::vector<int32_t> quad_order;
+    int32_t quad_levels = 0;
+    {
+        struct Q
+        {
+            size_t node;
+            int32_t level;
+        };
+        std::vector<Q> todo;
+        if (nodes[0].count == 0) todo.push_back({ 0, 1 });
+        while (!todo.empty()) {
+            const Q q = todo.back();
+            todo.pop_back();
+            quad_index[q.node] = int32_t(quad_order.size());
+            quad_order.push_back(int32_t(q.node));
+            quad_levels = std::max(quad_levels, q.level);
+            const size_t kids[2] = { q.node + 1, size_t(nodes[q.node].offset) };
+            for (int c = 1; c >= 0; --c) { // child1's entries first in the numbering
+                const size_t k = kids[c];
+                if (nodes[k].count != 0) continue;
+                const size_t gk[2] = { k + 1, size_t(nodes[k].offset) };
+                for (int g = 1; g >= 0; --g)
+                    if (nodes[gk[g]].count == 0) todo.push_back({ gk[g], q.level + 1 });
+            }
+        }
+    }
+    const int32_t n_quad = int32_t(quad_order.size());
+    auto qref_of = [&](size_t c) { return nodes[c].count == 0 ? quad_index[c] : ~nodes[c].offset; };
+    std::vector<float4> quads(size_t(8) * std::max(1, n_quad), make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    for (int32_t qi = 0; qi < n_quad; ++qi) {
+        const size_t nn = size_t(quad_order[qi]);
+        const size_t kids[2] = { nn + 1, size_t(nodes[nn].offset) };
+        size_t ent[4] = { 0, 0, 0, 0 };
+        bool has[4] = { true, false, true, false };
+        int32_t meta = int32_t(nodes[nn].axis);
+        for (int c = 0; c < 2; ++c) {
+            const size_t k = kids[c];
+            if (nodes[k].count == 0) {
+                ent[2 * c] = k + 1;
+                ent[2 * c + 1] = size_t(nodes[k].offset);
+                has[2 * c + 1] = true;
+                meta |= int32_t(nodes[k].axis) << (2 + 2 * c);
+                meta |= 1 << (6 + c);
+            } else {
+                ent[2 * c] = k;
+            }
+        }
+        float box[4][6] = {};
+        int32_t ref[4] = { 0, 0, 0, 0 };
+        for (int e = 0; e < 4; ++e) {
+            if (!has[e]) continue;
+            const BbBvhNode& b = nodes[ent[e]];
+            for (int a = 0; a < 3; ++a) {
+                box[e][a] = b.bmin[a];
+                box[e][3 + a] = b.bmax[a];
+            }
+            ref[e] = qref_of(ent[e]);
+        }
+        float4* r = &quads[size_t(8) * qi];
+        for (int h = 0; h < 2; ++h) {
+            const float* b0 = box[2 * h];
+            const float* b1 = box[2 * h + 1];
+            r[3 * h + 0] = make_float4(b0[0], b0[1], b0[2], b0[3]);
+            r[3 * h + 1] = make_float4(b0[4], b0[5], b1[0], b1[1]);
+            r[3 * h + 2] = make_float4(b1[2], b1[3], b1[4], b1[5]);
+        }
+        r[6] = make_float4(i2f(ref[0]), i2f(ref[1]), i2f(ref[2]), i2f(ref[3]));
+        r[7] = make_float4(i2f(meta), 0.0f, 0.0f, 0.0f);
+    }
+
     // every reference the kernels follow stays inside its array
     auto ref_ok = [&](int32_t ref) { return ref >= 0 ? ref < n_inner : (~ref >= 0 && ~ref < ns); };
+    auto qref_ok = [&](int32_t ref) { return ref >= 0 ? ref < n_quad : (~ref >= 0 && ~ref < ns); };
+    for (int32_t k = 0; k < n_quad; ++k) {
+        const float4 rr = quads[size_t(8) * k + 6];
+        const int32_t meta = f2i(quads[size_t(8) * k + 7].x);
+        const int32_t rs[4] = { f2i(rr.x), f2i(rr.y), f2i(rr.z), f2i(rr.w) };
+        const bool has1 = (meta >> 6) & 1, has3 = (meta >> 7) & 1;
+        if (!qref_ok(rs[0]) || !qref_ok(rs[2]) || (has1 && !qref_ok(rs[1])) || (has3 && !qref_ok(rs[3])) ||
+            (meta & 3) == 3 || ((meta >> 2) & 3) == 3 || ((meta >> 4) & 3) == 3) {
+            err = "internal error: two-level BVH record out of range";
+            return false;
+        }
+    }
     for (int32_t k = 0; k < n_inner; ++k) {
         const int32_t r1 = f2i(inner[size_t(4) * k + 3].x), r2 = f2i(inner[size_t(4) * k + 3].y);
         const int32_t ax = f2i(inner[size_t(4) * k + 3].z);
@@ -740,6 +826,10 @@ bool create_scene(const BbSceneDesc& d, BbScene* s, std::string& err)
         V.root_box[3 + k] = nodes[0].bmax[k];
     }
     V.root_ref = ref_of(0);
+    if (!upload(s, quads.data(), quads.size(), &V.quads, err)) return false;
+    V.quad_root = qref_of(0);
+    // each record pushes at most 3 entries
+    V.quad_stack_depth = 3 * quad_levels + 1;
     if (!upload(s, tris.data(), tris.size(), &V.tris, err)) return false;
     if (!upload(s, slot_shade.data(), slot_shade.size(), &V.slot_shade, err)) return false;
     if (!upload(s, slot_light.data(), slot_light.size(), &V.slot_light, err)) return false;

@@ -228,7 +228,7 @@ BB_D void lane_root(const SceneView& S, Lane& L, TraceCount& c)
     const BoxHit h = box_part(mk3(S.root_box[0], S.root_box[1], S.root_box[2]),
                               mk3(S.root_box[3], S.root_box[4], S.root_box[5]), L.o, L.t_min, L.inv, L.n0, L.n1,
                               L.n2);
-    L.cur = box_pass(h, L.t_max) ? S.root_ref : kNoRef;
+    L.cur = box_pass(h, L.t_max) ? (COUNT ? S.root_ref : S.quad_root) : kNoRef;
 }
 
 // Traversal stack: entries [0, lds_entries) in LDS ([entry][thread] layout,
@@ -286,12 +286,117 @@ constexpr int32_t kFailedLo = 0x7fbfffff;
 // IntersectAny, src/accel/bvh.cpp:299-361).  With COUNT, every far child is
 // pushed and counted when popped, which reproduces RayCast's node-test count.
 // Returns true when the lane's query is finished.
+// An entry of a two-level record: its reference and box test
+struct QEnt
+{
+    int32_t ref;
+    BoxHit h;
+};
+
+BB_D QEnt qsel(bool c, QEnt a, QEnt b)
+{
+    QEnt r;
+    r.ref = c ? a.ref : b.ref;
+    r.h.ok = c ? a.h.ok : b.h.ok;
+    r.h.lo = c ? a.h.lo : b.h.lo;
+    return r;
+}
+
+// Expands a two-level record (SceneView::quads) for one lane.  RayCast at
+// the record's node N tests N's near child now and pushes the far one; at
+// an inner child it again tests the near grandchild now and pushes the far
+// one.  Here the up-to-four entries (grandchildren, or a child that is a
+// leaf) are ordered the same way from the ray's direction signs: near
+// child's near and far entry, then far child's near and far entry.  The
+// first is tested now and the others are pushed in reverse, each with its
+// entry distance for the pop-time `lo > t_max` check.  That visits the same
+// leaves in the same order with the same t_max as RayCast: no leaf lies
+// between a child's test and its near grandchild's, so both meet the same
+// t_max, and a grandchild's box lies inside its parent's, so its slab
+// interval lies inside the parent's (the float slab distances are monotone
+// in the plane coordinates, NaN counting as ignored) and it fails whenever
+// the parent would.  Only the node-test count differs, so the counting
+// kernels keep the one-level records.
+template <bool ANY, bool FULL>
+BB_D bool quad_expand(const SceneView& S, Lane& L, const TraceStack<StackEntry<ANY, false>>& stk, int* err)
+{
+    constexpr bool kSlim = std::is_same<StackEntry<ANY, false>, int>::value;
+    const float4* r = S.quads + 8 * L.cur;
+    const float4 q0 = r[0], q1 = r[1], q2 = r[2], q3 = r[3], q4 = r[4], q5 = r[5], q6 = r[6], q7 = r[7];
+    // the record's 128 bytes in one round trip
+    asm volatile("" ::"v"(q0.x), "v"(q0.y), "v"(q0.z), "v"(q0.w), "v"(q1.x), "v"(q1.y), "v"(q1.z), "v"(q1.w),
+                 "v"(q2.x), "v"(q2.y), "v"(q2.z), "v"(q2.w), "v"(q3.x), "v"(q3.y), "v"(q3.z), "v"(q3.w), "v"(q4.x),
+                 "v"(q4.y), "v"(q4.z), "v"(q4.w), "v"(q5.x), "v"(q5.y), "v"(q5.z), "v"(q5.w), "v"(q6.x), "v"(q6.y),
+                 "v"(q6.z), "v"(q6.w), "v"(q7.x));
+    const int32_t meta = __float_as_int(q7.x);
+    const bool has1 = (meta >> 6) & 1, has3 = (meta >> 7) & 1;
+    QEnt e0, e1, e2, e3;
+    e0.ref = __float_as_int(q6.x);
+    e1.ref = __float_as_int(q6.y);
+    e2.ref = __float_as_int(q6.z);
+    e3.ref = __float_as_int(q6.w);
+    e0.h = box_part(mk3(q0.x, q0.y, q0.z), mk3(q0.w, q1.x, q1.y), L.o, L.t_min, L.inv, L.n0, L.n1, L.n2);
+    e1.h = box_part(mk3(q1.z, q1.w, q2.x), mk3(q2.y, q2.z, q2.w), L.o, L.t_min, L.inv, L.n0, L.n1, L.n2);
+    e2.h = box_part(mk3(q3.x, q3.y, q3.z), mk3(q3.w, q4.x, q4.y), L.o, L.t_min, L.inv, L.n0, L.n1, L.n2);
+    e3.h = box_part(mk3(q4.z, q4.w, q5.x), mk3(q5.y, q5.z, q5.w), L.o, L.t_min, L.inv, L.n0, L.n1, L.n2);
+    e1.h.ok = e1.h.ok && has1;
+    e3.h.ok = e3.h.ok && has3;
+    const int32_t aN = meta & 3, aA = (meta >> 2) & 3, aB = (meta >> 4) & 3;
+    const bool nN = aN == 0 ? L.n0 : (aN == 1 ? L.n1 : L.n2);
+    const bool nA = aA == 0 ? L.n0 : (aA == 1 ? L.n1 : L.n2);
+    const bool nB = aB == 0 ? L.n0 : (aB == 1 ? L.n1 : L.n2);
+    // each child's entries in near-far order (child2 is near when d[axis] < 0)
+    const bool sA = has1 && nA, sB = has3 && nB;
+    const QEnt a_near = qsel(sA, e1, e0), a_far = qsel(sA, e0, e1);
+    const QEnt b_near = qsel(sB, e3, e2), b_far = qsel(sB, e2, e3);
+    const QEnt x0 = qsel(nN, b_near, a_near), x1 = qsel(nN, b_far, a_far);
+    const QEnt x2 = qsel(nN, a_near, b_near), x3 = qsel(nN, a_far, b_far);
+    if (L.sp + 3 > stk.cap) {
+        atomicOr(err, 1);
+        return true;
+    }
+    // pushed only when they pass with the current t_max (t_max only shrinks)
+    if (box_pass(x3.h, L.t_max)) {
+        if constexpr (kSlim)
+            stack_put<FULL>(stk, L.sp, x3.ref);
+        else
+            stack_put<FULL>(stk, L.sp, make_int2(x3.ref, __float_as_int(x3.h.lo)));
+        ++L.sp;
+    }
+    if (box_pass(x2.h, L.t_max)) {
+        if constexpr (kSlim)
+            stack_put<FULL>(stk, L.sp, x2.ref);
+        else
+            stack_put<FULL>(stk, L.sp, make_int2(x2.ref, __float_as_int(x2.h.lo)));
+        ++L.sp;
+    }
+    if (box_pass(x1.h, L.t_max)) {
+        if constexpr (kSlim)
+            stack_put<FULL>(stk, L.sp, x1.ref);
+        else
+            stack_put<FULL>(stk, L.sp, make_int2(x1.ref, __float_as_int(x1.h.lo)));
+        ++L.sp;
+    }
+    if (box_pass(x0.h, L.t_max)) {
+        L.cur = x0.ref;
+        return false;
+    }
+    L.cur = kNoRef;
+    return false;
+}
+
 template <bool ANY, bool COUNT, bool FULL, bool ALPHA>
 BB_D bool trace_step(const SceneView& S, Lane& L, const TraceStack<StackEntry<ANY, COUNT>>& stk, TraceCount& c,
                      int* err)
 {
     constexpr bool kSlim = std::is_same<StackEntry<ANY, COUNT>, int>::value;
-    if (L.cur >= 0) {
+    if (!COUNT && L.cur >= 0) {
+        // two-level records (no node counters kept)
+        if constexpr (!COUNT) {
+            if (quad_expand<ANY, FULL>(S, L, stk, err)) return true;
+            if (L.cur != kNoRef) return false;
+        }
+    } else if (L.cur >= 0) {
         const float4* r = S.inner + 4 * L.cur;
         const float4 a = r[0], b = r[1], e = r[2], m = r[3];
         // the record's 64 bytes in one round trip
@@ -476,6 +581,10 @@ struct BatchSrc
     }
 };
 
+// most entries a traversal stack holds: one-level records (with node
+// counters) or two-level records
+BB_HD int trace_stack_depth(const SceneView& V, bool count) { return count ? V.stack_depth : V.quad_stack_depth; }
+
 // the calling thread's stack in the block's dynamic LDS and in the overflow
 // area (both hold entries of type E)
 template <typename E>
@@ -505,7 +614,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_batch(SceneView S, BatchS
                                                                unsigned long long* counters, int* err)
 {
     extern __shared__ int2 lds_stack[];
-    const auto stk = make_stack<StackEntry<ANY, COUNT>>(lds_stack, gstack, lds_entries, S.stack_depth);
+    const auto stk = make_stack<StackEntry<ANY, COUNT>>(lds_stack, gstack, lds_entries, trace_stack_depth(S, COUNT));
     TraceCount c{ 0, 0, 0 };
     persistent_trace<ANY, COUNT, FULL, ALPHA, kTraceSteps>(S, src, n, fetch, stk, c, err, refill_min);
     if (COUNT) wave_count_add(c, counters);
@@ -707,7 +816,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(SceneView S, cons
                                                                  unsigned long long* counters, int* err)
 {
     extern __shared__ int2 lds_stack[];
-    const auto stk = make_stack<StackEntry<false, COUNT>>(lds_stack, gstack, lds_entries, S.stack_depth);
+    const auto stk = make_stack<StackEntry<false, COUNT>>(lds_stack, gstack, lds_entries, trace_stack_depth(S, COUNT));
     TraceCount c{ 0, 0, 0 };
     persistent_trace<false, COUNT, FULL, ALPHA, kTraceSteps>(S, src, *n_ptr, fetch, stk, c, err, refill_min);
     if (COUNT) wave_count_add(c, counters);
@@ -719,7 +828,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_shadow(SceneView S, const
                                                                 unsigned long long* counters, int* err)
 {
     extern __shared__ int2 lds_stack[];
-    const auto stk = make_stack<StackEntry<true, COUNT>>(lds_stack, gstack, lds_entries, S.stack_depth);
+    const auto stk = make_stack<StackEntry<true, COUNT>>(lds_stack, gstack, lds_entries, trace_stack_depth(S, COUNT));
     TraceCount c{ 0, 0, 0 };
     persistent_trace<true, COUNT, FULL, ALPHA, kShadowSteps>(S, src, *n_ptr, fetch, stk, c, err, refill_min);
     if (COUNT) wave_count_add(c, counters + 3);
@@ -1451,10 +1560,10 @@ struct EventPool
 
 // LDS entries of the traversal stack per lane: `requested` or, by default,
 // the whole stack up to 32 entries (deeper entries overflow to global memory)
-int lds_entries_for(const SceneView& V, int requested)
+int lds_entries_for(const SceneView& V, int requested, bool count)
 {
     int e = requested > 0 ? requested : 32;
-    return std::max(1, std::min(e, V.stack_depth));
+    return std::max(1, std::min(e, trace_stack_depth(V, count)));
 }
 
 // dynamic LDS of a traversal kernel: 4-byte entries for any-hit queries
@@ -1542,12 +1651,13 @@ struct TracePlan
 TracePlan trace_plan(const SceneView& V, int32_t n_rays, int32_t any_hit, bool count)
 {
     TracePlan t{};
-    t.lds_entries = lds_entries_for(V, 0);
+    const int depth = trace_stack_depth(V, count);
+    t.lds_entries = lds_entries_for(V, 0, count);
     t.lds = trace_lds_bytes(t.lds_entries, any_hit != 0 && !count);
-    const bool full = V.stack_depth <= t.lds_entries;
+    const bool full = depth <= t.lds_entries;
     t.kfn = batch_kernel(any_hit != 0, count, full, V.has_alpha != 0);
     t.grid = std::min(trace_grid(reinterpret_cast<const void*>(t.kfn), t.lds), grid_for(std::max(n_rays, 1), kTraceBlock));
-    t.overflow_per_lane = size_t(std::max(0, V.stack_depth - t.lds_entries));
+    t.overflow_per_lane = size_t(std::max(0, depth - t.lds_entries));
     return t;
 }
 
@@ -1703,7 +1813,7 @@ int bbgpu_trace_noalloc(BbScene* scene, const BbRay* rays_dev, int32_t n_rays, i
         if (fit < 1) {
             bb::set_error("bbgpu_trace_noalloc: the traversal stack (%d entries, %d in LDS) needs %zu bytes of "
                           "scratch per block, more than the %zu bytes available",
-                          V.stack_depth, t.lds_entries, per_block, area);
+                          trace_stack_depth(V, counters != nullptr), t.lds_entries, per_block, area);
             return BBGPU_ERR_INVALID;
         }
         t.grid = int(std::min<size_t>(size_t(t.grid), fit));
@@ -1830,12 +1940,13 @@ int render_impl(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams
     const SceneView& V = scene->view;
     const bool timed = stats != nullptr;
     const bool count = timed && count_traversal;
-    const int lds_entries = lds_entries_for(V, p.trace_lds_stack);
+    const int depth = trace_stack_depth(V, count);
+    const int lds_entries = lds_entries_for(V, p.trace_lds_stack, count);
     const size_t lds = trace_lds_bytes(lds_entries);
     const size_t lds_shadow = trace_lds_bytes(lds_entries, !count); // reference-only entries without counters
     const int refill_min = p.trace_refill > 0 ? std::min(p.trace_refill, 64) : 48;
     const int refill_shadow = p.shadow_refill > 0 ? std::min(p.shadow_refill, 64) : kShadowRefillDefault;
-    const bool full = V.stack_depth <= lds_entries;
+    const bool full = depth <= lds_entries;
     const ClosestKernel k_closest = closest_kernel(count, full, V.has_alpha != 0);
     // shading with the texture code only for scenes that use textures
     void (*const k_shade_fn)(SceneView, RenderConsts, PathBufs, int, CacheBufs) =
@@ -1846,7 +1957,7 @@ int render_impl(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams
     const int g_shadow =
         std::min(trace_grid(reinterpret_cast<const void*>(k_shadow), lds_shadow), grid_for(cap, kTraceBlock));
     // the overflow area serves both grids (int2-sized entries hold the 4-byte ones too)
-    if (!ensure_gstack(x->ws, size_t(std::max(0, V.stack_depth - lds_entries)) * size_t(std::max(g_trace, g_shadow)) *
+    if (!ensure_gstack(x->ws, size_t(std::max(0, depth - lds_entries)) * size_t(std::max(g_trace, g_shadow)) *
                                   kTraceBlock)) {
         bb::set_error("bbgpu_render: traversal stack allocation failed");
         return BBGPU_ERR_NOMEM;
