@@ -12,7 +12,7 @@ from ctypes import POINTER, c_char_p, c_double, c_float, c_int32, c_size_t, c_ui
 
 import numpy as np
 
-ABI_VERSION = 16  # BBGPU_ABI_VERSION of include/bbgpu.h that these structures mirror
+ABI_VERSION = 17  # BBGPU_ABI_VERSION of include/bbgpu.h that these structures mirror
 
 LIB_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(LIB_DIR, "libbbgpu.so")
@@ -193,6 +193,7 @@ class BbRenderParams(ctypes.Structure):
         ("trace_refill", c_int32),
         ("shadow_refill", c_int32),
         ("graph", c_int32),
+        ("light_sampler", c_int32),
     ]
 
 
@@ -259,6 +260,7 @@ EXPORTS = [
     "bbgpu_abi_version",
     "bbgpu_last_error",
     "bbgpu_device_count",
+    "bbgpu_alias_table",
     "bbgpu_build_bvh",
     "bbgpu_reflectance_table",
     "bbgpu_scene_create",
@@ -309,6 +311,7 @@ def load_library(path: str | None = None):
     lib.bbgpu_abi_version.restype = c_int32
     lib.bbgpu_last_error.argtypes = [c_char_p, c_size_t]
     lib.bbgpu_device_count.argtypes = [POINTER(c_int32)]
+    lib.bbgpu_alias_table.argtypes = [c_void_p, c_int32, c_void_p, c_void_p]
     lib.bbgpu_build_bvh.argtypes = [POINTER(BbSceneDesc), c_void_p, c_void_p, POINTER(c_int32)]
     lib.bbgpu_reflectance_table.argtypes = [c_int32, c_void_p, c_int32, POINTER(c_int32)]
     lib.bbgpu_scene_create.argtypes = [POINTER(BbSceneDesc), POINTER(c_void_p)]

@@ -20,7 +20,7 @@
 //   normals, tangents [n_vertices] float4 (xyz used)
 //   spheres [n_spheres] BbSphere
 //   materials [n_materials] GpuMaterial, lights [n_lights] GpuLight,
-//   light_cdf [n_lights + 1], light_func [n_lights]
+//   light_cdf [n_lights + 1], light_func [n_lights], light_alias [n_lights]
 #pragma once
 
 #include "../../include/bbgpu.h"
@@ -153,6 +153,7 @@ struct SceneView
     const GpuLight* lights;
     const float* light_cdf;
     const float* light_func;
+    const float2* light_alias; // alias table over the lights: {probability of keeping, alias index bits}
     const int32_t* inf_lights;
     const float* ggx_E;     // ReflectanceTables::ggx_E
     const float* ggx_E_avg; // ReflectanceTables::ggx_E_avg
@@ -192,6 +193,11 @@ struct HostBvh
     std::vector<int32_t> ordered; // slot -> primitive
     int32_t max_depth = 0;
 };
+
+// Walker's alias table for weights w[0..n) (Vose's construction, in double):
+// draw i uniformly, keep it with probability prob[i], else take alias[i].
+// All-zero weights select uniformly.
+void build_alias_table(const float* w, int32_t n, float* prob, int32_t* alias);
 
 // Builds the reference's SAH BVH (src/accel/bvh.cpp:56-297) on the host.
 bool build_bvh(const BbSceneDesc& d, HostBvh& out, std::string& err);

@@ -1169,6 +1169,24 @@ BB_D int32_t light_select(const SceneView& S, float u, float& pmf)
     return offset;
 }
 
+// Opt-in alias-table selection over the same light powers (no counterpart
+// in the reference, whose PowerLightSampler searches the CDF): i from the
+// integer part of u n, kept when the fractional part is below its keep
+// probability, else its alias.  The returned pmf is the power sampler's, so
+// the estimator is the same and only the draw-to-light map differs.
+BB_D int32_t light_select_alias(const SceneView& S, float u, float& pmf)
+{
+    const int32_t n = S.n_lights;
+    const float x = u * float(n);
+    int32_t i = int32_t(x);
+    if (i > n - 1) i = n - 1;
+    if (i < 0) i = 0;
+    const float2 e = S.light_alias[i];
+    if (!(x - float(i) < e.x)) i = __float_as_int(e.y);
+    pmf = S.light_func[i] / (S.func_integral * float(n));
+    return i;
+}
+
 struct light_sample
 {
     f3 wi;

@@ -648,6 +648,14 @@ bool create_scene(const BbSceneDesc& d, BbScene* s, std::string& err)
     }
     s->n_lights = nl;
     s->n_inf = int32_t(inf.size());
+    // the opt-in alias-table selection over the same powers
+    std::vector<float2> alias_tab(std::max(1, nl), make_float2(1.0f, 0.0f));
+    if (nl > 0) {
+        std::vector<float> ap(nl);
+        std::vector<int32_t> aa(nl);
+        build_alias_table(func.data(), nl, ap.data(), aa.data());
+        for (int32_t i = 0; i < nl; ++i) alias_tab[i] = make_float2(ap[i], i2f(aa[i]));
+    }
 
     // BVH slot data
     const int32_t ns = s->n_slots;
@@ -843,6 +851,7 @@ bool create_scene(const BbSceneDesc& d, BbScene* s, std::string& err)
     cdf.resize(std::max<size_t>(2, cdf.size()));
     if (!upload(s, cdf.data(), cdf.size(), &V.light_cdf, err)) return false;
     if (!upload(s, func.data(), func.size(), &V.light_func, err)) return false;
+    if (!upload(s, alias_tab.data(), alias_tab.size(), &V.light_alias, err)) return false;
     inf.resize(std::max<size_t>(1, inf.size()));
     if (!upload(s, inf.data(), inf.size(), &V.inf_lights, err)) return false;
     const ReflectanceTables& tab = reflectance_tables();
@@ -891,6 +900,36 @@ void destroy_scene(BbScene* s)
     s->allocations.clear();
 }
 
+void build_alias_table(const float* w, int32_t n, float* prob, int32_t* alias)
+{
+    double sum = 0.0;
+    for (int32_t i = 0; i < n; ++i) sum += std::max(0.0, double(w[i]));
+    std::vector<double> q(n);
+    for (int32_t i = 0; i < n; ++i) q[i] = sum > 0.0 ? std::max(0.0, double(w[i])) * n / sum : 1.0;
+    std::vector<int32_t> small, large;
+    for (int32_t i = 0; i < n; ++i) (q[i] < 1.0 ? small : large).push_back(i);
+    while (!small.empty() && !large.empty()) {
+        const int32_t l = small.back(), g = large.back();
+        small.pop_back();
+        prob[l] = float(q[l]);
+        alias[l] = g;
+        q[g] = (q[g] + q[l]) - 1.0;
+        if (q[g] < 1.0) {
+            large.pop_back();
+            small.push_back(g);
+        }
+    }
+    // what is left has probability 1 up to rounding
+    for (int32_t i : large) {
+        prob[i] = 1.0f;
+        alias[i] = i;
+    }
+    for (int32_t i : small) {
+        prob[i] = 1.0f;
+        alias[i] = i;
+    }
+}
+
 } // namespace bb
 
 // =================================================================== C ABI
@@ -898,6 +937,16 @@ void destroy_scene(BbScene* s)
 extern "C" {
 
 int bbgpu_abi_version(void) { return BBGPU_ABI_VERSION; }
+
+int bbgpu_alias_table(const float* weights, int32_t n, float* prob_out, int32_t* alias_out)
+{
+    if (n < 0 || (n > 0 && (!weights || !prob_out || !alias_out))) {
+        bb::set_error("bbgpu_alias_table: invalid argument");
+        return BBGPU_ERR_INVALID;
+    }
+    if (n > 0) bb::build_alias_table(weights, n, prob_out, alias_out);
+    return BBGPU_OK;
+}
 
 int bbgpu_last_error(char* buf, size_t buf_size)
 {

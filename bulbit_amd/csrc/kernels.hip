@@ -698,6 +698,7 @@ struct RenderConsts
     int32_t sample0;              // the pass's first sample relative to the call's sample_begin
     const int32_t* sample_begin;  // device copy of the call's sample_begin (set per call, read by k_raygen)
     int32_t train_stride; // cached rendering: every train_stride-th path is a training path (0: none)
+    int32_t light_alias;  // 1: alias-table light selection (BB_LIGHTS_ALIAS)
 };
 
 // Path flags: bounce (bits 0-15), the last BSDF sample was specular (16),
@@ -1094,7 +1095,8 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView S, RenderConsts
                             lu12.y = pcg_float(st);
                             if (S.n_lights > 0) {
                                 float pmf;
-                                const int32_t li = light_select(S, lu0, pmf);
+                                const int32_t li =
+                                    rc.light_alias ? light_select_alias(S, lu0, pmf) : light_select(S, lu0, pmf);
                                 const GpuLight& Lt = S.lights[li];
                                 light_sample ls;
                                 if (light_sample_li(Lt, sh.point, lu12, ls)) {
@@ -1458,6 +1460,28 @@ bool ensure_workspace(Workspace& w, size_t paths, int depth_slots)
     w.max_depth_slots = depth_slots;
     w.P.gstack = reinterpret_cast<int2*>(w.gstack);
     return true;
+}
+
+// total memory of the current device in bytes (looked up once per device)
+long long device_memory()
+{
+    static std::mutex lock;
+    static long long bytes[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+        (void)hipGetLastError();
+        return 16ll << 30;
+    }
+    std::lock_guard<std::mutex> g(lock);
+    if (!bytes[dev]) {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || total_b == 0) {
+            (void)hipGetLastError();
+            total_b = size_t(16) << 30;
+        }
+        bytes[dev] = (long long)total_b;
+    }
+    return bytes[dev];
 }
 
 int grid_for(long long n, int block)
@@ -1909,6 +1933,10 @@ int render_impl(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams
         bb::set_error("bbgpu_render: regularize_bsdf is not supported");
         return BBGPU_ERR_UNSUPPORTED;
     }
+    if (p.light_sampler != BB_LIGHTS_POWER && p.light_sampler != BB_LIGHTS_ALIAS) {
+        bb::set_error("bbgpu_render: unknown light sampler %d", p.light_sampler);
+        return BBGPU_ERR_INVALID;
+    }
     if (scene->view.has_null_material) {
         bb::set_error("bbgpu_render: primitives without material are not supported");
         return BBGPU_ERR_UNSUPPORTED;
@@ -1923,7 +1951,16 @@ int render_impl(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams
         return BBGPU_ERR_INVALID;
     }
     if (p.spp == 0) return BBGPU_OK;
-    long long want = p.wavefront_paths > 0 ? p.wavefront_paths : (1ll << 22);
+    // Paths in flight per pass.  Deep bounces hold few rays, and a traversal
+    // launch lasts as long as its slowest ray, so fewer, fuller launches win:
+    // by default a pass takes as many samples as an eighth of the device's
+    // memory holds in path state (a whole 1080p frame of 64 spp is 133M
+    // paths, about 24 GB)
+    long long want = p.wavefront_paths;
+    if (want <= 0) {
+        const long long bytes_per_path = cache ? 384 : 192;
+        want = std::max<long long>(1ll << 22, std::min<long long>(1ll << 28, device_memory() / 8 / bytes_per_path));
+    }
     long long per_pass = std::max<long long>(1, want / npix); // samples per pass
     per_pass = std::min<long long>(per_pass, p.spp);
     const long long cap = per_pass * npix;
@@ -2090,6 +2127,7 @@ int render_impl(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams
         rc.sample_begin = P.sample_begin;
         rc.max_bounces = p.max_bounces;
         rc.rr_min_bounces = p.rr_min_bounces;
+        rc.light_alias = p.light_sampler == BB_LIGHTS_ALIAS ? 1 : 0;
         rc.train_stride = stride;
         paths_total += n_paths;
         hipEvent_t a = timed ? mark() : nullptr;
@@ -2354,11 +2392,17 @@ int bbgpu_render_batch(BbRenderJob* jobs, int32_t n_jobs, int32_t streams, BbBat
         HIP_OK(hipEventRecord(start[size_t(k)], pool[size_t(k)]));
     }
     int64_t samples = 0;
+    // every scene keeps its own path state: the automatic pass size shares a
+    // quarter of the device memory among the batch's scenes
+    const long long auto_paths = std::max<long long>(
+        1ll << 22, std::min<long long>(1ll << 28, device_memory() / 4 / 192 / (long long)scenes.size()));
     for (size_t oi : order)
         for (int32_t i = 0; i < n_jobs; ++i) {
             if (jobs[i].scene != scenes[oi]) continue;
             hipStream_t st = pool[size_t(scene_stream[oi])];
-            const int rc = render_impl(jobs[i].scene, &jobs[i].camera, &jobs[i].params, jobs[i].film_sum_dev,
+            BbRenderParams jp = jobs[i].params;
+            if (jp.wavefront_paths <= 0) jp.wavefront_paths = int32_t(auto_paths);
+            const int rc = render_impl(jobs[i].scene, &jobs[i].camera, &jp, jobs[i].film_sum_dev,
                                        jobs[i].film_count_dev, st, nullptr, 0, nullptr, false);
             if (rc != BBGPU_OK) return rc;
             done[size_t(i)] = evp.get();
@@ -2417,7 +2461,7 @@ int bbgpu_autotune(BbScene* scene, const BbCameraDesc* camera, const BbRenderPar
     if (rc == BBGPU_OK) rc = time_of(best, best_ms);
     const int refills[] = { 32, 48, 64 };
     const int lds[] = { 16, 24, 32 };
-    const long long paths[] = { 1ll << 20, 1ll << 21, 1ll << 22 };
+    const long long paths[] = { 1ll << 22, 1ll << 24, 1ll << 26 };
     for (int knob = 0; knob < 3 && rc == BBGPU_OK; ++knob)
         for (int v = 0; v < 3 && rc == BBGPU_OK; ++v) {
             BbRenderParams q = best;

@@ -726,8 +726,11 @@ def large_scene(width=3840, height=2160, n_spheres=4096, level=5, n_lights=1024,
     return sc, cam
 
 
+LIGHTS_POWER, LIGHTS_ALIAS = 0, 1  # BbRenderParams.light_sampler
+
+
 def render_params(spp, max_bounces=8, seed=0, rr_min_bounces=1, sample_begin=0, wavefront_paths=0, trace_lds_stack=0,
-                  trace_refill=0, shadow_refill=0, graph=0):
+                  trace_refill=0, shadow_refill=0, graph=0, light_sampler=LIGHTS_POWER):
     p = capi.BbRenderParams()
     p.spp = spp
     p.seed = seed
@@ -739,6 +742,7 @@ def render_params(spp, max_bounces=8, seed=0, rr_min_bounces=1, sample_begin=0, 
     p.trace_refill = trace_refill
     p.shadow_refill = shadow_refill
     p.sample_begin = sample_begin
+    p.light_sampler = light_sampler
     p.graph = graph
     return p
 

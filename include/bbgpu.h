@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define BBGPU_ABI_VERSION 16
+#define BBGPU_ABI_VERSION 17
 
 /* status codes */
 enum {
@@ -261,7 +261,8 @@ typedef struct BbRenderParams {
     int32_t max_bounces;
     int32_t rr_min_bounces;
     int32_t regularize_bsdf;  /* only 0 is supported */
-    int32_t wavefront_paths;  /* paths in flight per pass (0 = auto) */
+    int32_t wavefront_paths;  /* paths in flight per pass (0 = auto: as many whole samples per pixel as an
+                                 eighth of the device memory holds in path state, at least 2^22 paths) */
     int32_t trace_lds_stack;  /* traversal stack entries per lane kept in LDS (0 = auto) */
     int32_t sample_begin;     /* render samples [sample_begin, sample_begin+spp) */
     int32_t trace_refill;     /* a closest-hit traversal wave takes new rays once this many of its lanes are idle (0 = 48) */
@@ -269,7 +270,13 @@ typedef struct BbRenderParams {
     int32_t graph;            /* 1: launch the render as a HIP graph, captured on first use and kept with the
                                  scene for calls that differ only in sample_begin (speed only; not with stats
                                  or the radiance cache) */
+    int32_t light_sampler;    /* BB_LIGHTS_POWER (0): PowerLightSampler's CDF search, as the reference;
+                                 BB_LIGHTS_ALIAS: an alias table over the same powers (opt-in: same
+                                 probabilities and estimator, a different draw-to-light map, so images
+                                 agree in expectation, not sample for sample) */
 } BbRenderParams;
+
+enum { BB_LIGHTS_POWER = 0, BB_LIGHTS_ALIAS = 1 };
 
 /* Per-render statistics, filled when a pointer is passed to bbgpu_render.
  * Kernel times come from HIP events recorded around each launch on the
@@ -320,6 +327,13 @@ enum {
     BB_TABLE_DIELECTRIC_E_INV_AVG = 5
 };
 int bbgpu_reflectance_table(int32_t table, float* out, int32_t capacity, int32_t* length_out);
+
+/* Walker's alias table of n non-negative weights, as the render's
+ * BB_LIGHTS_ALIAS mode builds it over the light powers (no counterpart in
+ * the reference, which searches a CDF): draw i uniformly, keep it with
+ * probability prob_out[i], else take alias_out[i].  All-zero weights select
+ * uniformly.  Host-only: makes no HIP call. */
+int bbgpu_alias_table(const float* weights, int32_t n, float* prob_out, int32_t* alias_out);
 
 /* Builds the SAH BVH on the host exactly as BVH::BVH
  * (src/accel/bvh.cpp:8-49, 56-271, 273-297) and returns the flattened nodes and
