@@ -157,8 +157,14 @@ def main():
             "primitives": len(sc.prims),
         },
         "roofline": {
-            "kernel": "k_trace_closest (BVH2 closest-hit traversal)",
+            "kernel": "k_trace_closest (closest-hit traversal of the reference BVH, two levels per fetch)",
             "bound": "hbm",
+            # what the counters show (profiles/traversal_c2_counters.md): the
+            # kernel is below the HBM ceiling because of latency and lane
+            # divergence, with measured HBM traffic well under the algorithmic bytes
+            "limiter": "memory latency and lane divergence: 29 of 64 lanes active per VALU instruction, waves "
+                       "waiting 53% of their cycles, L2 hit rate 48%, measured HBM traffic 0.18x the "
+                       "algorithmic bytes (profiles/traversal_c2_counters.md)",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
