@@ -31,6 +31,9 @@ BYTES_NODE, BYTES_PRIM, BYTES_ACCEPT, BYTES_RAY = 32, 36, 4, 28 + 16
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "trace_closest_c2_traffic.json")
 
 
+T_START = time.time()
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -58,6 +61,10 @@ def main():
                          "frame, max_bounces 12, the 128 x 5 radiance cache)")
     ap.add_argument("--large-spheres", type=int, default=4096)
     ap.add_argument("--large-frames", type=int, default=64)
+    ap.add_argument("--no-extra", action="store_true",
+                    help="leave out the batched and large-scene workloads of the default run")
+    ap.add_argument("--large-after", type=float, default=240.0,
+                    help="skip the large-scene workload of the default run when more seconds than this are spent")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (scales the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -196,6 +203,18 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, cam, params, args, samples_per_step)
         out["speedup_vs_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
+    g.close()
+    del s_dev, c_dev
+    torch.cuda.empty_cache()
+    # the other workloads of the configuration set, as extra keys: the batched
+    # form (32 scenes per call) and the large scene with the 128 x 5 cache
+    if not args.no_extra:
+        out["batched"] = batch_workload(args)
+        spent = time.time() - T_START
+        if spent < args.large_after:
+            out["large_scene"] = large_workload(args, nested=True)
+        else:
+            out["large_scene"] = {"skipped": f"{spent:.0f}s spent before it (limit {args.large_after:.0f}s)"}
     print(json.dumps(out), flush=True)
 
 
@@ -399,7 +418,7 @@ def cache_comparisons(args, g_c2, cam_c2, ref_c2):
     return out
 
 
-def large_workload(args):
+def large_workload(args, nested=False):
     """The large scene: `--large-spheres` displaced level-5 icospheres (4096:
     83,886,080 triangles) Poisson-disk placed on a 100 x 100 plane under 1024
     emissive triangles of log-normal power, metallic-roughness materials;
@@ -414,6 +433,8 @@ def large_workload(args):
     from bulbit_amd.gpu import GpuScene
 
     W, H, SPP, MB = 3840, 2160, 4, 12
+    # inside the default run: a few frames of its own, whatever --steps says
+    steps, warmup = (3, 1) if nested else (args.steps, args.warmup)
     t0 = time.time()
     sc, cam = scenes.large_scene(width=W, height=H, n_spheres=args.large_spheres)
     gen_s = time.time() - t0
@@ -425,9 +446,9 @@ def large_workload(args):
     del sc
     params = scenes.render_params(SPP, max_bounces=MB)
     s_dev, c_dev = g.new_film(cam)
-    for i in range(args.warmup):
+    for i in range(warmup):
         g.render_device(cam, params, s_dev.data_ptr(), c_dev.data_ptr())
-        log(f"[bench] warmup {i + 1}/{args.warmup} done")
+        log(f"[bench] warmup {i + 1}/{warmup} done")
     cst = g.render_device(cam, params, s_dev.data_ptr(), c_dev.data_ptr(), stats=True, count_traversal=True)
     cl = cst.closest
     log(f"[bench] counted frame: {cl.rays} closest rays, {cl.nodes / max(cl.rays, 1):.1f} nodes/ray, "
@@ -435,27 +456,27 @@ def large_workload(args):
     torch.cuda.synchronize()
     stats = []
     t = time.perf_counter()
-    for i in range(args.steps):
+    for i in range(steps):
         s_dev.zero_()
         c_dev.zero_()
         stats.append(g.render_device(cam, params, s_dev.data_ptr(), c_dev.data_ptr(), stats=True))
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t
-    value = W * H * SPP * args.steps / elapsed / 1e6
+    value = W * H * SPP * steps / elapsed / 1e6
     ms_closest = float(np.mean([s.ms_trace_closest for s in stats]))
     bytes_frame = BYTES_NODE * cl.nodes + BYTES_PRIM * cl.prims + BYTES_ACCEPT * cl.accepts + BYTES_RAY * cl.rays
     achieved = bytes_frame / (ms_closest / 1e3) / 1e9
     rays = stats[0].rays_closest + stats[0].rays_shadow
-    log(f"[bench] plain: {value:.1f} Msamples/s, {elapsed / args.steps * 1e3:.1f} ms per frame (closest "
+    log(f"[bench] plain: {value:.1f} Msamples/s, {elapsed / steps * 1e3:.1f} ms per frame (closest "
         f"{ms_closest:.1f} ms, {achieved:.0f} GB/s algorithmic)")
     out = {
         "metric": "Msamples/s at 2160p, 12 bounces, large scene; relMSE at equal time, cache vs plain PT",
         "value": round(value, 3),
         "unit": "Msamples/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": round(elapsed / steps * 1e3, 3),
         "higher_is_better": True,
         "dtype": "f32",
         "data": "synthetic",
@@ -467,7 +488,7 @@ def large_workload(args):
                       "frac_of_hbm_peak": round(achieved / HBM_PEAK_GBS, 4), "ms_per_frame": round(ms_closest, 2),
                       "nodes_per_ray": round(cl.nodes / max(cl.rays, 1), 2),
                       "prims_per_ray": round(cl.prims / max(cl.rays, 1), 3)},
-        "rays_per_s": round(rays * args.steps / elapsed, 1),
+        "rays_per_s": round(rays * steps / elapsed, 1),
     }
     if args.ref_spp > 0:
         # the reference in chunks of 64 spp, samples disjoint from every frame
@@ -489,8 +510,9 @@ def large_workload(args):
             cq = cache_comparison(args, g, cam, ref, name, False, train_stride=16, learning_rate=1e-3,
                                   train_steps=4, spp=SPP, max_bounces=MB, shape=(128, 5), frames=args.large_frames,
                                   keep_every=8)
-            out["relmse_equal_time_cache_vs_plain"] = cq["per_frame"]["ratio_equal_time"]
+            out["relmse_equal_time_cache_vs_plain"] = cq["ratio"]
             out["relmse_equal_time_cache_vs_plain_accumulated"] = cq["ratio"]
+            out["relmse_equal_time_cache_vs_plain_per_frame"] = cq["per_frame"]["ratio_equal_time"]
             out["radiance_cache"] = cq
     g.close()
     return out
@@ -517,7 +539,8 @@ def batch_workload(args):
         if (w, h) not in tuned:
             tuned[(w, h)] = autotune(g, cam, p)
         t = tuned[(w, h)]
-        p.trace_refill, p.trace_lds_stack, p.wavefront_paths = t.trace_refill, t.trace_lds_stack, t.wavefront_paths
+        for knob in ("trace_refill", "shadow_refill", "trace_block", "trace_stack", "trace_lds_stack", "wavefront_paths"):
+            setattr(p, knob, getattr(t, knob))
         jobs.append((g, cam, p, g.new_film(cam)))
         log(f"[bench] job {i}: level {level} ({len(sc.prims)} prims), {w}x{h}, max_bounces {p.max_bounces}, "
             f"{n_lights} lights")
@@ -546,7 +569,9 @@ def batch_workload(args):
         "job_ms_p99": round(float(np.percentile(jm, 99)), 2),
         "serial_ms": round(serial_ms, 2),
         "serial_msamples_per_s": round(st.samples / serial_ms / 1e3, 3),
-        "tuned": {f"{w}x{h}": [t.trace_refill, t.trace_lds_stack, t.wavefront_paths] for (w, h), t in tuned.items()},
+        "tuned": {f"{w}x{h}": {"refill": t.trace_refill, "block": t.trace_block,
+                               "stack": "scratch" if t.trace_stack else f"lds {t.trace_lds_stack or 'auto'}",
+                               "wavefront_paths": t.wavefront_paths} for (w, h), t in tuned.items()},
     }
     log(f"[bench] batch: {out['value']} Msamples/s in {st.ms_total:.1f} ms; one by one {serial_ms:.1f} ms")
     return out

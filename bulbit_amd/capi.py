@@ -12,7 +12,7 @@ from ctypes import POINTER, c_char_p, c_double, c_float, c_int32, c_size_t, c_ui
 
 import numpy as np
 
-ABI_VERSION = 17  # BBGPU_ABI_VERSION of include/bbgpu.h that these structures mirror
+ABI_VERSION = 18  # BBGPU_ABI_VERSION of include/bbgpu.h that these structures mirror
 
 LIB_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(LIB_DIR, "libbbgpu.so")
@@ -193,6 +193,8 @@ class BbRenderParams(ctypes.Structure):
         ("trace_refill", c_int32),
         ("shadow_refill", c_int32),
         ("graph", c_int32),
+        ("trace_block", c_int32),
+        ("trace_stack", c_int32),
         ("light_sampler", c_int32),
     ]
 

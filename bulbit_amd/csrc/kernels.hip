@@ -38,7 +38,12 @@ namespace bb
 {
 
 constexpr int kTraceBlock = 128;
-constexpr int kShadowRefillDefault = 48;
+// defaults measured on the 1080p 64-spp frame (one pass per frame): refill a
+// wave once 56 of its 64 lanes are idle; keep 12 stack entries per lane in
+// LDS (higher occupancy beats fewer overflow accesses)
+constexpr int kTraceRefillDefault = 56;
+constexpr int kShadowRefillDefault = 56;
+constexpr int kLdsEntriesDefault = 12;
 constexpr int kShadeBlock = 256;
 constexpr int kMaxGridBlocks = 256 * 32;
 
@@ -243,6 +248,7 @@ struct TraceStack
     E* lds;
     E* gbl;
     int32_t lds_entries;
+    int32_t lstride; // LDS entries of one lane are this far apart (the block size)
     int32_t gstride;
     int32_t cap;
 };
@@ -251,20 +257,36 @@ struct TraceStack
 template <bool ANY, bool COUNT>
 using StackEntry = typename std::conditional<ANY && !COUNT, int, int2>::type;
 
-// FULL: the whole stack fits in LDS (cap <= lds_entries), no overflow branch
-template <bool FULL, typename E>
+// Where a lane's traversal stack lives (the SK template parameter):
+//   kStackSplit    the first lds_entries entries in LDS, deeper ones in the
+//                  global overflow area
+//   kStackLds      the whole stack in LDS (cap <= lds_entries): no overflow branch
+//   kStackScratch  the whole stack in a per-lane private array (scratch
+//                  memory, cached by L1/L2), no LDS at all
+enum : int
+{
+    kStackSplit = 0,
+    kStackLds = 1,
+    kStackScratch = 2
+};
+constexpr int kScratchStackEntries = 64; // cap of the private-array stack
+
+template <int SK, typename E>
 BB_D void stack_put(const TraceStack<E>& s, int32_t sp, E v)
 {
-    if (FULL || sp < s.lds_entries)
-        s.lds[sp * kTraceBlock] = v;
+    if (SK == kStackScratch)
+        s.lds[sp] = v;
+    else if (SK == kStackLds || sp < s.lds_entries)
+        s.lds[sp * s.lstride] = v;
     else
         s.gbl[(sp - s.lds_entries) * s.gstride] = v;
 }
 
-template <bool FULL, typename E>
+template <int SK, typename E>
 BB_D E stack_get(const TraceStack<E>& s, int32_t sp)
 {
-    return (FULL || sp < s.lds_entries) ? s.lds[sp * kTraceBlock] : s.gbl[(sp - s.lds_entries) * s.gstride];
+    if (SK == kStackScratch) return s.lds[sp];
+    return (SK == kStackLds || sp < s.lds_entries) ? s.lds[sp * s.lstride] : s.gbl[(sp - s.lds_entries) * s.gstride];
 }
 
 // a far child whose t_max-independent test failed, kept on the stack only
@@ -317,7 +339,7 @@ BB_D QEnt qsel(bool c, QEnt a, QEnt b)
 // in the plane coordinates, NaN counting as ignored) and it fails whenever
 // the parent would.  Only the node-test count differs, so the counting
 // kernels keep the one-level records.
-template <bool ANY, bool FULL>
+template <bool ANY, int SK>
 BB_D bool quad_expand(const SceneView& S, Lane& L, const TraceStack<StackEntry<ANY, false>>& stk, int* err)
 {
     constexpr bool kSlim = std::is_same<StackEntry<ANY, false>, int>::value;
@@ -358,23 +380,23 @@ BB_D bool quad_expand(const SceneView& S, Lane& L, const TraceStack<StackEntry<A
     // pushed only when they pass with the current t_max (t_max only shrinks)
     if (box_pass(x3.h, L.t_max)) {
         if constexpr (kSlim)
-            stack_put<FULL>(stk, L.sp, x3.ref);
+            stack_put<SK>(stk, L.sp, x3.ref);
         else
-            stack_put<FULL>(stk, L.sp, make_int2(x3.ref, __float_as_int(x3.h.lo)));
+            stack_put<SK>(stk, L.sp, make_int2(x3.ref, __float_as_int(x3.h.lo)));
         ++L.sp;
     }
     if (box_pass(x2.h, L.t_max)) {
         if constexpr (kSlim)
-            stack_put<FULL>(stk, L.sp, x2.ref);
+            stack_put<SK>(stk, L.sp, x2.ref);
         else
-            stack_put<FULL>(stk, L.sp, make_int2(x2.ref, __float_as_int(x2.h.lo)));
+            stack_put<SK>(stk, L.sp, make_int2(x2.ref, __float_as_int(x2.h.lo)));
         ++L.sp;
     }
     if (box_pass(x1.h, L.t_max)) {
         if constexpr (kSlim)
-            stack_put<FULL>(stk, L.sp, x1.ref);
+            stack_put<SK>(stk, L.sp, x1.ref);
         else
-            stack_put<FULL>(stk, L.sp, make_int2(x1.ref, __float_as_int(x1.h.lo)));
+            stack_put<SK>(stk, L.sp, make_int2(x1.ref, __float_as_int(x1.h.lo)));
         ++L.sp;
     }
     if (box_pass(x0.h, L.t_max)) {
@@ -385,7 +407,7 @@ BB_D bool quad_expand(const SceneView& S, Lane& L, const TraceStack<StackEntry<A
     return false;
 }
 
-template <bool ANY, bool COUNT, bool FULL, bool ALPHA>
+template <bool ANY, bool COUNT, int SK, bool ALPHA>
 BB_D bool trace_step(const SceneView& S, Lane& L, const TraceStack<StackEntry<ANY, COUNT>>& stk, TraceCount& c,
                      int* err)
 {
@@ -393,7 +415,7 @@ BB_D bool trace_step(const SceneView& S, Lane& L, const TraceStack<StackEntry<AN
     if (!COUNT && L.cur >= 0) {
         // two-level records (no node counters kept)
         if constexpr (!COUNT) {
-            if (quad_expand<ANY, FULL>(S, L, stk, err)) return true;
+            if (quad_expand<ANY, SK>(S, L, stk, err)) return true;
             if (L.cur != kNoRef) return false;
         }
     } else if (L.cur >= 0) {
@@ -417,9 +439,9 @@ BB_D bool trace_step(const SceneView& S, Lane& L, const TraceStack<StackEntry<AN
                 return true;
             }
             if constexpr (kSlim)
-                stack_put<FULL>(stk, L.sp, far_ref);
+                stack_put<SK>(stk, L.sp, far_ref);
             else
-                stack_put<FULL>(stk, L.sp, make_int2(far_ref, hf.ok ? __float_as_int(hf.lo) : kFailedLo));
+                stack_put<SK>(stk, L.sp, make_int2(far_ref, hf.ok ? __float_as_int(hf.lo) : kFailedLo));
             ++L.sp;
         }
         if (COUNT) c.nodes++;
@@ -471,9 +493,9 @@ BB_D bool trace_step(const SceneView& S, Lane& L, const TraceStack<StackEntry<AN
     if (L.sp == 0) return true;
     --L.sp;
     if constexpr (kSlim) {
-        L.cur = stack_get<FULL>(stk, L.sp); // pushed only if it passed with this same t_max
+        L.cur = stack_get<SK>(stk, L.sp); // pushed only if it passed with this same t_max
     } else {
-        const int2 e = stack_get<FULL>(stk, L.sp);
+        const int2 e = stack_get<SK>(stk, L.sp);
         if (COUNT) c.nodes++;
         const bool pass = (!COUNT || e.y != kFailedLo) && !(__int_as_float(e.y) > L.t_max);
         L.cur = pass ? e.x : kNoRef;
@@ -486,7 +508,7 @@ BB_D bool trace_step(const SceneView& S, Lane& L, const TraceStack<StackEntry<AN
 // `refill_min` of its 64 lanes are idle (64: whole-wave batches, one atomic
 // per 64 queries).  Between refill checks every lane takes STEPS traversal
 // steps.  Src provides load(index, Lane&) and finish(index, const Lane&).
-template <bool ANY, bool COUNT, bool FULL, bool ALPHA, int STEPS, typename Src>
+template <bool ANY, bool COUNT, int SK, bool ALPHA, int STEPS, typename Src>
 BB_D void persistent_trace(const SceneView& S, const Src& src, int n, int* fetch,
                            const TraceStack<StackEntry<ANY, COUNT>>& stk, TraceCount& c, int* err, int refill_min)
 {
@@ -515,7 +537,7 @@ BB_D void persistent_trace(const SceneView& S, const Src& src, int n, int* fetch
         if (idle == ~0ull) break;
 #pragma unroll
         for (int k = 0; k < STEPS; ++k) {
-            if (active && trace_step<ANY, COUNT, FULL, ALPHA>(S, L, stk, c, err)) {
+            if (active && trace_step<ANY, COUNT, SK, ALPHA>(S, L, stk, c, err)) {
                 src.finish(id, L);
                 active = false;
             }
@@ -592,10 +614,25 @@ BB_D TraceStack<E> make_stack(void* lds_stack, int2* gstack, int lds_entries, in
 {
     TraceStack<E> t;
     t.lds = reinterpret_cast<E*>(lds_stack) + threadIdx.x;
+    t.lstride = blockDim.x;
     t.gstride = gridDim.x * blockDim.x;
     t.gbl = reinterpret_cast<E*>(gstack) + blockIdx.x * blockDim.x + threadIdx.x;
     t.lds_entries = lds_entries;
     t.cap = cap;
+    return t;
+}
+
+// the calling lane's stack in its private array (kStackScratch)
+template <typename E>
+BB_D TraceStack<E> make_scratch_stack(E* priv, int cap)
+{
+    TraceStack<E> t;
+    t.lds = priv;
+    t.lstride = 1;
+    t.gstride = 0;
+    t.gbl = nullptr;
+    t.lds_entries = kScratchStackEntries;
+    t.cap = cap < kScratchStackEntries ? cap : kScratchStackEntries;
     return t;
 }
 
@@ -608,7 +645,7 @@ constexpr int kTraceSteps = BB_TRACE_STEPS;
 #endif
 constexpr int kShadowSteps = BB_SHADOW_STEPS;
 
-template <bool ANY, bool COUNT, bool FULL, bool ALPHA>
+template <bool ANY, bool COUNT, int SK, bool ALPHA>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_batch(SceneView S, BatchSrc src, int n, int* fetch,
                                                                int2* gstack, int lds_entries, int refill_min,
                                                                unsigned long long* counters, int* err)
@@ -616,7 +653,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_batch(SceneView S, BatchS
     extern __shared__ int2 lds_stack[];
     const auto stk = make_stack<StackEntry<ANY, COUNT>>(lds_stack, gstack, lds_entries, trace_stack_depth(S, COUNT));
     TraceCount c{ 0, 0, 0 };
-    persistent_trace<ANY, COUNT, FULL, ALPHA, kTraceSteps>(S, src, n, fetch, stk, c, err, refill_min);
+    persistent_trace<ANY, COUNT, SK, ALPHA, kTraceSteps>(S, src, n, fetch, stk, c, err, refill_min);
     if (COUNT) wave_count_add(c, counters);
 }
 
@@ -811,27 +848,37 @@ struct ShadowSrc
     }
 };
 
-template <bool COUNT, bool FULL, bool ALPHA>
-__global__ __launch_bounds__(kTraceBlock) void k_trace_closest(SceneView S, const int* __restrict__ n_ptr, int* fetch,
-                                                                 ClosestSrc src, int2* gstack, int lds_entries, int refill_min,
-                                                                 unsigned long long* counters, int* err)
+// the closest-hit and shadow kernels of the bounce loop, for blocks of
+// BLOCK threads and the stack placement SK
+template <bool COUNT, int SK, bool ALPHA, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_trace_closest(SceneView S, const int* __restrict__ n_ptr, int* fetch,
+                                                         ClosestSrc src, int2* gstack, int lds_entries, int refill_min,
+                                                         unsigned long long* counters, int* err)
 {
+    using E = StackEntry<false, COUNT>;
     extern __shared__ int2 lds_stack[];
-    const auto stk = make_stack<StackEntry<false, COUNT>>(lds_stack, gstack, lds_entries, trace_stack_depth(S, COUNT));
+    E priv[SK == kStackScratch ? kScratchStackEntries : 1];
+    const auto stk = SK == kStackScratch
+                         ? make_scratch_stack<E>(priv, trace_stack_depth(S, COUNT))
+                         : make_stack<E>(lds_stack, gstack, lds_entries, trace_stack_depth(S, COUNT));
     TraceCount c{ 0, 0, 0 };
-    persistent_trace<false, COUNT, FULL, ALPHA, kTraceSteps>(S, src, *n_ptr, fetch, stk, c, err, refill_min);
+    persistent_trace<false, COUNT, SK, ALPHA, kTraceSteps>(S, src, *n_ptr, fetch, stk, c, err, refill_min);
     if (COUNT) wave_count_add(c, counters);
 }
 
-template <bool COUNT, bool FULL, bool ALPHA>
-__global__ __launch_bounds__(kTraceBlock) void k_trace_shadow(SceneView S, const int* __restrict__ n_ptr, int* fetch,
-                                                                ShadowSrc src, int2* gstack, int lds_entries, int refill_min,
-                                                                unsigned long long* counters, int* err)
+template <bool COUNT, int SK, bool ALPHA, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_trace_shadow(SceneView S, const int* __restrict__ n_ptr, int* fetch,
+                                                        ShadowSrc src, int2* gstack, int lds_entries, int refill_min,
+                                                        unsigned long long* counters, int* err)
 {
+    using E = StackEntry<true, COUNT>;
     extern __shared__ int2 lds_stack[];
-    const auto stk = make_stack<StackEntry<true, COUNT>>(lds_stack, gstack, lds_entries, trace_stack_depth(S, COUNT));
+    E priv[SK == kStackScratch ? kScratchStackEntries : 1];
+    const auto stk = SK == kStackScratch
+                         ? make_scratch_stack<E>(priv, trace_stack_depth(S, COUNT))
+                         : make_stack<E>(lds_stack, gstack, lds_entries, trace_stack_depth(S, COUNT));
     TraceCount c{ 0, 0, 0 };
-    persistent_trace<true, COUNT, FULL, ALPHA, kShadowSteps>(S, src, *n_ptr, fetch, stk, c, err, refill_min);
+    persistent_trace<true, COUNT, SK, ALPHA, kShadowSteps>(S, src, *n_ptr, fetch, stk, c, err, refill_min);
     if (COUNT) wave_count_add(c, counters + 3);
 }
 
@@ -1586,19 +1633,20 @@ struct EventPool
 // the whole stack up to 32 entries (deeper entries overflow to global memory)
 int lds_entries_for(const SceneView& V, int requested, bool count)
 {
-    int e = requested > 0 ? requested : 32;
+    // the counting kernels (measurement only) keep their whole one-level stack in LDS
+    int e = requested > 0 ? requested : (count ? 32 : kLdsEntriesDefault);
     return std::max(1, std::min(e, trace_stack_depth(V, count)));
 }
 
 // dynamic LDS of a traversal kernel: 4-byte entries for any-hit queries
 // without counters, 8-byte ones otherwise
-size_t trace_lds_bytes(int lds_entries, bool slim = false)
+size_t trace_lds_bytes(int lds_entries, bool slim = false, int block = kTraceBlock)
 {
-    return size_t(lds_entries) * kTraceBlock * (slim ? sizeof(int) : sizeof(int2));
+    return size_t(lds_entries) * size_t(block) * (slim ? sizeof(int) : sizeof(int2));
 }
 
 // persistent grid of a traversal kernel: resident blocks per CU x CUs
-int trace_grid(const void* kernel, size_t lds_bytes)
+int trace_grid(const void* kernel, size_t lds_bytes, int block = kTraceBlock)
 {
     static int cus = 0;
     if (!cus) {
@@ -1608,7 +1656,7 @@ int trace_grid(const void* kernel, size_t lds_bytes)
         if (cus <= 0) cus = 256;
     }
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kTraceBlock, lds_bytes) != hipSuccess || per_cu <= 0)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, lds_bytes) != hipSuccess || per_cu <= 0)
         per_cu = 4;
     return per_cu * cus;
 }
@@ -1622,8 +1670,9 @@ using ShadowKernel = void (*)(SceneView, const int*, int*, ShadowSrc, int2*, int
 template <bool ANY, bool ALPHA>
 BatchKernel batch_kernel_t(bool count, bool full)
 {
-    if (count) return full ? k_trace_batch<ANY, true, true, ALPHA> : k_trace_batch<ANY, true, false, ALPHA>;
-    return full ? k_trace_batch<ANY, false, true, ALPHA> : k_trace_batch<ANY, false, false, ALPHA>;
+    if (count)
+        return full ? k_trace_batch<ANY, true, kStackLds, ALPHA> : k_trace_batch<ANY, true, kStackSplit, ALPHA>;
+    return full ? k_trace_batch<ANY, false, kStackLds, ALPHA> : k_trace_batch<ANY, false, kStackSplit, ALPHA>;
 }
 
 BatchKernel batch_kernel(bool any, bool count, bool full, bool alpha)
@@ -1632,28 +1681,61 @@ BatchKernel batch_kernel(bool any, bool count, bool full, bool alpha)
     return any ? batch_kernel_t<true, false>(count, full) : batch_kernel_t<false, false>(count, full);
 }
 
-template <bool ALPHA>
-ClosestKernel closest_kernel_t(bool count, bool full)
+// The bounce loop's traversal variants: counters (measurement only: 128-thread
+// blocks, stack in LDS), stack placement SK, the alpha test compiled in only
+// for scenes that have alpha textures, and the block size (64, 128 or 256)
+template <bool COUNT, bool ALPHA>
+ClosestKernel closest_sel(int sk, int block)
 {
-    if (count) return full ? k_trace_closest<true, true, ALPHA> : k_trace_closest<true, false, ALPHA>;
-    return full ? k_trace_closest<false, true, ALPHA> : k_trace_closest<false, false, ALPHA>;
+    if constexpr (COUNT)
+        return sk == kStackLds ? k_trace_closest<true, kStackLds, ALPHA, kTraceBlock>
+                               : k_trace_closest<true, kStackSplit, ALPHA, kTraceBlock>;
+    else {
+        if (sk == kStackScratch)
+            return block == 64 ? k_trace_closest<false, kStackScratch, ALPHA, 64>
+                               : block == 256 ? k_trace_closest<false, kStackScratch, ALPHA, 256>
+                                              : k_trace_closest<false, kStackScratch, ALPHA, kTraceBlock>;
+        if (sk == kStackLds)
+            return block == 64 ? k_trace_closest<false, kStackLds, ALPHA, 64>
+                               : block == 256 ? k_trace_closest<false, kStackLds, ALPHA, 256>
+                                              : k_trace_closest<false, kStackLds, ALPHA, kTraceBlock>;
+        return block == 64 ? k_trace_closest<false, kStackSplit, ALPHA, 64>
+                           : block == 256 ? k_trace_closest<false, kStackSplit, ALPHA, 256>
+                                          : k_trace_closest<false, kStackSplit, ALPHA, kTraceBlock>;
+    }
 }
 
-ClosestKernel closest_kernel(bool count, bool full, bool alpha)
+ClosestKernel closest_kernel(bool count, int sk, bool alpha, int block)
 {
-    return alpha ? closest_kernel_t<true>(count, full) : closest_kernel_t<false>(count, full);
+    if (alpha) return count ? closest_sel<true, true>(sk, block) : closest_sel<false, true>(sk, block);
+    return count ? closest_sel<true, false>(sk, block) : closest_sel<false, false>(sk, block);
 }
 
-template <bool ALPHA>
-ShadowKernel shadow_kernel_t(bool count, bool full)
+template <bool COUNT, bool ALPHA>
+ShadowKernel shadow_sel(int sk, int block)
 {
-    if (count) return full ? k_trace_shadow<true, true, ALPHA> : k_trace_shadow<true, false, ALPHA>;
-    return full ? k_trace_shadow<false, true, ALPHA> : k_trace_shadow<false, false, ALPHA>;
+    if constexpr (COUNT)
+        return sk == kStackLds ? k_trace_shadow<true, kStackLds, ALPHA, kTraceBlock>
+                               : k_trace_shadow<true, kStackSplit, ALPHA, kTraceBlock>;
+    else {
+        if (sk == kStackScratch)
+            return block == 64 ? k_trace_shadow<false, kStackScratch, ALPHA, 64>
+                               : block == 256 ? k_trace_shadow<false, kStackScratch, ALPHA, 256>
+                                              : k_trace_shadow<false, kStackScratch, ALPHA, kTraceBlock>;
+        if (sk == kStackLds)
+            return block == 64 ? k_trace_shadow<false, kStackLds, ALPHA, 64>
+                               : block == 256 ? k_trace_shadow<false, kStackLds, ALPHA, 256>
+                                              : k_trace_shadow<false, kStackLds, ALPHA, kTraceBlock>;
+        return block == 64 ? k_trace_shadow<false, kStackSplit, ALPHA, 64>
+                           : block == 256 ? k_trace_shadow<false, kStackSplit, ALPHA, 256>
+                                          : k_trace_shadow<false, kStackSplit, ALPHA, kTraceBlock>;
+    }
 }
 
-ShadowKernel shadow_kernel(bool count, bool full, bool alpha)
+ShadowKernel shadow_kernel(bool count, int sk, bool alpha, int block)
 {
-    return alpha ? shadow_kernel_t<true>(count, full) : shadow_kernel_t<false>(count, full);
+    if (alpha) return count ? shadow_sel<true, true>(sk, block) : shadow_sel<false, true>(sk, block);
+    return count ? shadow_sel<true, false>(sk, block) : shadow_sel<false, false>(sk, block);
 }
 
 } // namespace
@@ -1978,24 +2060,43 @@ int render_impl(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams
     const bool timed = stats != nullptr;
     const bool count = timed && count_traversal;
     const int depth = trace_stack_depth(V, count);
-    const int lds_entries = lds_entries_for(V, p.trace_lds_stack, count);
-    const size_t lds = trace_lds_bytes(lds_entries);
-    const size_t lds_shadow = trace_lds_bytes(lds_entries, !count); // reference-only entries without counters
-    const int refill_min = p.trace_refill > 0 ? std::min(p.trace_refill, 64) : 48;
+    // traversal launch shape: block size, and where the stacks live
+    const int block = p.trace_block > 0 ? p.trace_block : kTraceBlock;
+    if (block != 64 && block != 128 && block != 256) {
+        bb::set_error("bbgpu_render: trace_block %d (64, 128 or 256)", p.trace_block);
+        return BBGPU_ERR_INVALID;
+    }
+    if (p.trace_stack != BB_STACK_LDS && p.trace_stack != BB_STACK_SCRATCH) {
+        bb::set_error("bbgpu_render: unknown trace_stack %d", p.trace_stack);
+        return BBGPU_ERR_INVALID;
+    }
+    const bool scratch_stack = p.trace_stack == BB_STACK_SCRATCH && !count;
+    if (scratch_stack && depth > kScratchStackEntries) {
+        bb::set_error("bbgpu_render: the scratch stack holds %d entries, this scene's traversal needs %d",
+                      kScratchStackEntries, depth);
+        return BBGPU_ERR_UNSUPPORTED;
+    }
+    const int tblock = count ? kTraceBlock : block; // the counting kernels run 128-thread blocks
+    const int lds_entries = scratch_stack ? 0 : lds_entries_for(V, p.trace_lds_stack, count);
+    const size_t lds = scratch_stack ? 0 : trace_lds_bytes(lds_entries, false, tblock);
+    // reference-only entries without counters
+    const size_t lds_shadow = scratch_stack ? 0 : trace_lds_bytes(lds_entries, !count, tblock);
+    const int refill_min = p.trace_refill > 0 ? std::min(p.trace_refill, 64) : kTraceRefillDefault;
     const int refill_shadow = p.shadow_refill > 0 ? std::min(p.shadow_refill, 64) : kShadowRefillDefault;
-    const bool full = depth <= lds_entries;
-    const ClosestKernel k_closest = closest_kernel(count, full, V.has_alpha != 0);
+    const int sk = scratch_stack ? kStackScratch : (depth <= lds_entries ? kStackLds : kStackSplit);
+    const ClosestKernel k_closest = closest_kernel(count, sk, V.has_alpha != 0, tblock);
     // shading with the texture code only for scenes that use textures
     void (*const k_shade_fn)(SceneView, RenderConsts, PathBufs, int, CacheBufs) =
         cache ? (V.has_textures ? k_shade<true, true> : k_shade<true, false>)
               : (V.has_textures ? k_shade<false, true> : k_shade<false, false>);
-    const ShadowKernel k_shadow = shadow_kernel(count, full, V.has_alpha != 0);
-    const int g_trace = std::min(trace_grid(reinterpret_cast<const void*>(k_closest), lds), grid_for(cap, kTraceBlock));
+    const ShadowKernel k_shadow = shadow_kernel(count, sk, V.has_alpha != 0, tblock);
+    const int g_trace =
+        std::min(trace_grid(reinterpret_cast<const void*>(k_closest), lds, tblock), grid_for(cap, tblock));
     const int g_shadow =
-        std::min(trace_grid(reinterpret_cast<const void*>(k_shadow), lds_shadow), grid_for(cap, kTraceBlock));
+        std::min(trace_grid(reinterpret_cast<const void*>(k_shadow), lds_shadow, tblock), grid_for(cap, tblock));
     // the overflow area serves both grids (int2-sized entries hold the 4-byte ones too)
-    if (!ensure_gstack(x->ws, size_t(std::max(0, depth - lds_entries)) * size_t(std::max(g_trace, g_shadow)) *
-                                  kTraceBlock)) {
+    const size_t overflow = sk == kStackSplit ? size_t(depth - lds_entries) : 0;
+    if (!ensure_gstack(x->ws, overflow * size_t(std::max(g_trace, g_shadow)) * size_t(tblock))) {
         bb::set_error("bbgpu_render: traversal stack allocation failed");
         return BBGPU_ERR_NOMEM;
     }
@@ -2146,7 +2247,7 @@ int render_impl(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams
             const int in = depth & 1;
             a = timed ? mark() : nullptr;
             const ClosestSrc csrc{ P.ray[in], P.hit };
-            hipLaunchKernelGGL(k_closest, dim3(g_trace), dim3(kTraceBlock), lds, ls, V, P.counts + 4 * depth,
+            hipLaunchKernelGGL(k_closest, dim3(g_trace), dim3(tblock), lds, ls, V, P.counts + 4 * depth,
                                P.counts + 4 * depth + 2, csrc, P.gstack, lds_entries, refill_min, P.counters, P.err);
             hipEvent_t b = timed ? mark() : nullptr;
             if (timed) spans.push_back({ a, b, 0 });
@@ -2154,7 +2255,7 @@ int render_impl(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams
             hipEvent_t c2 = timed ? mark() : nullptr;
             if (timed) spans.push_back({ b, c2, 2 });
             const ShadowSrc ssrc{ P.sray, P.scontrib, P.L };
-            hipLaunchKernelGGL(k_shadow, dim3(g_shadow), dim3(kTraceBlock), lds_shadow, ls, V, P.counts + 4 * depth + 1,
+            hipLaunchKernelGGL(k_shadow, dim3(g_shadow), dim3(tblock), lds_shadow, ls, V, P.counts + 4 * depth + 1,
                                P.counts + 4 * depth + 3, ssrc, P.gstack, lds_entries, refill_shadow, P.counters, P.err);
             if (timed) spans.push_back({ c2, mark(), 1 });
         }
@@ -2459,17 +2560,30 @@ int bbgpu_autotune(BbScene* scene, const BbCameraDesc* camera, const BbRenderPar
     double best_ms = 0.0;
     int rc = time_of(best, best_ms); // warm-up, then the baseline
     if (rc == BBGPU_OK) rc = time_of(best, best_ms);
-    const int refills[] = { 32, 48, 64 };
-    const int lds[] = { 16, 24, 32 };
+    // one knob at a time, keeping the best so far: refill threshold, block
+    // size, stack placement (LDS entries, or the private scratch stack),
+    // paths per pass
+    const int refills[] = { 48, 56, 60 };
+    const int blocks[] = { 64, 128, 256 };
+    const int lds[] = { 8, 16, 24, -1 }; // -1: scratch stack
     const long long paths[] = { 1ll << 22, 1ll << 24, 1ll << 26 };
-    for (int knob = 0; knob < 3 && rc == BBGPU_OK; ++knob)
-        for (int v = 0; v < 3 && rc == BBGPU_OK; ++v) {
+    const int n_values[4] = { 3, 3, 4, 3 };
+    for (int knob = 0; knob < 4 && rc == BBGPU_OK; ++knob)
+        for (int v = 0; v < n_values[knob] && rc == BBGPU_OK; ++v) {
             BbRenderParams q = best;
-            if (knob == 0) q.trace_refill = refills[v];
-            if (knob == 1) q.trace_lds_stack = lds[v];
-            if (knob == 2) q.wavefront_paths = int32_t(std::min<long long>(paths[v], 1ll << 30));
+            if (knob == 0) q.trace_refill = q.shadow_refill = refills[v];
+            if (knob == 1) q.trace_block = blocks[v];
+            if (knob == 2) {
+                q.trace_stack = lds[v] < 0 ? BB_STACK_SCRATCH : BB_STACK_LDS;
+                q.trace_lds_stack = lds[v] < 0 ? 0 : lds[v];
+            }
+            if (knob == 3) q.wavefront_paths = int32_t(std::min<long long>(paths[v], 1ll << 30));
             double ms = 0.0;
             rc = time_of(q, ms);
+            if (rc == BBGPU_ERR_UNSUPPORTED) { // a variant this scene cannot use
+                rc = BBGPU_OK;
+                continue;
+            }
             if (rc == BBGPU_OK && ms < best_ms) {
                 best_ms = ms;
                 best = q;
@@ -2480,6 +2594,9 @@ int bbgpu_autotune(BbScene* scene, const BbCameraDesc* camera, const BbRenderPar
     if (rc != BBGPU_OK) return rc;
     *tuned_out = *params;
     tuned_out->trace_refill = best.trace_refill;
+    tuned_out->shadow_refill = best.shadow_refill;
+    tuned_out->trace_block = best.trace_block;
+    tuned_out->trace_stack = best.trace_stack;
     tuned_out->trace_lds_stack = best.trace_lds_stack;
     tuned_out->wavefront_paths = best.wavefront_paths;
     return BBGPU_OK;

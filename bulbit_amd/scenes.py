@@ -730,7 +730,7 @@ LIGHTS_POWER, LIGHTS_ALIAS = 0, 1  # BbRenderParams.light_sampler
 
 
 def render_params(spp, max_bounces=8, seed=0, rr_min_bounces=1, sample_begin=0, wavefront_paths=0, trace_lds_stack=0,
-                  trace_refill=0, shadow_refill=0, graph=0, light_sampler=LIGHTS_POWER):
+                  trace_refill=0, shadow_refill=0, graph=0, light_sampler=LIGHTS_POWER, trace_block=0, trace_stack=0):
     p = capi.BbRenderParams()
     p.spp = spp
     p.seed = seed
@@ -743,6 +743,8 @@ def render_params(spp, max_bounces=8, seed=0, rr_min_bounces=1, sample_begin=0, 
     p.shadow_refill = shadow_refill
     p.sample_begin = sample_begin
     p.light_sampler = light_sampler
+    p.trace_block = trace_block
+    p.trace_stack = trace_stack
     p.graph = graph
     return p
 

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define BBGPU_ABI_VERSION 17
+#define BBGPU_ABI_VERSION 18
 
 /* status codes */
 enum {
@@ -263,13 +263,17 @@ typedef struct BbRenderParams {
     int32_t regularize_bsdf;  /* only 0 is supported */
     int32_t wavefront_paths;  /* paths in flight per pass (0 = auto: as many whole samples per pixel as an
                                  eighth of the device memory holds in path state, at least 2^22 paths) */
-    int32_t trace_lds_stack;  /* traversal stack entries per lane kept in LDS (0 = auto) */
+    int32_t trace_lds_stack;  /* traversal stack entries per lane kept in LDS (0 = auto: 12) */
     int32_t sample_begin;     /* render samples [sample_begin, sample_begin+spp) */
-    int32_t trace_refill;     /* a closest-hit traversal wave takes new rays once this many of its lanes are idle (0 = 48) */
+    int32_t trace_refill;     /* a closest-hit traversal wave takes new rays once this many of its lanes are idle (0 = 56) */
     int32_t shadow_refill;    /* the same for shadow rays (0 = trace_refill's default) */
     int32_t graph;            /* 1: launch the render as a HIP graph, captured on first use and kept with the
                                  scene for calls that differ only in sample_begin (speed only; not with stats
                                  or the radiance cache) */
+    int32_t trace_block;      /* threads per block of the traversal kernels: 64, 128 or 256 (0 = 128) */
+    int32_t trace_stack;      /* BB_STACK_LDS (0): stacks in LDS, deeper entries in a global overflow area;
+                                 BB_STACK_SCRATCH: each lane's stack in a private scratch array of 64 entries
+                                 (no LDS; scenes whose traversal needs more are refused) */
     int32_t light_sampler;    /* BB_LIGHTS_POWER (0): PowerLightSampler's CDF search, as the reference;
                                  BB_LIGHTS_ALIAS: an alias table over the same powers (opt-in: same
                                  probabilities and estimator, a different draw-to-light map, so images
@@ -277,6 +281,7 @@ typedef struct BbRenderParams {
 } BbRenderParams;
 
 enum { BB_LIGHTS_POWER = 0, BB_LIGHTS_ALIAS = 1 };
+enum { BB_STACK_LDS = 0, BB_STACK_SCRATCH = 1 };
 
 /* Per-render statistics, filled when a pointer is passed to bbgpu_render.
  * Kernel times come from HIP events recorded around each launch on the
@@ -413,9 +418,11 @@ typedef struct BbBatchStats {
  * when all jobs are done; stats may be NULL. */
 int bbgpu_render_batch(BbRenderJob* jobs, int32_t n_jobs, int32_t streams, BbBatchStats* stats);
 
-/* Picks trace_refill, trace_lds_stack and wavefront_paths for a job by timing
- * short renders of probe_spp samples (coordinate descent over 3 values each);
- * the knobs change speed, never results.  tuned_out = *params with them set. */
+/* Picks the traversal refill thresholds, block size (64, 128, 256), stack
+ * placement (8, 16 or 24 LDS entries, or the scratch stack) and
+ * wavefront_paths for a job by timing short renders of probe_spp samples
+ * (coordinate descent, one knob at a time); the knobs change speed, never
+ * results.  tuned_out = *params with them set. */
 int bbgpu_autotune(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams* params, int32_t probe_spp,
                    BbRenderParams* tuned_out);
 

@@ -55,11 +55,33 @@ def test_autotune_keeps_the_image():
     p = scenes.render_params(4)
     t = autotune(g, cam, p)
     assert t.spp == p.spp and t.max_bounces == p.max_bounces and t.seed == p.seed
-    assert t.trace_refill in (0, 32, 48, 64) and t.trace_lds_stack in (0, 16, 24, 32)
+    assert t.trace_refill in (0, 48, 56, 60) and t.trace_lds_stack in (0, 8, 16, 24)
+    assert t.trace_block in (0, 64, 128, 256) and t.trace_stack in (0, 1)
     s0, c0, _ = g.render(cam, p)
     s1, c1, _ = g.render(cam, t)
     assert np.array_equal(c0, c1) and np.array_equal(s0.view(np.int32), s1.view(np.int32))
-    print(f"[batch] tuned refill {t.trace_refill}, lds {t.trace_lds_stack}, paths {t.wavefront_paths}", flush=True)
+    print(f"[batch] tuned refill {t.trace_refill}, block {t.trace_block}, stack {t.trace_stack}/{t.trace_lds_stack}, "
+          f"paths {t.wavefront_paths}", flush=True)
+
+
+@pytest.mark.parametrize("block", [64, 128, 256])
+@pytest.mark.parametrize("stack", ["lds8", "lds32", "scratch"])
+def test_traversal_launch_variants_give_the_same_image(block, stack):
+    """Block size and stack placement of the traversal kernels (the
+    autotuner's knobs) change nothing in the image, bit for bit, on a scene
+    with all the material kinds of the icosphere generator and on the
+    README scene (a sphere primitive)."""
+    from bulbit_amd.gpu import GpuScene
+
+    for sc, cam in (scenes.icosphere_scene(width=160, height=90, level=6, n_lights=16), scenes.readme_scene(96, 54)):
+        g = GpuScene(sc)
+        p = scenes.render_params(4)
+        s0, c0, _ = g.render(cam, p)
+        q = scenes.render_params(4, trace_block=block, trace_stack=1 if stack == "scratch" else 0,
+                                 trace_lds_stack=0 if stack == "scratch" else int(stack[3:]))
+        s1, c1, _ = g.render(cam, q)
+        assert np.array_equal(c0, c1) and np.array_equal(s0.view(np.int32), s1.view(np.int32))
+        g.close()
 
 
 def test_graph_renders_equal_eager_renders():
