@@ -163,6 +163,7 @@ struct SceneView
     const float2* uvs;       // per vertex, null when the scene has none (uv = (0, 0))
     int32_t has_alpha;       // some triangle's material has an alpha texture
     int32_t has_textures;    // some material has a textured parameter or a normal map
+    int32_t kind_mask;       // bit MK_* set for every BxDF kind the scene's materials use
     int32_t n_lights;
     int32_t n_inf;
     float func_integral;

@@ -945,39 +945,70 @@ BB_D bool dielectric_ms_sample(const bxdf& b, f3 wo, float u0, f2 u12, bsample& 
 }
 
 // BSDF::f / BSDF::PDF on local directions (bsdf.h:75-99)
+// Kind masks: a shading kernel built for the BxDF kinds of bit mask KM only
+// (the kinds a scene's materials use) leaves the other BxDFs' code out and
+// needs fewer registers.  kAllKinds keeps every BxDF.
+constexpr int kAllKinds = -1;
+template <int KM, int K>
+constexpr bool kind_on()
+{
+    return KM == kAllKinds || ((KM >> K) & 1) != 0;
+}
+// a kind outside the mask never reaches here (the host picks the kernel from
+// the scene's materials)
+#define BB_KIND_CASE(K, ...)                     \
+    case K:                                      \
+        if constexpr (kind_on<KM, K>()) {        \
+            __VA_ARGS__;                         \
+        } else {                                 \
+            __builtin_unreachable();             \
+        }
+
+template <int KM = kAllKinds>
 BB_D spec bxdf_f(const bxdf& b, f3 wo, f3 wi)
 {
     switch (b.kind) {
-    case MK_LAMBERT: return lambert_f(b, wo, wi);
-    case MK_EON: return eon_f(b, wo, wi);
-    case MK_DIELECTRIC: return dielectric_f(b, wo, wi);
-    case MK_DIELECTRIC_MS: return dielectric_ms_f(b, wo, wi);
-    case MK_CONDUCTOR: return conductor_f(b, wo, wi);
-    case MK_CONDUCTOR_MS: return conductor_ms_f(b, wo, wi);
-    case MK_MIRROR: return mks(0.0f); // specular_reflection_bxdf.cpp:7-13
-    default: return mr_f(b, wo, wi);
+        BB_KIND_CASE(MK_LAMBERT, return lambert_f(b, wo, wi))
+        BB_KIND_CASE(MK_EON, return eon_f(b, wo, wi))
+        BB_KIND_CASE(MK_DIELECTRIC, return dielectric_f(b, wo, wi))
+        BB_KIND_CASE(MK_DIELECTRIC_MS, return dielectric_ms_f(b, wo, wi))
+        BB_KIND_CASE(MK_CONDUCTOR, return conductor_f(b, wo, wi))
+        BB_KIND_CASE(MK_CONDUCTOR_MS, return conductor_ms_f(b, wo, wi))
+        BB_KIND_CASE(MK_MIRROR, return mks(0.0f)) // specular_reflection_bxdf.cpp:7-13
+    default:
+        if constexpr (kind_on<KM, MK_METALLIC_ROUGHNESS>())
+            return mr_f(b, wo, wi);
+        else
+            __builtin_unreachable();
     }
 }
+template <int KM = kAllKinds>
 BB_D float bxdf_pdf(const bxdf& b, f3 wo, f3 wi)
 {
     switch (b.kind) {
-    case MK_LAMBERT: return lambert_pdf(wo, wi);
-    case MK_EON: return eon_pdf(b, wo, wi);
-    case MK_DIELECTRIC: return dielectric_pdf(b, wo, wi);
-    case MK_DIELECTRIC_MS: return dielectric_ms_pdf(b, wo, wi);
-    case MK_CONDUCTOR: return conductor_pdf(b, wo, wi);
-    case MK_CONDUCTOR_MS: return conductor_ms_pdf(b, wo, wi);
-    case MK_MIRROR: return 0.0f;
-    default: return mr_pdf(b, wo, wi);
+        BB_KIND_CASE(MK_LAMBERT, return lambert_pdf(wo, wi))
+        BB_KIND_CASE(MK_EON, return eon_pdf(b, wo, wi))
+        BB_KIND_CASE(MK_DIELECTRIC, return dielectric_pdf(b, wo, wi))
+        BB_KIND_CASE(MK_DIELECTRIC_MS, return dielectric_ms_pdf(b, wo, wi))
+        BB_KIND_CASE(MK_CONDUCTOR, return conductor_pdf(b, wo, wi))
+        BB_KIND_CASE(MK_CONDUCTOR_MS, return conductor_ms_pdf(b, wo, wi))
+        BB_KIND_CASE(MK_MIRROR, return 0.0f)
+    default:
+        if constexpr (kind_on<KM, MK_METALLIC_ROUGHNESS>())
+            return mr_pdf(b, wo, wi);
+        else
+            __builtin_unreachable();
     }
 }
 
 // BxDF::Sample_f on local wo (returns false for a failed sample)
+template <int KM = kAllKinds>
 BB_D bool bxdf_sample(const bxdf& b, f3 wo, float u0, f2 u12, bsample& s)
 {
     s.eta = 1.0f;
     switch (b.kind) {
     case MK_LAMBERT: {
+        if constexpr (!kind_on<KM, MK_LAMBERT>()) __builtin_unreachable();
         f3 wi = sample_cosine_hemisphere(u12);
         float pdf = wi.z * kInvPi;
         if (wo.z < 0.0f) wi.z = -wi.z;
@@ -987,8 +1018,11 @@ BB_D bool bxdf_sample(const bxdf& b, f3 wo, float u0, f2 u12, bsample& s)
         s.flags = BX_DIFFUSE | BX_REFLECTION;
         return true;
     }
-    case MK_EON: return eon_sample(b, wo, u0, u12, s);
+    case MK_EON:
+        if constexpr (!kind_on<KM, MK_EON>()) __builtin_unreachable();
+        return eon_sample(b, wo, u0, u12, s);
     case MK_MIRROR: {
+        if constexpr (!kind_on<KM, MK_MIRROR>()) __builtin_unreachable();
         // specular_reflection_bxdf.cpp:24-38: wo mirrored about the normal, weight r / |cos|
         const f3 wi = mk3(-wo.x, -wo.y, wo.z);
         s.f = b.r / fabsf(wi.z);
@@ -997,8 +1031,11 @@ BB_D bool bxdf_sample(const bxdf& b, f3 wo, float u0, f2 u12, bsample& s)
         s.flags = BX_SPECULAR | BX_REFLECTION;
         return true;
     }
-    case MK_DIELECTRIC_MS: return dielectric_ms_sample(b, wo, u0, u12, s);
+    case MK_DIELECTRIC_MS:
+        if constexpr (!kind_on<KM, MK_DIELECTRIC_MS>()) __builtin_unreachable();
+        return dielectric_ms_sample(b, wo, u0, u12, s);
     case MK_CONDUCTOR_MS:
+        if constexpr (!kind_on<KM, MK_CONDUCTOR_MS>()) __builtin_unreachable();
         if (!ggx_smooth(b.mf)) {
             if (wo.z == 0.0f) return false;
             f3 wm, wi;
@@ -1029,6 +1066,7 @@ BB_D bool bxdf_sample(const bxdf& b, f3 wo, float u0, f2 u12, bsample& s)
         }
         [[fallthrough]]; // effectively smooth: the perfect specular conductor
     case MK_CONDUCTOR: {
+        if constexpr (!kind_on<KM, MK_CONDUCTOR>() && !kind_on<KM, MK_CONDUCTOR_MS>()) __builtin_unreachable(); // also the smooth compensated conductor
         if (ggx_smooth(b.mf)) {
             f3 wi = mk3(-wo.x, -wo.y, wo.z);
             spec f = fresnel_complex_s(abs_cos_theta(wi), b.ceta, b.ck) / abs_cos_theta(wi);
@@ -1054,6 +1092,7 @@ BB_D bool bxdf_sample(const bxdf& b, f3 wo, float u0, f2 u12, bsample& s)
         return true;
     }
     case MK_DIELECTRIC: {
+        if constexpr (!kind_on<KM, MK_DIELECTRIC>()) __builtin_unreachable();
         if (b.eta == 1.0f || ggx_smooth(b.mf)) {
             float R = fresnel_dielectric(wo.z, b.eta);
             float T = 1.0f - R;
@@ -1111,6 +1150,7 @@ BB_D bool bxdf_sample(const bxdf& b, f3 wo, float u0, f2 u12, bsample& s)
         return true;
     }
     default: {
+        if constexpr (!kind_on<KM, MK_METALLIC_ROUGHNESS>()) __builtin_unreachable();
         if (wo.z == 0.0f) return false;
         f3 wm = ggx_sample_wm(b.mf, wo, u12);
         f3 wi = reflect(wo, wm);
@@ -1149,13 +1189,14 @@ BB_D bool bxdf_sample(const bxdf& b, f3 wo, float u0, f2 u12, bsample& s)
 // ------------------------------------------------------------------- lights
 
 // Distribution1D::SampleDiscrete + FindInterval (sampling.h:383-397, math.h:117-135)
-BB_D int32_t light_select(const SceneView& S, float u, float& pmf)
+// cdf: the light CDF (S.light_cdf, or a copy of it in LDS)
+BB_D int32_t light_select(const SceneView& S, float u, float& pmf, const float* cdf)
 {
     int32_t size = S.n_lights + 1;
     int32_t first = 0, len = size;
     while (len > 0) {
         int32_t half = len >> 1, middle = first + half;
-        if (S.light_cdf[middle] <= u) {
+        if (cdf[middle] <= u) {
             first = middle + 1;
             len -= half + 1;
         } else {

@@ -881,6 +881,8 @@ bool create_scene(const BbSceneDesc& d, BbScene* s, std::string& err)
     V.has_alpha = 0;
     for (int32_t j = 0; j < ns; ++j)
         if (f2i(tris[3 * size_t(j) + 2].w) & kSlotAlpha) V.has_alpha = 1;
+    V.kind_mask = 0;
+    for (const GpuMaterial& g : mats) V.kind_mask |= 1 << g.kind;
     V.has_textures = std::any_of(mats.begin(), mats.end(),
                                  [](const GpuMaterial& g) { return g.textured || g.tex_normal >= 0; }) ? 1 : 0;
     V.n_lights = nl;

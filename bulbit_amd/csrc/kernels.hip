@@ -39,12 +39,18 @@ namespace bb
 
 constexpr int kTraceBlock = 128;
 // defaults measured on the 1080p 64-spp frame (one pass per frame): refill a
-// wave once 56 of its 64 lanes are idle; keep 12 stack entries per lane in
-// LDS (higher occupancy beats fewer overflow accesses)
-constexpr int kTraceRefillDefault = 56;
-constexpr int kShadowRefillDefault = 56;
+// wave once 32 of its 64 lanes are idle (from its reservoir: no atomic); keep
+// 12 stack entries per lane in LDS (higher occupancy beats fewer overflow
+// accesses)
+constexpr int kTraceRefillDefault = 32;
+constexpr int kShadowRefillDefault = 32;
 constexpr int kLdsEntriesDefault = 12;
 constexpr int kShadeBlock = 256;
+#ifndef BB_RESERVOIR
+#define BB_RESERVOIR 128
+#endif
+constexpr int kReservoir = BB_RESERVOIR; // queries a traversal wave claims per atomic
+constexpr int kLdsCdfEntries = 2049; // lights whose CDF the shading kernel keeps in LDS (up to 2048)
 constexpr int kMaxGridBlocks = 256 * 32;
 
 // ----------------------------------------------------------------- helpers
@@ -517,24 +523,35 @@ BB_D void persistent_trace(const SceneView& S, const Src& src, int n, int* fetch
     Lane L;
     bool active = false, more = true;
     int id = 0;
+    // the wave's reservoir of claimed queries [rb, re): claimed kReservoir
+    // at a time, so the shared counter sees one atomic per kReservoir
+    // queries instead of one per refill (same-address atomics limit the rate)
+    int rb = 0, re = 0;
     for (;;) {
         unsigned long long idle = __ballot(active ? 0 : 1);
-        if (more && (__popcll(idle) >= refill_min || idle == ~0ull)) {
-            const int leader = __builtin_ctzll(idle);
-            int base = 0;
-            if (lane == leader) base = atomicAdd(fetch, __popcll(idle));
-            base = __shfl(base, leader);
-            if (base >= n) more = false;
-            const int mine = base + __popcll(idle & below);
-            if (!active && mine < n) {
+        if ((more || rb < re) && (__popcll(idle) >= refill_min || idle == ~0ull)) {
+            if (rb >= re) {
+                int base = 0;
+                if (lane == 0) base = atomicAdd(fetch, kReservoir);
+                base = __shfl(base, 0);
+                rb = base;
+                re = base + kReservoir < n ? base + kReservoir : n;
+                if (base + kReservoir >= n) more = false;
+            }
+            const int rank = __popcll(idle & below);
+            const int mine = rb + rank;
+            if (!active && mine < re) {
                 id = mine;
                 src.load(id, L);
                 lane_root<COUNT>(S, L, c);
                 active = true;
             }
+            const int avail = re - rb, need = __popcll(idle);
+            rb += need < avail ? need : avail;
             idle = __ballot(active ? 0 : 1);
         }
-        if (idle == ~0ull) break;
+        if (idle == ~0ull && !more && rb >= re) break;
+        if (idle == ~0ull) continue;
 #pragma unroll
         for (int k = 0; k < STEPS; ++k) {
             if (active && trace_step<ANY, COUNT, SK, ALPHA>(S, L, stk, c, err)) {
@@ -558,6 +575,39 @@ BB_D int wave_alloc(bool pred, int* counter)
     base = __shfl(base, leader);
     const unsigned long long below = (lane == 0) ? 0ull : (mask & (~0ull >> (64 - lane)));
     return pred ? base + __popcll(below) : -1;
+}
+
+// Block-level compaction into two queues at once: one atomic per queue per
+// block instead of per wave (same-address atomics were the shading kernel's
+// limit).  Every thread of the block must call it; s holds
+// 2 * (waves per block) + 2 ints of LDS.  Slots: a = this thread's place in
+// the queue counted by ca (when pa), b likewise.
+BB_D void block_alloc2(bool pa, int* ca, int& a, bool pb, int* cb, int& b, int* s)
+{
+    const int nw = blockDim.x >> 6, wave = threadIdx.x >> 6, lane = __lane_id();
+    const unsigned long long ma = __ballot(pa ? 1 : 0), mb = __ballot(pb ? 1 : 0);
+    if (lane == 0) {
+        s[wave] = __popcll(ma);
+        s[nw + wave] = __popcll(mb);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ta = 0, tb = 0;
+        for (int w = 0; w < nw; ++w) {
+            const int x = s[w], y = s[nw + w];
+            s[w] = ta;
+            s[nw + w] = tb;
+            ta += x;
+            tb += y;
+        }
+        s[2 * nw] = ta ? atomicAdd(ca, ta) : 0;
+        s[2 * nw + 1] = tb ? atomicAdd(cb, tb) : 0;
+    }
+    __syncthreads();
+    const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    a = pa ? s[2 * nw] + s[wave] + __popcll(ma & below) : -1;
+    b = pb ? s[2 * nw + 1] + s[nw + wave] + __popcll(mb & below) : -1;
+    __syncthreads(); // s is reused by the next call
 }
 
 BB_D void wave_count_add(TraceCount c, unsigned long long* out)
@@ -980,13 +1030,24 @@ BB_D void cache_features(const CacheBufs& C, const GpuMaterial& m, f3 p, f3 wo, 
     r[13] = r[14] = r[15] = 0.0f;
 }
 
-template <bool CACHE, bool TEX>
+// KM: the BxDF kinds compiled in (the scene's materials' kinds, or all)
+template <bool CACHE, bool TEX, int KM>
 __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView S, RenderConsts rc, PathBufs P, int depth, CacheBufs C)
 {
     const int in = depth & 1, out = in ^ 1;
     const int n = P.counts[4 * depth];
     int* next_count = &P.counts[4 * (depth + 1)];
     int* shadow_count = &P.counts[4 * depth + 1];
+    // the light CDF in LDS for the binary search of every NEE sample (the
+    // same comparisons on the same values; large light sets stay in global memory)
+    __shared__ float s_cdf[kLdsCdfEntries];
+    const bool cdf_lds = S.n_lights + 1 <= kLdsCdfEntries;
+    if (cdf_lds) {
+        for (int i = threadIdx.x; i < S.n_lights + 1; i += blockDim.x) s_cdf[i] = S.light_cdf[i];
+        __syncthreads();
+    }
+    const float* cdf = cdf_lds ? s_cdf : S.light_cdf;
+    __shared__ int s_alloc[2 * (kShadeBlock / 64) + 2];
     for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
         const int k = base + threadIdx.x;
         bool cont = false, shadow = false;
@@ -1143,14 +1204,14 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView S, RenderConsts
                             if (S.n_lights > 0) {
                                 float pmf;
                                 const int32_t li =
-                                    rc.light_alias ? light_select_alias(S, lu0, pmf) : light_select(S, lu0, pmf);
+                                    rc.light_alias ? light_select_alias(S, lu0, pmf) : light_select(S, lu0, pmf, cdf);
                                 const GpuLight& Lt = S.lights[li];
                                 light_sample ls;
                                 if (light_sample_li(Lt, sh.point, lu12, ls)) {
                                     const f3 wi_l = to_local(fr, ls.wi);
-                                    const float bsdf_pdf = (wo_l.z == 0.0f) ? 0.0f : bxdf_pdf(b, wo_l, wi_l);
+                                    const float bsdf_pdf = (wo_l.z == 0.0f) ? 0.0f : bxdf_pdf<KM>(b, wo_l, wi_l);
                                     if (!(is_black(ls.Li) || bsdf_pdf == 0.0f)) {
-                                        const spec f = (wo_l.z == 0.0f) ? mks(0.0f) : bxdf_f(b, wo_l, wi_l);
+                                        const spec f = (wo_l.z == 0.0f) ? mks(0.0f) : bxdf_f<KM>(b, wo_l, wi_l);
                                         const float light_pdf = pmf * ls.pdf;
                                         const spec f_cos = f * absdot(sh.ns, ls.wi);
                                         if (Lt.type == BB_LIGHT_SPOT || Lt.type == BB_LIGHT_POINT ||
@@ -1176,7 +1237,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView S, RenderConsts
                         bsample s;
                         // at a query vertex with direct light the cache stands in for the continuation
                         bool ok = !nee_only && wo_l.z != 0.0f && (b.flags & (BX_REFLECTION | BX_TRANSMISSION)) &&
-                                  bxdf_sample(b, wo_l, u0, u12, s);
+                                  bxdf_sample<KM>(b, wo_l, u0, u12, s);
                         if (ok) ok = !(is_black(s.f) || s.pdf == 0.0f || s.wi.z == 0.0f);
                         if (ok) {
                             const f3 wi = from_local(fr, s.wi);
@@ -1206,13 +1267,13 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView S, RenderConsts
             P.rng[pid] = st;
             P.flags[pid] = (bounce & 0xffff) | (specular ? kFlagSpecular : 0) | (fl & (kFlagTraining | (0xff << kStopShift)));
         }
-        const int ss = wave_alloc(shadow, shadow_count);
+        int ss = 0, ns = 0;
+        block_alloc2(shadow, shadow_count, ss, cont, next_count, ns, s_alloc);
         if (shadow) {
             P.sray[2 * ss] = make_float4(sh_o.x, sh_o.y, sh_o.z, sh_tmax);
             P.sray[2 * ss + 1] = make_float4(sh_d.x, sh_d.y, sh_d.z, __int_as_float(pid));
             P.scontrib[ss] = make_float4(sh_c.r, sh_c.g, sh_c.b, 0.0f);
         }
-        const int ns = wave_alloc(cont, next_count);
         if (cont) {
             P.queue[out][ns] = pid;
             P.ray[out][2 * ns] = make_float4(new_o.x, new_o.y, new_o.z, 0.0f);
@@ -1661,6 +1722,28 @@ int trace_grid(const void* kernel, size_t lds_bytes, int block = kTraceBlock)
     return per_cu * cus;
 }
 
+// Shading kernel for a scene: the texture code only for scenes that use
+// textures, and the BxDF code of the scene's material kinds only (two
+// common sets: Lambertian + conductor + dielectric, and metallic-roughness +
+// Lambertian), so the common kernels need fewer registers
+constexpr int kKindsLcd = (1 << MK_LAMBERT) | (1 << MK_CONDUCTOR) | (1 << MK_DIELECTRIC);
+constexpr int kKindsMr = (1 << MK_METALLIC_ROUGHNESS) | (1 << MK_LAMBERT);
+using ShadeKernel = void (*)(SceneView, RenderConsts, PathBufs, int, CacheBufs);
+
+template <bool CACHE>
+ShadeKernel shade_kernel_t(const SceneView& V)
+{
+    if (V.has_textures) return k_shade<CACHE, true, kAllKinds>;
+    if ((V.kind_mask & ~kKindsLcd) == 0) return k_shade<CACHE, false, kKindsLcd>;
+    if ((V.kind_mask & ~kKindsMr) == 0) return k_shade<CACHE, false, kKindsMr>;
+    return k_shade<CACHE, false, kAllKinds>;
+}
+
+ShadeKernel shade_kernel(const SceneView& V, bool cache)
+{
+    return cache ? shade_kernel_t<true>(V) : shade_kernel_t<false>(V);
+}
+
 using BatchKernel = void (*)(SceneView, BatchSrc, int, int*, int2*, int, int, unsigned long long*, int*);
 using ClosestKernel = void (*)(SceneView, const int*, int*, ClosestSrc, int2*, int, int, unsigned long long*, int*);
 using ShadowKernel = void (*)(SceneView, const int*, int*, ShadowSrc, int2*, int, int, unsigned long long*, int*);
@@ -2086,9 +2169,7 @@ int render_impl(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams
     const int sk = scratch_stack ? kStackScratch : (depth <= lds_entries ? kStackLds : kStackSplit);
     const ClosestKernel k_closest = closest_kernel(count, sk, V.has_alpha != 0, tblock);
     // shading with the texture code only for scenes that use textures
-    void (*const k_shade_fn)(SceneView, RenderConsts, PathBufs, int, CacheBufs) =
-        cache ? (V.has_textures ? k_shade<true, true> : k_shade<true, false>)
-              : (V.has_textures ? k_shade<false, true> : k_shade<false, false>);
+    void (*const k_shade_fn)(SceneView, RenderConsts, PathBufs, int, CacheBufs) = shade_kernel(V, cache != nullptr);
     const ShadowKernel k_shadow = shadow_kernel(count, sk, V.has_alpha != 0, tblock);
     const int g_trace =
         std::min(trace_grid(reinterpret_cast<const void*>(k_closest), lds, tblock), grid_for(cap, tblock));
@@ -2563,7 +2644,7 @@ int bbgpu_autotune(BbScene* scene, const BbCameraDesc* camera, const BbRenderPar
     // one knob at a time, keeping the best so far: refill threshold, block
     // size, stack placement (LDS entries, or the private scratch stack),
     // paths per pass
-    const int refills[] = { 48, 56, 60 };
+    const int refills[] = { 16, 32, 48 };
     const int blocks[] = { 64, 128, 256 };
     const int lds[] = { 8, 16, 24, -1 }; // -1: scratch stack
     const long long paths[] = { 1ll << 22, 1ll << 24, 1ll << 26 };

@@ -265,7 +265,7 @@ typedef struct BbRenderParams {
                                  eighth of the device memory holds in path state, at least 2^22 paths) */
     int32_t trace_lds_stack;  /* traversal stack entries per lane kept in LDS (0 = auto: 12) */
     int32_t sample_begin;     /* render samples [sample_begin, sample_begin+spp) */
-    int32_t trace_refill;     /* a closest-hit traversal wave takes new rays once this many of its lanes are idle (0 = 56) */
+    int32_t trace_refill;     /* a closest-hit traversal wave takes new rays once this many of its lanes are idle (0 = 32) */
     int32_t shadow_refill;    /* the same for shadow rays (0 = trace_refill's default) */
     int32_t graph;            /* 1: launch the render as a HIP graph, captured on first use and kept with the
                                  scene for calls that differ only in sample_begin (speed only; not with stats

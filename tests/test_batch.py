@@ -55,7 +55,7 @@ def test_autotune_keeps_the_image():
     p = scenes.render_params(4)
     t = autotune(g, cam, p)
     assert t.spp == p.spp and t.max_bounces == p.max_bounces and t.seed == p.seed
-    assert t.trace_refill in (0, 48, 56, 60) and t.trace_lds_stack in (0, 8, 16, 24)
+    assert t.trace_refill in (0, 16, 32, 48) and t.trace_lds_stack in (0, 8, 16, 24)
     assert t.trace_block in (0, 64, 128, 256) and t.trace_stack in (0, 1)
     s0, c0, _ = g.render(cam, p)
     s1, c1, _ = g.render(cam, t)
