@@ -577,6 +577,20 @@ BB_D int wave_alloc(bool pred, int* counter)
     return pred ? base + __popcll(below) : -1;
 }
 
+// Wave-level compaction among the lanes that reach the call together
+// (divergent code allowed): one atomic per wave; returns this lane's slot
+BB_D int wave_alloc_active(int* counter)
+{
+    const unsigned long long mask = __ballot(1);
+    const int lane = __lane_id();
+    const int leader = __builtin_ctzll(mask);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counter, __popcll(mask));
+    base = __shfl(base, leader);
+    const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    return base + __popcll(mask & below);
+}
+
 // Block-level compaction into two queues at once: one atomic per queue per
 // block instead of per wave (same-address atomics were the shading kernel's
 // limit).  Every thread of the block must call it; s holds
@@ -1174,7 +1188,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView S, RenderConsts
                             }
                         }
                     } else if (query_here) {
-                        const int q = atomicAdd(&C.counters[0], 1);
+                        const int q = wave_alloc_active(&C.counters[0]);
                         cache_features(C, mat, sh.point, wo, sh.ns, C.qraw + 16 * size_t(q));
                         C.qbeta[q] = make_float4(beta.r, beta.g, beta.b, __int_as_float(pid));
                         if (C.query_nee && bounce < rc.max_bounces)
@@ -1350,7 +1364,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_cache_targets(CacheBufs C, cons
         if (C.query_nee && ls.w == 0.0f) ls = lf;
         const float tr = (lf.x - ls.x) / bs.x, tg = (lf.y - ls.y) / bs.y, tb = (lf.z - ls.z) / bs.z;
         if (!(isfinite(tr) && isfinite(tg) && isfinite(tb))) continue;
-        const int r = atomicAdd(&C.counters[1], 1);
+        const int r = wave_alloc_active(&C.counters[1]);
         if (r >= C.max_rows) continue;
         for (int k = 0; k < 16; ++k) C.rows[16 * size_t(r) + k] = C.traw[16 * rec + k];
         C.targets[3 * size_t(r)] = fmaxf(tr, 0.0f);

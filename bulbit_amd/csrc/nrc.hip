@@ -41,6 +41,12 @@ void set_error(const char* fmt, ...);
 namespace bb
 {
 
+#ifndef BB_QTILES_C
+#define BB_QTILES_C 2
+#endif
+#ifndef BB_QWAVES_C
+#define BB_QWAVES_C 3
+#endif
 constexpr int kIn = 80;      // encoded input width (74 used)
 constexpr int kOutCols = 16; // output columns (3 used)
 constexpr int kRawStride = 16;
@@ -77,6 +83,22 @@ struct Net
     // 80 x 64 blocks (5 x 4 tiles), the hidden layers in 64 x 64 blocks,
     // the output layer in 64 x 16 blocks
     static constexpr int groups = W / 64 + (L - 1) * (W / 64) * (W / 64) + W / 64;
+    // The query kernel's weights: per layer, W_l^T cut into the A-operand
+    // fragments of v_mfma_f32_16x16x32_bf16 (16 output features x 32 input
+    // features, K padded to a multiple of 32), fragment (l, k-step, m-tile)
+    // at frag_off(l) + ks * mt(l) + mt, 64 lanes x 8 bf16 each
+    static constexpr int kpad(int l) { return (k(l) + 31) / 32 * 32; }
+    static constexpr int mts(int l) { return n(l) / 16; }
+    static constexpr int kss(int l) { return kpad(l) / 32; }
+    static constexpr int frag_off(int l) { return l == 0 ? 0 : frag_off(l - 1) + mts(l - 1) * kss(l - 1); }
+    static constexpr int frags = frag_off(L + 1);
+    static constexpr int frag_elems = frags * 512;
+    // query rows per wave and iteration (16 per row tile) and block size:
+    // the wide network's fragments fill the LDS of a CU, so its blocks are
+    // large; two row tiles share each weight fragment read from LDS
+    static constexpr int qtiles = W > 64 ? 2 : BB_QTILES_C;
+    static constexpr int qblock = W > 64 ? 512 : 256;
+    static constexpr int qwaves = W > 64 ? 2 : BB_QWAVES_C; // waves per SIMD the query kernel is built for
 };
 
 using NetC = Net<64, 4>;  // the C4 cache
@@ -142,8 +164,10 @@ BB_D void encode_row(const float* r, uint16_t* out)
     for (int f = 0; f < 12 && k < 64; ++f) {
         const float s = float(1 << f);
         for (int a = 0; a < 3 && k < 64; ++a) {
-            out[k++] = to_bf16(sinpif(s * r[a]));
-            out[k++] = to_bf16(cospif(s * r[a]));
+            float sn, cs;
+            sincospif(s * r[a], &sn, &cs);
+            out[k++] = to_bf16(sn);
+            out[k++] = to_bf16(cs);
         }
     }
     for (int i = 3; i < 13; ++i) out[k++] = to_bf16(r[i]);
@@ -520,6 +544,168 @@ __global__ __launch_bounds__(256) void k_reduce_adam(float* __restrict__ w, floa
     WT[NT::off(l) + nn * K + k] = h;
 }
 
+// ------------------------------------------------ query (16x16x32 MFMA)
+//
+// v_mfma_f32_16x16x32_bf16: lane l holds 8 bf16 of A (row l % 16) and 8 of B
+// (column l % 16) for the same 8 values of k, chosen by the lane group
+// g = l / 16 and the slot j; the result D[4 g + j][l % 16], j = 0..3.  The
+// network runs transposed, y^T = W^T h^T: a layer's output tile (16
+// features x 16 rows) sits in D as features 4 g + j of row l % 16, so two
+// consecutive output tiles, relu'd and rounded to bf16, are the next layer's
+// B operand for one k-step when slot j < 4 means feature 32 ks + 4 g + j and
+// slot j >= 4 feature 32 ks + 16 + 4 g + (j - 4).  The weight fragments use
+// the same slot-to-feature map, so activations never leave the registers.
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef short s8v __attribute__((ext_vector_type(8)));
+
+BB_D f4v mfma32(s8v a, s8v b, f4v c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0,
+                                                  0, 0);
+}
+
+// the input feature of k-step ks, lane group g, slot j
+BB_HD int frag_feature(int ks, int g, int j) { return 32 * ks + (j < 4 ? 4 * g + j : 16 + 4 * g + (j - 4)); }
+
+// encoded input features k and k + 1 (k even) of a raw row (encode_row's
+// order and rounding): a sin/cos pair of one argument below 64
+BB_D void enc_pair(const float* r, int k, short& e0, short& e1)
+{
+    if (k < 64) {
+        const int f = k / 6, a = (k - 6 * f) >> 1;
+        float sn, cs;
+        sincospif(float(1 << f) * r[a], &sn, &cs);
+        e0 = short(to_bf16(sn));
+        e1 = short(to_bf16(cs));
+    } else {
+        e0 = k < 74 ? short(to_bf16(r[3 + (k - 64)])) : short(0);
+        e1 = k + 1 < 74 ? short(to_bf16(r[3 + (k + 1 - 64)])) : short(0);
+    }
+}
+
+// weight fragments from the fp32 master weights (zeros in the K padding)
+template <class NT>
+__global__ void k_weights_frag(const float* __restrict__ w, uint16_t* __restrict__ wf)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= NT::frag_elems) return;
+    const int frag = e >> 9, lane = (e >> 3) & 63, j = e & 7;
+    int l = 0;
+    while (l < NT::L && frag >= NT::frag_off(l + 1)) ++l;
+    const int fi = frag - NT::frag_off(l);
+    const int ks = fi / NT::mts(l), mt = fi % NT::mts(l);
+    const int nn = mt * 16 + (lane & 15), kk = frag_feature(ks, lane >> 4, j);
+    wf[e] = kk < NT::k(l) ? to_bf16(w[NT::off(l) + kk * NT::n(l) + nn]) : uint16_t(0);
+}
+
+// one layer for R row tiles: acc[mt][r] = sum over k-steps of A(l, ks, mt) B[ks][r]
+template <int KS, int MT, int R>
+BB_D void qlayer(const uint16_t* frags, const s8v (&B)[4][R], f4v (&acc)[MT][R], int lane)
+{
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[mt][r] = f4v{ 0.0f, 0.0f, 0.0f, 0.0f };
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+            const s8v a = *reinterpret_cast<const s8v*>(frags + (ks * MT + mt) * 512 + lane * 8);
+#pragma unroll
+            for (int r = 0; r < R; ++r) acc[mt][r] = mfma32(a, B[ks][r], acc[mt][r]);
+        }
+}
+
+// relu, bf16, and the output tiles paired into the next layer's B operands
+template <int MT, int R>
+BB_D void qactivate(const f4v (&acc)[MT][R], s8v (&B)[4][R])
+{
+#pragma unroll
+    for (int ks = 0; ks < MT / 2; ++ks)
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            s8v b;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                b[j] = short(to_bf16(fmaxf(acc[2 * ks][r][j], 0.0f)));
+                b[4 + j] = short(to_bf16(fmaxf(acc[2 * ks + 1][r][j], 0.0f)));
+            }
+            B[ks][r] = b;
+        }
+}
+
+// y = f(enc(raw)) for the rows span selects, weights as fragments in LDS
+template <class NT>
+__global__ __launch_bounds__(NT::qblock, NT::qwaves) void k_mlp_query(const float* __restrict__ raw, RowSpan span,
+                                                         const uint16_t* __restrict__ wf, float* __restrict__ out)
+{
+    constexpr int W = NT::W, L = NT::L, R = NT::qtiles;
+    __shared__ __align__(16) uint16_t sWF[NT::frag_elems];
+    stage16(sWF, wf, NT::frag_elems);
+    __syncthreads();
+    int off = 0, n = 0;
+    span_rows(span, off, n);
+    raw += size_t(off) * kRawStride;
+    const int lane = threadIdx.x & 63, g = lane >> 4, c16 = lane & 15;
+    const int waves = blockDim.x >> 6;
+    constexpr int rows = 16 * R;
+    for (int row0 = (blockIdx.x * waves + int(threadIdx.x >> 6)) * rows; row0 < n;
+         row0 += gridDim.x * waves * rows) {
+        s8v B[4][R];
+        // layer 0 input: each lane encodes 24 features of row c16 of each tile
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int row = row0 + r * 16 + c16;
+            float rr[13] = {};
+            if (row < n) {
+                const float4* q = reinterpret_cast<const float4*>(raw + size_t(row) * kRawStride);
+                const float4 a = q[0], b = q[1], c = q[2], d = q[3];
+                const float v[13] = { a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w, d.x };
+#pragma unroll
+                for (int i = 0; i < 13; ++i) rr[i] = v[i];
+            }
+#pragma unroll
+            for (int ks = 0; ks < NT::kss(0); ++ks) {
+                s8v b;
+#pragma unroll
+                for (int j = 0; j < 8; j += 2) {
+                    short e0 = 0, e1 = 0;
+                    if (row < n) enc_pair(rr, frag_feature(ks, g, j), e0, e1);
+                    b[j] = e0;
+                    b[j + 1] = e1;
+                }
+                B[ks][r] = b;
+            }
+        }
+        {
+            f4v acc[NT::mts(0)][R];
+            qlayer<NT::kss(0), NT::mts(0), R>(sWF + NT::frag_off(0) * 512, B, acc, lane);
+            qactivate<NT::mts(0), R>(acc, B);
+        }
+#pragma unroll 1
+        for (int l = 1; l < L; ++l) {
+            f4v acc[W / 16][R];
+            qlayer<W / 32, W / 16, R>(sWF + NT::frag_off(l) * 512, B, acc, lane);
+            qactivate<W / 16, R>(acc, B);
+        }
+        f4v acc[1][R];
+        qlayer<W / 32, 1, R>(sWF + NT::frag_off(L) * 512, B, acc, lane);
+        // channels 0..2 are rows 0..2 of the output tile: lanes of group 0
+        if (g == 0) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int row = row0 + r * 16 + c16;
+                if (row < n) {
+                    out[size_t(row) * 3] = acc[0][r][0];
+                    out[size_t(row) * 3 + 1] = acc[0][r][1];
+                    out[size_t(row) * 3 + 2] = acc[0][r][2];
+                }
+            }
+        }
+    }
+}
+
 __global__ void k_fill0(float* p, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -547,6 +733,8 @@ struct BbNrc
     int* status = nullptr; // [0]: diverged (non-finite loss, weight or prediction), sticky until new weights
     uint16_t* WT = nullptr;
     uint16_t* W = nullptr;
+    uint16_t* WF = nullptr;   // weight fragments of the query kernel
+    bool wf_stale = true;     // the master weights changed since WF was made
     uint16_t* acts = nullptr; // saved layer inputs of the last training batch
     uint16_t* dzs = nullptr;  // saved layer-output gradients of the last training batch
     float* y = nullptr;
@@ -591,6 +779,7 @@ bool nrc_reduce_t(BbNrc* x, hipStream_t st, bb::RowSpan span, bool reduce, bool 
 
 bool nrc_reduce(BbNrc* x, hipStream_t st, bb::RowSpan span, bool reduce, bool apply, float c1, float c2)
 {
+    x->wf_stale = true;
     return x->width == bb::NetC::W ? nrc_reduce_t<bb::NetC>(x, st, span, reduce, apply, c1, c2)
                                    : nrc_reduce_t<bb::NetL>(x, st, span, reduce, apply, c1, c2);
 }
@@ -601,9 +790,13 @@ bool nrc_refresh(BbNrc* x, hipStream_t st) { return nrc_reduce(x, st, bb::RowSpa
 template <class NT>
 void launch_query(BbNrc* x, const float* inputs_dev, bb::RowSpan span, float* out_dev, hipStream_t st)
 {
-    const bb::NrcWeights Wt{ x->WT, x->W };
-    hipLaunchKernelGGL((bb::k_mlp_forward<NT, false>), dim3(nrc_grid(span.cap)), dim3(bb::kNrcBlock), 0, st, inputs_dev,
-                       span, Wt, out_dev, nullptr);
+    if (x->wf_stale) {
+        hipLaunchKernelGGL(bb::k_weights_frag<NT>, dim3((NT::frag_elems + 255) / 256), dim3(256), 0, st, x->w, x->WF);
+        x->wf_stale = false;
+    }
+    const long long per_block = (long long)(NT::qblock / 64) * 16 * NT::qtiles;
+    const int grid = int(std::max<long long>(1, std::min<long long>((span.cap + per_block - 1) / per_block, 2048)));
+    hipLaunchKernelGGL(bb::k_mlp_query<NT>, dim3(grid), dim3(NT::qblock), 0, st, inputs_dev, span, x->WF, out_dev);
 }
 
 int query_span(BbNrc* x, const float* inputs_dev, bb::RowSpan span, float* out_dev, hipStream_t st)
@@ -740,6 +933,7 @@ int bbgpu_nrc_create(const BbNrcDesc* desc, BbNrc** out)
               nrc_ok(hipMalloc(&x->v, PS * 4), "hipMalloc") && nrc_ok(hipMalloc(&x->g, PS * 4), "hipMalloc") &&
               nrc_ok(hipMalloc(&x->loss, 4), "hipMalloc") && nrc_ok(hipMalloc(&x->status, 16), "hipMalloc") &&
               nrc_ok(hipMalloc(&x->WT, PS * 2), "hipMalloc") && nrc_ok(hipMalloc(&x->W, PS * 2), "hipMalloc") &&
+              nrc_ok(hipMalloc(&x->WF, size_t(std::max(bb::NetC::frag_elems, bb::NetL::frag_elems)) * 2), "hipMalloc") &&
               nrc_ok(hipMalloc(&x->partials, size_t(bb::kMaxSplits) * PS * 4), "hipMalloc");
     if (!ok) {
         bbgpu_nrc_destroy(x);
@@ -779,7 +973,8 @@ int bbgpu_nrc_destroy(BbNrc* x)
 {
     if (!x) return BBGPU_OK;
     for (void* p : { (void*)x->w, (void*)x->m, (void*)x->v, (void*)x->g, (void*)x->partials, (void*)x->loss,
-                     (void*)x->status, (void*)x->WT, (void*)x->W, (void*)x->acts, (void*)x->dzs, (void*)x->y })
+                     (void*)x->status, (void*)x->WT, (void*)x->W, (void*)x->WF, (void*)x->acts, (void*)x->dzs,
+                     (void*)x->y })
         if (p) (void)hipFree(p);
     delete x;
     return BBGPU_OK;
