@@ -273,7 +273,8 @@ BF16_PEAK_TFLOPS = 2500.0
 
 
 def cache_comparison(args, g, cam, ref, name, query_primary, train_stride=16, learning_rate=1e-3, train_steps=4,
-                     spp=1, max_bounces=8, shape=(64, 4), frames=None, keep_every=16, query_direct=False):
+                     spp=1, max_bounces=8, shape=(64, 4), frames=None, keep_every=16, query_direct=False,
+                     query_first=0):
     """Radiance cache vs plain path tracing at equal wall-clock time, `spp`
     samples per frame, against `ref` (a plain render with disjoint samples).
     Plain PT renders `frames` (default `cache_frames`) frames; the cached
@@ -321,7 +322,7 @@ def cache_comparison(args, g, cam, ref, name, query_primary, train_stride=16, le
         return err(acc[0], acc[1]), done, secs, float(np.mean(per_frame)) if per_frame else float("nan")
 
     cp = scenes.cache_params(train_stride=train_stride, train_batch=65536, train_steps=train_steps,
-                             query_primary=query_primary, query_direct=query_direct)
+                             query_primary=query_primary, query_direct=query_direct, query_first=query_first)
     frames = args.cache_frames if frames is None else frames
     flops_q = mlp_flops_query(*shape)
 
@@ -364,6 +365,7 @@ def cache_comparison(args, g, cam, ref, name, query_primary, train_stride=16, le
         "spp_per_frame": spp,
         "query_at_primary_hit": bool(query_primary),
         "direct_light_at_query_vertex": bool(query_direct),
+        "first_query_vertex": 0 if query_primary else max(1, query_first),
         "diverged": any(st.diverged for st in stats),
         "train_stride": train_stride,
         "learning_rate": learning_rate,
@@ -508,13 +510,20 @@ def large_workload(args, nested=False):
                           "relmse_4spp_frame_vs_reference": round(relmse(test, ref), 6)}
         if args.large_frames > 0:
             name = "large scene %dx%d, %d spp per frame, max_bounces %d" % (W, H, SPP, MB)
+            # the query vertex takes its direct light and the cache stands in
+            # for the indirect rest (the lower accumulated error); beside it the
+            # cache for everything the vertex reflects (the lower per-frame error)
             cq = cache_comparison(args, g, cam, ref, name, False, train_stride=16, learning_rate=1e-3,
+                                  train_steps=4, spp=SPP, max_bounces=MB, shape=(128, 5), frames=args.large_frames,
+                                  keep_every=8, query_direct=True)
+            ca = cache_comparison(args, g, cam, ref, name, False, train_stride=16, learning_rate=1e-3,
                                   train_steps=4, spp=SPP, max_bounces=MB, shape=(128, 5), frames=args.large_frames,
                                   keep_every=8)
             out["relmse_equal_time_cache_vs_plain"] = cq["ratio"]
             out["relmse_equal_time_cache_vs_plain_accumulated"] = cq["ratio"]
             out["relmse_equal_time_cache_vs_plain_per_frame"] = cq["per_frame"]["ratio_equal_time"]
             out["radiance_cache"] = cq
+            out["radiance_cache_all"] = ca
     g.close()
     return out
 

@@ -12,7 +12,7 @@ from ctypes import POINTER, c_char_p, c_double, c_float, c_int32, c_size_t, c_ui
 
 import numpy as np
 
-ABI_VERSION = 18  # BBGPU_ABI_VERSION of include/bbgpu.h that these structures mirror
+ABI_VERSION = 19  # BBGPU_ABI_VERSION of include/bbgpu.h that these structures mirror
 
 LIB_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(LIB_DIR, "libbbgpu.so")
@@ -237,6 +237,7 @@ class BbCacheParams(ctypes.Structure):
         ("train_batch", c_int32),
         ("train_steps", c_int32),
         ("query_primary", c_int32),
+        ("query_first", c_int32),
         ("query_direct", c_int32),
     ]
 

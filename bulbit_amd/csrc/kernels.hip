@@ -828,7 +828,7 @@ struct CacheBufs
                        // [2] 1 + the pass dropped for it (0: none)
     unsigned long long* tally; // [0] queries, [1] rows trained (summed over the call's passes)
     f3 wmin, winv;    // position normalisation to [0, 1]^3
-    int32_t query_bounce, query_primary, query_nee, extra, max_vertices, max_rows, max_queries;
+    int32_t query_bounce, query_first, query_nee, extra, max_vertices, max_rows, max_queries;
 };
 
 // UniDirectionalRayIntegrator::Render per (pixel, sample) start
@@ -1166,7 +1166,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView S, RenderConsts
                     const bool query_here =
                         mi >= 0 && C.state[0] == 0 &&
                         (depth >= C.query_bounce ||
-                         ((depth >= 1 || C.query_primary) && (mat.flags_front & (BX_DIFFUSE | BX_GLOSSY))));
+                         (depth >= C.query_first && (mat.flags_front & (BX_DIFFUSE | BX_GLOSSY))));
                     int32_t stop = (fl >> kStopShift) & 0xff;
                     if (training) {
                         if (query_here && stop == kStopUnknown) stop = min(depth + C.extra, 254);
@@ -2243,7 +2243,7 @@ int render_impl(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams
         C->wmin = lo;
         C->winv = mk3(1.0f / std::max(ext.x, 1e-6f), 1.0f / std::max(ext.y, 1e-6f), 1.0f / std::max(ext.z, 1e-6f));
         C->query_bounce = std::max(1, cache->cp.query_bounce);
-        C->query_primary = cache->cp.query_primary ? 1 : 0;
+        C->query_first = cache->cp.query_primary ? 0 : std::max(1, cache->cp.query_first);
         C->query_nee = cache->cp.query_direct ? 1 : 0;
         C->extra = std::max(0, cache->cp.train_extra_bounces);
         C->diverged = nrc_status(cache->nrc);
@@ -2712,7 +2712,8 @@ int bbgpu_render_cached(BbScene* scene, BbNrc* nrc, const BbCameraDesc* camera, 
         return BBGPU_ERR_INVALID;
     }
     const BbCacheParams& c = *cache_params;
-    if (c.query_bounce < 1 || c.train_stride < 1 || c.train_extra_bounces < 0 || c.train_batch < 0 || c.train_steps < 0) {
+    if (c.query_bounce < 1 || c.train_stride < 1 || c.train_extra_bounces < 0 || c.train_batch < 0 || c.train_steps < 0 ||
+        c.query_first < 0) {
         bb::set_error("bbgpu_render_cached: invalid cache parameters");
         return BBGPU_ERR_INVALID;
     }

@@ -810,7 +810,7 @@ def portable_random_rays(bmin, bmax, n, seed, t_max=np.inf):
 
 
 def cache_params(query_bounce=2, train_stride=16, train_extra_bounces=4, train_batch=65536, train_steps=1,
-                 query_primary=False, query_direct=False):
+                 query_primary=False, query_direct=False, query_first=0):
     """BbCacheParams of the cached path tracer (defaults: query at the first
     rough vertex from vertex 1 on, or at vertex 2; the cache stands in for all
     the vertex reflects unless query_direct; 1/16 training paths extended by
@@ -818,6 +818,7 @@ def cache_params(query_bounce=2, train_stride=16, train_extra_bounces=4, train_b
     c = capi.BbCacheParams()
     c.query_primary = int(query_primary)
     c.query_direct = int(query_direct)
+    c.query_first = int(query_first)
     c.query_bounce = query_bounce
     c.train_stride = train_stride
     c.train_extra_bounces = train_extra_bounces

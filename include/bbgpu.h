@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define BBGPU_ABI_VERSION 18
+#define BBGPU_ABI_VERSION 19
 
 /* status codes */
 enum {
@@ -482,6 +482,8 @@ typedef struct BbCacheParams {
     int32_t train_batch;          /* rows per optimizer step (0: 65536) */
     int32_t train_steps;          /* optimizer steps per pass (0: inference only) */
     int32_t query_primary;        /* 1: the primary hit may be the query vertex; 0: vertex 1 at the earliest */
+    int32_t query_first;          /* the first vertex that may be a query vertex when query_primary is 0
+                                     (0: vertex 1; 2: after a 2-bounce prefix) */
     int32_t query_direct;         /* 1: the query vertex takes its direct light (NEE) as the plain tracer does
                                      and the cache stands in for the indirect rest only; 0: the cache stands in
                                      for everything the vertex reflects */

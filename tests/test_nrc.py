@@ -394,3 +394,32 @@ def test_direct_light_at_the_query_vertex():
     assert (s_dir - s_all).sum() > 0.5 * (s0 - s_all).sum()
     net.close()
     g.close()
+
+
+@pytest.mark.gpu
+def test_a_later_first_query_vertex_keeps_more_of_the_path():
+    """query_first: with a zero cache, a rendering path keeps what the plain
+    path gathered before its query vertex, so letting the first query come
+    one vertex later gives every pixel at least as much, and never more than
+    the plain render (displaced icosphere, where paths meet second surfaces)."""
+    from bulbit_amd import scenes
+    from bulbit_amd.gpu import GpuScene
+    from bulbit_amd.nrc import Nrc
+
+    sc, cam = scenes.icosphere_scene(width=160, height=90, level=5, n_lights=16)
+    g = GpuScene(sc)
+    p = scenes.render_params(4)
+    s0, c0, _ = g.render(cam, p)
+    net = Nrc(seed=3)
+    net.set_weights(np.zeros(net.n_params, np.float32))
+    n_paths = 160 * 90 * 4
+    cp = dict(train_stride=n_paths + 1, train_steps=0, query_bounce=8)
+    s1, c1, st1 = g.render_cached(net, cam, p, scenes.cache_params(query_first=1, **cp))
+    s2, c2, st2 = g.render_cached(net, cam, p, scenes.cache_params(query_first=2, **cp))
+    assert np.array_equal(c0, c1) and np.array_equal(c0, c2)
+    assert st2.queries < st1.queries
+    assert np.all(s2 >= s1 * (1 - 1e-6) - 1e-7)
+    assert np.all(s2 <= s0 * (1 + 1e-6) + 1e-7)
+    assert s1.sum() < s2.sum() < s0.sum()
+    net.close()
+    g.close()
