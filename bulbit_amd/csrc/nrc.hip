@@ -156,22 +156,41 @@ BB_HD uint16_t to_bf16(float f)
 
 BB_D float from_bf16(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
 
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef uint32_t u4v __attribute__((ext_vector_type(4)));
+
+// two floats as bf16 (round to nearest even: one v_cvt_pk_bf16_f32), a in
+// the low half; the same bits as to_bf16 for finite values
+BB_D uint32_t pack_bf16(float a, float b)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(f2v{ a, b }, bf16x2));
+}
+
 // --------------------------------------------------------------- encoding
 
+// sin and cos of pi v 2^f, as a packed bf16 pair (sin low), for
+// half_scale = 2^(f - 1).  The angle in turns, t = v 2^(f - 1), less its
+// nearest integer (both steps exact) lies in [-1/2, 1/2], inside the range
+// of the hardware's v_sin_f32 / v_cos_f32 (sin and cos of 2 pi t).
+BB_D uint32_t sincos_pi_bf16(float v, float half_scale)
+{
+    float t = v * half_scale;
+    t = t - __builtin_rintf(t);
+    return pack_bf16(__builtin_amdgcn_sinf(t), __builtin_amdgcn_cosf(t));
+}
+
+// the kIn encoded features of one raw row (out 4-byte aligned): pair p < 32
+// is sin/cos(2^(p / 3) pi r[p mod 3]), features 64..73 are r[3..12]
 BB_D void encode_row(const float* r, uint16_t* out)
 {
-    int k = 0;
-    for (int f = 0; f < 12 && k < 64; ++f) {
-        const float s = float(1 << f);
-        for (int a = 0; a < 3 && k < 64; ++a) {
-            float sn, cs;
-            sincospif(s * r[a], &sn, &cs);
-            out[k++] = to_bf16(sn);
-            out[k++] = to_bf16(cs);
-        }
-    }
-    for (int i = 3; i < 13; ++i) out[k++] = to_bf16(r[i]);
-    while (k < kIn) out[k++] = 0;
+    uint32_t* o = reinterpret_cast<uint32_t*>(out);
+#pragma unroll
+    for (int p = 0; p < 32; ++p) o[p] = sincos_pi_bf16(r[p % 3], 0.5f * float(1 << (p / 3)));
+#pragma unroll
+    for (int i = 0; i < 5; ++i) o[32 + i] = pack_bf16(r[3 + 2 * i], r[4 + 2 * i]);
+    static_assert(kIn == 80, "features 74..79 are the padding");
+    o[37] = o[38] = o[39] = 0u;
 }
 
 // ------------------------------------------------------------ MFMA pieces
@@ -566,23 +585,7 @@ BB_D f4v mfma32(s8v a, s8v b, f4v c)
 }
 
 // the input feature of k-step ks, lane group g, slot j
-BB_HD int frag_feature(int ks, int g, int j) { return 32 * ks + (j < 4 ? 4 * g + j : 16 + 4 * g + (j - 4)); }
-
-// encoded input features k and k + 1 (k even) of a raw row (encode_row's
-// order and rounding): a sin/cos pair of one argument below 64
-BB_D void enc_pair(const float* r, int k, short& e0, short& e1)
-{
-    if (k < 64) {
-        const int f = k / 6, a = (k - 6 * f) >> 1;
-        float sn, cs;
-        sincospif(float(1 << f) * r[a], &sn, &cs);
-        e0 = short(to_bf16(sn));
-        e1 = short(to_bf16(cs));
-    } else {
-        e0 = k < 74 ? short(to_bf16(r[3 + (k - 64)])) : short(0);
-        e1 = k + 1 < 74 ? short(to_bf16(r[3 + (k + 1 - 64)])) : short(0);
-    }
-}
+BB_HD constexpr int frag_feature(int ks, int g, int j) { return 32 * ks + (j < 4 ? 4 * g + j : 16 + 4 * g + (j - 4)); }
 
 // weight fragments from the fp32 master weights (zeros in the K padding)
 template <class NT>
@@ -617,6 +620,15 @@ BB_D void qlayer(const uint16_t* frags, const s8v (&B)[4][R], f4v (&acc)[MT][R],
         }
 }
 
+// relu of a packed bf16 pair: as 16-bit integers the negative values (sign
+// bit set) are the negative ones, so max(x, 0) per half keeps x >= 0 and
+// zeroes the rest (-0 included)
+BB_D uint32_t relu_bf16x2(uint32_t x)
+{
+    typedef short s2v __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(s2v, x), s2v{ 0, 0 }));
+}
+
 // relu, bf16, and the output tiles paired into the next layer's B operands
 template <int MT, int R>
 BB_D void qactivate(const f4v (&acc)[MT][R], s8v (&B)[4][R])
@@ -625,13 +637,10 @@ BB_D void qactivate(const f4v (&acc)[MT][R], s8v (&B)[4][R])
     for (int ks = 0; ks < MT / 2; ++ks)
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            s8v b;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                b[j] = short(to_bf16(fmaxf(acc[2 * ks][r][j], 0.0f)));
-                b[4 + j] = short(to_bf16(fmaxf(acc[2 * ks + 1][r][j], 0.0f)));
-            }
-            B[ks][r] = b;
+            const f4v lo = acc[2 * ks][r], hi = acc[2 * ks + 1][r];
+            const u4v d{ relu_bf16x2(pack_bf16(lo[0], lo[1])), relu_bf16x2(pack_bf16(lo[2], lo[3])),
+                         relu_bf16x2(pack_bf16(hi[0], hi[1])), relu_bf16x2(pack_bf16(hi[2], hi[3])) };
+            B[ks][r] = __builtin_bit_cast(s8v, d);
         }
 }
 
@@ -650,6 +659,19 @@ __global__ __launch_bounds__(NT::qblock, NT::qwaves) void k_mlp_query(const floa
     const int lane = threadIdx.x & 63, g = lane >> 4, c16 = lane & 15;
     const int waves = blockDim.x >> 6;
     constexpr int rows = 16 * R;
+    static_assert(NT::kss(0) == 3 && frag_feature(1, 3, 7) == 63, "k-steps 0 and 1 hold the 64 sin/cos features");
+    // the lane's eight sin/cos pairs of k-steps 0 and 1: slots 2i, 2i + 1 of
+    // k-step ks = i / 4 are features 2p, 2p + 1 with
+    // p = 16 ks + 8 ((i / 2) mod 2) + 2 g + i mod 2 (frag_feature's map),
+    // the sin/cos of 2^(p / 3) pi times axis p mod 3
+    float pscale[8];
+    int pax[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int p = 16 * (i >> 2) + 8 * ((i >> 1) & 1) + 2 * g + (i & 1);
+        pscale[i] = 0.5f * float(1 << (p / 3));
+        pax[i] = p % 3;
+    }
     for (int row0 = (blockIdx.x * waves + int(threadIdx.x >> 6)) * rows; row0 < n;
          row0 += gridDim.x * waves * rows) {
         s8v B[4][R];
@@ -657,26 +679,34 @@ __global__ __launch_bounds__(NT::qblock, NT::qwaves) void k_mlp_query(const floa
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int row = row0 + r * 16 + c16;
-            float rr[13] = {};
-            if (row < n) {
-                const float4* q = reinterpret_cast<const float4*>(raw + size_t(row) * kRawStride);
-                const float4 a = q[0], b = q[1], c = q[2], d = q[3];
-                const float v[13] = { a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w, d.x };
-#pragma unroll
-                for (int i = 0; i < 13; ++i) rr[i] = v[i];
+            const bool ok = row < n;
+            const float* rp = raw + size_t(ok ? row : 0) * kRawStride;
+            float xyz[3] = { 0.0f, 0.0f, 0.0f };
+            if (ok) {
+                xyz[0] = rp[0];
+                xyz[1] = rp[1];
+                xyz[2] = rp[2];
             }
+            // k-steps 0 and 1: four sin/cos pairs each (rows beyond n encode
+            // zeros; their outputs are not stored)
 #pragma unroll
-            for (int ks = 0; ks < NT::kss(0); ++ks) {
-                s8v b;
+            for (int ks = 0; ks < 2; ++ks) {
+                u4v d;
 #pragma unroll
-                for (int j = 0; j < 8; j += 2) {
-                    short e0 = 0, e1 = 0;
-                    if (row < n) enc_pair(rr, frag_feature(ks, g, j), e0, e1);
-                    b[j] = e0;
-                    b[j + 1] = e1;
+                for (int i = 0; i < 4; ++i) {
+                    const int a = pax[4 * ks + i];
+                    const float v = a == 0 ? xyz[0] : (a == 1 ? xyz[1] : xyz[2]);
+                    d[i] = sincos_pi_bf16(v, pscale[4 * ks + i]);
                 }
-                B[ks][r] = b;
+                B[ks][r] = __builtin_bit_cast(s8v, d);
             }
+            // k-step 2: slots 0..3 are features 64 + 4 g + j = r[3 + 4 g + j]
+            // (below 74), slots 4..7 features 80.. (padding)
+            float e[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) e[j] = (ok && 4 * g + j < 10) ? rp[3 + 4 * g + j] : 0.0f;
+            const u4v d{ pack_bf16(e[0], e[1]), pack_bf16(e[2], e[3]), 0u, 0u };
+            B[2][r] = __builtin_bit_cast(s8v, d);
         }
         {
             f4v acc[NT::mts(0)][R];
