@@ -99,6 +99,19 @@ struct Net
     static constexpr int qtiles = W > 64 ? 2 : BB_QTILES_C;
     static constexpr int qblock = W > 64 ? 512 : 256;
     static constexpr int qwaves = W > 64 ? 2 : BB_QWAVES_C; // waves per SIMD the query kernel is built for
+    // The gradient chain's weights (fused training kernel): per layer
+    // l >= 1, W_l cut into A-operand fragments of 16 input features of the
+    // layer x 32 of its outputs (the output side padded to a multiple of 32,
+    // same slot-to-feature map), fragment (l, k-step, m-tile) at
+    // bfrag_off(l) + ks * bmts(l) + mt
+    static constexpr int bmts(int l) { return k(l) / 16; }
+    static constexpr int bkss(int l) { return (n(l) + 31) / 32; }
+    static constexpr int bfrag_off(int l) { return l <= 1 ? 0 : bfrag_off(l - 1) + bmts(l - 1) * bkss(l - 1); }
+    static constexpr int bfrags = bfrag_off(L + 1);
+    static constexpr int bfrag_elems = bfrags * 512;
+    // one fused kernel takes a training step of the narrow network (its
+    // weights, in both fragment sets, fit in LDS beside the gradient tiles)
+    static constexpr bool fused = W == 64;
 };
 
 using NetC = Net<64, 4>;  // the C4 cache
@@ -132,6 +145,15 @@ BB_D void span_rows(const RowSpan& s, int& off, int& len)
 
 // most row ranges of the dW GEMM (partial sums held)
 constexpr int kMaxSplits = 512;
+
+// the fused training kernel (64-wide network): waves per block, rows per
+// pass (16 per wave), row stride (bf16) of a feature's tile in LDS (272
+// bytes), most blocks (one per CU: its LDS)
+constexpr int kTrainWaves = 8;
+constexpr int kTrainRows = 16 * kTrainWaves;
+constexpr int kTrainLd = kTrainRows + 8;
+constexpr int kTrainBlocks = 256;
+static_assert(kTrainBlocks <= kMaxSplits, "one partial sum per block");
 
 // row ranges for the dW GEMM: about 2048 waves in all, at least four chunks each
 BB_HD int dw_splits(int chunks, int groups)
@@ -397,23 +419,25 @@ BB_D void dz_layer(int l, const uint16_t* hc, const uint16_t* Wk, uint16_t* dzl,
 
 // The gradient chain for n rows whose layer inputs the forward pass saved:
 // dz_L = dL/dy, then dz_{l-1} for l = L..1, every dz_l stored per chunk
-// (Net::dz_off).  One wave per chunk; the loss is summed into loss_sum.
+// (Net::dz_off).  One wave per chunk; each block leaves its loss in
+// loss_part[block] (zero without rows).
 template <class NT>
 __global__ __launch_bounds__(kNrcBlock) void k_mlp_dz(const float* __restrict__ y, const float* __restrict__ target,
                                                       RowSpan span, const uint16_t* __restrict__ Wk,
                                                       const uint16_t* __restrict__ acts, uint16_t* __restrict__ dzs,
-                                                      float* __restrict__ loss_sum)
+                                                      float* __restrict__ loss_part)
 {
     constexpr int W = NT::W, L = NT::L;
     int off = 0, n = 0;
     span_rows(span, off, n);
-    if (n == 0) return;
     target += size_t(off) * 3;
     __shared__ __align__(16) uint16_t sD[kNrcBlock / 64][kRows * W]; // dz of the current layer output, row-major
+    __shared__ float sLoss[kNrcBlock / 64];
+    static_assert(kNrcBlock / 64 == 4, "four wave sums");
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int r16 = lane & 15, q = lane >> 4;
     uint16_t* dzl = sD[wave];
-    const float inv_count = 1.0f / (3.0f * float(n));
+    const float inv_count = n > 0 ? 1.0f / (3.0f * float(n)) : 0.0f;
     float loss = 0.0f;
     const int chunks = (n + kRows - 1) / kRows;
     for (int ch = blockIdx.x * (kNrcBlock / 64) + wave; ch < chunks; ch += gridDim.x * (kNrcBlock / 64)) {
@@ -440,8 +464,10 @@ __global__ __launch_bounds__(kNrcBlock) void k_mlp_dz(const float* __restrict__ 
 #pragma unroll 1
         for (int l = L - 1; l >= 1; --l) dz_layer<NT, W>(l, hc, Wk, dzl, dzc, lane);
     }
-    for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
-    if (lane == 0) atomicAdd(loss_sum, loss);
+    for (int o = 32; o > 0; o >>= 1) loss += __shfl_xor(loss, o);
+    if (lane == 0) sLoss[wave] = loss;
+    __syncthreads();
+    if (threadIdx.x == 0) loss_part[blockIdx.x] = ((sLoss[0] + sLoss[1]) + sLoss[2]) + sLoss[3];
 }
 
 // dW_l rows [mt0, mt0 + 16 TM) x columns [nt0, nt0 + 16 TN) summed over
@@ -512,24 +538,44 @@ __global__ __launch_bounds__(64) void k_mlp_dw(const uint16_t* __restrict__ acts
     }
 }
 
+// k-step, lane group and slot of input feature k in the 16x16x32 operands
+// (the inverse of frag_feature)
+BB_HD void frag_slot(int k, int& ks, int& g, int& j)
+{
+    ks = k / 32;
+    const int r = k % 32;
+    g = (r % 16) / 4;
+    j = r % 4 + (r < 16 ? 0 : 4);
+}
+
 // The gradient as the sum of the row ranges' partial sums, kept in g: wave
 // w of a block adds ranges w, w + 4, ... in order, then the four sums are
 // added in wave order.  Then Adam (beta1 0.9, beta2 0.99, eps 1e-8) on the
-// fp32 master weights when apply, and the bf16 copies in both layouts; the
-// unused output columns stay zero.  splits = 0 only refreshes the copies.
+// fp32 master weights when apply, and the bf16 copies: both matrix layouts,
+// the query's fragments and (fused network) the gradient chain's; the unused
+// output columns stay zero.  reduce = 0 only refreshes the copies.  The
+// loss is the sum of the step's per-block partial sums (nloss of them) in a
+// fixed order.  fused_grid > 0: the partial sums came from that many blocks
+// of the fused kernel, one per pass while passes last.
 template <class NT>
 __global__ __launch_bounds__(256) void k_reduce_adam(float* __restrict__ w, float* __restrict__ m,
                                                      float* __restrict__ v, float* __restrict__ g,
                                                      const float* __restrict__ partials, RowSpan span, int reduce,
-                                                     float lr, float c1, float c2, int apply,
+                                                     int fused_grid, float lr, float c1, float c2, int apply,
                                                      uint16_t* __restrict__ WT, uint16_t* __restrict__ W,
-                                                     const float* __restrict__ loss, int* __restrict__ diverged)
+                                                     uint16_t* __restrict__ WF, uint16_t* __restrict__ WB,
+                                                     const float* __restrict__ loss_part, int nloss,
+                                                     float* __restrict__ loss, int* __restrict__ diverged)
 {
     __shared__ float part[4][64];
     int off = 0, n = 0;
     span_rows(span, off, n);
     // no rows: nothing to reduce and no step
-    const int splits = (reduce && n > 0) ? dw_splits((n + kRows - 1) / kRows, NT::groups) : 0;
+    int splits = 0;
+    if (reduce && n > 0) {
+        const int passes = (n + kTrainRows - 1) / kTrainRows;
+        splits = fused_grid > 0 ? min(fused_grid, passes) : dw_splits((n + kRows - 1) / kRows, NT::groups);
+    }
     if (reduce && n == 0) apply = 0;
     const int t = threadIdx.x, wv = t >> 6;
     const int i = blockIdx.x * 64 + (t & 63);
@@ -540,7 +586,14 @@ __global__ __launch_bounds__(256) void k_reduce_adam(float* __restrict__ w, floa
     }
     part[wv][t & 63] = s;
     __syncthreads();
-    if (wv != 0 || i >= NT::params) return;
+    if (wv != 0) return;
+    float ls = 0.0f;
+    if (reduce) {
+        for (int b = t; b < nloss; b += 64) ls += loss_part[b];
+        for (int o = 32; o > 0; o >>= 1) ls += __shfl_xor(ls, o);
+        if (blockIdx.x == 0 && t == 0) *loss = ls;
+    }
+    if (i >= NT::params) return;
     const float gi = ((part[0][t] + part[1][t]) + part[2][t]) + part[3][t];
     int l = 0;
     while (l < NT::L && i >= NT::off(l + 1)) ++l;
@@ -557,10 +610,23 @@ __global__ __launch_bounds__(256) void k_reduce_adam(float* __restrict__ w, floa
         w[i] = wi;
     }
     // a diverged network (non-finite loss or weight) is flagged for good
-    if (!isfinite(wi) || (apply && !isfinite(*loss))) *diverged = 1;
+    if (!isfinite(wi) || (apply && !isfinite(ls))) *diverged = 1;
     const uint16_t h = to_bf16(wi);
     W[i] = h;
     WT[NT::off(l) + nn * K + k] = h;
+    // query fragment (l, ks, nn / 16): W_l^T, output feature nn on the lane,
+    // input feature k in the slot
+    int ks, gg, j;
+    frag_slot(k, ks, gg, j);
+    WF[(NT::frag_off(l) + ks * NT::mts(l) + nn / 16) * 512 + (16 * gg + nn % 16) * 8 + j] = h;
+    if constexpr (NT::fused) {
+        // gradient-chain fragment (l, ks, k / 16): W_l, input feature k on
+        // the lane, output feature nn in the slot
+        if (l >= 1) {
+            frag_slot(nn, ks, gg, j);
+            WB[(NT::bfrag_off(l) + ks * NT::bmts(l) + k / 16) * 512 + (16 * gg + k % 16) * 8 + j] = h;
+        }
+    }
 }
 
 // ------------------------------------------------ query (16x16x32 MFMA)
@@ -586,21 +652,6 @@ BB_D f4v mfma32(s8v a, s8v b, f4v c)
 
 // the input feature of k-step ks, lane group g, slot j
 BB_HD constexpr int frag_feature(int ks, int g, int j) { return 32 * ks + (j < 4 ? 4 * g + j : 16 + 4 * g + (j - 4)); }
-
-// weight fragments from the fp32 master weights (zeros in the K padding)
-template <class NT>
-__global__ void k_weights_frag(const float* __restrict__ w, uint16_t* __restrict__ wf)
-{
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= NT::frag_elems) return;
-    const int frag = e >> 9, lane = (e >> 3) & 63, j = e & 7;
-    int l = 0;
-    while (l < NT::L && frag >= NT::frag_off(l + 1)) ++l;
-    const int fi = frag - NT::frag_off(l);
-    const int ks = fi / NT::mts(l), mt = fi % NT::mts(l);
-    const int nn = mt * 16 + (lane & 15), kk = frag_feature(ks, lane >> 4, j);
-    wf[e] = kk < NT::k(l) ? to_bf16(w[NT::off(l) + kk * NT::n(l) + nn]) : uint16_t(0);
-}
 
 // one layer for R row tiles: acc[mt][r] = sum over k-steps of A(l, ks, mt) B[ks][r]
 template <int KS, int MT, int R>
@@ -644,6 +695,59 @@ BB_D void qactivate(const f4v (&acc)[MT][R], s8v (&B)[4][R])
         }
 }
 
+// The lane's eight sin/cos pairs of k-steps 0 and 1 of the layer-0 operand:
+// slots 2i, 2i + 1 of k-step ks = i / 4 are features 2p, 2p + 1 with
+// p = 16 ks + 8 ((i / 2) mod 2) + 2 g + i mod 2 (frag_feature's map), the
+// sin/cos of 2^(p / 3) pi times axis p mod 3
+struct EncLane
+{
+    float scale[8];
+    int axis[8];
+};
+
+BB_D EncLane enc_lane(int g)
+{
+    static_assert(frag_feature(1, 3, 7) == 63, "k-steps 0 and 1 hold the 64 sin/cos features");
+    EncLane e;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int p = 16 * (i >> 2) + 8 * ((i >> 1) & 1) + 2 * g + (i & 1);
+        e.scale[i] = 0.5f * float(1 << (p / 3));
+        e.axis[i] = p % 3;
+    }
+    return e;
+}
+
+// the layer-0 B operands (k-steps 0..2) of one row: the lane's 24 features
+// of the raw row rp (zeros when !ok)
+BB_D void enc_operands(const float* rp, bool ok, int g, const EncLane& el, s8v& b0, s8v& b1, s8v& b2)
+{
+    float xyz[3] = { 0.0f, 0.0f, 0.0f };
+    if (ok) {
+        xyz[0] = rp[0];
+        xyz[1] = rp[1];
+        xyz[2] = rp[2];
+    }
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+        u4v d;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int a = el.axis[4 * ks + i];
+            const float v = a == 0 ? xyz[0] : (a == 1 ? xyz[1] : xyz[2]);
+            d[i] = sincos_pi_bf16(v, el.scale[4 * ks + i]);
+        }
+        (ks == 0 ? b0 : b1) = __builtin_bit_cast(s8v, d);
+    }
+    // k-step 2: slots 0..3 are features 64 + 4 g + j = r[3 + 4 g + j] (below
+    // 74), slots 4..7 features 80.. (padding)
+    float e[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) e[j] = (ok && 4 * g + j < 10) ? rp[3 + 4 * g + j] : 0.0f;
+    const u4v d{ pack_bf16(e[0], e[1]), pack_bf16(e[2], e[3]), 0u, 0u };
+    b2 = __builtin_bit_cast(s8v, d);
+}
+
 // y = f(enc(raw)) for the rows span selects, weights as fragments in LDS
 template <class NT>
 __global__ __launch_bounds__(NT::qblock, NT::qwaves) void k_mlp_query(const float* __restrict__ raw, RowSpan span,
@@ -659,54 +763,18 @@ __global__ __launch_bounds__(NT::qblock, NT::qwaves) void k_mlp_query(const floa
     const int lane = threadIdx.x & 63, g = lane >> 4, c16 = lane & 15;
     const int waves = blockDim.x >> 6;
     constexpr int rows = 16 * R;
-    static_assert(NT::kss(0) == 3 && frag_feature(1, 3, 7) == 63, "k-steps 0 and 1 hold the 64 sin/cos features");
-    // the lane's eight sin/cos pairs of k-steps 0 and 1: slots 2i, 2i + 1 of
-    // k-step ks = i / 4 are features 2p, 2p + 1 with
-    // p = 16 ks + 8 ((i / 2) mod 2) + 2 g + i mod 2 (frag_feature's map),
-    // the sin/cos of 2^(p / 3) pi times axis p mod 3
-    float pscale[8];
-    int pax[8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int p = 16 * (i >> 2) + 8 * ((i >> 1) & 1) + 2 * g + (i & 1);
-        pscale[i] = 0.5f * float(1 << (p / 3));
-        pax[i] = p % 3;
-    }
+    static_assert(NT::kss(0) == 3, "layer 0 takes three k-steps");
+    const EncLane el = enc_lane(g);
     for (int row0 = (blockIdx.x * waves + int(threadIdx.x >> 6)) * rows; row0 < n;
          row0 += gridDim.x * waves * rows) {
         s8v B[4][R];
-        // layer 0 input: each lane encodes 24 features of row c16 of each tile
+        // layer 0 input: each lane encodes 24 features of row c16 of each
+        // tile (rows beyond n encode zeros; their outputs are not stored)
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int row = row0 + r * 16 + c16;
             const bool ok = row < n;
-            const float* rp = raw + size_t(ok ? row : 0) * kRawStride;
-            float xyz[3] = { 0.0f, 0.0f, 0.0f };
-            if (ok) {
-                xyz[0] = rp[0];
-                xyz[1] = rp[1];
-                xyz[2] = rp[2];
-            }
-            // k-steps 0 and 1: four sin/cos pairs each (rows beyond n encode
-            // zeros; their outputs are not stored)
-#pragma unroll
-            for (int ks = 0; ks < 2; ++ks) {
-                u4v d;
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int a = pax[4 * ks + i];
-                    const float v = a == 0 ? xyz[0] : (a == 1 ? xyz[1] : xyz[2]);
-                    d[i] = sincos_pi_bf16(v, pscale[4 * ks + i]);
-                }
-                B[ks][r] = __builtin_bit_cast(s8v, d);
-            }
-            // k-step 2: slots 0..3 are features 64 + 4 g + j = r[3 + 4 g + j]
-            // (below 74), slots 4..7 features 80.. (padding)
-            float e[4];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) e[j] = (ok && 4 * g + j < 10) ? rp[3 + 4 * g + j] : 0.0f;
-            const u4v d{ pack_bf16(e[0], e[1]), pack_bf16(e[2], e[3]), 0u, 0u };
-            B[2][r] = __builtin_bit_cast(s8v, d);
+            enc_operands(raw + size_t(ok ? row : 0) * kRawStride, ok, g, el, B[0][r], B[1][r], B[2][r]);
         }
         {
             f4v acc[NT::mts(0)][R];
@@ -736,10 +804,209 @@ __global__ __launch_bounds__(NT::qblock, NT::qwaves) void k_mlp_query(const floa
     }
 }
 
-__global__ void k_fill0(float* p, int n)
+// ------------------------------------- fused training step (64-wide network)
+//
+// One block of kTrainWaves waves takes kTrainRows rows per pass, 16 per
+// wave: the forward pass as in the query kernel (transposed, activations in
+// registers), keeping every layer's input operand; dL/dy; then the gradient
+// chain dz_{l-1}^T = W_l dz_l^T masked by h_l > 0, also transposed, with the
+// gradient-chain fragments in LDS.  For dW_l = h_l^T dz_l, whose MFMA needs
+// the rows along k, each wave writes h_l and dz_l feature-major
+// ([feature][row]) into LDS; after a barrier the block's waves add the
+// products of their share of the 16x16 dW tiles over the pass's rows to
+// accumulators held across passes.  The tiles are double-buffered, so one
+// barrier per layer suffices: a buffer is written again only two layers
+// later, after every wave has passed the barrier that follows the reads.
+// Each block leaves one partial sum of dW and of the loss.
+
+
+// most blocks of the fused training kernel for max_len rows
+BB_HD int train_grid(int max_len)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0.0f;
+    const int passes = (max_len + kTrainRows - 1) / kTrainRows;
+    return passes < 1 ? 1 : (passes < kTrainBlocks ? passes : kTrainBlocks);
+}
+
+// bf16 pair x where the matching bf16 of h is positive (as 16-bit integers:
+// nonzero with the sign bit clear), zero elsewhere
+BB_D uint32_t mask_positive(uint32_t x, uint32_t h)
+{
+    typedef short s2v __attribute__((ext_vector_type(2)));
+    const s2v on = __builtin_bit_cast(s2v, h) > s2v{ 0, 0 };
+    return x & __builtin_bit_cast(uint32_t, on);
+}
+
+// a wave's 16 rows of operand B (slot j of k-step ks is feature
+// frag_feature(ks, g, j)) into the feature-major tile s, features below NF
+template <int KS, int NF>
+BB_D void store_features(uint16_t* s, const s8v (&B)[4][1], int wave, int lane)
+{
+    const int g = lane >> 4, c16 = lane & 15;
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int f = frag_feature(ks, g, j);
+            if (f < NF) s[f * kTrainLd + 16 * wave + c16] = uint16_t(B[ks][0][j]);
+        }
+}
+
+// dW tiles t = wave + kTrainWaves i of an MT x NT grid (t = mt NT + nt):
+// acc[i] += h^T dz over the pass's rows, both from feature-major tiles
+template <int MT, int NT, int NI>
+BB_D void dw_pass(const uint16_t* sh, const uint16_t* sz, f4v (&acc)[NI], int wave, int lane)
+{
+    const int g = lane >> 4, c16 = lane & 15;
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+        const int t = wave + kTrainWaves * i;
+        if (t >= MT * NT) continue;
+        const int mt = t / NT, nt = t % NT;
+        const uint16_t* a = sh + (16 * mt + c16) * kTrainLd + 8 * g;
+        const uint16_t* b = sz + (16 * nt + c16) * kTrainLd + 8 * g;
+#pragma unroll
+        for (int ks = 0; ks < kTrainRows / 32; ++ks)
+            acc[i] = mfma32(*reinterpret_cast<const s8v*>(a + 32 * ks), *reinterpret_cast<const s8v*>(b + 32 * ks),
+                            acc[i]);
+    }
+}
+
+// the accumulated dW tiles of one layer into a partial sum ([k][n], k-major)
+template <int MT, int NT, int NI>
+BB_D void dw_store(float* out, int N, const f4v (&acc)[NI], int wave, int lane)
+{
+    const int g = lane >> 4, c16 = lane & 15;
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+        const int t = wave + kTrainWaves * i;
+        if (t >= MT * NT) continue;
+        const int mt = t / NT, nt = t % NT;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) out[(16 * mt + 4 * g + j) * N + 16 * nt + c16] = acc[i][j];
+    }
+}
+
+constexpr int tiles_per_wave(int tiles) { return (tiles + kTrainWaves - 1) / kTrainWaves; }
+
+template <class NT>
+__global__ __launch_bounds__(64 * kTrainWaves) void k_mlp_train(const float* __restrict__ raw,
+                                                                const float* __restrict__ target, RowSpan span,
+                                                                const uint16_t* __restrict__ wf,
+                                                                const uint16_t* __restrict__ wb,
+                                                                float* __restrict__ partials,
+                                                                float* __restrict__ loss_part)
+{
+    static_assert(NT::fused && NT::W == 64, "the fused step is laid out for 64-wide hidden layers");
+    constexpr int W = NT::W, L = NT::L, HS = W / 32;
+    constexpr int T0 = NT::k(0) / 16 * (W / 16), TH = (W / 16) * (W / 16), TL = W / 16; // dW tiles per layer
+    __shared__ __align__(16) uint16_t sWF[NT::frag_elems];
+    __shared__ __align__(16) uint16_t sWB[NT::bfrag_elems];
+    __shared__ __align__(16) uint16_t sH[2][kIn * kTrainLd];
+    __shared__ __align__(16) uint16_t sZ[2][W * kTrainLd];
+    __shared__ float sLoss[kTrainWaves];
+    stage16(sWF, wf, NT::frag_elems);
+    stage16(sWB, wb, NT::bfrag_elems);
+    __syncthreads();
+    int off = 0, n = 0;
+    span_rows(span, off, n);
+    raw += size_t(off) * kRawStride;
+    target += size_t(off) * 3;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, c16 = lane & 15;
+    const float inv_count = n > 0 ? 1.0f / (3.0f * float(n)) : 0.0f;
+    const EncLane el = enc_lane(g);
+    f4v a0[tiles_per_wave(T0)], ah[L - 1][tiles_per_wave(TH)], aL[tiles_per_wave(TL)];
+#pragma unroll
+    for (int i = 0; i < tiles_per_wave(T0); ++i) a0[i] = f4v{ 0.0f, 0.0f, 0.0f, 0.0f };
+#pragma unroll
+    for (int l = 0; l < L - 1; ++l)
+#pragma unroll
+        for (int i = 0; i < tiles_per_wave(TH); ++i) ah[l][i] = f4v{ 0.0f, 0.0f, 0.0f, 0.0f };
+#pragma unroll
+    for (int i = 0; i < tiles_per_wave(TL); ++i) aL[i] = f4v{ 0.0f, 0.0f, 0.0f, 0.0f };
+    float loss = 0.0f;
+    int buf = 0;
+    const int passes = (n + kTrainRows - 1) / kTrainRows;
+    for (int ps = blockIdx.x; ps < passes; ps += gridDim.x) {
+        const int row = ps * kTrainRows + 16 * wave + c16;
+        const bool ok = row < n;
+        // forward: h[0] is layer 0's input, h[l] layer l's (l = 1..L)
+        s8v h0[4][1], h[L + 1][4][1];
+        enc_operands(raw + size_t(ok ? row : 0) * kRawStride, ok, g, el, h0[0][0], h0[1][0], h0[2][0]);
+        {
+            f4v acc[W / 16][1];
+            qlayer<NT::kss(0), W / 16, 1>(sWF + NT::frag_off(0) * 512, h0, acc, lane);
+            qactivate<W / 16, 1>(acc, h[1]);
+        }
+#pragma unroll
+        for (int l = 1; l < L; ++l) {
+            f4v acc[W / 16][1];
+            qlayer<HS, W / 16, 1>(sWF + NT::frag_off(l) * 512, h[l], acc, lane);
+            qactivate<W / 16, 1>(acc, h[l + 1]);
+        }
+        f4v y[1][1];
+        qlayer<HS, 1, 1>(sWF + NT::frag_off(L) * 512, h[L], y, lane);
+        // dz_L: dL/dy of channels 0..2 (lanes of group 0, slots 0..2), as the
+        // B operand of the first gradient-chain product (features 0..15 in
+        // slots 0..3 of k-step 0, the padding to 32 zero)
+        float gy[3] = { 0.0f, 0.0f, 0.0f };
+        if (g == 0 && ok) {
+#pragma unroll
+            for (int j = 0; j < 3; ++j) gy[j] = loss_grad(y[0][0][j], target[size_t(row) * 3 + j], inv_count, loss);
+        }
+        s8v dz[4][1];
+        {
+            const u4v d{ pack_bf16(gy[0], gy[1]), pack_bf16(gy[2], 0.0f), 0u, 0u };
+            dz[0][0] = __builtin_bit_cast(s8v, d);
+        }
+        store_features<HS, W>(sH[buf], h[L], wave, lane);
+        store_features<1, kOutCols>(sZ[buf], dz, wave, lane);
+        __syncthreads();
+        dw_pass<W / 16, 1>(sH[buf], sZ[buf], aL, wave, lane);
+        buf ^= 1;
+        // the chain: dz_{l-1} from dz_l, then dW_{l-1}
+#pragma unroll
+        for (int l = L; l >= 1; --l) {
+            f4v acc[W / 16][1];
+            static_assert(NT::bkss(L) == 1 && NT::bkss(1) == HS, "one k-step from the output layer's 16 columns");
+            if (l == L)
+                qlayer<1, W / 16, 1>(sWB + NT::bfrag_off(L) * 512, dz, acc, lane);
+            else
+                qlayer<HS, W / 16, 1>(sWB + NT::bfrag_off(l) * 512, dz, acc, lane);
+#pragma unroll
+            for (int ks = 0; ks < HS; ++ks) {
+                const f4v lo = acc[2 * ks][0], hi = acc[2 * ks + 1][0];
+                const u4v m = __builtin_bit_cast(u4v, h[l][ks][0]);
+                const u4v d{ mask_positive(pack_bf16(lo[0], lo[1]), m[0]), mask_positive(pack_bf16(lo[2], lo[3]), m[1]),
+                             mask_positive(pack_bf16(hi[0], hi[1]), m[2]), mask_positive(pack_bf16(hi[2], hi[3]), m[3]) };
+                dz[ks][0] = __builtin_bit_cast(s8v, d);
+            }
+            if (l == 1)
+                store_features<NT::kss(0), kIn>(sH[buf], h0, wave, lane);
+            else
+                store_features<HS, W>(sH[buf], h[l - 1], wave, lane);
+            store_features<HS, W>(sZ[buf], dz, wave, lane);
+            __syncthreads();
+            if (l == 1)
+                dw_pass<NT::k(0) / 16, W / 16>(sH[buf], sZ[buf], a0, wave, lane);
+            else
+                dw_pass<W / 16, W / 16>(sH[buf], sZ[buf], ah[l - 2], wave, lane);
+            buf ^= 1;
+        }
+    }
+    // this block's partial sums (blocks beyond the passes leave zeros)
+    float* out = partials + size_t(blockIdx.x) * NT::params;
+    dw_store<NT::k(0) / 16, W / 16>(out + NT::off(0), W, a0, wave, lane);
+#pragma unroll
+    for (int l = 1; l < L; ++l) dw_store<W / 16, W / 16>(out + NT::off(l), W, ah[l - 1], wave, lane);
+    dw_store<W / 16, 1>(out + NT::off(L), kOutCols, aL, wave, lane);
+    for (int o = 32; o > 0; o >>= 1) loss += __shfl_xor(loss, o);
+    if (lane == 0) sLoss[wave] = loss;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float t = 0.0f;
+        for (int w = 0; w < kTrainWaves; ++w) t += sLoss[w];
+        loss_part[blockIdx.x] = t;
+    }
 }
 
 } // namespace bb
@@ -760,11 +1027,12 @@ struct BbNrc
     float* g = nullptr;
     float* partials = nullptr; // [row range][params]
     float* loss = nullptr;
+    float* loss_part = nullptr; // [block] loss partial sums of the last step
     int* status = nullptr; // [0]: diverged (non-finite loss, weight or prediction), sticky until new weights
     uint16_t* WT = nullptr;
     uint16_t* W = nullptr;
-    uint16_t* WF = nullptr;   // weight fragments of the query kernel
-    bool wf_stale = true;     // the master weights changed since WF was made
+    uint16_t* WF = nullptr;   // weight fragments of the query kernel (and the fused forward)
+    uint16_t* WB = nullptr;   // weight fragments of the fused gradient chain
     uint16_t* acts = nullptr; // saved layer inputs of the last training batch
     uint16_t* dzs = nullptr;  // saved layer-output gradients of the last training batch
     float* y = nullptr;
@@ -783,6 +1051,10 @@ bool nrc_ok(hipError_t e, const char* what)
 }
 
 // blocks of a forward or gradient-chain launch over at most cap rows
+// bytes of the query fragments (both shapes) and of the gradient-chain fragments
+constexpr size_t kFragBytes = size_t(bb::NetC::frag_elems > bb::NetL::frag_elems ? bb::NetC::frag_elems : bb::NetL::frag_elems) * 2;
+constexpr size_t kBFragBytes = size_t(bb::NetC::bfrag_elems) * 2;
+
 int nrc_grid(int cap)
 {
     const int waves = (cap + bb::kRows - 1) / bb::kRows;
@@ -797,33 +1069,34 @@ int params_of(int width, int hidden)
     return -1;
 }
 
-// the Adam step (apply), or only the bf16 copies (reduce = 0: no gradient)
+// the Adam step (apply), or only the bf16 copies (reduce = 0: no gradient);
+// nloss per-block loss sums, fused_grid as k_reduce_adam takes it
 template <class NT>
-bool nrc_reduce_t(BbNrc* x, hipStream_t st, bb::RowSpan span, bool reduce, bool apply, float c1, float c2)
+bool nrc_reduce_t(BbNrc* x, hipStream_t st, bb::RowSpan span, bool reduce, int fused_grid, int nloss, bool apply,
+                  float c1, float c2)
 {
     hipLaunchKernelGGL(bb::k_reduce_adam<NT>, dim3((NT::params + 63) / 64), dim3(256), 0, st, x->w, x->m, x->v, x->g,
-                       x->partials, span, reduce ? 1 : 0, x->lr, c1, c2, apply ? 1 : 0, x->WT, x->W, x->loss,
-                       x->status);
+                       x->partials, span, reduce ? 1 : 0, fused_grid, x->lr, c1, c2, apply ? 1 : 0, x->WT, x->W, x->WF,
+                       x->WB, x->loss_part, nloss, x->loss, x->status);
     return nrc_ok(hipGetLastError(), "k_reduce_adam");
 }
 
-bool nrc_reduce(BbNrc* x, hipStream_t st, bb::RowSpan span, bool reduce, bool apply, float c1, float c2)
+bool nrc_reduce(BbNrc* x, hipStream_t st, bb::RowSpan span, bool reduce, int fused_grid, int nloss, bool apply,
+                float c1, float c2)
 {
-    x->wf_stale = true;
-    return x->width == bb::NetC::W ? nrc_reduce_t<bb::NetC>(x, st, span, reduce, apply, c1, c2)
-                                   : nrc_reduce_t<bb::NetL>(x, st, span, reduce, apply, c1, c2);
+    return x->width == bb::NetC::W ? nrc_reduce_t<bb::NetC>(x, st, span, reduce, fused_grid, nloss, apply, c1, c2)
+                                   : nrc_reduce_t<bb::NetL>(x, st, span, reduce, fused_grid, nloss, apply, c1, c2);
 }
 
 // refreshes the bf16 copies from the fp32 master weights
-bool nrc_refresh(BbNrc* x, hipStream_t st) { return nrc_reduce(x, st, bb::RowSpan{ nullptr, 0, 0, 0 }, false, false, 1.0f, 1.0f); }
+bool nrc_refresh(BbNrc* x, hipStream_t st)
+{
+    return nrc_reduce(x, st, bb::RowSpan{ nullptr, 0, 0, 0 }, false, 0, 0, false, 1.0f, 1.0f);
+}
 
 template <class NT>
 void launch_query(BbNrc* x, const float* inputs_dev, bb::RowSpan span, float* out_dev, hipStream_t st)
 {
-    if (x->wf_stale) {
-        hipLaunchKernelGGL(bb::k_weights_frag<NT>, dim3((NT::frag_elems + 255) / 256), dim3(256), 0, st, x->w, x->WF);
-        x->wf_stale = false;
-    }
     const long long per_block = (long long)(NT::qblock / 64) * 16 * NT::qtiles;
     const int grid = int(std::max<long long>(1, std::min<long long>((span.cap + per_block - 1) / per_block, 2048)));
     hipLaunchKernelGGL(bb::k_mlp_query<NT>, dim3(grid), dim3(NT::qblock), 0, st, inputs_dev, span, x->WF, out_dev);
@@ -835,7 +1108,7 @@ int query_span(BbNrc* x, const float* inputs_dev, bb::RowSpan span, float* out_d
         launch_query<bb::NetC>(x, inputs_dev, span, out_dev, st);
     else
         launch_query<bb::NetL>(x, inputs_dev, span, out_dev, st);
-    return nrc_ok(hipGetLastError(), "k_mlp_forward") ? BBGPU_OK : BBGPU_ERR_DEVICE;
+    return nrc_ok(hipGetLastError(), "k_mlp_query") ? BBGPU_OK : BBGPU_ERR_DEVICE;
 }
 
 // the saved activations, gradients and outputs of a step on up to rows rows
@@ -859,24 +1132,35 @@ bool ensure_train_buffers(BbNrc* x, int rows)
 }
 
 // forward, gradient chain, dW partial sums and the reduction (with the Adam
-// step when apply) of one step over the rows span selects; at most
-// max_len rows (which sizes the launches)
+// step when apply) of one step over the rows span selects; at most max_len
+// rows (which sizes the launches).  The 64-wide network takes one fused
+// kernel; the wide one the forward, gradient-chain and dW kernels, with its
+// activations and gradients saved in between.
 template <class NT>
 int train_step(BbNrc* x, const float* inputs_dev, const float* targets_dev, bb::RowSpan span, int max_len, bool apply,
                hipStream_t st)
 {
-    if (!ensure_train_buffers<NT>(x, max_len)) return BBGPU_ERR_NOMEM;
-    const int chunks = (max_len + bb::kRows - 1) / bb::kRows;
-    const bb::NrcWeights Wt{ x->WT, x->W };
-    const int grid = nrc_grid(max_len);
-    hipLaunchKernelGGL(bb::k_fill0, dim3(1), dim3(64), 0, st, x->loss, 1);
-    hipLaunchKernelGGL((bb::k_mlp_forward<NT, true>), dim3(grid), dim3(bb::kNrcBlock), 0, st, inputs_dev, span, Wt, x->y,
-                       x->acts);
-    hipLaunchKernelGGL(bb::k_mlp_dz<NT>, dim3(grid), dim3(bb::kNrcBlock), 0, st, x->y, targets_dev, span, x->W, x->acts,
-                       x->dzs, x->loss);
-    hipLaunchKernelGGL(bb::k_mlp_dw<NT>, dim3(NT::groups * bb::dw_splits(chunks, NT::groups)), dim3(64), 0, st, x->acts,
-                       x->dzs, span, x->partials);
-    if (!nrc_ok(hipGetLastError(), "k_mlp_dw")) return BBGPU_ERR_DEVICE;
+    int fused_grid = 0, nloss = 0;
+    if constexpr (NT::fused) {
+        fused_grid = bb::train_grid(max_len);
+        nloss = fused_grid;
+        hipLaunchKernelGGL(bb::k_mlp_train<NT>, dim3(fused_grid), dim3(64 * bb::kTrainWaves), 0, st, inputs_dev,
+                           targets_dev, span, x->WF, x->WB, x->partials, x->loss_part);
+        if (!nrc_ok(hipGetLastError(), "k_mlp_train")) return BBGPU_ERR_DEVICE;
+    } else {
+        if (!ensure_train_buffers<NT>(x, max_len)) return BBGPU_ERR_NOMEM;
+        const int chunks = (max_len + bb::kRows - 1) / bb::kRows;
+        const bb::NrcWeights Wt{ x->WT, x->W };
+        const int grid = nrc_grid(max_len);
+        nloss = grid;
+        hipLaunchKernelGGL((bb::k_mlp_forward<NT, true>), dim3(grid), dim3(bb::kNrcBlock), 0, st, inputs_dev, span, Wt,
+                           x->y, x->acts);
+        hipLaunchKernelGGL(bb::k_mlp_dz<NT>, dim3(grid), dim3(bb::kNrcBlock), 0, st, x->y, targets_dev, span, x->W,
+                           x->acts, x->dzs, x->loss_part);
+        hipLaunchKernelGGL(bb::k_mlp_dw<NT>, dim3(NT::groups * bb::dw_splits(chunks, NT::groups)), dim3(64), 0, st,
+                           x->acts, x->dzs, span, x->partials);
+        if (!nrc_ok(hipGetLastError(), "k_mlp_dw")) return BBGPU_ERR_DEVICE;
+    }
     // Adam bias corrections of the step about to be taken (every launched
     // step counts, also one that finds no rows and leaves the weights alone)
     float c1 = 1.0f, c2 = 1.0f;
@@ -885,7 +1169,7 @@ int train_step(BbNrc* x, const float* inputs_dev, const float* targets_dev, bb::
         c1 = 1.0f - std::pow(0.9f, float(x->step));
         c2 = 1.0f - std::pow(0.99f, float(x->step));
     }
-    if (!nrc_reduce(x, st, span, true, apply, c1, c2)) return BBGPU_ERR_DEVICE;
+    if (!nrc_reduce(x, st, span, true, fused_grid, nloss, apply, c1, c2)) return BBGPU_ERR_DEVICE;
     return BBGPU_OK;
 }
 
@@ -963,7 +1247,8 @@ int bbgpu_nrc_create(const BbNrcDesc* desc, BbNrc** out)
               nrc_ok(hipMalloc(&x->v, PS * 4), "hipMalloc") && nrc_ok(hipMalloc(&x->g, PS * 4), "hipMalloc") &&
               nrc_ok(hipMalloc(&x->loss, 4), "hipMalloc") && nrc_ok(hipMalloc(&x->status, 16), "hipMalloc") &&
               nrc_ok(hipMalloc(&x->WT, PS * 2), "hipMalloc") && nrc_ok(hipMalloc(&x->W, PS * 2), "hipMalloc") &&
-              nrc_ok(hipMalloc(&x->WF, size_t(std::max(bb::NetC::frag_elems, bb::NetL::frag_elems)) * 2), "hipMalloc") &&
+              nrc_ok(hipMalloc(&x->WF, kFragBytes), "hipMalloc") && nrc_ok(hipMalloc(&x->WB, kBFragBytes), "hipMalloc") &&
+              nrc_ok(hipMalloc(&x->loss_part, size_t(bb::kMaxSplits) * 4), "hipMalloc") &&
               nrc_ok(hipMalloc(&x->partials, size_t(bb::kMaxSplits) * PS * 4), "hipMalloc");
     if (!ok) {
         bbgpu_nrc_destroy(x);
@@ -989,7 +1274,10 @@ int bbgpu_nrc_create(const BbNrcDesc* desc, BbNrc** out)
     ok = nrc_ok(hipMemcpy(x->w, w.data(), PS * 4, hipMemcpyHostToDevice), "hipMemcpy") &&
          nrc_ok(hipMemset(x->m, 0, PS * 4), "hipMemset") && nrc_ok(hipMemset(x->v, 0, PS * 4), "hipMemset") &&
          nrc_ok(hipMemset(x->g, 0, PS * 4), "hipMemset") && nrc_ok(hipMemset(x->status, 0, 16), "hipMemset") &&
-         nrc_ok(hipMemset(x->loss, 0, 4), "hipMemset") && nrc_refresh(x, nullptr) &&
+         nrc_ok(hipMemset(x->loss, 0, 4), "hipMemset") && nrc_ok(hipMemset(x->loss_part, 0, bb::kMaxSplits * 4), "hipMemset") &&
+         // the fragments' padding (features beyond a layer's) stays zero
+         nrc_ok(hipMemset(x->WF, 0, kFragBytes), "hipMemset") && nrc_ok(hipMemset(x->WB, 0, kBFragBytes), "hipMemset") &&
+         nrc_refresh(x, nullptr) &&
          nrc_ok(hipDeviceSynchronize(), "hipDeviceSynchronize");
     if (!ok) {
         bbgpu_nrc_destroy(x);
@@ -1003,8 +1291,8 @@ int bbgpu_nrc_destroy(BbNrc* x)
 {
     if (!x) return BBGPU_OK;
     for (void* p : { (void*)x->w, (void*)x->m, (void*)x->v, (void*)x->g, (void*)x->partials, (void*)x->loss,
-                     (void*)x->status, (void*)x->WT, (void*)x->W, (void*)x->WF, (void*)x->acts, (void*)x->dzs,
-                     (void*)x->y })
+                     (void*)x->loss_part, (void*)x->status, (void*)x->WT, (void*)x->W, (void*)x->WF, (void*)x->WB,
+                     (void*)x->acts, (void*)x->dzs, (void*)x->y })
         if (p) (void)hipFree(p);
     delete x;
     return BBGPU_OK;
