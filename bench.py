@@ -269,12 +269,16 @@ def mlp_flops_query(width=64, layers=4):
 
 
 MLP_FLOPS_QUERY = mlp_flops_query()
+# rows per Adam step of the cached renders: up to 65,536, taking a frame's
+# training rows batch by batch (the README scene makes about 68K per frame,
+# so its four steps take 65,536, the rest, 65,536, the rest)
+CACHE_TRAIN_BATCH = 65536
 BF16_PEAK_TFLOPS = 2500.0
 
 
 def cache_comparison(args, g, cam, ref, name, query_primary, train_stride=16, learning_rate=1e-3, train_steps=4,
                      spp=1, max_bounces=8, shape=(64, 4), frames=None, keep_every=16, query_direct=False,
-                     query_first=0):
+                     query_first=0, train_batch=None):
     """Radiance cache vs plain path tracing at equal wall-clock time, `spp`
     samples per frame, against `ref` (a plain render with disjoint samples).
     Plain PT renders `frames` (default `cache_frames`) frames; the cached
@@ -321,7 +325,9 @@ def cache_comparison(args, g, cam, ref, name, query_primary, train_stride=16, le
         per_frame = [err(sd, cd) for f, sd, cd in keep if f >= done // 2]
         return err(acc[0], acc[1]), done, secs, float(np.mean(per_frame)) if per_frame else float("nan")
 
-    cp = scenes.cache_params(train_stride=train_stride, train_batch=65536, train_steps=train_steps,
+    if train_batch is None:
+        train_batch = CACHE_TRAIN_BATCH
+    cp = scenes.cache_params(train_stride=train_stride, train_batch=train_batch, train_steps=train_steps,
                              query_primary=query_primary, query_direct=query_direct, query_first=query_first)
     frames = args.cache_frames if frames is None else frames
     flops_q = mlp_flops_query(*shape)
@@ -351,6 +357,7 @@ def cache_comparison(args, g, cam, ref, name, query_primary, train_stride=16, le
     rows = sum(st.train_rows for st in stats)
     ms_q = sum(st.ms_query for st in stats)
     ms_t = sum(st.ms_train for st in stats)
+    ms_r = sum(st.ms_rows for st in stats)
     tflops = (flops_q * q + 3 * flops_q * rows) / max(ms_q + ms_t, 1e-9) / 1e9
     frame_plain = t_plain / n_plain
     frame_cache = t_cache / n_cache
@@ -370,6 +377,7 @@ def cache_comparison(args, g, cam, ref, name, query_primary, train_stride=16, le
         "train_stride": train_stride,
         "learning_rate": learning_rate,
         "train_steps_per_frame": train_steps,
+        "train_batch": train_batch,
         "seconds": round(t_plain, 3),
         "plain_frames": n_plain,
         "plain_relmse": round(e_plain, 7),
@@ -381,6 +389,7 @@ def cache_comparison(args, g, cam, ref, name, query_primary, train_stride=16, le
                       "ratio_equal_time": round(f_cache / f_plain_equal_time, 4)},
         "cache_ms_query": round(ms_q, 1),
         "cache_ms_train": round(ms_t, 1),
+        "cache_ms_rows": round(ms_r, 1),
         "mlp": {"bound": "mfma", "achieved": round(tflops, 3), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(tflops / BF16_PEAK_TFLOPS, 6),
                 "flops_per_query": flops_q, "flops_per_training_row": 3 * flops_q},

@@ -12,7 +12,7 @@ from ctypes import POINTER, c_char_p, c_double, c_float, c_int32, c_size_t, c_ui
 
 import numpy as np
 
-ABI_VERSION = 19  # BBGPU_ABI_VERSION of include/bbgpu.h that these structures mirror
+ABI_VERSION = 20  # BBGPU_ABI_VERSION of include/bbgpu.h that these structures mirror
 
 LIB_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(LIB_DIR, "libbbgpu.so")
@@ -252,6 +252,7 @@ class BbCacheStats(ctypes.Structure):
         ("loss", c_float),
         ("diverged", c_int32),
         ("passes_redone", c_int32),
+        ("ms_rows", c_double),
     ]
 
 

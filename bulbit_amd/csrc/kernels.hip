@@ -2357,24 +2357,30 @@ int render_impl(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams
         if (cache) {
             // cache queries of the rendering paths, then this pass's training
             // rows; every count stays on the device
+            // spans: the network's inference (3), the predictions applied and
+            // the training rows built (5), the optimizer steps (4)
             hipEvent_t q0 = cache_timed ? mark() : nullptr;
             const int rq = nrc_query_counted(cache->nrc, Cv.qraw, Cv.counters, Cv.max_queries, Cv.qpred, st);
             if (rq != BBGPU_OK) return rq;
+            hipEvent_t q1 = cache_timed ? mark() : nullptr;
             hipLaunchKernelGGL(k_cache_apply, dim3(std::min(grid_for(n_paths, kShadeBlock), 4096)), dim3(kShadeBlock), 0,
                                st, Cv, P.L, pass);
             hipLaunchKernelGGL(k_cache_targets, dim3((n_train + kShadeBlock - 1) / kShadeBlock), dim3(kShadeBlock), 0,
                                st, Cv, P.L, n_train, stride);
-            hipEvent_t q1 = cache_timed ? mark() : nullptr;
+            hipEvent_t q2 = cache_timed ? mark() : nullptr;
             const int batch = cache->cp.train_batch > 0 ? cache->cp.train_batch : 65536;
             for (int s1 = 0; s1 < cache->cp.train_steps; ++s1) {
                 const int rt = nrc_train_counted(cache->nrc, Cv.rows, Cv.targets, Cv.counters + 1, Cv.max_rows, batch,
                                                  s1, st);
                 if (rt != BBGPU_OK) return rt;
             }
+            hipEvent_t q3 = cache_timed ? mark() : nullptr;
             hipLaunchKernelGGL(k_cache_tally, dim3(1), dim3(64), 0, st, Cv, cache->cp.train_steps, batch);
             if (cache_timed) {
                 spans.push_back({ q0, q1, 3 });
-                spans.push_back({ q1, mark(), 4 });
+                spans.push_back({ q1, q2, 5 });
+                spans.push_back({ q2, q3, 4 });
+                spans.push_back({ q3, mark(), 5 });
             }
         }
         a = timed ? mark() : nullptr;
@@ -2446,6 +2452,7 @@ int render_impl(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams
                 hipEventElapsedTime(&d, sp.a, sp.b);
                 if (sp.kind == 3) cache->stats->ms_query += d;
                 if (sp.kind == 4) cache->stats->ms_train += d;
+                if (sp.kind == 5) cache->stats->ms_rows += d;
             }
         }
     }

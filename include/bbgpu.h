@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define BBGPU_ABI_VERSION 19
+#define BBGPU_ABI_VERSION 20
 
 /* status codes */
 enum {
@@ -491,8 +491,8 @@ typedef struct BbCacheParams {
 
 typedef struct BbCacheStats {
     double ms_total;   /* the whole call on the stream */
-    double ms_query;   /* cache inference and its application */
-    double ms_train;   /* training rows and optimizer steps */
+    double ms_query;   /* the network's inference (query kernel) */
+    double ms_train;   /* the network's optimizer steps (training kernels) */
     int64_t queries;
     int64_t train_rows; /* rows trained on, counted once per step that used them */
     float loss;        /* relative L2 loss of the last step */
@@ -500,6 +500,7 @@ typedef struct BbCacheStats {
                           detection on were rendered by the plain tracer, and so are later calls until new
                           weights are set (bbgpu_nrc_set_weights) */
     int32_t passes_redone; /* passes whose cached samples were dropped and rendered again without the cache */
+    double ms_rows;    /* the predictions applied to the paths and the training rows built and counted */
 } BbCacheStats;
 
 /* bbgpu_render with the radiance cache (same film and parameter contract;

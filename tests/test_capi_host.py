@@ -22,7 +22,7 @@ def test_library_exports_every_declared_entry_point(bblib):
     assert names and sorted(capi.EXPORTS) == names
     for n in names:
         assert hasattr(bblib, n), n
-    assert bblib.bbgpu_abi_version() == capi.ABI_VERSION == 19
+    assert bblib.bbgpu_abi_version() == capi.ABI_VERSION == 20
 
 
 def _build(bblib, sc):
