@@ -155,6 +155,39 @@ def test_nrc_gradient_matches_oracle(nrc):
 
 
 @pytest.mark.gpu
+def test_nrc_gradient_over_many_passes_matches_oracle(nrc):
+    """40,000 rows: more row passes than the fused step has blocks (256 x
+    128 rows), so blocks accumulate dW over several passes; the three-kernel
+    step of the wide network splits them into many row ranges."""
+    w = _init_weights(14, (nrc.width, nrc.layers))
+    nrc.set_weights(w)
+    raw = _batch(40000, 15)
+    t = _field(raw)
+    loss, g = nrc.train(raw, t, apply=False, want_grad=True)
+    lr, gr = ref.loss_and_grad(w, raw, t)
+    cos = g @ gr / np.linalg.norm(g) / np.linalg.norm(gr)
+    ratio = np.linalg.norm(g) / np.linalg.norm(gr)
+    print(f"[nrc] 40000 rows: loss gpu {loss:.6f} oracle {lr:.6f}, cosine {cos:.6f}, norm ratio {ratio:.5f}",
+          flush=True)
+    assert abs(loss - lr) <= 2e-3 * lr
+    assert cos > 0.999 and abs(ratio - 1) < 0.01
+
+
+@pytest.mark.gpu
+def test_nrc_gradient_is_deterministic(nrc):
+    """The partial sums of dW and of the loss are added in a fixed order (no
+    atomics): the same batch gives the same bits."""
+    w = _init_weights(16, (nrc.width, nrc.layers))
+    nrc.set_weights(w)
+    raw = _batch(20000, 17)
+    t = _field(raw)
+    l1, g1 = nrc.train(raw, t, apply=False, want_grad=True)
+    l2, g2 = nrc.train(raw, t, apply=False, want_grad=True)
+    assert np.array_equal(g1.view(np.uint32), g2.view(np.uint32))
+    assert np.float32(l1).view(np.uint32) == np.float32(l2).view(np.uint32)
+
+
+@pytest.mark.gpu
 def test_nrc_adam_step_matches_oracle(nrc):
     w = _init_weights(12, (nrc.width, nrc.layers))
     nrc.set_weights(w)
