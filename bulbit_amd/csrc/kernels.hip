@@ -1347,13 +1347,14 @@ __global__ void k_cache_tally(CacheBufs C, int steps, int batch)
 // training rows of the pass: at each recorded vertex of a training path the
 // radiance it reflected towards the previous vertex, (L_final - L_vertex) /
 // throughput_vertex per channel; vertices with a vanishing throughput or a
-// non-finite target are skipped
+// non-finite target are skipped.  One thread per training path; its rows
+// are copied as whole float4s.
 __global__ __launch_bounds__(kShadeBlock) void k_cache_targets(CacheBufs C, const float4* __restrict__ L, int n_train,
                                                                 int stride)
 {
     const int tp = blockIdx.x * blockDim.x + threadIdx.x;
     if (tp >= n_train || C.state[0]) return; // no training once the network diverged
-    const float4 lf = L[tp * stride];
+    const float4 lf = L[size_t(tp) * stride];
     const int nv = C.tcount[tp];
     for (int v = 0; v < nv; ++v) {
         const size_t rec = size_t(tp) * C.max_vertices + v;
@@ -1366,7 +1367,10 @@ __global__ __launch_bounds__(kShadeBlock) void k_cache_targets(CacheBufs C, cons
         if (!(isfinite(tr) && isfinite(tg) && isfinite(tb))) continue;
         const int r = wave_alloc_active(&C.counters[1]);
         if (r >= C.max_rows) continue;
-        for (int k = 0; k < 16; ++k) C.rows[16 * size_t(r) + k] = C.traw[16 * rec + k];
+        const float4* src = reinterpret_cast<const float4*>(C.traw + 16 * rec);
+        float4* dst = reinterpret_cast<float4*>(C.rows + 16 * size_t(r));
+#pragma unroll
+        for (int k = 0; k < 4; ++k) dst[k] = src[k];
         C.targets[3 * size_t(r)] = fmaxf(tr, 0.0f);
         C.targets[3 * size_t(r) + 1] = fmaxf(tg, 0.0f);
         C.targets[3 * size_t(r) + 2] = fmaxf(tb, 0.0f);
