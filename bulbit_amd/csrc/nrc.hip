@@ -1094,11 +1094,33 @@ bool nrc_refresh(BbNrc* x, hipStream_t st)
     return nrc_reduce(x, st, bb::RowSpan{ nullptr, 0, 0, 0 }, false, 0, 0, false, 1.0f, 1.0f);
 }
 
+// blocks of the query kernel that are resident at once on the device (one
+// round): each block stages the weight fragments into LDS once, so a grid of
+// one round, with the waves looping over the rows, stages them least (the
+// row count is on the device; a grid sized for the capacity staged them up
+// to 8 times per CU)
+template <class NT>
+int query_round()
+{
+    static int blocks = 0;
+    if (!blocks) {
+        int dev = 0, cus = 0, per_cu = 0;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bb::k_mlp_query<NT>, NT::qblock, 0) != hipSuccess ||
+            per_cu <= 0)
+            per_cu = 1;
+        blocks = cus * per_cu;
+    }
+    return blocks;
+}
+
 template <class NT>
 void launch_query(BbNrc* x, const float* inputs_dev, bb::RowSpan span, float* out_dev, hipStream_t st)
 {
     const long long per_block = (long long)(NT::qblock / 64) * 16 * NT::qtiles;
-    const int grid = int(std::max<long long>(1, std::min<long long>((span.cap + per_block - 1) / per_block, 2048)));
+    const int grid =
+        int(std::max<long long>(1, std::min<long long>((span.cap + per_block - 1) / per_block, query_round<NT>())));
     hipLaunchKernelGGL(bb::k_mlp_query<NT>, dim3(grid), dim3(NT::qblock), 0, st, inputs_dev, span, x->WF, out_dev);
 }
 
