@@ -1347,26 +1347,72 @@ __global__ void k_cache_tally(CacheBufs C, int steps, int batch)
 // training rows of the pass: at each recorded vertex of a training path the
 // radiance it reflected towards the previous vertex, (L_final - L_vertex) /
 // throughput_vertex per channel; vertices with a vanishing throughput or a
-// non-finite target are skipped.  One thread per training path; its rows
-// are copied as whole float4s.
-__global__ __launch_bounds__(kShadeBlock) void k_cache_targets(CacheBufs C, const float4* __restrict__ L, int n_train,
-                                                                int stride)
+// non-finite target are skipped.  One thread per training path: it counts
+// its rows, the block places all its rows with one atomic (an exclusive scan
+// of the counts: same-address atomics per vertex were this kernel's limit),
+// and the thread copies them as whole float4s.  Only the first `used` rows
+// are written: the optimizer steps of the pass read no others (steps x batch
+// rows, batch by batch from the start, when the pass makes more), and a
+// block that finds the row counter at `used` already stops: the counter
+// then still ends at `used` or more, which is all the steps' batching and
+// the tally depend on.
+__device__ __forceinline__ bool cache_target(const CacheBufs& C, float4 lf, size_t rec, float& tr, float& tg, float& tb)
 {
+    float4 ls = C.tsnap[2 * rec];
+    const float4 bs = C.tsnap[2 * rec + 1];
+    if (!(bs.x > 1e-6f && bs.y > 1e-6f && bs.z > 1e-6f)) return false;
+    // indirect-only targets: a vertex the path did not continue from saw none
+    if (C.query_nee && ls.w == 0.0f) ls = lf;
+    tr = (lf.x - ls.x) / bs.x;
+    tg = (lf.y - ls.y) / bs.y;
+    tb = (lf.z - ls.z) / bs.z;
+    return isfinite(tr) && isfinite(tg) && isfinite(tb);
+}
+
+__global__ __launch_bounds__(kShadeBlock) void k_cache_targets(CacheBufs C, const float4* __restrict__ L, int n_train,
+                                                                int stride, int used)
+{
+    constexpr int kWaves = kShadeBlock / 64;
+    __shared__ int s_wave[kWaves + 1];
     const int tp = blockIdx.x * blockDim.x + threadIdx.x;
-    if (tp >= n_train || C.state[0]) return; // no training once the network diverged
-    const float4 lf = L[size_t(tp) * stride];
-    const int nv = C.tcount[tp];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (threadIdx.x == 0) s_wave[kWaves] = __atomic_load_n(&C.counters[1], __ATOMIC_RELAXED) >= used;
+    __syncthreads();
+    // no training once the network diverged; no rows any step would read
+    if (C.state[0] || s_wave[kWaves]) return; // uniform over the block
+    const bool on = tp < n_train;
+    const float4 lf = on ? L[size_t(tp) * stride] : float4{ 0.0f, 0.0f, 0.0f, 0.0f };
+    const int nv = on ? C.tcount[tp] : 0;
+    int cnt = 0;
     for (int v = 0; v < nv; ++v) {
+        float tr, tg, tb;
+        cnt += cache_target(C, lf, size_t(tp) * C.max_vertices + v, tr, tg, tb) ? 1 : 0;
+    }
+    // inclusive scan of the counts over the wave, then over the waves
+    int inc = cnt;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(inc, o);
+        if (lane >= o) inc += y;
+    }
+    __syncthreads(); // s_wave[kWaves] read by every thread above
+    if (lane == 63) s_wave[wave] = inc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < kWaves; ++w) {
+            const int x = s_wave[w];
+            s_wave[w] = t;
+            t += x;
+        }
+        s_wave[kWaves] = t ? atomicAdd(&C.counters[1], t) : 0;
+    }
+    __syncthreads();
+    int r = s_wave[kWaves] + s_wave[wave] + inc - cnt;
+    if (cnt == 0 || r >= used) return;
+    for (int v = 0; v < nv && r < used; ++v) {
         const size_t rec = size_t(tp) * C.max_vertices + v;
-        float4 ls = C.tsnap[2 * rec];
-        const float4 bs = C.tsnap[2 * rec + 1];
-        if (!(bs.x > 1e-6f && bs.y > 1e-6f && bs.z > 1e-6f)) continue;
-        // indirect-only targets: a vertex the path did not continue from saw none
-        if (C.query_nee && ls.w == 0.0f) ls = lf;
-        const float tr = (lf.x - ls.x) / bs.x, tg = (lf.y - ls.y) / bs.y, tb = (lf.z - ls.z) / bs.z;
-        if (!(isfinite(tr) && isfinite(tg) && isfinite(tb))) continue;
-        const int r = wave_alloc_active(&C.counters[1]);
-        if (r >= C.max_rows) continue;
+        float tr, tg, tb;
+        if (!cache_target(C, lf, rec, tr, tg, tb)) continue;
         const float4* src = reinterpret_cast<const float4*>(C.traw + 16 * rec);
         float4* dst = reinterpret_cast<float4*>(C.rows + 16 * size_t(r));
 #pragma unroll
@@ -1374,6 +1420,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_cache_targets(CacheBufs C, cons
         C.targets[3 * size_t(r)] = fmaxf(tr, 0.0f);
         C.targets[3 * size_t(r) + 1] = fmaxf(tg, 0.0f);
         C.targets[3 * size_t(r) + 2] = fmaxf(tb, 0.0f);
+        ++r;
     }
 }
 
@@ -2369,10 +2416,11 @@ int render_impl(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams
             hipEvent_t q1 = cache_timed ? mark() : nullptr;
             hipLaunchKernelGGL(k_cache_apply, dim3(std::min(grid_for(n_paths, kShadeBlock), 4096)), dim3(kShadeBlock), 0,
                                st, Cv, P.L, pass);
-            hipLaunchKernelGGL(k_cache_targets, dim3((n_train + kShadeBlock - 1) / kShadeBlock), dim3(kShadeBlock), 0,
-                               st, Cv, P.L, n_train, stride);
-            hipEvent_t q2 = cache_timed ? mark() : nullptr;
             const int batch = cache->cp.train_batch > 0 ? cache->cp.train_batch : 65536;
+            const int used_rows = int(std::min<long long>(Cv.max_rows, (long long)cache->cp.train_steps * batch));
+            hipLaunchKernelGGL(k_cache_targets, dim3((n_train + kShadeBlock - 1) / kShadeBlock), dim3(kShadeBlock), 0,
+                               st, Cv, P.L, n_train, stride, used_rows);
+            hipEvent_t q2 = cache_timed ? mark() : nullptr;
             for (int s1 = 0; s1 < cache->cp.train_steps; ++s1) {
                 const int rt = nrc_train_counted(cache->nrc, Cv.rows, Cv.targets, Cv.counters + 1, Cv.max_rows, batch,
                                                  s1, st);
