@@ -11,17 +11,21 @@
 // f = 0..11 and axis a (72 values, the first 64 kept), then direction (3),
 // normal (3), roughness (1), albedo (3), zeros to 80.
 //
-// Matrix products use v_mfma_f32_16x16x16_bf16: a 16x16 tile of A (rows x k)
-// comes from lane l as A[l % 16][4 (l / 16) + j], j = 0..3; B (k x n) as
-// B[4 (l / 16) + j][l % 16]; the f32 result D[4 (l / 16) + j][l % 16].
+// Matrix products use v_mfma_f32_16x16x32_bf16: a 16x16 tile of A (rows x k)
+// comes from lane l as A[l % 16][8 (l / 16) + j], j = 0..7; B (k x n) as
+// B[8 (l / 16) + j][l % 16]; the f32 result D[4 (l / 16) + j][l % 16],
+// j = 0..3 (a K of 16 left over takes the 16x16x16 form, 4 values per lane,
+// whose result layout is the same).  The query kernel and the fused
+// training step of the 64-wide network permute the k slots (frag_feature).
 //
-// Training step: the forward pass saves every layer's bf16 input per chunk of
-// 64 rows, feature-major; k_mlp_dz runs the gradient chain dz_{l-1} =
-// (dz_l W_l^T) masked by h_l > 0 per chunk and saves each dz_l, also
-// feature-major; k_mlp_dw forms dW_l = h_l^T dz_l as a GEMM over the rows
-// (whose operands are exactly those saved layouts), split over row ranges
-// into partial sums; k_reduce_adam adds the partial sums in a fixed order
-// and takes the Adam step.
+// Training step of the 128-wide network: the forward pass saves every
+// layer's bf16 input per chunk of 64 rows, feature-major; k_mlp_dz runs the
+// gradient chain dz_{l-1} = (dz_l W_l^T) masked by h_l > 0 per chunk and
+// saves each dz_l, also feature-major; k_mlp_dw forms dW_l = h_l^T dz_l as a
+// GEMM over the rows (whose operands are exactly those saved layouts), split
+// over row ranges into partial sums; k_reduce_adam adds the partial sums in
+// a fixed order and takes the Adam step.  The 64-wide network's step is one
+// fused kernel (k_mlp_train).
 #include "bb_math.h"
 #include "../../include/bbgpu.h"
 
@@ -71,7 +75,7 @@ struct Net
     static constexpr bool resident = params * 2 + 4 * kRows * ld * 2 <= 96 * 1024;
     // Layer inputs saved by the training forward pass, per chunk of kRows
     // rows, feature-major ([feature][row in chunk]): h_0 (kIn features), then
-    // h_1..h_L (W each).  Four consecutive rows of one feature are one 8-byte
+    // h_1..h_L (W each).  Eight consecutive rows of one feature are one 16-byte
     // MFMA operand (k = rows) of dW = h^T dz, and the ReLU mask of the dz
     // chain in its result layout.
     static constexpr int act_off(int l) { return l == 0 ? 0 : kRows * kIn + (l - 1) * kRows * W; }
@@ -221,6 +225,20 @@ BB_D s4 load4(const uint16_t* p) { return *reinterpret_cast<const s4*>(p); }
 
 BB_D f4v mfma(s4 a, s4 b, f4v c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
 
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef short s8v __attribute__((ext_vector_type(8)));
+
+BB_D s8v load8(const uint16_t* p) { return *reinterpret_cast<const s8v*>(p); }
+
+// v_mfma_f32_16x16x32_bf16: K = 32 per instruction, the same result layout
+// as the 16x16x16 form; lane l holds k = 8 (l / 16) .. + 7 of its A row and
+// B column (in the plain-layout kernels)
+BB_D f4v mfma32(s8v a, s8v b, f4v c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0,
+                                                  0, 0);
+}
+
 // one layer for a wave's kRows rows: act [kRows][ld_in] (bf16, LDS) times
 // W_l (WT: [n][k] bf16, LDS) -> acc[row tile][col tile]
 template <int K, int N>
@@ -232,17 +250,30 @@ BB_D void layer_forward(const uint16_t* act, int ld_in, const uint16_t* WT, f4v 
 #pragma unroll
         for (int ct = 0; ct < N / 16; ++ct) acc[rt][ct] = f4v{ 0.0f, 0.0f, 0.0f, 0.0f };
 #pragma unroll
-    for (int kk = 0; kk < K / 16; ++kk) {
-        s4 b[N / 16];
+    for (int kk = 0; kk < K / 32; ++kk) {
+        s8v b[N / 16];
 #pragma unroll
-        for (int ct = 0; ct < N / 16; ++ct) b[ct] = load4(WT + (ct * 16 + r16) * K + kk * 16 + q * 4);
+        for (int ct = 0; ct < N / 16; ++ct) b[ct] = load8(WT + (ct * 16 + r16) * K + kk * 32 + q * 8);
 #pragma unroll
         for (int rt = 0; rt < 4; ++rt) {
-            const s4 a = load4(act + (rt * 16 + r16) * ld_in + kk * 16 + q * 4);
+            const s8v a = load8(act + (rt * 16 + r16) * ld_in + kk * 32 + q * 8);
+#pragma unroll
+            for (int ct = 0; ct < N / 16; ++ct) acc[rt][ct] = mfma32(a, b[ct], acc[rt][ct]);
+        }
+    }
+    if constexpr (K % 32 == 16) { // the last 16 of K (layer 0: K = 80) in the 16x16x16 form
+        constexpr int k0 = K - 16;
+        s4 b[N / 16];
+#pragma unroll
+        for (int ct = 0; ct < N / 16; ++ct) b[ct] = load4(WT + (ct * 16 + r16) * K + k0 + q * 4);
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt) {
+            const s4 a = load4(act + (rt * 16 + r16) * ld_in + k0 + q * 4);
 #pragma unroll
             for (int ct = 0; ct < N / 16; ++ct) acc[rt][ct] = mfma(a, b[ct], acc[rt][ct]);
         }
     }
+    static_assert(K % 16 == 0, "K in steps of 16");
 }
 
 // bf16 copy of count values (a multiple of 8) into LDS by the whole block
@@ -388,13 +419,25 @@ BB_D void dz_layer(int l, const uint16_t* hc, const uint16_t* Wk, uint16_t* dzl,
         for (int ct = 0; ct < W / 16; ++ct) acc[rt][ct] = f4v{ 0.0f, 0.0f, 0.0f, 0.0f };
     // (rows x N) (N x W): A = dz rows, B = W_l^T (B[n][k] = W_l[k][n])
 #pragma unroll
-    for (int kk = 0; kk < N / 16; ++kk) {
-        s4 b[W / 16];
+    for (int kk = 0; kk < N / 32; ++kk) {
+        s8v b[W / 16];
 #pragma unroll
-        for (int ct = 0; ct < W / 16; ++ct) b[ct] = load4(Wl + (ct * 16 + r16) * N + kk * 16 + q * 4);
+        for (int ct = 0; ct < W / 16; ++ct) b[ct] = load8(Wl + (ct * 16 + r16) * N + kk * 32 + q * 8);
 #pragma unroll
         for (int rt = 0; rt < 4; ++rt) {
-            const s4 av = load4(dzl + (rt * 16 + r16) * W + kk * 16 + q * 4);
+            const s8v av = load8(dzl + (rt * 16 + r16) * W + kk * 32 + q * 8);
+#pragma unroll
+            for (int ct = 0; ct < W / 16; ++ct) acc[rt][ct] = mfma32(av, b[ct], acc[rt][ct]);
+        }
+    }
+    if constexpr (N % 32 == 16) { // the last 16 of K (the output layer: N = 16) in the 16x16x16 form
+        constexpr int k0 = N - 16;
+        s4 b[W / 16];
+#pragma unroll
+        for (int ct = 0; ct < W / 16; ++ct) b[ct] = load4(Wl + (ct * 16 + r16) * N + k0 + q * 4);
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt) {
+            const s4 av = load4(dzl + (rt * 16 + r16) * W + k0 + q * 4);
 #pragma unroll
             for (int ct = 0; ct < W / 16; ++ct) acc[rt][ct] = mfma(av, b[ct], acc[rt][ct]);
         }
@@ -486,19 +529,19 @@ BB_D void dw_tile(int l, int mt0, int nt0, int c0, int c1, const uint16_t* acts,
 #pragma unroll
         for (int j = 0; j < TN; ++j) acc[i][j] = f4v{ 0.0f, 0.0f, 0.0f, 0.0f };
     for (int c = c0; c < c1; ++c) {
-        const uint16_t* h = acts + size_t(c) * NT::chunk_acts + NT::act_off(l) + (mt0 * 16 + r16) * kRows + q * 4;
-        const uint16_t* d = dzs + size_t(c) * NT::chunk_dz + NT::dz_off(l) + (nt0 * 16 + r16) * kRows + q * 4;
+        const uint16_t* h = acts + size_t(c) * NT::chunk_acts + NT::act_off(l) + (mt0 * 16 + r16) * kRows + q * 8;
+        const uint16_t* d = dzs + size_t(c) * NT::chunk_dz + NT::dz_off(l) + (nt0 * 16 + r16) * kRows + q * 8;
 #pragma unroll
-        for (int st = 0; st < kRows / 16; ++st) {
-            s4 a[TM], b[TN];
+        for (int st = 0; st < kRows / 32; ++st) {
+            s8v a[TM], b[TN];
 #pragma unroll
-            for (int i = 0; i < TM; ++i) a[i] = load4(h + i * 16 * kRows + st * 16);
+            for (int i = 0; i < TM; ++i) a[i] = load8(h + i * 16 * kRows + st * 32);
 #pragma unroll
-            for (int j = 0; j < TN; ++j) b[j] = load4(d + j * 16 * kRows + st * 16);
+            for (int j = 0; j < TN; ++j) b[j] = load8(d + j * 16 * kRows + st * 32);
 #pragma unroll
             for (int i = 0; i < TM; ++i)
 #pragma unroll
-                for (int j = 0; j < TN; ++j) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
+                for (int j = 0; j < TN; ++j) acc[i][j] = mfma32(a[i], b[j], acc[i][j]);
         }
     }
 #pragma unroll
@@ -641,14 +684,6 @@ __global__ __launch_bounds__(256) void k_reduce_adam(float* __restrict__ w, floa
 // slot j >= 4 feature 32 ks + 16 + 4 g + (j - 4).  The weight fragments use
 // the same slot-to-feature map, so activations never leave the registers.
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef short s8v __attribute__((ext_vector_type(8)));
-
-BB_D f4v mfma32(s8v a, s8v b, f4v c)
-{
-    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0,
-                                                  0, 0);
-}
 
 // the input feature of k-step ks, lane group g, slot j
 BB_HD constexpr int frag_feature(int ks, int g, int j) { return 32 * ks + (j < 4 ? 4 * g + j : 16 + 4 * g + (j - 4)); }
