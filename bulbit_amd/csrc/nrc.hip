@@ -1137,16 +1137,20 @@ bool nrc_refresh(BbNrc* x, hipStream_t st)
 template <class NT>
 int query_round()
 {
-    static int blocks = 0;
-    if (!blocks) {
+    static const int blocks = [] { // computed once (thread-safe), on the first caller's device
         int dev = 0, cus = 0, per_cu = 0;
-        (void)hipGetDevice(&dev);
-        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bb::k_mlp_query<NT>, NT::qblock, 0) != hipSuccess ||
-            per_cu <= 0)
+        bool failed = hipGetDevice(&dev) != hipSuccess;
+        if (failed || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+            failed = true;
+            cus = 256;
+        }
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bb::k_mlp_query<NT>, NT::qblock, 0) != hipSuccess) {
+            failed = true;
             per_cu = 1;
-        blocks = cus * per_cu;
-    }
+        }
+        if (failed) (void)hipGetLastError(); // the fallbacks above stand in; the launch reports its own error
+        return std::max(cus, 1) * std::max(per_cu, 1);
+    }();
     return blocks;
 }
 
