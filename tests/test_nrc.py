@@ -263,6 +263,30 @@ def test_cached_render_with_all_paths_training_equals_plain():
 
 
 @pytest.mark.gpu
+def test_training_rows_beyond_the_steps_are_not_built():
+    """A pass that makes far more rows than its steps read (every path
+    training, 2 steps of 512 rows): the row kernel writes only the rows the
+    steps take and stops early, and the steps still take full batches."""
+    from bulbit_amd import scenes
+    from bulbit_amd.gpu import GpuScene
+    from bulbit_amd.nrc import Nrc
+
+    sc, cam = scenes.icosphere_scene(width=160, height=90, level=4, n_lights=8)
+    g = GpuScene(sc)
+    p = scenes.render_params(4)
+    s0, c0, _ = g.render(cam, p)
+    net = Nrc(seed=3)
+    cp = scenes.cache_params(train_stride=1, train_extra_bounces=8, train_batch=512, train_steps=2)
+    s1, c1, st = g.render_cached(net, cam, p, cp)
+    assert np.array_equal(s0.view(np.int32), s1.view(np.int32)) and np.array_equal(c0, c1)
+    assert st.queries == 0
+    assert st.train_rows == 2 * 512  # two full batches out of tens of thousands of rows
+    assert np.isfinite(st.loss) and st.loss > 0
+    net.close()
+    g.close()
+
+
+@pytest.mark.gpu
 def test_cached_render_with_a_zero_cache_keeps_the_path_prefix():
     """With a cache that predicts 0, a rendering path contributes exactly what
     the plain path gathered before its query vertex, so every pixel is at
