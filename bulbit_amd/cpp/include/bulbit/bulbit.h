@@ -172,6 +172,9 @@ inline Vec4 Mul(const Mat4& m, const Vec4& v)
                 m.ex.w * v.x + m.ey.w * v.y + m.ez.w * v.z + m.ew.w * v.w);
 }
 
+// A * B (include/bulbit/matrix.h:498-501): B's columns through A
+inline Mat4 Mul(const Mat4& a, const Mat4& b) { return Mat4(Mul(a, b.ex), Mul(a, b.ey), Mul(a, b.ez), Mul(a, b.ew)); }
+
 // 3x3 matrix; ex, ey, ez are the columns
 struct Mat3
 {
@@ -846,10 +849,15 @@ struct ModelLoaderOptions
 // LoadOBJ (cli/model_loader.cpp:803-1043): meshes per shape and material,
 // MetallicRoughness materials from the MTL file (CreateOBJMaterial,
 // :742-790), every triangle of a material group an area light with the
-// material's emission.  LoadModel dispatches on the extension; .gltf and
-// .glb are reported as unsupported.
+// material's emission.  LoadGLTF (:405-740): the meshes of a .gltf or .glb
+// model under its node matrices, drawn with the fallback material (the
+// model's own materials would be PrincipledMaterials, which the GPU path
+// does not have: such a model is reported and not loaded).  LoadModel
+// dispatches on the extension.
 void LoadOBJ(Scene& scene, std::filesystem::path filename, const Transform& transform,
              const ModelLoaderOptions& options = {});
+void LoadGLTF(Scene& scene, std::filesystem::path filename, const Transform& transform,
+              const ModelLoaderOptions& options = {});
 void LoadModel(Scene& scene, std::filesystem::path filename, const Transform& transform,
                const ModelLoaderOptions& options = {});
 

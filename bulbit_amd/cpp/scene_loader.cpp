@@ -983,7 +983,8 @@ private:
                 else if (name == "face_normals") ParseBoolean(c.attribute("value")); // read, not applied
                 else if (name == "use_model_material") use_model_material = ParseBoolean(c.attribute("value"));
             }
-            if (type == "gltf") throw Unsupported("glTF models");
+            // a glTF model's own materials would be PrincipledMaterials
+            if (type == "gltf" && use_model_material && !mat) throw Unsupported("glTF model materials");
             ModelLoaderOptions options;
             options.gen_smooth_normal = true;
             options.use_fallback_material = !use_model_material;

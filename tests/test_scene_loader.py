@@ -374,7 +374,8 @@ def test_filters_and_defaults(tmp_path):
     ('<medium type="homogeneous" id="fog"/>', "participating media"),
     ('<integrator type="bdpt"/>', "bdpt integrator"),
     ('<sensor type="perspective"><sampler type="stratified"/></sensor>', "stratified sampler"),
-    ('<shape type="gltf"><string name="filename" value="m.gltf"/><bsdf type="diffuse"/></shape>', "glTF"),
+    ('<shape type="gltf"><string name="filename" value="m.gltf"/><boolean name="use_model_material" '
+     'value="true"/></shape>', "glTF model materials"),
 ])
 def test_features_outside_the_path_fail_loudly(tmp_path, xml, what):
     rc, err = load_fails(tmp_path, f"<scene>{xml}</scene>")
@@ -535,3 +536,213 @@ def test_xml_scene_renders_like_its_description(tmp_path):
     frac = pixel_agreement(img, os_ / oc[..., None])
     print(f"[xml] XML scene vs oracle: {frac:.6f} within 1e-3, mean {img.mean():.4f}", flush=True)
     assert frac >= 0.995 and img.mean() > 0.01
+
+
+# ------------------------------------------------------------------ glTF
+
+def _gltf_model():
+    """A glTF 2.0 model (cli/model_loader.cpp:405-740 reads it): node 0 (a
+    matrix, no mesh) with child node 1 (a matrix, mesh 0: a quad with
+    tangents of both handedness and 16-bit indices, a triangle without
+    tangents and 8-bit indices), node 2 (no matrix, mesh 1: a triangle with
+    32-bit indices and a line primitive, which is skipped).  Returns the JSON
+    document (buffer 0 without uri) and the binary buffer."""
+    f = np.float32
+    quad_p = np.array([[-1, 0, -1], [1, 0, -1], [1, 0, 1], [-1, 0, 1]], f)
+    quad_n = np.array([[0, 1, 0], [0, 1, 0], [0.6, 0.8, 0], [0, 0.8, 0.6]], f)
+    quad_uv = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], f)
+    quad_t = np.array([[1, 0, 0, 1], [1, 0, 0, -1], [0, 0, 1, 1], [1, 0, 0, -1]], f)
+    quad_i = np.array([0, 1, 2, 0, 2, 3], np.uint16)
+    tri_p = np.array([[0, 0, 0], [0.5, 0, 0], [0, 0.5, 0.25]], f)
+    tri_n = np.array([[0, 0, 1], [0, 0.6, 0.8], [0, 0, -1]], f)
+    tri_uv = np.array([[0.25, 0.5], [0.75, 0.5], [0.5, 0.125]], f)
+    tri_i = np.array([0, 1, 2], np.uint8)
+    t2_i = np.array([2, 1, 0], np.uint32)
+    blobs, views, accessors = [], [], []
+
+    def add(arr, type_, ct):
+        off = sum(len(b) for b in blobs)
+        data = arr.tobytes()
+        data += b"\0" * (-len(data) % 4)
+        blobs.append(data)
+        views.append({"buffer": 0, "byteOffset": off, "byteLength": arr.nbytes})
+        accessors.append({"bufferView": len(views) - 1, "componentType": ct, "count": len(arr), "type": type_})
+        return len(accessors) - 1
+
+    A = {"qp": add(quad_p, "VEC3", 5126), "qn": add(quad_n, "VEC3", 5126), "quv": add(quad_uv, "VEC2", 5126),
+         "qt": add(quad_t, "VEC4", 5126), "qi": add(quad_i, "SCALAR", 5123), "tp": add(tri_p, "VEC3", 5126),
+         "tn": add(tri_n, "VEC3", 5126), "tuv": add(tri_uv, "VEC2", 5126), "ti": add(tri_i, "SCALAR", 5121),
+         "t2i": add(t2_i, "SCALAR", 5125)}
+    s0 = [2, 0, 0, 0, 0, 2, 0, 0, 0, 0, 2, 0, 1, 0, 0, 1]       # scale 2, then translate x 1 (column-major)
+    r1 = [0, 0, -1, 0, 0, 1, 0, 0, 1, 0, 0, 0, 0, 0.5, 0, 1]    # rotate 90 degrees about y, translate y 0.5
+    doc = {
+        "asset": {"version": "2.0"}, "scene": 0, "scenes": [{"nodes": [0, 2]}],
+        "nodes": [{"matrix": s0, "children": [1]}, {"matrix": r1, "mesh": 0}, {"mesh": 1, "name": "tést"}],
+        "meshes": [
+            {"primitives": [
+                {"attributes": {"POSITION": A["qp"], "NORMAL": A["qn"], "TEXCOORD_0": A["quv"], "TANGENT": A["qt"]},
+                 "indices": A["qi"]},
+                {"attributes": {"POSITION": A["tp"], "NORMAL": A["tn"], "TEXCOORD_0": A["tuv"]}, "indices": A["ti"]}]},
+            {"primitives": [
+                {"attributes": {"POSITION": A["tp"], "NORMAL": A["tn"], "TEXCOORD_0": A["tuv"]}, "indices": A["t2i"]},
+                {"attributes": {"POSITION": A["tp"], "NORMAL": A["tn"], "TEXCOORD_0": A["tuv"]}, "indices": A["t2i"],
+                 "mode": 1}]}],
+        "accessors": accessors, "bufferViews": views,
+        "buffers": [{"byteLength": sum(len(b) for b in blobs)}],
+    }
+    mats = {"s0": np.array(s0, float).reshape(4, 4).T, "r1": np.array(r1, float).reshape(4, 4).T}
+    parts = {"quad": (quad_p, quad_n, quad_uv, quad_t, quad_i), "tri": (tri_p, tri_n, tri_uv, None, tri_i),
+             "tri2": (tri_p, tri_n, tri_uv, None, t2_i)}
+    return doc, b"".join(blobs), mats, parts
+
+
+def _write_gltf(tmp_path, kind):
+    import base64
+    import struct
+    doc, blob, _, _ = _gltf_model()
+    if kind == "gltf":
+        doc["buffers"][0]["uri"] = "model%20data.bin"
+        (tmp_path / "model data.bin").write_bytes(blob)
+        (tmp_path / "m.gltf").write_text(json.dumps(doc))
+        return "m.gltf"
+    if kind == "datauri":
+        doc["buffers"][0]["uri"] = "data:application/octet-stream;base64," + base64.b64encode(blob).decode()
+        (tmp_path / "m.gltf").write_text(json.dumps(doc))
+        return "m.gltf"
+    js = json.dumps(doc).encode()
+    js += b" " * (-len(js) % 4)
+    body = struct.pack("<II", len(js), 0x4E4F534A) + js + struct.pack("<II", len(blob), 0x004E4942) + blob
+    (tmp_path / "m.glb").write_bytes(struct.pack("<III", 0x46546C67, 2, 12 + len(body)) + body)
+    return "m.glb"
+
+
+def _expected_gltf_triangles(to_world):
+    """Mesh (cli: Mesh::Mesh) of each primitive: positions through the
+    node's world matrix, normals and tangents through it (w = 0) and
+    normalized, texture v negated (flip_tex_coord false); tangents as
+    given (w != 1 negates) or CoordinateSystem of the normal"""
+    _, _, mats, parts = _gltf_model()
+    node1 = to_world @ mats["s0"] @ mats["r1"]
+    node2 = to_world
+    out = []
+    for M, name in ((node1, "quad"), (node1, "tri"), (node2, "tri2")):
+        p, n, uv, t, idx = parts[name]
+        if t is None:
+            t = np.array([list(_coordinate_system_f64(v)) + [1] for v in n], float)
+        tv = np.where(t[:, 3:4] == 1, t[:, :3], -t[:, :3])
+        for k in range(0, len(idx), 3):
+            corners = []
+            for v in idx[k:k + 3]:
+                pw = M @ np.append(p[v], 1)
+                nw = M[:3, :3] @ n[v]
+                tw = M[:3, :3] @ tv[v]
+                corners.append((pw[:3], nw / np.linalg.norm(nw), tw / np.linalg.norm(tw), [uv[v][0], -uv[v][1]]))
+            out.append(corners)
+    return out
+
+
+def _coordinate_system_f64(n):
+    sign = np.copysign(1.0, n[2])
+    a = -1 / (sign + n[2])
+    b = n[0] * n[1] * a
+    return np.array([1 + sign * n[0] * n[0] * a, sign * b, -sign * n[0]])
+
+
+GLTF_XML = """<scene version="3.0.0">
+    <shape type="gltf">
+        <string name="filename" value="{name}"/>
+        <transform name="to_world"><translate y="0.5" z="-0.25"/></transform>
+        <bsdf type="diffuse"><rgb name="reflectance" value="0.3 0.4 0.5"/></bsdf>
+    </shape>
+</scene>
+"""
+
+
+@pytest.mark.parametrize("kind", ["gltf", "datauri", "glb"])
+def test_gltf_model_geometry(tmp_path, kind):
+    """LoadGLTF through the scene loader (a gltf shape with a bsdf: the
+    model is drawn with it): every triangle's corners, normals, tangents and
+    texture coordinates against the reference's rules, from an external
+    buffer file (with a %-escaped name), a base64 data URI and a .glb"""
+    name = _write_gltf(tmp_path, kind)
+    (tmp_path / "scene.xml").write_text(GLTF_XML.format(name=name))
+    d = describe(tmp_path / "scene.xml")
+    prims = d["primitives"]
+    T = np.eye(4)
+    T[:3, 3] = [0, 0.5, -0.25]
+    want = _expected_gltf_triangles(T)
+    assert len(prims) == len(want) == 4
+    for prim, corners in zip(prims, want):
+        assert prim["material"] == 0
+        for c, (p, n, t, uv) in zip(prim["triangle"], corners):
+            assert np.allclose(c["p"], p, atol=2e-6)
+            assert np.allclose(c["n"], n, atol=2e-6)
+            assert np.allclose(c["t"], t, atol=2e-6)
+            assert np.allclose(c["uv"], uv, atol=1e-7)
+    assert len(d["materials"]) == 1 and d["materials"][0]["type"] == "diffuse"
+
+
+def test_gltf_model_materials_are_refused(tmp_path):
+    """A model whose primitives name their own materials (PrincipledMaterial
+    in the reference) and no fallback material: reported, nothing loaded"""
+    doc, blob, _, _ = _gltf_model()
+    doc["meshes"][0]["primitives"][0]["material"] = 0
+    doc["materials"] = [{"pbrMetallicRoughness": {"baseColorFactor": [1, 1, 1, 1]}}]
+    doc["buffers"][0]["uri"] = "b.bin"
+    (tmp_path / "b.bin").write_bytes(blob)
+    (tmp_path / "m.gltf").write_text(json.dumps(doc))
+    rc, err = load_fails(tmp_path, '<scene><shape type="gltf"><string name="filename" value="m.gltf"/>'
+                                   '<boolean name="use_model_material" value="true"/></shape></scene>')
+    assert rc == 1 and "glTF model materials" in err, err
+
+
+GLTF_RENDER_XML = """<scene version="3.0.0">
+    <integrator type="path"><integer name="max_depth" value="5"/></integrator>
+    <sensor type="perspective">
+        <float name="fov" value="40"/>
+        <transform name="to_world"><lookat origin="2, 3.5, 6" target="1, 0.5, 0" up="0, 1, 0"/></transform>
+        <sampler type="independent"><integer name="sample_count" value="8"/></sampler>
+        <film type="hdrfilm"><integer name="width" value="96"/><integer name="height" value="64"/>
+            <rfilter type="box"/></film>
+    </sensor>
+    <shape type="gltf">
+        <string name="filename" value="m.glb"/>
+        <transform name="to_world"><translate y="0.5" z="-0.25"/></transform>
+        <bsdf type="twosided"><bsdf type="diffuse"><rgb name="reflectance" value="0.7 0.5 0.3"/></bsdf></bsdf>
+    </shape>
+    <shape type="rectangle">
+        <transform name="to_world"><scale value="1.5"/><rotate x="1" angle="90"/><translate x="1" y="4"/></transform>
+        <bsdf type="diffuse"/>
+        <emitter type="area"><rgb name="radiance" value="6 6 5"/></emitter>
+    </shape>
+    <emitter type="constant"><rgb name="radiance" value="0.15"/></emitter>
+</scene>
+"""
+
+
+@pytest.mark.gpu
+def test_gltf_scene_renders_like_its_description(tmp_path):
+    """An XML scene with a .glb model: render_scene equals its description
+    rendered through the C ABI bit for bit, and the oracle within 1e-3"""
+    from bulbit_amd.gpu import GpuScene
+    from oracle import OracleScene
+
+    _write_gltf(tmp_path, "glb")
+    path = tmp_path / "scene.xml"
+    path.write_text(GLTF_RENDER_XML)
+    d = describe(path)
+    assert len(d["primitives"]) == 4 + 2 and len(d["lights"]) == 3
+    out = tmp_path / "gltf.pfm"
+    print("[gltf] render_scene 96x64 x 8 spp", flush=True)
+    r = subprocess.run([_exe(build.CPP_RENDER_SCENE), str(path), str(out)], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr
+    img = read_pfm(out)
+    sc, cam, params = scene_from_description(d)
+    s, c, _ = GpuScene(sc).render(cam, params)
+    ref = s / c[..., None]
+    assert np.array_equal(img.view(np.int32), ref.astype(np.float32).view(np.int32))
+    os_, oc, _ = OracleScene(sc).render(cam, params, threads=max(1, min(16, len(os.sched_getaffinity(0)))))
+    frac = pixel_agreement(img, os_ / oc[..., None])
+    print(f"[gltf] glTF scene vs oracle: {frac:.6f} within 1e-3, mean {img.mean():.4f}", flush=True)
+    assert frac >= 0.999 and img.mean() > 0.01
