@@ -939,9 +939,22 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_mlp_train(const float* __r
     __shared__ __align__(16) uint16_t sH[2][kIn * kTrainLd];
     __shared__ __align__(16) uint16_t sZ[2][W * kTrainLd];
     __shared__ float sLoss[kTrainWaves];
+    // the forward fragments are staged first; the gradient chain's are
+    // loaded into registers behind the first pass's forward and stored
+    // into LDS before its first barrier (the chain reads them only after it)
+    constexpr int kWbVec = NT::bfrag_elems / 8, kWbPer = (kWbVec + 64 * kTrainWaves - 1) / (64 * kTrainWaves);
+    static_assert(NT::bfrag_elems % 8 == 0, "16-byte pieces");
     stage16(sWF, wf, NT::frag_elems);
-    stage16(sWB, wb, NT::bfrag_elems);
     __syncthreads();
+    static_assert(kWbPer == 4, "four 16-byte pieces per thread");
+    const uint4* wb4 = reinterpret_cast<const uint4*>(wb);
+    constexpr int kStride = 64 * kTrainWaves;
+    const int t0 = int(threadIdx.x);
+    // in bounds: a piece past the end loads the last one (it is not stored)
+    const uint4 wb0 = wb4[t0], wb1 = wb4[t0 + kStride], wb2 = wb4[t0 + 2 * kStride],
+                wb3 = wb4[t0 + 3 * kStride < kWbVec ? t0 + 3 * kStride : kWbVec - 1];
+    static_assert(3 * 64 * kTrainWaves <= kWbVec, "the first three pieces of every thread are in bounds");
+    bool wb_pending = true;
     int off = 0, n = 0;
     span_rows(span, off, n);
     raw += size_t(off) * kRawStride;
@@ -992,6 +1005,14 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_mlp_train(const float* __r
         {
             const u4v d{ pack_bf16(gy[0], gy[1]), pack_bf16(gy[2], 0.0f), 0u, 0u };
             dz[0][0] = __builtin_bit_cast(s8v, d);
+        }
+        if (wb_pending) {
+            uint4* s4 = reinterpret_cast<uint4*>(sWB);
+            s4[t0] = wb0;
+            s4[t0 + kStride] = wb1;
+            s4[t0 + 2 * kStride] = wb2;
+            if (t0 + 3 * kStride < kWbVec) s4[t0 + 3 * kStride] = wb3;
+            wb_pending = false;
         }
         store_features<HS, W>(sH[buf], h[L], wave, lane);
         store_features<1, kOutCols>(sZ[buf], dz, wave, lane);
