@@ -87,7 +87,7 @@ def build_cpp(verbose: bool = False) -> str:
     inc = os.path.join(CPP, "include")
     tmp = CPP_LIB + ".tmp"
     _run([cxx, *CXX_FLAGS, "-fPIC", "-shared", "-I", inc, os.path.join(CPP, "bulbit.cpp"),
-          os.path.join(CPP, "image_io.cpp"), os.path.join(CPP, "model_loader.cpp"), os.path.join(CPP, "scene_loader.cpp"), "-o", tmp,
+          os.path.join(CPP, "image_io.cpp"), os.path.join(CPP, "jpeg_decode.cpp"), os.path.join(CPP, "model_loader.cpp"), os.path.join(CPP, "scene_loader.cpp"), "-o", tmp,
           "-L", HERE, "-lbbgpu", "-lz", "-Wl,-rpath,$ORIGIN"], "g++ building libbulbit.so", verbose)
     os.replace(tmp, CPP_LIB)
     os.makedirs(os.path.dirname(CPP_EXAMPLE), exist_ok=True)

@@ -1,4 +1,4 @@
-// Reads an image with ReadImage3 (PNM, PNG, Radiance .hdr, PFM), or one
+// Reads an image with ReadImage3 (PNM, PNG, JPEG, Radiance .hdr, PFM), or one
 // channel of it with ReadImage1, and writes it with WriteImage:
 //   convert_image input output.{pfm,hdr,png} [non_color] [channel=K]
 // With channel=K the channel is written to all three components; an image

@@ -415,9 +415,14 @@ bool WriteImage(const Image3& image, const std::string& filename, ToneMappingCal
 // data (stbi__ldr_to_hdr), an alpha channel becomes c / 255, Radiance .hdr
 // RGBE becomes r * 2^(e - 136) (stbi__hdr_convert), and every value is
 // clamped at 0.  Decoded here: binary PNM (P5 grey, P6 RGB, 8 or 16 bits,
-// 16-bit samples reduced to their high byte as stb does), PNG (below), Radiance .hdr
-// (flat and RLE scanlines), and PFM (beyond stb: linear floats, already
-// bottom row first).
+// 16-bit samples reduced to their high byte as stb does), PNG (below), JPEG
+// (jpeg_decode.cpp), Radiance .hdr (flat and RLE scanlines), and PFM
+// (beyond stb: linear floats, already bottom row first).
+
+namespace jpeg
+{
+bool Decode(const uint8_t* data, size_t size, std::vector<uint8_t>& rgba, int& width, int& height, int& file_comps);
+}
 
 namespace
 {
@@ -758,6 +763,32 @@ bool read_png(std::FILE* f, bool non_color, Decoded& out)
     return true;
 }
 
+// JPEG: baseline and progressive, grey, YCbCr with any whole sampling
+// ratios, Adobe RGB/CMYK/YCCK; expanded to RGBA with alpha 255 as
+// STBI_rgb_alpha asks, reporting 1 channel for grey and 3 for colour
+bool read_jpeg(std::FILE* f, bool non_color, Decoded& out)
+{
+    std::vector<unsigned char> file;
+    unsigned char buf[65536];
+    for (size_t n; (n = std::fread(buf, 1, sizeof buf, f)) > 0;) file.insert(file.end(), buf, buf + n);
+    std::vector<uint8_t> rgba;
+    int w = 0, h = 0, comps = 0;
+    if (!jpeg::Decode(file.data(), file.size(), rgba, w, h, comps)) return false;
+    out.width = w;
+    out.height = h;
+    out.comps = 4;
+    out.file_comps = comps;
+    out.data.resize(size_t(w) * size_t(h) * 4);
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x)
+            for (int k = 0; k < 4; ++k) {
+                const unsigned char c = rgba[(size_t(y) * size_t(w) + size_t(x)) * 4 + size_t(k)];
+                out.data[(size_t(h - 1 - y) * size_t(w) + size_t(x)) * 4 + size_t(k)] =
+                    k == 3 ? c / 255.0f : ldr_to_float(c, non_color);
+            }
+    return true;
+}
+
 bool decode(const std::string& filename, bool non_color, Decoded& out)
 {
     std::FILE* f = std::fopen(filename.c_str(), "rb");
@@ -775,6 +806,8 @@ bool decode(const std::string& filename, bool non_color, Decoded& out)
             ok = read_pnm(f, non_color, out);
         else if (ext == ".png" || ext == ".PNG")
             ok = read_png(f, non_color, out);
+        else if (ext == ".jpg" || ext == ".JPG" || ext == "jpeg" || ext == "JPEG")
+            ok = read_jpeg(f, non_color, out);
     } catch (const std::bad_alloc&) {
         ok = false;
     } catch (const std::length_error&) {

@@ -221,7 +221,8 @@ def _expected_scene(tmp_path, floor, floor_n, ty, width, height):
 
 
 @pytest.mark.gpu
-def test_obj_model_equals_hand_built_scene(tmp_path):
+@pytest.mark.parametrize("texture", ["ppm", "jpg"])
+def test_obj_model_equals_hand_built_scene(tmp_path, texture):
     from bulbit_amd.gpu import GpuScene
     from oracle import OracleScene
 
@@ -230,18 +231,28 @@ def test_obj_model_equals_hand_built_scene(tmp_path):
     norm8 = np.full((2, 2, 3), 128, np.uint8)
     norm8[0, 0] = (180, 128, 230)
     norm8[1, 1] = (90, 150, 220)
-    _write_pnm(tmp_path / "floor.ppm", floor8)
     _write_pnm(tmp_path / "floor_n.ppm", norm8)
+    if texture == "ppm":
+        _write_pnm(tmp_path / "floor.ppm", floor8)
+        floor = _decode8(floor8, False)[::-1].copy()
+    else:
+        # a 4:2:0 JPEG written from chosen coefficients; the hand-built scene
+        # takes the texture as ReadImage3 decodes it
+        from test_jpeg import _random_coefs, write_jpeg
+        coefs = [_random_coefs(2, 2), _random_coefs(1, 1, 0.5), _random_coefs(1, 1, 0.5)]
+        (tmp_path / "floor.jpg").write_bytes(
+            write_jpeg(13, 11, [(1, 2, 2, 0), (2, 1, 1, 0), (3, 1, 1, 0)], coefs, {0: np.full(64, 2)}))
+        floor = _convert(tmp_path, tmp_path / "floor.jpg")
+        assert floor.shape == (11, 13, 3)
     (tmp_path / "scene.obj").write_text(OBJ)
-    (tmp_path / "scene.mtl").write_text(MTL)
+    (tmp_path / "scene.mtl").write_text(MTL.replace("floor.ppm", f"floor.{texture}"))
     out = tmp_path / "obj.pfm"
     print("[obj] load_obj 160 90 8", flush=True)
     r = subprocess.run([_exe(build.CPP_LOAD_OBJ), str(tmp_path / "scene.obj"), "160", "90", "8", str(out), "0.25"],
                        capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr
     img = read_pfm(out)
-    sc, cam = _expected_scene(tmp_path, _decode8(floor8, False)[::-1].copy(), _decode8(norm8, True)[::-1].copy(),
-                              0.25, 160, 90)
+    sc, cam = _expected_scene(tmp_path, floor, _decode8(norm8, True)[::-1].copy(), 0.25, 160, 90)
     s, c, _ = GpuScene(sc).render(cam, scenes.render_params(8))
     ref = s / c[..., None]
     assert np.array_equal(img.view(np.int32), ref.astype(np.float32).view(np.int32))
