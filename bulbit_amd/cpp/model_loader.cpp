@@ -142,22 +142,29 @@ bool parse_obj(const std::string& filename, const std::string& base, Obj& obj)
         } else if (key == "f") {
             std::vector<Index> face;
             std::string tok;
+            const size_t nv = obj.vertices.size() / 3, nt = obj.texcoords.size() / 2, nn = obj.normals.size() / 3;
+            // an index the corner names must exist: 0, or one beyond either end, fails the read
+            const auto bad = [](int i, size_t n) { return i < 0 || size_t(i) >= n; };
             while (in >> tok) {
                 Index ix;
                 int a = 0, b = 0, c = 0;
+                bool ok;
                 if (std::sscanf(tok.c_str(), "%d/%d/%d", &a, &b, &c) == 3) {
-                    ix = { fix_index(a, obj.vertices.size() / 3), fix_index(b, obj.texcoords.size() / 2),
-                           fix_index(c, obj.normals.size() / 3) };
+                    ix = { fix_index(a, nv), fix_index(b, nt), fix_index(c, nn) };
+                    ok = !bad(ix.vt, nt) && !bad(ix.vn, nn);
                 } else if (std::sscanf(tok.c_str(), "%d//%d", &a, &c) == 2) {
-                    ix = { fix_index(a, obj.vertices.size() / 3), -1, fix_index(c, obj.normals.size() / 3) };
+                    ix = { fix_index(a, nv), -1, fix_index(c, nn) };
+                    ok = !bad(ix.vn, nn);
                 } else if (std::sscanf(tok.c_str(), "%d/%d", &a, &b) == 2) {
-                    ix = { fix_index(a, obj.vertices.size() / 3), fix_index(b, obj.texcoords.size() / 2), -1 };
+                    ix = { fix_index(a, nv), fix_index(b, nt), -1 };
+                    ok = !bad(ix.vt, nt);
                 } else if (std::sscanf(tok.c_str(), "%d", &a) == 1) {
-                    ix = { fix_index(a, obj.vertices.size() / 3), -1, -1 };
+                    ix = { fix_index(a, nv), -1, -1 };
+                    ok = true;
                 } else {
                     return false;
                 }
-                if (ix.v < 0 || size_t(ix.v) >= obj.vertices.size() / 3) return false;
+                if (!ok || bad(ix.v, nv)) return false;
                 face.push_back(ix);
             }
             ObjShape& s = obj.shapes.back();

@@ -182,6 +182,18 @@ def desc(tmp_path_factory):
     return describe(_write_files(tmp_path_factory.mktemp("xml")))
 
 
+@pytest.mark.parametrize("face", ["f 1//2 2//1 3//1", "f 1/4 2/1 3/1", "f 1/1/1 2/1/1 3/1/0", "f 1 2 -4", "f 0 2 3"])
+def test_obj_faces_naming_missing_vertices_are_refused(tmp_path, desc, face):
+    # a texture coordinate, normal or position the file does not have fails
+    # the model's read instead of reading past its arrays
+    path = _write_files(tmp_path)
+    (tmp_path / "tri.obj").write_text(OBJ.replace("f 1 2 3", "vt 0 0\nvn 0 0 1\n" + face))
+    d = describe(path)
+    assert len(d["primitives"]) == len(desc["primitives"]) - 1  # the OBJ's one triangle is missing
+    (tmp_path / "tri.obj").write_text(OBJ.replace("f 1 2 3", "vt 0 0\nvn 0 0 1\nf 1/1/1 2/1/1 -1/-1/-1"))
+    assert len(describe(path)["primitives"]) == len(desc["primitives"])
+
+
 def _rot(axis, deg):
     """rotation matrix about a unit axis (Rodrigues), float64"""
     a = np.asarray(axis, float)
