@@ -1154,7 +1154,8 @@ using Image1 = Image<Float>;
 // ReadImage1 / ReadImage3 (include/bulbit/image.h:76-87): channel `channel`,
 // or RGB, of an image file as linear floats, bottom row first; 8-bit colour
 // data decoded with gamma 2.2 unless non_color.  Reads binary PNM (.ppm,
-// .pgm, .pnm), PNG, Radiance .hdr and .pfm; an empty image on failure.
+// .pgm, .pnm), PNG, JPEG (.jpg, .jpeg; baseline and progressive), Radiance
+// .hdr and .pfm; an empty image on failure.
 Image1 ReadImage1(const std::string& filename, int32 channel, bool non_color = false,
                   std::function<Float(Float)> transform = {});
 Image3 ReadImage3(const std::string& filename, bool non_color = false,
