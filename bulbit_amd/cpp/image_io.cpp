@@ -789,25 +789,42 @@ bool read_jpeg(std::FILE* f, bool non_color, Decoded& out)
     return true;
 }
 
+// The format comes from the file's first bytes, as stb_image decides it
+// (each of its decoders tests a signature; the file name plays no part), so
+// a PNG saved as .jpg reads as a PNG.  PFM, which stb does not read, is
+// recognised by its own signature.
+enum class Format { unknown, png, jpeg, hdr, pnm, pfm };
+
+Format sniff(std::FILE* f)
+{
+    unsigned char h[8] = {};
+    const size_t n = std::fread(h, 1, sizeof h, f);
+    std::rewind(f);
+    const auto space = [](unsigned char c) { return c == ' ' || c == '\t' || c == '\n' || c == '\r'; };
+    if (n >= 8 && std::memcmp(h, "\x89PNG\r\n\x1a\n", 8) == 0) return Format::png;
+    if (n >= 2 && h[0] == 0xFF && h[1] == 0xD8) return Format::jpeg; // start-of-image marker
+    if (n >= 2 && h[0] == '#' && h[1] == '?') return Format::hdr;    // "#?RADIANCE", "#?RGBE"
+    if (n >= 3 && h[0] == 'P' && (h[1] == '5' || h[1] == '6') && space(h[2])) return Format::pnm;
+    if (n >= 3 && h[0] == 'P' && (h[1] == 'F' || h[1] == 'f') && space(h[2])) return Format::pfm;
+    return Format::unknown;
+}
+
 bool decode(const std::string& filename, bool non_color, Decoded& out)
 {
     std::FILE* f = std::fopen(filename.c_str(), "rb");
     if (!f) return false;
     bool ok = false;
-    const std::string ext = filename.size() >= 4 ? filename.substr(filename.size() - 4) : "";
     // a header whose sizes cannot be allocated fails the read (stb_image
     // returns NULL for it) instead of throwing out of ReadImage1/ReadImage3
     try {
-        if (ext == ".hdr")
-            ok = read_hdr(f, out);
-        else if (ext == ".pfm")
-            ok = read_pfm(f, out);
-        else if (ext == ".ppm" || ext == ".pgm" || ext == ".pnm")
-            ok = read_pnm(f, non_color, out);
-        else if (ext == ".png" || ext == ".PNG")
-            ok = read_png(f, non_color, out);
-        else if (ext == ".jpg" || ext == ".JPG" || ext == "jpeg" || ext == "JPEG")
-            ok = read_jpeg(f, non_color, out);
+        switch (sniff(f)) {
+        case Format::png: ok = read_png(f, non_color, out); break;
+        case Format::jpeg: ok = read_jpeg(f, non_color, out); break;
+        case Format::hdr: ok = read_hdr(f, out); break;
+        case Format::pnm: ok = read_pnm(f, non_color, out); break;
+        case Format::pfm: ok = read_pfm(f, out); break;
+        case Format::unknown: break;
+        }
     } catch (const std::bad_alloc&) {
         ok = false;
     } catch (const std::length_error&) {

@@ -575,3 +575,26 @@ def test_libjpeg_written_files_decode_like_libjpeg(tmp_path, mode, sub, progress
     # tables round each term, and its 2x2 upsampling alternates its bias
     assert d.max() <= (1 if mode == "L" else 3), d.max()
     assert d.mean() < 0.25, d.mean()
+
+
+def test_the_format_comes_from_the_content_not_the_name(tmp_path):
+    # stb_image picks its decoder from the file's signature: a PNG saved as
+    # .jpg, a JPEG saved as .png, an HDR or PPM under any name read as they are
+    import test_png
+    from test_model_loader import _write_hdr, _write_pnm
+
+    c = _random_coefs(2, 3)
+    jpg = write_jpeg(20, 12, [(1, 1, 1, 0)], [c], {0: ONES})
+    assert np.array_equal(_read(tmp_path, jpg, "a.png"), _read(tmp_path, jpg, "a.jpg"))
+    rng = np.random.default_rng(4)
+    test_png.write_png(tmp_path / "p.png", rng.integers(0, 256, (5, 6, 3)).astype(np.uint8), 2, 8)
+    png = (tmp_path / "p.png").read_bytes()
+    assert np.array_equal(_read(tmp_path, png, "p.jpg"), _read(tmp_path, png, "p2.png"))
+    _write_hdr(tmp_path / "h.hdr", rng.integers(1, 256, (3, 4, 4)))
+    _write_pnm(tmp_path / "i.ppm", rng.integers(0, 256, (3, 4, 3)).astype(np.uint8))
+    for name, other in (("h.hdr", "h.img"), ("i.ppm", "i.texture")):
+        data = (tmp_path / name).read_bytes()
+        a = _read(tmp_path, data, "x" + name[1:], non_color=False)
+        b = _read(tmp_path, data, other, non_color=False)
+        assert np.array_equal(a, b), name
+    assert _read(tmp_path, b"not an image at all", "n.png") == 1
