@@ -171,8 +171,9 @@ def main():
             # divergence, with measured HBM traffic well under the algorithmic bytes
             "limiter": "memory latency and lane divergence: 30 of 64 lanes active per VALU instruction, waves "
                        "waiting 45% of their cycles, L2 hit rate 48%; measured HBM traffic 0.20x the algorithmic "
-                       "bytes (1.95 TB/s), the rest served from L2 and the Infinity Cache, so the algorithmic "
-                       "rate can pass the HBM peak (profiles/traversal_c2_counters.md, trace_closest_c2_traffic.md)",
+                       "bytes (1.89 TB/s), the rest served from L2 and the Infinity Cache, so the algorithmic "
+                       "rate can pass the HBM peak; each launch shares the GPU with the previous depth's shadow "
+                       "traversal (profiles/traversal_c2_counters.md, trace_closest_c2_traffic.md)",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
