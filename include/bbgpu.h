@@ -285,11 +285,14 @@ enum { BB_STACK_LDS = 0, BB_STACK_SCRATCH = 1 };
 
 /* Per-render statistics, filled when a pointer is passed to bbgpu_render.
  * Kernel times come from HIP events recorded around each launch on the
- * render stream. */
+ * stream it runs on.  On scenes of 4,096 primitives or more (not in graphs
+ * or counting frames) each depth's shadow rays run on a second stream beside
+ * the next depth's closest hits, so the two sums overlap in time. */
 typedef struct BbRenderStats {
     double ms_total;          /* render wall time on the stream (events) */
     double ms_trace_closest;  /* summed over all closest-hit launches */
-    double ms_trace_shadow;   /* summed over all any-hit launches */
+    double ms_trace_shadow;   /* summed over all any-hit launches, each from
+                                 the end of its depth's shading to its own end */
     double ms_shade;          /* raygen + shade + film kernels */
     int64_t launches_closest;
     int64_t launches_shadow;
