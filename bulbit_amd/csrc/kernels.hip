@@ -2256,12 +2256,12 @@ int render_impl(BbScene* scene, const BbCameraDesc* camera, const BbRenderParams
     // depth d + 1 waits for both (it adds to the radiance the shadow rays
     // add to, and rewrites the shadow-ray queue), so every sum keeps its
     // order and the film is bit-identical.  The two kernels then take
-    // separate halves of the overflow area.  Not for counting frames or
-    // graphs, which keep one stream, nor for scenes so small that the two
-    // handoffs per depth (some microseconds each) cost more than the
-    // traversal tails they hide: a one-sphere 1080p frame measured 2.27 ms
-    // on one stream and 2.40 ms on two.
-    const bool split = overlap && !count && !p.graph && scene->n_prims >= kOverlapMinPrims;
+    // separate halves of the overflow area.  A captured graph records the
+    // same fork and join.  Not for counting frames, which keep one stream,
+    // nor for scenes so small that the two handoffs per depth (some
+    // microseconds each) cost more than the traversal tails they hide: a
+    // one-sphere 1080p frame measured 2.27 ms on one stream and 2.40 ms on two.
+    const bool split = overlap && !count && scene->n_prims >= kOverlapMinPrims;
     const size_t ov_half = overflow * size_t(std::max(g_trace, g_shadow)) * size_t(tblock);
     if (!ensure_gstack(x->ws, ov_half * (split ? 2 : 1))) {
         bb::set_error("bbgpu_render: traversal stack allocation failed");
