@@ -285,7 +285,7 @@ enum { BB_STACK_LDS = 0, BB_STACK_SCRATCH = 1 };
 
 /* Per-render statistics, filled when a pointer is passed to bbgpu_render.
  * Kernel times come from HIP events recorded around each launch on the
- * stream it runs on.  On scenes of 4,096 primitives or more (not in
+ * stream it runs on.  On scenes of 8 primitives or more (not in
  * counting frames) each depth's shadow rays run on a second stream beside
  * the next depth's closest hits, so the two sums overlap in time. */
 typedef struct BbRenderStats {

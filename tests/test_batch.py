@@ -134,7 +134,7 @@ def test_batch_with_graphs_equals_individual_renders():
 
 
 def test_shadow_rays_beside_closest_hits_keep_the_film():
-    """On scenes of 4,096 primitives or more each depth's shadow rays run on
+    """On scenes of 8 primitives or more each depth's shadow rays run on
     a second stream beside the next depth's closest hits; the next shading
     waits for them, so the film equals, bit for bit, that of a counting
     render, which keeps every launch on one stream.  Several passes per
@@ -142,7 +142,7 @@ def test_shadow_rays_beside_closest_hits_keep_the_film():
     from bulbit_amd.gpu import GpuScene
 
     sc, cam = scenes.icosphere_scene(width=240, height=135, level=6, n_lights=32, seed=3)
-    assert len(sc.prims) >= 4096
+    assert len(sc.prims) >= 8
     g = GpuScene(sc)
     p = scenes.render_params(6, max_bounces=10, wavefront_paths=240 * 135 * 2)  # passes of 2 samples
     s0, c0, _ = g.render(cam, p, count_traversal=True)
